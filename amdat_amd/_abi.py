@@ -1,0 +1,335 @@
+"""ctypes binding of the C ABI in include/amdat_hip.h.
+
+The library is amdat_amd/lib/libamdat_hip.so, built in-tree by
+``make -C amdat_amd`` (``__graft_entry__.build()``).  Loading fails loudly
+when it is missing: there is no CPU fallback for any device entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libamdat_hip.so")
+
+AMDAT_OK = 0
+PREC_FP32C = 0
+PREC_FP64 = 1
+
+_ERRNAMES = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ERANGE", -5: "EIO"}
+
+# Every exported entry point: name -> (restype, argtypes).
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_D = C.c_double
+_PI = C.POINTER(C.c_int)
+_PF = C.POINTER(C.c_float)
+_PD = C.POINTER(C.c_double)
+_PS = C.POINTER(C.c_int16)
+_PP = C.POINTER(C.c_void_p)
+
+SIGNATURES = {
+    "amdat_last_error": (C.c_char_p, []),
+    "amdat_version": (_I, []),
+    "amdat_ctx_create": (_I, [_I, _PP]),
+    "amdat_ctx_destroy": (_I, [_P]),
+    "amdat_ctx_set_stream": (_I, [_P, _P]),
+    "amdat_ctx_synchronize": (_I, [_P]),
+    "amdat_ts_n_timesteps": (_I, [_I, _I]),
+    "amdat_ts_n_timegaps": (_I, [_I, _I, _I]),
+    "amdat_ts_timelist": (_I, [_I, _I, _F, _I, _F, _PF, _I]),
+    "amdat_ts_displacement_times": (_I, [_I, _I, _I, _PF, _PF, _I]),
+    "amdat_ts_pairs": (_I, [_I, _I, _I, _I, _I, _PI, _PI, _PI, _I, _PI]),
+    "amdat_ts_weighting": (_I, [_I, _I, _I, _I, _I, _PI, _I]),
+    "amdat_shells_read": (_I, [C.c_char_p, _I, _I, _PI, _PI, _I, _PI]),
+    "amdat_shells_generate": (_I, [_I, _I, _I, C.c_uint64, _PI, _PI, _I, _PI]),
+    "amdat_shells_upload": (_I, [_P, _I, _PI, _PI, _PP]),
+    "amdat_shells_free": (_I, [_P, _P]),
+    "amdat_traj_upload": (_I, [_P, _I, _I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PP]),
+    "amdat_traj_wrap_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _PF, _PP]),
+    "amdat_traj_free": (_I, [_P, _P]),
+    "amdat_list_create": (_I, [_P, _P, _I, _PI, _PP]),
+    "amdat_list_free": (_I, [_P, _P]),
+    "amdat_pairs_upload": (_I, [_P, _I, _PI, _PI, _PI, _I, _PP]),
+    "amdat_pairs_free": (_I, [_P, _P]),
+    "amdat_rho_compute": (_I, [_P, _P, _P, _P, _I, _I, _F, _PI, _I, _I, _PP]),
+    "amdat_rho_download": (_I, [_P, _P, _PD, C.c_size_t]),
+    "amdat_rho_shape": (_I, [_P, _PI, _PI]),
+    "amdat_rho_free": (_I, [_P, _P]),
+    "amdat_sk": (_I, [_P, _P, _P, _PD]),
+    "amdat_fkt": (_I, [_P, _P, _P, _P, _I, _I, _PD, _PD]),
+    "amdat_fs": (_I, [_P, _P, _P, _P, _I, _I, _F, _P, _I, _PD, _PD]),
+    "amdat_msd_ngp": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
+}
+
+
+class AmdatError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (_ERRNAMES.get(code, "?"), code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the library once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libamdat_hip.so is not built (%s); run __graft_entry__.build() "
+                               "or make -C amdat_amd" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        raise AmdatError(rc, lib().amdat_last_error().decode())
+    return rc
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+def ip(a):
+    return _p(a, C.c_int)
+
+
+def fp(a):
+    return _p(a, C.c_float)
+
+
+def dp(a):
+    return _p(a, C.c_double)
+
+
+def sp(a):
+    return _p(a, C.c_int16)
+
+
+# ---- host integer work --------------------------------------------------
+
+def ts_n_timegaps(nexp, nes, frt=0):
+    return check(lib().amdat_ts_n_timegaps(nexp, nes, frt))
+
+
+def ts_timelist(nexp, nes, base, first_exp=0, dt=1.0):
+    T = check(lib().amdat_ts_n_timesteps(nexp, nes))
+    out = np.zeros(T, np.float32)
+    check(lib().amdat_ts_timelist(nexp, nes, base, first_exp, dt, fp(out), T))
+    return out
+
+
+def ts_displacement_times(nexp, nes, frt, timelist):
+    ntg = ts_n_timegaps(nexp, nes, frt)
+    tl = np.ascontiguousarray(timelist, np.float32)
+    out = np.zeros(ntg, np.float32)
+    check(lib().amdat_ts_displacement_times(nexp, nes, frt, fp(tl), fp(out), ntg))
+    return out
+
+
+def ts_pairs(nexp, nes, frt=0, fullblock=0, limit=0):
+    n = C.c_int(0)
+    check(lib().amdat_ts_pairs(nexp, nes, frt, fullblock, limit, None, None, None, 0, C.byref(n)))
+    tg = np.zeros(n.value, np.int32)
+    t0 = np.zeros(n.value, np.int32)
+    t1 = np.zeros(n.value, np.int32)
+    check(lib().amdat_ts_pairs(nexp, nes, frt, fullblock, limit, ip(tg), ip(t0), ip(t1), n.value, C.byref(n)))
+    return tg, t0, t1
+
+
+def ts_weighting(nexp, nes, frt=0, fullblock=0, limit=0):
+    ntg = ts_n_timegaps(nexp, nes, frt)
+    out = np.zeros(ntg, np.int32)
+    check(lib().amdat_ts_weighting(nexp, nes, frt, fullblock, limit, ip(out), ntg))
+    return out
+
+
+def shells_read(stem, first_bin, n_bins, cap=400000):
+    off = np.zeros(n_bins + 1, np.int32)
+    nvec = np.zeros(3 * cap, np.int32)
+    n = C.c_int(0)
+    check(lib().amdat_shells_read(stem.encode(), first_bin, n_bins, ip(off), ip(nvec), cap, C.byref(n)))
+    return off, nvec[: 3 * n.value].reshape(-1, 3).copy()
+
+
+def shells_generate(first_bin, n_bins, max_per_shell=300, seed=0xA11DA7, cap=2000000):
+    off = np.zeros(n_bins + 1, np.int32)
+    n = C.c_int(0)
+    nvec = np.zeros(3 * cap, np.int32)
+    check(lib().amdat_shells_generate(first_bin, n_bins, max_per_shell, seed, ip(off), ip(nvec), cap, C.byref(n)))
+    return off, nvec[: 3 * n.value].reshape(-1, 3).copy()
+
+
+# ---- device objects -------------------------------------------------------
+
+class Context:
+    """One device context (amdat_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().amdat_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().amdat_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(lib().amdat_ctx_synchronize(self.h))
+
+    def set_stream(self, stream_ptr):
+        check(lib().amdat_ctx_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None))
+
+
+class _Handle:
+    _free = None
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+
+    def close(self):
+        if self.h and self.ctx is not None and self.ctx.h:
+            getattr(lib(), self._free)(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Shells(_Handle):
+    _free = "amdat_shells_free"
+
+    def __init__(self, ctx, shell_off, nvec):
+        off = np.ascontiguousarray(shell_off, np.int32)
+        nv = np.ascontiguousarray(nvec, np.int32).reshape(-1)
+        h = C.c_void_p()
+        check(lib().amdat_shells_upload(ctx.h, len(off) - 1, ip(off), ip(nv), C.byref(h)))
+        super().__init__(ctx, h)
+        self.off = off
+        self.nvec = nv.reshape(-1, 3)
+
+
+class Trajectory(_Handle):
+    """Device-resident trajectory: SoA [T][N] FP32 wrapped + int16 images."""
+    _free = "amdat_traj_free"
+
+    def __init__(self, ctx, x, y, z, ix, iy, iz, box):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        ix, iy, iz = (np.ascontiguousarray(a, np.int16) for a in (ix, iy, iz))
+        box = np.ascontiguousarray(box, np.float32).reshape(-1, 3)
+        T, N = x.shape
+        h = C.c_void_p()
+        check(lib().amdat_traj_upload(ctx.h, N, T, fp(x), fp(y), fp(z), sp(ix), sp(iy), sp(iz), fp(box), C.byref(h)))
+        super().__init__(ctx, h)
+        self.N, self.T = N, T
+
+    @classmethod
+    def from_device(cls, ctx, N, T, ptrs, box):
+        """Wrap device buffers (x, y, z, ix, iy, iz data pointers) without copying."""
+        self = cls.__new__(cls)
+        box = np.ascontiguousarray(box, np.float32).reshape(-1, 3)
+        h = C.c_void_p()
+        check(lib().amdat_traj_wrap_device(ctx.h, N, T, *[C.c_void_p(p) for p in ptrs], fp(box), C.byref(h)))
+        _Handle.__init__(self, ctx, h)
+        self.N, self.T = N, T
+        return self
+
+
+class AtomList(_Handle):
+    _free = "amdat_list_free"
+
+    def __init__(self, ctx, traj, indices=None):
+        h = C.c_void_p()
+        if indices is None:
+            check(lib().amdat_list_create(ctx.h, traj.h, traj.N, None, C.byref(h)))
+            self.n = traj.N
+        else:
+            idx = np.ascontiguousarray(indices, np.int32)
+            check(lib().amdat_list_create(ctx.h, traj.h, len(idx), ip(idx), C.byref(h)))
+            self.n = len(idx)
+        super().__init__(ctx, h)
+
+
+class Pairs(_Handle):
+    _free = "amdat_pairs_free"
+
+    def __init__(self, ctx, tg, t0, t1, n_timegaps):
+        tg, t0, t1 = (np.ascontiguousarray(a, np.int32) for a in (tg, t0, t1))
+        h = C.c_void_p()
+        check(lib().amdat_pairs_upload(ctx.h, len(tg), ip(tg), ip(t0), ip(t1), n_timegaps, C.byref(h)))
+        super().__init__(ctx, h)
+        self.n = len(tg)
+        self.ntg = n_timegaps
+
+
+class Rho(_Handle):
+    _free = "amdat_rho_free"
+
+    def __init__(self, ctx, traj, alist, shells, first, last, length_scale, frames=None, precision=PREC_FP32C):
+        h = C.c_void_p()
+        fr = None if frames is None else np.ascontiguousarray(frames, np.int32)
+        check(lib().amdat_rho_compute(ctx.h, traj.h, alist.h if alist is not None else None, shells.h, first, last,
+                                      length_scale, ip(fr), 0 if fr is None else len(fr), precision, C.byref(h)))
+        super().__init__(ctx, h)
+        nf, nv = C.c_int(), C.c_int()
+        check(lib().amdat_rho_shape(h, C.byref(nf), C.byref(nv)))
+        self.n_frames, self.K = nf.value, nv.value
+        self.n_shells = last - first + 1
+
+    def download(self):
+        out = np.zeros((self.n_frames, self.K, 2), np.float64)
+        check(lib().amdat_rho_download(self.ctx.h, self.h, dp(out), out.size))
+        return out[..., 0] + 1j * out[..., 1]
+
+
+def sk(ctx, rho1, rho2=None):
+    S = np.zeros(rho1.n_shells, np.float64)
+    check(lib().amdat_sk(ctx.h, rho1.h, (rho2 or rho1).h, dp(S)))
+    return S
+
+
+def fkt(ctx, rho1, rho2, pairs, wn1=0, wn2=None):
+    wn2 = rho1.n_shells - 1 if wn2 is None else wn2
+    F = np.zeros((pairs.ntg, wn2 - wn1 + 1), np.float64)
+    cnt = np.zeros(pairs.ntg, np.float64)
+    check(lib().amdat_fkt(ctx.h, rho1.h, rho2.h, pairs.h, wn1, wn2, dp(F), dp(cnt)))
+    return F, cnt
+
+
+def fs(ctx, traj, alist, shells, first, last, length_scale, pairs, precision=PREC_FP32C):
+    F = np.zeros((pairs.ntg, last - first + 1), np.float64)
+    cnt = np.zeros(pairs.ntg, np.float64)
+    check(lib().amdat_fs(ctx.h, traj.h, alist.h if alist is not None else None, shells.h, first, last,
+                         length_scale, pairs.h, precision, dp(F), dp(cnt)))
+    return F, cnt
+
+
+def msd_ngp(ctx, traj, alist, pairs):
+    s2 = np.zeros(pairs.ntg, np.float64)
+    s4 = np.zeros(pairs.ntg, np.float64)
+    cnt = np.zeros(pairs.ntg, np.float64)
+    check(lib().amdat_msd_ngp(ctx.h, traj.h, alist.h if alist is not None else None, pairs.h, dp(s2), dp(s4),
+                              dp(cnt)))
+    return s2, s4, cnt

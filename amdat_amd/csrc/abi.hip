@@ -1,0 +1,661 @@
+// abi.hip -- C ABI of the device path (include/amdat_hip.h): contexts,
+// device-resident trajectories, shell tables, pair tables and densities,
+// and the entry points that launch the kernels of kernels.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/amdat_hip.h"
+#include "abi_internal.h"
+#include "kernels.h"
+
+namespace amdat {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+namespace {
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return set_error(e_ == hipErrorOutOfMemory ? AMDAT_ENOMEM : AMDAT_EDEVICE,  \
+                             std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        reset();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+    hipError_t upload(const T *host, size_t count, hipStream_t st)
+    {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess || count == 0) return e;
+        e = hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+        return hipStreamSynchronize(st);
+    }
+};
+
+}  // namespace
+}  // namespace amdat
+
+using namespace amdat;
+
+struct amdat_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+struct amdat_traj {
+    TrajDev dev;
+    std::vector<float> box;  // host copy [T][3]
+    DevBuf<float> x, y, z, dbox;
+    DevBuf<int16_t> ix, iy, iz;
+};
+
+struct amdat_list {
+    int n = 0;
+    int n_atoms = 0;
+    DevBuf<int> idx;  // empty = all atoms in order
+};
+
+struct KPlanOwned {
+    KPlan kp;
+    DevBuf<uint32_t> kpack;
+    DevBuf<int> kdst, chunkM;
+};
+
+struct amdat_shells {
+    int n_shells = 0;
+    std::vector<int> off, nvec;
+    std::map<std::tuple<int, int, int>, std::unique_ptr<KPlanOwned>> plans;
+};
+
+struct amdat_pairs {
+    int n = 0, ntg = 0;
+    std::vector<int> tg, t0, t1;
+    std::vector<int> group_off, order;  // pairs grouped by time gap, reference order within a gap
+    DevBuf<int> d_t0, d_t1, d_group_off, d_order;
+};
+
+struct amdat_rho {
+    int n_frames = 0, K = 0, n_list = 0, T = 0;
+    int first_shell = 0, n_shells = 0;
+    std::vector<int> frames;     // frame index per row
+    std::vector<int> frame_row;  // row per frame, -1 absent
+    std::vector<int> shell_off;  // relative to the first computed vector
+    DevBuf<double2> d;
+    DevBuf<int> d_shell_off;
+};
+
+namespace {
+
+hipStream_t stream_of(amdat_ctx *c) { return c->stream ? c->stream : c->own; }
+
+// Vectors per thread and the resulting chunking for K vectors with max
+// component M: minimise padded work plus table builds (each chunk rebuilds
+// 3(M+1) table entries per atom, about 25 VALU ops each, against 8 ops per
+// padded (atom, vector) update).
+int choose_R(int K, int M)
+{
+    static const int Rs[] = {4, 8, 16};
+    int best = 4;
+    double best_cost = 1e300;
+    for (int R : Rs) {
+        const int KB = kBlock * R;
+        const int nch = (K + KB - 1) / KB;
+        const double cost = (double)nch * ((double)KB * 8.0 + 75.0 * (M + 1));
+        if (cost < best_cost * 0.98) { best_cost = cost; best = R; }
+    }
+    return best;
+}
+
+int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned **out)
+{
+    const int v0 = sh->off[first], v1 = sh->off[last + 1];
+    const int K = v1 - v0;
+    if (K <= 0) return set_error(AMDAT_EINVAL, "shell range holds no vectors");
+    int M = 0;
+    for (int v = v0; v < v1; v++)
+        for (int c = 0; c < 3; c++) M = std::max(M, std::abs(sh->nvec[3 * v + c]));
+    if (M > 127) return set_error(AMDAT_ERANGE, "k-vector component beyond +-127");
+    const int R = choose_R(K, M);
+    auto key = std::make_tuple(first, last, R);
+    auto it = sh->plans.find(key);
+    if (it != sh->plans.end()) { *out = it->second.get(); return AMDAT_OK; }
+
+    auto P = std::make_unique<KPlanOwned>();
+    KPlan &kp = P->kp;
+    kp.K = K;
+    kp.R = R;
+    kp.KB = kBlock * R;
+    kp.n_chunks = (K + kp.KB - 1) / kp.KB;
+    kp.MM = plan_table_width(M);
+    const size_t Kp = (size_t)kp.n_chunks * kp.KB;
+    std::vector<uint32_t> pack(Kp);
+    std::vector<int> dst(Kp), cm(kp.n_chunks);
+    struct V { int nx, ny, nz, d; };
+    for (int c = 0; c < kp.n_chunks; c++) {
+        std::vector<V> vs;
+        const int lo = c * kp.KB, hi = std::min(K, lo + kp.KB);
+        int Mc = 0;
+        for (int v = lo; v < hi; v++) {
+            int nx = sh->nvec[3 * (v0 + v)], ny = sh->nvec[3 * (v0 + v) + 1], nz = sh->nvec[3 * (v0 + v) + 2];
+            int d = v;
+            if (nz < 0) { nx = -nx; ny = -ny; nz = -nz; d |= (1 << 30); }
+            Mc = std::max(Mc, std::max(std::abs(nx), std::max(std::abs(ny), nz)));
+            vs.push_back({nx, ny, nz, d});
+        }
+        std::stable_sort(vs.begin(), vs.end(), [](const V &a, const V &b) {
+            if (a.nx != b.nx) return a.nx < b.nx;
+            if (a.ny != b.ny) return a.ny < b.ny;
+            return a.nz < b.nz;
+        });
+        cm[c] = Mc;
+        for (int q = 0; q < kp.KB; q++) {
+            const size_t pos = (size_t)c * kp.KB + q;
+            if (q < (int)vs.size()) {
+                pack[pos] = (uint32_t)(vs[q].nx + 128) | ((uint32_t)(vs[q].ny + 128) << 8) | ((uint32_t)vs[q].nz << 16);
+                dst[pos] = vs[q].d;
+            } else {
+                pack[pos] = 128u | (128u << 8);
+                dst[pos] = -1;
+            }
+        }
+    }
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(P->kpack.upload(pack.data(), pack.size(), st));
+    HIP_TRY(P->kdst.upload(dst.data(), dst.size(), st));
+    HIP_TRY(P->chunkM.upload(cm.data(), cm.size(), st));
+    kp.kpack = P->kpack.p;
+    kp.kdst = P->kdst.p;
+    kp.chunkM = P->chunkM.p;
+    *out = P.get();
+    sh->plans[key] = std::move(P);
+    return AMDAT_OK;
+}
+
+// Atom splits so that the launch holds enough blocks to fill the chip,
+// while each split still streams at least 8 tiles of atoms.
+int choose_splits(long blocks_without_split, int n_list)
+{
+    const long target = 2048;
+    int s = 1;
+    if (blocks_without_split < target) s = (int)((target + blocks_without_split - 1) / blocks_without_split);
+    const int max_s = std::max(1, n_list / 256);
+    return std::max(1, std::min(s, max_s));
+}
+
+const int *list_ptr(const amdat_list *l) { return l && l->idx.p ? l->idx.p : nullptr; }
+int list_n(const amdat_traj *tr, const amdat_list *l) { return l ? l->n : tr->dev.N; }
+
+}  // namespace
+
+extern "C" {
+
+const char *amdat_last_error(void) { return g_last_error.c_str(); }
+
+int amdat_ctx_create(int device, amdat_ctx **out)
+{
+    if (!out) return set_error(AMDAT_EINVAL, "amdat_ctx_create: null output");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_error(AMDAT_EINVAL, "amdat_ctx_create: no such device");
+    HIP_TRY(hipSetDevice(device));
+    auto c = std::make_unique<amdat_ctx>();
+    c->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+    *out = c.release();
+    return AMDAT_OK;
+}
+
+int amdat_ctx_destroy(amdat_ctx *ctx)
+{
+    if (!ctx) return AMDAT_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return AMDAT_OK;
+}
+
+int amdat_ctx_set_stream(amdat_ctx *ctx, void *stream)
+{
+    if (!ctx) return set_error(AMDAT_EINVAL, "null context");
+    ctx->stream = (hipStream_t)stream;
+    return AMDAT_OK;
+}
+
+int amdat_ctx_synchronize(amdat_ctx *ctx)
+{
+    if (!ctx) return set_error(AMDAT_EINVAL, "null context");
+    HIP_TRY(hipStreamSynchronize(stream_of(ctx)));
+    return AMDAT_OK;
+}
+
+int amdat_shells_upload(amdat_ctx *ctx, int n_shells, const int *shell_off, const int *nvec, amdat_shells **out)
+{
+    if (!ctx || !shell_off || !out || n_shells < 1) return set_error(AMDAT_EINVAL, "amdat_shells_upload: bad arguments");
+    const int K = shell_off[n_shells];
+    if (shell_off[0] != 0 || K < 0 || (K > 0 && !nvec)) return set_error(AMDAT_EINVAL, "amdat_shells_upload: bad offsets");
+    for (int s = 0; s < n_shells; s++)
+        if (shell_off[s + 1] < shell_off[s]) return set_error(AMDAT_EINVAL, "amdat_shells_upload: offsets not ascending");
+    auto sh = std::make_unique<amdat_shells>();
+    sh->n_shells = n_shells;
+    sh->off.assign(shell_off, shell_off + n_shells + 1);
+    sh->nvec.assign(nvec, nvec + 3 * (size_t)K);
+    for (int v = 0; v < 3 * K; v++)
+        if (std::abs(sh->nvec[v]) > 127) return set_error(AMDAT_ERANGE, "amdat_shells_upload: component beyond +-127");
+    *out = sh.release();
+    return AMDAT_OK;
+}
+
+int amdat_shells_free(amdat_ctx *, amdat_shells *sh)
+{
+    delete sh;
+    return AMDAT_OK;
+}
+
+static int traj_finish(amdat_ctx *ctx, amdat_traj *t, int n_atoms, int n_frames, const float *box)
+{
+    t->dev.N = n_atoms;
+    t->dev.T = n_frames;
+    t->box.assign(box, box + 3 * (size_t)n_frames);
+    HIP_TRY(t->dbox.upload(t->box.data(), t->box.size(), stream_of(ctx)));
+    t->dev.box = t->dbox.p;
+    return AMDAT_OK;
+}
+
+int amdat_traj_upload(amdat_ctx *ctx, int n_atoms, int n_frames, const float *x, const float *y, const float *z,
+                      const int16_t *ix, const int16_t *iy, const int16_t *iz, const float *box, amdat_traj **out)
+{
+    if (!ctx || !out || n_atoms < 1 || n_frames < 1 || !x || !y || !z || !ix || !iy || !iz || !box)
+        return set_error(AMDAT_EINVAL, "amdat_traj_upload: bad arguments");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto t = std::make_unique<amdat_traj>();
+    const size_t n = (size_t)n_atoms * n_frames;
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(t->x.upload(x, n, st));
+    HIP_TRY(t->y.upload(y, n, st));
+    HIP_TRY(t->z.upload(z, n, st));
+    HIP_TRY(t->ix.upload(ix, n, st));
+    HIP_TRY(t->iy.upload(iy, n, st));
+    HIP_TRY(t->iz.upload(iz, n, st));
+    t->dev.x = t->x.p; t->dev.y = t->y.p; t->dev.z = t->z.p;
+    t->dev.ix = t->ix.p; t->dev.iy = t->iy.p; t->dev.iz = t->iz.p;
+    int rc = traj_finish(ctx, t.get(), n_atoms, n_frames, box);
+    if (rc) return rc;
+    *out = t.release();
+    return AMDAT_OK;
+}
+
+int amdat_traj_wrap_device(amdat_ctx *ctx, int n_atoms, int n_frames, const float *d_x, const float *d_y,
+                           const float *d_z, const int16_t *d_ix, const int16_t *d_iy, const int16_t *d_iz,
+                           const float *box_host, amdat_traj **out)
+{
+    if (!ctx || !out || n_atoms < 1 || n_frames < 1 || !d_x || !d_y || !d_z || !d_ix || !d_iy || !d_iz || !box_host)
+        return set_error(AMDAT_EINVAL, "amdat_traj_wrap_device: bad arguments");
+    auto t = std::make_unique<amdat_traj>();
+    t->dev.x = d_x; t->dev.y = d_y; t->dev.z = d_z;
+    t->dev.ix = d_ix; t->dev.iy = d_iy; t->dev.iz = d_iz;
+    int rc = traj_finish(ctx, t.get(), n_atoms, n_frames, box_host);
+    if (rc) return rc;
+    *out = t.release();
+    return AMDAT_OK;
+}
+
+int amdat_traj_free(amdat_ctx *, amdat_traj *tr)
+{
+    delete tr;
+    return AMDAT_OK;
+}
+
+int amdat_list_create(amdat_ctx *ctx, const amdat_traj *tr, int n, const int *indices, amdat_list **out)
+{
+    if (!ctx || !tr || !out || n < 0) return set_error(AMDAT_EINVAL, "amdat_list_create: bad arguments");
+    auto l = std::make_unique<amdat_list>();
+    l->n = n;
+    l->n_atoms = tr->dev.N;
+    if (indices) {
+        for (int a = 0; a < n; a++)
+            if (indices[a] < 0 || indices[a] >= tr->dev.N)
+                return set_error(AMDAT_ERANGE, "amdat_list_create: atom index out of range");
+        HIP_TRY(l->idx.upload(indices, (size_t)n, stream_of(ctx)));
+    } else if (n != tr->dev.N && n != 0) {
+        return set_error(AMDAT_EINVAL, "amdat_list_create: NULL indices needs n == n_atoms or n == 0");
+    }
+    *out = l.release();
+    return AMDAT_OK;
+}
+
+int amdat_list_free(amdat_ctx *, amdat_list *l)
+{
+    delete l;
+    return AMDAT_OK;
+}
+
+int amdat_pairs_upload(amdat_ctx *ctx, int n_pairs, const int *tg, const int *t0, const int *t1, int n_timegaps,
+                       amdat_pairs **out)
+{
+    if (!ctx || !out || n_pairs < 0 || n_timegaps < 1 || (n_pairs > 0 && (!tg || !t0 || !t1)))
+        return set_error(AMDAT_EINVAL, "amdat_pairs_upload: bad arguments");
+    auto p = std::make_unique<amdat_pairs>();
+    p->n = n_pairs;
+    p->ntg = n_timegaps;
+    p->tg.assign(tg, tg + n_pairs);
+    p->t0.assign(t0, t0 + n_pairs);
+    p->t1.assign(t1, t1 + n_pairs);
+    for (int i = 0; i < n_pairs; i++) {
+        if (tg[i] < 0 || tg[i] >= n_timegaps) return set_error(AMDAT_ERANGE, "amdat_pairs_upload: time gap out of range");
+        if (t0[i] < 0 || t1[i] < 0) return set_error(AMDAT_ERANGE, "amdat_pairs_upload: negative frame index");
+    }
+    p->group_off.assign(n_timegaps + 1, 0);
+    for (int i = 0; i < n_pairs; i++) p->group_off[tg[i] + 1]++;
+    for (int g = 0; g < n_timegaps; g++) p->group_off[g + 1] += p->group_off[g];
+    p->order.resize(n_pairs);
+    std::vector<int> fill(p->group_off.begin(), p->group_off.end() - 1);
+    for (int i = 0; i < n_pairs; i++) p->order[fill[tg[i]]++] = i;
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(p->d_t0.upload(t0, n_pairs, st));
+    HIP_TRY(p->d_t1.upload(t1, n_pairs, st));
+    HIP_TRY(p->d_group_off.upload(p->group_off.data(), p->group_off.size(), st));
+    HIP_TRY(p->d_order.upload(p->order.data(), p->order.size(), st));
+    *out = p.release();
+    return AMDAT_OK;
+}
+
+int amdat_pairs_free(amdat_ctx *, amdat_pairs *p)
+{
+    delete p;
+    return AMDAT_OK;
+}
+
+static int check_pairs_frames(const amdat_pairs *p, int T)
+{
+    for (int i = 0; i < p->n; i++)
+        if (p->t0[i] >= T || p->t1[i] >= T) return set_error(AMDAT_ERANGE, "pair frame index beyond the trajectory");
+    return AMDAT_OK;
+}
+
+int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_shells *sh_c,
+                      int first_shell, int last_shell, float length_scale, const int *frames, int n_frames,
+                      int precision, amdat_rho **out)
+{
+    clear_error();
+    if (!ctx || !tr || !sh_c || !out) return set_error(AMDAT_EINVAL, "amdat_rho_compute: null argument");
+    amdat_shells *sh = const_cast<amdat_shells *>(sh_c);
+    if (first_shell < 0 || last_shell < first_shell || last_shell >= sh->n_shells)
+        return set_error(AMDAT_ERANGE, "amdat_rho_compute: shell range outside the table");
+    if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_rho_compute: length scale must be > 0");
+    if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64)
+        return set_error(AMDAT_EINVAL, "amdat_rho_compute: unknown precision");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int T = tr->dev.T;
+    std::vector<int> fr;
+    if (frames) {
+        if (n_frames < 1) return set_error(AMDAT_EINVAL, "amdat_rho_compute: empty frame list");
+        fr.assign(frames, frames + n_frames);
+        for (int f : fr)
+            if (f < 0 || f >= T) return set_error(AMDAT_ERANGE, "amdat_rho_compute: frame out of range");
+    } else {
+        fr.resize(T);
+        std::iota(fr.begin(), fr.end(), 0);
+    }
+    KPlanOwned *plan = nullptr;
+    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan);
+    if (rc) return rc;
+    const KPlan &kp = plan->kp;
+    const int nl = list_n(tr, list);
+    auto r = std::make_unique<amdat_rho>();
+    r->n_frames = (int)fr.size();
+    r->K = kp.K;
+    r->n_list = nl;
+    r->T = T;
+    r->first_shell = first_shell;
+    r->n_shells = last_shell - first_shell + 1;
+    r->frames = fr;
+    r->frame_row.assign(T, -1);
+    for (int i = 0; i < r->n_frames; i++) r->frame_row[fr[i]] = i;
+    r->shell_off.resize(r->n_shells + 1);
+    for (int s = 0; s <= r->n_shells; s++) r->shell_off[s] = sh->off[first_shell + s] - sh->off[first_shell];
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(r->d_shell_off.upload(r->shell_off.data(), r->shell_off.size(), st));
+    HIP_TRY(r->d.alloc((size_t)r->n_frames * kp.K));
+    if (nl > 0) {
+        DevBuf<int> dfr;
+        HIP_TRY(dfr.upload(fr.data(), fr.size(), st));
+        const int n_splits = choose_splits((long)kp.n_chunks * r->n_frames, nl);
+        const double inv_ell = 1.0 / (double)length_scale;
+        // Launch in frame batches so a launch stays within grid limits.
+        const int fb = 65535;
+        DevBuf<double2> part;
+        if (n_splits > 1) HIP_TRY(part.alloc((size_t)std::min(fb, r->n_frames) * n_splits * kp.K));
+        for (int f0 = 0; f0 < r->n_frames; f0 += fb) {
+            const int nf = std::min(fb, r->n_frames - f0);
+            double2 *dst = r->d.p + (size_t)f0 * kp.K;
+            HIP_TRY(launch_rho(precision, tr->dev, list_ptr(list), nl, dfr.p + f0, nf, kp, inv_ell, n_splits,
+                               n_splits > 1 ? part.p : dst, st));
+            if (n_splits > 1) HIP_TRY(launch_split_reduce2(part.p, nf, n_splits, kp.K, dst, st));
+        }
+        HIP_TRY(hipStreamSynchronize(st));
+    } else {
+        HIP_TRY(hipMemsetAsync(r->d.p, 0, r->d.n * sizeof(double2), st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    *out = r.release();
+    return AMDAT_OK;
+}
+
+int amdat_rho_download(amdat_ctx *ctx, const amdat_rho *rho, double *out, size_t n_doubles)
+{
+    if (!ctx || !rho || !out) return set_error(AMDAT_EINVAL, "amdat_rho_download: null argument");
+    const size_t need = (size_t)rho->n_frames * rho->K * 2;
+    if (n_doubles < need) return set_error(AMDAT_EINVAL, "amdat_rho_download: output too small");
+    HIP_TRY(hipMemcpyAsync(out, rho->d.p, need * sizeof(double), hipMemcpyDeviceToHost, stream_of(ctx)));
+    HIP_TRY(hipStreamSynchronize(stream_of(ctx)));
+    return AMDAT_OK;
+}
+
+int amdat_rho_shape(const amdat_rho *rho, int *n_frames, int *n_vectors)
+{
+    if (!rho) return set_error(AMDAT_EINVAL, "amdat_rho_shape: null argument");
+    if (n_frames) *n_frames = rho->n_frames;
+    if (n_vectors) *n_vectors = rho->K;
+    return AMDAT_OK;
+}
+
+int amdat_rho_free(amdat_ctx *, amdat_rho *rho)
+{
+    delete rho;
+    return AMDAT_OK;
+}
+
+int amdat_sk(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, double *S)
+{
+    clear_error();
+    if (!ctx || !r1 || !r2 || !S) return set_error(AMDAT_EINVAL, "amdat_sk: null argument");
+    if (r1->K != r2->K || r1->n_frames != r2->n_frames || r1->frames != r2->frames || r1->shell_off != r2->shell_off)
+        return set_error(AMDAT_EINVAL, "amdat_sk: densities cover different frames or vectors");
+    hipStream_t st = stream_of(ctx);
+    const int nf = r1->n_frames;
+    std::vector<int> rows(nf), goff = {0, nf};
+    std::iota(rows.begin(), rows.end(), 0);
+    const double norm = (double)nf * (double)r1->n_list;
+    DevBuf<int> d_rows, d_goff;
+    DevBuf<double> d_norm, d_out;
+    HIP_TRY(d_rows.upload(rows.data(), rows.size(), st));
+    HIP_TRY(d_goff.upload(goff.data(), goff.size(), st));
+    HIP_TRY(d_norm.upload(&norm, 1, st));
+    HIP_TRY(d_out.alloc(r1->n_shells));
+    HIP_TRY(launch_corr_shell(r1->d.p, r2->d.p, r1->K, d_goff.p, d_rows.p, d_rows.p, 1, r1->d_shell_off.p,
+                              r1->n_shells, d_norm.p, d_out.p, st));
+    HIP_TRY(hipMemcpyAsync(S, d_out.p, sizeof(double) * r1->n_shells, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_fkt(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs, int wn1, int wn2,
+              double *F, double *count)
+{
+    clear_error();
+    if (!ctx || !r1 || !r2 || !pairs || !F || !count) return set_error(AMDAT_EINVAL, "amdat_fkt: null argument");
+    if (r1->K != r2->K || r1->shell_off != r2->shell_off || r1->T != r2->T)
+        return set_error(AMDAT_EINVAL, "amdat_fkt: densities cover different vectors");
+    if (wn1 < 0 || wn2 < wn1 || wn2 >= r1->n_shells)
+        return set_error(AMDAT_ERANGE, "amdat_fkt: shell range outside the densities");
+    int rc = check_pairs_frames(pairs, r1->T);
+    if (rc) return rc;
+    hipStream_t st = stream_of(ctx);
+    const int ntg = pairs->ntg, nsh = wn2 - wn1 + 1;
+    std::vector<int> row0(pairs->n), row1(pairs->n);
+    std::vector<double> norm(ntg, 0.0);
+    for (int q = 0; q < pairs->n; q++) {
+        const int i = pairs->order[q];
+        row0[q] = r1->frame_row[pairs->t0[i]];
+        row1[q] = r2->frame_row[pairs->t1[i]];
+        if (row0[q] < 0 || row1[q] < 0) return set_error(AMDAT_ERANGE, "amdat_fkt: a pair uses a frame the density lacks");
+        norm[pairs->tg[i]] += (double)r1->n_list;
+    }
+    for (int g = 0; g < ntg; g++) count[g] = norm[g];
+    if (pairs->n == 0) {
+        std::fill(F, F + (size_t)ntg * nsh, 0.0);
+        return AMDAT_OK;
+    }
+    DevBuf<int> d_r0, d_r1;
+    DevBuf<double> d_norm, d_out;
+    HIP_TRY(d_r0.upload(row0.data(), row0.size(), st));
+    HIP_TRY(d_r1.upload(row1.data(), row1.size(), st));
+    HIP_TRY(d_norm.upload(norm.data(), norm.size(), st));
+    HIP_TRY(d_out.alloc((size_t)ntg * nsh));
+    HIP_TRY(launch_corr_shell(r1->d.p, r2->d.p, r1->K, pairs->d_group_off.p, d_r0.p, d_r1.p, ntg,
+                              r1->d_shell_off.p + wn1, nsh, d_norm.p, d_out.p, st));
+    HIP_TRY(hipMemcpyAsync(F, d_out.p, sizeof(double) * ntg * nsh, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_shells *sh_c, int first_shell,
+             int last_shell, float length_scale, const amdat_pairs *pairs, int precision, double *Fs, double *count)
+{
+    clear_error();
+    if (!ctx || !tr || !sh_c || !pairs || !Fs || !count) return set_error(AMDAT_EINVAL, "amdat_fs: null argument");
+    amdat_shells *sh = const_cast<amdat_shells *>(sh_c);
+    if (first_shell < 0 || last_shell < first_shell || last_shell >= sh->n_shells)
+        return set_error(AMDAT_ERANGE, "amdat_fs: shell range outside the table");
+    if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_fs: length scale must be > 0");
+    if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64)
+        return set_error(AMDAT_EINVAL, "amdat_fs: unknown precision");
+    int rc = check_pairs_frames(pairs, tr->dev.T);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    KPlanOwned *plan = nullptr;
+    rc = build_plan(ctx, sh, first_shell, last_shell, &plan);
+    if (rc) return rc;
+    const KPlan &kp = plan->kp;
+    const int nl = list_n(tr, list);
+    const int ntg = pairs->ntg, nsh = last_shell - first_shell + 1;
+    std::vector<double> norm(ntg, 0.0);
+    for (int i = 0; i < pairs->n; i++) norm[pairs->tg[i]] += (double)nl;
+    for (int g = 0; g < ntg; g++) count[g] = norm[g];
+    if (pairs->n == 0 || nl == 0) {
+        std::fill(Fs, Fs + (size_t)ntg * nsh, 0.0);
+        return AMDAT_OK;
+    }
+    hipStream_t st = stream_of(ctx);
+    const int n_splits = choose_splits((long)kp.n_chunks * pairs->n, nl);
+    DevBuf<double> V, d_norm, d_out;
+    DevBuf<int> d_soff;
+    std::vector<int> soff(nsh + 1);
+    for (int s = 0; s <= nsh; s++) soff[s] = sh->off[first_shell + s] - sh->off[first_shell];
+    HIP_TRY(d_soff.upload(soff.data(), soff.size(), st));
+    HIP_TRY(d_norm.upload(norm.data(), norm.size(), st));
+    HIP_TRY(V.alloc((size_t)pairs->n * n_splits * kp.K));
+    const double inv_ell = 1.0 / (double)length_scale;
+    const int pb = 65535;
+    for (int p0 = 0; p0 < pairs->n; p0 += pb) {
+        const int np = std::min(pb, pairs->n - p0);
+        HIP_TRY(launch_fs(precision, tr->dev, list_ptr(list), nl, pairs->d_t0.p + p0, pairs->d_t1.p + p0, np, kp,
+                          inv_ell, n_splits, V.p + (size_t)p0 * n_splits * kp.K, st));
+    }
+    HIP_TRY(d_out.alloc((size_t)ntg * nsh));
+    HIP_TRY(launch_fs_shell(V.p, n_splits, kp.K, pairs->d_group_off.p, pairs->d_order.p, ntg, d_soff.p, nsh,
+                            d_norm.p, d_out.p, st));
+    HIP_TRY(hipMemcpyAsync(Fs, d_out.p, sizeof(double) * ntg * nsh, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs, double *s2,
+                  double *s4, double *count)
+{
+    clear_error();
+    if (!ctx || !tr || !pairs || !s2 || !s4 || !count) return set_error(AMDAT_EINVAL, "amdat_msd_ngp: null argument");
+    int rc = check_pairs_frames(pairs, tr->dev.T);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int nl = list_n(tr, list);
+    const int ntg = pairs->ntg;
+    for (int g = 0; g < ntg; g++) count[g] = 0.0;
+    for (int i = 0; i < pairs->n; i++) count[pairs->tg[i]] += (double)nl;
+    if (pairs->n == 0 || nl == 0) {
+        std::fill(s2, s2 + ntg, 0.0);
+        std::fill(s4, s4 + ntg, 0.0);
+        return AMDAT_OK;
+    }
+    hipStream_t st = stream_of(ctx);
+    const int nb = std::max(1, std::min(64, (nl + kBlock * 4 - 1) / (kBlock * 4)));
+    DevBuf<double> part, d2, d4;
+    HIP_TRY(part.alloc((size_t)pairs->n * nb * 2));
+    const int pb = 65535;
+    for (int p0 = 0; p0 < pairs->n; p0 += pb) {
+        const int np = std::min(pb, pairs->n - p0);
+        HIP_TRY(launch_msd(tr->dev, list_ptr(list), nl, pairs->d_t0.p + p0, pairs->d_t1.p + p0, np, nb,
+                           part.p + (size_t)p0 * nb * 2, st));
+    }
+    HIP_TRY(d2.alloc(ntg));
+    HIP_TRY(d4.alloc(ntg));
+    HIP_TRY(launch_msd_reduce(part.p, nb, pairs->d_group_off.p, pairs->d_order.p, ntg, d2.p, d4.p, st));
+    HIP_TRY(hipMemcpyAsync(s2, d2.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s4, d4.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,240 @@
+// host_core.cpp -- host-side integer work of the C ABI: the blocked-
+// exponential time scheme, time-origin pairs and weights, and shell
+// tables (file reader and rule-based generator).  No device code.
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/amdat_hip.h"
+#include "abi_internal.h"
+
+using namespace amdat;
+
+extern "C" {
+
+int amdat_version(void) { return 1 * 10000 + 0 * 100 + 0; }
+
+int amdat_ts_n_timesteps(int nexp, int nes)
+{
+    if (nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "time scheme: n_exponentials must be >= 1 and n_exponential_steps >= 0");
+    return nes * nexp + 1;
+}
+
+int amdat_ts_n_timegaps(int nexp, int nes, int frt)
+{
+    if (nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "time scheme: n_exponentials must be >= 1 and n_exponential_steps >= 0");
+    return nes + nexp - 1 + (frt ? 0 : 1);
+}
+
+// System::create_timelist, src/system.cpp:3542-3568.  exp_base is a float
+// member; pow(float, int) is evaluated in double.
+int amdat_ts_timelist(int nexp, int nes, float base, int first_exp, float dt, float *out, int cap)
+{
+    if (!out || nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "amdat_ts_timelist: bad arguments");
+    int T = nes * nexp + 1;
+    if (cap < T) return set_error(AMDAT_EINVAL, "amdat_ts_timelist: output capacity smaller than n_timesteps");
+    int ti = 0;
+    float block_start = 0.0f;
+    out[0] = 0.0f;
+    for (int b = 0; b < nexp; b++) {
+        for (int e = 1; e <= nes; e++) {
+            ++ti;
+            double p = std::pow((double)base, (double)(e - 1 + first_exp));
+            out[ti] = (p <= (double)e) ? block_start + (float)e * dt
+                                       : block_start + (float)std::floor(p) * dt;
+        }
+        block_start = out[ti];
+    }
+    return AMDAT_OK;
+}
+
+// System::displacement_times, src/system.cpp:3763-3784.
+int amdat_ts_displacement_times(int nexp, int nes, int frt, const float *tl, float *out, int cap)
+{
+    if (!tl || !out || nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "amdat_ts_displacement_times: bad arguments");
+    int ntg = nes + nexp - 1 + (frt ? 0 : 1);
+    if (cap < ntg) return set_error(AMDAT_EINVAL, "amdat_ts_displacement_times: output capacity smaller than n_timegaps");
+    int T = nes * nexp + 1;
+    int f = frt ? 1 : 0;
+    for (int i = 0; i < nes; i++) out[i] = tl[i + f] - tl[f];
+    for (int i = 1; i < nexp; i++) out[i + nes - 1] = tl[nes * i + f] - tl[f];
+    if (!frt) out[ntg - 1] = tl[T - 1] - tl[0];
+    return AMDAT_OK;
+}
+
+// System::displacement_list, src/system.cpp:4090-4179.  The pair order is
+// the reference's: in-block gaps by gap then block, cross-block gaps by gap,
+// block offset, block; then the (0, last) pair when frt == 0.
+int amdat_ts_pairs(int nexp, int nes, int frt, int fullblock, int limit, int *tg, int *t0, int *t1,
+                   int cap, int *n_pairs)
+{
+    if (nexp < 1 || nes < 0 || limit < 0) return set_error(AMDAT_EINVAL, "amdat_ts_pairs: bad time scheme");
+    if (cap > 0 && (!tg || !t0 || !t1)) return set_error(AMDAT_EINVAL, "amdat_ts_pairs: null output");
+    int f = frt ? 1 : 0;
+    int ntg = nes + nexp - 1 + (1 - f);
+    int T = nes * nexp + 1;
+    long n = 0;
+    auto emit = [&](int g, int a, int b) {
+        if (n < cap) { tg[n] = g; t0[n] = a; t1[n] = b; }
+        ++n;
+    };
+    for (int g = 0; g < nes; g++) {
+        int count = 0;
+        for (int b = 0; b < nexp; b++) {
+            int a = nes * b + f;
+            emit(g, a, a + g);
+            if (++count == limit && limit != 0) break;
+        }
+    }
+    for (int g = nes; g < ntg - 1 + f; g++) {
+        int count = 0;
+        bool stop = false;
+        int btg = g - nes + 1;
+        int n_off = (fullblock ? 1 : 0) * (nes - 1) + 1;
+        for (int e = 0; e < n_off && !stop; e++) {
+            for (int b = 0; b < nexp - btg; b++) {
+                int a = nes * b + e + f;
+                emit(g, a, a + nes * btg);
+                if (++count == limit && limit != 0) { stop = true; break; }
+            }
+        }
+    }
+    if (!frt) emit(ntg - 1, 0, T - 1);
+    if (n > 0x7fffffffL) return set_error(AMDAT_EINVAL, "amdat_ts_pairs: too many pairs");
+    if (n_pairs) *n_pairs = (int)n;
+    return AMDAT_OK;
+}
+
+// System::timegap_weighting, src/system.cpp:4599-4628 (cross-block entries
+// carry the reference's +1).
+int amdat_ts_weighting(int nexp, int nes, int frt, int fullblock, int limit, int *w, int cap)
+{
+    if (!w || nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "amdat_ts_weighting: bad arguments");
+    int f = frt ? 1 : 0;
+    int ntg = nes + nexp - 1 + (1 - f);
+    if (cap < ntg) return set_error(AMDAT_EINVAL, "amdat_ts_weighting: output capacity smaller than n_timegaps");
+    for (int g = 0; g < nes; g++) {
+        w[g] = nexp;
+        if (w[g] > limit && limit != 0) w[g] = limit;
+    }
+    for (int g = nes; g < ntg - 1 + f; g++) {
+        int btg = g - nes + 1;
+        w[g] = (nexp - btg) + (fullblock ? 1 : 0) * (nexp - btg) * (nes - 1) + 1;
+        if (w[g] > limit && limit != 0) w[g] = limit;
+    }
+    if (!frt) w[ntg - 1] = 1;
+    return AMDAT_OK;
+}
+
+// Wave_Vectors::read_vectors, src/wave_vectors.cpp:430-484: one file per
+// bin, whitespace-separated numbers, three per vector.  Numbers must be
+// integral (the reference scales them by the grid spacing).
+int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_off, int *nvec,
+                      int cap_vectors, int *n_vectors)
+{
+    if (!stem || !shell_off || n_bins < 1 || first_bin < 0 || first_bin + n_bins - 1 > 999)
+        return set_error(AMDAT_EINVAL, "amdat_shells_read: bad arguments");
+    int n = 0;
+    shell_off[0] = 0;
+    for (int s = 0; s < n_bins; s++) {
+        char suffix[16];
+        std::snprintf(suffix, sizeof(suffix), ".%03d", first_bin + s);
+        std::string path = std::string(stem) + suffix;
+        FILE *fp = std::fopen(path.c_str(), "r");
+        if (!fp) return set_error(AMDAT_EIO, "Error opening vector file " + path);
+        double v[3];
+        int c = 0;
+        while (std::fscanf(fp, "%lf", &v[c]) == 1) {
+            if (++c == 3) {
+                for (int d = 0; d < 3; d++) {
+                    if (v[d] != std::floor(v[d]) || std::fabs(v[d]) > 127.0) {
+                        std::fclose(fp);
+                        return set_error(AMDAT_EIO, "non-integral or out-of-range vector component in " + path);
+                    }
+                }
+                if (n < cap_vectors && nvec) {
+                    nvec[3 * n] = (int)v[0];
+                    nvec[3 * n + 1] = (int)v[1];
+                    nvec[3 * n + 2] = (int)v[2];
+                }
+                ++n;
+                c = 0;
+            }
+        }
+        std::fclose(fp);
+        if (c != 0) return set_error(AMDAT_EIO, "vector file " + path + " has a trailing partial vector");
+        shell_off[s + 1] = n;
+    }
+    if (n_vectors) *n_vectors = n;
+    if (n > cap_vectors) return set_error(AMDAT_EINVAL, "amdat_shells_read: vector capacity too small");
+    return AMDAT_OK;
+}
+
+namespace {
+inline uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+}  // namespace
+
+// The shell rule of src/qvectors/generate_3d_qvectors.m:6-64: half-space
+// integer vectors with bin = floor((|n|+0.25)*2), components within +-100,
+// at most max_per_shell per shell by a seeded shuffle.  The reference
+// ships one fixed random choice; this generator makes a reproducible one
+// of the same sizes.
+int amdat_shells_generate(int first_bin, int n_bins, int max_per_shell, uint64_t seed,
+                          int *shell_off, int *nvec, int cap_vectors, int *n_vectors)
+{
+    if (!shell_off || n_bins < 1 || first_bin < 1 || first_bin + n_bins - 1 > 300 || max_per_shell < 1)
+        return set_error(AMDAT_EINVAL, "amdat_shells_generate: bad arguments");
+    const int maxc = 100;
+    int last_bin = first_bin + n_bins - 1;
+    std::vector<std::vector<int>> bins(n_bins);
+    auto add = [&](int i, int j, int k) {
+        double l = std::sqrt((double)(i * i + j * j + k * k));
+        int b = (int)std::floor((l + 0.25) * 2.0);
+        if (b >= first_bin && b <= last_bin) {
+            auto &v = bins[b - first_bin];
+            v.push_back(i); v.push_back(j); v.push_back(k);
+        }
+    };
+    double rmax = last_bin / 2.0 + 0.25;
+    int lim = std::min(maxc, (int)std::ceil(rmax));
+    for (int j = 0; j <= lim; j++) add(0, j, 0);
+    for (int i = 1; i <= lim; i++)
+        for (int j = -lim; j <= lim; j++) add(i, j, 0);
+    for (int k = 1; k <= lim; k++)
+        for (int j = -lim; j <= lim; j++)
+            for (int i = -lim; i <= lim; i++) add(i, j, k);
+    uint64_t st = seed;
+    int n = 0;
+    shell_off[0] = 0;
+    for (int s = 0; s < n_bins; s++) {
+        auto &v = bins[s];
+        int cnt = (int)v.size() / 3;
+        // Fisher-Yates on the vector order, then keep the first max_per_shell.
+        for (int a = cnt - 1; a > 0; a--) {
+            int b = (int)(splitmix64(st) % (uint64_t)(a + 1));
+            for (int d = 0; d < 3; d++) std::swap(v[3 * a + d], v[3 * b + d]);
+        }
+        int keep = std::min(cnt, max_per_shell);
+        for (int a = 0; a < keep; a++) {
+            if (n < cap_vectors && nvec)
+                for (int d = 0; d < 3; d++) nvec[3 * n + d] = v[3 * a + d];
+            ++n;
+        }
+        shell_off[s + 1] = n;
+    }
+    if (n_vectors) *n_vectors = n;
+    if (n > cap_vectors) return set_error(AMDAT_EINVAL, "amdat_shells_generate: vector capacity too small");
+    return AMDAT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,651 @@
+// kernels.hip -- gfx950 kernels of the reciprocal-space and time-correlation
+// path: rho_k(t), Fs(k,t), MSD/NGP moments, and the FP64 reductions that turn
+// per-vector sums into per-shell S(k), F(k,t) and Fs(k,t).
+//
+// Phase kernels (rho, Fs), compensated-FP32 form:
+//   * a block owns one chunk of KB = 256*R k-vectors (R per thread, kept in
+//     registers as LDS offsets) and one frame (rho) or time-origin pair (Fs),
+//     and streams the atoms of its split through LDS in tiles of A atoms;
+//   * per tile, every atom gets separable phase tables
+//         e_c(m) = exp(2*pi*i * m * u_c),  u_c = frac(r_c / l),
+//     for m in [-M, M] on x and y and [0, M] on z (vectors are stored with
+//     nz >= 0; a vector with nz < 0 is flipped and its result conjugated).
+//     u_c is an exact 32-bit fixed-point fraction formed in FP64 from the
+//     FP32 coordinate (for Fs from the FP64 displacement rebuilt from image
+//     counts), so m*u_c mod 1 is exact integer arithmetic and each table
+//     entry is one FP32 sincos_turns();
+//   * the hot loop is per (atom, vector): three LDS reads and two complex
+//     products, exp(i k.r) = e_x(nx) e_y(ny) e_z(nz), accumulated with FMA
+//     into an FP32 partial per tile; tile partials are folded into a
+//     TwoSum-compensated FP32 (hi, lo) pair; the block writes hi+lo as FP64.
+// The plain-FP64 form evaluates exp(i k.r) with FP64 sincos per update and
+// accumulates in FP64; it is the reference the FP32 form is measured against.
+#include <algorithm>
+
+#include "kernels.h"
+#include "phase_math.h"
+
+namespace amdat {
+
+namespace {
+
+// Table geometry for half-width MM: row of W float2 per atom,
+// x at [0, 2MM], y at [2MM+1, 4MM+1], z (m >= 0 only) at [4MM+2, 5MM+2].
+// A atoms per tile keeps the tile under 64 KiB so every (atom, vector)
+// read is one ds_read_b64 with a per-lane base and an immediate offset.
+template <int MM>
+struct Geom {
+    static constexpr int W = 5 * MM + 3;
+    static constexpr int OX = MM;
+    static constexpr int OY = 2 * MM + 1 + MM;
+    static constexpr int OZ = 2 * (2 * MM + 1);
+    static constexpr int A_raw = 65536 / (W * 8);
+    static constexpr int A = A_raw >= 32 ? 32 : (A_raw >= 16 ? 16 : 8);
+    static constexpr int G = 8;  // atoms per unrolled group (immediate offsets j*W*8)
+};
+
+__device__ __forceinline__ void unpack_k(uint32_t kp, int &nx, int &ny, int &nz)
+{
+    nx = (int)(kp & 0xFFu) - 128;
+    ny = (int)((kp >> 8) & 0xFFu) - 128;
+    nz = (int)((kp >> 16) & 0xFFu);
+}
+
+// Fill one tile of tables.  sU holds the 3*A fixed-point fractions and
+// sValid the per-atom flags; wave w builds rows w, w+4, ...; lane handles
+// m = lane, lane+64, ... <= Mc.
+template <int MM>
+__device__ __forceinline__ void build_tables(float2 *tab, const uint32_t *sU, const int *sValid, int Mc)
+{
+    using Gm = Geom<MM>;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int row = wave; row < 3 * Gm::A; row += kBlock / 64) {
+        const int i = row / 3;
+        const int c = row - 3 * i;
+        const uint32_t U = sU[row];
+        const bool valid = sValid[i] != 0;
+        float2 *dst = tab + i * Gm::W;
+        for (int m = lane; m <= Mc; m += 64) {
+            float2 v = valid ? sincos_turns(U * (uint32_t)m) : make_float2(0.0f, 0.0f);
+            if (c == 0) {
+                dst[Gm::OX + m] = v;
+                if (m) dst[Gm::OX - m] = make_float2(v.x, -v.y);
+            } else if (c == 1) {
+                dst[Gm::OY + m] = v;
+                if (m) dst[Gm::OY - m] = make_float2(v.x, -v.y);
+            } else {
+                dst[Gm::OZ + m] = v;
+            }
+        }
+    }
+}
+
+// rho_k(t), compensated FP32.  grid = (chunks, frames, splits).
+template <int R, int MM>
+__global__ __launch_bounds__(kBlock) void rho_c32_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
+    const int *__restrict__ list, int n_list, const int *__restrict__ frames,
+    const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, const int *__restrict__ chunkM,
+    int K, int KB, double inv_ell, int per_split, int n_splits, double2 *__restrict__ out)
+{
+    using Gm = Geom<MM>;
+    extern __shared__ float2 tab[];
+    __shared__ uint32_t sU[3 * Gm::A];
+    __shared__ int sValid[Gm::A];
+
+    const int chunk = blockIdx.x, fslot = blockIdx.y, split = blockIdx.z;
+    const int tid = threadIdx.x;
+    const int Mc = chunkM[chunk];
+    const size_t foff = (size_t)frames[fslot] * (size_t)N;
+    const int a_begin = split * per_split;
+    const int a_end = min(n_list, a_begin + per_split);
+
+    const float2 *px[R], *py[R], *pz[R];
+    float hr[R], hi[R], lr[R], li[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int nx, ny, nz;
+        unpack_k(kpack[(size_t)chunk * KB + r * kBlock + tid], nx, ny, nz);
+        px[r] = tab + Gm::OX + nx;
+        py[r] = tab + Gm::OY + ny;
+        pz[r] = tab + Gm::OZ + nz;
+        hr[r] = hi[r] = lr[r] = li[r] = 0.0f;
+    }
+
+    for (int a0 = a_begin; a0 < a_end; a0 += Gm::A) {
+        __syncthreads();
+        for (int e = tid; e < 3 * Gm::A; e += kBlock) {
+            const int i = e / 3, c = e - 3 * (e / 3);
+            const int a = a0 + i;
+            uint32_t U = 0;
+            if (a < a_end) {
+                const int j = list ? list[a] : a;
+                const float *src = c == 0 ? X : (c == 1 ? Y : Z);
+                U = turns_fixed((double)src[foff + j] * inv_ell);
+            }
+            sU[e] = U;
+            if (c == 0) sValid[i] = a < a_end;
+        }
+        __syncthreads();
+        build_tables<MM>(tab, sU, sValid, Mc);
+        __syncthreads();
+
+        float pr[R], pi[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pr[r] = pi[r] = 0.0f;
+        for (int g = 0; g < Gm::A; g += Gm::G) {
+            const int base = g * Gm::W;
+#pragma unroll
+            for (int j = 0; j < Gm::G; ++j) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float2 ex = px[r][base + j * Gm::W];
+                    const float2 ey = py[r][base + j * Gm::W];
+                    const float2 ez = pz[r][base + j * Gm::W];
+                    const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+                    const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                    pr[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, pr[r]));
+                    pi[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, pi[r]));
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            two_sum_acc(hr[r], lr[r], pr[r]);
+            two_sum_acc(hi[r], li[r], pi[r]);
+        }
+    }
+
+    double2 *dst = out + ((size_t)fslot * n_splits + split) * (size_t)K;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int d = kdst[(size_t)chunk * KB + r * kBlock + tid];
+        if (d >= 0) {
+            const double re = (double)hr[r] + (double)lr[r];
+            double im = (double)hi[r] + (double)li[r];
+            if (d & (1 << 30)) im = -im;
+            dst[d & ~(1 << 30)] = make_double2(re, im);
+        }
+    }
+}
+
+// Fs per (pair, vector), compensated FP32: sum over the split's atoms of
+// cos(k.dr).  grid = (chunks, pairs, splits).
+template <int R, int MM>
+__global__ __launch_bounds__(kBlock) void fs_c32_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+    const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
+    const int *__restrict__ T0, const int *__restrict__ T1, const uint32_t *__restrict__ kpack,
+    const int *__restrict__ kdst, const int *__restrict__ chunkM, int K, int KB, double inv_ell,
+    int per_split, int n_splits, double *__restrict__ out)
+{
+    using Gm = Geom<MM>;
+    extern __shared__ float2 tab[];
+    __shared__ uint32_t sU[3 * Gm::A];
+    __shared__ int sValid[Gm::A];
+
+    const int chunk = blockIdx.x, pslot = blockIdx.y, split = blockIdx.z;
+    const int tid = threadIdx.x;
+    const int Mc = chunkM[chunk];
+    const int t0 = T0[pslot], t1 = T1[pslot];
+    const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
+    const int a_begin = split * per_split;
+    const int a_end = min(n_list, a_begin + per_split);
+
+    const float2 *px[R], *py[R], *pz[R];
+    float hs[R], ls[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int nx, ny, nz;
+        unpack_k(kpack[(size_t)chunk * KB + r * kBlock + tid], nx, ny, nz);
+        px[r] = tab + Gm::OX + nx;
+        py[r] = tab + Gm::OY + ny;
+        pz[r] = tab + Gm::OZ + nz;
+        hs[r] = ls[r] = 0.0f;
+    }
+
+    for (int a0 = a_begin; a0 < a_end; a0 += Gm::A) {
+        __syncthreads();
+        for (int e = tid; e < 3 * Gm::A; e += kBlock) {
+            const int i = e / 3, c = e - 3 * (e / 3);
+            const int a = a0 + i;
+            uint32_t U = 0;
+            if (a < a_end) {
+                const int j = list ? list[a] : a;
+                const float *src = c == 0 ? X : (c == 1 ? Y : Z);
+                const int16_t *img = c == 0 ? IX : (c == 1 ? IY : IZ);
+                // dr = (x1 + i1*L1) - (x0 + i0*L0), exact in FP64.
+                const double d = ((double)src[o1 + j] - (double)src[o0 + j]) +
+                                 ((double)img[o1 + j] * (double)box[3 * t1 + c] -
+                                  (double)img[o0 + j] * (double)box[3 * t0 + c]);
+                U = turns_fixed(d * inv_ell);
+            }
+            sU[e] = U;
+            if (c == 0) sValid[i] = a < a_end;
+        }
+        __syncthreads();
+        build_tables<MM>(tab, sU, sValid, Mc);
+        __syncthreads();
+
+        float ps[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) ps[r] = 0.0f;
+        for (int g = 0; g < Gm::A; g += Gm::G) {
+            const int base = g * Gm::W;
+#pragma unroll
+            for (int j = 0; j < Gm::G; ++j) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float2 ex = px[r][base + j * Gm::W];
+                    const float2 ey = py[r][base + j * Gm::W];
+                    const float2 ez = pz[r][base + j * Gm::W];
+                    const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+                    const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                    ps[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, ps[r]));
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) two_sum_acc(hs[r], ls[r], ps[r]);
+    }
+
+    double *dst = out + ((size_t)pslot * n_splits + split) * (size_t)K;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int d = kdst[(size_t)chunk * KB + r * kBlock + tid];
+        if (d >= 0) dst[d & ~(1 << 30)] = (double)hs[r] + (double)ls[r];
+    }
+}
+
+// Plain FP64 rho: FP64 phase, FP64 sincos, FP64 sums.
+template <int R>
+__global__ __launch_bounds__(kBlock) void rho_f64_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
+    const int *__restrict__ list, int n_list, const int *__restrict__ frames,
+    const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, int K, int KB, double kscale,
+    int per_split, int n_splits, double2 *__restrict__ out)
+{
+    const int chunk = blockIdx.x, fslot = blockIdx.y, split = blockIdx.z;
+    const int tid = threadIdx.x;
+    const size_t foff = (size_t)frames[fslot] * (size_t)N;
+    const int a_begin = split * per_split;
+    const int a_end = min(n_list, a_begin + per_split);
+    double kx[R], ky[R], kz[R], ar[R], ai[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int nx, ny, nz;
+        unpack_k(kpack[(size_t)chunk * KB + r * kBlock + tid], nx, ny, nz);
+        kx[r] = kscale * nx; ky[r] = kscale * ny; kz[r] = kscale * nz;
+        ar[r] = ai[r] = 0.0;
+    }
+    for (int a = a_begin; a < a_end; ++a) {
+        const int j = list ? list[a] : a;
+        const double x = X[foff + j], y = Y[foff + j], z = Z[foff + j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double s, c;
+            sincos(kx[r] * x + ky[r] * y + kz[r] * z, &s, &c);
+            ar[r] += c;
+            ai[r] += s;
+        }
+    }
+    double2 *dst = out + ((size_t)fslot * n_splits + split) * (size_t)K;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int d = kdst[(size_t)chunk * KB + r * kBlock + tid];
+        if (d >= 0) dst[d & ~(1 << 30)] = make_double2(ar[r], (d & (1 << 30)) ? -ai[r] : ai[r]);
+    }
+}
+
+// Plain FP64 Fs.
+template <int R>
+__global__ __launch_bounds__(kBlock) void fs_f64_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+    const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
+    const int *__restrict__ T0, const int *__restrict__ T1, const uint32_t *__restrict__ kpack,
+    const int *__restrict__ kdst, int K, int KB, double kscale, int per_split, int n_splits,
+    double *__restrict__ out)
+{
+    const int chunk = blockIdx.x, pslot = blockIdx.y, split = blockIdx.z;
+    const int tid = threadIdx.x;
+    const int t0 = T0[pslot], t1 = T1[pslot];
+    const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
+    const double L0x = box[3 * t0], L0y = box[3 * t0 + 1], L0z = box[3 * t0 + 2];
+    const double L1x = box[3 * t1], L1y = box[3 * t1 + 1], L1z = box[3 * t1 + 2];
+    const int a_begin = split * per_split;
+    const int a_end = min(n_list, a_begin + per_split);
+    double kx[R], ky[R], kz[R], as[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int nx, ny, nz;
+        unpack_k(kpack[(size_t)chunk * KB + r * kBlock + tid], nx, ny, nz);
+        kx[r] = kscale * nx; ky[r] = kscale * ny; kz[r] = kscale * nz;
+        as[r] = 0.0;
+    }
+    for (int a = a_begin; a < a_end; ++a) {
+        const int j = list ? list[a] : a;
+        const double dx = ((double)X[o1 + j] - (double)X[o0 + j]) + ((double)IX[o1 + j] * L1x - (double)IX[o0 + j] * L0x);
+        const double dy = ((double)Y[o1 + j] - (double)Y[o0 + j]) + ((double)IY[o1 + j] * L1y - (double)IY[o0 + j] * L0y);
+        const double dz = ((double)Z[o1 + j] - (double)Z[o0 + j]) + ((double)IZ[o1 + j] * L1z - (double)IZ[o0 + j] * L0z);
+#pragma unroll
+        for (int r = 0; r < R; ++r) as[r] += cos(kx[r] * dx + ky[r] * dy + kz[r] * dz);
+    }
+    double *dst = out + ((size_t)pslot * n_splits + split) * (size_t)K;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int d = kdst[(size_t)chunk * KB + r * kBlock + tid];
+        if (d >= 0) dst[d & ~(1 << 30)] = as[r];
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void split_reduce2_kernel(const double2 *__restrict__ in, int n_rows,
+                                                                int n_splits, int K, double2 *__restrict__ out)
+{
+    const size_t n = (size_t)n_rows * K;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock) {
+        const size_t f = e / K, k = e - f * K;
+        double re = 0.0, im = 0.0;
+        for (int s = 0; s < n_splits; ++s) {
+            const double2 v = in[(f * n_splits + s) * K + k];
+            re += v.x;
+            im += v.y;
+        }
+        out[e] = make_double2(re, im);
+    }
+}
+
+// grid = (shells, groups).
+__global__ __launch_bounds__(kBlock) void corr_shell_kernel(
+    const double2 *__restrict__ rho1, const double2 *__restrict__ rho2, int K,
+    const int *__restrict__ group_off, const int *__restrict__ row0, const int *__restrict__ row1,
+    const int *__restrict__ shell_off, int n_shells, const double *__restrict__ norm,
+    double *__restrict__ out)
+{
+    __shared__ double red[kBlock];
+    const int s = blockIdx.x, g = blockIdx.y;
+    const int v0 = shell_off[s], H = shell_off[s + 1] - v0;
+    const int p0 = group_off[g], np = group_off[g + 1] - p0;
+    const long items = (long)np * H;
+    double acc = 0.0;
+    for (long it = threadIdx.x; it < items; it += kBlock) {
+        const int p = p0 + (int)(it / H);
+        const int v = v0 + (int)(it % H);
+        const double2 a = rho1[(size_t)row0[p] * K + v];
+        const double2 b = rho2[(size_t)row1[p] * K + v];
+        acc += a.x * b.x + a.y * b.y;
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+        const double nrm = norm[g];
+        out[(size_t)g * n_shells + s] = (H > 0 && nrm != 0.0) ? acc / (double)H / nrm : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void fs_shell_kernel(
+    const double *__restrict__ V, int n_splits, int K, const int *__restrict__ group_off,
+    const int *__restrict__ order, const int *__restrict__ shell_off, int n_shells,
+    const double *__restrict__ norm, double *__restrict__ out)
+{
+    __shared__ double red[kBlock];
+    const int s = blockIdx.x, g = blockIdx.y;
+    const int v0 = shell_off[s], H = shell_off[s + 1] - v0;
+    const int p0 = group_off[g], np = group_off[g + 1] - p0;
+    const long per_pair = (long)n_splits * H;
+    const long items = (long)np * per_pair;
+    double acc = 0.0;
+    for (long it = threadIdx.x; it < items; it += kBlock) {
+        const int pl = (int)(it / per_pair);
+        const long rem = it - (long)pl * per_pair;
+        const int sp = (int)(rem / H);
+        const int v = v0 + (int)(rem - (long)sp * H);
+        acc += V[((size_t)order[p0 + pl] * n_splits + sp) * K + v];
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+        const double nrm = norm[g];
+        out[(size_t)g * n_shells + s] = (H > 0 && nrm != 0.0) ? acc / (double)H / nrm : 0.0;
+    }
+}
+
+// grid = (atom blocks, pairs).
+__global__ __launch_bounds__(kBlock) void msd_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+    const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
+    const int *__restrict__ T0, const int *__restrict__ T1, double *__restrict__ partial)
+{
+    __shared__ double red[kBlock];
+    const int p = blockIdx.y;
+    const int t0 = T0[p], t1 = T1[p];
+    const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
+    const double L0x = box[3 * t0], L0y = box[3 * t0 + 1], L0z = box[3 * t0 + 2];
+    const double L1x = box[3 * t1], L1y = box[3 * t1 + 1], L1z = box[3 * t1 + 2];
+    double a2 = 0.0, a4 = 0.0;
+    for (int a = blockIdx.x * kBlock + threadIdx.x; a < n_list; a += gridDim.x * kBlock) {
+        const int j = list ? list[a] : a;
+        const double dx = ((double)X[o1 + j] - (double)X[o0 + j]) + ((double)IX[o1 + j] * L1x - (double)IX[o0 + j] * L0x);
+        const double dy = ((double)Y[o1 + j] - (double)Y[o0 + j]) + ((double)IY[o1 + j] * L1y - (double)IY[o0 + j] * L0y);
+        const double dz = ((double)Z[o1 + j] - (double)Z[o0 + j]) + ((double)IZ[o1 + j] * L1z - (double)IZ[o0 + j] * L0z);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        a2 += r2;
+        a4 += r2 * r2;
+    }
+    a2 = block_sum(a2, red);
+    a4 = block_sum(a4, red);
+    if (threadIdx.x == 0) {
+        partial[((size_t)p * gridDim.x + blockIdx.x) * 2] = a2;
+        partial[((size_t)p * gridDim.x + blockIdx.x) * 2 + 1] = a4;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void msd_reduce_kernel(const double *__restrict__ partial, int n_blocks,
+                                                             const int *__restrict__ group_off,
+                                                             const int *__restrict__ order,
+                                                             double *__restrict__ s2, double *__restrict__ s4)
+{
+    __shared__ double red[kBlock];
+    const int g = blockIdx.x;
+    const int p0 = group_off[g], np = group_off[g + 1] - p0;
+    const long items = (long)np * n_blocks;
+    double a2 = 0.0, a4 = 0.0;
+    for (long it = threadIdx.x; it < items; it += kBlock) {
+        const int pl = (int)(it / n_blocks);
+        const int b = (int)(it - (long)pl * n_blocks);
+        const size_t e = ((size_t)order[p0 + pl] * n_blocks + b) * 2;
+        a2 += partial[e];
+        a4 += partial[e + 1];
+    }
+    a2 = block_sum(a2, red);
+    a4 = block_sum(a4, red);
+    if (threadIdx.x == 0) {
+        s2[g] = a2;
+        s4[g] = a4;
+    }
+}
+
+template <int MM>
+size_t tab_bytes()
+{
+    return (size_t)Geom<MM>::A * Geom<MM>::W * sizeof(float2);
+}
+
+template <int R, int MM>
+hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, const int *frames, int n_frames,
+                            const KPlan &kp, double inv_ell, int n_splits, double2 *out, hipStream_t st)
+{
+    const int per_split = (n_list + n_splits - 1) / n_splits;
+    const size_t lds = tab_bytes<MM>();
+    auto kfn = rho_c32_kernel<R, MM>;
+    hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    dim3 grid(kp.n_chunks, n_frames, n_splits);
+    hipLaunchKernelGGL(kfn, grid, dim3(kBlock), lds, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames,
+                       kp.kpack, kp.kdst, kp.chunkM, kp.K, kp.KB, inv_ell, per_split, n_splits, out);
+    return hipGetLastError();
+}
+
+template <int R, int MM>
+hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
+                           int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
+                           hipStream_t st)
+{
+    const int per_split = (n_list + n_splits - 1) / n_splits;
+    const size_t lds = tab_bytes<MM>();
+    auto kfn = fs_c32_kernel<R, MM>;
+    hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    dim3 grid(kp.n_chunks, n_pairs, n_splits);
+    hipLaunchKernelGGL(kfn, grid, dim3(kBlock), lds, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box,
+                       tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.chunkM, kp.K, kp.KB, inv_ell,
+                       per_split, n_splits, out);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t dispatch_rho_c32(int MM, const TrajDev &tr, const int *list, int n_list, const int *frames,
+                            int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,
+                            hipStream_t st)
+{
+    switch (MM) {
+    case 24: return launch_rho_c32_t<R, 24>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 50: return launch_rho_c32_t<R, 50>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 100: return launch_rho_c32_t<R, 100>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 127: return launch_rho_c32_t<R, 127>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int R>
+hipError_t dispatch_fs_c32(int MM, const TrajDev &tr, const int *list, int n_list, const int *t0,
+                           const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits,
+                           double *out, hipStream_t st)
+{
+    switch (MM) {
+    case 24: return launch_fs_c32_t<R, 24>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 50: return launch_fs_c32_t<R, 50>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 100: return launch_fs_c32_t<R, 100>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 127: return launch_fs_c32_t<R, 127>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+int plan_table_width(int max_component)
+{
+    if (max_component <= 24) return 24;
+    if (max_component <= 50) return 50;
+    if (max_component <= 100) return 100;
+    if (max_component <= 127) return 127;
+    return -1;
+}
+
+hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_list, const int *frames,
+                      int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,
+                      hipStream_t st)
+{
+    if (precision == 1) {
+        const int per_split = (n_list + n_splits - 1) / n_splits;
+        const double kscale = 2.0 * M_PI * inv_ell;
+        dim3 grid(kp.n_chunks, n_frames, n_splits);
+        switch (kp.R) {
+        case 4: hipLaunchKernelGGL(rho_f64_kernel<4>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 8: hipLaunchKernelGGL(rho_f64_kernel<8>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 16: hipLaunchKernelGGL(rho_f64_kernel<16>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    switch (kp.R) {
+    case 4: return dispatch_rho_c32<4>(kp.MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 8: return dispatch_rho_c32<8>(kp.MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 16: return dispatch_rho_c32<16>(kp.MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_list, const int *t0,
+                     const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
+                     hipStream_t st)
+{
+    if (precision == 1) {
+        const int per_split = (n_list + n_splits - 1) / n_splits;
+        const double kscale = 2.0 * M_PI * inv_ell;
+        dim3 grid(kp.n_chunks, n_pairs, n_splits);
+        switch (kp.R) {
+        case 4: hipLaunchKernelGGL(fs_f64_kernel<4>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 8: hipLaunchKernelGGL(fs_f64_kernel<8>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 16: hipLaunchKernelGGL(fs_f64_kernel<16>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    switch (kp.R) {
+    case 4: return dispatch_fs_c32<4>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 8: return dispatch_fs_c32<8>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 16: return dispatch_fs_c32<16>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_split_reduce2(const double2 *in, int n_rows, int n_splits, int K, double2 *out, hipStream_t st)
+{
+    const size_t n = (size_t)n_rows * K;
+    int blocks = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 65535);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(split_reduce2_kernel, dim3(blocks), dim3(kBlock), 0, st, in, n_rows, n_splits, K, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_corr_shell(const double2 *rho1, const double2 *rho2, int K, const int *group_off,
+                             const int *row0, const int *row1, int n_groups, const int *shell_off,
+                             int n_shells, const double *norm, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(corr_shell_kernel, dim3(n_shells, n_groups), dim3(kBlock), 0, st, rho1, rho2, K,
+                       group_off, row0, row1, shell_off, n_shells, norm, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *group_off, const int *order,
+                           int n_groups, const int *shell_off, int n_shells, const double *norm,
+                           double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(fs_shell_kernel, dim3(n_shells, n_groups), dim3(kBlock), 0, st, V, n_splits, K,
+                       group_off, order, shell_off, n_shells, norm, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
+                      int n_pairs, int n_blocks, double *partial, hipStream_t st)
+{
+    hipLaunchKernelGGL(msd_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
+                       tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_msd_reduce(const double *partial, int n_blocks, const int *group_off, const int *order,
+                             int n_groups, double *s2, double *s4, hipStream_t st)
+{
+    hipLaunchKernelGGL(msd_reduce_kernel, dim3(n_groups), dim3(kBlock), 0, st, partial, n_blocks, group_off,
+                       order, s2, s4);
+    return hipGetLastError();
+}
+
+}  // namespace amdat

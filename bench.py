@@ -1,0 +1,204 @@
+"""Benchmark of the reciprocal-space / time-correlation path on one MI355X.
+
+Workload (precision form): bead-spring melt of 400 chains x 160 beads
+(N = 64,000), cubic box L = 42.2, blocked-exponential time scheme
+`exponential 10 100 1.2 0 0 1` (T = 1,001 frames, 110 time gaps,
+1,046 time-origin pairs with fullblock = 0), Gaussian steps sigma = 0.02.
+One step is the whole path:
+  rho_k(t) for every frame on 100 shells (bins 2,4,...,200 at k = 2 pi n/L,
+           |k| up to 14.9; the isf command's box-L grid),
+  F(k,t) over all pairs and S(k) from rho,
+  Fs(k,t) on 99 shells at max_length_scale 21.05 (bins 2..100, |k| to 14.9),
+  MSD and the non-Gaussian parameter's fourth moment.
+updates = N*K_F*T (rho/S) + N*K_Fs*P (Fs) + K_F*P (F).
+
+Prints one JSON line (see the keys below).  --gpus > 1 is refused: the path
+is measured on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Fs/F(k,t)+S(k) atom·k·time-pair updates/s, 1 GPU, ≤1e-6 rel err vs plain FP64"
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(small):
+    from amdat_amd import _abi, synth
+    if small:
+        cfg = dict(n_chains=40, beads=160, L=19.6, nexp=4, nes=25, base=1.2, sigma=0.02, ell_fs=9.8)
+    else:
+        cfg = dict(n_chains=400, beads=160, L=42.2, nexp=10, nes=100, base=1.2, sigma=0.02, ell_fs=21.05)
+    T = cfg["nes"] * cfg["nexp"] + 1
+    t0 = time.time()
+    tr = synth.chain_melt(cfg["n_chains"], cfg["beads"], cfg["L"], T, cfg["sigma"], seed=synth.MASTER_SEED + 1)
+    log("generated trajectory N=%d T=%d in %.1fs" % (tr.N, tr.T, time.time() - t0))
+    # F/S shells: bins 2,4,...,200 of the box-L grid; Fs shells: bins 2..100.
+    offA, nA = _abi.shells_generate(2, 199, 300, seed=synth.MASTER_SEED + 11)
+    keep = list(range(0, 199, 2))
+    rows = [nA[offA[s]:offA[s + 1]] for s in keep]
+    offF = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    nF = np.concatenate(rows).astype(np.int32)
+    offS, nS = _abi.shells_generate(2, 99, 300, seed=synth.MASTER_SEED + 12)
+    tg, t0_, t1_ = _abi.ts_pairs(cfg["nexp"], cfg["nes"], 0, 0, 0)
+    ntg = _abi.ts_n_timegaps(cfg["nexp"], cfg["nes"], 0)
+    return cfg, tr, (offF, nF), (offS, nS), (tg, t0_, t1_, ntg)
+
+
+def cpu_baseline(tr, shF, shS, L, ell_fs, n_sample, pairs):
+    """Reference-arithmetic FP32 restatement (oracle, one core) on a sample:
+    rho for one frame and Fs for one pair over n_sample atoms, all vectors."""
+    from oracle import oracle as orc
+    orc.set_threads(1)
+    atoms = np.arange(n_sample, dtype=np.int32)
+    t = time.perf_counter()
+    orc.ref32_rho(tr.x[500 % tr.T], tr.y[500 % tr.T], tr.z[500 % tr.T], shF[1], np.float32(L), atoms=atoms)
+    tg, t0, t1, ntg = pairs
+    sel = (np.array([tg[-2]]), np.array([t0[-2]]), np.array([t1[-2]]))
+    orc.ref32_fs(tr, sel, ntg, shS[0], shS[1], np.float32(ell_fs), atoms=atoms)
+    dt = time.perf_counter() - t
+    upd = n_sample * len(shF[1]) + n_sample * len(shS[1])
+    return dict(value=upd / dt, unit="updates/s", cores=1, kind="port",
+                sample="rho_k one frame + Fs one time-origin pair, %d atoms x %d/%d vectors, "
+                       "reference FP32 arithmetic (cosf/sinf per update), %.1f s" % (
+                           n_sample, len(shF[1]), len(shS[1]), dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--small", action="store_true", help="reduced workload for a quick check")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 error check")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the path is built and measured on a single MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from amdat_amd import _abi
+
+    cfg, tr, (offF, nF), (offS, nS), (tg, t0, t1, ntg) = build_workload(args.small)
+    L = np.float32(cfg["L"])
+    ell_fs = np.float32(cfg["ell_fs"])
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    ctx = _abi.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    d = _abi.Trajectory(ctx, *tr.arrays)
+    shF = _abi.Shells(ctx, offF, nF)
+    shS = _abi.Shells(ctx, offS, nS)
+    P = _abi.Pairs(ctx, tg, t0, t1, ntg)
+    nshF, nshS = len(offF) - 1, len(offS) - 1
+    KF, KS, NP, N, T = len(nF), len(nS), len(tg), tr.N, tr.T
+    U_rho = N * KF * T
+    U_fs = N * KS * NP
+    U_f = KF * NP
+    updates = U_rho + U_fs + U_f
+    log("workload: N=%d T=%d K_F=%d (%d shells) K_Fs=%d (%d shells) pairs=%d updates/step=%.3e"
+        % (N, T, KF, nshF, KS, nshS, NP, updates))
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    def step(prec, timings=None):
+        e0, e1, e2, e3 = ev(), ev(), ev(), ev()
+        e0.record(stream)
+        rho = _abi.Rho(ctx, d, None, shF, 0, nshF - 1, L, precision=prec)
+        e1.record(stream)
+        F, cnt = _abi.fkt(ctx, rho, rho, P)
+        S = _abi.sk(ctx, rho)
+        e2.record(stream)
+        Fs, _ = _abi.fs(ctx, d, None, shS, 0, nshS - 1, ell_fs, P, precision=prec)
+        e3.record(stream)
+        s2, s4, c = _abi.msd_ngp(ctx, d, None, P)
+        rho.close()
+        if timings is not None:
+            torch.cuda.synchronize()
+            timings["rho"].append(e0.elapsed_time(e1))
+            timings["fs"].append(e2.elapsed_time(e3))
+        return dict(S=S, F=F, Fs=Fs, msd=s2 / np.maximum(c, 1), s4=s4, count=c)
+
+    for i in range(args.warmup):
+        step(_abi.PREC_FP32C)
+        log("warmup %d done" % i)
+    timings = {"rho": [], "fs": []}
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        out = step(_abi.PREC_FP32C, timings)
+        log("step %d: rho %.1f ms, Fs %.1f ms" % (i, timings["rho"][-1], timings["fs"][-1]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_per_step = 1e3 * wall / args.steps
+    value = updates * args.steps / wall
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "precision form: bead-spring melt %d chains x %d beads (N=%d), L=%.1f, "
+                               "exponential %d %d %.2f (T=%d), %d pairs (fullblock=0); rho/F/S on %d shells "
+                               "(%d vectors, box-L grid), Fs on %d shells (%d vectors, l=%.2f); MSD/NGP"
+                               % (cfg["n_chains"], cfg["beads"], N, cfg["L"], cfg["nexp"], cfg["nes"], cfg["base"],
+                                  T, NP, nshF, KF, nshS, KS, cfg["ell_fs"]),
+                   "n_atoms": N, "n_frames": T, "n_pairs": NP, "k_vectors_F": KF, "k_vectors_Fs": KS},
+    }
+
+    # Roofline of the dominant kernel (FP32 VALU bound): algorithmic flops per
+    # launch / measured launch time.  rho: 8 flops per (atom, vector, frame);
+    # Fs: 4 flops per (atom, vector, pair).
+    rho_ms = float(np.mean(timings["rho"]))
+    fs_ms = float(np.mean(timings["fs"]))
+    rho_tf = 8.0 * U_rho / (rho_ms * 1e-3) / 1e12
+    fs_tf = 4.0 * U_fs / (fs_ms * 1e-3) / 1e12
+    dom = "rho" if rho_ms >= fs_ms else "fs"
+    ach = rho_tf if dom == "rho" else fs_tf
+    result["roofline"] = {"bound": "valu", "kernel": "rho_c32_kernel" if dom == "rho" else "fs_c32_kernel",
+                          "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                          "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                          "launch_ms": {"rho": rho_ms, "fs": fs_ms},
+                          "tflops": {"rho": rho_tf, "fs": fs_tf}}
+
+    if not args.no_fp64:
+        t = time.perf_counter()
+        ref = step(_abi.PREC_FP64)
+        torch.cuda.synchronize()
+        fp64_ms = 1e3 * (time.perf_counter() - t)
+        S64 = ref["S"]
+        err = {
+            "S": float(np.max(np.abs(out["S"] - S64) / np.abs(S64))),
+            "F": float(np.max(np.abs(out["F"] - ref["F"]) / np.maximum(np.abs(ref["F"]), np.abs(S64)[None]))),
+            "Fs": float(np.max(np.abs(out["Fs"] - ref["Fs"]) / np.maximum(np.abs(ref["Fs"]), 1.0))),
+            "MSD": float(np.max(np.abs(out["msd"][1:] - ref["msd"][1:]) / np.abs(ref["msd"][1:]))),
+        }
+        result["rel_err_vs_fp64"] = err
+        result["fp64_ms_per_step"] = fp64_ms
+        result["speedup_vs_fp64"] = fp64_ms / ms_per_step
+        log("plain FP64 step %.1f ms; errors %s" % (fp64_ms, err))
+
+    if not args.no_cpu:
+        n_sample = 600 if args.small else 3000
+        result["cpu_baseline"] = cpu_baseline(tr, (offF, nF), (offS, nS), cfg["L"], cfg["ell_fs"], n_sample,
+                                              (tg, t0, t1, ntg))
+        log("cpu baseline %.3e updates/s" % result["cpu_baseline"]["value"])
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,216 @@
+/*
+ * amdat_hip.h -- C ABI of the MI355X (gfx950) implementation of AMDAT's
+ * reciprocal-space and time-correlation path.
+ *
+ * Plain C: no exceptions cross this boundary, every call returns an int
+ * status (AMDAT_OK or a negative code) and amdat_last_error() gives the
+ * message of the calling thread's last failure.  Host buffers belong to
+ * the caller.  Device objects (trajectory, shell table, pair table,
+ * density) are opaque handles owned by the context that made them.  All
+ * calls are synchronous unless a stream has been set with
+ * amdat_ctx_set_stream(); then results written to host memory are valid
+ * after amdat_ctx_synchronize().
+ *
+ * The reference (dssimmons-codes/AMDAT) has no FFI: its operator API is the
+ * C++ Analysis class hierarchy.  Each entry point below names the
+ * reference function or loop it stands in for; the C++ Analysis-class
+ * layer in amdat_amd/host calls these in the places where the reference's
+ * classes run their loops.
+ *
+ * Data layout of a trajectory: frame-major structure of arrays,
+ * x[t*n_atoms + j], FP32 wrapped coordinates as the trajectory file gives
+ * them, int16 image counts in the same layout, box[3*t + c] the box edge
+ * length of frame t along axis c (FP32).
+ *
+ * k-vectors: integer triples n[3*v + c]; the wave vector is
+ * k = 2*pi*n/length_scale.  Shell s holds vectors
+ * [shell_off[s], shell_off[s+1]) (AMDAT's wavenumber index wn = s, read
+ * from file qvector.(wn+2); src/wave_vectors.cpp:430-484).
+ */
+#ifndef AMDAT_HIP_H
+#define AMDAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define AMDAT_OK 0
+#define AMDAT_EINVAL (-1)  /* bad argument: null pointer, size or index out of range */
+#define AMDAT_ENOMEM (-2)  /* host or device allocation failed */
+#define AMDAT_EDEVICE (-3) /* HIP runtime or kernel launch error */
+#define AMDAT_ERANGE (-4)  /* shell / time-gap / pair index outside the object's range */
+#define AMDAT_EIO (-5)     /* file could not be opened or parsed */
+
+/* Precision of the phase kernels. */
+#define AMDAT_PREC_FP32C 0 /* FP32 phase tables + compensated FP32 sums, FP64 reductions */
+#define AMDAT_PREC_FP64 1  /* plain FP64: FP64 phase, FP64 sin/cos, FP64 sums */
+
+typedef struct amdat_ctx amdat_ctx;
+typedef struct amdat_traj amdat_traj;
+typedef struct amdat_shells amdat_shells;
+typedef struct amdat_pairs amdat_pairs;
+typedef struct amdat_list amdat_list;
+typedef struct amdat_rho amdat_rho;
+
+/* ---- errors, version ---------------------------------------------- */
+
+/* Message of this thread's last failed call ("" if none).  Stands in for
+ * the reference's Error(msg, code) / cout-and-exit convention
+ * (src/error.cpp:51-146): the C++ layer turns these codes back into that
+ * convention. */
+const char *amdat_last_error(void);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int amdat_version(void);
+
+/* ---- context ------------------------------------------------------ */
+
+/* Select a device and create a context (device memory pools, streams).
+ * No reference counterpart: the reference is a single CPU process. */
+int amdat_ctx_create(int device, amdat_ctx **out);
+int amdat_ctx_destroy(amdat_ctx *ctx);
+/* Run subsequent work on this HIP stream (hipStream_t passed as void*);
+ * NULL restores the context's own stream. */
+int amdat_ctx_set_stream(amdat_ctx *ctx, void *stream);
+int amdat_ctx_synchronize(amdat_ctx *ctx);
+
+/* ---- time scheme (host integer work; bit-exact with the reference) -- */
+
+/* n_timesteps = nes*nexp + 1  (src/system.cpp:120). */
+int amdat_ts_n_timesteps(int n_exponentials, int n_exponential_steps);
+/* n_timegaps = nes + nexp - 1 + !frt  (src/system.cpp:218). */
+int amdat_ts_n_timegaps(int n_exponentials, int n_exponential_steps, int frt);
+/* Frame times, System::create_timelist (src/system.cpp:3542-3568).
+ * out[n_timesteps]. */
+int amdat_ts_timelist(int n_exponentials, int n_exponential_steps, float exp_base,
+                      int first_exponent, float time_unit, float *out, int cap);
+/* Lag time per time gap, System::displacement_times()
+ * (src/system.cpp:3763-3784).  out[n_timegaps]. */
+int amdat_ts_displacement_times(int n_exponentials, int n_exponential_steps, int frt,
+                                const float *timelist, float *out, int cap);
+/* Ordered (time gap, t0, t1) pairs of System::displacement_list
+ * (src/system.cpp:4090-4179; limit as set by the `limit` command,
+ * src/control.cpp:1860-1865, 0 = none).  Writes at most cap pairs;
+ * *n_pairs receives the full count. */
+int amdat_ts_pairs(int n_exponentials, int n_exponential_steps, int frt, int fullblock,
+                   int limit, int *tg, int *t0, int *t1, int cap, int *n_pairs);
+/* System::timegap_weighting(fullblock) (src/system.cpp:4599-4628),
+ * including its +1 on cross-block gaps.  out[n_timegaps]. */
+int amdat_ts_weighting(int n_exponentials, int n_exponential_steps, int frt, int fullblock,
+                       int limit, int *out, int cap);
+
+/* ---- shells -------------------------------------------------------- */
+
+/* Read AMDAT q-vector files stem.NNN (NNN = bin, 3 digits; integer
+ * triples in any float text format) for bins first_bin ..
+ * first_bin+n_bins-1, as Wave_Vectors::read_vectors does
+ * (src/wave_vectors.cpp:430-484; the reference reads bins 2..100).
+ * shell_off[n_bins+1] and nvec[3*cap_vectors] are written; *n_vectors gets
+ * the total. */
+int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_off, int *nvec,
+                      int cap_vectors, int *n_vectors);
+/* Build shells by the shell rule of src/qvectors/generate_3d_qvectors.m
+ * (half-space integer vectors, bin = floor((|n|+0.25)*2), |n_c| <= 100)
+ * and keep at most max_per_shell vectors per shell, chosen by a seeded
+ * shuffle.  Same output convention as amdat_shells_read. */
+int amdat_shells_generate(int first_bin, int n_bins, int max_per_shell, uint64_t seed,
+                          int *shell_off, int *nvec, int cap_vectors, int *n_vectors);
+/* Upload a shell table to the device. */
+int amdat_shells_upload(amdat_ctx *ctx, int n_shells, const int *shell_off, const int *nvec,
+                        amdat_shells **out);
+int amdat_shells_free(amdat_ctx *ctx, amdat_shells *sh);
+
+/* ---- trajectory and atom lists ------------------------------------- */
+
+/* Upload a trajectory (host arrays, layout above).  Stands in for the
+ * reference's per-atom Coordinate arrays (src/trajectory.h:37-38) and the
+ * image-count unwrap of the custom reader (src/system.cpp:1290-1297):
+ * unwrapped positions are rebuilt on the device as x + ix*L. */
+int amdat_traj_upload(amdat_ctx *ctx, int n_atoms, int n_frames, const float *x, const float *y,
+                      const float *z, const int16_t *ix, const int16_t *iy, const int16_t *iz,
+                      const float *box, amdat_traj **out);
+/* Same, from device pointers the caller keeps alive (no copy). */
+int amdat_traj_wrap_device(amdat_ctx *ctx, int n_atoms, int n_frames, const float *d_x,
+                           const float *d_y, const float *d_z, const int16_t *d_ix,
+                           const int16_t *d_iy, const int16_t *d_iz, const float *box_host,
+                           amdat_traj **out);
+int amdat_traj_free(amdat_ctx *ctx, amdat_traj *tr);
+
+/* Static atom selection (Static_Trajectory_List, src/static_trajectory_
+ * list.cpp:59-101): atom indices in list order.  indices == NULL with
+ * n == n_atoms of the trajectory means "all atoms in order". */
+int amdat_list_create(amdat_ctx *ctx, const amdat_traj *tr, int n, const int *indices,
+                      amdat_list **out);
+int amdat_list_free(amdat_ctx *ctx, amdat_list *l);
+
+/* ---- pairs --------------------------------------------------------- */
+
+/* Upload (tg, t0, t1) pairs as produced by amdat_ts_pairs. */
+int amdat_pairs_upload(amdat_ctx *ctx, int n_pairs, const int *tg, const int *t0, const int *t1,
+                       int n_timegaps, amdat_pairs **out);
+int amdat_pairs_free(amdat_ctx *ctx, amdat_pairs *p);
+
+/* ---- the path ------------------------------------------------------ */
+
+/* rho_k(t) = sum_j exp(i k.r_j(t)) over wrapped coordinates for every
+ * frame in frames[n_frames] (NULL = all frames) and every vector of
+ * shells first_shell..last_shell, kept on the device.
+ * Wave_Density::analyze + listkernel (src/wave_density.cpp:313-365);
+ * k = 2*pi*n/length_scale (the reference's Wave_Vectors spacing,
+ * src/wave_vectors.cpp:150). */
+int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
+                      const amdat_shells *sh, int first_shell, int last_shell,
+                      float length_scale, const int *frames, int n_frames, int precision,
+                      amdat_rho **out);
+/* Copy rho to the host: out[2*(f*K + v)] = Re, +1 = Im, f over the
+ * computed frames, v over the vectors of the computed shells in order. */
+int amdat_rho_download(amdat_ctx *ctx, const amdat_rho *rho, double *out, size_t n_doubles);
+/* Number of frames and vectors held. */
+int amdat_rho_shape(const amdat_rho *rho, int *n_frames, int *n_vectors);
+int amdat_rho_free(amdat_ctx *ctx, amdat_rho *rho);
+
+/* S(k) numerator per shell: sum over the rho's frames of
+ * sum_{v in shell} Re(rho1 conj rho2)/H, FP64, and the atom count
+ * divisor is applied: S[s] = numerator / (n_frames * list size).
+ * Structure_Factor::timekernel/timekernel2 + postprocess_list
+ * (src/structure_factor.cpp:201-285).  rho2 may equal rho1
+ * (symmetric form).  S[n_shells of rho]. */
+int amdat_sk(amdat_ctx *ctx, const amdat_rho *rho1, const amdat_rho *rho2, double *S);
+
+/* F(k,t) from two densities over all frames:
+ * F[tg][s] = sum_pairs sum_{v in s} Re(rho1(t0) conj rho2(t1))/H
+ *            / sum_pairs N(t0).
+ * Intermediate_Scattering_Function(sys,wd1,wd2,wn1,wn2,fullblock)
+ * (src/intermediate_scattering_function.cpp:237-322, 551-566) with the
+ * pair loop of System::displacement_loop (src/system.cpp:3918-3964).
+ * Shells wn1..wn2 are given relative to the densities' first shell.
+ * F[n_timegaps*(wn2-wn1+1)], count[n_timegaps] (sum_pairs N(t0)). */
+int amdat_fkt(amdat_ctx *ctx, const amdat_rho *rho1, const amdat_rho *rho2,
+              const amdat_pairs *pairs, int wn1, int wn2, double *F, double *count);
+
+/* Fs(k,t) = sum_pairs sum_atoms mean_{v in s} cos(k.(r(t1)-r(t0)))
+ *           / (atom-pair count), with r(t1)-r(t0) rebuilt in FP64 from
+ * wrapped coordinates and image counts.
+ * Incoherent_Scattering_Function (src/incoherent_scattering_function.cpp:
+ * 176-299) + Correlation_2D::postprocess_list (src/correlation_2d.cpp:
+ * 150-164).  Fs[n_timegaps*(last-first+1)], count[n_timegaps]. */
+int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
+             const amdat_shells *sh, int first_shell, int last_shell, float length_scale,
+             const amdat_pairs *pairs, int precision, double *Fs, double *count);
+
+/* MSD and the non-Gaussian parameter's fourth moment: per time gap the
+ * FP64 sums of |dr|^2 and |dr|^4 over pairs and atoms and the atom-pair
+ * count.  Mean_Square_Displacement (src/mean_square_displacement.cpp:
+ * 140-180) and Non_Gaussian_Parameter (src/non_gaussian_parameter.cpp:
+ * 96-143).  s2, s4, count [n_timegaps]. */
+int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
+                  const amdat_pairs *pairs, double *s2, double *s4, double *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMDAT_HIP_H */

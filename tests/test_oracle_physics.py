@@ -1,0 +1,220 @@
+"""Pins for the oracle's FP64 physics and its reference-FP32 restatement.
+
+Each check is a closed form, an invariant or an independent brute force,
+chosen so that a dropped term, a wrong sign, a swapped axis or a wrong
+phase scale fails one of them."""
+import numpy as np
+import pytest
+
+from amdat_amd import synth
+from oracle import oracle as orc
+from tests.conftest import ref_shells
+
+
+def kvecs(nvec, ell):
+    return 2.0 * np.pi * np.asarray(nvec, np.float64) / ell
+
+
+def test_rho_one_atom_phase_sign(qvec):
+    _, n = ref_shells(qvec, 2, 12)
+    r = np.array([1.25, -0.5, 3.0], np.float32)
+    rho = orc.rho_f64(r[:1], r[1:2], r[2:], n, 7.0)
+    ph = kvecs(n, 7.0) @ r.astype(np.float64)
+    assert np.allclose(np.abs(rho), 1.0, atol=1e-13)
+    # e^{+i k.r}: Im has the sign of sin(k.r)
+    assert np.allclose(rho, np.exp(1j * ph), atol=1e-12)
+
+
+def test_rho_brute_force_matrix_form(qvec):
+    _, n = ref_shells(qvec, 2, 40)
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(0, 6.0, size=(37, 3)).astype(np.float32)
+    rho = orc.rho_f64(pos[:, 0], pos[:, 1], pos[:, 2], n, 6.0)
+    brute = np.exp(1j * (pos.astype(np.float64) @ kvecs(n, 6.0).T)).sum(0)
+    assert np.allclose(rho, brute, atol=1e-10)
+
+
+def test_rho_axis_permutation(qvec):
+    _, n = ref_shells(qvec, 2, 30)
+    rng = np.random.default_rng(2)
+    pos = rng.uniform(0, 5.0, size=(20, 3)).astype(np.float32)
+    a = orc.rho_f64(pos[:, 0], pos[:, 1], pos[:, 2], n, 5.0)
+    b = orc.rho_f64(pos[:, 1], pos[:, 2], pos[:, 0], n[:, [1, 2, 0]], 5.0)
+    assert np.allclose(a, b, atol=1e-11)
+
+
+def test_rho_translation(qvec):
+    _, n = ref_shells(qvec, 2, 30)
+    rng = np.random.default_rng(3)
+    L = 8.0
+    # dyadic positions and shifts so both configurations are exact in FP32
+    pos = np.round(rng.uniform(0, 4.0, size=(25, 3)) * 64) / 64
+    d = np.array([0.75, 1.5, -0.25])
+    a = orc.rho_f64(*pos.T.astype(np.float32), n, L)
+    b = orc.rho_f64(*(pos + d).T.astype(np.float32), n, L)
+    assert np.allclose(b, a * np.exp(1j * kvecs(n, L) @ d), atol=1e-10)
+    # a whole box length is invisible when ell = L
+    c = orc.rho_f64(*(pos + np.array([L, 0, -L])).T.astype(np.float32), n, L)
+    assert np.allclose(c, a, atol=1e-10)
+
+
+def test_rho_fcc_bragg_peaks():
+    cells, L = 4, 8.0
+    t = synth.fcc_lattice(cells, L)
+    N = t.N
+    assert N == 4 * cells ** 3
+    nv = np.array([[8, 0, 0], [4, 4, 0], [4, 4, 4], [4, 0, 0], [2, 0, 0], [1, 0, 0], [8, 4, 4], [12, 4, 0]])
+    rho = orc.rho_f64(t.x[0], t.y[0], t.z[0], nv, L)
+    # k = 2 pi n / L = (2 pi / a) (n / cells): Bragg iff n/cells integer, all even or all odd
+    h = nv / cells
+    integral = np.all(h == np.round(h), axis=1)
+    par = np.round(h).astype(int) % 2
+    bragg = integral & (np.all(par == 0, axis=1) | np.all(par == 1, axis=1))
+    assert np.allclose(np.abs(rho[bragg]), N, rtol=1e-9)
+    assert np.allclose(np.abs(rho[~bragg]), 0.0, atol=1e-9)
+
+
+def test_sk_forms(qvec):
+    off, n = ref_shells(qvec, 2, 25)
+    rng = np.random.default_rng(4)
+    L = 6.0
+    pos = rng.uniform(0, L, size=(3, 40, 3)).astype(np.float32)
+    A, B = np.arange(0, 25), np.arange(25, 40)
+    rA = np.stack([orc.rho_f64(*pos[f].T, n, L, atoms=A) for f in range(3)])
+    rB = np.stack([orc.rho_f64(*pos[f].T, n, L, atoms=B) for f in range(3)])
+    rAll = np.stack([orc.rho_f64(*pos[f].T, n, L) for f in range(3)])
+    assert np.allclose(rA + rB, rAll, atol=1e-10)
+    N = np.array([40, 40, 40])
+    s_all = orc.sk_f64(rAll, rAll, N, off)
+    s_aa = orc.sk_f64(rA, rA, N, off)
+    s_bb = orc.sk_f64(rB, rB, N, off)
+    s_ab = orc.sk_f64(rA, rB, N, off)
+    assert np.allclose(s_all, s_aa + s_bb + 2 * s_ab, rtol=1e-10)
+    # a single atom: S = 1 for every shell
+    one = np.stack([orc.rho_f64(*pos[f, :1].T, n, L) for f in range(3)])
+    assert np.allclose(orc.sk_f64(one, one, np.ones(3, np.int64), off), 1.0)
+
+
+def test_sk_ideal_gas_mean_one(qvec):
+    off, n = ref_shells(qvec, 10, 60)
+    rng = np.random.default_rng(5)
+    L, N, F = 10.0, 300, 6
+    pos = rng.uniform(0, L, size=(F, N, 3)).astype(np.float32)
+    r = np.stack([orc.rho_f64(*pos[f].T, n, L) for f in range(F)])
+    S = orc.sk_f64(r, r, np.full(F, N), off)
+    # each shell averages ~300*6 exponential(1) variables
+    assert abs(S.mean() - 1.0) < 0.02
+    assert np.all(np.abs(S - 1.0) < 0.35)
+
+
+def _translating(N, L, T, d, seed=6):
+    rng = np.random.default_rng(seed)
+    # dyadic start positions and steps keep every stored value exact in FP32
+    u0 = np.round(rng.uniform(0, L, size=(N, 3)) * 1024) / 1024
+    u = u0[None] + np.arange(T)[:, None, None] * np.asarray(d)[None, None, :]
+    (x, y, z), (ix, iy, iz) = synth._store(u, L)
+    return synth.Traj(x, y, z, ix, iy, iz, np.full((T, 3), L, np.float32)), u
+
+
+def test_fs_msd_uniform_translation(qvec):
+    """All atoms move by d per frame: Fs = mean_v cos(k_v.d dt) exactly,
+    MSD = |d|^2 dt^2, alpha2 = 3/5 - 1; the image-count rebuild must see
+    the boundary crossings."""
+    off, n = ref_shells(qvec, 2, 30)
+    L = 5.0
+    d = np.array([0.625, -0.375, 0.25])
+    tr, _ = _translating(50, L, 7, d)
+    assert np.abs(tr.ix).max() > 0
+    pairs = orc.ts_pairs(3, 2)
+    ntg = orc.ts_n_timegaps(3, 2)
+    Fs, cnt = orc.fs_f64(tr, pairs, ntg, off, n, L)
+    lag = np.array([0, 1, 2, 4, 6])
+    k = kvecs(n, L)
+    exp = np.array([[np.cos(k[off[s]:off[s + 1]] @ d * g).mean() for s in range(len(off) - 1)] for g in lag])
+    assert np.allclose(Fs, exp, atol=1e-9)
+    assert np.allclose(Fs[0], 1.0)
+    m = orc.msd_ngp_f64(tr, pairs, ntg)
+    assert np.allclose(m["msd"], (d @ d) * lag.astype(float) ** 2, rtol=1e-9, atol=1e-12)
+    assert m["msd"][0] == 0.0 and np.isnan(m["ngp"][0])
+    assert np.allclose(m["ngp"][1:], -0.4, atol=1e-9)
+    assert np.allclose(cnt, np.bincount(pairs[0], minlength=ntg) * 50)
+
+
+def test_fs_gaussian_walk_expectation(qvec):
+    off, n = ref_shells(qvec, 8, 14)
+    tr = synth.random_walk(2000, 8.0, 5, 0.15, seed=7)
+    pairs = orc.ts_pairs(2, 2)
+    ntg = orc.ts_n_timegaps(2, 2)
+    Fs, _ = orc.fs_f64(tr, pairs, ntg, off, n, 8.0)
+    m = orc.msd_ngp_f64(tr, pairs, ntg)
+    lag = np.array([0, 1, 2, 4])
+    assert np.allclose(m["msd"], 3 * 0.15 ** 2 * lag, rtol=0.05)
+    assert np.all(np.abs(m["ngp"][1:]) < 0.1)
+    kk = 2 * np.pi * (np.arange(8, 15) / 2) / 8.0
+    for g in range(1, ntg):
+        assert np.allclose(Fs[g], np.exp(-kk ** 2 * m["msd"][g] / 6.0), atol=0.03)
+
+
+def test_fs_image_counts_equal_explicit_unwrap(qvec):
+    off, n = ref_shells(qvec, 2, 40)
+    tr = synth.random_walk(64, 3.0, 9, 0.6, seed=8)
+    assert np.abs(tr.ix).max() > 1
+    u = tr.unwrapped64()
+    # same motion stored unwrapped (no images) in a box large enough
+    raw = synth.Traj(*(u[..., c].astype(np.float32) for c in range(3)),
+                     *(np.zeros((tr.T, tr.N), np.int16) for _ in range(3)), tr.box.copy())
+    # FP32 storage of the unwrapped coordinates differs from x + ix*L by rounding only
+    pairs = orc.ts_pairs(2, 4)
+    ntg = orc.ts_n_timegaps(2, 4)
+    a, _ = orc.fs_f64(tr, pairs, ntg, off, n, 3.0)
+    b, _ = orc.fs_f64(raw, pairs, ntg, off, n, 3.0)
+    assert np.allclose(a, b, atol=2e-5)
+    ma = orc.msd_ngp_f64(tr, pairs, ntg)["msd"]
+    mb = orc.msd_ngp_f64(raw, pairs, ntg)["msd"]
+    assert np.allclose(ma, mb, rtol=1e-5)
+
+
+def test_fkt_static_and_translating(qvec):
+    off, n = ref_shells(qvec, 2, 20)
+    L = 5.0
+    d = np.array([0.1875, 0.125, -0.3125])
+    tr, _ = _translating(30, L, 7, d, seed=9)
+    rho = orc.rho_f64_traj(tr, n, L)
+    pairs = orc.ts_pairs(3, 2)
+    ntg = orc.ts_n_timegaps(3, 2)
+    natoms = np.full(tr.T, tr.N)
+    F, cnt = orc.fkt_f64(rho, rho, natoms, pairs, ntg, off)
+    # F(k,0) = S(k) over the time-origin frames of gap 0
+    frames0 = pairs[1][pairs[0] == 0]
+    S0 = orc.sk_f64(rho[frames0], rho[frames0], natoms[frames0], off)
+    assert np.allclose(F[0], S0, rtol=1e-12)
+    # uniform translation: rho(t+g) = rho(t) e^{i k.d g}
+    lag = np.array([0, 1, 2, 4, 6])
+    k = kvecs(n, L)
+    for gi, g in enumerate(lag):
+        sel = pairs[0] == gi
+        exp = np.zeros(len(off) - 1)
+        for t0 in pairs[1][sel]:
+            w = (rho[t0] * np.conj(rho[t0] * np.exp(1j * k @ d * g))).real
+            exp += np.array([w[off[s]:off[s + 1]].mean() for s in range(len(off) - 1)])
+        assert np.allclose(F[gi], exp / (sel.sum() * tr.N), rtol=1e-9, atol=1e-12)
+
+
+def test_ref32_close_to_f64(qvec):
+    off, n = ref_shells(qvec, 2, 60)
+    tr = synth.plain_config(N=200, L=9.41, nexp=2, nes=3)
+    L = float(np.float32(9.41))
+    r32 = orc.ref32_rho(tr.x[1], tr.y[1], tr.z[1], n, np.float32(9.41))
+    r64 = orc.rho_f64(tr.x[1], tr.y[1], tr.z[1], n, L)
+    # FP32 k.r carries |k.r| * 2^-24 rad per term: the reference's own error
+    err = np.max(np.abs(r32 - r64)) / np.sqrt(tr.N)
+    assert 1e-7 < err < 1e-4
+    S32 = orc.ref32_sk(tr, np.arange(tr.T), off, n, np.float32(9.41))
+    rr = orc.rho_f64_traj(tr, n, L)
+    S64 = orc.sk_f64(rr, rr, np.full(tr.T, tr.N), off)
+    assert np.allclose(S32, S64, rtol=1e-3)
+    pairs = orc.ts_pairs(2, 3)
+    ntg = orc.ts_n_timegaps(2, 3)
+    Fs32, _ = orc.ref32_fs(tr, pairs, ntg, off, n, np.float32(9.41))
+    Fs64, _ = orc.fs_f64(tr, pairs, ntg, off, n, L)
+    assert np.allclose(Fs32, Fs64, atol=1e-4)
