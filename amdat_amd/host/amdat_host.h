@@ -1,0 +1,319 @@
+// amdat_host.h -- C++ drop-in for AMDAT's reciprocal-space and
+// time-correlation analyses, running on the C ABI of libamdat_hip.
+//
+// Class names, constructor signatures, analyze()/write() and the show_*
+// accessors follow the reference's Analysis hierarchy so that code written
+// against AMDAT's classes, and AMDAT input scripts using the msd, ngp, isfs,
+// isf and structure_factor commands, run unchanged:
+//   System                          src/system.h
+//   Trajectory_List                 src/trajectory_list.h (static lists)
+//   Wave_Vectors                    src/wave_vectors.h
+//   Wave_Density                    src/wave_density.h
+//   Structure_Factor                src/structure_factor.h
+//   Intermediate_Scattering_Function src/intermediate_scattering_function.h
+//   Incoherent_Scattering_Function  src/incoherent_scattering_function.h
+//   Mean_Square_Displacement        src/mean_square_displacement.h
+//   Non_Gaussian_Parameter          src/non_gaussian_parameter.h
+//   Control                         src/control.h (the commands above)
+// The arithmetic runs on the device; results are FP64 there and are
+// stored as float here, as the reference stores them, before writing.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/amdat_hip.h"
+
+namespace amdat_hip {
+
+// Version string written into output headers ("... created by AMDAT v.<X>").
+extern const char *kSemver;
+
+// Error severities of the reference's Error(msg, code) (src/error.cpp:
+// 51-146): code < 1 is severe (the program exits with that code), code <
+// 10000 is moderate (the current command is skipped), otherwise minor (a
+// warning).  Severe and moderate errors are thrown as Error.
+class Error : public std::runtime_error {
+  public:
+    Error(const std::string &msg, int code) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+    bool severe() const { return code_ < 1; }
+
+  private:
+    int code_;
+};
+void warn(const std::string &msg);
+
+// Process-wide device context (one per host thread in the C ABI's sense).
+amdat_ctx *device_context();
+
+class Coordinate {
+  public:
+    Coordinate(float x = 0, float y = 0, float z = 0) : x_(x), y_(y), z_(z) {}
+    float show_x() const { return x_; }
+    float show_y() const { return y_; }
+    float show_z() const { return z_; }
+    float min() const { return x_ < y_ ? (x_ < z_ ? x_ : z_) : (y_ < z_ ? y_ : z_); }
+    float length() const;
+
+  private:
+    float x_, y_, z_;
+};
+
+class System;
+class Trajectory_List;
+
+// Static selection of atoms (Static_Trajectory_List semantics: the same
+// atoms at every frame, in the reference's species -> molecule -> type ->
+// index order).
+class Trajectory_List {
+  public:
+    Trajectory_List(System *sys, std::vector<int> atoms);
+    ~Trajectory_List();
+    Trajectory_List(const Trajectory_List &) = delete;
+    Trajectory_List &operator=(const Trajectory_List &) = delete;
+    int show_n_trajectories(int timeii) const { (void)timeii; return (int)atoms_.size(); }
+    const std::vector<int> &atoms() const { return atoms_; }
+    amdat_list *handle() const { return list_; }
+
+  private:
+    System *system_;
+    std::vector<int> atoms_;
+    amdat_list *list_ = nullptr;
+};
+
+class System {
+  public:
+    // Parse the system block (src/system.cpp:49-226): system type,
+    // trajectory type (custom), file, time scheme, species/type table.
+    explicit System(const std::vector<std::string> &block);
+    // Build from arrays already in memory (frame-major SoA, file order).
+    System(int n_exponentials, int n_exponential_steps, float exp_base, int frt, int first_exponent,
+           float time_unit, int n_atoms, const float *x, const float *y, const float *z, const int16_t *ix,
+           const int16_t *iy, const int16_t *iz, const float *box, const int *types);
+    ~System();
+    System(const System &) = delete;
+    System &operator=(const System &) = delete;
+
+    int show_n_timesteps() const { return n_timesteps_; }
+    int show_n_timegaps() const { return n_timegaps_; }
+    int show_n_exponentials() const { return n_exponentials_; }
+    int show_n_exponential_steps() const { return n_exponential_steps_; }
+    int show_frt() const { return frt_; }
+    int show_n_atoms() const { return n_atoms_; }
+    float show_exp_base() const { return exp_base_; }
+    const float *show_timelist() const { return timelist_.data(); }
+    Coordinate min_box_dimensions() const;
+    Coordinate size(int timeii) const;
+    void set_limit(int limit) { displacement_limit_ = limit; }
+    int show_limit() const { return displacement_limit_; }
+
+    // src/system.cpp:4599-4628
+    std::vector<int> timegap_weighting(bool fullblock) const;
+    // src/system.cpp:3763-3784
+    std::vector<float> displacement_times() const;
+    // (tg, t0, t1) in the order of src/system.cpp:4090-4179, as a device table.
+    amdat_pairs *displacement_pairs(bool fullblock, std::vector<int> *tg = nullptr,
+                                    std::vector<int> *t0 = nullptr) const;
+
+    // Atom selections (create_list keywords all / type_system / species /
+    // type_species); indices in the reference's list order.
+    std::vector<int> select(const std::vector<std::string> &args) const;
+
+    amdat_traj *traj() const { return traj_; }
+
+  private:
+    void init_timescheme(const std::vector<std::string> &args);
+    void read_custom(const std::string &file);
+    void finish();
+
+    int n_exponentials_ = 1, n_exponential_steps_ = 0, frt_ = 0, first_exponent_ = 0;
+    float exp_base_ = 1.0f, time_unit_ = 1.0f;
+    int n_timesteps_ = 1, n_timegaps_ = 1, n_atoms_ = 0;
+    int displacement_limit_ = 0;
+    std::vector<float> timelist_;
+    // host copies in file order
+    std::vector<float> x_, y_, z_, box_;
+    std::vector<int16_t> ix_, iy_, iz_;
+    std::vector<int> type_;                     // atom type per atom
+    std::vector<std::string> species_names_;
+    std::vector<int> species_mols_;             // molecules per species
+    std::vector<int> type_ids_;                 // types line
+    std::vector<std::vector<int>> species_type_counts_;
+    std::vector<int> list_order_;               // reference list order over all atoms
+    std::vector<int> atom_species_;             // species index per atom (file order)
+    amdat_traj *traj_ = nullptr;
+};
+
+class Wave_Vectors {
+  public:
+    // src/wave_vectors.cpp:77-181: plane "xyz"; max_length_scale <= 0 means
+    // the smallest box dimension.  Reads n_wavenumbers shells (99 in the
+    // reference: files stem.002 .. stem.100) from the q-vector files named
+    // by the AMDAT_WV3D environment variable (the reference's WV3D path);
+    // n_wavenumbers > 99 opts into the files up to stem.300.
+    Wave_Vectors(System *sys, const std::string &plane, float max_length_scale = 0, int n_wavenumbers = 99);
+    ~Wave_Vectors();
+    Wave_Vectors(const Wave_Vectors &) = delete;
+    Wave_Vectors &operator=(const Wave_Vectors &) = delete;
+
+    int show_n_wavenumbers() const { return (int)off_.size() - 1; }
+    int vectorcount(int index) const { return off_[index + 1] - off_[index]; }
+    std::vector<Coordinate> vectorlist(int index) const;
+    float show_delta_wavenumber() const { return delta_wavenumber_; }
+    float show_approx_wavenumber(int index) const { return approx_[index]; }
+    float show_mean_wavenumber(int index) const;
+    float show_stdev_wavenumber(int index) const;
+    Coordinate show_mean_wavevector(int index) const;
+    float show_length_scale() const { return length_scale_; }
+    amdat_shells *handle() const { return shells_; }
+    const std::vector<int> &offsets() const { return off_; }
+    const std::vector<int> &integer_vectors() const { return nvec_; }
+
+  private:
+    System *system_;
+    float length_scale_, wavegrid_spacing_, delta_wavenumber_;
+    std::vector<float> approx_;
+    std::vector<int> off_, nvec_;
+    amdat_shells *shells_ = nullptr;
+};
+
+class Wave_Density {
+  public:
+    Wave_Density(System *sys, const Wave_Vectors *wv, int inner, int outer);
+    ~Wave_Density();
+    Wave_Density(const Wave_Density &) = delete;
+    Wave_Density &operator=(const Wave_Density &) = delete;
+    // src/wave_density.cpp:313-336: rho_k(t) for every frame.
+    void analyze(Trajectory_List *t_list);
+    // Deep copy of the computed density (the reference's wd2 = wd1).
+    void copy_from(const Wave_Density &other);
+    double show_density_re(int time, int wavenumberindex, int vectorindex) const;
+    double show_density_im(int time, int wavenumberindex, int vectorindex) const;
+    int show_n_atoms(int timeii) const { (void)timeii; return n_atoms_; }
+    int show_first() const { return first_; }
+    int show_last() const { return last_; }
+    const Wave_Vectors *show_wavevectors() const { return wv_; }
+    System *show_system() const { return system_; }
+    // Device density (this object's, or the one it was copied from).
+    const amdat_rho *device() const;
+
+  private:
+    void fetch() const;
+    System *system_;
+    const Wave_Vectors *wv_;
+    int first_, last_, n_atoms_ = 0;
+    amdat_rho *rho_ = nullptr;
+    const Wave_Density *shared_ = nullptr;
+    mutable std::vector<double> host_;
+};
+
+class Structure_Factor {
+  public:
+    Structure_Factor(System *sys, const Wave_Vectors *wv, int timescheme = -1);
+    void analyze(Trajectory_List *t_list);
+    void analyze(Trajectory_List *t_list, Trajectory_List *t_list2);
+    void write(const std::string &filename) const;
+    float show(int wavenumberii) const { return structure_factor_[wavenumberii]; }
+    // Frames the time scheme selects (src/analysis_onetime.cpp:22-52).
+    std::vector<int> frames() const;
+
+  private:
+    System *system_;
+    const Wave_Vectors *wv_;
+    int time_scheme_;
+    std::vector<float> structure_factor_;
+};
+
+class Intermediate_Scattering_Function {
+  public:
+    // src/intermediate_scattering_function.cpp:237-322
+    Intermediate_Scattering_Function(System *sys, Wave_Density *wd1, Wave_Density *wd2, int wn1, int wn2,
+                                     bool parallel);
+    void write(const std::string &filename) const;
+    float show(int timeii, int binii) const { return correlation_[timeii][binii]; }
+
+  private:
+    System *system_;
+    const Wave_Vectors *wv_;
+    int first_bin_, last_bin_;
+    std::vector<float> timetable_;
+    std::vector<std::vector<float>> correlation_;
+};
+
+class Incoherent_Scattering_Function {
+  public:
+    // src/incoherent_scattering_function.cpp:176-212
+    Incoherent_Scattering_Function(System *sys, const Wave_Vectors *wv, int inner, int outer, bool fblock);
+    void analyze(Trajectory_List *t_list);
+    void write(const std::string &filename) const;
+    float show(int timeii, int binii) const { return correlation_[timeii][binii]; }
+
+  private:
+    System *system_;
+    const Wave_Vectors *wv_;
+    int first_bin_, last_bin_;
+    bool fullblock_;
+    std::vector<float> timetable_;
+    std::vector<std::vector<float>> correlation_;
+};
+
+class Mean_Square_Displacement {
+  public:
+    explicit Mean_Square_Displacement(System *sys);
+    void analyze(Trajectory_List *t_list);
+    void write(const std::string &filename) const;
+    float show(int t) const { return msd_[t]; }
+    double show_f64(int t) const { return msd64_[t]; }
+    int show_n_times() const { return (int)msd_.size(); }
+
+  private:
+    friend class Non_Gaussian_Parameter;
+    System *system_;
+    std::vector<float> timetable_, msd_;
+    std::vector<double> msd64_, s4_, count_;
+};
+
+class Non_Gaussian_Parameter {
+  public:
+    Non_Gaussian_Parameter(System *sys, const Mean_Square_Displacement *msd);
+    void analyze(Trajectory_List *t_list);
+    void write(const std::string &filename) const;
+    int max() const;
+    float show(int t) const { return ngp_[t]; }
+
+  private:
+    System *system_;
+    const Mean_Square_Displacement *msd_;
+    std::vector<float> timetable_, ngp_;
+};
+
+// Input-script interpreter for the commands above (src/control.cpp:108-531
+// for the script structure).  Runs a script file; returns the exit code.
+class Control {
+  public:
+    explicit Control(const std::string &input_file);
+    int run();
+
+  private:
+    std::string read_line();
+    void create_list(const std::vector<std::string> &args);
+    void msd(const std::vector<std::string> &args);
+    void ngp(const std::vector<std::string> &args);
+    void isfs(const std::vector<std::string> &args);
+    void isf(const std::vector<std::string> &args);
+    void structure_factor(const std::vector<std::string> &args);
+    Trajectory_List *target(const std::string &line);
+
+    std::vector<std::string> lines_;
+    size_t cursor_ = 0;
+    std::unique_ptr<System> system_;
+    std::vector<std::pair<std::string, std::unique_ptr<Trajectory_List>>> lists_;
+};
+
+std::vector<std::string> tokenize(const std::string &line);
+
+}  // namespace amdat_hip

@@ -1,0 +1,408 @@
+// analyses.cpp -- Wave_Vectors and the Analysis classes of the drop-in
+// layer.  Each analyze() runs the device path through the C ABI; each
+// write() reproduces the reference writer's text format.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+
+#include "amdat_host.h"
+
+namespace amdat_hip {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;  // src/wave_vectors.cpp:16
+
+void check(int rc, const char *what)
+{
+    if (rc < 0) throw Error(std::string(what) + ": " + amdat_last_error(), 0);
+}
+
+std::string wv3d_stem()
+{
+    const char *e = std::getenv("AMDAT_WV3D");
+    if (!e || !*e)
+        throw Error("AMDAT_WV3D is not set: point it at the q-vector file stem "
+                    "(the reference's qvectors3d/qvector)", 0);
+    return e;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Wave_Vectors (src/wave_vectors.cpp:77-181, 430-484, 527-566)
+
+Wave_Vectors::Wave_Vectors(System *sys, const std::string &plane, float max_length_scale, int n_wavenumbers)
+    : system_(sys)
+{
+    if (plane != "xyz")
+        throw Error("plane " + plane + " is not supported by this build (xyz only)", 1);
+    if (n_wavenumbers < 1 || n_wavenumbers > 299) throw Error("n_wavenumbers must be in 1..299", 0);
+    float min_size;
+    if (max_length_scale <= 0.0f) {
+        std::cout << "Maximum lengthscale for wavevector decomposition <= 0; defaulting to box size.\n";
+        min_size = sys->min_box_dimensions().min();
+    } else {
+        if (max_length_scale > sys->min_box_dimensions().min())
+            std::cout << "\nWARNING: Maximum length scale larger than smallest box dimension. Consider choosing a "
+                         "different maximum for wavevector decomposition.";
+        min_size = max_length_scale;
+    }
+    length_scale_ = min_size;
+    wavegrid_spacing_ = (float)(2.0 * kPi / min_size);
+    delta_wavenumber_ = (float)(wavegrid_spacing_ / 2.0);
+    for (int wn = 0; wn < n_wavenumbers; wn++) approx_.push_back(wavegrid_spacing_ + delta_wavenumber_ * wn);
+    off_.assign(n_wavenumbers + 1, 0);
+    const int cap = 300 * n_wavenumbers + 1024;
+    nvec_.assign(3 * (size_t)cap, 0);
+    int n = 0;
+    check(amdat_shells_read(wv3d_stem().c_str(), 2, n_wavenumbers, off_.data(), nvec_.data(), cap, &n),
+          "reading wave vectors");
+    nvec_.resize(3 * (size_t)n);
+    check(amdat_shells_upload(device_context(), n_wavenumbers, off_.data(), nvec_.data(), &shells_),
+          "wave vector upload");
+}
+
+Wave_Vectors::~Wave_Vectors() { amdat_shells_free(device_context(), shells_); }
+
+std::vector<Coordinate> Wave_Vectors::vectorlist(int index) const
+{
+    std::vector<Coordinate> out;
+    for (int v = off_[index]; v < off_[index + 1]; v++)
+        out.emplace_back((float)nvec_[3 * v] * wavegrid_spacing_, (float)nvec_[3 * v + 1] * wavegrid_spacing_,
+                         (float)nvec_[3 * v + 2] * wavegrid_spacing_);
+    return out;
+}
+
+float Wave_Vectors::show_mean_wavenumber(int index) const
+{
+    float mean = 0;
+    auto vl = vectorlist(index);
+    for (auto &c : vl) mean += c.length();
+    mean /= vl.size();
+    return mean;
+}
+
+float Wave_Vectors::show_stdev_wavenumber(int index) const
+{
+    float variance = 0;
+    const float mean = show_mean_wavenumber(index);
+    auto vl = vectorlist(index);
+    for (auto &c : vl) variance += std::pow(c.length() - mean, 2.0);
+    variance /= vl.size();
+    return (float)std::pow(variance, 0.5);
+}
+
+Coordinate Wave_Vectors::show_mean_wavevector(int index) const
+{
+    float mx = 0, my = 0, mz = 0;
+    auto vl = vectorlist(index);
+    for (auto &c : vl) { mx += c.show_x(); my += c.show_y(); mz += c.show_z(); }
+    const float n = (float)vl.size();
+    return Coordinate(mx / n, my / n, mz / n);
+}
+
+// ---------------------------------------------------------------------------
+// Wave_Density (src/wave_density.cpp:273-365)
+
+Wave_Density::Wave_Density(System *sys, const Wave_Vectors *wv, int inner, int outer)
+    : system_(sys), wv_(wv), first_(inner), last_(outer)
+{
+    if (inner < 0 || outer < inner || outer >= wv->show_n_wavenumbers())
+        throw Error("wavenumber range outside the wave vector table", 0);
+}
+
+Wave_Density::~Wave_Density() { amdat_rho_free(device_context(), rho_); }
+
+void Wave_Density::analyze(Trajectory_List *t_list)
+{
+    amdat_rho_free(device_context(), rho_);
+    rho_ = nullptr;
+    shared_ = nullptr;
+    host_.clear();
+    n_atoms_ = t_list->show_n_trajectories(0);
+    check(amdat_rho_compute(device_context(), system_->traj(), t_list->handle(), wv_->handle(), first_, last_,
+                            wv_->show_length_scale(), nullptr, 0, AMDAT_PREC_FP32C, &rho_),
+          "wave density");
+}
+
+void Wave_Density::copy_from(const Wave_Density &o)
+{
+    if (!o.rho_) throw Error("wave density copied before it was computed", 0);
+    // The reference deep-copies (wd2 = wd1); the density is read-only after
+    // analyze(), so the copy shares the source's device buffer.
+    amdat_rho_free(device_context(), rho_);
+    rho_ = nullptr;
+    n_atoms_ = o.n_atoms_;
+    first_ = o.first_;
+    last_ = o.last_;
+    wv_ = o.wv_;
+    shared_ = &o;
+}
+
+const amdat_rho *Wave_Density::device() const { return shared_ ? shared_->device() : rho_; }
+
+void Wave_Density::fetch() const
+{
+    if (!host_.empty()) return;
+    const amdat_rho *r = device();
+    if (!r) throw Error("wave density not computed", 0);
+    int nf = 0, nv = 0;
+    amdat_rho_shape(r, &nf, &nv);
+    host_.assign(2 * (size_t)nf * nv, 0.0);
+    check(amdat_rho_download(device_context(), r, host_.data(), host_.size()), "wave density download");
+}
+
+double Wave_Density::show_density_re(int t, int wn, int v) const
+{
+    fetch();
+    const int K = wv_->offsets()[last_ + 1] - wv_->offsets()[first_];
+    const int idx = wv_->offsets()[first_ + wn] - wv_->offsets()[first_] + v;
+    return host_[2 * ((size_t)t * K + idx)];
+}
+
+double Wave_Density::show_density_im(int t, int wn, int v) const
+{
+    fetch();
+    const int K = wv_->offsets()[last_ + 1] - wv_->offsets()[first_];
+    const int idx = wv_->offsets()[first_ + wn] - wv_->offsets()[first_] + v;
+    return host_[2 * ((size_t)t * K + idx) + 1];
+}
+
+// ---------------------------------------------------------------------------
+// Structure_Factor (src/structure_factor.cpp:55-344, src/analysis_onetime.cpp:22-85)
+
+Structure_Factor::Structure_Factor(System *sys, const Wave_Vectors *wv, int timescheme)
+    : system_(sys), wv_(wv), time_scheme_(timescheme), structure_factor_(wv->show_n_wavenumbers(), 0.0f)
+{
+}
+
+std::vector<int> Structure_Factor::frames() const
+{
+    std::vector<int> f;
+    if (time_scheme_ == -1) {
+        for (int t = 0; t < system_->show_n_timesteps(); t++) f.push_back(t);
+    } else if (time_scheme_ < -1) {
+        f.push_back(-time_scheme_ - 2);
+    } else {
+        // The reference bounds this loop by the block count, not the frame
+        // count (src/analysis_onetime.cpp:43); kept for parity.
+        for (int t = time_scheme_; t < system_->show_n_exponentials(); t += system_->show_n_exponential_steps()) {
+            f.push_back(t);
+            if (system_->show_n_exponential_steps() == 0) break;
+        }
+    }
+    return f;
+}
+
+static void structure_factor_run(System *sys, const Wave_Vectors *wv, const std::vector<int> &fr,
+                                 Trajectory_List *l1, Trajectory_List *l2, std::vector<float> &S)
+{
+    const int nsh = wv->show_n_wavenumbers();
+    std::fill(S.begin(), S.end(), 0.0f);
+    if (fr.empty()) return;
+    for (int f : fr)
+        if (f < 0 || f >= sys->show_n_timesteps()) throw Error("structure factor frame out of range", 0);
+    amdat_rho *r1 = nullptr, *r2 = nullptr;
+    check(amdat_rho_compute(device_context(), sys->traj(), l1->handle(), wv->handle(), 0, nsh - 1,
+                            wv->show_length_scale(), fr.data(), (int)fr.size(), AMDAT_PREC_FP32C, &r1),
+          "structure factor density");
+    if (l2) {
+        int rc = amdat_rho_compute(device_context(), sys->traj(), l2->handle(), wv->handle(), 0, nsh - 1,
+                                   wv->show_length_scale(), fr.data(), (int)fr.size(), AMDAT_PREC_FP32C, &r2);
+        if (rc < 0) { amdat_rho_free(device_context(), r1); check(rc, "structure factor density"); }
+    }
+    std::vector<double> s(nsh);
+    int rc = amdat_sk(device_context(), r1, r2 ? r2 : r1, s.data());
+    amdat_rho_free(device_context(), r1);
+    amdat_rho_free(device_context(), r2);
+    check(rc, "structure factor");
+    for (int i = 0; i < nsh; i++) S[i] = (float)s[i];
+}
+
+void Structure_Factor::analyze(Trajectory_List *t_list)
+{
+    structure_factor_run(system_, wv_, frames(), t_list, nullptr, structure_factor_);
+}
+
+void Structure_Factor::analyze(Trajectory_List *t_list, Trajectory_List *t_list2)
+{
+    structure_factor_run(system_, wv_, frames(), t_list, t_list2, structure_factor_);
+}
+
+void Structure_Factor::write(const std::string &filename) const
+{
+    std::ofstream out(filename);
+    std::cout << "\nWriting to file " << filename << ".";
+    out << "Structure factor data created by AMDAT v." << kSemver << "\n";
+    out << "Nominal_q\tMean_q\tStddev_q\tMean_qx\tMean_qy\tMean_qz\tS(q)\n";
+    for (int b = 0; b < wv_->show_n_wavenumbers(); b++) {
+        Coordinate m = wv_->show_mean_wavevector(b);
+        out << wv_->show_approx_wavenumber(b) << "\t" << wv_->show_mean_wavenumber(b) << "\t"
+            << wv_->show_stdev_wavenumber(b) << "\t" << m.show_x() << "\t" << m.show_y() << "\t" << m.show_z() << "\t"
+            << structure_factor_[b] << std::endl;
+    }
+    out << "\n";
+}
+
+// ---------------------------------------------------------------------------
+// Correlation_2D-style writer (src/correlation_2d.cpp:475-504)
+
+static void write_correlation(const std::string &filename, const Wave_Vectors *wv, int first, int last,
+                              const std::vector<float> &timetable, const std::vector<std::vector<float>> &corr)
+{
+    std::ofstream out(filename);
+    std::cout << "\nWriting to file " << filename << "." << std::endl;
+    out << "Correlation data created by AMDAT v." << kSemver << "\n";
+    out << "\t";
+    for (int b = first; b <= last; b++) out << wv->show_approx_wavenumber(b) << "\t";
+    out << "\n";
+    for (size_t t = 0; t < corr.size(); t++) {
+        out << timetable[t] << "\t";
+        for (float v : corr[t]) out << v << "\t";
+        out << "\n";
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Intermediate_Scattering_Function (src/intermediate_scattering_function.cpp:237-322)
+
+Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, Wave_Density *wd1,
+                                                                   Wave_Density *wd2, int wn1, int wn2,
+                                                                   bool parallel)
+    : system_(sys), first_bin_(wn1), last_bin_(wn2)
+{
+    if (wd1->show_wavevectors() != wd2->show_wavevectors())
+        throw Error("Wave vectors for density distributions do not match!", 0);
+    if (wd1->show_system() != wd2->show_system()) throw Error("Systems for density distributions do not match!", 0);
+    wv_ = wd1->show_wavevectors();
+    if (wn1 < wd1->show_first() || wn1 < wd2->show_first())
+        throw Error("requested wavenumber below range of that contained in fourier density object.", 0);
+    if (wn1 > wd1->show_last() || wn1 > wd2->show_last() || wn2 > wd1->show_last() || wn2 > wd2->show_last() ||
+        wn2 < wn1)
+        throw Error("requested wavenumber above range of that contained in fourier density object.", 0);
+    timetable_ = sys->displacement_times();
+    amdat_pairs *p = sys->displacement_pairs(parallel);
+    const int ntg = sys->show_n_timegaps(), nsh = wn2 - wn1 + 1;
+    std::vector<double> F((size_t)ntg * nsh), cnt(ntg);
+    int rc = amdat_fkt(device_context(), wd1->device(), wd2->device(), p, wn1 - wd1->show_first(),
+                       wn2 - wd1->show_first(), F.data(), cnt.data());
+    amdat_pairs_free(device_context(), p);
+    check(rc, "intermediate scattering function");
+    correlation_.assign(ntg, std::vector<float>(nsh));
+    for (int t = 0; t < ntg; t++)
+        for (int s = 0; s < nsh; s++) correlation_[t][s] = (float)F[(size_t)t * nsh + s];
+}
+
+void Intermediate_Scattering_Function::write(const std::string &filename) const
+{
+    write_correlation(filename, wv_, first_bin_, last_bin_, timetable_, correlation_);
+}
+
+// ---------------------------------------------------------------------------
+// Incoherent_Scattering_Function (src/incoherent_scattering_function.cpp:176-299)
+
+Incoherent_Scattering_Function::Incoherent_Scattering_Function(System *sys, const Wave_Vectors *wv, int inner,
+                                                               int outer, bool fblock)
+    : system_(sys), wv_(wv), first_bin_(inner), last_bin_(outer), fullblock_(fblock)
+{
+    if (inner < 0 || outer < inner || outer >= wv->show_n_wavenumbers())
+        throw Error("wavenumber range outside the wave vector table", 0);
+    timetable_ = sys->displacement_times();
+    correlation_.assign(sys->show_n_timegaps(), std::vector<float>(outer - inner + 1, 0.0f));
+}
+
+void Incoherent_Scattering_Function::analyze(Trajectory_List *t_list)
+{
+    amdat_pairs *p = system_->displacement_pairs(fullblock_);
+    const int ntg = system_->show_n_timegaps(), nsh = last_bin_ - first_bin_ + 1;
+    std::vector<double> F((size_t)ntg * nsh), cnt(ntg);
+    int rc = amdat_fs(device_context(), system_->traj(), t_list->handle(), wv_->handle(), first_bin_, last_bin_,
+                      wv_->show_length_scale(), p, AMDAT_PREC_FP32C, F.data(), cnt.data());
+    amdat_pairs_free(device_context(), p);
+    check(rc, "incoherent scattering function");
+    for (int t = 0; t < ntg; t++)
+        for (int s = 0; s < nsh; s++) correlation_[t][s] = (float)F[(size_t)t * nsh + s];
+}
+
+void Incoherent_Scattering_Function::write(const std::string &filename) const
+{
+    write_correlation(filename, wv_, first_bin_, last_bin_, timetable_, correlation_);
+}
+
+// ---------------------------------------------------------------------------
+// Mean_Square_Displacement (src/mean_square_displacement.cpp:54-200)
+
+Mean_Square_Displacement::Mean_Square_Displacement(System *sys) : system_(sys)
+{
+    timetable_ = sys->displacement_times();
+    msd_.assign(sys->show_n_timegaps(), 0.0f);
+}
+
+void Mean_Square_Displacement::analyze(Trajectory_List *t_list)
+{
+    // MSD always uses block-start time origins (fullblock = false).
+    amdat_pairs *p = system_->displacement_pairs(false);
+    const int ntg = system_->show_n_timegaps();
+    std::vector<double> s2(ntg);
+    s4_.assign(ntg, 0.0);
+    count_.assign(ntg, 0.0);
+    int rc = amdat_msd_ngp(device_context(), system_->traj(), t_list->handle(), p, s2.data(), s4_.data(),
+                           count_.data());
+    amdat_pairs_free(device_context(), p);
+    check(rc, "mean square displacement");
+    msd64_.assign(ntg, 0.0);
+    for (int t = 0; t < ntg; t++) {
+        msd64_[t] = s2[t] / count_[t];
+        msd_[t] = (float)msd64_[t];
+    }
+}
+
+void Mean_Square_Displacement::write(const std::string &filename) const
+{
+    std::cout << "\nWriting msd to file " << filename << ".";
+    std::ofstream out(filename);
+    out << "Mean square displacement data created by AMDAT v." << kSemver << "\n";
+    for (size_t t = 0; t < msd_.size(); t++) out << timetable_[t] << "\t" << msd_[t] << "\n";
+}
+
+// ---------------------------------------------------------------------------
+// Non_Gaussian_Parameter (src/non_gaussian_parameter.cpp:96-190)
+
+Non_Gaussian_Parameter::Non_Gaussian_Parameter(System *sys, const Mean_Square_Displacement *msd)
+    : system_(sys), msd_(msd)
+{
+    timetable_ = sys->displacement_times();
+    ngp_.assign(sys->show_n_timegaps(), 0.0f);
+}
+
+void Non_Gaussian_Parameter::analyze(Trajectory_List *t_list)
+{
+    // Fourth moment from the same pass as the MSD's second moment.
+    Mean_Square_Displacement m(system_);
+    m.analyze(t_list);
+    for (size_t t = 0; t < ngp_.size(); t++) {
+        const double msd = msd_ && (int)msd_->msd64_.size() == (int)ngp_.size() ? msd_->msd64_[t] : m.msd64_[t];
+        // alpha2 = 3 <r^4> / (5 <r^2>^2) - 1; NaN at t = 0 as in the reference.
+        ngp_[t] = (float)(m.s4_[t] * (3.0 / m.count_[t]) / (5.0 * msd * msd) - 1.0);
+    }
+}
+
+void Non_Gaussian_Parameter::write(const std::string &filename) const
+{
+    std::cout << "\nWriting non-Gaussian parameter to file " << filename << ".";
+    std::ofstream out(filename);
+    out << "Non-Gaussian parameter data created by AMDAT v." << kSemver << "\n";
+    for (size_t t = 0; t < ngp_.size(); t++) out << timetable_[t] << "\t" << ngp_[t] << "\n";
+}
+
+int Non_Gaussian_Parameter::max() const
+{
+    int maxtime = -1;
+    float maxvalue = 0;
+    for (size_t t = 1; t < ngp_.size(); t++)
+        if (ngp_[t] > maxvalue) { maxvalue = ngp_[t]; maxtime = (int)t; }
+    return maxtime;
+}
+
+}  // namespace amdat_hip

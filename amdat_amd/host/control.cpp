@@ -1,0 +1,226 @@
+// control.cpp -- input-script interpreter and command-line entry of the
+// drop-in layer, for the commands of the reciprocal-space and
+// time-correlation path.  Script structure: src/control.cpp:125-531
+// (comments with #, a system block, then command lines each followed by
+// their target lines); command syntax: src/control.cpp:1860-1916 (limit,
+// create_list), 2163-2210 (msd), 2655-2758 (structure_factor), 3107-3192
+// (isf), 3198-3238 (isfs), 3345-3370 (ngp).
+#include <cstdlib>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+
+#include "amdat_host.h"
+
+namespace amdat_hip {
+
+Control::Control(const std::string &input_file)
+{
+    std::ifstream in(input_file);
+    if (!in) throw Error("Error opening input file " + input_file, 0);
+    std::string line;
+    bool skipping = false;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        auto t = tokenize(line);
+        if (!t.empty() && t[0] == "skip") { skipping = true; continue; }
+        if (!t.empty() && t[0] == "endskip") { skipping = false; continue; }
+        if (skipping) continue;
+        lines_.push_back(line);
+    }
+}
+
+std::string Control::read_line()
+{
+    while (cursor_ < lines_.size()) {
+        const std::string &l = lines_[cursor_++];
+        if (!tokenize(l).empty()) return l;
+    }
+    throw Error("unexpected end of input file", 0);
+}
+
+Trajectory_List *Control::target(const std::string &line)
+{
+    auto t = tokenize(line);
+    if (t.size() != 2 || t[0] != "list")
+        throw Error("target '" + line + "' is not supported by this build (list <name> only)", 1);
+    for (auto &p : lists_)
+        if (p.first == t[1]) return p.second.get();
+    throw Error("Trajectory list '" + t[1] + "' not found.", 0);
+}
+
+static std::vector<std::string> strip_store_flag(std::vector<std::string> a)
+{
+    std::vector<std::string> out;
+    for (auto &s : a)
+        if (s.rfind("\\s", 0) != 0) out.push_back(s);
+    return out;
+}
+
+void Control::create_list(const std::vector<std::string> &args)
+{
+    if (args.size() != 2) throw Error("create_list expects one list name", 0);
+    auto sel = tokenize(read_line());
+    auto atoms = system_->select(sel);
+    lists_.emplace_back(args[1], std::make_unique<Trajectory_List>(system_.get(), atoms));
+    std::cout << "\nTrajectory list " << args[1] << " created with " << atoms.size() << " trajectories." << std::endl;
+}
+
+void Control::msd(const std::vector<std::string> &args)
+{
+    if (args.size() != 2) throw Error("msd expects an output file", 0);
+    Trajectory_List *l = target(read_line());
+    Mean_Square_Displacement m(system_.get());
+    std::cout << "\nCalculating mean square displacement.\n";
+    const time_t start = time(nullptr);
+    m.analyze(l);
+    m.write(args[1]);
+    std::cout << "\nCalculated mean square displacement in " << time(nullptr) - start << " seconds." << std::endl;
+}
+
+void Control::ngp(const std::vector<std::string> &args)
+{
+    if (args.size() != 2) throw Error("ngp expects an output file", 0);
+    Trajectory_List *l = target(read_line());
+    Mean_Square_Displacement m(system_.get());
+    m.analyze(l);
+    Non_Gaussian_Parameter g(system_.get(), &m);
+    g.analyze(l);
+    g.write(args[1]);
+    std::cout << "\n Peak time index of non-Gaussian parameter is " << g.max() << ".";
+}
+
+void Control::isfs(const std::vector<std::string> &args)
+{
+    // isfs <file> <inner> <outer> <plane> <max_length_scale> [fullblock]
+    if (args.size() != 6 && args.size() != 7) throw Error("Incorrect number of arguments for isfs.", 0);
+    const int inner = std::atoi(args[2].c_str()), outer = std::atoi(args[3].c_str());
+    const float ell = (float)std::atof(args[5].c_str());
+    const bool fb = args.size() == 7 ? std::atoi(args[6].c_str()) != 0 : false;
+    Trajectory_List *l = target(read_line());
+    const time_t start = time(nullptr);
+    Wave_Vectors wv(system_.get(), args[4], ell);
+    Incoherent_Scattering_Function f(system_.get(), &wv, inner, outer, fb);
+    f.analyze(l);
+    f.write(args[1]);
+    std::cout << "\nSelf Intermediate Scattering Function calculated in " << time(nullptr) - start
+              << " seconds.\n";
+}
+
+void Control::isf(const std::vector<std::string> &args)
+{
+    // isf <file> auto <symmetric|asymmetric> <plane> <fullblock> <inner> <outer>
+    if (args.size() < 6) throw Error("Incorrect number of arguments for isf.", 0);
+    const std::string method = args[2], symmetry = args[3], plane = args[4];
+    const bool fb = std::atoi(args[5].c_str()) != 0;
+    if (method != "auto")
+        throw Error("isf method " + method + " (interactive wavenumber entry) is not supported by this build", 1);
+    if (args.size() != 8) throw Error("Incorrect number of arguments for isf auto.", 0);
+    const int inner = std::atoi(args[6].c_str()), outer = std::atoi(args[7].c_str());
+    Wave_Vectors wv(system_.get(), plane);  // isf always uses the box size
+    Wave_Density wd1(system_.get(), &wv, inner, outer), wd2(system_.get(), &wv, inner, outer);
+    wd1.analyze(target(read_line()));
+    if (symmetry == "symmetric") {
+        wd2.copy_from(wd1);
+    } else if (symmetry == "asymmetric") {
+        wd2.analyze(target(read_line()));
+    } else {
+        throw Error("command for structure factor calculation atom set types not understood.", 0);
+    }
+    Intermediate_Scattering_Function f(system_.get(), &wd1, &wd2, inner, outer, fb);
+    f.write(args[1]);
+}
+
+void Control::structure_factor(const std::vector<std::string> &args)
+{
+    // structure_factor <file> <symmetry> <plane> <max_length_scale> [timescheme]
+    if (args.size() != 5 && args.size() != 6) throw Error("Incorrect number of arguments for structure_factor.", 0);
+    float ell = (float)std::atof(args[4].c_str());
+    // A 5-token line has no timescheme: the reference reads atoi("") = 0
+    // (src/control.cpp:2664-2673).
+    const int ts = args.size() == 6 ? std::atoi(args[5].c_str()) : 0;
+    if (ell < 0 && ts < -1) ell = system_->size(-ts - 2).min();
+    Wave_Vectors wv(system_.get(), args[3], ell);
+    Structure_Factor sf(system_.get(), &wv, ts);
+    Trajectory_List *l1 = target(read_line());
+    const time_t start = time(nullptr);
+    if (args[2] == "symmetric") {
+        sf.analyze(l1);
+    } else if (args[2] == "asymmetric") {
+        sf.analyze(l1, target(read_line()));
+    } else {
+        throw Error("command for structure factor calculation atom set types not understood.", 0);
+    }
+    std::cout << "\nCalculated structure factor in " << time(nullptr) - start << " seconds.\n";
+    sf.write(args[1]);
+}
+
+int Control::run()
+{
+    // System block: type, trajectory type, file, time scheme, species,
+    // types, one count row per species.
+    std::vector<std::string> block;
+    for (int i = 0; i < 6; i++) block.push_back(read_line());
+    const size_t n_species = tokenize(block[4]).size() / 2;
+    for (size_t s = 0; s < n_species; s++) block.push_back(read_line());
+    system_ = std::make_unique<System>(block);
+    std::cout << "\nSystem created." << std::endl;
+
+    while (cursor_ < lines_.size()) {
+        std::string line;
+        try {
+            line = read_line();
+        } catch (const Error &) {
+            break;
+        }
+        auto args = strip_store_flag(tokenize(line));
+        const std::string &cmd = args[0];
+        std::cout << "\n" << line << std::endl;
+        try {
+            if (cmd == "exit") break;
+            else if (cmd == "create_list") create_list(args);
+            else if (cmd == "limit") system_->set_limit(std::atoi(args.at(1).c_str()));
+            else if (cmd == "processors") continue;
+            else if (cmd == "msd") msd(args);
+            else if (cmd == "ngp") ngp(args);
+            else if (cmd == "isfs") isfs(args);
+            else if (cmd == "isf") isf(args);
+            else if (cmd == "structure_factor") structure_factor(args);
+            else if (cmd == "print") { std::cout << line.substr(line.find("print") + 5) << std::endl; }
+            else throw Error("command " + cmd + " is not supported by this build", 0);
+        } catch (const Error &e) {
+            if (e.severe()) {
+                std::cout << "\nError: " << e.what() << std::endl;
+                return e.code() == 0 ? 1 : -e.code();
+            }
+            std::cout << "\nError: " << e.what() << " Skipping command." << std::endl;
+        }
+    }
+    return 0;
+}
+
+}  // namespace amdat_hip
+
+int main(int argc, char **argv)
+{
+    std::string input;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        if (a == "-i" && i + 1 < argc) input = argv[++i];
+        else if (a == "-v") { std::cout << "AMDAT " << amdat_hip::kSemver << std::endl; return 0; }
+        else if (a == "-n" && i + 1 < argc) ++i;  // thread count: the device path ignores it
+    }
+    if (input.empty()) {
+        std::cerr << "usage: amdat_hip -i <input file>" << std::endl;
+        return 2;
+    }
+    try {
+        amdat_hip::Control c(input);
+        return c.run();
+    } catch (const amdat_hip::Error &e) {
+        std::cout << "\nError: " << e.what() << std::endl;
+        return 1;
+    }
+}

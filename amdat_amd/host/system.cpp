@@ -1,0 +1,363 @@
+// system.cpp -- System, Trajectory_List, Coordinate and the device
+// context of the C++ drop-in layer.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+
+#include "amdat_host.h"
+
+namespace amdat_hip {
+
+const char *kSemver = "1.0.1+mi355x";
+
+void warn(const std::string &msg) { std::cout << "\nWarning: " << msg << std::endl; }
+
+static void check(int rc, const char *what)
+{
+    if (rc < 0) throw Error(std::string(what) + ": " + amdat_last_error(), 0);
+}
+
+amdat_ctx *device_context()
+{
+    static amdat_ctx *ctx = nullptr;
+    if (!ctx) {
+        int dev = 0;
+        if (const char *e = std::getenv("AMDAT_DEVICE")) dev = std::atoi(e);
+        check(amdat_ctx_create(dev, &ctx), "device context");
+    }
+    return ctx;
+}
+
+float Coordinate::length() const { return std::sqrt(x_ * x_ + y_ * y_ + z_ * z_); }
+
+std::vector<std::string> tokenize(const std::string &line)
+{
+    std::vector<std::string> out;
+    std::istringstream is(line);
+    std::string t;
+    while (is >> t) out.push_back(t);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Trajectory_List
+
+Trajectory_List::Trajectory_List(System *sys, std::vector<int> atoms) : system_(sys), atoms_(std::move(atoms))
+{
+    check(amdat_list_create(device_context(), sys->traj(), (int)atoms_.size(),
+                            atoms_.empty() ? nullptr : atoms_.data(), &list_),
+          "trajectory list");
+}
+
+Trajectory_List::~Trajectory_List() { amdat_list_free(device_context(), list_); }
+
+// ---------------------------------------------------------------------------
+// System
+
+void System::init_timescheme(const std::vector<std::string> &a)
+{
+    if (a.empty()) throw Error("missing time scheme line", 0);
+    const std::string &type = a[0];
+    // src/system.cpp:88-141
+    if (type == "linear") {
+        if (a.size() != 3) throw Error("Incorrect number of arguments for linear time style.", 0);
+        n_timesteps_ = std::atoi(a[1].c_str());
+        n_exponentials_ = n_timesteps_ - 1;
+        exp_base_ = 1.0f;
+        n_exponential_steps_ = 1;
+        first_exponent_ = 0;
+        frt_ = 0;
+        time_unit_ = (float)std::atof(a[2].c_str());
+    } else if (type == "exponential") {
+        if (a.size() != 7) throw Error("Incorrect number of arguments for exponential time style.", 0);
+        n_exponentials_ = std::atoi(a[1].c_str());
+        n_exponential_steps_ = std::atoi(a[2].c_str());
+        exp_base_ = (float)std::atof(a[3].c_str());
+        frt_ = std::atoi(a[4].c_str());
+        first_exponent_ = std::atoi(a[5].c_str());
+        n_timesteps_ = n_exponential_steps_ * n_exponentials_ + 1;
+        time_unit_ = (float)std::atof(a[6].c_str());
+    } else if (type == "snapshot") {
+        if (a.size() != 1) throw Error("Incorrect number of arguments for snapshot time style.", 0);
+        n_timesteps_ = 1;
+        n_exponentials_ = 1;
+        exp_base_ = 1.0f;
+        n_exponential_steps_ = 0;
+        first_exponent_ = 0;
+        frt_ = 0;
+        time_unit_ = 1.0f;
+    } else {
+        throw Error("Time scheme " + type + " not recognized!", 0);
+    }
+    if (n_exponentials_ < 1 || n_exponential_steps_ < 0) throw Error("bad time scheme", 0);
+}
+
+void System::finish()
+{
+    n_timegaps_ = n_exponential_steps_ + n_exponentials_ - 1 + (frt_ ? 0 : 1);
+    timelist_.assign(n_timesteps_, 0.0f);
+    check(amdat_ts_timelist(n_exponentials_, n_exponential_steps_, exp_base_, first_exponent_, time_unit_,
+                            timelist_.data(), n_timesteps_),
+          "time list");
+    check(amdat_traj_upload(device_context(), n_atoms_, n_timesteps_, x_.data(), y_.data(), z_.data(), ix_.data(),
+                            iy_.data(), iz_.data(), box_.data(), &traj_),
+          "trajectory upload");
+}
+
+System::System(int nexp, int nes, float base, int frt, int first_exp, float dt, int n_atoms, const float *x,
+               const float *y, const float *z, const int16_t *ix, const int16_t *iy, const int16_t *iz,
+               const float *box, const int *types)
+{
+    n_exponentials_ = nexp;
+    n_exponential_steps_ = nes;
+    exp_base_ = base;
+    frt_ = frt;
+    first_exponent_ = first_exp;
+    time_unit_ = dt;
+    n_timesteps_ = nes * nexp + 1;
+    if (nes == 0 && nexp == 1) n_timesteps_ = 1;
+    n_atoms_ = n_atoms;
+    const size_t n = (size_t)n_atoms * n_timesteps_;
+    x_.assign(x, x + n); y_.assign(y, y + n); z_.assign(z, z + n);
+    ix_.assign(ix, ix + n); iy_.assign(iy, iy + n); iz_.assign(iz, iz + n);
+    box_.assign(box, box + 3 * (size_t)n_timesteps_);
+    type_.assign(n_atoms, 1);
+    if (types) type_.assign(types, types + n_atoms);
+    // One species of single-atom molecules: list order is file order.
+    list_order_.resize(n_atoms);
+    for (int i = 0; i < n_atoms; i++) list_order_[i] = i;
+    atom_species_.assign(n_atoms, 0);
+    species_names_ = {"all"};
+    finish();
+}
+
+System::System(const std::vector<std::string> &block)
+{
+    // block: system type, trajectory type, file line, time scheme, species
+    // line, types line, one count row per species (src/system.cpp:49-226;
+    // the custom layout of Manual/custom.md).
+    if (block.size() < 6) throw Error("system block too short", 0);
+    const std::string systype = tokenize(block[0]).at(0);
+    if (systype != "system" && systype != "system_nv" && systype != "system_np")
+        throw Error("unknown system type " + systype, 0);
+    const std::string trajtype = tokenize(block[1]).at(0);
+    if (trajtype != "custom")
+        throw Error("trajectory type " + trajtype + " is not supported by this build (custom only)", 0);
+    auto files = tokenize(block[2]);
+    if (files.empty()) throw Error("missing trajectory file name", 0);
+    init_timescheme(tokenize(block[3]));
+    auto sp = tokenize(block[4]);
+    if (sp.size() % 2) throw Error("species line must be name/count pairs", 0);
+    for (size_t i = 0; i < sp.size(); i += 2) {
+        species_names_.push_back(sp[i]);
+        species_mols_.push_back(std::atoi(sp[i + 1].c_str()));
+    }
+    for (auto &t : tokenize(block[5])) type_ids_.push_back(std::atoi(t.c_str()));
+    if (block.size() < 6 + species_names_.size()) throw Error("missing per-species type counts", 0);
+    for (size_t s = 0; s < species_names_.size(); s++) {
+        std::vector<int> row;
+        for (auto &t : tokenize(block[6 + s])) row.push_back(std::atoi(t.c_str()));
+        if (row.size() != type_ids_.size()) throw Error("type count row does not match the types line", 0);
+        species_type_counts_.push_back(row);
+    }
+    read_custom(files[0]);
+    finish();
+}
+
+// LAMMPS custom dump with wrapped x y z and image counts ix iy iz (or
+// unwrapped xu yu zu, from which wrapped coordinates and images follow),
+// atoms in the same order in every frame (src/system.cpp:896-1367).
+void System::read_custom(const std::string &file)
+{
+    std::ifstream in(file);
+    if (!in) throw Error("Error opening trajectory file " + file, 0);
+    int per_mol_total = 0;
+    n_atoms_ = 0;
+    for (size_t s = 0; s < species_names_.size(); s++) {
+        int per = 0;
+        for (int c : species_type_counts_[s]) per += c;
+        n_atoms_ += per * species_mols_[s];
+        per_mol_total += per;
+    }
+    (void)per_mol_total;
+    const int N = n_atoms_, T = n_timesteps_;
+    x_.assign((size_t)N * T, 0); y_.assign((size_t)N * T, 0); z_.assign((size_t)N * T, 0);
+    ix_.assign((size_t)N * T, 0); iy_.assign((size_t)N * T, 0); iz_.assign((size_t)N * T, 0);
+    box_.assign(3 * (size_t)T, 0);
+    type_.assign(N, 0);
+    std::string line;
+    for (int t = 0; t < T; t++) {
+        int natoms = -1;
+        float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        std::vector<std::string> cols;
+        while (std::getline(in, line)) {
+            if (line.rfind("ITEM: NUMBER OF ATOMS", 0) == 0) {
+                std::getline(in, line);
+                natoms = std::atoi(line.c_str());
+            } else if (line.rfind("ITEM: BOX BOUNDS", 0) == 0) {
+                for (int c = 0; c < 3; c++) {
+                    std::getline(in, line);
+                    auto b = tokenize(line);
+                    lo[c] = (float)std::atof(b.at(0).c_str());
+                    hi[c] = (float)std::atof(b.at(1).c_str());
+                }
+            } else if (line.rfind("ITEM: ATOMS", 0) == 0) {
+                auto h = tokenize(line);
+                cols.assign(h.begin() + 2, h.end());
+                break;
+            }
+        }
+        if (cols.empty()) throw Error("trajectory file ended before frame " + std::to_string(t), 0);
+        if (natoms != N) throw Error("frame atom count does not match the system block", 0);
+        std::map<std::string, int> at;
+        for (size_t c = 0; c < cols.size(); c++) at[cols[c]] = (int)c;
+        const bool wrapped = at.count("x") && at.count("y") && at.count("z");
+        const bool unwrapped = at.count("xu") && at.count("yu") && at.count("zu");
+        const bool images = at.count("ix") && at.count("iy") && at.count("iz");
+        if (!at.count("type")) throw Error("custom file lacks the type column", 0);
+        if (!(wrapped && images) && !unwrapped)
+            throw Error("custom file needs x y z with ix iy iz, or xu yu zu", 0);
+        float L[3];
+        for (int c = 0; c < 3; c++) {
+            L[c] = hi[c] - lo[c];
+            box_[3 * t + c] = L[c];
+        }
+        for (int j = 0; j < N; j++) {
+            if (!std::getline(in, line)) throw Error("trajectory file truncated", 0);
+            auto v = tokenize(line);
+            const size_t o = (size_t)t * N + j;
+            type_[j] = std::atoi(v.at(at["type"]).c_str());
+            float w[3];
+            int16_t im[3];
+            if (wrapped && images && !unwrapped) {
+                const char *nm[3] = {"x", "y", "z"}, *inm[3] = {"ix", "iy", "iz"};
+                for (int c = 0; c < 3; c++) {
+                    w[c] = (float)std::atof(v.at(at[nm[c]]).c_str());
+                    im[c] = (int16_t)std::atoi(v.at(at[inm[c]]).c_str());
+                }
+            } else {
+                // unwrapped given: wrapped = u - L*floor((u-lo)/L) (src/trajectory.cpp wrap())
+                const char *nm[3] = {"xu", "yu", "zu"};
+                for (int c = 0; c < 3; c++) {
+                    const double u = (float)std::atof(v.at(at[nm[c]]).c_str());
+                    const double k = std::floor((u - lo[c]) / L[c]);
+                    w[c] = (float)(u - k * L[c]);
+                    im[c] = (int16_t)k;
+                }
+            }
+            x_[o] = w[0]; y_[o] = w[1]; z_[o] = w[2];
+            ix_[o] = im[0]; iy_[o] = im[1]; iz_[o] = im[2];
+        }
+    }
+    // Reference list order (src/system.cpp:4783-4791): species -> molecule
+    // -> type (types-line order) -> index within the molecule.
+    list_order_.clear();
+    atom_species_.assign(N, 0);
+    int pos = 0;
+    for (size_t s = 0; s < species_names_.size(); s++) {
+        int per = 0;
+        for (int c : species_type_counts_[s]) per += c;
+        for (int m = 0; m < species_mols_[s]; m++) {
+            for (int a = 0; a < per; a++) atom_species_[pos + a] = (int)s;
+            for (size_t ti = 0; ti < type_ids_.size(); ti++)
+                for (int a = 0; a < per; a++)
+                    if (type_[pos + a] == type_ids_[ti]) list_order_.push_back(pos + a);
+            pos += per;
+        }
+    }
+    if ((int)list_order_.size() != N) throw Error("atom types in the file do not match the types line", 0);
+}
+
+System::~System() { amdat_traj_free(device_context(), traj_); }
+
+Coordinate System::min_box_dimensions() const
+{
+    float m[3] = {box_[0], box_[1], box_[2]};
+    for (int t = 1; t < n_timesteps_; t++)
+        for (int c = 0; c < 3; c++) m[c] = std::min(m[c], box_[3 * t + c]);
+    return Coordinate(m[0], m[1], m[2]);
+}
+
+Coordinate System::size(int t) const { return Coordinate(box_[3 * t], box_[3 * t + 1], box_[3 * t + 2]); }
+
+std::vector<int> System::timegap_weighting(bool fullblock) const
+{
+    std::vector<int> w(n_timegaps_);
+    check(amdat_ts_weighting(n_exponentials_, n_exponential_steps_, frt_, fullblock, displacement_limit_, w.data(),
+                             n_timegaps_),
+          "timegap weighting");
+    return w;
+}
+
+std::vector<float> System::displacement_times() const
+{
+    std::vector<float> out(n_timegaps_);
+    check(amdat_ts_displacement_times(n_exponentials_, n_exponential_steps_, frt_, timelist_.data(), out.data(),
+                                      n_timegaps_),
+          "displacement times");
+    return out;
+}
+
+amdat_pairs *System::displacement_pairs(bool fullblock, std::vector<int> *tg_out, std::vector<int> *t0_out) const
+{
+    int n = 0;
+    check(amdat_ts_pairs(n_exponentials_, n_exponential_steps_, frt_, fullblock, displacement_limit_, nullptr, nullptr,
+                         nullptr, 0, &n),
+          "pairs");
+    std::vector<int> tg(n), t0(n), t1(n);
+    check(amdat_ts_pairs(n_exponentials_, n_exponential_steps_, frt_, fullblock, displacement_limit_, tg.data(),
+                         t0.data(), t1.data(), n, &n),
+          "pairs");
+    amdat_pairs *p = nullptr;
+    check(amdat_pairs_upload(device_context(), n, tg.data(), t0.data(), t1.data(), n_timegaps_, &p), "pair upload");
+    if (tg_out) *tg_out = tg;
+    if (t0_out) *t0_out = t0;
+    return p;
+}
+
+std::vector<int> System::select(const std::vector<std::string> &args) const
+{
+    // create_list keywords (Manual/create_list.md): all, type_system <t>,
+    // species <name>, type_species <name> <t>; several keywords on one
+    // line select the union.
+    std::vector<char> keep(n_atoms_, 0);
+    size_t i = 0;
+    auto species_index = [&](const std::string &nm) {
+        for (size_t s = 0; s < species_names_.size(); s++)
+            if (species_names_[s] == nm) return (int)s;
+        throw Error("species " + nm + " not found", 1);
+    };
+    while (i < args.size()) {
+        const std::string &k = args[i];
+        if (k == "all") {
+            std::fill(keep.begin(), keep.end(), 1);
+            i += 1;
+        } else if (k == "type_system" && i + 1 < args.size()) {
+            const int t = std::atoi(args[i + 1].c_str());
+            for (int a = 0; a < n_atoms_; a++) if (type_[a] == t) keep[a] = 1;
+            i += 2;
+        } else if (k == "species" && i + 1 < args.size()) {
+            const int s = species_index(args[i + 1]);
+            for (int a = 0; a < n_atoms_; a++) if (atom_species_[a] == s) keep[a] = 1;
+            i += 2;
+        } else if (k == "type_species" && i + 2 < args.size()) {
+            const int s = species_index(args[i + 1]);
+            const int t = std::atoi(args[i + 2].c_str());
+            for (int a = 0; a < n_atoms_; a++) if (atom_species_[a] == s && type_[a] == t) keep[a] = 1;
+            i += 3;
+        } else {
+            throw Error("create_list keyword " + k + " is not supported by this build", 1);
+        }
+    }
+    std::vector<int> out;
+    for (int a : list_order_) if (keep[a]) out.push_back(a);
+    return out;
+}
+
+}  // namespace amdat_hip

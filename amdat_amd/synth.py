@@ -98,6 +98,24 @@ def chain_melt(n_chains=400, beads=160, L=42.2, T=1001, sigma=0.02, seed=MASTER_
     return Traj(xs, ys, zs, ixs, iys, izs, box)
 
 
+def write_lammps_custom(tr, path, types=None, box_lo=0.0):
+    """Write the trajectory as a LAMMPS custom dump with columns
+    ``id type x y z ix iy iz``; coordinates in %.9g so FP32 values
+    round-trip exactly."""
+    types = tr.types if types is None else types
+    if types is None:
+        types = np.ones(tr.N, np.int32)
+    with open(path, "w") as fh:
+        for t in range(tr.T):
+            fh.write("ITEM: TIMESTEP\n%d\nITEM: NUMBER OF ATOMS\n%d\n" % (t, tr.N))
+            fh.write("ITEM: BOX BOUNDS pp pp pp\n")
+            for c in range(3):
+                fh.write("%.9g %.9g\n" % (box_lo, box_lo + float(tr.box[t, c])))
+            fh.write("ITEM: ATOMS id type x y z ix iy iz\n")
+            rows = zip(range(1, tr.N + 1), types, tr.x[t], tr.y[t], tr.z[t], tr.ix[t], tr.iy[t], tr.iz[t])
+            fh.write("".join("%d %d %.9g %.9g %.9g %d %d %d\n" % r for r in rows))
+
+
 def fcc_lattice(cells, L, jitter=0.0, n_keep=None, seed=MASTER_SEED + 3):
     """Single frame of an fcc lattice of cells^3 unit cells (4 sites each)
     in a cubic box of edge L, optional Gaussian jitter and a seeded
