@@ -8,13 +8,28 @@
 // plain FP32 dot product k.r carries an error of |k.r| * 2^-24 radians.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define AMDAT_HD __host__ __device__
+#else
+// Host-only build (the CPU self-test of these helpers): plain C++.
+#include <cmath>
+#define AMDAT_HD
+using std::floor;
+using std::fmaf;
+using std::rint;
+struct float2 {
+    float x, y;
+};
+inline float2 make_float2(float a, float b) { return float2{a, b}; }
+#endif
 
 namespace amdat {
 
 // Fixed-point fraction of u turns, in units of 2^-32 turn (u in FP64).
-__device__ __forceinline__ uint32_t turns_fixed(double u)
+AMDAT_HD inline uint32_t turns_fixed(double u)
 {
     double f = u - floor(u);                 // [0, 1]
     double s = rint(f * 4294967296.0);       // [0, 2^32]
@@ -38,7 +53,7 @@ constexpr float kTurnLo = -4.070940268e-17f;
 // residual r in [-2^29, 2^29) is split into a multiple of 4096 and a
 // remainder so both convert to float exactly, and the angle is formed
 // with the split constant.
-__device__ __forceinline__ float2 sincos_turns(uint32_t P)
+AMDAT_HD inline float2 sincos_turns(uint32_t P)
 {
     uint32_t q = (P + 0x20000000u) >> 30;
     int32_t r = (int32_t)(P - (q << 30));
@@ -60,7 +75,7 @@ __device__ __forceinline__ float2 sincos_turns(uint32_t P)
 
 // Error-free transformation a + b = s + e (Knuth TwoSum); the error is
 // folded into a running FP32 compensation term.
-__device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float x)
+AMDAT_HD inline void two_sum_acc(float &hi, float &lo, float x)
 {
     float s = hi + x;
     float bp = s - hi;

@@ -84,7 +84,18 @@ def main():
     ap.add_argument("--small", action="store_true", help="reduced workload for a quick check")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 256 MiB device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the path is built and measured on a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)

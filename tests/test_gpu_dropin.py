@@ -104,7 +104,7 @@ def test_isfs_file(run, qvec):
     lines, rows = read_rows(d / "isfs.dat", 2)
     assert lines[0].startswith("Correlation data created by AMDAT v.")
     k = [float(v) for v in lines[1].strip("\t").split("\t")]
-    assert np.allclose(k, orc.approx_wavenumbers(np.float32(3.5), 99)[5:10], rtol=1e-6)
+    assert np.allclose(k, orc.approx_wavenumbers(np.float32(3.5), 99)[5:10], rtol=RTOL)
     assert np.allclose(rows[:, 1:], Fs, rtol=RTOL, atol=1e-6)
 
 
@@ -130,7 +130,7 @@ def test_structure_factor_files(run, qvec):
     assert lines[1] == "Nominal_q\tMean_q\tStddev_q\tMean_qx\tMean_qy\tMean_qz\tS(q)"
     assert rows.shape == (99, 7)
     assert np.allclose(rows[:, 6], S, rtol=RTOL)
-    assert np.allclose(rows[:, 0], orc.approx_wavenumbers(np.float32(7.0), 99), rtol=1e-6)
+    assert np.allclose(rows[:, 0], orc.approx_wavenumbers(np.float32(7.0), 99), rtol=RTOL)
     kmag = np.array([np.mean(np.linalg.norm(n[off[s]:off[s + 1]] * 2 * np.pi / L, axis=1)) for s in range(99)])
     assert np.allclose(rows[:, 1], kmag, rtol=1e-5)
     # asymmetric: Re(rho_big conj rho_all) / N_big
