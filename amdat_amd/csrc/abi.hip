@@ -132,17 +132,18 @@ hipStream_t stream_of(amdat_ctx *c) { return c->stream ? c->stream : c->own; }
 
 // Vectors per thread and the resulting chunking for K vectors with max
 // component M: minimise padded work plus table builds (each chunk rebuilds
-// 3(M+1) table entries per atom, about 25 VALU ops each, against 8 ops per
-// padded (atom, vector) update).
+// 3(M+1) table entries per atom, about 10 VALU ops each with anchor
+// products, against 8 ops per padded (atom, vector) update).
 int choose_R(int K, int M)
 {
-    static const int Rs[] = {4, 8, 16};
-    int best = 4;
+    static const int Rs[] = {2, 4, 8};
+    int best = 2;
     double best_cost = 1e300;
     for (int R : Rs) {
-        const int KB = kBlock * R;
+        const int KB = kPhaseBlock * R;
         const int nch = (K + KB - 1) / KB;
-        const double cost = (double)nch * ((double)KB * 8.0 + 75.0 * (M + 1));
+        // per atom: KB padded updates at 8 ops, 3(M+1) table entries at ~10 ops
+        const double cost = (double)nch * ((double)KB * 8.0 + 30.0 * (M + 1));
         if (cost < best_cost * 0.98) { best_cost = cost; best = R; }
     }
     return best;
@@ -166,7 +167,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
     KPlan &kp = P->kp;
     kp.K = K;
     kp.R = R;
-    kp.KB = kBlock * R;
+    kp.KB = kPhaseBlock * R;
     kp.n_chunks = (K + kp.KB - 1) / kp.KB;
     kp.MM = plan_table_width(M);
     const size_t Kp = (size_t)kp.n_chunks * kp.KB;

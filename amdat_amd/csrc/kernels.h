@@ -6,7 +6,8 @@
 
 namespace amdat {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;       // reduction and MSD kernels
+constexpr int kPhaseBlock = 512;  // rho / Fs kernels: 8 waves share one atom tile
 
 // Device view of a trajectory: frame-major SoA, x[t*N + j].
 struct TrajDev {
@@ -17,9 +18,10 @@ struct TrajDev {
 };
 
 // Device k-vector plan for one shell range.  Vectors are cut into chunks
-// of KB = kBlock*R in their original order, each chunk sorted by
+// of KB = kPhaseBlock*R in their original order, each chunk sorted by
 // (nx, ny, nz) so a wave's lanes read nearby table entries, and padded
-// with (0,0,0) entries whose destination is -1.
+// with (0,0,0) entries whose destination is -1.  Thread t of a block owns
+// positions r*kPhaseBlock + t, r < R, of its chunk.
 struct KPlan {
     int K = 0;         // real vectors
     int R = 0;         // vectors per thread

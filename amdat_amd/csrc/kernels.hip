@@ -3,21 +3,26 @@
 // per-vector sums into per-shell S(k), F(k,t) and Fs(k,t).
 //
 // Phase kernels (rho, Fs), compensated-FP32 form:
-//   * a block owns one chunk of KB = 256*R k-vectors (R per thread, kept in
-//     registers as LDS offsets) and one frame (rho) or time-origin pair (Fs),
-//     and streams the atoms of its split through LDS in tiles of A atoms;
+//   * a block of 512 threads owns one chunk of KB = 512*R k-vectors (R per
+//     thread, kept in registers as LDS addresses) and one frame (rho) or
+//     time-origin pair (Fs), and streams the atoms of its split through LDS
+//     in tiles of A atoms;
 //   * per tile, every atom gets separable phase tables
 //         e_c(m) = exp(2*pi*i * m * u_c),  u_c = frac(r_c / l),
 //     for m in [-M, M] on x and y and [0, M] on z (vectors are stored with
 //     nz >= 0; a vector with nz < 0 is flipped and its result conjugated).
 //     u_c is an exact 32-bit fixed-point fraction formed in FP64 from the
 //     FP32 coordinate (for Fs from the FP64 displacement rebuilt from image
-//     counts), so m*u_c mod 1 is exact integer arithmetic and each table
-//     entry is one FP32 sincos_turns();
-//   * the hot loop is per (atom, vector): three LDS reads and two complex
-//     products, exp(i k.r) = e_x(nx) e_y(ny) e_z(nz), accumulated with FMA
-//     into an FP32 partial per tile; tile partials are folded into a
-//     TwoSum-compensated FP32 (hi, lo) pair; the block writes hi+lo as FP64.
+//     counts), so m*u_c mod 1 is exact integer arithmetic.  Entries are
+//     e(8j + d) = e(8j) * e(d) with both factors from sincos_turns(), so
+//     every entry carries one FP32 complex product over exact anchors;
+//   * the hot loop is per (atom, vector): three LDS reads, all with a
+//     per-lane base and a compile-time offset (the tile loop is fully
+//     unrolled), and two complex products,
+//         exp(i k.r) = e_x(nx) e_y(ny) e_z(nz),
+//     accumulated with FMA into an FP32 partial per tile; tile partials are
+//     folded into a TwoSum-compensated FP32 (hi, lo) pair; the block writes
+//     hi + lo as FP64.
 // The plain-FP64 form evaluates exp(i k.r) with FP64 sincos per update and
 // accumulates in FP64; it is the reference the FP32 form is measured against.
 #include <algorithm>
@@ -29,19 +34,24 @@ namespace amdat {
 
 namespace {
 
+constexpr int kWaves = kPhaseBlock / 64;
+
 // Table geometry for half-width MM: row of W float2 per atom,
 // x at [0, 2MM], y at [2MM+1, 4MM+1], z (m >= 0 only) at [4MM+2, 5MM+2].
-// A atoms per tile keeps the tile under 64 KiB so every (atom, vector)
-// read is one ds_read_b64 with a per-lane base and an immediate offset.
+// A atoms per tile: (A-1)*W*8 stays below 64 KiB so every read is one
+// ds_read_b64 with an immediate offset.  NA anchor slots per table row:
+// e(0..7) and e(8j), j = 0..MM/8.
 template <int MM>
 struct Geom {
-    static constexpr int W = 5 * MM + 3;
+    // at least 257 float2 per row: reads of neighbouring atoms then never
+    // pair into one ds_read2 (8-bit offsets), which keeps register use low
+    static constexpr int W = (5 * MM + 3) > 257 ? (5 * MM + 3) : 257;
     static constexpr int OX = MM;
     static constexpr int OY = 2 * MM + 1 + MM;
     static constexpr int OZ = 2 * (2 * MM + 1);
-    static constexpr int A_raw = 65536 / (W * 8);
-    static constexpr int A = A_raw >= 32 ? 32 : (A_raw >= 16 ? 16 : 8);
-    static constexpr int G = 8;  // atoms per unrolled group (immediate offsets j*W*8)
+    static constexpr int A = (15 * W * 8 < 65536) ? 16 : 8;
+    static constexpr int NA = 8 + MM / 8 + 1;
+    static constexpr size_t tab_bytes = (size_t)A * W * sizeof(float2);
 };
 
 __device__ __forceinline__ void unpack_k(uint32_t kp, int &nx, int &ny, int &nz)
@@ -51,23 +61,40 @@ __device__ __forceinline__ void unpack_k(uint32_t kp, int &nx, int &ny, int &nz)
     nz = (int)((kp >> 16) & 0xFFu);
 }
 
-// Fill one tile of tables.  sU holds the 3*A fixed-point fractions and
-// sValid the per-atom flags; wave w builds rows w, w+4, ...; lane handles
-// m = lane, lane+64, ... <= Mc.
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+
+// Build one tile of tables from the fixed-point fractions sU[3A] (row
+// 3i+c = atom i, axis c; invalid atoms carry sValid[i] == 0 and get zero
+// rows).  Two phases: exact anchors into anc[3A][NA], then products.
 template <int MM>
-__device__ __forceinline__ void build_tables(float2 *tab, const uint32_t *sU, const int *sValid, int Mc)
+__device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uint32_t *sU, const int *sValid, int Mc)
 {
     using Gm = Geom<MM>;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    for (int row = wave; row < 3 * Gm::A; row += kBlock / 64) {
+    const int tid = threadIdx.x;
+    const int nj = (Mc >> 3) + 1;           // anchors e(8j), j < nj
+    const int nslot = 8 + nj;
+    for (int e = tid; e < 3 * Gm::A * Gm::NA; e += kPhaseBlock) {
+        const int row = e / Gm::NA;
+        const int s = e - row * Gm::NA;
+        if (s < nslot) {
+            const uint32_t m = s < 8 ? (uint32_t)s : 8u * (uint32_t)(s - 8);
+            anc[e] = sincos_turns(sU[row] * m);
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    for (int row = wave; row < 3 * Gm::A; row += kWaves) {
         const int i = row / 3;
         const int c = row - 3 * i;
-        const uint32_t U = sU[row];
         const bool valid = sValid[i] != 0;
+        const float2 *a = anc + row * Gm::NA;
         float2 *dst = tab + i * Gm::W;
         for (int m = lane; m <= Mc; m += 64) {
-            float2 v = valid ? sincos_turns(U * (uint32_t)m) : make_float2(0.0f, 0.0f);
+            float2 v = valid ? cmul(a[8 + (m >> 3)], a[m & 7]) : make_float2(0.0f, 0.0f);
             if (c == 0) {
                 dst[Gm::OX + m] = v;
                 if (m) dst[Gm::OX - m] = make_float2(v.x, -v.y);
@@ -83,7 +110,7 @@ __device__ __forceinline__ void build_tables(float2 *tab, const uint32_t *sU, co
 
 // rho_k(t), compensated FP32.  grid = (chunks, frames, splits).
 template <int R, int MM>
-__global__ __launch_bounds__(kBlock) void rho_c32_kernel(
+__global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
     const int *__restrict__ list, int n_list, const int *__restrict__ frames,
     const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, const int *__restrict__ chunkM,
@@ -91,6 +118,7 @@ __global__ __launch_bounds__(kBlock) void rho_c32_kernel(
 {
     using Gm = Geom<MM>;
     extern __shared__ float2 tab[];
+    __shared__ float2 anc[3 * Gm::A * Gm::NA];
     __shared__ uint32_t sU[3 * Gm::A];
     __shared__ int sValid[Gm::A];
 
@@ -106,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void rho_c32_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int nx, ny, nz;
-        unpack_k(kpack[(size_t)chunk * KB + r * kBlock + tid], nx, ny, nz);
+        unpack_k(kpack[(size_t)chunk * KB + r * kPhaseBlock + tid], nx, ny, nz);
         px[r] = tab + Gm::OX + nx;
         py[r] = tab + Gm::OY + ny;
         pz[r] = tab + Gm::OZ + nz;
@@ -115,8 +143,8 @@ __global__ __launch_bounds__(kBlock) void rho_c32_kernel(
 
     for (int a0 = a_begin; a0 < a_end; a0 += Gm::A) {
         __syncthreads();
-        for (int e = tid; e < 3 * Gm::A; e += kBlock) {
-            const int i = e / 3, c = e - 3 * (e / 3);
+        if (tid < 3 * Gm::A) {
+            const int i = tid / 3, c = tid - 3 * (tid / 3);
             const int a = a0 + i;
             uint32_t U = 0;
             if (a < a_end) {
@@ -124,30 +152,27 @@ __global__ __launch_bounds__(kBlock) void rho_c32_kernel(
                 const float *src = c == 0 ? X : (c == 1 ? Y : Z);
                 U = turns_fixed((double)src[foff + j] * inv_ell);
             }
-            sU[e] = U;
+            sU[tid] = U;
             if (c == 0) sValid[i] = a < a_end;
         }
         __syncthreads();
-        build_tables<MM>(tab, sU, sValid, Mc);
+        build_tables<MM>(tab, anc, sU, sValid, Mc);
         __syncthreads();
 
         float pr[R], pi[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) pr[r] = pi[r] = 0.0f;
-        for (int g = 0; g < Gm::A; g += Gm::G) {
-            const int base = g * Gm::W;
 #pragma unroll
-            for (int j = 0; j < Gm::G; ++j) {
+        for (int i = 0; i < Gm::A; ++i) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const float2 ex = px[r][base + j * Gm::W];
-                    const float2 ey = py[r][base + j * Gm::W];
-                    const float2 ez = pz[r][base + j * Gm::W];
-                    const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
-                    const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
-                    pr[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, pr[r]));
-                    pi[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, pi[r]));
-                }
+            for (int r = 0; r < R; ++r) {
+                const float2 ex = px[r][i * Gm::W];
+                const float2 ey = py[r][i * Gm::W];
+                const float2 ez = pz[r][i * Gm::W];
+                const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+                const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                pr[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, pr[r]));
+                pi[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, pi[r]));
             }
         }
 #pragma unroll
@@ -160,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void rho_c32_kernel(
     double2 *dst = out + ((size_t)fslot * n_splits + split) * (size_t)K;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int d = kdst[(size_t)chunk * KB + r * kBlock + tid];
+        const int d = kdst[(size_t)chunk * KB + r * kPhaseBlock + tid];
         if (d >= 0) {
             const double re = (double)hr[r] + (double)lr[r];
             double im = (double)hi[r] + (double)li[r];
@@ -173,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void rho_c32_kernel(
 // Fs per (pair, vector), compensated FP32: sum over the split's atoms of
 // cos(k.dr).  grid = (chunks, pairs, splits).
 template <int R, int MM>
-__global__ __launch_bounds__(kBlock) void fs_c32_kernel(
+__global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
     const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
@@ -183,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void fs_c32_kernel(
 {
     using Gm = Geom<MM>;
     extern __shared__ float2 tab[];
+    __shared__ float2 anc[3 * Gm::A * Gm::NA];
     __shared__ uint32_t sU[3 * Gm::A];
     __shared__ int sValid[Gm::A];
 
@@ -199,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void fs_c32_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int nx, ny, nz;
-        unpack_k(kpack[(size_t)chunk * KB + r * kBlock + tid], nx, ny, nz);
+        unpack_k(kpack[(size_t)chunk * KB + r * kPhaseBlock + tid], nx, ny, nz);
         px[r] = tab + Gm::OX + nx;
         py[r] = tab + Gm::OY + ny;
         pz[r] = tab + Gm::OZ + nz;
@@ -208,8 +234,8 @@ __global__ __launch_bounds__(kBlock) void fs_c32_kernel(
 
     for (int a0 = a_begin; a0 < a_end; a0 += Gm::A) {
         __syncthreads();
-        for (int e = tid; e < 3 * Gm::A; e += kBlock) {
-            const int i = e / 3, c = e - 3 * (e / 3);
+        if (tid < 3 * Gm::A) {
+            const int i = tid / 3, c = tid - 3 * (tid / 3);
             const int a = a0 + i;
             uint32_t U = 0;
             if (a < a_end) {
@@ -222,29 +248,26 @@ __global__ __launch_bounds__(kBlock) void fs_c32_kernel(
                                   (double)img[o0 + j] * (double)box[3 * t0 + c]);
                 U = turns_fixed(d * inv_ell);
             }
-            sU[e] = U;
+            sU[tid] = U;
             if (c == 0) sValid[i] = a < a_end;
         }
         __syncthreads();
-        build_tables<MM>(tab, sU, sValid, Mc);
+        build_tables<MM>(tab, anc, sU, sValid, Mc);
         __syncthreads();
 
         float ps[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) ps[r] = 0.0f;
-        for (int g = 0; g < Gm::A; g += Gm::G) {
-            const int base = g * Gm::W;
 #pragma unroll
-            for (int j = 0; j < Gm::G; ++j) {
+        for (int i = 0; i < Gm::A; ++i) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const float2 ex = px[r][base + j * Gm::W];
-                    const float2 ey = py[r][base + j * Gm::W];
-                    const float2 ez = pz[r][base + j * Gm::W];
-                    const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
-                    const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
-                    ps[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, ps[r]));
-                }
+            for (int r = 0; r < R; ++r) {
+                const float2 ex = px[r][i * Gm::W];
+                const float2 ey = py[r][i * Gm::W];
+                const float2 ez = pz[r][i * Gm::W];
+                const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+                const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                ps[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, ps[r]));
             }
         }
 #pragma unroll
@@ -254,14 +277,14 @@ __global__ __launch_bounds__(kBlock) void fs_c32_kernel(
     double *dst = out + ((size_t)pslot * n_splits + split) * (size_t)K;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int d = kdst[(size_t)chunk * KB + r * kBlock + tid];
+        const int d = kdst[(size_t)chunk * KB + r * kPhaseBlock + tid];
         if (d >= 0) dst[d & ~(1 << 30)] = (double)hs[r] + (double)ls[r];
     }
 }
 
 // Plain FP64 rho: FP64 phase, FP64 sincos, FP64 sums.
 template <int R>
-__global__ __launch_bounds__(kBlock) void rho_f64_kernel(
+__global__ __launch_bounds__(kPhaseBlock) void rho_f64_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
     const int *__restrict__ list, int n_list, const int *__restrict__ frames,
     const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, int K, int KB, double kscale,
@@ -276,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void rho_f64_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int nx, ny, nz;
-        unpack_k(kpack[(size_t)chunk * KB + r * kBlock + tid], nx, ny, nz);
+        unpack_k(kpack[(size_t)chunk * KB + r * kPhaseBlock + tid], nx, ny, nz);
         kx[r] = kscale * nx; ky[r] = kscale * ny; kz[r] = kscale * nz;
         ar[r] = ai[r] = 0.0;
     }
@@ -294,14 +317,14 @@ __global__ __launch_bounds__(kBlock) void rho_f64_kernel(
     double2 *dst = out + ((size_t)fslot * n_splits + split) * (size_t)K;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int d = kdst[(size_t)chunk * KB + r * kBlock + tid];
+        const int d = kdst[(size_t)chunk * KB + r * kPhaseBlock + tid];
         if (d >= 0) dst[d & ~(1 << 30)] = make_double2(ar[r], (d & (1 << 30)) ? -ai[r] : ai[r]);
     }
 }
 
 // Plain FP64 Fs.
 template <int R>
-__global__ __launch_bounds__(kBlock) void fs_f64_kernel(
+__global__ __launch_bounds__(kPhaseBlock) void fs_f64_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
     const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
@@ -321,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void fs_f64_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int nx, ny, nz;
-        unpack_k(kpack[(size_t)chunk * KB + r * kBlock + tid], nx, ny, nz);
+        unpack_k(kpack[(size_t)chunk * KB + r * kPhaseBlock + tid], nx, ny, nz);
         kx[r] = kscale * nx; ky[r] = kscale * ny; kz[r] = kscale * nz;
         as[r] = 0.0;
     }
@@ -336,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void fs_f64_kernel(
     double *dst = out + ((size_t)pslot * n_splits + split) * (size_t)K;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int d = kdst[(size_t)chunk * KB + r * kBlock + tid];
+        const int d = kdst[(size_t)chunk * KB + r * kPhaseBlock + tid];
         if (d >= 0) dst[d & ~(1 << 30)] = as[r];
     }
 }
@@ -480,23 +503,19 @@ __global__ __launch_bounds__(kBlock) void msd_reduce_kernel(const double *__rest
     }
 }
 
-template <int MM>
-size_t tab_bytes()
-{
-    return (size_t)Geom<MM>::A * Geom<MM>::W * sizeof(float2);
-}
-
 template <int R, int MM>
 hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, const int *frames, int n_frames,
                             const KPlan &kp, double inv_ell, int n_splits, double2 *out, hipStream_t st)
 {
     const int per_split = (n_list + n_splits - 1) / n_splits;
-    const size_t lds = tab_bytes<MM>();
+    const size_t lds = Geom<MM>::tab_bytes;
     auto kfn = rho_c32_kernel<R, MM>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     dim3 grid(kp.n_chunks, n_frames, n_splits);
-    hipLaunchKernelGGL(kfn, grid, dim3(kBlock), lds, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames,
+    (void)hipGetLastError();
+
+    hipLaunchKernelGGL(kfn, grid, dim3(kPhaseBlock), lds, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames,
                        kp.kpack, kp.kdst, kp.chunkM, kp.K, kp.KB, inv_ell, per_split, n_splits, out);
     return hipGetLastError();
 }
@@ -507,12 +526,14 @@ hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const
                            hipStream_t st)
 {
     const int per_split = (n_list + n_splits - 1) / n_splits;
-    const size_t lds = tab_bytes<MM>();
+    const size_t lds = Geom<MM>::tab_bytes;
     auto kfn = fs_c32_kernel<R, MM>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     dim3 grid(kp.n_chunks, n_pairs, n_splits);
-    hipLaunchKernelGGL(kfn, grid, dim3(kBlock), lds, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box,
+    (void)hipGetLastError();
+
+    hipLaunchKernelGGL(kfn, grid, dim3(kPhaseBlock), lds, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box,
                        tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.chunkM, kp.K, kp.KB, inv_ell,
                        per_split, n_splits, out);
     return hipGetLastError();
@@ -565,10 +586,11 @@ hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_l
         const int per_split = (n_list + n_splits - 1) / n_splits;
         const double kscale = 2.0 * M_PI * inv_ell;
         dim3 grid(kp.n_chunks, n_frames, n_splits);
+        (void)hipGetLastError();
         switch (kp.R) {
-        case 4: hipLaunchKernelGGL(rho_f64_kernel<4>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
-        case 8: hipLaunchKernelGGL(rho_f64_kernel<8>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
-        case 16: hipLaunchKernelGGL(rho_f64_kernel<16>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 4: hipLaunchKernelGGL(rho_f64_kernel<4>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 8: hipLaunchKernelGGL(rho_f64_kernel<8>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 2: hipLaunchKernelGGL(rho_f64_kernel<2>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
@@ -576,7 +598,7 @@ hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_l
     switch (kp.R) {
     case 4: return dispatch_rho_c32<4>(kp.MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
     case 8: return dispatch_rho_c32<8>(kp.MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
-    case 16: return dispatch_rho_c32<16>(kp.MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 2: return dispatch_rho_c32<2>(kp.MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -589,10 +611,11 @@ hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_li
         const int per_split = (n_list + n_splits - 1) / n_splits;
         const double kscale = 2.0 * M_PI * inv_ell;
         dim3 grid(kp.n_chunks, n_pairs, n_splits);
+        (void)hipGetLastError();
         switch (kp.R) {
-        case 4: hipLaunchKernelGGL(fs_f64_kernel<4>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
-        case 8: hipLaunchKernelGGL(fs_f64_kernel<8>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
-        case 16: hipLaunchKernelGGL(fs_f64_kernel<16>, grid, dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 4: hipLaunchKernelGGL(fs_f64_kernel<4>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 8: hipLaunchKernelGGL(fs_f64_kernel<8>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 2: hipLaunchKernelGGL(fs_f64_kernel<2>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
@@ -600,7 +623,7 @@ hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_li
     switch (kp.R) {
     case 4: return dispatch_fs_c32<4>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     case 8: return dispatch_fs_c32<8>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 16: return dispatch_fs_c32<16>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 2: return dispatch_fs_c32<2>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -610,6 +633,8 @@ hipError_t launch_split_reduce2(const double2 *in, int n_rows, int n_splits, int
     const size_t n = (size_t)n_rows * K;
     int blocks = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 65535);
     if (blocks < 1) blocks = 1;
+    (void)hipGetLastError();
+
     hipLaunchKernelGGL(split_reduce2_kernel, dim3(blocks), dim3(kBlock), 0, st, in, n_rows, n_splits, K, out);
     return hipGetLastError();
 }
@@ -618,6 +643,8 @@ hipError_t launch_corr_shell(const double2 *rho1, const double2 *rho2, int K, co
                              const int *row0, const int *row1, int n_groups, const int *shell_off,
                              int n_shells, const double *norm, double *out, hipStream_t st)
 {
+    (void)hipGetLastError();
+
     hipLaunchKernelGGL(corr_shell_kernel, dim3(n_shells, n_groups), dim3(kBlock), 0, st, rho1, rho2, K,
                        group_off, row0, row1, shell_off, n_shells, norm, out);
     return hipGetLastError();
@@ -627,6 +654,8 @@ hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *grou
                            int n_groups, const int *shell_off, int n_shells, const double *norm,
                            double *out, hipStream_t st)
 {
+    (void)hipGetLastError();
+
     hipLaunchKernelGGL(fs_shell_kernel, dim3(n_shells, n_groups), dim3(kBlock), 0, st, V, n_splits, K,
                        group_off, order, shell_off, n_shells, norm, out);
     return hipGetLastError();
@@ -635,7 +664,9 @@ hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *grou
 hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
                       int n_pairs, int n_blocks, double *partial, hipStream_t st)
 {
-    hipLaunchKernelGGL(msd_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
+    (void)hipGetLastError();
+
+    hipLaunchKernelGGL(msd_kernel, dim3(n_blocks, n_pairs), dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
                        tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, partial);
     return hipGetLastError();
 }
@@ -643,6 +674,8 @@ hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int 
 hipError_t launch_msd_reduce(const double *partial, int n_blocks, const int *group_off, const int *order,
                              int n_groups, double *s2, double *s4, hipStream_t st)
 {
+    (void)hipGetLastError();
+
     hipLaunchKernelGGL(msd_reduce_kernel, dim3(n_groups), dim3(kBlock), 0, st, partial, n_blocks, group_off,
                        order, s2, s4);
     return hipGetLastError();
