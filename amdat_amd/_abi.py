@@ -61,6 +61,7 @@ SIGNATURES = {
     "amdat_fkt": (_I, [_P, _P, _P, _P, _I, _I, _PD, _PD]),
     "amdat_fs": (_I, [_P, _P, _P, _P, _I, _I, _F, _P, _I, _PD, _PD]),
     "amdat_msd_ngp": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
+    "amdat_sk_loss_grad": (_I, [_P, _I, _P, _P, _P, _P, _I, _I, _F, _PD, _PD, _PD, _PD, _P, _P, _P]),
 }
 
 
@@ -333,3 +334,20 @@ def msd_ngp(ctx, traj, alist, pairs):
     check(lib().amdat_msd_ngp(ctx.h, traj.h, alist.h if alist is not None else None, pairs.h, dp(s2), dp(s4),
                               dp(cnt)))
     return s2, s4, cnt
+
+
+def sk_loss_grad(ctx, n_atoms, x_ptr, y_ptr, z_ptr, shells, first, last, length_scale, weights, target,
+                 grad_ptrs=None):
+    """S(k)-misfit loss on one frame held in device memory (pointers to FP32
+    [n_atoms] arrays).  grad_ptrs: three device FP64 [n_atoms] outputs, or
+    None for the loss only.  Returns (loss, S per shell)."""
+    nsh = last - first + 1
+    w = np.ascontiguousarray(weights, np.float64)
+    t = np.ascontiguousarray(target, np.float64)
+    assert len(w) == nsh and len(t) == nsh
+    S = np.zeros(nsh)
+    loss = C.c_double()
+    g = [C.c_void_p(p) for p in grad_ptrs] if grad_ptrs else [None, None, None]
+    check(lib().amdat_sk_loss_grad(ctx.h, n_atoms, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(z_ptr),
+                                   shells.h, first, last, length_scale, dp(w), dp(t), C.byref(loss), dp(S), *g))
+    return loss.value, S

@@ -659,4 +659,71 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
     return AMDAT_OK;
 }
 
+int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const float *d_y, const float *d_z,
+                       const amdat_shells *sh_c, int first_shell, int last_shell, float length_scale,
+                       const double *weights, const double *target, double *loss, double *S, double *d_gx,
+                       double *d_gy, double *d_gz)
+{
+    clear_error();
+    if (!ctx || !sh_c || !d_x || !d_y || !d_z || !weights || !target || !loss || n_atoms < 1)
+        return set_error(AMDAT_EINVAL, "amdat_sk_loss_grad: bad arguments");
+    if ((d_gx || d_gy || d_gz) && !(d_gx && d_gy && d_gz))
+        return set_error(AMDAT_EINVAL, "amdat_sk_loss_grad: give all three gradient outputs or none");
+    amdat_shells *sh = const_cast<amdat_shells *>(sh_c);
+    if (first_shell < 0 || last_shell < first_shell || last_shell >= sh->n_shells)
+        return set_error(AMDAT_ERANGE, "amdat_sk_loss_grad: shell range outside the table");
+    if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_sk_loss_grad: length scale must be > 0");
+    HIP_TRY(hipSetDevice(ctx->device));
+    KPlanOwned *plan = nullptr;
+    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan);
+    if (rc) return rc;
+    const KPlan &kp = plan->kp;
+    const int K = kp.K, nsh = last_shell - first_shell + 1, v0 = sh->off[first_shell];
+    hipStream_t st = stream_of(ctx);
+    TrajDev view;
+    view.N = n_atoms;
+    view.T = 1;
+    view.x = d_x; view.y = d_y; view.z = d_z;
+    const double inv_ell = 1.0 / (double)length_scale;
+    // forward: rho for the frame (compensated FP32 kernel, FP64 result)
+    const int zero = 0;
+    DevBuf<int> dfr;
+    HIP_TRY(dfr.upload(&zero, 1, st));
+    const int n_splits = choose_splits((long)kp.n_chunks, n_atoms);
+    DevBuf<double2> part, rho;
+    HIP_TRY(rho.alloc(K));
+    if (n_splits > 1) HIP_TRY(part.alloc((size_t)n_splits * K));
+    HIP_TRY(launch_rho(AMDAT_PREC_FP32C, view, nullptr, n_atoms, dfr.p, 1, kp, inv_ell, n_splits,
+                       n_splits > 1 ? part.p : rho.p, st));
+    if (n_splits > 1) HIP_TRY(launch_split_reduce2(part.p, 1, n_splits, K, rho.p, st));
+    std::vector<double2> h(K);
+    HIP_TRY(hipMemcpyAsync(h.data(), rho.p, sizeof(double2) * K, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // S per shell, loss, and the adjoint weights q_v = c_v rho_v (FP64 on the host: K values)
+    std::vector<float2> q(K);
+    double L = 0.0;
+    for (int s = 0; s < nsh; s++) {
+        const int a = sh->off[first_shell + s] - v0, b = sh->off[first_shell + s + 1] - v0;
+        const int H = b - a;
+        double acc = 0.0;
+        for (int v = a; v < b; v++) acc += h[v].x * h[v].x + h[v].y * h[v].y;
+        const double Ss = H ? acc / ((double)H * n_atoms) : 0.0;
+        if (S) S[s] = Ss;
+        const double d = Ss - target[s];
+        L += weights[s] * d * d;
+        const double c = H ? -4.0 * weights[s] * d / ((double)H * n_atoms) : 0.0;
+        for (int v = a; v < b; v++) q[v] = make_float2((float)(c * h[v].x), (float)(c * h[v].y));
+    }
+    *loss = L;
+    if (!d_gx) return AMDAT_OK;
+    // backward: the adjoint kernel recomputes every phase
+    DevBuf<float2> dq;
+    DevBuf<int> dn;
+    HIP_TRY(dq.upload(q.data(), q.size(), st));
+    HIP_TRY(dn.upload(sh->nvec.data() + 3 * (size_t)v0, 3 * (size_t)K, st));
+    HIP_TRY(launch_sk_adjoint(d_x, d_y, d_z, n_atoms, inv_ell, dn.p, dq.p, K, d_gx, d_gy, d_gz, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
 }  // extern "C"

@@ -65,4 +65,8 @@ hipError_t launch_msd_reduce(const double *partial, int n_blocks, const int *gro
                              const int *order, int n_groups, double *s2, double *s4,
                              hipStream_t st);
 
+// Adjoint of the S(k) misfit: g_j = (2 pi/l) sum_v n_v Im(conj(q_v) e^{ik_v.r_j}).
+hipError_t launch_sk_adjoint(const float *x, const float *y, const float *z, int N, double inv_ell, const int *nvec,
+                             const float2 *q, int K, double *gx, double *gy, double *gz, hipStream_t st);
+
 }  // namespace amdat

@@ -503,6 +503,39 @@ __global__ __launch_bounds__(kBlock) void msd_reduce_kernel(const double *__rest
     }
 }
 
+// Adjoint of the S(k) misfit on one frame: for each atom j,
+//   g_j = (2 pi / l) sum_v n_v Im(conj(q_v) e^{i k_v.r_j}),  q_v = c_v rho_v,
+// with the phase recomputed per (atom, vector) from the exact fixed-point
+// fractions (no N x K table is stored).  Lanes are atoms, vectors are
+// wave-uniform; sums are FP64.
+__global__ __launch_bounds__(kBlock) void sk_adjoint_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N, double inv_ell,
+    const int *__restrict__ nvec, const float2 *__restrict__ q, int K, double kscale, double *__restrict__ gx,
+    double *__restrict__ gy, double *__restrict__ gz)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    const int jj = j < N ? j : N - 1;
+    const uint32_t ux = turns_fixed((double)X[jj] * inv_ell);
+    const uint32_t uy = turns_fixed((double)Y[jj] * inv_ell);
+    const uint32_t uz = turns_fixed((double)Z[jj] * inv_ell);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int v = 0; v < K; ++v) {
+        const int nx = nvec[3 * v], ny = nvec[3 * v + 1], nz = nvec[3 * v + 2];
+        const uint32_t P = (uint32_t)nx * ux + (uint32_t)ny * uy + (uint32_t)nz * uz;
+        const float2 e = sincos_turns(P);
+        const float2 qq = q[v];
+        const double t = (double)fmaf(qq.x, e.y, -qq.y * e.x);
+        ax = fma((double)nx, t, ax);
+        ay = fma((double)ny, t, ay);
+        az = fma((double)nz, t, az);
+    }
+    if (j < N) {
+        gx[j] = ax * kscale;
+        gy[j] = ay * kscale;
+        gz[j] = az * kscale;
+    }
+}
+
 template <int R, int MM>
 hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, const int *frames, int n_frames,
                             const KPlan &kp, double inv_ell, int n_splits, double2 *out, hipStream_t st)
@@ -665,8 +698,7 @@ hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int 
                       int n_pairs, int n_blocks, double *partial, hipStream_t st)
 {
     (void)hipGetLastError();
-
-    hipLaunchKernelGGL(msd_kernel, dim3(n_blocks, n_pairs), dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
+    hipLaunchKernelGGL(msd_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
                        tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, partial);
     return hipGetLastError();
 }
@@ -678,6 +710,16 @@ hipError_t launch_msd_reduce(const double *partial, int n_blocks, const int *gro
 
     hipLaunchKernelGGL(msd_reduce_kernel, dim3(n_groups), dim3(kBlock), 0, st, partial, n_blocks, group_off,
                        order, s2, s4);
+    return hipGetLastError();
+}
+
+hipError_t launch_sk_adjoint(const float *x, const float *y, const float *z, int N, double inv_ell, const int *nvec,
+                             const float2 *q, int K, double *gx, double *gy, double *gz, hipStream_t st)
+{
+    const int blocks = (N + kBlock - 1) / kBlock;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(sk_adjoint_kernel, dim3(blocks), dim3(kBlock), 0, st, x, y, z, N, inv_ell, nvec, q, K,
+                       2.0 * M_PI * inv_ell, gx, gy, gz);
     return hipGetLastError();
 }
 

@@ -209,6 +209,21 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
 int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
                   const amdat_pairs *pairs, double *s2, double *s4, double *count);
 
+/* ---- differentiation (no reference counterpart) ----------------------- */
+
+/* Structure-matching loss on one frame given by device FP32 coordinate
+ * arrays d_x, d_y, d_z [n_atoms] (wrapped or not: k = 2*pi*n/length_scale):
+ *   L = sum_s w_s (S_s - T_s)^2,  S_s = sum_{v in s} |rho_v|^2 / (H_s N),
+ * over shells first_shell..last_shell; S[n] receives S_s (may be NULL).
+ * When d_gx, d_gy, d_gz (device FP64 [n_atoms]) are given, also writes
+ * dL/dr_j from an adjoint kernel that recomputes every phase
+ * exp(i k.r_j) instead of storing the N x K of them.  Used by the
+ * structure-matching torch.autograd.Function in amdat_amd/torch_sk.py. */
+int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const float *d_y, const float *d_z,
+                       const amdat_shells *sh, int first_shell, int last_shell, float length_scale,
+                       const double *weights, const double *target, double *loss, double *S,
+                       double *d_gx, double *d_gy, double *d_gz);
+
 #ifdef __cplusplus
 }
 #endif

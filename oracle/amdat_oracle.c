@@ -514,3 +514,69 @@ int orc_ref32_msd_pair(int N, const float *x, const float *y, const float *z,
     }
     return cnt;
 }
+
+/* ------------------------------------------------------------------ */
+/* S(k) misfit loss and its gradient (no reference counterpart)        */
+/* ------------------------------------------------------------------ */
+
+/* Structure-matching loss on one frame,
+ *   L = sum_s w_s (S_s - T_s)^2,   S_s = sum_{v in s} |rho_v|^2 / (H_s N),
+ * with rho_v = sum_j exp(i k_v.r_j), k_v = 2 pi n_v / ell, and its gradient
+ *   dL/dr_j = sum_v c_v k_v Im(conj(rho_v) e^{i k_v.r_j}),
+ *   c_v = -4 w_s (S_s - T_s) / (H_s N)   for v in shell s.
+ * (d|rho|^2/dr_j = 2 Re(conj(rho) i k e^{ik.r_j}) = -2 k Im(conj(rho) e^{ik.r_j}).)
+ * Pinned by central finite differences of this same loss
+ * (tests/test_oracle_adjoint.py).  S[n_shells] receives S_s. */
+double orc_sk_loss_grad_f64(int N, const double *x, const double *y, const double *z, int n_shells,
+                            const int *shell_off, const int *nvec, double ell, const double *w,
+                            const double *target, double *S, double *gx, double *gy, double *gz)
+{
+    int K = shell_off[n_shells];
+    double c = 2.0 * M_PI / ell;
+    double *rho = (double *)malloc(sizeof(double) * 2 * (size_t)(K > 0 ? K : 1));
+    double *cv = (double *)malloc(sizeof(double) * (size_t)(K > 0 ? K : 1));
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int v = 0; v < K; v++) {
+        double kx = c * nvec[3*v], ky = c * nvec[3*v+1], kz = c * nvec[3*v+2];
+        double re = 0.0, im = 0.0;
+        for (int j = 0; j < N; j++) {
+            double ph = kx * x[j] + ky * y[j] + kz * z[j];
+            re += cos(ph);
+            im += sin(ph);
+        }
+        rho[2*v] = re;
+        rho[2*v+1] = im;
+    }
+    double loss = 0.0;
+    for (int s = 0; s < n_shells; s++) {
+        int H = shell_off[s+1] - shell_off[s];
+        double acc = 0.0;
+        for (int v = shell_off[s]; v < shell_off[s+1]; v++) acc += rho[2*v]*rho[2*v] + rho[2*v+1]*rho[2*v+1];
+        S[s] = H ? acc / ((double)H * N) : 0.0;
+        double d = S[s] - target[s];
+        loss += w[s] * d * d;
+        for (int v = shell_off[s]; v < shell_off[s+1]; v++) cv[v] = -4.0 * w[s] * d / ((double)H * N);
+    }
+    if (gx) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+        for (int j = 0; j < N; j++) {
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            for (int v = 0; v < K; v++) {
+                double kx = c * nvec[3*v], ky = c * nvec[3*v+1], kz = c * nvec[3*v+2];
+                double ph = kx * x[j] + ky * y[j] + kz * z[j];
+                double s = sin(ph), co = cos(ph);
+                /* Im(conj(rho) e^{i ph}) = re*sin - im*cos */
+                double t = cv[v] * (rho[2*v] * s - rho[2*v+1] * co);
+                ax += kx * t;
+                ay += ky * t;
+                az += kz * t;
+            }
+            gx[j] = ax;
+            gy[j] = ay;
+            gz[j] = az;
+        }
+    }
+    free(rho);
+    free(cv);
+    return loss;
+}

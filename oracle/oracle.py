@@ -38,6 +38,7 @@ SIGS = {
     "orc_ref32_sk": (None, [_I, _PF, _PF, _PF, _PI, _I, _I, _PI, _I, _PI, _PI, _F, _PF]),
     "orc_ref32_fs_pair": (_I, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _I, _I, _PI, _PI, _F, _PF]),
     "orc_ref32_msd_pair": (_I, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _I, _PF, _PF]),
+    "orc_sk_loss_grad_f64": (_D, [_I, _PD, _PD, _PD, _I, _PI, _PI, _D, _PD, _PD, _PD, _PD, _PD, _PD]),
 }
 
 _lib = None
@@ -250,3 +251,25 @@ def ref32_fs(tr, pairs, ntg, shell_off, nvec, length_scale, atoms=None):
         for _ in range(n):
             natoms[g] = np.float32(natoms[g] + np.float32(1.0))
     return corr / natoms[:, None], natoms
+
+
+# ---- S(k) misfit loss and gradient ----
+def sk_loss_grad_f64(pos, shell_off, nvec, ell, w, target, grad=True):
+    """pos [N,3] float64 (one frame).  Returns (loss, S[n_shells], grad [N,3] or None)."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    x, y, z = (np.ascontiguousarray(pos[:, c]) for c in range(3))
+    off = _c(shell_off, np.int32)
+    nv = _c(nvec, np.int32).reshape(-1)
+    ns = len(off) - 1
+    w = _c(w, np.float64)
+    t = _c(target, np.float64)
+    S = np.zeros(ns)
+    g = np.zeros((3, len(x))) if grad else None
+    gp = [_p(np.ascontiguousarray(g[c]), C.c_double) for c in range(3)] if grad else [None, None, None]
+    if grad:
+        gx, gy, gz = (np.zeros(len(x)) for _ in range(3))
+        gp = [_p(gx, C.c_double), _p(gy, C.c_double), _p(gz, C.c_double)]
+    loss = lib().orc_sk_loss_grad_f64(len(x), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double), ns,
+                                      _p(off, C.c_int), _p(nv, C.c_int), float(ell), _p(w, C.c_double),
+                                      _p(t, C.c_double), _p(S, C.c_double), *gp)
+    return loss, S, (np.stack([gx, gy, gz], 1) if grad else None)

@@ -1,0 +1,26 @@
+"""Static check of amdat_amd/csrc/kernels.hip: every launch uses the block
+size its kernel was compiled for (__launch_bounds__)."""
+import os
+import re
+
+from tests.conftest import ROOT
+
+
+def test_launch_block_sizes_match_bounds():
+    src = open(os.path.join(ROOT, "amdat_amd", "csrc", "kernels.hip")).read()
+    bounds = {}
+    for m in re.finditer(r"__launch_bounds__\((\w+)(?:,\s*\d+)?\)\s+void\s+(\w+)\(", src):
+        bounds[m.group(2)] = m.group(1)
+    assert len(bounds) >= 10
+    launches = re.findall(r"hipLaunchKernelGGL\((\w+)(?:<[^>]*>)?,\s*[^,]+(?:\([^)]*\))?,\s*dim3\((\w+)\)", src)
+    assert len(launches) >= 10
+    kfn = dict(re.findall(r"auto kfn = (\w+)<", src) and [(k, k) for k in re.findall(r"auto kfn = (\w+)<", src)])
+    for name, block in launches:
+        if name == "kfn":
+            continue
+        assert name in bounds, name
+        assert bounds[name] == block, (name, bounds[name], block)
+    # the templated phase kernels launched through kfn use kPhaseBlock
+    for k in kfn:
+        assert bounds[k] == "kPhaseBlock"
+    assert src.count("dim3(kPhaseBlock), lds") == 2
