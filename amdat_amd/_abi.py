@@ -83,6 +83,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libamdat_hip.so is not built (%s); run __graft_entry__.build() "
                                "or make -C amdat_amd" % LIB_PATH)
+        # torch ships its own HIP runtime under the same soname.  Loading it
+        # first makes this library bind to that one runtime, so torch tensors
+        # and this library can share a process in either import order.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
