@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libamdat_hip.so")
 AMDAT_OK = 0
 PREC_FP32C = 0
 PREC_FP64 = 1
+PREC_FP32X = 2
 
 _ERRNAMES = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ERANGE", -5: "EIO"}
 

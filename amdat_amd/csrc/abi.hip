@@ -425,7 +425,7 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
     if (first_shell < 0 || last_shell < first_shell || last_shell >= sh->n_shells)
         return set_error(AMDAT_ERANGE, "amdat_rho_compute: shell range outside the table");
     if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_rho_compute: length scale must be > 0");
-    if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64)
+    if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64 && precision != AMDAT_PREC_FP32X)
         return set_error(AMDAT_EINVAL, "amdat_rho_compute: unknown precision");
     HIP_TRY(hipSetDevice(ctx->device));
     const int T = tr->dev.T;
@@ -685,7 +685,7 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     view.T = 1;
     view.x = d_x; view.y = d_y; view.z = d_z;
     const double inv_ell = 1.0 / (double)length_scale;
-    // forward: rho for the frame (compensated FP32 kernel, FP64 result)
+    // forward: rho for the frame (FP64 result)
     const int zero = 0;
     DevBuf<int> dfr;
     HIP_TRY(dfr.upload(&zero, 1, st));
@@ -693,7 +693,9 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     DevBuf<double2> part, rho;
     HIP_TRY(rho.alloc(K));
     if (n_splits > 1) HIP_TRY(part.alloc((size_t)n_splits * K));
-    HIP_TRY(launch_rho(AMDAT_PREC_FP32C, view, nullptr, n_atoms, dfr.p, 1, kp, inv_ell, n_splits,
+    // one frame and few vectors per shell: no averaging over frames, so the
+    // direct exact-phase kernel (about 1 ulp per term) is used
+    HIP_TRY(launch_rho(AMDAT_PREC_FP32X, view, nullptr, n_atoms, dfr.p, 1, kp, inv_ell, n_splits,
                        n_splits > 1 ? part.p : rho.p, st));
     if (n_splits > 1) HIP_TRY(launch_split_reduce2(part.p, 1, n_splits, K, rho.p, st));
     std::vector<double2> h(K);

@@ -37,6 +37,8 @@ struct KPlan {
 int plan_table_width(int max_component);
 
 // rho_k(t) for frames[n_frames] (device); out rows [frame slot][split][K].
+// precision: 0 table products (compensated FP32), 1 plain FP64,
+// 2 direct exact-phase FP32 evaluation (compensated sums).
 hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_list,
                       const int *frames, int n_frames, const KPlan &kp, double inv_ell,
                       int n_splits, double2 *out, hipStream_t st);

@@ -503,6 +503,85 @@ __global__ __launch_bounds__(kBlock) void msd_reduce_kernel(const double *__rest
     }
 }
 
+// rho_k by direct evaluation: one exact fixed-point phase
+// P = nx*ux + ny*uy + nz*uz (mod 2^32) and one sincos_turns() per
+// (atom, vector), about 1 ulp per term against about 5 ulp for the table
+// products; compensated FP32 sums.  Used where a single frame with few
+// vectors per shell leaves no averaging (the S(k)-misfit entry point).
+// grid = (chunks, frames, splits).
+constexpr int kDirectTile = 256;
+
+template <int R>
+__global__ __launch_bounds__(kPhaseBlock) void rho_direct_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
+    const int *__restrict__ list, int n_list, const int *__restrict__ frames,
+    const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, int K, int KB, double inv_ell,
+    int per_split, int n_splits, double2 *__restrict__ out)
+{
+    __shared__ uint32_t sU[3][kDirectTile];
+    const int chunk = blockIdx.x, fslot = blockIdx.y, split = blockIdx.z;
+    const int tid = threadIdx.x;
+    const size_t foff = (size_t)frames[fslot] * (size_t)N;
+    const int a_begin = split * per_split;
+    const int a_end = min(n_list, a_begin + per_split);
+    uint32_t nx[R], ny[R], nz[R];
+    float hr[R], hi[R], lr[R], li[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int a, b, c;
+        unpack_k(kpack[(size_t)chunk * KB + r * kPhaseBlock + tid], a, b, c);
+        nx[r] = (uint32_t)a; ny[r] = (uint32_t)b; nz[r] = (uint32_t)c;
+        hr[r] = hi[r] = lr[r] = li[r] = 0.0f;
+    }
+    for (int a0 = a_begin; a0 < a_end; a0 += kDirectTile) {
+        const int cnt = min(kDirectTile, a_end - a0);
+        __syncthreads();
+        if (tid < cnt) {
+            const int j = list ? list[a0 + tid] : a0 + tid;
+            sU[0][tid] = turns_fixed((double)X[foff + j] * inv_ell);
+            sU[1][tid] = turns_fixed((double)Y[foff + j] * inv_ell);
+            sU[2][tid] = turns_fixed((double)Z[foff + j] * inv_ell);
+        }
+        __syncthreads();
+        float pr[R], pi[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pr[r] = pi[r] = 0.0f;
+        for (int i = 0; i < cnt; ++i) {
+            const uint32_t ux = sU[0][i], uy = sU[1][i], uz = sU[2][i];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float2 e = sincos_turns(nx[r] * ux + ny[r] * uy + nz[r] * uz);
+                pr[r] += e.x;
+                pi[r] += e.y;
+            }
+            if ((i & 15) == 15) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    two_sum_acc(hr[r], lr[r], pr[r]);
+                    two_sum_acc(hi[r], li[r], pi[r]);
+                    pr[r] = pi[r] = 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            two_sum_acc(hr[r], lr[r], pr[r]);
+            two_sum_acc(hi[r], li[r], pi[r]);
+        }
+    }
+    double2 *dst = out + ((size_t)fslot * n_splits + split) * (size_t)K;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int d = kdst[(size_t)chunk * KB + r * kPhaseBlock + tid];
+        if (d >= 0) {
+            const double re = (double)hr[r] + (double)lr[r];
+            double im = (double)hi[r] + (double)li[r];
+            if (d & (1 << 30)) im = -im;
+            dst[d & ~(1 << 30)] = make_double2(re, im);
+        }
+    }
+}
+
 // Adjoint of the S(k) misfit on one frame: for each atom j,
 //   g_j = (2 pi / l) sum_v n_v Im(conj(q_v) e^{i k_v.r_j}),  q_v = c_v rho_v,
 // with the phase recomputed per (atom, vector) from the exact fixed-point
@@ -615,6 +694,18 @@ hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_l
                       int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,
                       hipStream_t st)
 {
+    if (precision == 2) {
+        const int per_split = (n_list + n_splits - 1) / n_splits;
+        dim3 grid(kp.n_chunks, n_frames, n_splits);
+        (void)hipGetLastError();
+        switch (kp.R) {
+        case 2: hipLaunchKernelGGL(rho_direct_kernel<2>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, inv_ell, per_split, n_splits, out); break;
+        case 4: hipLaunchKernelGGL(rho_direct_kernel<4>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, inv_ell, per_split, n_splits, out); break;
+        case 8: hipLaunchKernelGGL(rho_direct_kernel<8>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames, kp.kpack, kp.kdst, kp.K, kp.KB, inv_ell, per_split, n_splits, out); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (precision == 1) {
         const int per_split = (n_list + n_splits - 1) / n_splits;
         const double kscale = 2.0 * M_PI * inv_ell;

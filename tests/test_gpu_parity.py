@@ -51,15 +51,18 @@ def rho_pair(ctx, plain, shells99):
     return r32, r64, ref
 
 
-def test_rho_per_vector(rho_pair, plain):
+def test_rho_per_vector(ctx, rho_pair, plain, shells99):
     r32, r64, ref = rho_pair
     N = plain[0].N
     a, b = r32.download(), r64.download()
+    rx = _abi.Rho(ctx, plain[1], None, shells99[2], 0, 98, np.float32(9.41), precision=_abi.PREC_FP32X).download()
     assert a.shape == ref.shape
     e32 = np.max(np.abs(a - ref)) / np.sqrt(N)
     e64 = np.max(np.abs(b - ref)) / np.sqrt(N)
-    print("rho err fp32c %.3e fp64 %.3e" % (e32, e64))
+    ex = np.max(np.abs(rx - ref)) / np.sqrt(N)
+    print("rho err fp32c %.3e fp32x %.3e fp64 %.3e" % (e32, ex, e64))
     assert e32 < TOL
+    assert ex < TOL
     assert e64 < TOL64
 
 
