@@ -27,7 +27,8 @@ def shells_c4(qvec):
 @pytest.fixture(scope="module")
 def c4(qvec):
     print("case: adjoint N=32768 L=30.3 K=2000", flush=True)
-    L = 30.3
+    # the device takes the length scale as FP32: give the oracle the same value
+    L = float(np.float32(30.3))
     a = synth.fcc_lattice(21, L, jitter=0.1, n_keep=32768, seed=synth.MASTER_SEED + 3)
     b = synth.fcc_lattice(21, L, jitter=0.2, n_keep=32768, seed=synth.MASTER_SEED + 103)
     pos = np.stack([a.x[0], a.y[0], a.z[0]], 1).astype(np.float32)
