@@ -35,6 +35,8 @@ namespace amdat {
 namespace {
 
 constexpr int kWaves = kPhaseBlock / 64;
+// Prefetch depth (updates in flight) of the hot loops.
+constexpr int kDepth = 4;
 
 // Table geometry for half-width MM: row of W float2 per atom,
 // x at [0, 2MM], y at [2MM+1, 4MM+1], z (m >= 0 only) at [4MM+2, 5MM+2].
@@ -162,18 +164,31 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
         float pr[R], pi[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) pr[r] = pi[r] = 0.0f;
+        // Software pipeline over the flattened (atom, vector) sequence: the
+        // reads of update u + kDepth are issued before update u is computed.
+        float2 bx[kDepth], by[kDepth], bz[kDepth];
 #pragma unroll
-        for (int i = 0; i < Gm::A; ++i) {
+        for (int u = 0; u < kDepth; ++u) {
+            bx[u] = px[u % R][(u / R) * Gm::W];
+            by[u] = py[u % R][(u / R) * Gm::W];
+            bz[u] = pz[u % R][(u / R) * Gm::W];
+        }
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float2 ex = px[r][i * Gm::W];
-                const float2 ey = py[r][i * Gm::W];
-                const float2 ez = pz[r][i * Gm::W];
-                const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
-                const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
-                pr[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, pr[r]));
-                pi[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, pi[r]));
+        for (int u = 0; u < Gm::A * R; ++u) {
+            const int r = u % R, slot = u % kDepth;
+            const float2 ex = bx[slot], ey = by[slot], ez = bz[slot];
+            if (u + kDepth < Gm::A * R) {
+                const int v = u + kDepth;
+                bx[slot] = px[v % R][(v / R) * Gm::W];
+                by[slot] = py[v % R][(v / R) * Gm::W];
+                bz[slot] = pz[v % R][(v / R) * Gm::W];
             }
+            __builtin_amdgcn_sched_barrier(0);
+            const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+            const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+            pr[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, pr[r]));
+            pi[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, pi[r]));
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -258,17 +273,28 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
         float ps[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) ps[r] = 0.0f;
+        float2 bx[kDepth], by[kDepth], bz[kDepth];
 #pragma unroll
-        for (int i = 0; i < Gm::A; ++i) {
+        for (int u = 0; u < kDepth; ++u) {
+            bx[u] = px[u % R][(u / R) * Gm::W];
+            by[u] = py[u % R][(u / R) * Gm::W];
+            bz[u] = pz[u % R][(u / R) * Gm::W];
+        }
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float2 ex = px[r][i * Gm::W];
-                const float2 ey = py[r][i * Gm::W];
-                const float2 ez = pz[r][i * Gm::W];
-                const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
-                const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
-                ps[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, ps[r]));
+        for (int u = 0; u < Gm::A * R; ++u) {
+            const int r = u % R, slot = u % kDepth;
+            const float2 ex = bx[slot], ey = by[slot], ez = bz[slot];
+            if (u + kDepth < Gm::A * R) {
+                const int v = u + kDepth;
+                bx[slot] = px[v % R][(v / R) * Gm::W];
+                by[slot] = py[v % R][(v / R) * Gm::W];
+                bz[slot] = pz[v % R][(v / R) * Gm::W];
             }
+            __builtin_amdgcn_sched_barrier(0);
+            const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+            const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+            ps[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, ps[r]));
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) two_sum_acc(hs[r], ls[r], ps[r]);
