@@ -126,6 +126,13 @@ struct amdat_rho {
     DevBuf<int> d_shell_off;
 };
 
+struct amdat_batch {
+    int n = 0, T = 0;
+    std::vector<const amdat_traj *> trajs;
+    std::vector<const amdat_list *> lists;
+    DevBuf<int> counter;  // work-queue counter of the persistent launches
+};
+
 namespace {
 
 hipStream_t stream_of(amdat_ctx *c) { return c->stream ? c->stream : c->own; }
@@ -149,7 +156,16 @@ int choose_R(int K, int M)
     return best;
 }
 
-int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned **out)
+int shell_range_M(const amdat_shells *sh, int first, int last)
+{
+    int M = 0;
+    for (int v = sh->off[first]; v < sh->off[last + 1]; v++)
+        for (int c = 0; c < 3; c++) M = std::max(M, std::abs(sh->nvec[3 * v + c]));
+    return M;
+}
+
+// R_force > 0 fixes the vectors per thread (batched launches need one R).
+int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned **out, int R_force = 0)
 {
     const int v0 = sh->off[first], v1 = sh->off[last + 1];
     const int K = v1 - v0;
@@ -158,7 +174,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
     for (int v = v0; v < v1; v++)
         for (int c = 0; c < 3; c++) M = std::max(M, std::abs(sh->nvec[3 * v + c]));
     if (M > 127) return set_error(AMDAT_ERANGE, "k-vector component beyond +-127");
-    const int R = choose_R(K, M);
+    const int R = R_force > 0 ? R_force : choose_R(K, M);
     auto key = std::make_tuple(first, last, R);
     auto it = sh->plans.find(key);
     if (it != sh->plans.end()) { *out = it->second.get(); return AMDAT_OK; }
@@ -724,6 +740,282 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     HIP_TRY(dq.upload(q.data(), q.size(), st));
     HIP_TRY(dn.upload(sh->nvec.data() + 3 * (size_t)v0, 3 * (size_t)K, st));
     HIP_TRY(launch_sk_adjoint(d_x, d_y, d_z, n_atoms, inv_ell, dn.p, dq.p, K, d_gx, d_gy, d_gz, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_batch_create(amdat_ctx *ctx, int n_traj, const amdat_traj *const *trajs, const amdat_list *const *lists,
+                       amdat_batch **out)
+{
+    clear_error();
+    if (!ctx || !trajs || !out || n_traj < 1) return set_error(AMDAT_EINVAL, "amdat_batch_create: bad arguments");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto b = std::make_unique<amdat_batch>();
+    b->n = n_traj;
+    for (int i = 0; i < n_traj; i++) {
+        const amdat_traj *t = trajs[i];
+        const amdat_list *l = lists ? lists[i] : nullptr;
+        if (!t) return set_error(AMDAT_EINVAL, "amdat_batch_create: null trajectory");
+        if (i == 0) b->T = t->dev.T;
+        else if (t->dev.T != b->T)
+            return set_error(AMDAT_EINVAL, "amdat_batch_create: trajectories differ in frame count");
+        if (l && l->n_atoms != t->dev.N)
+            return set_error(AMDAT_EINVAL, "amdat_batch_create: list built for another trajectory size");
+        b->trajs.push_back(t);
+        b->lists.push_back(l);
+    }
+    HIP_TRY(b->counter.alloc(1));
+    *out = b.release();
+    return AMDAT_OK;
+}
+
+int amdat_batch_free(amdat_ctx *, amdat_batch *b)
+{
+    delete b;
+    return AMDAT_OK;
+}
+
+static BatchTraj batch_entry(const amdat_batch *b, int i, double inv_ell, int K, double *out)
+{
+    const TrajDev &d = b->trajs[i]->dev;
+    BatchTraj e;
+    e.x = d.x; e.y = d.y; e.z = d.z;
+    e.ix = d.ix; e.iy = d.iy; e.iz = d.iz;
+    e.box = d.box;
+    e.list = list_ptr(b->lists[i]);
+    e.N = d.N;
+    e.n_list = list_n(b->trajs[i], b->lists[i]);
+    e.K = K;
+    e.inv_ell = inv_ell;
+    e.out = out;
+    return e;
+}
+
+// Work items in queue order: costly first, ties in (trajectory, chunk, group) order.
+static void order_items(std::vector<int4> &items, const std::vector<double> &cost)
+{
+    std::vector<int> idx(items.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int4> sorted(items.size());
+    for (size_t i = 0; i < idx.size(); i++) sorted[i] = items[idx[i]];
+    items.swap(sorted);
+}
+
+int amdat_batch_sk(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c, int first_shell, int last_shell,
+                   const float *length_scales, const int *frames, int n_frames, double *S, double *count)
+{
+    clear_error();
+    if (!ctx || !batch || !sh_c || !length_scales || !S || !count)
+        return set_error(AMDAT_EINVAL, "amdat_batch_sk: null argument");
+    amdat_shells *sh = const_cast<amdat_shells *>(sh_c);
+    if (first_shell < 0 || last_shell < first_shell || last_shell >= sh->n_shells)
+        return set_error(AMDAT_ERANGE, "amdat_batch_sk: shell range outside the table");
+    const int n = batch->n, T = batch->T;
+    for (int i = 0; i < n; i++)
+        if (!(length_scales[i] > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_batch_sk: length scale must be > 0");
+    std::vector<int> fr;
+    if (frames) {
+        if (n_frames < 1) return set_error(AMDAT_EINVAL, "amdat_batch_sk: empty frame list");
+        fr.assign(frames, frames + n_frames);
+        for (int f : fr)
+            if (f < 0 || f >= T) return set_error(AMDAT_ERANGE, "amdat_batch_sk: frame out of range");
+    } else {
+        fr.resize(T);
+        std::iota(fr.begin(), fr.end(), 0);
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    KPlanOwned *plan = nullptr;
+    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan);
+    if (rc) return rc;
+    const KPlan &kp = plan->kp;
+    const int K = kp.K, nsh = last_shell - first_shell + 1, F = (int)fr.size();
+    // frame groups: enough items (about 16k) to balance the queue
+    const long units = (long)n * kp.n_chunks;
+    int G = (int)std::max(1L, std::min((long)F, (16384 + units - 1) / units));
+    const int FG = (F + G - 1) / G;
+    G = (F + FG - 1) / FG;
+    hipStream_t st = stream_of(ctx);
+    DevBuf<double> rows, d_norm, d_S;
+    HIP_TRY(rows.alloc((size_t)n * G * K));
+    std::vector<BatchTraj> bt(n);
+    std::vector<double> norm(n);
+    std::vector<int4> items;
+    std::vector<double> cost;
+    for (int i = 0; i < n; i++) {
+        bt[i] = batch_entry(batch, i, 1.0 / (double)length_scales[i], K, rows.p + (size_t)i * G * K);
+        norm[i] = (double)F * (double)bt[i].n_list;
+        count[i] = norm[i];
+        if (bt[i].n_list == 0) continue;
+        for (int c = 0; c < kp.n_chunks; c++)
+            for (int g = 0; g < G; g++) {
+                items.push_back(make_int4(i, c, g, 0));
+                cost.push_back((double)bt[i].n_list * (double)(std::min(F, (g + 1) * FG) - g * FG));
+            }
+    }
+    order_items(items, cost);
+    std::vector<int> soff(nsh + 1);
+    for (int s = 0; s <= nsh; s++) soff[s] = sh->off[first_shell + s] - sh->off[first_shell];
+    DevBuf<BatchTraj> d_bt;
+    DevBuf<int4> d_items;
+    DevBuf<int> d_fr, d_soff;
+    HIP_TRY(d_bt.upload(bt.data(), bt.size(), st));
+    HIP_TRY(d_items.upload(items.data(), items.size(), st));
+    HIP_TRY(d_fr.upload(fr.data(), fr.size(), st));
+    HIP_TRY(d_soff.upload(soff.data(), soff.size(), st));
+    HIP_TRY(d_norm.upload(norm.data(), norm.size(), st));
+    HIP_TRY(d_S.alloc((size_t)n * nsh));
+    HIP_TRY(launch_sk_batch(d_bt.p, d_items.p, (int)items.size(), batch->counter.p, d_fr.p, F, FG, kp, st));
+    HIP_TRY(launch_sk_batch_reduce(d_bt.p, n, G, K, d_soff.p, nsh, d_norm.p, d_S.p, st));
+    HIP_TRY(hipMemcpyAsync(S, d_S.p, sizeof(double) * n * nsh, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c, const int *first_shell,
+                   const int *last_shell, const float *length_scales, const amdat_pairs *pairs, double *Fs,
+                   double *count)
+{
+    clear_error();
+    if (!ctx || !batch || !sh_c || !first_shell || !last_shell || !length_scales || !pairs || !Fs || !count)
+        return set_error(AMDAT_EINVAL, "amdat_batch_fs: null argument");
+    amdat_shells *sh = const_cast<amdat_shells *>(sh_c);
+    const int n = batch->n, ntg = pairs->ntg;
+    int rc = check_pairs_frames(pairs, batch->T);
+    if (rc) return rc;
+    int Kmax = 0, M = 0;
+    for (int i = 0; i < n; i++) {
+        if (first_shell[i] < 0 || last_shell[i] < first_shell[i] || last_shell[i] >= sh->n_shells)
+            return set_error(AMDAT_ERANGE, "amdat_batch_fs: shell range outside the table");
+        if (!(length_scales[i] > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_batch_fs: length scale must be > 0");
+        const int K = sh->off[last_shell[i] + 1] - sh->off[first_shell[i]];
+        if (K <= 0) return set_error(AMDAT_EINVAL, "amdat_batch_fs: shell range holds no vectors");
+        Kmax = std::max(Kmax, K);
+        M = std::max(M, shell_range_M(sh, first_shell[i], last_shell[i]));
+    }
+    if (M > 127) return set_error(AMDAT_ERANGE, "amdat_batch_fs: k-vector component beyond +-127");
+    HIP_TRY(hipSetDevice(ctx->device));
+    // one R and one table width for the launch; per-trajectory plans concatenated
+    const int R = choose_R(Kmax, M);
+    std::vector<KPlanOwned *> plans(n);
+    std::vector<int> chunk_base(n + 1, 0), row_off(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+        rc = build_plan(ctx, sh, first_shell[i], last_shell[i], &plans[i], R);
+        if (rc) return rc;
+        chunk_base[i + 1] = chunk_base[i] + plans[i]->kp.n_chunks;
+    }
+    KPlan cat;
+    cat.R = R;
+    cat.KB = kPhaseBlock * R;
+    cat.n_chunks = chunk_base[n];
+    cat.MM = plan_table_width(M);
+    cat.K = 0;
+    hipStream_t st = stream_of(ctx);
+    DevBuf<uint32_t> kpack;
+    DevBuf<int> kdst, chunkM;
+    HIP_TRY(kpack.alloc((size_t)cat.n_chunks * cat.KB));
+    HIP_TRY(kdst.alloc((size_t)cat.n_chunks * cat.KB));
+    HIP_TRY(chunkM.alloc(cat.n_chunks));
+    for (int i = 0; i < n; i++) {
+        const KPlan &p = plans[i]->kp;
+        const size_t o = (size_t)chunk_base[i] * cat.KB, m = (size_t)p.n_chunks * cat.KB;
+        HIP_TRY(hipMemcpyAsync(kpack.p + o, p.kpack, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(kdst.p + o, p.kdst, m * sizeof(int), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(chunkM.p + chunk_base[i], p.chunkM, p.n_chunks * sizeof(int), hipMemcpyDeviceToDevice, st));
+    }
+    cat.kpack = kpack.p;
+    cat.kdst = kdst.p;
+    cat.chunkM = chunkM.p;
+    // rows [gap][K_b] per trajectory; shells, norms and output offsets
+    std::vector<int> soff, shell_base(n), nsh(n), out_off(n + 1, 0);
+    std::vector<double> norm((size_t)n * ntg, 0.0);
+    int max_nsh = 0;
+    for (int i = 0; i < n; i++) {
+        const int K = plans[i]->kp.K;
+        row_off[i + 1] = row_off[i] + ntg * K;
+        nsh[i] = last_shell[i] - first_shell[i] + 1;
+        max_nsh = std::max(max_nsh, nsh[i]);
+        shell_base[i] = (int)soff.size();
+        for (int s = 0; s <= nsh[i]; s++) soff.push_back(sh->off[first_shell[i] + s] - sh->off[first_shell[i]]);
+        out_off[i + 1] = out_off[i] + ntg * nsh[i];
+        const int nl = list_n(batch->trajs[i], batch->lists[i]);
+        for (int q = 0; q < pairs->n; q++) norm[(size_t)i * ntg + pairs->tg[q]] += (double)nl;
+    }
+    for (size_t q = 0; q < norm.size(); q++) count[q] = norm[q];
+    DevBuf<double> rows, d_norm, d_out;
+    HIP_TRY(rows.alloc((size_t)row_off[n]));
+    std::vector<BatchTraj> bt(n);
+    std::vector<int4> items;
+    std::vector<double> cost;
+    for (int i = 0; i < n; i++) {
+        bt[i] = batch_entry(batch, i, 1.0 / (double)length_scales[i], plans[i]->kp.K, rows.p + row_off[i]);
+        if (bt[i].n_list == 0) continue;
+        for (int c = 0; c < plans[i]->kp.n_chunks; c++)
+            for (int g = 0; g < ntg; g++) {
+                const int np = pairs->group_off[g + 1] - pairs->group_off[g];
+                if (np == 0) continue;
+                items.push_back(make_int4(i, chunk_base[i] + c, g, 0));
+                cost.push_back((double)bt[i].n_list * np);
+            }
+    }
+    order_items(items, cost);
+    DevBuf<BatchTraj> d_bt;
+    DevBuf<int4> d_items;
+    DevBuf<int> d_soff, d_base, d_nsh, d_oo;
+    HIP_TRY(d_bt.upload(bt.data(), bt.size(), st));
+    HIP_TRY(d_items.upload(items.data(), items.size(), st));
+    HIP_TRY(d_soff.upload(soff.data(), soff.size(), st));
+    HIP_TRY(d_base.upload(shell_base.data(), shell_base.size(), st));
+    HIP_TRY(d_nsh.upload(nsh.data(), nsh.size(), st));
+    HIP_TRY(d_oo.upload(out_off.data(), out_off.size(), st));
+    HIP_TRY(d_norm.upload(norm.data(), norm.size(), st));
+    HIP_TRY(d_out.alloc((size_t)out_off[n]));
+    HIP_TRY(launch_fs_batch(d_bt.p, d_items.p, (int)items.size(), batch->counter.p, pairs->d_t0.p, pairs->d_t1.p,
+                            pairs->d_group_off.p, pairs->d_order.p, cat, st));
+    HIP_TRY(launch_fs_batch_reduce(d_bt.p, n, d_soff.p, d_base.p, d_nsh.p, max_nsh, ntg, d_norm.p, d_oo.p, d_out.p,
+                                   st));
+    HIP_TRY(hipMemcpyAsync(Fs, d_out.p, sizeof(double) * out_off[n], hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *pairs, double *s2, double *s4,
+                        double *count)
+{
+    clear_error();
+    if (!ctx || !batch || !pairs || !s2 || !s4 || !count)
+        return set_error(AMDAT_EINVAL, "amdat_batch_msd_ngp: null argument");
+    int rc = check_pairs_frames(pairs, batch->T);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int n = batch->n, ntg = pairs->ntg, P = pairs->n;
+    std::vector<BatchTraj> bt(n);
+    std::vector<int2> items;
+    for (int i = 0; i < n; i++) {
+        bt[i] = batch_entry(batch, i, 1.0, 0, nullptr);
+        for (int g = 0; g < ntg; g++) count[(size_t)i * ntg + g] = 0.0;
+        for (int q = 0; q < P; q++) count[(size_t)i * ntg + pairs->tg[q]] += (double)bt[i].n_list;
+        if (bt[i].n_list == 0) continue;
+        for (int q = 0; q < P; q++) items.push_back(make_int2(i, q));
+    }
+    hipStream_t st = stream_of(ctx);
+    DevBuf<BatchTraj> d_bt;
+    DevBuf<int2> d_items;
+    DevBuf<double> part, d2, d4;
+    HIP_TRY(d_bt.upload(bt.data(), bt.size(), st));
+    HIP_TRY(d_items.upload(items.data(), items.size(), st));
+    HIP_TRY(part.alloc(std::max<size_t>(1, (size_t)n * P * 2)));
+    HIP_TRY(hipMemsetAsync(part.p, 0, part.n * sizeof(double), st));
+    const int pb = 1 << 20;  // launch in slices of at most 2^20 blocks
+    for (size_t i0 = 0; i0 < items.size(); i0 += pb) {
+        const int ni = (int)std::min<size_t>(pb, items.size() - i0);
+        HIP_TRY(launch_msd_batch(d_bt.p, d_items.p + i0, ni, pairs->d_t0.p, pairs->d_t1.p, P, part.p, st));
+    }
+    HIP_TRY(d2.alloc(std::max(1, n * ntg)));
+    HIP_TRY(d4.alloc(std::max(1, n * ntg)));
+    HIP_TRY(launch_msd_batch_reduce(part.p, P, pairs->d_group_off.p, pairs->d_order.p, ntg, n, d2.p, d4.p, st));
+    HIP_TRY(hipMemcpyAsync(s2, d2.p, sizeof(double) * n * ntg, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s4, d4.p, sizeof(double) * n * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
 }

@@ -33,6 +33,18 @@ struct KPlan {
     int *chunkM = nullptr;      // [n_chunks] max |component| in the chunk
 };
 
+// One trajectory of a batch (device-side table entry).
+struct BatchTraj {
+    const float *x, *y, *z;
+    const int16_t *ix, *iy, *iz;
+    const float *box;   // device [T][3]
+    const int *list;    // nullptr = all atoms in order
+    int N, n_list;      // atoms per frame, atoms selected
+    int K;              // vectors per output row
+    double inv_ell;     // 1 / length scale
+    double *out;        // output rows [group][K]
+};
+
 // Table half-widths with compiled kernel variants.
 int plan_table_width(int max_component);
 
@@ -66,6 +78,35 @@ hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int 
 hipError_t launch_msd_reduce(const double *partial, int n_blocks, const int *group_off,
                              const int *order, int n_groups, double *s2, double *s4,
                              hipStream_t st);
+
+// Batched launches over work items of many trajectories.  The phase
+// kernels are persistent: as many blocks as stay resident draw items from
+// a queue (counter: one device int, reset on the stream before the launch).
+// Items are processed in queue order, so callers list the costly ones first.
+//
+// S(k) items (trajectory, chunk, frame group, -): rows [group][K] of
+// sum_{f in group} |rho_v(f)|^2; frames of group g are frames[g*group_size ..].
+hipError_t launch_sk_batch(const BatchTraj *trajs, const int4 *items, int n_items, int *counter, const int *frames,
+                           int n_frames, int group_size, const KPlan &kp, hipStream_t st);
+// Fs items (trajectory, global chunk, time gap, -) over concatenated
+// per-trajectory plans (common R and MM): rows [gap][K_b] of
+// sum over the gap's pairs of sum_atoms cos(k.dr).
+hipError_t launch_fs_batch(const BatchTraj *trajs, const int4 *items, int n_items, int *counter, const int *t0,
+                           const int *t1, const int *group_off, const int *order, const KPlan &kp, hipStream_t st);
+// MSD items (trajectory b, pair p): partial[2*(b*n_pairs + p)] = sum r^2, [+1] = sum r^4.
+hipError_t launch_msd_batch(const BatchTraj *trajs, const int2 *items, int n_items, const int *t0, const int *t1,
+                            int n_pairs, double *partial, hipStream_t st);
+// Per (gap, trajectory): sums of partial rows [trajectory][pair] over the gap's pairs.
+hipError_t launch_msd_batch_reduce(const double *partial, int n_pairs, const int *group_off, const int *order,
+                                   int ntg, int n_traj, double *s2, double *s4, hipStream_t st);
+// S[b*n_shells + s] = sum_g sum_{v in s} rows_b[g][v] / H / norm[b].
+hipError_t launch_sk_batch_reduce(const BatchTraj *trajs, int n_traj, int n_groups, int K, const int *shell_off,
+                                  int n_shells, const double *norm, double *S, hipStream_t st);
+// Fs per (trajectory, time gap, shell) over ragged per-trajectory shells:
+// out[out_off[b] + g*nsh[b] + s] = sum_{v in s} rows_b[g][v] / H / norm[b*ntg + g].
+hipError_t launch_fs_batch_reduce(const BatchTraj *trajs, int n_traj, const int *shell_off, const int *shell_base,
+                                  const int *nsh, int max_nsh, int ntg, const double *norm, const int *out_off,
+                                  double *out, hipStream_t st);
 
 // Adjoint of the S(k) misfit: g_j = (2 pi/l) sum_v n_v Im(conj(q_v) e^{ik_v.r_j}).
 hipError_t launch_sk_adjoint(const float *x, const float *y, const float *z, int N, double inv_ell, const int *nvec,

@@ -23,4 +23,4 @@ def test_launch_block_sizes_match_bounds():
     # the templated phase kernels launched through kfn use kPhaseBlock
     for k in kfn:
         assert bounds[k] == "kPhaseBlock"
-    assert src.count("dim3(kPhaseBlock), lds") == 2
+    assert src.count("dim3(kPhaseBlock), lds") >= 2
