@@ -63,6 +63,11 @@ SIGNATURES = {
     "amdat_fs": (_I, [_P, _P, _P, _P, _I, _I, _F, _P, _I, _PD, _PD]),
     "amdat_msd_ngp": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
     "amdat_sk_loss_grad": (_I, [_P, _I, _P, _P, _P, _P, _I, _I, _F, _PD, _PD, _PD, _PD, _P, _P, _P]),
+    "amdat_batch_create": (_I, [_P, _I, _PP, _PP, _PP]),
+    "amdat_batch_free": (_I, [_P, _P]),
+    "amdat_batch_sk": (_I, [_P, _P, _P, _I, _I, _PF, _PI, _I, _PD, _PD]),
+    "amdat_batch_fs": (_I, [_P, _P, _P, _PI, _PI, _PF, _P, _PD, _PD]),
+    "amdat_batch_msd_ngp": (_I, [_P, _P, _P, _PD, _PD, _PD]),
 }
 
 
@@ -359,3 +364,59 @@ def sk_loss_grad(ctx, n_atoms, x_ptr, y_ptr, z_ptr, shells, first, last, length_
     check(lib().amdat_sk_loss_grad(ctx.h, n_atoms, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(z_ptr),
                                    shells.h, first, last, length_scale, dp(w), dp(t), C.byref(loss), dp(S), *g))
     return loss.value, S
+
+
+class Batch(_Handle):
+    """Many trajectories (same frame count) analysed by single persistent
+    launches.  Keeps references to its members so they outlive the handle."""
+    _free = "amdat_batch_free"
+
+    def __init__(self, ctx, trajs, lists=None):
+        n = len(trajs)
+        th = (C.c_void_p * n)(*[t.h for t in trajs])
+        lh = None
+        if lists is not None:
+            lh = (C.c_void_p * n)(*[(l.h if l is not None else None) for l in lists])
+        h = C.c_void_p()
+        check(lib().amdat_batch_create(ctx.h, n, th, lh, C.byref(h)))
+        super().__init__(ctx, h)
+        self.members = list(trajs)
+        self.lists = list(lists) if lists is not None else None
+        self.n = n
+
+    def sk(self, shells, first, last, length_scales, frames=None):
+        """S[n_traj][n_shells] and count[n_traj] (= frames x atoms)."""
+        nsh = last - first + 1
+        ell = np.ascontiguousarray(length_scales, np.float32)
+        assert len(ell) == self.n
+        fr = None if frames is None else np.ascontiguousarray(frames, np.int32)
+        S = np.zeros((self.n, nsh), np.float64)
+        cnt = np.zeros(self.n, np.float64)
+        check(lib().amdat_batch_sk(self.ctx.h, self.h, shells.h, first, last, fp(ell), ip(fr),
+                                   0 if fr is None else len(fr), dp(S), dp(cnt)))
+        return S, cnt
+
+    def fs(self, shells, first, last, length_scales, pairs):
+        """Per member i: Fs_i [n_timegaps][last[i]-first[i]+1]; count [n_traj][n_timegaps]."""
+        fi = np.ascontiguousarray(first, np.int32)
+        la = np.ascontiguousarray(last, np.int32)
+        ell = np.ascontiguousarray(length_scales, np.float32)
+        assert len(fi) == len(la) == len(ell) == self.n
+        nsh = la - fi + 1
+        out = np.zeros(int(np.sum(nsh)) * pairs.ntg, np.float64)
+        cnt = np.zeros((self.n, pairs.ntg), np.float64)
+        check(lib().amdat_batch_fs(self.ctx.h, self.h, shells.h, ip(fi), ip(la), fp(ell), pairs.h, dp(out),
+                                   dp(cnt)))
+        res, o = [], 0
+        for k in nsh:
+            res.append(out[o:o + pairs.ntg * k].reshape(pairs.ntg, k))
+            o += pairs.ntg * k
+        return res, cnt
+
+    def msd_ngp(self, pairs):
+        """s2, s4, count: [n_traj][n_timegaps]."""
+        s2 = np.zeros((self.n, pairs.ntg), np.float64)
+        s4 = np.zeros((self.n, pairs.ntg), np.float64)
+        cnt = np.zeros((self.n, pairs.ntg), np.float64)
+        check(lib().amdat_batch_msd_ngp(self.ctx.h, self.h, pairs.h, dp(s2), dp(s4), dp(cnt)))
+        return s2, s4, cnt

@@ -3,6 +3,7 @@
 // and the entry points that launch the kernels of kernels.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -830,9 +831,11 @@ int amdat_batch_sk(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
     if (rc) return rc;
     const KPlan &kp = plan->kp;
     const int K = kp.K, nsh = last_shell - first_shell + 1, F = (int)fr.size();
-    // frame groups: enough items (about 16k) to balance the queue
+    // frame groups: enough items to balance the queue (AMDAT_BATCH_ITEMS, default 16384)
+    long target = 16384;
+    if (const char *e = std::getenv("AMDAT_BATCH_ITEMS")) target = std::max(1L, std::atol(e));
     const long units = (long)n * kp.n_chunks;
-    int G = (int)std::max(1L, std::min((long)F, (16384 + units - 1) / units));
+    int G = (int)std::max(1L, std::min((long)F, (target + units - 1) / units));
     const int FG = (F + G - 1) / G;
     G = (F + FG - 1) / FG;
     hipStream_t st = stream_of(ctx);

@@ -845,7 +845,7 @@ __global__ __launch_bounds__(kBlock) void fs_batch_reduce_kernel(
         out[out_off[b] + (size_t)g * nsh[b] + s] = (H > 0 && nrm != 0.0) ? acc / (double)H / nrm : 0.0;
 }
 
-__global__ void fill_int_kernel(int *p, int n, int v)
+__global__ __launch_bounds__(64) void fill_int_kernel(int *p, int n, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

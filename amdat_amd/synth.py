@@ -73,6 +73,25 @@ def plain_config(N=1000, L=9.41, nexp=5, nes=20, sigma=0.05, seed=MASTER_SEED + 
     return t
 
 
+def batch_configs(n_traj, n_min, n_max, nexp, nes, seed=MASTER_SEED + 2):
+    """Many independent systems of uneven size (the batched workload):
+    N_i ~ U{n_min..n_max}, density rho_i ~ U[1.0, 1.2], L_i = (N_i/rho_i)^(1/3),
+    T = nes*nexp + 1 frames, Gaussian steps sigma_i ~ U[0.01, 0.1].
+    Returns a list of Traj, each with attribute L (its box length)."""
+    rng = np.random.default_rng(seed)
+    T = nes * nexp + 1
+    out = []
+    for i in range(n_traj):
+        N = int(rng.integers(n_min, n_max + 1))
+        dens = rng.uniform(1.0, 1.2)
+        L = float(np.float32((N / dens) ** (1.0 / 3.0)))
+        sigma = rng.uniform(0.01, 0.1)
+        t = random_walk(N, L, T, sigma, seed=int(rng.integers(1 << 62)))
+        t.L = L
+        out.append(t)
+    return out
+
+
 def chain_melt(n_chains=400, beads=160, L=42.2, T=1001, sigma=0.02, seed=MASTER_SEED + 1):
     """Bead-spring melt: seeded random-walk chains (bond U[0.95,0.99],
     uniform directions, uniform chain starts), then Gaussian steps."""

@@ -56,6 +56,7 @@ typedef struct amdat_shells amdat_shells;
 typedef struct amdat_pairs amdat_pairs;
 typedef struct amdat_list amdat_list;
 typedef struct amdat_rho amdat_rho;
+typedef struct amdat_batch amdat_batch;
 
 /* ---- errors, version ---------------------------------------------- */
 
@@ -224,6 +225,51 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
                        const amdat_shells *sh, int first_shell, int last_shell, float length_scale,
                        const double *weights, const double *target, double *loss, double *S,
                        double *d_gx, double *d_gy, double *d_gz);
+
+/* ---- batches of trajectories (no reference counterpart) ---------------- */
+/* The reference analyses one system per process run.  A batch analyses
+ * many trajectories of uneven size in one persistent launch per analysis:
+ * the launch keeps as many blocks resident as fit and they draw work items
+ * (trajectory x vector chunk x frame group or time gap) from a queue,
+ * costliest first.  Each member follows the single-trajectory semantics
+ * cited below.  Members must hold the same number of frames (one time
+ * scheme, one pair table).  The handle keeps pointers to the members:
+ * free the batch before them.  Concurrent batch calls on one batch race on
+ * its queue counter. */
+
+/* Build a batch of n_traj trajectories; lists[i] (or lists == NULL, or a
+ * NULL entry) selects the atoms of member i (NULL = all atoms in order). */
+int amdat_batch_create(amdat_ctx *ctx, int n_traj, const amdat_traj *const *trajs,
+                       const amdat_list *const *lists, amdat_batch **out);
+int amdat_batch_free(amdat_ctx *ctx, amdat_batch *batch);
+
+/* S(k) of every member over one shell range and one frame list (NULL = all
+ * frames), k = 2*pi*n/length_scales[i] (each member's own length scale):
+ * S[i*n_shells + s] = sum_frames sum_{v in s} |rho_v|^2 / (H_s * count[i]),
+ * count[i] = n_frames * atoms of member i.  Per member this is
+ * amdat_rho_compute + amdat_sk (Structure_Factor::analyze,
+ * src/structure_factor.cpp:171-320; Wave_Density::listkernel,
+ * src/wave_density.cpp:339-365). */
+int amdat_batch_sk(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh, int first_shell,
+                   int last_shell, const float *length_scales, const int *frames, int n_frames,
+                   double *S, double *count);
+
+/* Fs(k,t) of every member, member i on its own shell range
+ * first_shell[i]..last_shell[i] of one table and its own length scale, all
+ * over one pair table.  Member i's rows [n_timegaps][n_shells_i] start at
+ * Fs + sum_{j<i} n_timegaps * n_shells_j; count[i*n_timegaps + g] = atom
+ * pairs of gap g.  Per member this is amdat_fs
+ * (Incoherent_Scattering_Function, src/incoherent_scattering_function.cpp:176-299). */
+int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh, const int *first_shell,
+                   const int *last_shell, const float *length_scales, const amdat_pairs *pairs,
+                   double *Fs, double *count);
+
+/* MSD / NGP sums of every member: s2, s4, count are [n_traj][n_timegaps].
+ * Per member this is amdat_msd_ngp (Mean_Square_Displacement,
+ * src/mean_square_displacement.cpp:140-180; Non_Gaussian_Parameter,
+ * src/non_gaussian_parameter.cpp:96-143). */
+int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *pairs, double *s2,
+                        double *s4, double *count);
 
 #ifdef __cplusplus
 }
