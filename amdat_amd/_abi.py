@@ -68,7 +68,13 @@ SIGNATURES = {
     "amdat_batch_sk": (_I, [_P, _P, _P, _I, _I, _PF, _PI, _I, _PD, _PD]),
     "amdat_batch_fs": (_I, [_P, _P, _P, _PI, _PI, _PF, _P, _PD, _PD]),
     "amdat_batch_msd_ngp": (_I, [_P, _P, _P, _PD, _PD, _PD]),
+    "amdat_batch_run": (_I, [_P, _P, _PF, _P, _I, _I, _PI, _I, _P, _PI, _PI, _P, _I, _I,
+                             _PD, _PD, _PD, _PD, _PD, _PD, _PD]),
+    "amdat_batch_job_info": (_I, [_P, _PI, _PI, _PI, _PI]),
 }
+
+RUN_GRAPH = 1
+RUN_TUNE = 2
 
 
 class AmdatError(RuntimeError):
@@ -420,3 +426,52 @@ class Batch(_Handle):
         cnt = np.zeros((self.n, pairs.ntg), np.float64)
         check(lib().amdat_batch_msd_ngp(self.ctx.h, self.h, pairs.h, dp(s2), dp(s4), dp(cnt)))
         return s2, s4, cnt
+
+    def run(self, length_scales, sk=None, frames=None, fs=None, pairs=None, msd=False, graph=False, tune=False):
+        """One job in one persistent launch.  sk = (shells, first, last);
+        fs = (shells, first[], last[]); pairs serves Fs and MSD.  Returns a
+        dict with the outputs of sk(), fs() and msd_ngp() for the parts asked."""
+        n = self.n
+        ell = np.ascontiguousarray(length_scales, np.float32)
+        assert len(ell) == n
+        out = {}
+        S = Sc = Fs = Fc = s2 = s4 = mc = None
+        sk_h, f0, f1, fr = None, 0, -1, None
+        if sk is not None:
+            sk_h, f0, f1 = sk[0].h, sk[1], sk[2]
+            S = np.zeros((n, f1 - f0 + 1), np.float64)
+            Sc = np.zeros(n, np.float64)
+            fr = None if frames is None else np.ascontiguousarray(frames, np.int32)
+        fs_h, fi, la, nsh = None, None, None, None
+        ntg = pairs.ntg if pairs is not None else 0
+        if fs is not None:
+            fs_h = fs[0].h
+            fi = np.ascontiguousarray(fs[1], np.int32)
+            la = np.ascontiguousarray(fs[2], np.int32)
+            nsh = la - fi + 1
+            Fs = np.zeros(int(np.sum(nsh)) * ntg, np.float64)
+            Fc = np.zeros((n, ntg), np.float64)
+        if msd:
+            s2 = np.zeros((n, ntg), np.float64)
+            s4 = np.zeros((n, ntg), np.float64)
+            mc = np.zeros((n, ntg), np.float64)
+        flags = (RUN_GRAPH if graph else 0) | (RUN_TUNE if tune else 0)
+        check(lib().amdat_batch_run(self.ctx.h, self.h, fp(ell), sk_h, f0, f1, ip(fr), 0 if fr is None else len(fr),
+                                    fs_h, ip(fi), ip(la), pairs.h if pairs is not None else None, int(bool(msd)),
+                                    flags, dp(S), dp(Sc), dp(Fs), dp(Fc), dp(s2), dp(s4), dp(mc)))
+        if sk is not None:
+            out["S"], out["S_count"] = S, Sc
+        if fs is not None:
+            res, o = [], 0
+            for k in nsh:
+                res.append(Fs[o:o + ntg * k].reshape(ntg, k))
+                o += ntg * k
+            out["Fs"], out["Fs_count"] = res, Fc
+        if msd:
+            out["s2"], out["s4"], out["msd_count"] = s2, s4, mc
+        return out
+
+    def job_info(self):
+        rs, rf, ni, g = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().amdat_batch_job_info(self.h, C.byref(rs), C.byref(rf), C.byref(ni), C.byref(g)))
+        return dict(RS=rs.value, RF=rf.value, n_items=ni.value, graph=bool(g.value))

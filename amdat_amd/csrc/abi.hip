@@ -2,6 +2,7 @@
 // device-resident trajectories, shell tables, pair tables and densities,
 // and the entry points that launch the kernels of kernels.hip.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -104,13 +105,24 @@ struct KPlanOwned {
     DevBuf<int> kdst, chunkM;
 };
 
+// Serial number of a table handle: cached batch jobs compare (address,
+// serial), so a new handle at a freed handle's address is never mistaken
+// for it.
+static uint64_t next_serial()
+{
+    static std::atomic<uint64_t> c{1};
+    return c.fetch_add(1);
+}
+
 struct amdat_shells {
+    uint64_t serial = next_serial();
     int n_shells = 0;
     std::vector<int> off, nvec;
     std::map<std::tuple<int, int, int>, std::unique_ptr<KPlanOwned>> plans;
 };
 
 struct amdat_pairs {
+    uint64_t serial = next_serial();
     int n = 0, ntg = 0;
     std::vector<int> tg, t0, t1;
     std::vector<int> group_off, order;  // pairs grouped by time gap, reference order within a gap
@@ -127,11 +139,60 @@ struct amdat_rho {
     DevBuf<int> d_shell_off;
 };
 
+// Parameters of one batch job (amdat_batch_run); equal parameters reuse
+// the cached plan and captured graph.
+struct JobParams {
+    const amdat_shells *sk_sh = nullptr;
+    int sk_first = 0, sk_last = -1;
+    std::vector<int> frames;
+    const amdat_shells *fs_sh = nullptr;
+    std::vector<int> fs_first, fs_last;
+    const amdat_pairs *pairs = nullptr;
+    bool msd = false;
+    std::vector<float> ell;
+    uint64_t sk_serial = 0, fs_serial = 0, pairs_serial = 0;
+    bool operator==(const JobParams &o) const
+    {
+        return sk_sh == o.sk_sh && sk_first == o.sk_first && sk_last == o.sk_last && frames == o.frames &&
+               fs_sh == o.fs_sh && fs_first == o.fs_first && fs_last == o.fs_last && pairs == o.pairs &&
+               msd == o.msd && ell == o.ell && sk_serial == o.sk_serial && fs_serial == o.fs_serial &&
+               pairs_serial == o.pairs_serial;
+    }
+};
+
+// A planned job: device tables, queue, outputs and (optionally) the
+// instantiated graph of its launch sequence.
+struct JobCache {
+    JobParams prm;
+    int RS = 0, RF = 0, MM = 0;
+    int n = 0, ntg = 0, nsh_sk = 0, G = 0, max_nsh = 0;
+    std::vector<int> out_off;              // Fs offsets per trajectory
+    std::vector<double> sk_count, fs_count, msd_count;
+    std::vector<char> msd_item;            // [n*ntg]: an MSD item wrote (b, g)
+    DevBuf<BatchTraj> sk_bt, fs_bt;
+    DevBuf<int4> items;
+    DevBuf<int> frames, sk_soff, fs_soff, fs_base, fs_nsh, fs_oo;
+    DevBuf<uint32_t> fs_kpack;
+    DevBuf<int> fs_kdst, fs_chunkM;
+    DevBuf<double> sk_rows, fs_rows, sk_norm, fs_norm, S, Fs, s2, s4;
+    BatchJob job;
+    JobLaunch L;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~JobCache()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
 struct amdat_batch {
     int n = 0, T = 0;
     std::vector<const amdat_traj *> trajs;
     std::vector<const amdat_list *> lists;
     DevBuf<int> counter;  // work-queue counter of the persistent launches
+    std::unique_ptr<JobCache> job;
+    std::vector<std::pair<JobParams, std::pair<int, int>>> tuned;  // autotuned (RS, RF) per job
 };
 
 namespace {
@@ -1020,6 +1081,405 @@ int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *p
     HIP_TRY(hipMemcpyAsync(s2, d2.p, sizeof(double) * n * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(s4, d4.p, sizeof(double) * n * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Enqueue a planned job: persistent launch, then the per-shell reductions.
+hipError_t enqueue_job_sequence(JobCache &jc, hipStream_t st)
+{
+    hipError_t e = enqueue_job(jc.L, jc.job, st);
+    if (e == hipSuccess && jc.nsh_sk > 0)
+        e = launch_sk_batch_reduce(jc.sk_bt.p, jc.n, jc.G, jc.job.sk.K, jc.sk_soff.p, jc.nsh_sk, jc.sk_norm.p, jc.S.p,
+                                   st);
+    if (e == hipSuccess && jc.max_nsh > 0)
+        e = launch_fs_batch_reduce(jc.fs_bt.p, jc.n, jc.fs_soff.p, jc.fs_base.p, jc.fs_nsh.p, jc.max_nsh, jc.ntg,
+                                   jc.fs_norm.p, jc.fs_oo.p, jc.Fs.p, st);
+    return e;
+}
+
+// Plan a job for the trajectories with use[b] != 0 (others get no items)
+// with R = RS for S(k) items and RF for Fs items.
+int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std::vector<char> &use, int RS, int RF,
+             JobCache &jc)
+{
+    const int n = batch->n;
+    hipStream_t st = stream_of(ctx);
+    jc.prm = prm;
+    jc.n = n;
+    jc.RS = RS;
+    jc.RF = RF;
+    jc.ntg = prm.pairs ? prm.pairs->ntg : 0;
+    std::vector<int> nl(n);
+    for (int b = 0; b < n; b++) nl[b] = list_n(batch->trajs[b], batch->lists[b]);
+    std::vector<int4> items;
+    std::vector<double> cost;
+    int M = 0;
+    // S(k): one plan, frame groups
+    KPlan skp;
+    if (prm.sk_sh) {
+        amdat_shells *sh = const_cast<amdat_shells *>(prm.sk_sh);
+        KPlanOwned *plan = nullptr;
+        int rc = build_plan(ctx, sh, prm.sk_first, prm.sk_last, &plan, RS);
+        if (rc) return rc;
+        skp = plan->kp;
+        M = std::max(M, shell_range_M(sh, prm.sk_first, prm.sk_last));
+        const int F = (int)prm.frames.size(), K = skp.K;
+        jc.nsh_sk = prm.sk_last - prm.sk_first + 1;
+        long target = 16384;
+        if (const char *e = std::getenv("AMDAT_BATCH_ITEMS")) target = std::max(1L, std::atol(e));
+        const long units = (long)n * skp.n_chunks;
+        int G = (int)std::max(1L, std::min((long)F, (target + units - 1) / units));
+        const int FG = (F + G - 1) / G;
+        G = (F + FG - 1) / FG;
+        jc.G = G;
+        HIP_TRY(jc.sk_rows.alloc((size_t)n * G * K));
+        std::vector<BatchTraj> bt(n);
+        std::vector<double> norm(n);
+        jc.sk_count.assign(n, 0.0);
+        for (int b = 0; b < n; b++) {
+            bt[b] = batch_entry(batch, b, 1.0 / (double)prm.ell[b], K, jc.sk_rows.p + (size_t)b * G * K);
+            norm[b] = (double)F * nl[b];
+            jc.sk_count[b] = norm[b];
+            if (!use[b] || nl[b] == 0) continue;
+            for (int c = 0; c < skp.n_chunks; c++)
+                for (int g = 0; g < G; g++) {
+                    items.push_back(make_int4(b, c, g, 0));
+                    cost.push_back((double)nl[b] * (std::min(F, (g + 1) * FG) - g * FG) * skp.KB);
+                }
+        }
+        std::vector<int> soff(jc.nsh_sk + 1);
+        for (int s = 0; s <= jc.nsh_sk; s++) soff[s] = sh->off[prm.sk_first + s] - sh->off[prm.sk_first];
+        HIP_TRY(jc.sk_bt.upload(bt.data(), bt.size(), st));
+        HIP_TRY(jc.frames.upload(prm.frames.data(), prm.frames.size(), st));
+        HIP_TRY(jc.sk_soff.upload(soff.data(), soff.size(), st));
+        HIP_TRY(jc.sk_norm.upload(norm.data(), norm.size(), st));
+        HIP_TRY(jc.S.alloc((size_t)n * jc.nsh_sk));
+        jc.job.frames = jc.frames.p;
+        jc.job.n_frames = F;
+        jc.job.group_size = FG;
+        jc.job.sk_trajs = jc.sk_bt.p;
+    }
+    // Fs: per-trajectory plans, concatenated
+    KPlan fsp;
+    const amdat_pairs *pairs = prm.pairs;
+    if (prm.fs_sh) {
+        amdat_shells *sh = const_cast<amdat_shells *>(prm.fs_sh);
+        const int ntg = jc.ntg;
+        std::vector<KPlanOwned *> plans(n);
+        std::vector<int> chunk_base(n + 1, 0), row_off(n + 1, 0);
+        for (int b = 0; b < n; b++) {
+            int rc = build_plan(ctx, sh, prm.fs_first[b], prm.fs_last[b], &plans[b], RF);
+            if (rc) return rc;
+            M = std::max(M, shell_range_M(sh, prm.fs_first[b], prm.fs_last[b]));
+            chunk_base[b + 1] = chunk_base[b] + plans[b]->kp.n_chunks;
+        }
+        fsp.R = RF;
+        fsp.KB = kPhaseBlock * RF;
+        fsp.n_chunks = chunk_base[n];
+        HIP_TRY(jc.fs_kpack.alloc((size_t)fsp.n_chunks * fsp.KB));
+        HIP_TRY(jc.fs_kdst.alloc((size_t)fsp.n_chunks * fsp.KB));
+        HIP_TRY(jc.fs_chunkM.alloc(fsp.n_chunks));
+        for (int b = 0; b < n; b++) {
+            const KPlan &p = plans[b]->kp;
+            const size_t o = (size_t)chunk_base[b] * fsp.KB, m = (size_t)p.n_chunks * fsp.KB;
+            HIP_TRY(hipMemcpyAsync(jc.fs_kpack.p + o, p.kpack, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(jc.fs_kdst.p + o, p.kdst, m * sizeof(int), hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(jc.fs_chunkM.p + chunk_base[b], p.chunkM, p.n_chunks * sizeof(int),
+                                   hipMemcpyDeviceToDevice, st));
+        }
+        fsp.kpack = jc.fs_kpack.p;
+        fsp.kdst = jc.fs_kdst.p;
+        fsp.chunkM = jc.fs_chunkM.p;
+        std::vector<int> soff, base(n), nsh(n);
+        jc.out_off.assign(n + 1, 0);
+        std::vector<double> norm((size_t)n * ntg, 0.0);
+        for (int b = 0; b < n; b++) {
+            row_off[b + 1] = row_off[b] + ntg * plans[b]->kp.K;
+            nsh[b] = prm.fs_last[b] - prm.fs_first[b] + 1;
+            jc.max_nsh = std::max(jc.max_nsh, nsh[b]);
+            base[b] = (int)soff.size();
+            for (int s = 0; s <= nsh[b]; s++) soff.push_back(sh->off[prm.fs_first[b] + s] - sh->off[prm.fs_first[b]]);
+            jc.out_off[b + 1] = jc.out_off[b] + ntg * nsh[b];
+            for (int q = 0; q < pairs->n; q++) norm[(size_t)b * ntg + pairs->tg[q]] += (double)nl[b];
+        }
+        jc.fs_count = norm;
+        HIP_TRY(jc.fs_rows.alloc((size_t)row_off[n]));
+        std::vector<BatchTraj> bt(n);
+        for (int b = 0; b < n; b++) {
+            bt[b] = batch_entry(batch, b, 1.0 / (double)prm.ell[b], plans[b]->kp.K, jc.fs_rows.p + row_off[b]);
+            if (!use[b] || nl[b] == 0) continue;
+            for (int c = 0; c < plans[b]->kp.n_chunks; c++)
+                for (int g = 0; g < ntg; g++) {
+                    const int np = pairs->group_off[g + 1] - pairs->group_off[g];
+                    if (np == 0) continue;
+                    items.push_back(make_int4(b, chunk_base[b] + c, g, 1));
+                    cost.push_back((double)nl[b] * np * fsp.KB);
+                }
+        }
+        HIP_TRY(jc.fs_bt.upload(bt.data(), bt.size(), st));
+        HIP_TRY(jc.fs_soff.upload(soff.data(), soff.size(), st));
+        HIP_TRY(jc.fs_base.upload(base.data(), base.size(), st));
+        HIP_TRY(jc.fs_nsh.upload(nsh.data(), nsh.size(), st));
+        HIP_TRY(jc.fs_oo.upload(jc.out_off.data(), jc.out_off.size(), st));
+        HIP_TRY(jc.fs_norm.upload(norm.data(), norm.size(), st));
+        HIP_TRY(jc.Fs.alloc((size_t)jc.out_off[n]));
+        jc.job.fs_trajs = jc.fs_bt.p;
+    }
+    // MSD/NGP: one item per (trajectory, gap)
+    if (prm.msd) {
+        const int ntg = jc.ntg;
+        jc.msd_count.assign((size_t)n * ntg, 0.0);
+        jc.msd_item.assign((size_t)n * ntg, 0);
+        std::vector<BatchTraj> bt(n);
+        for (int b = 0; b < n; b++) {
+            bt[b] = batch_entry(batch, b, 1.0, 0, nullptr);
+            for (int q = 0; q < pairs->n; q++) jc.msd_count[(size_t)b * ntg + pairs->tg[q]] += (double)nl[b];
+            if (!use[b] || nl[b] == 0) continue;
+            for (int g = 0; g < ntg; g++) {
+                const int np = pairs->group_off[g + 1] - pairs->group_off[g];
+                if (np == 0) continue;
+                items.push_back(make_int4(b, g, 0, 2));
+                cost.push_back((double)nl[b] * np * 4.0);
+                jc.msd_item[(size_t)b * ntg + g] = 1;
+            }
+        }
+        if (!jc.sk_bt.p && !jc.fs_bt.p) HIP_TRY(jc.sk_bt.upload(bt.data(), bt.size(), st));
+        jc.job.msd_trajs = jc.sk_bt.p ? jc.sk_bt.p : jc.fs_bt.p;
+        HIP_TRY(jc.s2.alloc(std::max(1, n * ntg)));
+        HIP_TRY(jc.s4.alloc(std::max(1, n * ntg)));
+        jc.job.s2 = jc.s2.p;
+        jc.job.s4 = jc.s4.p;
+    }
+    if (pairs) {
+        jc.job.t0 = pairs->d_t0.p;
+        jc.job.t1 = pairs->d_t1.p;
+        jc.job.group_off = pairs->d_group_off.p;
+        jc.job.order = pairs->d_order.p;
+        jc.job.ntg = jc.ntg;
+    }
+    if (M > 127) return set_error(AMDAT_ERANGE, "amdat_batch_run: k-vector component beyond +-127");
+    jc.MM = plan_table_width(M);
+    skp.MM = fsp.MM = jc.MM;
+    jc.job.sk = skp;
+    jc.job.fs = fsp;
+    order_items(items, cost);
+    HIP_TRY(jc.items.upload(items.data(), items.size(), st));
+    jc.job.items = jc.items.p;
+    jc.job.n_items = (int)items.size();
+    jc.job.counter = batch->counter.p;
+    HIP_TRY(prepare_job(RS, RF, jc.MM, std::max(1, jc.job.n_items), &jc.L));
+    return AMDAT_OK;
+}
+
+// Time one planned job (second of two runs), in ms.
+int time_job(amdat_ctx *ctx, JobCache &jc, float *ms)
+{
+    hipStream_t st = stream_of(ctx);
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    hipError_t e = enqueue_job_sequence(jc, st);
+    if (e == hipSuccess) e = hipEventRecord(a, st);
+    if (e == hipSuccess) e = enqueue_job_sequence(jc, st);
+    if (e == hipSuccess) e = hipEventRecord(b, st);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    HIP_TRY(e);
+    return AMDAT_OK;
+}
+
+// Autotuner: time R in {2, 4, 8} for the S(k) items and, separately, for
+// the Fs items, on a sample of about an eighth of the trajectories (every
+// k-th by size), and keep the fastest of each.
+int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, int *RS, int *RF)
+{
+    const int n = batch->n;
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+        return list_n(batch->trajs[a], batch->lists[a]) > list_n(batch->trajs[b], batch->lists[b]);
+    });
+    std::vector<char> use(n, 0);
+    const int step = std::max(1, n / 8);
+    for (int i = 0; i < n; i += step) use[idx[i]] = 1;
+    static const int Rs[] = {2, 4, 8};
+    for (int part = 0; part < 2; part++) {
+        JobParams p = prm;
+        p.msd = false;
+        if (part == 0) { p.fs_sh = nullptr; if (!p.sk_sh) continue; }
+        else { p.sk_sh = nullptr; if (!p.fs_sh) continue; }
+        float best = 1e30f;
+        for (int R : Rs) {
+            JobCache jc;
+            int rc = plan_job(ctx, batch, p, use, part == 0 ? R : *RS, part == 0 ? *RF : R, jc);
+            if (rc) return rc;
+            float ms = 0.0f;
+            rc = time_job(ctx, jc, &ms);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                (part == 0 ? *RS : *RF) = R;
+            }
+        }
+    }
+    return AMDAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scales, const amdat_shells *sk_shells,
+                    int sk_first, int sk_last, const int *frames, int n_frames, const amdat_shells *fs_shells,
+                    const int *fs_first, const int *fs_last, const amdat_pairs *pairs, int want_msd, int flags,
+                    double *S, double *S_count, double *Fs, double *Fs_count, double *s2, double *s4,
+                    double *msd_count)
+{
+    clear_error();
+    if (!ctx || !batch || !length_scales) return set_error(AMDAT_EINVAL, "amdat_batch_run: null argument");
+    if (!sk_shells && !fs_shells && !want_msd) return set_error(AMDAT_EINVAL, "amdat_batch_run: nothing to compute");
+    if ((fs_shells || want_msd) && !pairs) return set_error(AMDAT_EINVAL, "amdat_batch_run: Fs and MSD need pairs");
+    if (sk_shells && (!S || !S_count)) return set_error(AMDAT_EINVAL, "amdat_batch_run: null S output");
+    if (fs_shells && (!Fs || !Fs_count || !fs_first || !fs_last))
+        return set_error(AMDAT_EINVAL, "amdat_batch_run: null Fs argument");
+    if (want_msd && (!s2 || !s4 || !msd_count)) return set_error(AMDAT_EINVAL, "amdat_batch_run: null MSD output");
+    const int n = batch->n, T = batch->T;
+    JobParams prm;
+    prm.ell.assign(length_scales, length_scales + n);
+    for (float l : prm.ell)
+        if (!(l > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_batch_run: length scale must be > 0");
+    if (sk_shells) {
+        if (sk_first < 0 || sk_last < sk_first || sk_last >= sk_shells->n_shells)
+            return set_error(AMDAT_ERANGE, "amdat_batch_run: S(k) shell range outside the table");
+        prm.sk_sh = sk_shells;
+        prm.sk_serial = sk_shells->serial;
+        prm.sk_first = sk_first;
+        prm.sk_last = sk_last;
+        if (frames) {
+            if (n_frames < 1) return set_error(AMDAT_EINVAL, "amdat_batch_run: empty frame list");
+            prm.frames.assign(frames, frames + n_frames);
+            for (int f : prm.frames)
+                if (f < 0 || f >= T) return set_error(AMDAT_ERANGE, "amdat_batch_run: frame out of range");
+        } else {
+            prm.frames.resize(T);
+            std::iota(prm.frames.begin(), prm.frames.end(), 0);
+        }
+    }
+    if (pairs) {
+        int rc = check_pairs_frames(pairs, T);
+        if (rc) return rc;
+        prm.pairs = pairs;
+        prm.pairs_serial = pairs->serial;
+    }
+    int Kfs = 0, Mfs = 0;
+    if (fs_shells) {
+        prm.fs_sh = fs_shells;
+        prm.fs_serial = fs_shells->serial;
+        prm.fs_first.assign(fs_first, fs_first + n);
+        prm.fs_last.assign(fs_last, fs_last + n);
+        for (int b = 0; b < n; b++) {
+            if (fs_first[b] < 0 || fs_last[b] < fs_first[b] || fs_last[b] >= fs_shells->n_shells)
+                return set_error(AMDAT_ERANGE, "amdat_batch_run: Fs shell range outside the table");
+            const int K = fs_shells->off[fs_last[b] + 1] - fs_shells->off[fs_first[b]];
+            if (K <= 0) return set_error(AMDAT_EINVAL, "amdat_batch_run: Fs shell range holds no vectors");
+            Kfs = std::max(Kfs, K);
+            Mfs = std::max(Mfs, shell_range_M(fs_shells, fs_first[b], fs_last[b]));
+        }
+    }
+    prm.msd = want_msd != 0;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const bool use_graph = (flags & AMDAT_RUN_GRAPH) != 0;
+    hipStream_t user = stream_of(ctx);
+    if (!batch->job || !(batch->job->prm == prm) || (use_graph && !batch->job->exec)) {
+        batch->job.reset();
+        // vectors per thread: autotuned, or from the cost model
+        int RS = 0, RF = 0;
+        if (sk_shells) {
+            const int K = sk_shells->off[sk_last + 1] - sk_shells->off[sk_first];
+            if (K <= 0) return set_error(AMDAT_EINVAL, "amdat_batch_run: S(k) shell range holds no vectors");
+            RS = choose_R(K, shell_range_M(sk_shells, sk_first, sk_last));
+        }
+        if (fs_shells) RF = choose_R(Kfs, Mfs);
+        if (!RS) RS = RF ? RF : 2;
+        if (!RF) RF = RS;
+        if (flags & AMDAT_RUN_TUNE) {
+            bool found = false;
+            for (auto &t : batch->tuned)
+                if (t.first == prm) { RS = t.second.first; RF = t.second.second; found = true; }
+            if (!found) {
+                int rc = autotune_job(ctx, batch, prm, &RS, &RF);
+                if (rc) return rc;
+                batch->tuned.push_back({prm, {RS, RF}});
+            }
+        }
+        auto jc = std::make_unique<JobCache>();
+        std::vector<char> use(n, 1);
+        int rc = plan_job(ctx, batch, prm, use, RS, RF, *jc);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(user));
+        if (use_graph) {
+            // capture on the context's own stream (the caller's may be the null stream)
+            HIP_TRY(hipStreamBeginCapture(ctx->own, hipStreamCaptureModeThreadLocal));
+            hipError_t e = enqueue_job_sequence(*jc, ctx->own);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(ctx->own, &g);
+            jc->graph = g;
+            HIP_TRY(e);
+            HIP_TRY(e2);
+            HIP_TRY(hipGraphInstantiate(&jc->exec, g, nullptr, nullptr, 0));
+        }
+        batch->job = std::move(jc);
+    }
+    JobCache &jc = *batch->job;
+    hipStream_t st = user;
+    if (use_graph) {
+        // order the replay after the caller's stream, then run it on the own stream
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        hipError_t e = hipEventRecord(ev, user);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->own, ev, 0);
+        (void)hipEventDestroy(ev);
+        HIP_TRY(e);
+        st = ctx->own;
+        HIP_TRY(hipGraphLaunch(jc.exec, st));
+    } else {
+        HIP_TRY(enqueue_job_sequence(jc, st));
+    }
+    if (sk_shells) {
+        HIP_TRY(hipMemcpyAsync(S, jc.S.p, sizeof(double) * n * jc.nsh_sk, hipMemcpyDeviceToHost, st));
+        std::copy(jc.sk_count.begin(), jc.sk_count.end(), S_count);
+    }
+    if (fs_shells) {
+        HIP_TRY(hipMemcpyAsync(Fs, jc.Fs.p, sizeof(double) * jc.out_off[n], hipMemcpyDeviceToHost, st));
+        std::copy(jc.fs_count.begin(), jc.fs_count.end(), Fs_count);
+    }
+    if (want_msd) {
+        HIP_TRY(hipMemcpyAsync(s2, jc.s2.p, sizeof(double) * n * jc.ntg, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(s4, jc.s4.p, sizeof(double) * n * jc.ntg, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    if (want_msd) {
+        for (size_t i = 0; i < jc.msd_item.size(); i++) {
+            if (!jc.msd_item[i]) s2[i] = s4[i] = 0.0;
+            msd_count[i] = jc.msd_count[i];
+        }
+    }
+    return AMDAT_OK;
+}
+
+int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *n_items, int *graph)
+{
+    if (!batch || !batch->job) return set_error(AMDAT_EINVAL, "amdat_batch_job_info: no job has run");
+    if (RS) *RS = batch->job->RS;
+    if (RF) *RF = batch->job->RF;
+    if (n_items) *n_items = batch->job->job.n_items;
+    if (graph) *graph = batch->job->exec != nullptr;
     return AMDAT_OK;
 }
 

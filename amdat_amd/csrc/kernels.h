@@ -45,6 +45,29 @@ struct BatchTraj {
     double *out;        // output rows [group][K]
 };
 
+// One analysis job over a batch (device view): S(k), Fs and MSD/NGP items in
+// one queue.  item = (trajectory, chunk, frame group or gap, kind) with kind
+// 0 = S(k), 1 = Fs; MSD items are (trajectory, gap, -, 2).
+struct BatchJob {
+    const int4 *items = nullptr;
+    int n_items = 0;
+    int *counter = nullptr;
+    const BatchTraj *sk_trajs = nullptr, *fs_trajs = nullptr, *msd_trajs = nullptr;
+    const int *frames = nullptr;
+    int n_frames = 0, group_size = 1;
+    KPlan sk, fs;  // fs: per-trajectory plans concatenated
+    const int *t0 = nullptr, *t1 = nullptr, *group_off = nullptr, *order = nullptr;
+    int ntg = 0;
+    double *s2 = nullptr, *s4 = nullptr;  // [trajectory][gap]
+};
+
+// Launch shape of a job kernel, fixed before any stream capture.
+struct JobLaunch {
+    const void *fn = nullptr;
+    int grid = 0;
+    size_t lds = 0;
+};
+
 // Table half-widths with compiled kernel variants.
 int plan_table_width(int max_component);
 
@@ -93,6 +116,11 @@ hipError_t launch_sk_batch(const BatchTraj *trajs, const int4 *items, int n_item
 // sum over the gap's pairs of sum_atoms cos(k.dr).
 hipError_t launch_fs_batch(const BatchTraj *trajs, const int4 *items, int n_items, int *counter, const int *t0,
                            const int *t1, const int *group_off, const int *order, const KPlan &kp, hipStream_t st);
+// Job kernel for R = RS (S(k) items), RF (Fs items), table width MM:
+// function attributes and the persistent grid for n_items.  Call outside
+// stream capture; enqueue_job (counter reset + launch) may be captured.
+hipError_t prepare_job(int RS, int RF, int MM, int n_items, JobLaunch *out);
+hipError_t enqueue_job(const JobLaunch &L, const BatchJob &j, hipStream_t st);
 // MSD items (trajectory b, pair p): partial[2*(b*n_pairs + p)] = sum r^2, [+1] = sum r^4.
 hipError_t launch_msd_batch(const BatchTraj *trajs, const int2 *items, int n_items, const int *t0, const int *t1,
                             int n_pairs, double *partial, hipStream_t st);

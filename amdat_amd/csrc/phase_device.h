@@ -1,0 +1,308 @@
+// phase_device.h -- device code shared by the phase kernels (kernels.hip,
+// batch_kernels.hip): table geometry, table build, the tile loop and the
+// per-atom phase-fraction functors.
+//
+// Phase kernels (rho, Fs), compensated-FP32 form:
+//   * a block of 512 threads owns one chunk of KB = 512*R k-vectors (R per
+//     thread, kept in registers as LDS addresses) and one frame (rho) or
+//     time-origin pair (Fs), and streams the atoms of its split through LDS
+//     in tiles of A atoms;
+//   * per tile, every atom gets separable phase tables
+//         e_c(m) = exp(2*pi*i * m * u_c),  u_c = frac(r_c / l),
+//     for m in [-M, M] on x and y and [0, M] on z (vectors are stored with
+//     nz >= 0; a vector with nz < 0 is flipped and its result conjugated).
+//     u_c is an exact 32-bit fixed-point fraction formed in FP64 from the
+//     FP32 coordinate (for Fs from the FP64 displacement rebuilt from image
+//     counts), so m*u_c mod 1 is exact integer arithmetic.  Entries are
+//     e(8j + d) = e(8j) * e(d) with both factors from sincos_turns(), so
+//     every entry carries one FP32 complex product over exact anchors;
+//   * the hot loop is per (atom, vector): three LDS reads, all with a
+//     per-lane base and a compile-time offset (the tile loop is fully
+//     unrolled), and two complex products,
+//         exp(i k.r) = e_x(nx) e_y(ny) e_z(nz),
+//     accumulated with FMA into an FP32 partial per tile; tile partials are
+//     folded into a TwoSum-compensated FP32 (hi, lo) pair; the block writes
+//     hi + lo as FP64.
+// The plain-FP64 form evaluates exp(i k.r) with FP64 sincos per update and
+// accumulates in FP64; it is the reference the FP32 form is measured against.
+#pragma once
+
+#include "kernels.h"
+#include "phase_math.h"
+
+namespace amdat {
+namespace {
+
+constexpr int kWaves = kPhaseBlock / 64;
+// Prefetch depth (updates in flight) of the hot loops.
+constexpr int kDepth = 4;
+
+// Table geometry for half-width MM: row of W float2 per atom,
+// x at [0, 2MM], y at [2MM+1, 4MM+1], z (m >= 0 only) at [4MM+2, 5MM+2].
+// A atoms per tile: (A-1)*W*8 stays below 64 KiB so every read is one
+// ds_read_b64 with an immediate offset.  NA anchor slots per table row:
+// e(0..7) and e(8j), j = 0..MM/8.
+template <int MM>
+struct Geom {
+    // at least 257 float2 per row: reads of neighbouring atoms then never
+    // pair into one ds_read2 (8-bit offsets), which keeps register use low
+    static constexpr int W = (5 * MM + 3) > 257 ? (5 * MM + 3) : 257;
+    static constexpr int OX = MM;
+    static constexpr int OY = 2 * MM + 1 + MM;
+    static constexpr int OZ = 2 * (2 * MM + 1);
+    static constexpr int A = (15 * W * 8 < 65536) ? 16 : 8;
+    static constexpr int NA = 8 + MM / 8 + 1;
+    static constexpr size_t tab_bytes = (size_t)A * W * sizeof(float2);
+};
+
+__device__ __forceinline__ void unpack_k(uint32_t kp, int &nx, int &ny, int &nz)
+{
+    nx = (int)(kp & 0xFFu) - 128;
+    ny = (int)((kp >> 8) & 0xFFu) - 128;
+    nz = (int)((kp >> 16) & 0xFFu);
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+
+// Build one tile of tables from the fixed-point fractions sU[3A] (row
+// 3i+c = atom i, axis c; invalid atoms carry sValid[i] == 0 and get zero
+// rows).  Two phases: exact anchors into anc[3A][NA], then products.
+template <int MM>
+__device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uint32_t *sU, const int *sValid, int Mc)
+{
+    using Gm = Geom<MM>;
+    const int tid = threadIdx.x;
+    const int nj = (Mc >> 3) + 1;           // anchors e(8j), j < nj
+    const int nslot = 8 + nj;
+    for (int e = tid; e < 3 * Gm::A * Gm::NA; e += kPhaseBlock) {
+        const int row = e / Gm::NA;
+        const int s = e - row * Gm::NA;
+        if (s < nslot) {
+            const uint32_t m = s < 8 ? (uint32_t)s : 8u * (uint32_t)(s - 8);
+            anc[e] = sincos_turns(sU[row] * m);
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    for (int row = wave; row < 3 * Gm::A; row += kWaves) {
+        const int i = row / 3;
+        const int c = row - 3 * i;
+        const bool valid = sValid[i] != 0;
+        const float2 *a = anc + row * Gm::NA;
+        float2 *dst = tab + i * Gm::W;
+        for (int m = lane; m <= Mc; m += 64) {
+            float2 v = valid ? cmul(a[8 + (m >> 3)], a[m & 7]) : make_float2(0.0f, 0.0f);
+            if (c == 0) {
+                dst[Gm::OX + m] = v;
+                if (m) dst[Gm::OX - m] = make_float2(v.x, -v.y);
+            } else if (c == 1) {
+                dst[Gm::OY + m] = v;
+                if (m) dst[Gm::OY - m] = make_float2(v.x, -v.y);
+            } else {
+                dst[Gm::OZ + m] = v;
+            }
+        }
+    }
+}
+
+// Hot loop over the atom tiles of one block, shared by the single-trajectory
+// and batched phase kernels.  FS = false: rho (complex sums into h0/l0 = Re,
+// h1/l1 = Im); FS = true: Fs (real part only, into h0/l0).  uf(a, c) gives
+// the fixed-point fraction of list position a on axis c.
+template <int R, int MM, bool FS, typename UF>
+__device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int Mc, int a_begin, int a_end,
+                                            float2 *tab, float2 *anc, uint32_t *sU, int *sValid, const UF uf,
+                                            float (&h0)[R], float (&l0)[R], float (&h1)[R], float (&l1)[R])
+{
+    using Gm = Geom<MM>;
+    const int tid = threadIdx.x;
+    const float2 *px[R], *py[R], *pz[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int nx, ny, nz;
+        unpack_k(kp[r * kPhaseBlock + tid], nx, ny, nz);
+        px[r] = tab + Gm::OX + nx;
+        py[r] = tab + Gm::OY + ny;
+        pz[r] = tab + Gm::OZ + nz;
+        h0[r] = l0[r] = h1[r] = l1[r] = 0.0f;
+    }
+    for (int a0 = a_begin; a0 < a_end; a0 += Gm::A) {
+        __syncthreads();
+        if (tid < 3 * Gm::A) {
+            const int i = tid / 3, c = tid - 3 * (tid / 3);
+            const int a = a0 + i;
+            sU[tid] = a < a_end ? uf(a, c) : 0u;
+            if (c == 0) sValid[i] = a < a_end;
+        }
+        __syncthreads();
+        build_tables<MM>(tab, anc, sU, sValid, Mc);
+        __syncthreads();
+        float p0[R], p1[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) p0[r] = p1[r] = 0.0f;
+        // Software pipeline over the flattened (atom, vector) sequence: the
+        // reads of update u + kDepth are issued before update u is computed.
+        float2 bx[kDepth], by[kDepth], bz[kDepth];
+#pragma unroll
+        for (int u = 0; u < kDepth; ++u) {
+            bx[u] = px[u % R][(u / R) * Gm::W];
+            by[u] = py[u % R][(u / R) * Gm::W];
+            bz[u] = pz[u % R][(u / R) * Gm::W];
+        }
+#pragma unroll
+        for (int u = 0; u < Gm::A * R; ++u) {
+            const int r = u % R, slot = u % kDepth;
+            const float2 ex = bx[slot], ey = by[slot], ez = bz[slot];
+            if (u + kDepth < Gm::A * R) {
+                const int v = u + kDepth;
+                bx[slot] = px[v % R][(v / R) * Gm::W];
+                by[slot] = py[v % R][(v / R) * Gm::W];
+                bz[slot] = pz[v % R][(v / R) * Gm::W];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+            const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+            p0[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, p0[r]));
+            if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            two_sum_acc(h0[r], l0[r], p0[r]);
+            if constexpr (!FS) two_sum_acc(h1[r], l1[r], p1[r]);
+        }
+    }
+}
+
+// Fixed-point fraction of a coordinate (rho).
+struct CoordFrac {
+    const float *X, *Y, *Z;
+    const int *list;
+    size_t foff;
+    double inv_ell;
+    __device__ __forceinline__ uint32_t operator()(int a, int c) const
+    {
+        const size_t o = foff + (list ? list[a] : a);
+        const float v = c == 0 ? X[o] : (c == 1 ? Y[o] : Z[o]);
+        return turns_fixed((double)v * inv_ell);
+    }
+};
+
+// Fixed-point fraction of a displacement dr = (x1 + i1*L1) - (x0 + i0*L0),
+// rebuilt exactly in FP64 (Fs).
+struct DispFrac {
+    const float *X, *Y, *Z;
+    const int16_t *IX, *IY, *IZ;
+    const float *L0, *L1;
+    const int *list;
+    size_t o0, o1;
+    double inv_ell;
+    __device__ __forceinline__ uint32_t operator()(int a, int c) const
+    {
+        const int j = list ? list[a] : a;
+        const size_t p0 = o0 + j, p1 = o1 + j;
+        float x0, x1, l0, l1;
+        int i0, i1;
+        if (c == 0) {
+            x0 = X[p0], x1 = X[p1], i0 = IX[p0], i1 = IX[p1], l0 = L0[0], l1 = L1[0];
+        } else if (c == 1) {
+            x0 = Y[p0], x1 = Y[p1], i0 = IY[p0], i1 = IY[p1], l0 = L0[1], l1 = L1[1];
+        } else {
+            x0 = Z[p0], x1 = Z[p1], i0 = IZ[p0], i1 = IZ[p1], l0 = L0[2], l1 = L1[2];
+        }
+        const double d = ((double)x1 - (double)x0) + ((double)i1 * (double)l1 - (double)i0 * (double)l0);
+        return turns_fixed(d * inv_ell);
+    }
+};
+
+// rho over one block's atoms: compensated sums (hr + lr, hi + li) per slot.
+template <int R, int MM>
+__device__ __forceinline__ void rho_block(const float *__restrict__ X, const float *__restrict__ Y,
+                                          const float *__restrict__ Z, const int *__restrict__ list, size_t foff,
+                                          int a_begin, int a_end, double inv_ell, const uint32_t *__restrict__ kp,
+                                          int Mc, float2 *tab, float2 *anc, uint32_t *sU, int *sValid, float (&hr)[R],
+                                          float (&lr)[R], float (&hi)[R], float (&li)[R])
+{
+    const CoordFrac uf{X, Y, Z, list, foff, inv_ell};
+    phase_tiles<R, MM, false>(kp, Mc, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
+}
+
+template <int R>
+__device__ __forceinline__ void rho_write(const int *__restrict__ kd, const float (&hr)[R], const float (&lr)[R],
+                                          const float (&hi)[R], const float (&li)[R], double2 *__restrict__ dst)
+{
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int d = kd[r * kPhaseBlock + threadIdx.x];
+        if (d >= 0) {
+            const double re = (double)hr[r] + (double)lr[r];
+            double im = (double)hi[r] + (double)li[r];
+            if (d & (1 << 30)) im = -im;
+            dst[d & ~(1 << 30)] = make_double2(re, im);
+        }
+    }
+}
+
+// Fs over one block's atoms for the pair (o0, o1): compensated sum of cos(k.dr).
+template <int R, int MM>
+__device__ __forceinline__ void fs_block(const float *__restrict__ X, const float *__restrict__ Y,
+                                         const float *__restrict__ Z, const int16_t *__restrict__ IX,
+                                         const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
+                                         const float *__restrict__ L0, const float *__restrict__ L1,
+                                         const int *__restrict__ list, size_t o0, size_t o1, int a_begin, int a_end,
+                                         double inv_ell, const uint32_t *__restrict__ kp, int Mc, float2 *tab,
+                                         float2 *anc, uint32_t *sU, int *sValid, float (&hs)[R], float (&ls)[R])
+{
+    const DispFrac uf{X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell};
+    float u1[R], u2[R];
+    phase_tiles<R, MM, true>(kp, Mc, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
+}
+
+template <int R>
+__device__ __forceinline__ void fs_write(const int *__restrict__ kd, const float (&hs)[R], const float (&ls)[R],
+                                         double *__restrict__ dst)
+{
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int d = kd[r * kPhaseBlock + threadIdx.x];
+        if (d >= 0) dst[d & ~(1 << 30)] = (double)hs[r] + (double)ls[r];
+    }
+}
+
+#define AMDAT_PHASE_SHARED(MM)                         \
+    extern __shared__ float2 tab[];                    \
+    __shared__ float2 anc[3 * Geom<MM>::A * Geom<MM>::NA]; \
+    __shared__ uint32_t sU[3 * Geom<MM>::A];           \
+    __shared__ int sValid[Geom<MM>::A]
+
+// Next work item of a persistent launch: thread 0 draws from the queue
+// counter and the block reads the value from LDS.
+__device__ __forceinline__ int next_item(int *counter, int *slot)
+{
+    __syncthreads();  // every thread has read the previous value, and the tables are free
+    if (threadIdx.x == 0) *slot = atomicAdd(counter, 1);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*slot);  // same value in every lane: keep it scalar
+}
+
+// Sum over a block of kBlock threads (tree order, deterministic).
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace
+}  // namespace amdat

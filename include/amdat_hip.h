@@ -271,6 +271,33 @@ int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh, c
 int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *pairs, double *s2,
                         double *s4, double *count);
 
+/* One analysis job over a batch in a single persistent launch: S(k) items
+ * (trajectory x vector chunk x frame group), Fs items (trajectory x chunk
+ * x time gap) and MSD/NGP items (trajectory x time gap) share one queue,
+ * costliest first, followed by the per-shell reductions.  Any part may be
+ * left out: sk_shells == NULL (no S(k)), fs_shells == NULL (no Fs),
+ * want_msd == 0.  Arguments and outputs are those of amdat_batch_sk,
+ * amdat_batch_fs and amdat_batch_msd_ngp (pairs serves Fs and MSD).
+ * flags: AMDAT_RUN_GRAPH captures the launch sequence in a hipGraph on the
+ * first call and replays it while the arguments stay the same;
+ * AMDAT_RUN_TUNE picks the vectors per thread of the S(k) and the Fs items
+ * (k-vectors per LDS tile: 1024, 2048 or 4096) by timing each on about an
+ * eighth of the trajectories, once per argument set.  The batch keeps the
+ * planned job (device tables, queue, graph) until another job replaces it
+ * or the batch is freed: free the batch before the shells and pairs its
+ * last job used. */
+#define AMDAT_RUN_GRAPH 1
+#define AMDAT_RUN_TUNE 2
+int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scales,
+                    const amdat_shells *sk_shells, int sk_first, int sk_last, const int *frames,
+                    int n_frames, const amdat_shells *fs_shells, const int *fs_first, const int *fs_last,
+                    const amdat_pairs *pairs, int want_msd, int flags, double *S, double *S_count,
+                    double *Fs, double *Fs_count, double *s2, double *s4, double *msd_count);
+
+/* The last job's vectors per thread for S(k) (RS) and Fs (RF) items, its
+ * number of work items, and whether it runs as a captured graph. */
+int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *n_items, int *graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -159,3 +159,102 @@ def test_batch_rejects_mixed_frame_counts(ctx, members):
     d = _abi.Trajectory(ctx, *other.arrays)
     with pytest.raises(_abi.AmdatError):
         _abi.Batch(ctx, [devs[0], d])
+
+
+_QVEC = {}
+
+
+def qvec_cache():
+    return _QVEC["q"]
+
+
+@pytest.fixture(scope="module")
+def job_args(ctx, members, qvec, sk_shells):
+    _QVEC["q"] = qvec
+    trs, devs, lists, atoms, b = members
+    off, n = ref_shells(qvec, 2, 100)
+    fsh = _abi.Shells(ctx, off, n)
+    first = [int(round(7.25 * t.L / np.pi)) - 3 for t in trs]
+    last = [f + 2 for f in first]
+    tg, t0, t1 = _abi.ts_pairs(NEXP, NES, 0, 0, 0)
+    ntg = _abi.ts_n_timegaps(NEXP, NES, 0)
+    P = _abi.Pairs(ctx, tg, t0, t1, ntg)
+    return dict(fsh=fsh, first=first, last=last, P=P, ell=[t.L for t in trs])
+
+
+def _msd_close(x, y, count):
+    """Same sums to rounding; lag 0 and members without atoms exactly 0."""
+    assert np.all(x[:, 0] == 0.0) and np.all(x[count == 0] == 0.0)
+    nz = count[:, 1:] > 0
+    assert rel(x[:, 1:][nz], y[:, 1:][nz], np.abs(y[:, 1:][nz])) < 1e-12
+
+
+def _separate(b, sk_shells, a):
+    S, Sc = b.sk(sk_shells[2], 0, len(sk_shells[0]) - 2, a["ell"])
+    Fs, Fc = b.fs(a["fsh"], a["first"], a["last"], a["ell"], a["P"])
+    s2, s4, mc = b.msd_ngp(a["P"])
+    return S, Sc, Fs, Fc, s2, s4, mc
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_job_matches_separate_calls(ctx, members, sk_shells, job_args, graph):
+    """One persistent launch for S(k) + Fs + MSD gives what the three
+    separate batched launches give (same items and sum orders)."""
+    trs, devs, lists, atoms, b = members
+    a = job_args
+    S, Sc, Fs, Fc, s2, s4, mc = _separate(b, sk_shells, a)
+    for rep in range(3):          # first call plans (and captures); later calls reuse or replay
+        o = b.run(a["ell"], sk=(sk_shells[2], 0, len(sk_shells[0]) - 2), fs=(a["fsh"], a["first"], a["last"]),
+                  pairs=a["P"], msd=True, graph=graph)
+        info = b.job_info()
+        assert info["graph"] == graph
+        assert np.array_equal(o["S_count"], Sc) and np.array_equal(o["Fs_count"], Fc)
+        assert np.array_equal(o["msd_count"], mc)
+        full = Sc > 0
+        assert rel(o["S"][full], S[full], np.abs(S[full])) < 1e-12
+        assert np.all(o["S"][~full] == 0.0)
+        for i in range(len(trs)):
+            assert rel(o["Fs"][i], Fs[i], 1.0) < 1e-12
+        _msd_close(o["s2"], s2, mc)
+        _msd_close(o["s4"], s4, mc)
+
+
+def test_batch_job_tuned_against_oracle(ctx, members, sk_shells, job_args):
+    trs, devs, lists, atoms, b = members
+    a = job_args
+    off, n, sh = sk_shells
+    o = b.run(a["ell"], sk=(sh, 0, len(off) - 2), fs=(a["fsh"], a["first"], a["last"]), pairs=a["P"], msd=True,
+              graph=True, tune=True)
+    info = b.job_info()
+    print("tuned", info)
+    assert info["RS"] in (2, 4, 8) and info["RF"] in (2, 4, 8)
+    tg, t0, t1 = _abi.ts_pairs(NEXP, NES, 0, 0, 0)
+    ntg = _abi.ts_n_timegaps(NEXP, NES, 0)
+    qoff, qn = ref_shells(qvec_cache(), 2, 100)
+    for i in (0, 5, 17):
+        t = trs[i]
+        S64 = _sk_oracle(t, n, off, np.arange(t.T), atoms[i])
+        assert rel(o["S"][i], S64, np.abs(S64)) < TOL
+        so = qoff[a["first"][i]:a["last"][i] + 2] - qoff[a["first"][i]]
+        nv = qn[qoff[a["first"][i]]:qoff[a["last"][i] + 1]]
+        F64, _ = orc.fs_f64(t, (tg, t0, t1), ntg, so, nv, t.L, atoms=atoms[i])
+        assert rel(o["Fs"][i], F64, 1.0) < TOL
+        m = orc.msd_ngp_f64(t, (tg, t0, t1), ntg, atoms=atoms[i])
+        assert rel(o["s2"][i][1:], m["s2"][1:], np.abs(m["s2"][1:])) < 1e-12
+
+
+def test_batch_job_parts_and_errors(ctx, members, sk_shells, job_args):
+    trs, devs, lists, atoms, b = members
+    a = job_args
+    o = b.run(a["ell"], pairs=a["P"], msd=True)
+    s2, s4, mc = b.msd_ngp(a["P"])
+    _msd_close(o["s2"], s2, mc)
+    _msd_close(o["s4"], s4, mc)
+    o = b.run(a["ell"], fs=(a["fsh"], a["first"], a["last"]), pairs=a["P"])
+    assert set(o) == {"Fs", "Fs_count"}
+    with pytest.raises(_abi.AmdatError):
+        b.run(a["ell"])                                   # nothing asked
+    with pytest.raises(_abi.AmdatError):
+        b.run(a["ell"], msd=True)                         # MSD needs pairs
+    with pytest.raises(_abi.AmdatError):
+        b.run([-1.0] * len(trs), pairs=a["P"], msd=True)  # bad length scale

@@ -1,5 +1,6 @@
-"""Static check of amdat_amd/csrc/kernels.hip: every launch uses the block
-size its kernel was compiled for (__launch_bounds__)."""
+"""Static check of the HIP sources in amdat_amd/csrc: every launch uses the
+block size its kernel was compiled for (__launch_bounds__)."""
+import glob
 import os
 import re
 
@@ -7,7 +8,7 @@ from tests.conftest import ROOT
 
 
 def test_launch_block_sizes_match_bounds():
-    src = open(os.path.join(ROOT, "amdat_amd", "csrc", "kernels.hip")).read()
+    src = "\n".join(open(f).read() for f in sorted(glob.glob(os.path.join(ROOT, "amdat_amd", "csrc", "*.hip"))))
     bounds = {}
     for m in re.finditer(r"__launch_bounds__\((\w+)(?:,\s*\d+)?\)\s+void\s+(\w+)\(", src):
         bounds[m.group(2)] = m.group(1)
