@@ -199,20 +199,21 @@ namespace {
 
 hipStream_t stream_of(amdat_ctx *c) { return c->stream ? c->stream : c->own; }
 
-// Vectors per thread and the resulting chunking for K vectors with max
-// component M: minimise padded work plus table builds (each chunk rebuilds
-// 3(M+1) table entries per atom, about 10 VALU ops each with anchor
-// products, against 8 ops per padded (atom, vector) update).
-int choose_R(int K, int M)
+// Vectors per thread R (k-vectors per block: KB = 512 R) for K vectors.
+// Each (atom, chunk) costs KB padded updates at c VALU ops per lane plus a
+// fixed part (tile staging, table build, barriers, folds) measured at about
+// 12,000 lane-ops: a 14,543-vector S(k) at R = 2 ran 1.63x slower than at
+// R = 8.  c = 9 for rho (complex sums), 6 for Fs (real part only).
+int choose_R(int K, int M, double c = 9.0)
 {
+    (void)M;
     static const int Rs[] = {2, 4, 8};
     int best = 2;
     double best_cost = 1e300;
     for (int R : Rs) {
         const int KB = kPhaseBlock * R;
         const int nch = (K + KB - 1) / KB;
-        // per atom: KB padded updates at 8 ops, 3(M+1) table entries at ~10 ops
-        const double cost = (double)nch * ((double)KB * 8.0 + 30.0 * (M + 1));
+        const double cost = (double)nch * ((double)KB * c + 12000.0);
         if (cost < best_cost * 0.98) { best_cost = cost; best = R; }
     }
     return best;
@@ -227,7 +228,9 @@ int shell_range_M(const amdat_shells *sh, int first, int last)
 }
 
 // R_force > 0 fixes the vectors per thread (batched launches need one R).
-int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned **out, int R_force = 0)
+// c: VALU ops per update of the kernel the plan serves (see choose_R).
+int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned **out, int R_force = 0,
+               double c = 9.0)
 {
     const int v0 = sh->off[first], v1 = sh->off[last + 1];
     const int K = v1 - v0;
@@ -236,7 +239,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
     for (int v = v0; v < v1; v++)
         for (int c = 0; c < 3; c++) M = std::max(M, std::abs(sh->nvec[3 * v + c]));
     if (M > 127) return set_error(AMDAT_ERANGE, "k-vector component beyond +-127");
-    const int R = R_force > 0 ? R_force : choose_R(K, M);
+    const int R = R_force > 0 ? R_force : choose_R(K, M, c);
     auto key = std::make_tuple(first, last, R);
     auto it = sh->plans.find(key);
     if (it != sh->plans.end()) { *out = it->second.get(); return AMDAT_OK; }
@@ -665,7 +668,7 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     KPlanOwned *plan = nullptr;
-    rc = build_plan(ctx, sh, first_shell, last_shell, &plan);
+    rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, 6.0);
     if (rc) return rc;
     const KPlan &kp = plan->kp;
     const int nl = list_n(tr, list);
@@ -960,7 +963,7 @@ int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
     if (M > 127) return set_error(AMDAT_ERANGE, "amdat_batch_fs: k-vector component beyond +-127");
     HIP_TRY(hipSetDevice(ctx->device));
     // one R and one table width for the launch; per-trajectory plans concatenated
-    const int R = choose_R(Kmax, M);
+    const int R = choose_R(Kmax, M, 6.0);
     std::vector<KPlanOwned *> plans(n);
     std::vector<int> chunk_base(n + 1, 0), row_off(n + 1, 0);
     for (int i = 0; i < n; i++) {
@@ -1405,7 +1408,7 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
             if (K <= 0) return set_error(AMDAT_EINVAL, "amdat_batch_run: S(k) shell range holds no vectors");
             RS = choose_R(K, shell_range_M(sk_shells, sk_first, sk_last));
         }
-        if (fs_shells) RF = choose_R(Kfs, Mfs);
+        if (fs_shells) RF = choose_R(Kfs, Mfs, 6.0);
         if (!RS) RS = RF ? RF : 2;
         if (!RF) RF = RS;
         if (flags & AMDAT_RUN_TUNE) {

@@ -50,7 +50,10 @@ struct Geom {
     static constexpr int OX = MM;
     static constexpr int OY = 2 * MM + 1 + MM;
     static constexpr int OZ = 2 * (2 * MM + 1);
-    static constexpr int A = (15 * W * 8 < 65536) ? 16 : 8;
+    static constexpr int A = (31 * W * 8 < 65536) ? 32 : ((15 * W * 8 < 65536) ? 16 : 8);
+    // tiles per fold of the FP32 partials into the compensated sums: one
+    // fold per 32 atoms
+    static constexpr int FT = 32 / A;
     static constexpr int NA = 8 + MM / 8 + 1;
     static constexpr size_t tab_bytes = (size_t)A * W * sizeof(float2);
 };
@@ -130,6 +133,10 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         pz[r] = tab + Gm::OZ + nz;
         h0[r] = l0[r] = h1[r] = l1[r] = 0.0f;
     }
+    float p0[R], p1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) p0[r] = p1[r] = 0.0f;
+    int since_fold = 0;
     for (int a0 = a_begin; a0 < a_end; a0 += Gm::A) {
         __syncthreads();
         if (tid < 3 * Gm::A) {
@@ -141,9 +148,6 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         __syncthreads();
         build_tables<MM>(tab, anc, sU, sValid, Mc);
         __syncthreads();
-        float p0[R], p1[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) p0[r] = p1[r] = 0.0f;
         // Software pipeline over the flattened (atom, vector) sequence: the
         // reads of update u + kDepth are issued before update u is computed.
         float2 bx[kDepth], by[kDepth], bz[kDepth];
@@ -170,10 +174,15 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
             __builtin_amdgcn_sched_barrier(0);
         }
+        // the condition is uniform across the block
+        if (++since_fold == Gm::FT || a0 + Gm::A >= a_end) {
+            since_fold = 0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            two_sum_acc(h0[r], l0[r], p0[r]);
-            if constexpr (!FS) two_sum_acc(h1[r], l1[r], p1[r]);
+            for (int r = 0; r < R; ++r) {
+                two_sum_acc(h0[r], l0[r], p0[r]);
+                if constexpr (!FS) two_sum_acc(h1[r], l1[r], p1[r]);
+                p0[r] = p1[r] = 0.0f;
+            }
         }
     }
 }
