@@ -102,7 +102,8 @@ struct amdat_list {
 struct KPlanOwned {
     KPlan kp;
     DevBuf<uint32_t> kpack;
-    DevBuf<int> kdst, chunkM;
+    DevBuf<int> kdst;
+    DevBuf<int4> range;
 };
 
 // Serial number of a table handle: cached batch jobs compare (address,
@@ -173,7 +174,8 @@ struct JobCache {
     DevBuf<int4> items;
     DevBuf<int> frames, sk_soff, fs_soff, fs_base, fs_nsh, fs_oo;
     DevBuf<uint32_t> fs_kpack;
-    DevBuf<int> fs_kdst, fs_chunkM;
+    DevBuf<int> fs_kdst;
+    DevBuf<int4> fs_range;
     DevBuf<double> sk_rows, fs_rows, sk_norm, fs_norm, S, Fs, s2, s4;
     BatchJob job;
     JobLaunch L;
@@ -246,50 +248,19 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
 
     auto P = std::make_unique<KPlanOwned>();
     KPlan &kp = P->kp;
+    PlanLayout lay = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, R);
     kp.K = K;
     kp.R = R;
-    kp.KB = kPhaseBlock * R;
-    kp.n_chunks = (K + kp.KB - 1) / kp.KB;
+    kp.KB = lay.KB;
+    kp.n_chunks = lay.n_chunks;
     kp.MM = plan_table_width(M);
-    const size_t Kp = (size_t)kp.n_chunks * kp.KB;
-    std::vector<uint32_t> pack(Kp);
-    std::vector<int> dst(Kp), cm(kp.n_chunks);
-    struct V { int nx, ny, nz, d; };
-    for (int c = 0; c < kp.n_chunks; c++) {
-        std::vector<V> vs;
-        const int lo = c * kp.KB, hi = std::min(K, lo + kp.KB);
-        int Mc = 0;
-        for (int v = lo; v < hi; v++) {
-            int nx = sh->nvec[3 * (v0 + v)], ny = sh->nvec[3 * (v0 + v) + 1], nz = sh->nvec[3 * (v0 + v) + 2];
-            int d = v;
-            if (nz < 0) { nx = -nx; ny = -ny; nz = -nz; d |= (1 << 30); }
-            Mc = std::max(Mc, std::max(std::abs(nx), std::max(std::abs(ny), nz)));
-            vs.push_back({nx, ny, nz, d});
-        }
-        std::stable_sort(vs.begin(), vs.end(), [](const V &a, const V &b) {
-            if (a.nx != b.nx) return a.nx < b.nx;
-            if (a.ny != b.ny) return a.ny < b.ny;
-            return a.nz < b.nz;
-        });
-        cm[c] = Mc;
-        for (int q = 0; q < kp.KB; q++) {
-            const size_t pos = (size_t)c * kp.KB + q;
-            if (q < (int)vs.size()) {
-                pack[pos] = (uint32_t)(vs[q].nx + 128) | ((uint32_t)(vs[q].ny + 128) << 8) | ((uint32_t)vs[q].nz << 16);
-                dst[pos] = vs[q].d;
-            } else {
-                pack[pos] = 128u | (128u << 8);
-                dst[pos] = -1;
-            }
-        }
-    }
     hipStream_t st = stream_of(ctx);
-    HIP_TRY(P->kpack.upload(pack.data(), pack.size(), st));
-    HIP_TRY(P->kdst.upload(dst.data(), dst.size(), st));
-    HIP_TRY(P->chunkM.upload(cm.data(), cm.size(), st));
+    HIP_TRY(P->kpack.upload(lay.pack.data(), lay.pack.size(), st));
+    HIP_TRY(P->kdst.upload(lay.dst.data(), lay.dst.size(), st));
+    HIP_TRY(P->range.upload((const int4 *)lay.range.data(), lay.n_chunks, st));
     kp.kpack = P->kpack.p;
     kp.kdst = P->kdst.p;
-    kp.chunkM = P->chunkM.p;
+    kp.range = P->range.p;
     *out = P.get();
     sh->plans[key] = std::move(P);
     return AMDAT_OK;
@@ -979,20 +950,21 @@ int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
     cat.K = 0;
     hipStream_t st = stream_of(ctx);
     DevBuf<uint32_t> kpack;
-    DevBuf<int> kdst, chunkM;
+    DevBuf<int> kdst;
+    DevBuf<int4> range;
     HIP_TRY(kpack.alloc((size_t)cat.n_chunks * cat.KB));
     HIP_TRY(kdst.alloc((size_t)cat.n_chunks * cat.KB));
-    HIP_TRY(chunkM.alloc(cat.n_chunks));
+    HIP_TRY(range.alloc(cat.n_chunks));
     for (int i = 0; i < n; i++) {
         const KPlan &p = plans[i]->kp;
         const size_t o = (size_t)chunk_base[i] * cat.KB, m = (size_t)p.n_chunks * cat.KB;
         HIP_TRY(hipMemcpyAsync(kpack.p + o, p.kpack, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
         HIP_TRY(hipMemcpyAsync(kdst.p + o, p.kdst, m * sizeof(int), hipMemcpyDeviceToDevice, st));
-        HIP_TRY(hipMemcpyAsync(chunkM.p + chunk_base[i], p.chunkM, p.n_chunks * sizeof(int), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(range.p + chunk_base[i], p.range, p.n_chunks * sizeof(int4), hipMemcpyDeviceToDevice, st));
     }
     cat.kpack = kpack.p;
     cat.kdst = kdst.p;
-    cat.chunkM = chunkM.p;
+    cat.range = range.p;
     // rows [gap][K_b] per trajectory; shells, norms and output offsets
     std::vector<int> soff, shell_base(n), nsh(n), out_off(n + 1, 0);
     std::vector<double> norm((size_t)n * ntg, 0.0);
@@ -1185,18 +1157,18 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
         fsp.n_chunks = chunk_base[n];
         HIP_TRY(jc.fs_kpack.alloc((size_t)fsp.n_chunks * fsp.KB));
         HIP_TRY(jc.fs_kdst.alloc((size_t)fsp.n_chunks * fsp.KB));
-        HIP_TRY(jc.fs_chunkM.alloc(fsp.n_chunks));
+        HIP_TRY(jc.fs_range.alloc(fsp.n_chunks));
         for (int b = 0; b < n; b++) {
             const KPlan &p = plans[b]->kp;
             const size_t o = (size_t)chunk_base[b] * fsp.KB, m = (size_t)p.n_chunks * fsp.KB;
             HIP_TRY(hipMemcpyAsync(jc.fs_kpack.p + o, p.kpack, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
             HIP_TRY(hipMemcpyAsync(jc.fs_kdst.p + o, p.kdst, m * sizeof(int), hipMemcpyDeviceToDevice, st));
-            HIP_TRY(hipMemcpyAsync(jc.fs_chunkM.p + chunk_base[b], p.chunkM, p.n_chunks * sizeof(int),
+            HIP_TRY(hipMemcpyAsync(jc.fs_range.p + chunk_base[b], p.range, p.n_chunks * sizeof(int4),
                                    hipMemcpyDeviceToDevice, st));
         }
         fsp.kpack = jc.fs_kpack.p;
         fsp.kdst = jc.fs_kdst.p;
-        fsp.chunkM = jc.fs_chunkM.p;
+        fsp.range = jc.fs_range.p;
         std::vector<int> soff, base(n), nsh(n);
         jc.out_off.assign(n + 1, 0);
         std::vector<double> norm((size_t)n * ntg, 0.0);

@@ -1,7 +1,10 @@
 // abi_internal.h -- shared internals of the C ABI implementation.
 #pragma once
 
+#include <stdint.h>
+
 #include <string>
+#include <vector>
 
 namespace amdat {
 
@@ -9,4 +12,30 @@ namespace amdat {
 int set_error(int code, const std::string &msg);
 void clear_error();
 
+// Block layout of a k-vector plan (host side of the phase kernels).
+// Vectors are canonicalised (a vector with nz < 0 is flipped and flagged
+// with bit 30 of its destination), sorted by (nx, ny, nz), and dealt into
+// thread units of R slots that all share one nx (a thread then reads
+// e_x(nx) once per atom for its R vectors).  Units of one nx take the
+// group's sorted vectors with stride (units in the group), so the lanes of
+// a wave read neighbouring table entries; each unit's slots are then
+// permuted so the 32 lanes of a half-wave read y and z entries in distinct
+// LDS bank pairs where possible.  Chunks are 512 consecutive units; unused
+// slots carry (nx of their unit, 0, 0) and destination -1.
+struct PlanLayout {
+    int R = 0, KB = 0, n_chunks = 0;
+    std::vector<uint32_t> pack;  // [n_chunks][R][512]: (nx+128) | (ny+128)<<8 | nz<<16
+    std::vector<int> dst;        // same layout: destination index | bit 30 (conjugate), or -1
+    std::vector<int> range;      // [n_chunks][4]: nx lo, nx hi, max |ny| | max nz << 16, units with vectors
+    long conflicts = 0;          // extra LDS passes summed over (chunk, slot, half-wave, axis)
+};
+PlanLayout plan_layout(const int *nvec, int K, int R);
+
 }  // namespace amdat
+
+extern "C" {
+// Test hook: the layout of plan_layout for K vectors nvec[K][3].  With
+// pack == NULL only *n_chunks is set.  Returns 0 or a negative status.
+int amdat_internal_plan_layout(const int *nvec, int K, int R, uint32_t *pack, int *dst, int *range,
+                               int *n_chunks, long *conflicts);
+}

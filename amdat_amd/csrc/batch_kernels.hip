@@ -25,7 +25,7 @@ __device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, co
     for (int f = f_begin; f < f_end; ++f) {
         float hr[R], lr[R], hi[R], li[R];
         rho_block<R, MM>(t.x, t.y, t.z, t.list, (size_t)frames[f] * (size_t)t.N, 0, t.n_list, t.inv_ell,
-                         kp.kpack + (size_t)chunk * kp.KB, kp.chunkM[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
+                         kp.kpack + (size_t)chunk * kp.KB, kp.range[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int d = kd[r * kPhaseBlock + threadIdx.x];
@@ -58,7 +58,7 @@ __device__ __forceinline__ void fs_item(const BatchTraj &t, int chunk, int g, co
         float hs[R], ls[R];
         fs_block<R, MM>(t.x, t.y, t.z, t.ix, t.iy, t.iz, t.box + 3 * t0, t.box + 3 * t1, t.list, (size_t)t0 * t.N,
                         (size_t)t1 * t.N, 0, t.n_list, t.inv_ell, kp.kpack + (size_t)chunk * kp.KB,
-                        kp.chunkM[chunk], tab, anc, sU, sValid, hs, ls);
+                        kp.range[chunk], tab, anc, sU, sValid, hs, ls);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int d = kd[r * kPhaseBlock + threadIdx.x];

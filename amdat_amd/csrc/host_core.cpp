@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -238,3 +240,163 @@ int amdat_shells_generate(int first_bin, int n_bins, int max_per_shell, uint64_t
 }
 
 }  // extern "C"
+
+// ---- block layout of k-vector plans -------------------------------------
+
+namespace amdat {
+
+namespace {
+
+constexpr int kLanesPerPass = 32;  // a wave's 8-byte LDS reads are served 32 lanes at a time
+constexpr int kBankPairs = 32;     // 64 banks of 4 bytes = 32 eight-byte entries
+
+struct Vec {
+    int nx, ny, nz, d;
+};
+
+// Extra LDS passes of one read instruction: the largest number of distinct
+// entries falling into one bank pair, minus one, over the lanes of a pass.
+long passes_minus_one(const std::vector<int> &vals)
+{
+    std::array<std::vector<int>, kBankPairs> seen;
+    int worst = 0;
+    for (int v : vals) {
+        auto &s = seen[((v % kBankPairs) + kBankPairs) % kBankPairs];
+        if (std::find(s.begin(), s.end(), v) == s.end()) s.push_back(v);
+        worst = std::max(worst, (int)s.size());
+    }
+    return worst > 0 ? worst - 1 : 0;
+}
+
+}  // namespace
+
+PlanLayout plan_layout(const int *nvec, int K, int R)
+{
+    PlanLayout L;
+    L.R = R;
+    L.KB = 512 * R;
+    std::vector<Vec> vs(K);
+    for (int v = 0; v < K; v++) {
+        int nx = nvec[3 * v], ny = nvec[3 * v + 1], nz = nvec[3 * v + 2], d = v;
+        if (nz < 0) { nx = -nx; ny = -ny; nz = -nz; d |= (1 << 30); }
+        vs[v] = {nx, ny, nz, d};
+    }
+    std::stable_sort(vs.begin(), vs.end(), [](const Vec &a, const Vec &b) {
+        if (a.nx != b.nx) return a.nx < b.nx;
+        if (a.ny != b.ny) return a.ny < b.ny;
+        return a.nz < b.nz;
+    });
+    // thread units: per nx group, stride-dealt
+    std::vector<std::vector<Vec>> units;
+    for (int g0 = 0; g0 < K;) {
+        int g1 = g0;
+        while (g1 < K && vs[g1].nx == vs[g0].nx) g1++;
+        const int cnt = g1 - g0, U = (cnt + R - 1) / R;
+        for (int u = 0; u < U; u++) {
+            std::vector<Vec> un;
+            for (int q = u; q < cnt; q += U) un.push_back(vs[g0 + q]);
+            while ((int)un.size() < R) un.push_back({vs[g0].nx, 0, 0, -1});
+            units.push_back(un);
+        }
+        g0 = g1;
+    }
+    const int n_units = (int)units.size();
+    L.n_chunks = std::max(1, (n_units + 511) / 512);
+    while ((int)units.size() < L.n_chunks * 512) {
+        const int nx = units.empty() ? 0 : units.back()[0].nx;
+        units.push_back(std::vector<Vec>(R, Vec{nx, 0, 0, -1}));
+    }
+    // slot permutation per unit, greedy over each half-wave of units
+    for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) {
+        // per slot r: values placed in each bank pair, for y and for z
+        std::vector<std::array<std::vector<int>, kBankPairs>> by(R), bz(R);
+        for (size_t u = h0; u < h0 + kLanesPerPass && u < units.size(); u++) {
+            std::vector<Vec> &un = units[u];
+            std::vector<Vec> placed(R);
+            std::vector<char> used_v(R, 0), used_r(R, 0);
+            auto cost = [&](const Vec &v, int r) {
+                auto &sy = by[r][((v.ny % kBankPairs) + kBankPairs) % kBankPairs];
+                auto &sz = bz[r][v.nz % kBankPairs];
+                int c = 0;
+                if (std::find(sy.begin(), sy.end(), v.ny) == sy.end()) c += (int)sy.size();
+                if (std::find(sz.begin(), sz.end(), v.nz) == sz.end()) c += (int)sz.size();
+                return c;
+            };
+            for (int k = 0; k < R; k++) {
+                int bv = -1, br = -1, bc = 1 << 30;
+                for (int i = 0; i < R; i++) {
+                    if (used_v[i]) continue;
+                    for (int r = 0; r < R; r++) {
+                        if (used_r[r]) continue;
+                        const int c = cost(un[i], r);
+                        if (c < bc) { bc = c; bv = i; br = r; }
+                    }
+                }
+                used_v[bv] = used_r[br] = 1;
+                placed[br] = un[bv];
+                const Vec &v = un[bv];
+                auto &sy = by[br][((v.ny % kBankPairs) + kBankPairs) % kBankPairs];
+                if (std::find(sy.begin(), sy.end(), v.ny) == sy.end()) sy.push_back(v.ny);
+                auto &sz = bz[br][v.nz % kBankPairs];
+                if (std::find(sz.begin(), sz.end(), v.nz) == sz.end()) sz.push_back(v.nz);
+            }
+            un.swap(placed);
+        }
+    }
+    // emit
+    const size_t n = (size_t)L.n_chunks * L.KB;
+    L.pack.resize(n);
+    L.dst.resize(n);
+    L.range.assign((size_t)L.n_chunks * 4, 0);
+    for (int c = 0; c < L.n_chunks; c++) {
+        int xlo = 1 << 30, xhi = -(1 << 30), my = 0, mz = 0;
+        for (int t = 0; t < 512; t++) {
+            const std::vector<Vec> &un = units[(size_t)c * 512 + t];
+            for (int r = 0; r < R; r++) {
+                const Vec &v = un[r];
+                const size_t pos = (size_t)c * L.KB + (size_t)r * 512 + t;
+                L.pack[pos] = (uint32_t)(v.nx + 128) | ((uint32_t)(v.ny + 128) << 8) | ((uint32_t)v.nz << 16);
+                L.dst[pos] = v.d;
+                xlo = std::min(xlo, v.nx);
+                xhi = std::max(xhi, v.nx);
+                my = std::max(my, std::abs(v.ny));
+                mz = std::max(mz, v.nz);
+            }
+        }
+        L.range[4 * c] = xlo;
+        L.range[4 * c + 1] = xhi;
+        L.range[4 * c + 2] = my | (mz << 16);
+        L.range[4 * c + 3] = std::min(512, n_units - c * 512);
+        // conflict estimate of the chunk's reads
+        for (int r = 0; r < R; r++)
+            for (int h = 0; h < 512; h += kLanesPerPass) {
+                std::vector<int> ys, zs, xs;
+                for (int t = h; t < h + kLanesPerPass; t++) {
+                    const Vec &v = units[(size_t)c * 512 + t][r];
+                    ys.push_back(v.ny);
+                    zs.push_back(v.nz);
+                    if (r == 0) xs.push_back(v.nx);
+                }
+                L.conflicts += passes_minus_one(ys) + passes_minus_one(zs);
+                if (r == 0) L.conflicts += passes_minus_one(xs);
+            }
+    }
+    return L;
+}
+
+}  // namespace amdat
+
+extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, uint32_t *pack, int *dst, int *range,
+                                          int *n_chunks, long *conflicts)
+{
+    if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks) return AMDAT_EINVAL;
+    amdat::PlanLayout L = amdat::plan_layout(nvec, K, R);
+    *n_chunks = L.n_chunks;
+    if (conflicts) *conflicts = L.conflicts;
+    if (pack) {
+        std::copy(L.pack.begin(), L.pack.end(), pack);
+        std::copy(L.dst.begin(), L.dst.end(), dst);
+        std::copy(L.range.begin(), L.range.end(), range);
+    }
+    return AMDAT_OK;
+}

@@ -17,11 +17,10 @@ struct TrajDev {
     const float *box = nullptr;  // device [T][3]
 };
 
-// Device k-vector plan for one shell range.  Vectors are cut into chunks
-// of KB = kPhaseBlock*R in their original order, each chunk sorted by
-// (nx, ny, nz) so a wave's lanes read nearby table entries, and padded
-// with (0,0,0) entries whose destination is -1.  Thread t of a block owns
-// positions r*kPhaseBlock + t, r < R, of its chunk.
+// Device k-vector plan for one shell range (layout built on the host by
+// plan_layout): chunks of KB = kPhaseBlock*R slots; thread t of a block
+// owns positions r*kPhaseBlock + t, r < R, of its chunk, all with one nx;
+// unused slots have destination -1.
 struct KPlan {
     int K = 0;         // real vectors
     int R = 0;         // vectors per thread
@@ -30,7 +29,7 @@ struct KPlan {
     int MM = 0;        // table half-width the kernels were built for (max |n_c| <= MM)
     uint32_t *kpack = nullptr;  // [n_chunks*KB] (nx+128) | (ny+128)<<8 | nz<<16
     int *kdst = nullptr;        // [n_chunks*KB] destination index, -1 padding, bit 30 = conjugate
-    int *chunkM = nullptr;      // [n_chunks] max |component| in the chunk
+    int4 *range = nullptr;      // [n_chunks] (nx lo, nx hi, max|ny| | max nz << 16, units with vectors)
 };
 
 // One trajectory of a batch (device-side table entry).

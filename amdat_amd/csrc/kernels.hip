@@ -17,7 +17,7 @@ template <int R, int MM>
 __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
     const int *__restrict__ list, int n_list, const int *__restrict__ frames,
-    const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, const int *__restrict__ chunkM,
+    const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, const int4 *__restrict__ range,
     int K, int KB, double inv_ell, int per_split, int n_splits, double2 *__restrict__ out)
 {
     AMDAT_PHASE_SHARED(MM);
@@ -26,7 +26,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const int a_end = min(n_list, a_begin + per_split);
     float hr[R], lr[R], hi[R], li[R];
     rho_block<R, MM>(X, Y, Z, list, (size_t)frames[fslot] * (size_t)N, a_begin, a_end, inv_ell,
-                     kpack + (size_t)chunk * KB, chunkM[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
+                     kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
     rho_write<R>(kdst + (size_t)chunk * KB, hr, lr, hi, li, out + ((size_t)fslot * n_splits + split) * (size_t)K);
 }
 
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
     const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
     const int *__restrict__ T0, const int *__restrict__ T1, const uint32_t *__restrict__ kpack,
-    const int *__restrict__ kdst, const int *__restrict__ chunkM, int K, int KB, double inv_ell,
+    const int *__restrict__ kdst, const int4 *__restrict__ range, int K, int KB, double inv_ell,
     int per_split, int n_splits, double *__restrict__ out)
 {
     AMDAT_PHASE_SHARED(MM);
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const int a_end = min(n_list, a_begin + per_split);
     float hs[R], ls[R];
     fs_block<R, MM>(X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N, a_begin,
-                    a_end, inv_ell, kpack + (size_t)chunk * KB, chunkM[chunk], tab, anc, sU, sValid, hs, ls);
+                    a_end, inv_ell, kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, hs, ls);
     fs_write<R>(kdst + (size_t)chunk * KB, hs, ls, out + ((size_t)pslot * n_splits + split) * (size_t)K);
 }
 
@@ -384,7 +384,7 @@ hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, cons
     (void)hipGetLastError();
 
     hipLaunchKernelGGL(kfn, grid, dim3(kPhaseBlock), lds, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames,
-                       kp.kpack, kp.kdst, kp.chunkM, kp.K, kp.KB, inv_ell, per_split, n_splits, out);
+                       kp.kpack, kp.kdst, kp.range, kp.K, kp.KB, inv_ell, per_split, n_splits, out);
     return hipGetLastError();
 }
 
@@ -402,7 +402,7 @@ hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const
     (void)hipGetLastError();
 
     hipLaunchKernelGGL(kfn, grid, dim3(kPhaseBlock), lds, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box,
-                       tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.chunkM, kp.K, kp.KB, inv_ell,
+                       tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.range, kp.K, kp.KB, inv_ell,
                        per_split, n_splits, out);
     return hipGetLastError();
 }

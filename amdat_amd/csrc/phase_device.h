@@ -40,8 +40,7 @@ constexpr int kDepth = 4;
 // Table geometry for half-width MM: row of W float2 per atom,
 // x at [0, 2MM], y at [2MM+1, 4MM+1], z (m >= 0 only) at [4MM+2, 5MM+2].
 // A atoms per tile: (A-1)*W*8 stays below 64 KiB so every read is one
-// ds_read_b64 with an immediate offset.  NA anchor slots per table row:
-// e(0..7) and e(8j), j = 0..MM/8.
+// ds_read_b64 with an immediate offset.
 template <int MM>
 struct Geom {
     // at least 257 float2 per row: reads of neighbouring atoms then never
@@ -54,7 +53,6 @@ struct Geom {
     // tiles per fold of the FP32 partials into the compensated sums: one
     // fold per 32 atoms
     static constexpr int FT = 32 / A;
-    static constexpr int NA = 8 + MM / 8 + 1;
     static constexpr size_t tab_bytes = (size_t)A * W * sizeof(float2);
 };
 
@@ -70,44 +68,54 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
 
+// Per-chunk range descriptor (int4): x = lowest nx, y = highest nx,
+// z = max |ny| | max nz << 16, w = thread units holding vectors (waves past
+// them skip the hot loop).
+__device__ __forceinline__ int range_my(int4 r) { return r.z & 0xFFFF; }
+__device__ __forceinline__ int range_mz(int4 r) { return r.z >> 16; }
+
 // Build one tile of tables from the fixed-point fractions sU[3A] (row
 // 3i+c = atom i, axis c; invalid atoms carry sValid[i] == 0 and get zero
-// rows).  Two phases: exact anchors into anc[3A][NA], then products.
+// rows), only over the entries the chunk reads: x in [xlo, xhi], y in
+// [-my, my], z in [0, mz].  Phase 1: exact small anchors e(d), d < 8, per
+// row into anc[3A][8].  Phase 2: one work item per (row, j) computes the
+// exact anchor e(8j) and the entries e(8j + d) = e(8j) e(d), writing each
+// at +m and its conjugate at -m where the range holds them.
 template <int MM>
-__device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uint32_t *sU, const int *sValid, int Mc)
+__device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uint32_t *sU, const int *sValid, int4 rng)
 {
     using Gm = Geom<MM>;
     const int tid = threadIdx.x;
-    const int nj = (Mc >> 3) + 1;           // anchors e(8j), j < nj
-    const int nslot = 8 + nj;
-    for (int e = tid; e < 3 * Gm::A * Gm::NA; e += kPhaseBlock) {
-        const int row = e / Gm::NA;
-        const int s = e - row * Gm::NA;
-        if (s < nslot) {
-            const uint32_t m = s < 8 ? (uint32_t)s : 8u * (uint32_t)(s - 8);
-            anc[e] = sincos_turns(sU[row] * m);
-        }
-    }
+    const int xlo = rng.x, xhi = rng.y, my = range_my(rng), mz = range_mz(rng);
+    for (int e = tid; e < 3 * Gm::A * 8; e += kPhaseBlock) anc[e] = sincos_turns(sU[e >> 3] * (uint32_t)(e & 7));
     __syncthreads();
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    for (int row = wave; row < 3 * Gm::A; row += kWaves) {
-        const int i = row / 3;
-        const int c = row - 3 * i;
+    // |m| needed on x: [xa, xb]
+    const int xa = xlo >= 0 ? xlo : (xhi <= 0 ? -xhi : 0);
+    const int xb = max(abs(xlo), abs(xhi));
+    const int jx0 = xa >> 3, njx = (xb >> 3) - jx0 + 1;
+    const int njy = (my >> 3) + 1, njz = (mz >> 3) + 1;
+    const int nj = njx + njy + njz;
+    for (int it = tid; it < Gm::A * nj; it += kPhaseBlock) {
+        const int i = it / nj, k = it - i * nj;
+        int c, j, lo, hi, off;
+        if (k < njx) {
+            c = 0; j = jx0 + k; lo = xlo; hi = xhi; off = Gm::OX;
+        } else if (k < njx + njy) {
+            c = 1; j = k - njx; lo = -my; hi = my; off = Gm::OY;
+        } else {
+            c = 2; j = k - njx - njy; lo = 0; hi = mz; off = Gm::OZ;
+        }
+        const int row = 3 * i + c;
         const bool valid = sValid[i] != 0;
-        const float2 *a = anc + row * Gm::NA;
-        float2 *dst = tab + i * Gm::W;
-        for (int m = lane; m <= Mc; m += 64) {
-            float2 v = valid ? cmul(a[8 + (m >> 3)], a[m & 7]) : make_float2(0.0f, 0.0f);
-            if (c == 0) {
-                dst[Gm::OX + m] = v;
-                if (m) dst[Gm::OX - m] = make_float2(v.x, -v.y);
-            } else if (c == 1) {
-                dst[Gm::OY + m] = v;
-                if (m) dst[Gm::OY - m] = make_float2(v.x, -v.y);
-            } else {
-                dst[Gm::OZ + m] = v;
-            }
+        const float2 a8 = sincos_turns(sU[row] * (uint32_t)(8 * j));
+        const float2 *small = anc + row * 8;
+        float2 *dst = tab + i * Gm::W + off;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            const int m = 8 * j + d;
+            const float2 v = valid ? cmul(a8, small[d]) : make_float2(0.0f, 0.0f);
+            if (m >= lo && m <= hi) dst[m] = v;
+            if (m > 0 && -m >= lo && -m <= hi) dst[-m] = make_float2(v.x, -v.y);
         }
     }
 }
@@ -115,20 +123,23 @@ __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uin
 // Hot loop over the atom tiles of one block, shared by the single-trajectory
 // and batched phase kernels.  FS = false: rho (complex sums into h0/l0 = Re,
 // h1/l1 = Im); FS = true: Fs (real part only, into h0/l0).  uf(a, c) gives
-// the fixed-point fraction of list position a on axis c.
+// the fixed-point fraction of list position a on axis c.  The R vectors of
+// a thread share nx: e_x(nx) is read once per atom.
 template <int R, int MM, bool FS, typename UF>
-__device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int Mc, int a_begin, int a_end,
+__device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
                                             float2 *tab, float2 *anc, uint32_t *sU, int *sValid, const UF uf,
                                             float (&h0)[R], float (&l0)[R], float (&h1)[R], float (&l1)[R])
 {
     using Gm = Geom<MM>;
     const int tid = threadIdx.x;
-    const float2 *px[R], *py[R], *pz[R];
+    // waves whose units all lie past the chunk's last real unit skip the sums
+    const bool active = __builtin_amdgcn_readfirstlane(tid >> 6) * 64 < rng.w;
+    const float2 *px = nullptr, *py[R], *pz[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int nx, ny, nz;
         unpack_k(kp[r * kPhaseBlock + tid], nx, ny, nz);
-        px[r] = tab + Gm::OX + nx;
+        if (r == 0) px = tab + Gm::OX + nx;
         py[r] = tab + Gm::OY + ny;
         pz[r] = tab + Gm::OZ + nz;
         h0[r] = l0[r] = h1[r] = l1[r] = 0.0f;
@@ -146,33 +157,36 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             if (c == 0) sValid[i] = a < a_end;
         }
         __syncthreads();
-        build_tables<MM>(tab, anc, sU, sValid, Mc);
+        build_tables<MM>(tab, anc, sU, sValid, rng);
         __syncthreads();
+        if (!active) continue;
         // Software pipeline over the flattened (atom, vector) sequence: the
-        // reads of update u + kDepth are issued before update u is computed.
-        float2 bx[kDepth], by[kDepth], bz[kDepth];
+        // y and z reads of update u + kDepth, and the x read of the next
+        // atom, are issued before update u is computed.
+        float2 by[kDepth], bz[kDepth];
 #pragma unroll
         for (int u = 0; u < kDepth; ++u) {
-            bx[u] = px[u % R][(u / R) * Gm::W];
             by[u] = py[u % R][(u / R) * Gm::W];
             bz[u] = pz[u % R][(u / R) * Gm::W];
         }
-#pragma unroll
+        float2 ex = px[0], ex_next = ex;
+#pragma clang loop unroll(full)
         for (int u = 0; u < Gm::A * R; ++u) {
-            const int r = u % R, slot = u % kDepth;
-            const float2 ex = bx[slot], ey = by[slot], ez = bz[slot];
+            const int a = u / R, r = u % R, slot = u % kDepth;
+            const float2 ey = by[slot], ez = bz[slot];
             if (u + kDepth < Gm::A * R) {
                 const int v = u + kDepth;
-                bx[slot] = px[v % R][(v / R) * Gm::W];
                 by[slot] = py[v % R][(v / R) * Gm::W];
                 bz[slot] = pz[v % R][(v / R) * Gm::W];
             }
+            if (r == 0 && a + 1 < Gm::A) ex_next = px[(a + 1) * Gm::W];
             __builtin_amdgcn_sched_barrier(0);
             const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
             const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
             p0[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, p0[r]));
             if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
             __builtin_amdgcn_sched_barrier(0);
+            if (r == R - 1) ex = ex_next;
         }
         // the condition is uniform across the block
         if (++since_fold == Gm::FT || a0 + Gm::A >= a_end) {
@@ -233,11 +247,11 @@ template <int R, int MM>
 __device__ __forceinline__ void rho_block(const float *__restrict__ X, const float *__restrict__ Y,
                                           const float *__restrict__ Z, const int *__restrict__ list, size_t foff,
                                           int a_begin, int a_end, double inv_ell, const uint32_t *__restrict__ kp,
-                                          int Mc, float2 *tab, float2 *anc, uint32_t *sU, int *sValid, float (&hr)[R],
+                                          int4 rng, float2 *tab, float2 *anc, uint32_t *sU, int *sValid, float (&hr)[R],
                                           float (&lr)[R], float (&hi)[R], float (&li)[R])
 {
     const CoordFrac uf{X, Y, Z, list, foff, inv_ell};
-    phase_tiles<R, MM, false>(kp, Mc, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
+    phase_tiles<R, MM, false>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
 }
 
 template <int R>
@@ -263,12 +277,12 @@ __device__ __forceinline__ void fs_block(const float *__restrict__ X, const floa
                                          const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
                                          const float *__restrict__ L0, const float *__restrict__ L1,
                                          const int *__restrict__ list, size_t o0, size_t o1, int a_begin, int a_end,
-                                         double inv_ell, const uint32_t *__restrict__ kp, int Mc, float2 *tab,
+                                         double inv_ell, const uint32_t *__restrict__ kp, int4 rng, float2 *tab,
                                          float2 *anc, uint32_t *sU, int *sValid, float (&hs)[R], float (&ls)[R])
 {
     const DispFrac uf{X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell};
     float u1[R], u2[R];
-    phase_tiles<R, MM, true>(kp, Mc, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
+    phase_tiles<R, MM, true>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
 }
 
 template <int R>
@@ -284,7 +298,7 @@ __device__ __forceinline__ void fs_write(const int *__restrict__ kd, const float
 
 #define AMDAT_PHASE_SHARED(MM)                         \
     extern __shared__ float2 tab[];                    \
-    __shared__ float2 anc[3 * Geom<MM>::A * Geom<MM>::NA]; \
+    __shared__ float2 anc[3 * Geom<MM>::A * 8];          \
     __shared__ uint32_t sU[3 * Geom<MM>::A];           \
     __shared__ int sValid[Geom<MM>::A]
 

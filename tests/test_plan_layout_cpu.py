@@ -1,0 +1,92 @@
+"""Host layout of k-vector plans (plan_layout, host C++ behind the
+amdat_internal_plan_layout hook): every vector lands exactly once with its
+conjugation flag, a thread's R slots share nx, the per-chunk ranges cover
+every slot the phase kernels read, and units past the active count are
+padding only."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from amdat_amd import _abi
+from tests.conftest import ref_shells
+
+
+def layout(nvec, R):
+    L = _abi.lib()
+    fn = L.amdat_internal_plan_layout
+    fn.restype = C.c_int
+    nv = np.ascontiguousarray(nvec, np.int32).reshape(-1)
+    K = len(nv) // 3
+    nc, cf = C.c_int(), C.c_long()
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+    assert fn(P(nv, C.c_int), K, R, None, None, None, C.byref(nc), C.byref(cf)) == 0
+    n, KB = nc.value, 512 * R
+    pack = np.zeros(n * KB, np.uint32)
+    dst = np.zeros(n * KB, np.int32)
+    rng = np.zeros(n * 4, np.int32)
+    assert fn(P(nv, C.c_int), K, R, P(pack, C.c_uint32), P(dst, C.c_int), P(rng, C.c_int), C.byref(nc),
+              C.byref(cf)) == 0
+    nx = (pack & 0xFF).astype(np.int64) - 128
+    ny = ((pack >> 8) & 0xFF).astype(np.int64) - 128
+    nz = ((pack >> 16) & 0xFF).astype(np.int64)
+    return n, nx.reshape(n, R, 512), ny.reshape(n, R, 512), nz.reshape(n, R, 512), dst.reshape(n, R, 512), \
+        rng.reshape(n, 4), cf.value
+
+
+@pytest.mark.parametrize("bins,R", [((2, 100), 8), ((2, 100), 4), ((2, 40), 2), ((40, 41), 8), ((2, 3), 2),
+                                    ((95, 100), 4)])
+def test_layout_invariants(qvec, bins, R):
+    off, nvec = ref_shells(qvec, *bins)
+    K = len(nvec)
+    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R)
+    assert n == max(1, -(-int(np.sum([-(-c // R) for c in np.unique(
+        np.where(nvec[:, 2] < 0, -nvec[:, 0], nvec[:, 0]), return_counts=True)[1]])) // 512))
+    real = dst >= 0
+    idx = dst[real] & ~(1 << 30)
+    assert np.array_equal(np.sort(idx), np.arange(K))          # each vector once
+    flip = (dst[real] & (1 << 30)) != 0
+    orig = nvec[idx]
+    sign = np.where(flip, -1, 1)[:, None]
+    got = np.stack([nx[real], ny[real], nz[real]], 1)
+    assert np.array_equal(got, orig * sign)                      # canonical nz >= 0, flag on flips
+    assert np.all(flip == (orig[:, 2] < 0))
+    assert np.all(nz >= 0)
+    assert np.all(nx == nx[:, :1, :])                            # a thread's slots share nx
+    xlo, xhi = rng[:, 0][:, None, None], rng[:, 1][:, None, None]
+    my, mz = (rng[:, 2] & 0xFFFF)[:, None, None], (rng[:, 2] >> 16)[:, None, None]
+    assert np.all((nx >= xlo) & (nx <= xhi) & (np.abs(ny) <= my) & (nz <= mz))
+    act = rng[:, 3]
+    for c in range(n):
+        unit_real = real[c].any(axis=0)                          # [512]
+        assert np.all(unit_real[:act[c]]) and not np.any(unit_real[act[c]:])
+    assert conflicts >= 0
+
+
+def test_layout_lowers_bank_conflicts(qvec):
+    """The slot permutation leaves fewer extra LDS passes than the plain
+    sorted order of the same units would (estimate from the same metric)."""
+    off, nvec = ref_shells(qvec, 2, 100)
+    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, 8)
+
+    def extra(vals):
+        worst = 0
+        banks = {}
+        for v in set(vals.tolist()):
+            banks.setdefault(v % 32, set()).add(v)
+            worst = max(worst, len(banks[v % 32]))
+        return worst - 1
+
+    # same units with slots re-sorted by (ny, nz): undoes the permutation
+    total = 0
+    for c in range(n):
+        key = ny[c] * 1000 + nz[c]
+        order = np.argsort(key, axis=0, kind="stable")
+        sy = np.take_along_axis(ny[c], order, 0)
+        sz = np.take_along_axis(nz[c], order, 0)
+        for r in range(8):
+            for h in range(0, 512, 32):
+                total += extra(sy[r, h:h + 32]) + extra(sz[r, h:h + 32])
+                if r == 0:
+                    total += extra(nx[c, 0, h:h + 32])
+    assert conflicts < total
