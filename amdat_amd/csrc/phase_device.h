@@ -50,9 +50,10 @@ struct Geom {
     static constexpr int OY = 2 * MM + 1 + MM;
     static constexpr int OZ = 2 * (2 * MM + 1);
     static constexpr int A = (31 * W * 8 < 65536) ? 32 : ((15 * W * 8 < 65536) ? 16 : 8);
+    static constexpr int LOG2A = A == 32 ? 5 : (A == 16 ? 4 : 3);
     // tiles per fold of the FP32 partials into the compensated sums: one
-    // fold per 32 atoms
-    static constexpr int FT = 32 / A;
+    // fold per 64 atoms
+    static constexpr int FT = 64 / A;
     static constexpr size_t tab_bytes = (size_t)A * W * sizeof(float2);
 };
 
@@ -77,17 +78,23 @@ __device__ __forceinline__ int range_mz(int4 r) { return r.z >> 16; }
 // Build one tile of tables from the fixed-point fractions sU[3A] (row
 // 3i+c = atom i, axis c; invalid atoms carry sValid[i] == 0 and get zero
 // rows), only over the entries the chunk reads: x in [xlo, xhi], y in
-// [-my, my], z in [0, mz].  Phase 1: exact small anchors e(d), d < 8, per
-// row into anc[3A][8].  Phase 2: one work item per (row, j) computes the
-// exact anchor e(8j) and the entries e(8j + d) = e(8j) e(d), writing each
-// at +m and its conjugate at -m where the range holds them.
+// [-my, my], z in [0, mz].  Phase 1: exact small anchors e(d), d < 8, into
+// anc[d][3A].  Phase 2: one work item per (axis, j, atom), atom fastest,
+// computes the exact anchor e(8j) and the entries e(8j + d) = e(8j) e(d),
+// writing each at +m and its conjugate at -m where the range holds them.
+// Neighbouring lanes work on neighbouring atoms, whose rows start W
+// entries apart, so their stores and anchor reads fall in distinct banks.
 template <int MM>
 __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uint32_t *sU, const int *sValid, int4 rng)
 {
     using Gm = Geom<MM>;
+    constexpr int NR = 3 * Gm::A;  // table rows of a tile
     const int tid = threadIdx.x;
     const int xlo = rng.x, xhi = rng.y, my = range_my(rng), mz = range_mz(rng);
-    for (int e = tid; e < 3 * Gm::A * 8; e += kPhaseBlock) anc[e] = sincos_turns(sU[e >> 3] * (uint32_t)(e & 7));
+    for (int e = tid; e < NR * 8; e += kPhaseBlock) {
+        const int d = e / NR, row = e - d * NR;
+        anc[e] = sincos_turns(sU[row] * (uint32_t)d);
+    }
     __syncthreads();
     // |m| needed on x: [xa, xb]
     const int xa = xlo >= 0 ? xlo : (xhi <= 0 ? -xhi : 0);
@@ -95,8 +102,8 @@ __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uin
     const int jx0 = xa >> 3, njx = (xb >> 3) - jx0 + 1;
     const int njy = (my >> 3) + 1, njz = (mz >> 3) + 1;
     const int nj = njx + njy + njz;
-    for (int it = tid; it < Gm::A * nj; it += kPhaseBlock) {
-        const int i = it / nj, k = it - i * nj;
+    for (int it = tid; it < nj << Gm::LOG2A; it += kPhaseBlock) {
+        const int i = it & (Gm::A - 1), k = it >> Gm::LOG2A;
         int c, j, lo, hi, off;
         if (k < njx) {
             c = 0; j = jx0 + k; lo = xlo; hi = xhi; off = Gm::OX;
@@ -106,14 +113,13 @@ __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uin
             c = 2; j = k - njx - njy; lo = 0; hi = mz; off = Gm::OZ;
         }
         const int row = 3 * i + c;
-        const bool valid = sValid[i] != 0;
-        const float2 a8 = sincos_turns(sU[row] * (uint32_t)(8 * j));
-        const float2 *small = anc + row * 8;
+        float2 a8 = sincos_turns(sU[row] * (uint32_t)(8 * j));
+        if (!sValid[i]) a8 = make_float2(0.0f, 0.0f);
         float2 *dst = tab + i * Gm::W + off;
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
             const int m = 8 * j + d;
-            const float2 v = valid ? cmul(a8, small[d]) : make_float2(0.0f, 0.0f);
+            const float2 v = cmul(a8, anc[d * NR + row]);
             if (m >= lo && m <= hi) dst[m] = v;
             if (m > 0 && -m >= lo && -m <= hi) dst[-m] = make_float2(v.x, -v.y);
         }
@@ -144,59 +150,58 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         pz[r] = tab + Gm::OZ + nz;
         h0[r] = l0[r] = h1[r] = l1[r] = 0.0f;
     }
-    float p0[R], p1[R];
+    // Groups of FT tiles (32 atoms); the FP32 partials of a group are folded
+    // into the compensated sums once, after its last tile.
+    for (int g0 = a_begin; g0 < a_end; g0 += Gm::FT * Gm::A) {
+        float p0[R], p1[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) p0[r] = p1[r] = 0.0f;
-    int since_fold = 0;
-    for (int a0 = a_begin; a0 < a_end; a0 += Gm::A) {
-        __syncthreads();
-        if (tid < 3 * Gm::A) {
-            const int i = tid / 3, c = tid - 3 * (tid / 3);
-            const int a = a0 + i;
-            sU[tid] = a < a_end ? uf(a, c) : 0u;
-            if (c == 0) sValid[i] = a < a_end;
-        }
-        __syncthreads();
-        build_tables<MM>(tab, anc, sU, sValid, rng);
-        __syncthreads();
-        if (!active) continue;
-        // Software pipeline over the flattened (atom, vector) sequence: the
-        // y and z reads of update u + kDepth, and the x read of the next
-        // atom, are issued before update u is computed.
-        float2 by[kDepth], bz[kDepth];
+        for (int r = 0; r < R; ++r) p0[r] = p1[r] = 0.0f;
+        const int g_end = min(a_end, g0 + Gm::FT * Gm::A);
+        for (int a0 = g0; a0 < g_end; a0 += Gm::A) {
+            __syncthreads();
+            if (tid < 3 * Gm::A) {
+                const int i = tid / 3, c = tid - 3 * (tid / 3);
+                const int a = a0 + i;
+                sU[tid] = a < a_end ? uf(a, c) : 0u;
+                if (c == 0) sValid[i] = a < a_end;
+            }
+            __syncthreads();
+            build_tables<MM>(tab, anc, sU, sValid, rng);
+            __syncthreads();
+            if (!active) continue;
+            // Software pipeline over the flattened (atom, vector) sequence:
+            // the y and z reads of update u + kDepth, and the x read of the
+            // next atom, are issued before update u is computed.
+            float2 by[kDepth], bz[kDepth];
 #pragma unroll
-        for (int u = 0; u < kDepth; ++u) {
-            by[u] = py[u % R][(u / R) * Gm::W];
-            bz[u] = pz[u % R][(u / R) * Gm::W];
-        }
-        float2 ex = px[0], ex_next = ex;
+            for (int u = 0; u < kDepth; ++u) {
+                by[u] = py[u % R][(u / R) * Gm::W];
+                bz[u] = pz[u % R][(u / R) * Gm::W];
+            }
+            float2 ex = px[0], ex_next = ex;
 #pragma clang loop unroll(full)
-        for (int u = 0; u < Gm::A * R; ++u) {
-            const int a = u / R, r = u % R, slot = u % kDepth;
-            const float2 ey = by[slot], ez = bz[slot];
-            if (u + kDepth < Gm::A * R) {
-                const int v = u + kDepth;
-                by[slot] = py[v % R][(v / R) * Gm::W];
-                bz[slot] = pz[v % R][(v / R) * Gm::W];
+            for (int u = 0; u < Gm::A * R; ++u) {
+                const int a = u / R, r = u % R, slot = u % kDepth;
+                const float2 ey = by[slot], ez = bz[slot];
+                if (u + kDepth < Gm::A * R) {
+                    const int v = u + kDepth;
+                    by[slot] = py[v % R][(v / R) * Gm::W];
+                    bz[slot] = pz[v % R][(v / R) * Gm::W];
+                }
+                if (r == 0 && a + 1 < Gm::A) ex_next = px[(a + 1) * Gm::W];
+                __builtin_amdgcn_sched_barrier(0);
+                const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+                const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                p0[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, p0[r]));
+                if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
+                __builtin_amdgcn_sched_barrier(0);
+                if (r == R - 1) ex = ex_next;
             }
-            if (r == 0 && a + 1 < Gm::A) ex_next = px[(a + 1) * Gm::W];
-            __builtin_amdgcn_sched_barrier(0);
-            const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
-            const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
-            p0[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, p0[r]));
-            if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
-            __builtin_amdgcn_sched_barrier(0);
-            if (r == R - 1) ex = ex_next;
         }
-        // the condition is uniform across the block
-        if (++since_fold == Gm::FT || a0 + Gm::A >= a_end) {
-            since_fold = 0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                two_sum_acc(h0[r], l0[r], p0[r]);
-                if constexpr (!FS) two_sum_acc(h1[r], l1[r], p1[r]);
-                p0[r] = p1[r] = 0.0f;
-            }
+        for (int r = 0; r < R; ++r) {
+            two_sum_acc(h0[r], l0[r], p0[r]);
+            if constexpr (!FS) two_sum_acc(h1[r], l1[r], p1[r]);
         }
     }
 }
