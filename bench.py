@@ -290,11 +290,23 @@ def main():
     fs_tf = 4.0 * U_fs / (fs_ms * 1e-3) / 1e12
     dom = "rho" if rho_ms >= fs_ms else "fs"
     ach = rho_tf if dom == "rho" else fs_tf
-    result["roofline"] = {"bound": "valu", "kernel": "rho_c32_kernel" if dom == "rho" else "fs_c32_kernel",
+    kname = "rho_c32_kernel" if dom == "rho" else "fs_c32_kernel"
+    result["roofline"] = {"bound": "valu", "kernel": kname,
                           "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                           "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
                           "launch_ms": {"rho": rho_ms, "fs": fs_ms},
                           "tflops": {"rho": rho_tf, "fs": fs_tf}}
+    # HBM bytes per launch of the same kernel on this workload, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (counters
+    # cannot be read from inside the timed run).
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not args.small and os.path.exists(tpath):
+        t = json.load(open(tpath))
+        k = t["kernels"].get(kname)
+        if k:
+            result["roofline"]["traffic"] = int(1024 * (k["fetch_kib"] * t["fetch_factor"]
+                                                        + k["write_kib"] * t["write_factor"]))
+            result["roofline"]["traffic_source"] = "profiles/hbm_traffic.json: %s" % k["instance"]
 
     if not args.no_fp64:
         t = time.perf_counter()
