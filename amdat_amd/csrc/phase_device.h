@@ -51,9 +51,6 @@ struct Geom {
     static constexpr int OZ = 2 * (2 * MM + 1);
     static constexpr int A = (31 * W * 8 < 65536) ? 32 : ((15 * W * 8 < 65536) ? 16 : 8);
     static constexpr int LOG2A = A == 32 ? 5 : (A == 16 ? 4 : 3);
-    // tiles per fold of the FP32 partials into the compensated sums: one
-    // fold per 64 atoms
-    static constexpr int FT = 64 / A;
     static constexpr size_t tab_bytes = (size_t)A * W * sizeof(float2);
 };
 
@@ -150,13 +147,16 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         pz[r] = tab + Gm::OZ + nz;
         h0[r] = l0[r] = h1[r] = l1[r] = 0.0f;
     }
-    // Groups of FT tiles (32 atoms); the FP32 partials of a group are folded
-    // into the compensated sums once, after its last tile.
-    for (int g0 = a_begin; g0 < a_end; g0 += Gm::FT * Gm::A) {
+    // Groups of FT tiles; the FP32 partials of a group are folded into the
+    // compensated sums once, after its last tile: every 32 atoms for rho
+    // (per-vector sums are outputs), every 64 for Fs (averaged over pairs
+    // and shells).
+    constexpr int FT = (FS ? 64 : 32) / Gm::A > 0 ? (FS ? 64 : 32) / Gm::A : 1;
+    for (int g0 = a_begin; g0 < a_end; g0 += FT * Gm::A) {
         float p0[R], p1[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) p0[r] = p1[r] = 0.0f;
-        const int g_end = min(a_end, g0 + Gm::FT * Gm::A);
+        const int g_end = min(a_end, g0 + FT * Gm::A);
         for (int a0 = g0; a0 < g_end; a0 += Gm::A) {
             __syncthreads();
             if (tid < 3 * Gm::A) {

@@ -326,7 +326,7 @@ def main():
         log("plain FP64 step %.1f ms; errors %s" % (fp64_ms, err))
 
     if not args.no_cpu:
-        n_sample = 600 if args.small else 3000
+        n_sample = 600 if args.small else 20000   # about 10 s of one core
         result["cpu_baseline"] = cpu_baseline(tr, (offF, nF), (offS, nS), cfg["L"], cfg["ell_fs"], n_sample,
                                               (tg, t0, t1, ntg))
         log("cpu baseline %.3e updates/s" % result["cpu_baseline"]["value"])
