@@ -44,6 +44,7 @@ SIGNATURES = {
     "amdat_ts_pairs": (_I, [_I, _I, _I, _I, _I, _PI, _PI, _PI, _I, _PI]),
     "amdat_ts_weighting": (_I, [_I, _I, _I, _I, _I, _PI, _I]),
     "amdat_shells_read": (_I, [C.c_char_p, _I, _I, _PI, _PI, _I, _PI]),
+    "amdat_shells_read_plane": (_I, [C.c_char_p, C.c_char_p, _I, _I, _PI, _PI, _I, _PI, _PI]),
     "amdat_shells_generate": (_I, [_I, _I, _I, C.c_uint64, _PI, _PI, _I, _PI]),
     "amdat_shells_upload": (_I, [_P, _I, _PI, _PI, _PP]),
     "amdat_shells_free": (_I, [_P, _P]),
@@ -181,6 +182,16 @@ def shells_read(stem, first_bin, n_bins, cap=400000):
     n = C.c_int(0)
     check(lib().amdat_shells_read(stem.encode(), first_bin, n_bins, ip(off), ip(nvec), cap, C.byref(n)))
     return off, nvec[: 3 * n.value].reshape(-1, 3).copy()
+
+
+def shells_read_plane(stem, plane, first_bin, n_bins, cap=400000):
+    """(offsets, integer vectors on x, y, z, denominator): k = 2 pi n / (denominator * l)."""
+    off = np.zeros(n_bins + 1, np.int32)
+    nvec = np.zeros(3 * cap, np.int32)
+    n, den = C.c_int(0), C.c_int(0)
+    check(lib().amdat_shells_read_plane(stem.encode(), plane.encode(), first_bin, n_bins, ip(off), ip(nvec), cap,
+                                        C.byref(n), C.byref(den)))
+    return off, nvec[: 3 * n.value].reshape(-1, 3).copy(), den.value
 
 
 def shells_generate(first_bin, n_bins, max_per_shell=300, seed=0xA11DA7, cap=2000000):

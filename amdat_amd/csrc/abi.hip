@@ -236,7 +236,6 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
 {
     const int v0 = sh->off[first], v1 = sh->off[last + 1];
     const int K = v1 - v0;
-    if (K <= 0) return set_error(AMDAT_EINVAL, "shell range holds no vectors");
     int M = 0;
     for (int v = v0; v < v1; v++)
         for (int c = 0; c < 3; c++) M = std::max(M, std::abs(sh->nvec[3 * v + c]));
@@ -248,6 +247,15 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
 
     auto P = std::make_unique<KPlanOwned>();
     KPlan &kp = P->kp;
+    if (K == 0) {
+        // empty shells (the 2D tables have some): no chunks, every sum is 0
+        kp.R = R;
+        kp.KB = kPhaseBlock * R;
+        kp.MM = plan_table_width(0);
+        *out = P.get();
+        sh->plans[key] = std::move(P);
+        return AMDAT_OK;
+    }
     PlanLayout lay = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, R);
     kp.K = K;
     kp.R = R;
@@ -511,7 +519,7 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
     hipStream_t st = stream_of(ctx);
     HIP_TRY(r->d_shell_off.upload(r->shell_off.data(), r->shell_off.size(), st));
     HIP_TRY(r->d.alloc((size_t)r->n_frames * kp.K));
-    if (nl > 0) {
+    if (nl > 0 && kp.K > 0) {
         DevBuf<int> dfr;
         HIP_TRY(dfr.upload(fr.data(), fr.size(), st));
         const int n_splits = choose_splits((long)kp.n_chunks * r->n_frames, nl);
@@ -528,7 +536,7 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
             if (n_splits > 1) HIP_TRY(launch_split_reduce2(part.p, nf, n_splits, kp.K, dst, st));
         }
         HIP_TRY(hipStreamSynchronize(st));
-    } else {
+    } else if (kp.K > 0) {
         HIP_TRY(hipMemsetAsync(r->d.p, 0, r->d.n * sizeof(double2), st));
         HIP_TRY(hipStreamSynchronize(st));
     }
@@ -647,7 +655,7 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
     std::vector<double> norm(ntg, 0.0);
     for (int i = 0; i < pairs->n; i++) norm[pairs->tg[i]] += (double)nl;
     for (int g = 0; g < ntg; g++) count[g] = norm[g];
-    if (pairs->n == 0 || nl == 0) {
+    if (pairs->n == 0 || nl == 0 || kp.K == 0) {
         std::fill(Fs, Fs + (size_t)ntg * nsh, 0.0);
         return AMDAT_OK;
     }
@@ -731,6 +739,22 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     if (rc) return rc;
     const KPlan &kp = plan->kp;
     const int K = kp.K, nsh = last_shell - first_shell + 1, v0 = sh->off[first_shell];
+    if (K == 0) {
+        // no vectors: S = 0 on every shell, a constant loss, zero gradient
+        double L = 0.0;
+        for (int s = 0; s < nsh; s++) {
+            if (S) S[s] = 0.0;
+            L += weights[s] * target[s] * target[s];
+        }
+        *loss = L;
+        if (d_gx) {
+            HIP_TRY(hipMemsetAsync(d_gx, 0, sizeof(double) * n_atoms, stream_of(ctx)));
+            HIP_TRY(hipMemsetAsync(d_gy, 0, sizeof(double) * n_atoms, stream_of(ctx)));
+            HIP_TRY(hipMemsetAsync(d_gz, 0, sizeof(double) * n_atoms, stream_of(ctx)));
+            HIP_TRY(hipStreamSynchronize(stream_of(ctx)));
+        }
+        return AMDAT_OK;
+    }
     hipStream_t st = stream_of(ctx);
     TrajDev view;
     view.N = n_atoms;
@@ -869,7 +893,7 @@ int amdat_batch_sk(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
     // frame groups: enough items to balance the queue (AMDAT_BATCH_ITEMS, default 16384)
     long target = 16384;
     if (const char *e = std::getenv("AMDAT_BATCH_ITEMS")) target = std::max(1L, std::atol(e));
-    const long units = (long)n * kp.n_chunks;
+    const long units = std::max(1L, (long)n * kp.n_chunks);
     int G = (int)std::max(1L, std::min((long)F, (target + units - 1) / units));
     const int FG = (F + G - 1) / G;
     G = (F + FG - 1) / FG;
@@ -927,14 +951,13 @@ int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
             return set_error(AMDAT_ERANGE, "amdat_batch_fs: shell range outside the table");
         if (!(length_scales[i] > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_batch_fs: length scale must be > 0");
         const int K = sh->off[last_shell[i] + 1] - sh->off[first_shell[i]];
-        if (K <= 0) return set_error(AMDAT_EINVAL, "amdat_batch_fs: shell range holds no vectors");
         Kmax = std::max(Kmax, K);
         M = std::max(M, shell_range_M(sh, first_shell[i], last_shell[i]));
     }
     if (M > 127) return set_error(AMDAT_ERANGE, "amdat_batch_fs: k-vector component beyond +-127");
     HIP_TRY(hipSetDevice(ctx->device));
     // one R and one table width for the launch; per-trajectory plans concatenated
-    const int R = choose_R(Kmax, M, 6.0);
+    const int R = choose_R(std::max(Kmax, 1), M, 6.0);
     std::vector<KPlanOwned *> plans(n);
     std::vector<int> chunk_base(n + 1, 0), row_off(n + 1, 0);
     for (int i = 0; i < n; i++) {
@@ -958,6 +981,7 @@ int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
     for (int i = 0; i < n; i++) {
         const KPlan &p = plans[i]->kp;
         const size_t o = (size_t)chunk_base[i] * cat.KB, m = (size_t)p.n_chunks * cat.KB;
+        if (m == 0) continue;
         HIP_TRY(hipMemcpyAsync(kpack.p + o, p.kpack, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
         HIP_TRY(hipMemcpyAsync(kdst.p + o, p.kdst, m * sizeof(int), hipMemcpyDeviceToDevice, st));
         HIP_TRY(hipMemcpyAsync(range.p + chunk_base[i], p.range, p.n_chunks * sizeof(int4), hipMemcpyDeviceToDevice, st));
@@ -1106,7 +1130,7 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
         jc.nsh_sk = prm.sk_last - prm.sk_first + 1;
         long target = 16384;
         if (const char *e = std::getenv("AMDAT_BATCH_ITEMS")) target = std::max(1L, std::atol(e));
-        const long units = (long)n * skp.n_chunks;
+        const long units = std::max(1L, (long)n * skp.n_chunks);
         int G = (int)std::max(1L, std::min((long)F, (target + units - 1) / units));
         const int FG = (F + G - 1) / G;
         G = (F + FG - 1) / FG;
@@ -1161,6 +1185,7 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
         for (int b = 0; b < n; b++) {
             const KPlan &p = plans[b]->kp;
             const size_t o = (size_t)chunk_base[b] * fsp.KB, m = (size_t)p.n_chunks * fsp.KB;
+            if (m == 0) continue;
             HIP_TRY(hipMemcpyAsync(jc.fs_kpack.p + o, p.kpack, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
             HIP_TRY(hipMemcpyAsync(jc.fs_kdst.p + o, p.kdst, m * sizeof(int), hipMemcpyDeviceToDevice, st));
             HIP_TRY(hipMemcpyAsync(jc.fs_range.p + chunk_base[b], p.range, p.n_chunks * sizeof(int4),
@@ -1362,7 +1387,6 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
             if (fs_first[b] < 0 || fs_last[b] < fs_first[b] || fs_last[b] >= fs_shells->n_shells)
                 return set_error(AMDAT_ERANGE, "amdat_batch_run: Fs shell range outside the table");
             const int K = fs_shells->off[fs_last[b] + 1] - fs_shells->off[fs_first[b]];
-            if (K <= 0) return set_error(AMDAT_EINVAL, "amdat_batch_run: Fs shell range holds no vectors");
             Kfs = std::max(Kfs, K);
             Mfs = std::max(Mfs, shell_range_M(fs_shells, fs_first[b], fs_last[b]));
         }
@@ -1377,10 +1401,9 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
         int RS = 0, RF = 0;
         if (sk_shells) {
             const int K = sk_shells->off[sk_last + 1] - sk_shells->off[sk_first];
-            if (K <= 0) return set_error(AMDAT_EINVAL, "amdat_batch_run: S(k) shell range holds no vectors");
-            RS = choose_R(K, shell_range_M(sk_shells, sk_first, sk_last));
+            RS = choose_R(std::max(K, 1), shell_range_M(sk_shells, sk_first, sk_last));
         }
-        if (fs_shells) RF = choose_R(Kfs, Mfs, 6.0);
+        if (fs_shells) RF = choose_R(std::max(Kfs, 1), Mfs, 6.0);
         if (!RS) RS = RF ? RF : 2;
         if (!RF) RF = RS;
         if (flags & AMDAT_RUN_TUNE) {

@@ -135,11 +135,31 @@ int amdat_ts_weighting(int nexp, int nes, int frt, int fullblock, int limit, int
 // Wave_Vectors::read_vectors, src/wave_vectors.cpp:430-484: one file per
 // bin, whitespace-separated numbers, three per vector.  Numbers must be
 // integral (the reference scales them by the grid spacing).
-int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_off, int *nvec,
-                      int cap_vectors, int *n_vectors)
+// Shell files stem.%03d for bins first_bin .. first_bin+n_bins-1: whitespace
+// separated number triples (t0, t1, t2).  Each triple is placed on the axes
+// by the plane as the reference's readers place it, times `scale`, and must
+// then be an integer within +-127.
+static int read_shell_files(const char *stem, const char *plane, int first_bin, int n_bins, int *shell_off, int *nvec,
+                            int cap_vectors, int *n_vectors, int *denominator)
 {
-    if (!stem || !shell_off || n_bins < 1 || first_bin < 0 || first_bin + n_bins - 1 > 999)
+    if (!stem || !plane || !shell_off || n_bins < 1 || first_bin < 0 || first_bin + n_bins - 1 > 999)
         return set_error(AMDAT_EINVAL, "amdat_shells_read: bad arguments");
+    // axis of each file column (-1: ignored), and the scale that makes the
+    // entries integers: src/wave_vectors.cpp:314-331 (2D), 394-413 (1D)
+    int ax[3];
+    double scale = 1.0;
+    const std::string pl(plane);
+    if (pl == "xyz" || pl == "xy") { ax[0] = 0; ax[1] = 1; ax[2] = 2; }
+    else if (pl == "xz") { ax[0] = 0; ax[1] = 2; ax[2] = 1; }
+    else if (pl == "yz") { ax[0] = 1; ax[1] = 2; ax[2] = 0; }
+    else if (pl == "x" || pl == "y" || pl == "z") {
+        ax[0] = pl == "x" ? 0 : (pl == "y" ? 1 : 2);
+        ax[1] = ax[2] = -1;
+        scale = 2.0;  // 1D shells hold half-integers (bin/2)
+    } else {
+        return set_error(AMDAT_EINVAL, "plane command not recognized. valid inputs are: x, y, z, xy, xz, yz, xyz");
+    }
+    if (denominator) *denominator = (int)scale;
     int n = 0;
     shell_off[0] = 0;
     for (int s = 0; s < n_bins; s++) {
@@ -152,17 +172,18 @@ int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_of
         int c = 0;
         while (std::fscanf(fp, "%lf", &v[c]) == 1) {
             if (++c == 3) {
+                int out[3] = {0, 0, 0};
                 for (int d = 0; d < 3; d++) {
-                    if (v[d] != std::floor(v[d]) || std::fabs(v[d]) > 127.0) {
+                    if (ax[d] < 0) continue;
+                    const double w = v[d] * scale;
+                    if (w != std::floor(w) || std::fabs(w) > 127.0) {
                         std::fclose(fp);
-                        return set_error(AMDAT_EIO, "non-integral or out-of-range vector component in " + path);
+                        return set_error(AMDAT_EIO, "vector component not representable on the integer grid in " + path);
                     }
+                    out[ax[d]] = (int)w;
                 }
-                if (n < cap_vectors && nvec) {
-                    nvec[3 * n] = (int)v[0];
-                    nvec[3 * n + 1] = (int)v[1];
-                    nvec[3 * n + 2] = (int)v[2];
-                }
+                if (n < cap_vectors && nvec)
+                    for (int d = 0; d < 3; d++) nvec[3 * n + d] = out[d];
                 ++n;
                 c = 0;
             }
@@ -174,6 +195,18 @@ int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_of
     if (n_vectors) *n_vectors = n;
     if (n > cap_vectors) return set_error(AMDAT_EINVAL, "amdat_shells_read: vector capacity too small");
     return AMDAT_OK;
+}
+
+int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_off, int *nvec,
+                      int cap_vectors, int *n_vectors)
+{
+    return read_shell_files(stem, "xyz", first_bin, n_bins, shell_off, nvec, cap_vectors, n_vectors, nullptr);
+}
+
+int amdat_shells_read_plane(const char *stem, const char *plane, int first_bin, int n_bins, int *shell_off,
+                            int *nvec, int cap_vectors, int *n_vectors, int *denominator)
+{
+    return read_shell_files(stem, plane, first_bin, n_bins, shell_off, nvec, cap_vectors, n_vectors, denominator);
 }
 
 namespace {

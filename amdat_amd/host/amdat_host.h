@@ -150,11 +150,12 @@ class System {
 
 class Wave_Vectors {
   public:
-    // src/wave_vectors.cpp:77-181: plane "xyz"; max_length_scale <= 0 means
-    // the smallest box dimension.  Reads n_wavenumbers shells (99 in the
-    // reference: files stem.002 .. stem.100) from the q-vector files named
-    // by the AMDAT_WV3D environment variable (the reference's WV3D path);
-    // n_wavenumbers > 99 opts into the files up to stem.300.
+    // src/wave_vectors.cpp:77-181: plane "xyz", "xy", "xz", "yz", "x", "y"
+    // or "z"; max_length_scale <= 0 means the smallest box dimension in the
+    // plane.  Reads n_wavenumbers shells (99 in the reference: files
+    // stem.002 .. stem.100) from the q-vector files named by AMDAT_WV3D,
+    // AMDAT_WV2D or AMDAT_WV1D (the reference's WV3D / WV2D / WV1D paths);
+    // n_wavenumbers > 99 opts into later files.
     Wave_Vectors(System *sys, const std::string &plane, float max_length_scale = 0, int n_wavenumbers = 99);
     ~Wave_Vectors();
     Wave_Vectors(const Wave_Vectors &) = delete;
@@ -168,7 +169,9 @@ class Wave_Vectors {
     float show_mean_wavenumber(int index) const;
     float show_stdev_wavenumber(int index) const;
     Coordinate show_mean_wavevector(int index) const;
+    // length scale l of the integer vectors held: k = 2 pi n / l
     float show_length_scale() const { return length_scale_; }
+    int show_denominator() const { return denominator_; }
     amdat_shells *handle() const { return shells_; }
     const std::vector<int> &offsets() const { return off_; }
     const std::vector<int> &integer_vectors() const { return nvec_; }
@@ -176,6 +179,7 @@ class Wave_Vectors {
   private:
     System *system_;
     float length_scale_, wavegrid_spacing_, delta_wavenumber_;
+    int denominator_ = 1;
     std::vector<float> approx_;
     std::vector<int> off_, nvec_;
     amdat_shells *shells_ = nullptr;

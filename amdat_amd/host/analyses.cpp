@@ -19,12 +19,16 @@ void check(int rc, const char *what)
     if (rc < 0) throw Error(std::string(what) + ": " + amdat_last_error(), 0);
 }
 
-std::string wv3d_stem()
+// q-vector file stem of a plane: AMDAT_WV3D, AMDAT_WV2D or AMDAT_WV1D (the
+// reference's compile-time WV3D / WV2D / WV1D paths).
+std::string wv_stem(const std::string &plane)
 {
-    const char *e = std::getenv("AMDAT_WV3D");
+    const char *var = plane == "xyz" ? "AMDAT_WV3D" : (plane.size() == 2 ? "AMDAT_WV2D" : "AMDAT_WV1D");
+    const char *dir = plane == "xyz" ? "qvectors3d" : (plane.size() == 2 ? "qvectors2d" : "qvectors1d");
+    const char *e = std::getenv(var);
     if (!e || !*e)
-        throw Error("AMDAT_WV3D is not set: point it at the q-vector file stem "
-                    "(the reference's qvectors3d/qvector)", 0);
+        throw Error(std::string(var) + " is not set: point it at the q-vector file stem (the reference's " + dir +
+                        "/qvector)", 0);
     return e;
 }
 }  // namespace
@@ -35,20 +39,25 @@ std::string wv3d_stem()
 Wave_Vectors::Wave_Vectors(System *sys, const std::string &plane, float max_length_scale, int n_wavenumbers)
     : system_(sys)
 {
-    if (plane != "xyz")
-        throw Error("plane " + plane + " is not supported by this build (xyz only)", 1);
     if (n_wavenumbers < 1 || n_wavenumbers > 299) throw Error("n_wavenumbers must be in 1..299", 0);
     float min_size;
+    const Coordinate mb = sys->min_box_dimensions();
     if (max_length_scale <= 0.0f) {
         std::cout << "Maximum lengthscale for wavevector decomposition <= 0; defaulting to box size.\n";
-        min_size = sys->min_box_dimensions().min();
+        if (plane == "xy") min_size = mb.show_x() < mb.show_y() ? mb.show_x() : mb.show_y();
+        else if (plane == "xz") min_size = mb.show_x() < mb.show_z() ? mb.show_x() : mb.show_z();
+        else if (plane == "yz") min_size = mb.show_z() < mb.show_y() ? mb.show_z() : mb.show_y();
+        else if (plane == "xyz") min_size = mb.min();
+        else if (plane == "x") min_size = mb.show_x();
+        else if (plane == "y") min_size = mb.show_y();
+        else if (plane == "z") min_size = mb.show_z();
+        else throw Error("plane command not recognized. valid inputs are: x, y, z, xy, xz, yz, xyz", 1);
     } else {
-        if (max_length_scale > sys->min_box_dimensions().min())
+        if (max_length_scale > mb.min())
             std::cout << "\nWARNING: Maximum length scale larger than smallest box dimension. Consider choosing a "
                          "different maximum for wavevector decomposition.";
         min_size = max_length_scale;
     }
-    length_scale_ = min_size;
     wavegrid_spacing_ = (float)(2.0 * kPi / min_size);
     delta_wavenumber_ = (float)(wavegrid_spacing_ / 2.0);
     for (int wn = 0; wn < n_wavenumbers; wn++) approx_.push_back(wavegrid_spacing_ + delta_wavenumber_ * wn);
@@ -56,9 +65,13 @@ Wave_Vectors::Wave_Vectors(System *sys, const std::string &plane, float max_leng
     const int cap = 300 * n_wavenumbers + 1024;
     nvec_.assign(3 * (size_t)cap, 0);
     int n = 0;
-    check(amdat_shells_read(wv3d_stem().c_str(), 2, n_wavenumbers, off_.data(), nvec_.data(), cap, &n),
+    check(amdat_shells_read_plane(wv_stem(plane).c_str(), plane.c_str(), 2, n_wavenumbers, off_.data(), nvec_.data(),
+                                  cap, &n, &denominator_),
           "reading wave vectors");
     nvec_.resize(3 * (size_t)n);
+    // the kernels take k = 2 pi n / l with integer n: 1D shells (half-integer
+    // multiples of the spacing) are held as 2n over 2 l
+    length_scale_ = min_size * (float)denominator_;
     check(amdat_shells_upload(device_context(), n_wavenumbers, off_.data(), nvec_.data(), &shells_),
           "wave vector upload");
 }
@@ -68,9 +81,11 @@ Wave_Vectors::~Wave_Vectors() { amdat_shells_free(device_context(), shells_); }
 std::vector<Coordinate> Wave_Vectors::vectorlist(int index) const
 {
     std::vector<Coordinate> out;
+    const float inv_d = 1.0f / (float)denominator_;  // exact: 1 or 1/2
     for (int v = off_[index]; v < off_[index + 1]; v++)
-        out.emplace_back((float)nvec_[3 * v] * wavegrid_spacing_, (float)nvec_[3 * v + 1] * wavegrid_spacing_,
-                         (float)nvec_[3 * v + 2] * wavegrid_spacing_);
+        out.emplace_back((float)nvec_[3 * v] * inv_d * wavegrid_spacing_,
+                         (float)nvec_[3 * v + 1] * inv_d * wavegrid_spacing_,
+                         (float)nvec_[3 * v + 2] * inv_d * wavegrid_spacing_);
     return out;
 }
 

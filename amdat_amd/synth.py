@@ -65,6 +65,19 @@ def random_walk(N, L, T, sigma, seed, start=None):
     return Traj(x, y, z, ix, iy, iz, box)
 
 
+def random_walk_box(N, box, T, sigma, seed):
+    """random_walk in an orthorhombic box (Lx, Ly, Lz)."""
+    rng = np.random.default_rng(seed)
+    L = np.asarray(box, np.float64)
+    u = np.empty((T, N, 3))
+    u[0] = rng.uniform(0.0, 1.0, size=(N, 3)) * L
+    steps = rng.normal(0.0, sigma, size=(T - 1, N, 3))
+    np.cumsum(steps, axis=0, out=steps)
+    u[1:] = u[0][None] + steps
+    (x, y, z), (ix, iy, iz) = _store(u, L)
+    return Traj(x, y, z, ix, iy, iz, np.tile(L.astype(np.float32), (T, 1)))
+
+
 def plain_config(N=1000, L=9.41, nexp=5, nes=20, sigma=0.05, seed=MASTER_SEED + 0):
     """3D binary 80:20 mixture, uniform frame 0, Gaussian steps."""
     T = nes * nexp + 1

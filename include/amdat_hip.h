@@ -114,6 +114,18 @@ int amdat_ts_weighting(int n_exponentials, int n_exponential_steps, int frt, int
  * the total. */
 int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_off, int *nvec,
                       int cap_vectors, int *n_vectors);
+
+/* The same for any plane of the reference's Wave_Vectors: "xyz" (3D files),
+ * "xy", "xz", "yz" (2D files: columns placed on the axes as
+ * src/wave_vectors.cpp:266-336 places them) or "x", "y", "z" (1D files,
+ * src/wave_vectors.cpp:343-424, whose shells hold half-integers bin/2).
+ * Vectors come back as integers n on (x, y, z) with *denominator set to 1,
+ * or 2 for the 1D planes: k = 2 pi n / (denominator * length_scale), so a
+ * caller passes denominator * length_scale wherever a length scale is
+ * asked for.  Other arguments as amdat_shells_read. */
+int amdat_shells_read_plane(const char *stem, const char *plane, int first_bin, int n_bins,
+                            int *shell_off, int *nvec, int cap_vectors, int *n_vectors, int *denominator);
+
 /* Build shells by the shell rule of src/qvectors/generate_3d_qvectors.m
  * (half-space integer vectors, bin = floor((|n|+0.25)*2), |n_c| <= 100)
  * and keep at most max_per_shell vectors per shell, chosen by a seeded

@@ -253,7 +253,7 @@ void orc_sk_f64(int n_frames, int K, const double *rho1, const double *rho2,
                 const double *b = rho2 + 2*((size_t)f*K + v);
                 part += a[0]*b[0] + a[1]*b[1];
             }
-            acc += part / (double)H;
+            acc += H ? part / (double)H : 0.0;  /* empty shell: the per-vector division never runs */
         }
         S[s] = ntot ? acc / (double)ntot : 0.0;
     }
@@ -283,7 +283,7 @@ void orc_fkt_f64(int T, int K, const double *rho1, const double *rho2,
                 const double *b = rho2 + 2*((size_t)t1[p]*K + v);
                 part += a[0]*b[0] + a[1]*b[1];
             }
-            F[tg[p]*n_shells + s] += part / (double)H;
+            F[tg[p]*n_shells + s] += H ? part / (double)H : 0.0;
         }
         count[tg[p]] += (double)natoms[t0[p]];
     }
@@ -325,7 +325,7 @@ void orc_fs_f64(int N, const float *x, const float *y, const float *z,
                 double part = 0.0;
                 for (int v = shell_off[s]; v < shell_off[s+1]; v++)
                     part += cos(c * (nvec[3*v]*dx + nvec[3*v+1]*dy + nvec[3*v+2]*dz));
-                acc += part / (double)H;
+                acc += H ? part / (double)H : 0.0;
             }
             row[s] += acc;
         }

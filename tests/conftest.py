@@ -39,3 +39,39 @@ def ctx():
     c = _abi.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def qvec2d():
+    """AMDAT's 2D q-vector shells: shell b holds n[off[b]:off[b+1]] (files .000-.284)."""
+    d = np.load(os.path.join(GOLDEN, "qvectors2d.npz"))
+    return d["off"].astype(np.int64), d["n"].astype(np.int32)
+
+
+@pytest.fixture(scope="session")
+def qvec1d():
+    """AMDAT's 1D q-vector shells: twice the triple of files .001-.300 (row b-1)."""
+    return np.load(os.path.join(GOLDEN, "qvectors1d.npz"))["n2"].astype(np.int32)
+
+
+def write_plane_files(dirpath, qvec2d=None, qvec1d=None, bins=range(2, 101)):
+    """The reference's 2D (tab-separated integers) and 1D (%e, CRLF) files
+    for the given bins; returns the stems."""
+    stems = {}
+    if qvec2d is not None:
+        off, n = qvec2d
+        d = os.path.join(dirpath, "qvectors2d")
+        os.makedirs(d, exist_ok=True)
+        for b in bins:
+            with open(os.path.join(d, "qvector.%03d" % b), "w") as fh:
+                for v in n[off[b]:off[b + 1]]:
+                    fh.write("\t%d\t%d\t%d\n" % tuple(v))
+        stems["2d"] = os.path.join(d, "qvector")
+    if qvec1d is not None:
+        d = os.path.join(dirpath, "qvectors1d")
+        os.makedirs(d, exist_ok=True)
+        for b in bins:
+            with open(os.path.join(d, "qvector.%03d" % b), "w", newline="") as fh:
+                fh.write("   %.7e   %.7e   %.7e\r\n" % tuple(qvec1d[b - 1] / 2.0))
+        stems["1d"] = os.path.join(d, "qvector")
+    return stems

@@ -218,3 +218,26 @@ def test_ref32_close_to_f64(qvec):
     Fs32, _ = orc.ref32_fs(tr, pairs, ntg, off, n, np.float32(9.41))
     Fs64, _ = orc.fs_f64(tr, pairs, ntg, off, n, L)
     assert np.allclose(Fs32, Fs64, atol=1e-4)
+
+
+def test_empty_shell_gives_zero():
+    """The reference divides by the shell's vector count inside the vector
+    loop (src/structure_factor.cpp:226, src/intermediate_scattering_function.cpp:562,
+    src/incoherent_scattering_function.cpp:264): an empty shell contributes 0,
+    and the other shells are unchanged."""
+    tr = synth.random_walk(40, 5.0, 4, 0.1, seed=9)
+    n = np.array([[1, 0, 0], [0, 2, 0], [1, 1, 1]], np.int32)
+    off_full = np.array([0, 2, 3], np.int32)
+    off_gap = np.array([0, 2, 2, 3], np.int32)        # an empty shell in the middle
+    rho = orc.rho_f64_traj(tr, n, 5.0)
+    S_full = orc.sk_f64(rho, rho, np.full(tr.T, tr.N), off_full)
+    S_gap = orc.sk_f64(rho, rho, np.full(tr.T, tr.N), off_gap)
+    assert S_gap[1] == 0.0 and S_gap[0] == S_full[0] and S_gap[2] == S_full[1]
+    tg, t0, t1 = orc.ts_pairs(2, 1)
+    ntg = orc.ts_n_timegaps(2, 1)
+    F_full, _ = orc.fkt_f64(rho, rho, np.full(tr.T, tr.N), (tg, t0, t1), ntg, off_full)
+    F_gap, _ = orc.fkt_f64(rho, rho, np.full(tr.T, tr.N), (tg, t0, t1), ntg, off_gap)
+    assert np.all(F_gap[:, 1] == 0.0) and np.array_equal(F_gap[:, [0, 2]], F_full)
+    Fs_full, _ = orc.fs_f64(tr, (tg, t0, t1), ntg, off_full, n, 5.0)
+    Fs_gap, _ = orc.fs_f64(tr, (tg, t0, t1), ntg, off_gap, n, 5.0)
+    assert np.all(Fs_gap[:, 1] == 0.0) and np.array_equal(Fs_gap[:, [0, 2]], Fs_full)
