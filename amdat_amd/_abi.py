@@ -51,6 +51,10 @@ SIGNATURES = {
     "amdat_traj_upload": (_I, [_P, _I, _I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PP]),
     "amdat_traj_wrap_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _PF, _PP]),
     "amdat_traj_free": (_I, [_P, _P]),
+    "amdat_read_custom": (_I, [C.c_char_p, _I, _I, _I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PF, _PI]),
+    "amdat_read_xyz": (_I, [C.c_char_p, _I, _I, _PF, _I, _PF, _PF, _PF, _PS, _PS, _PS, _PI]),
+    "amdat_traj_read": (_I, [_P, C.c_char_p, C.c_char_p, _I, _I, _PF, _I, _PI, _PP]),
+    "amdat_traj_shape": (_I, [_P, _PI, _PI, _PF]),
     "amdat_list_create": (_I, [_P, _P, _I, _PI, _PP]),
     "amdat_list_free": (_I, [_P, _P]),
     "amdat_pairs_upload": (_I, [_P, _I, _PI, _PI, _PI, _I, _PP]),
@@ -286,6 +290,53 @@ class Trajectory(_Handle):
         _Handle.__init__(self, ctx, h)
         self.N, self.T = N, T
         return self
+
+
+    def box(self):
+        """Per-frame box lengths [T, 3] (host copy)."""
+        b = np.zeros((self.T, 3), np.float32)
+        check(lib().amdat_traj_shape(self.h, None, None, fp(b)))
+        return b
+
+    @classmethod
+    def read(cls, ctx, fmt, path, N, T, box3=None, threads=0):
+        """Read a "custom" or "xyz" trajectory file straight into device
+        memory.  Returns (trajectory, types of frame 0)."""
+        self = cls.__new__(cls)
+        b = None if box3 is None else np.ascontiguousarray(box3, np.float32)
+        types = np.zeros(N, np.int32)
+        h = C.c_void_p()
+        check(lib().amdat_traj_read(ctx.h, fmt.encode(), str(path).encode(), N, T, fp(b), threads, ip(types),
+                                    C.byref(h)))
+        _Handle.__init__(self, ctx, h)
+        self.N, self.T = N, T
+        return self, types
+
+
+def read_custom(path, N, T, threads=0):
+    """Host SoA arrays of a LAMMPS custom dump: dict of x y z ix iy iz [T, N],
+    box [T, 3], lo [T, 3], types [N]."""
+    out = {k: np.zeros((T, N), np.float32) for k in ("x", "y", "z")}
+    out.update({k: np.zeros((T, N), np.int16) for k in ("ix", "iy", "iz")})
+    out["box"] = np.zeros((T, 3), np.float32)
+    out["lo"] = np.zeros((T, 3), np.float32)
+    out["types"] = np.zeros(N, np.int32)
+    check(lib().amdat_read_custom(str(path).encode(), N, T, threads, fp(out["x"]), fp(out["y"]), fp(out["z"]),
+                                  sp(out["ix"]), sp(out["iy"]), sp(out["iz"]), fp(out["box"]), fp(out["lo"]),
+                                  ip(out["types"])))
+    return out
+
+
+def read_xyz(path, N, T, box3, threads=0):
+    """Host SoA arrays of an xyz trajectory with image counts from the
+    reference's unwrapping rule: dict of x y z ix iy iz [T, N], types [N]."""
+    out = {k: np.zeros((T, N), np.float32) for k in ("x", "y", "z")}
+    out.update({k: np.zeros((T, N), np.int16) for k in ("ix", "iy", "iz")})
+    out["types"] = np.zeros(N, np.int32)
+    b = np.ascontiguousarray(box3, np.float32)
+    check(lib().amdat_read_xyz(str(path).encode(), N, T, fp(b), threads, fp(out["x"]), fp(out["y"]), fp(out["z"]),
+                               sp(out["ix"]), sp(out["iy"]), sp(out["iz"]), ip(out["types"])))
+    return out
 
 
 class AtomList(_Handle):

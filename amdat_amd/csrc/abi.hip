@@ -408,6 +408,121 @@ int amdat_traj_free(amdat_ctx *, amdat_traj *tr)
     return AMDAT_OK;
 }
 
+int amdat_traj_shape(const amdat_traj *tr, int *n_atoms, int *n_frames, float *box)
+{
+    if (!tr) return set_error(AMDAT_EINVAL, "amdat_traj_shape: null trajectory");
+    if (n_atoms) *n_atoms = tr->dev.N;
+    if (n_frames) *n_frames = tr->dev.T;
+    if (box) std::copy(tr->box.begin(), tr->box.end(), box);
+    return AMDAT_OK;
+}
+
+namespace {
+// Page-locked host buffer (hipHostMalloc), freed on scope exit.
+struct PinnedBuf {
+    void *p = nullptr;
+    ~PinnedBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+}  // namespace
+
+int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_atoms, int n_frames,
+                    const float *box3, int threads, int *types, amdat_traj **out)
+{
+    clear_error();
+    if (!ctx || !format || !path || !out || n_atoms < 1 || n_frames < 1)
+        return set_error(AMDAT_EINVAL, "amdat_traj_read: bad arguments");
+    const std::string fmt(format);
+    if (fmt != "custom" && fmt != "xyz") return set_error(AMDAT_EINVAL, "amdat_traj_read: format must be custom or xyz");
+    if (fmt == "xyz" && !box3) return set_error(AMDAT_EINVAL, "amdat_traj_read: xyz needs the box");
+    HIP_TRY(hipSetDevice(ctx->device));
+    MappedFile f;
+    int rc = f.open(path);
+    if (rc) return rc;
+    std::vector<size_t> off;
+    rc = fmt == "custom" ? index_custom(f, n_frames, threads, off) : index_xyz(f, n_atoms, n_frames, threads, off);
+    if (rc) return rc;
+    const int N = n_atoms, T = n_frames;
+    const size_t n = (size_t)N * T;
+    auto t = std::make_unique<amdat_traj>();
+    HIP_TRY(t->x.alloc(n));
+    HIP_TRY(t->y.alloc(n));
+    HIP_TRY(t->z.alloc(n));
+    HIP_TRY(t->ix.alloc(n));
+    HIP_TRY(t->iy.alloc(n));
+    HIP_TRY(t->iz.alloc(n));
+    // chunks of about 64 MiB of parsed frames (or AMDAT_READ_CHUNK_FRAMES),
+    // double-buffered in pinned memory
+    int CF = (int)std::max<size_t>(1, std::min<size_t>((size_t)T, (64u << 20) / ((size_t)N * 18)));
+    if (const char *e = std::getenv("AMDAT_READ_CHUNK_FRAMES")) CF = std::max(1, std::min(T, std::atoi(e)));
+    const size_t cn = (size_t)CF * N;
+    PinnedBuf stage[2];
+    hipEvent_t done[2] = {nullptr, nullptr};
+    for (int b = 0; b < 2; b++) {
+        HIP_TRY(hipHostMalloc(&stage[b].p, cn * 18, hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
+    }
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard()
+        {
+            for (int b = 0; b < 2; b++)
+                if (e[b]) (void)hipEventDestroy(e[b]);
+        }
+    } evg{done};
+    hipStream_t st = stream_of(ctx);
+    std::vector<float> box((size_t)3 * T), prev_xyz;
+    std::vector<int16_t> prev_img;
+    for (int f0 = 0, c = 0; f0 < T; f0 += CF, c++) {
+        const int f1 = std::min(T, f0 + CF), nf = f1 - f0, b = c & 1;
+        if (c >= 2) HIP_TRY(hipEventSynchronize(done[b]));  // the copy out of this buffer has finished
+        float *sx = (float *)stage[b].p, *sy = sx + cn, *sz = sy + cn;
+        int16_t *six = (int16_t *)(sz + cn), *siy = six + cn, *siz = siy + cn;
+        if (fmt == "custom") {
+            rc = parse_custom(f, off, f0, f1, N, threads, sx, sy, sz, six, siy, siz, box.data() + 3 * (size_t)f0,
+                              nullptr, f0 == 0 ? types : nullptr);
+        } else {
+            rc = parse_xyz(f, off, f0, f1, N, threads, sx, sy, sz, f0 == 0 ? types : nullptr);
+            if (!rc)
+                rc = unwrap_images(N, nf, box3, sx, sy, sz, six, siy, siz, f0 ? prev_xyz.data() : nullptr,
+                                   f0 ? prev_img.data() : nullptr, threads);
+            if (!rc) {
+                // the last frame of this chunk seeds the next one
+                const size_t last = (size_t)(nf - 1) * N;
+                prev_xyz.assign(sx + last, sx + last + N);
+                prev_xyz.insert(prev_xyz.end(), sy + last, sy + last + N);
+                prev_xyz.insert(prev_xyz.end(), sz + last, sz + last + N);
+                prev_img.assign(six + last, six + last + N);
+                prev_img.insert(prev_img.end(), siy + last, siy + last + N);
+                prev_img.insert(prev_img.end(), siz + last, siz + last + N);
+                for (int r = f0; r < f1; r++)
+                    for (int k = 0; k < 3; k++) box[3 * (size_t)r + k] = box3[k];
+            }
+        }
+        if (rc) {
+            (void)hipStreamSynchronize(st);
+            return rc;
+        }
+        const size_t o = (size_t)f0 * N, m = (size_t)nf * N;
+        HIP_TRY(hipMemcpyAsync(t->x.p + o, sx, m * sizeof(float), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(t->y.p + o, sy, m * sizeof(float), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(t->z.p + o, sz, m * sizeof(float), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(t->ix.p + o, six, m * sizeof(int16_t), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(t->iy.p + o, siy, m * sizeof(int16_t), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(t->iz.p + o, siz, m * sizeof(int16_t), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(done[b], st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    t->dev.x = t->x.p; t->dev.y = t->y.p; t->dev.z = t->z.p;
+    t->dev.ix = t->ix.p; t->dev.iy = t->iy.p; t->dev.iz = t->iz.p;
+    rc = traj_finish(ctx, t.get(), N, T, box.data());
+    if (rc) return rc;
+    *out = t.release();
+    return AMDAT_OK;
+}
+
 int amdat_list_create(amdat_ctx *ctx, const amdat_traj *tr, int n, const int *indices, amdat_list **out)
 {
     if (!ctx || !tr || !out || n < 0) return set_error(AMDAT_EINVAL, "amdat_list_create: bad arguments");

@@ -31,6 +31,29 @@ struct PlanLayout {
 };
 PlanLayout plan_layout(const int *nvec, int K, int R);
 
+// ---- trajectory ingestion (host_io.cpp) ----
+struct MappedFile {
+    const char *data = nullptr;
+    size_t size = 0;
+    int fd = -1;
+    int open(const char *path);
+    ~MappedFile();
+};
+int io_threads(int threads);
+// frame start offsets of a custom dump / xyz file (the first `want` frames)
+int index_custom(const MappedFile &f, int want, int threads, std::vector<size_t> &off);
+int index_xyz(const MappedFile &f, int N, int want, int threads, std::vector<size_t> &off);
+// frames [f0, f1) into rows 0 .. f1-f0 of the outputs
+int parse_custom(const MappedFile &f, const std::vector<size_t> &off, int f0, int f1, int N, int threads,
+                 float *x, float *y, float *z, int16_t *ix, int16_t *iy, int16_t *iz, float *box, float *lo,
+                 int *types);
+int parse_xyz(const MappedFile &f, const std::vector<size_t> &off, int f0, int f1, int N, int threads, float *x,
+              float *y, float *z, int *types);
+// image counts from consecutive-frame jumps; prev_xyz / prev_img ([3][N])
+// carry the frame before row 0 when a trajectory is read in pieces
+int unwrap_images(int N, int T, const float *L, const float *x, const float *y, const float *z, int16_t *ix,
+                  int16_t *iy, int16_t *iz, const float *prev_xyz, const int16_t *prev_img, int threads);
+
 }  // namespace amdat
 
 extern "C" {

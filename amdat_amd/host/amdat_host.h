@@ -127,7 +127,7 @@ class System {
 
   private:
     void init_timescheme(const std::vector<std::string> &args);
-    void read_custom(const std::string &file);
+    void read_trajectory(const std::string &trajtype, const std::string &file);
     void finish();
 
     int n_exponentials_ = 1, n_exponential_steps_ = 0, frt_ = 0, first_exponent_ = 0;
@@ -146,6 +146,7 @@ class System {
     std::vector<int> list_order_;               // reference list order over all atoms
     std::vector<int> atom_species_;             // species index per atom (file order)
     amdat_traj *traj_ = nullptr;
+    float xyz_box_[3] = {0, 0, 0};  // xyz: constant box lengths of the system block
 };
 
 class Wave_Vectors {

@@ -165,6 +165,9 @@ int Control::run()
     for (int i = 0; i < 6; i++) block.push_back(read_line());
     const size_t n_species = tokenize(block[4]).size() / 2;
     for (size_t s = 0; s < n_species; s++) block.push_back(read_line());
+    // xyz trajectories: box lengths and box bounds follow (src/system.cpp:385-397)
+    if (!tokenize(block[1]).empty() && tokenize(block[1])[0] == "xyz")
+        for (int i = 0; i < 2; i++) block.push_back(read_line());
     system_ = std::make_unique<System>(block);
     std::cout << "\nSystem created." << std::endl;
 

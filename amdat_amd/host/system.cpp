@@ -105,9 +105,10 @@ void System::finish()
     check(amdat_ts_timelist(n_exponentials_, n_exponential_steps_, exp_base_, first_exponent_, time_unit_,
                             timelist_.data(), n_timesteps_),
           "time list");
-    check(amdat_traj_upload(device_context(), n_atoms_, n_timesteps_, x_.data(), y_.data(), z_.data(), ix_.data(),
-                            iy_.data(), iz_.data(), box_.data(), &traj_),
-          "trajectory upload");
+    if (!traj_)
+        check(amdat_traj_upload(device_context(), n_atoms_, n_timesteps_, x_.data(), y_.data(), z_.data(),
+                                ix_.data(), iy_.data(), iz_.data(), box_.data(), &traj_),
+              "trajectory upload");
 }
 
 System::System(int nexp, int nes, float base, int frt, int first_exp, float dt, int n_atoms, const float *x,
@@ -147,8 +148,8 @@ System::System(const std::vector<std::string> &block)
     if (systype != "system" && systype != "system_nv" && systype != "system_np")
         throw Error("unknown system type " + systype, 0);
     const std::string trajtype = tokenize(block[1]).at(0);
-    if (trajtype != "custom")
-        throw Error("trajectory type " + trajtype + " is not supported by this build (custom only)", 0);
+    if (trajtype != "custom" && trajtype != "xyz")
+        throw Error(std::string("Trajectory file type ") + trajtype + " not recognized.", 0);
     auto files = tokenize(block[2]);
     if (files.empty()) throw Error("missing trajectory file name", 0);
     init_timescheme(tokenize(block[3]));
@@ -166,95 +167,37 @@ System::System(const std::vector<std::string> &block)
         if (row.size() != type_ids_.size()) throw Error("type count row does not match the types line", 0);
         species_type_counts_.push_back(row);
     }
-    read_custom(files[0]);
+    if (trajtype == "xyz") {
+        // box lengths, then bounds (src/system.cpp:385-397)
+        const size_t at = 6 + species_names_.size();
+        if (block.size() < at + 2) throw Error("xyz system block needs the box and bounds lines", 0);
+        auto L = tokenize(block[at]);
+        if (L.size() < 3) throw Error("xyz box line needs L_x L_y L_z", 0);
+        for (int c = 0; c < 3; c++) xyz_box_[c] = (float)std::atof(L[c].c_str());
+    }
+    read_trajectory(trajtype, files[0]);
     finish();
 }
 
-// LAMMPS custom dump with wrapped x y z and image counts ix iy iz (or
-// unwrapped xu yu zu, from which wrapped coordinates and images follow),
-// atoms in the same order in every frame (src/system.cpp:896-1367).
-void System::read_custom(const std::string &file)
+// Trajectory file straight into device memory (amdat_traj_read): a LAMMPS
+// custom dump (x y z with ix iy iz, or xu yu zu; src/system.cpp:1026-1367)
+// or an xyz file (src/system.cpp:610-705, images by the reference's
+// unwrapping rule), atoms in the same order in every frame.
+void System::read_trajectory(const std::string &trajtype, const std::string &file)
 {
-    std::ifstream in(file);
-    if (!in) throw Error("Error opening trajectory file " + file, 0);
-    int per_mol_total = 0;
     n_atoms_ = 0;
     for (size_t s = 0; s < species_names_.size(); s++) {
         int per = 0;
         for (int c : species_type_counts_[s]) per += c;
         n_atoms_ += per * species_mols_[s];
-        per_mol_total += per;
     }
-    (void)per_mol_total;
     const int N = n_atoms_, T = n_timesteps_;
-    x_.assign((size_t)N * T, 0); y_.assign((size_t)N * T, 0); z_.assign((size_t)N * T, 0);
-    ix_.assign((size_t)N * T, 0); iy_.assign((size_t)N * T, 0); iz_.assign((size_t)N * T, 0);
-    box_.assign(3 * (size_t)T, 0);
     type_.assign(N, 0);
-    std::string line;
-    for (int t = 0; t < T; t++) {
-        int natoms = -1;
-        float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-        std::vector<std::string> cols;
-        while (std::getline(in, line)) {
-            if (line.rfind("ITEM: NUMBER OF ATOMS", 0) == 0) {
-                std::getline(in, line);
-                natoms = std::atoi(line.c_str());
-            } else if (line.rfind("ITEM: BOX BOUNDS", 0) == 0) {
-                for (int c = 0; c < 3; c++) {
-                    std::getline(in, line);
-                    auto b = tokenize(line);
-                    lo[c] = (float)std::atof(b.at(0).c_str());
-                    hi[c] = (float)std::atof(b.at(1).c_str());
-                }
-            } else if (line.rfind("ITEM: ATOMS", 0) == 0) {
-                auto h = tokenize(line);
-                cols.assign(h.begin() + 2, h.end());
-                break;
-            }
-        }
-        if (cols.empty()) throw Error("trajectory file ended before frame " + std::to_string(t), 0);
-        if (natoms != N) throw Error("frame atom count does not match the system block", 0);
-        std::map<std::string, int> at;
-        for (size_t c = 0; c < cols.size(); c++) at[cols[c]] = (int)c;
-        const bool wrapped = at.count("x") && at.count("y") && at.count("z");
-        const bool unwrapped = at.count("xu") && at.count("yu") && at.count("zu");
-        const bool images = at.count("ix") && at.count("iy") && at.count("iz");
-        if (!at.count("type")) throw Error("custom file lacks the type column", 0);
-        if (!(wrapped && images) && !unwrapped)
-            throw Error("custom file needs x y z with ix iy iz, or xu yu zu", 0);
-        float L[3];
-        for (int c = 0; c < 3; c++) {
-            L[c] = hi[c] - lo[c];
-            box_[3 * t + c] = L[c];
-        }
-        for (int j = 0; j < N; j++) {
-            if (!std::getline(in, line)) throw Error("trajectory file truncated", 0);
-            auto v = tokenize(line);
-            const size_t o = (size_t)t * N + j;
-            type_[j] = std::atoi(v.at(at["type"]).c_str());
-            float w[3];
-            int16_t im[3];
-            if (wrapped && images && !unwrapped) {
-                const char *nm[3] = {"x", "y", "z"}, *inm[3] = {"ix", "iy", "iz"};
-                for (int c = 0; c < 3; c++) {
-                    w[c] = (float)std::atof(v.at(at[nm[c]]).c_str());
-                    im[c] = (int16_t)std::atoi(v.at(at[inm[c]]).c_str());
-                }
-            } else {
-                // unwrapped given: wrapped = u - L*floor((u-lo)/L) (src/trajectory.cpp wrap())
-                const char *nm[3] = {"xu", "yu", "zu"};
-                for (int c = 0; c < 3; c++) {
-                    const double u = (float)std::atof(v.at(at[nm[c]]).c_str());
-                    const double k = std::floor((u - lo[c]) / L[c]);
-                    w[c] = (float)(u - k * L[c]);
-                    im[c] = (int16_t)k;
-                }
-            }
-            x_[o] = w[0]; y_[o] = w[1]; z_[o] = w[2];
-            ix_[o] = im[0]; iy_[o] = im[1]; iz_[o] = im[2];
-        }
-    }
+    check(amdat_traj_read(device_context(), trajtype.c_str(), file.c_str(), N, T,
+                          trajtype == "xyz" ? xyz_box_ : nullptr, 0, type_.data(), &traj_),
+          "reading the trajectory");
+    box_.assign(3 * (size_t)T, 0.0f);
+    check(amdat_traj_shape(traj_, nullptr, nullptr, box_.data()), "trajectory shape");
     // Reference list order (src/system.cpp:4783-4791): species -> molecule
     // -> type (types-line order) -> index within the molecule.
     list_order_.clear();

@@ -152,6 +152,41 @@ int amdat_traj_wrap_device(amdat_ctx *ctx, int n_atoms, int n_frames, const floa
                            const int16_t *d_iy, const int16_t *d_iz, const float *box_host,
                            amdat_traj **out);
 int amdat_traj_free(amdat_ctx *ctx, amdat_traj *tr);
+/* Atoms, frames and (box != NULL) the per-frame box lengths [n_frames][3]
+ * of a trajectory. */
+int amdat_traj_shape(const amdat_traj *tr, int *n_atoms, int *n_frames, float *box);
+
+/* ---- trajectory files (src/system.cpp:237-299 read_trajectory) ---------- */
+
+/* Parse a LAMMPS custom dump (src/system.cpp:1026-1367 read_custom) into
+ * frame-major SoA host arrays [n_frames][n_atoms], atoms in file order:
+ * columns x y z with ix iy iz; or xu yu zu (wrapped coordinates and images
+ * then follow from the box: image = floor((u - lo)/L)); or x y z alone
+ * (images 0).  box [n_frames][3] = hi - lo; box_lo (optional) = lo; types
+ * (optional) = the type column of frame 0.  Frames are parsed in parallel
+ * on `threads` threads (0: all hardware threads).  Fails if the file holds
+ * fewer than n_frames frames or a frame's atom count differs. */
+int amdat_read_custom(const char *path, int n_atoms, int n_frames, int threads, float *x, float *y, float *z,
+                      int16_t *ix, int16_t *iy, int16_t *iz, float *box, float *box_lo, int *types);
+
+/* Parse an xyz trajectory (src/system.cpp:610-705 read_xyz; the box is
+ * constant, box3 = L_x L_y L_z of the system block, src/system.cpp:385-397):
+ * frames of a count line, a comment line and "type x y z" lines.  Image
+ * counts follow the reference's unwrapping (src/trajectory.cpp:233-270):
+ * between consecutive frames the FP32 jump d moves an atom by
+ * -L * int(d / (0.55 L)). */
+int amdat_read_xyz(const char *path, int n_atoms, int n_frames, const float *box3, int threads, float *x,
+                   float *y, float *z, int16_t *ix, int16_t *iy, int16_t *iz, int *types);
+
+/* Read a trajectory file straight into device memory: format "custom" or
+ * "xyz" (box3 required for xyz, ignored for custom).  Frames are parsed in
+ * parallel into pinned staging buffers, chunk by chunk, and each chunk's
+ * copy to the device runs while the next chunk is parsed; no host copy of
+ * the trajectory is kept.  types (optional, host [n_atoms]) receives frame
+ * 0's atom types.  Replaces the reader plus the upload of the reference's
+ * System construction (src/system.cpp:152). */
+int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_atoms, int n_frames,
+                    const float *box3, int threads, int *types, amdat_traj **out);
 
 /* Static atom selection (Static_Trajectory_List, src/static_trajectory_
  * list.cpp:59-101): atom indices in list order.  indices == NULL with
