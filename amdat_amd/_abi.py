@@ -67,6 +67,7 @@ SIGNATURES = {
     "amdat_fkt": (_I, [_P, _P, _P, _P, _I, _I, _PD, _PD]),
     "amdat_fs": (_I, [_P, _P, _P, _P, _I, _I, _F, _P, _I, _PD, _PD]),
     "amdat_msd_ngp": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
+    "amdat_displacements": (_I, [_P, _P, _P, _P, _I, _F, _PD, _PD, C.POINTER(C.c_ulonglong), _PD]),
     "amdat_sk_loss_grad": (_I, [_P, _I, _P, _P, _P, _P, _I, _I, _F, _PD, _PD, _PD, _PD, _P, _P, _P]),
     "amdat_batch_create": (_I, [_P, _I, _PP, _PP, _PP]),
     "amdat_batch_free": (_I, [_P, _P]),
@@ -415,6 +416,19 @@ def msd_ngp(ctx, traj, alist, pairs):
     check(lib().amdat_msd_ngp(ctx.h, traj.h, alist.h if alist is not None else None, pairs.h, dp(s2), dp(s4),
                               dp(cnt)))
     return s2, s4, cnt
+
+
+def displacements(ctx, traj, alist, pairs, n_bins=0, max_range=0.0):
+    """Per time gap: sums of d [ntg, 3], of d^2 [ntg, 3], counts [ntg] and
+    (n_bins > 0) the van Hove histogram [ntg, n_bins] (uint64)."""
+    sd = np.zeros((pairs.ntg, 3))
+    sd2 = np.zeros((pairs.ntg, 3))
+    cnt = np.zeros(pairs.ntg)
+    hist = np.zeros((pairs.ntg, max(n_bins, 1)), np.uint64)
+    check(lib().amdat_displacements(ctx.h, traj.h, alist.h if alist is not None else None, pairs.h, n_bins,
+                                    max_range, dp(sd), dp(sd2), hist.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+                                    dp(cnt)))
+    return sd, sd2, cnt, (hist if n_bins > 0 else None)
 
 
 def sk_loss_grad(ctx, n_atoms, x_ptr, y_ptr, z_ptr, shells, first, last, length_scale, weights, target,

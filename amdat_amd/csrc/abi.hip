@@ -834,6 +834,59 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
     return AMDAT_OK;
 }
 
+int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                        int n_bins, float max_range, double *sum_d, double *sum_d2, unsigned long long *hist,
+                        double *count)
+{
+    clear_error();
+    if (!ctx || !tr || !pairs || !sum_d || !sum_d2 || !count)
+        return set_error(AMDAT_EINVAL, "amdat_displacements: null argument");
+    if (n_bins < 0 || n_bins > 16384 || (n_bins > 0 && (!hist || !(max_range > 0.0f))))
+        return set_error(AMDAT_EINVAL, "amdat_displacements: bins must be 0..16384 with a histogram and max_range > 0");
+    int rc = check_pairs_frames(pairs, tr->dev.T);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int nl = list_n(tr, list);
+    const int ntg = pairs->ntg;
+    for (int g = 0; g < ntg; g++) count[g] = 0.0;
+    for (int i = 0; i < pairs->n; i++) count[pairs->tg[i]] += (double)nl;
+    std::fill(sum_d, sum_d + 3 * (size_t)ntg, 0.0);
+    std::fill(sum_d2, sum_d2 + 3 * (size_t)ntg, 0.0);
+    if (n_bins > 0) std::fill(hist, hist + (size_t)ntg * n_bins, 0ull);
+    if (pairs->n == 0 || nl == 0) return AMDAT_OK;
+    hipStream_t st = stream_of(ctx);
+    const int nb = std::max(1, std::min(64, (nl + kBlock * 4 - 1) / (kBlock * 4)));
+    const float bin_size = n_bins > 0 ? max_range / (float)n_bins : 1.0f;  // src/space_time_correlation_function.cpp
+    DevBuf<double> part, out;
+    DevBuf<unsigned long long> dh;
+    DevBuf<int> dtg;
+    HIP_TRY(part.alloc((size_t)pairs->n * nb * 6));
+    HIP_TRY(dtg.upload(pairs->tg.data(), pairs->tg.size(), st));
+    if (n_bins > 0) {
+        HIP_TRY(dh.alloc((size_t)ntg * n_bins));
+        HIP_TRY(hipMemsetAsync(dh.p, 0, dh.n * sizeof(unsigned long long), st));
+    }
+    const int pb = 65535;
+    for (int p0 = 0; p0 < pairs->n; p0 += pb) {
+        const int np = std::min(pb, pairs->n - p0);
+        HIP_TRY(launch_disp(tr->dev, list_ptr(list), nl, pairs->d_t0.p + p0, pairs->d_t1.p + p0, dtg.p + p0, np, nb,
+                            n_bins, bin_size, part.p + (size_t)p0 * nb * 6, dh.p, st));
+    }
+    HIP_TRY(out.alloc((size_t)ntg * 6));
+    HIP_TRY(launch_disp_reduce(part.p, nb, pairs->d_group_off.p, pairs->d_order.p, ntg, out.p, st));
+    std::vector<double> h((size_t)ntg * 6);
+    HIP_TRY(hipMemcpyAsync(h.data(), out.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    if (n_bins > 0)
+        HIP_TRY(hipMemcpyAsync(hist, dh.p, sizeof(unsigned long long) * dh.n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int g = 0; g < ntg; g++)
+        for (int c = 0; c < 3; c++) {
+            sum_d[3 * g + c] = h[6 * (size_t)g + c];
+            sum_d2[3 * g + c] = h[6 * (size_t)g + 3 + c];
+        }
+    return AMDAT_OK;
+}
+
 int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const float *d_y, const float *d_z,
                        const amdat_shells *sh_c, int first_shell, int last_shell, float length_scale,
                        const double *weights, const double *target, double *loss, double *S, double *d_gx,

@@ -96,6 +96,15 @@ hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *grou
 // Per (pair, atom block): FP64 sums of |dr|^2 and |dr|^4.
 hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int *t0,
                       const int *t1, int n_pairs, int n_blocks, double *partial, hipStream_t st);
+// Per (pair, atom block): FP64 sums of d_c and d_c^2 into partial[(p*n_blocks
+// + block)*6 + k]; with n_bins > 0 the reference-form |dr| counts are added to
+// hist[tg[p]*n_bins + bin] (bin = int(|dr| / bin_size)).
+hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, const int *tg,
+                       int n_pairs, int n_blocks, int n_bins, float bin_size, double *partial,
+                       unsigned long long *hist, hipStream_t st);
+// Per group: the six sums of its pairs' partials (pairs in order[]).
+hipError_t launch_disp_reduce(const double *partial, int n_blocks, const int *group_off, const int *order,
+                              int n_groups, double *out, hipStream_t st);
 // Per group: sums of the partials of its pairs.
 hipError_t launch_msd_reduce(const double *partial, int n_blocks, const int *group_off,
                              const int *order, int n_groups, double *s2, double *s4,

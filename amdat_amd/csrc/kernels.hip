@@ -234,6 +234,86 @@ __global__ __launch_bounds__(kBlock) void msd_kernel(
     }
 }
 
+// Displacement statistics per (pair, atom block): FP64 sums of d_c and
+// d_c^2 (c = x, y, z) from the FP64 rebuild d = (x1 - x0) + (i1 L1 - i0 L0),
+// into partial[(p * blocks + block) * 6 + k] (k = dx, dy, dz, dx^2, dy^2,
+// dz^2).  With n_bins > 0 also the self van Hove counts: |dr| as the
+// reference forms it (unwrapped FP32 u = float(x + i L), FP32 difference,
+// sqrtf((dx dx + dy dy) + dz dz)), bin = int(|dr| / bin_size), counted in
+// LDS and added to hist[TG[p] * n_bins + bin] (integer atomics: the counts
+// do not depend on the order).
+__global__ __launch_bounds__(kBlock) void disp_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+    const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int *__restrict__ T0,
+    const int *__restrict__ T1, const int *__restrict__ TG, int n_bins, float bin_size, double *__restrict__ partial,
+    unsigned long long *__restrict__ hist)
+{
+#pragma clang fp contract(off)
+    extern __shared__ unsigned int sh_hist[];
+    __shared__ double red[kBlock];
+    const int p = blockIdx.y;
+    const int t0 = T0[p], t1 = T1[p];
+    const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
+    const float *B0 = box + 3 * t0, *B1 = box + 3 * t1;
+    for (int b = threadIdx.x; b < n_bins; b += kBlock) sh_hist[b] = 0u;
+    if (n_bins > 0) __syncthreads();
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const float *src[3] = {X, Y, Z};
+    const int16_t *img[3] = {IX, IY, IZ};
+    for (int a = blockIdx.x * kBlock + threadIdx.x; a < n_list; a += gridDim.x * kBlock) {
+        const int j = list ? list[a] : a;
+        float u[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double x0 = src[c][o0 + j], x1 = src[c][o1 + j];
+            const double w0 = (double)img[c][o0 + j] * (double)B0[c], w1 = (double)img[c][o1 + j] * (double)B1[c];
+            const double d = (x1 - x0) + (w1 - w0);
+            s[c] += d;
+            s[3 + c] += d * d;
+            u[c] = (float)(x1 + w1) - (float)(x0 + w0);
+        }
+        if (n_bins > 0) {
+            const float len = sqrtf((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+            const int bin = (int)(len / bin_size);
+            if (bin >= 0 && bin < n_bins) atomicAdd(&sh_hist[bin], 1u);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) s[k] = block_sum(s[k], red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) partial[((size_t)p * gridDim.x + blockIdx.x) * 6 + k] = s[k];
+    if (n_bins > 0) {
+        __syncthreads();
+        unsigned long long *row = hist + (size_t)TG[p] * n_bins;
+        for (int b = threadIdx.x; b < n_bins; b += kBlock)
+            if (sh_hist[b]) atomicAdd(&row[b], (unsigned long long)sh_hist[b]);
+    }
+}
+
+// Per time gap: the six sums of its pairs' partials, pairs in reference order.
+__global__ __launch_bounds__(kBlock) void disp_reduce_kernel(const double *__restrict__ partial, int n_blocks,
+                                                              const int *__restrict__ group_off,
+                                                              const int *__restrict__ order,
+                                                              double *__restrict__ out)
+{
+    __shared__ double red[kBlock];
+    const int g = blockIdx.x;
+    const int p0 = group_off[g], np = group_off[g + 1] - p0;
+    const long items = (long)np * n_blocks;
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long it = threadIdx.x; it < items; it += kBlock) {
+        const int pl = (int)(it / n_blocks);
+        const int b = (int)(it - (long)pl * n_blocks);
+        const size_t e = ((size_t)order[p0 + pl] * n_blocks + b) * 6;
+        for (int k = 0; k < 6; k++) s[k] += partial[e + k];
+    }
+    for (int k = 0; k < 6; k++) {
+        const double v = block_sum(s[k], red);
+        if (threadIdx.x == 0) out[(size_t)g * 6 + k] = v;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void msd_reduce_kernel(const double *__restrict__ partial, int n_blocks,
                                                              const int *__restrict__ group_off,
                                                              const int *__restrict__ order,
@@ -547,6 +627,27 @@ hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int 
     (void)hipGetLastError();
     hipLaunchKernelGGL(msd_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
                        tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, const int *tg,
+                       int n_pairs, int n_blocks, int n_bins, float bin_size, double *partial,
+                       unsigned long long *hist, hipStream_t st)
+{
+    if (n_pairs <= 0) return hipSuccess;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(disp_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), sizeof(unsigned int) * (size_t)n_bins, st,
+                       tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, tg, n_bins, bin_size,
+                       partial, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_disp_reduce(const double *partial, int n_blocks, const int *group_off, const int *order,
+                              int n_groups, double *out, hipStream_t st)
+{
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(disp_reduce_kernel, dim3(n_groups), dim3(kBlock), 0, st, partial, n_blocks, group_off, order,
+                       out);
     return hipGetLastError();
 }
 

@@ -282,6 +282,60 @@ class Mean_Square_Displacement {
     std::vector<double> msd64_, s4_, count_;
 };
 
+// Displacement statistics shared by the three analyses below: one device
+// pass (amdat_displacements) over the fullblock = false pairs.
+struct Displacement_Stats {
+    std::vector<double> sum_d, sum_d2, count;  // [gap][3], [gap][3], [gap]
+    std::vector<unsigned long long> hist;      // [gap][bins] when bins > 0
+    void run(System *sys, Trajectory_List *t_list, int n_bins, float max_range);
+};
+
+// src/mean_square_displacement_2d.cpp: MSD in the xy, xz or yz plane.
+class Mean_Square_Displacement_2D {
+  public:
+    Mean_Square_Displacement_2D(System *sys, const std::string &plane);
+    void analyze(Trajectory_List *t_list);
+    void write(const std::string &filename) const;
+    float show(int t) const { return msd_[t]; }
+
+  private:
+    System *system_;
+    std::string plane_;
+    int a0_ = 0, a1_ = 1;
+    std::vector<float> timetable_, msd_;
+};
+
+// src/mean_displacement.cpp: mean displacement vector per time gap.
+class Mean_Displacement {
+  public:
+    explicit Mean_Displacement(System *sys);
+    void analyze(Trajectory_List *t_list);
+    void write(const std::string &filename) const;
+    Coordinate show(int t) const { return md_[t]; }
+
+  private:
+    System *system_;
+    std::vector<float> timetable_;
+    std::vector<Coordinate> md_;
+};
+
+// src/van_hove_self.cpp + src/space_time_correlation_function.cpp: self
+// part of the van Hove function, counts of |dr| per bin over atom pairs /
+// (weighting * shell volume).
+class Van_Hove_Self {
+  public:
+    Van_Hove_Self(System *sys, int bin_count, float value_max);
+    void analyze(Trajectory_List *t_list);
+    void write(const std::string &filename) const;
+    float show(int t, int b) const { return corr_[(size_t)t * n_bins_ + b]; }
+
+  private:
+    System *system_;
+    int n_bins_;
+    float max_value_, bin_size_;
+    std::vector<float> timetable_, corr_;
+};
+
 class Non_Gaussian_Parameter {
   public:
     Non_Gaussian_Parameter(System *sys, const Mean_Square_Displacement *msd);
@@ -308,6 +362,9 @@ class Control {
     void create_list(const std::vector<std::string> &args);
     void msd(const std::vector<std::string> &args);
     void ngp(const std::vector<std::string> &args);
+    void msd_2d(const std::vector<std::string> &args);
+    void mean_displacement(const std::vector<std::string> &args);
+    void vhs(const std::vector<std::string> &args);
     void isfs(const std::vector<std::string> &args);
     void isf(const std::vector<std::string> &args);
     void structure_factor(const std::vector<std::string> &args);

@@ -420,4 +420,117 @@ int Non_Gaussian_Parameter::max() const
     return maxtime;
 }
 
+
+// ---------------------------------------------------------------------------
+// Displacement statistics (src/mean_square_displacement_2d.cpp,
+// src/mean_displacement.cpp, src/van_hove_self.cpp): pairs with
+// fullblock = false (System::displacement_list's default)
+
+void Displacement_Stats::run(System *sys, Trajectory_List *t_list, int n_bins, float max_range)
+{
+    amdat_pairs *p = sys->displacement_pairs(false);
+    const int ntg = sys->show_n_timegaps();
+    sum_d.assign(3 * (size_t)ntg, 0.0);
+    sum_d2.assign(3 * (size_t)ntg, 0.0);
+    count.assign(ntg, 0.0);
+    hist.assign((size_t)ntg * std::max(n_bins, 1), 0ull);
+    int rc = amdat_displacements(device_context(), sys->traj(), t_list->handle(), p, n_bins, max_range,
+                                 sum_d.data(), sum_d2.data(), hist.data(), count.data());
+    amdat_pairs_free(device_context(), p);
+    check(rc, "displacement statistics");
+}
+
+Mean_Square_Displacement_2D::Mean_Square_Displacement_2D(System *sys, const std::string &plane)
+    : system_(sys), plane_(plane)
+{
+    if (plane == "xy") { a0_ = 0; a1_ = 1; }
+    else if (plane == "xz") { a0_ = 0; a1_ = 2; }
+    else if (plane == "yz") { a0_ = 1; a1_ = 2; }
+    else throw Error("Error: plane command " + plane + " not understood.", 1);
+    timetable_ = sys->displacement_times();
+    msd_.assign(sys->show_n_timegaps(), 0.0f);
+}
+
+void Mean_Square_Displacement_2D::analyze(Trajectory_List *t_list)
+{
+    Displacement_Stats d;
+    d.run(system_, t_list, 0, 0.0f);
+    for (size_t t = 0; t < msd_.size(); t++)
+        msd_[t] = (float)((d.sum_d2[3 * t + a0_] + d.sum_d2[3 * t + a1_]) / d.count[t]);
+}
+
+void Mean_Square_Displacement_2D::write(const std::string &filename) const
+{
+    std::cout << "\nWriting msd to file.";
+    std::ofstream out(filename);
+    out << "2-D mean square displacement data for " << plane_ << " plane created by AMDAT v." << kSemver << "\n";
+    for (size_t t = 0; t < msd_.size(); t++) out << timetable_[t] << "\t" << msd_[t] << "\n";
+}
+
+Mean_Displacement::Mean_Displacement(System *sys) : system_(sys)
+{
+    timetable_ = sys->displacement_times();
+    md_.assign(sys->show_n_timegaps(), Coordinate());
+}
+
+void Mean_Displacement::analyze(Trajectory_List *t_list)
+{
+    Displacement_Stats d;
+    d.run(system_, t_list, 0, 0.0f);
+    for (size_t t = 0; t < md_.size(); t++)
+        md_[t] = Coordinate((float)(d.sum_d[3 * t] / d.count[t]), (float)(d.sum_d[3 * t + 1] / d.count[t]),
+                            (float)(d.sum_d[3 * t + 2] / d.count[t]));
+}
+
+void Mean_Displacement::write(const std::string &filename) const
+{
+    std::cout << "\nWriting mean displacement to file " << filename << ".";
+    std::ofstream out(filename);
+    out << "Mean displacement data created by AMDAT v." << kSemver << "\n";
+    for (size_t t = 0; t < md_.size(); t++)
+        out << timetable_[t] << "\t" << md_[t].show_x() << "\t" << md_[t].show_y() << "\t" << md_[t].show_z()
+            << "\n";
+}
+
+Van_Hove_Self::Van_Hove_Self(System *sys, int bin_count, float value_max)
+    : system_(sys), n_bins_(bin_count), max_value_(value_max)
+{
+    if (bin_count < 1 || bin_count > 16384) throw Error("vhs: number of bins must be 1..16384", 0);
+    bin_size_ = max_value_ / (float)n_bins_;
+    timetable_ = sys->displacement_times();
+    corr_.assign((size_t)sys->show_n_timegaps() * n_bins_, 0.0f);
+}
+
+void Van_Hove_Self::analyze(Trajectory_List *t_list)
+{
+    Displacement_Stats d;
+    d.run(system_, t_list, n_bins_, max_value_);
+    const int ntg = (int)timetable_.size();
+    for (int b = 0; b < n_bins_; b++) {
+        // src/space_time_correlation_function.cpp:345-360
+        const float rshell = b * bin_size_;
+        const float shellvolume = (float)((4.0 / 3.0) * kPi * (std::pow(rshell + bin_size_, 3.0) - std::pow(rshell, 3.0)));
+        for (int t = 0; t < ntg; t++)
+            corr_[(size_t)t * n_bins_ + b] =
+                (float)((double)d.hist[(size_t)t * n_bins_ + b] / (d.count[t] * (double)shellvolume));
+    }
+}
+
+void Van_Hove_Self::write(const std::string &filename) const
+{
+    std::ofstream out(filename);
+    out << "Correlation data created by AMDAT v." << kSemver << "\n";
+    out << n_bins_ << " bins\n";
+    out << timetable_.size() << " times\n\n";
+    std::cout << "\nWriting correlation function to file " << filename << ".";
+    out << "\t";
+    for (int b = 0; b < n_bins_; b++) out << bin_size_ / 2 + float(b) * bin_size_ << "\t";
+    out << "\n";
+    for (size_t t = 0; t < timetable_.size(); t++) {
+        out << timetable_[t] << "\t";
+        for (int b = 0; b < n_bins_ - 1; b++) out << corr_[t * n_bins_ + b] << "\t";
+        out << corr_[t * n_bins_ + n_bins_ - 1] << "\n";
+    }
+}
+
 }  // namespace amdat_hip

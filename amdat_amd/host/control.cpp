@@ -4,7 +4,8 @@
 // (comments with #, a system block, then command lines each followed by
 // their target lines); command syntax: src/control.cpp:1860-1916 (limit,
 // create_list), 2163-2210 (msd), 2655-2758 (structure_factor), 3107-3192
-// (isf), 3198-3238 (isfs), 3345-3370 (ngp).
+// (isf), 3198-3238 (isfs), 3345-3370 (ngp), 2219-2271 (msd_2d,
+// mean_displacement), 2517-2550 (vhs).
 #include <cstdlib>
 #include <ctime>
 #include <fstream>
@@ -78,6 +79,45 @@ void Control::msd(const std::vector<std::string> &args)
     m.analyze(l);
     m.write(args[1]);
     std::cout << "\nCalculated mean square displacement in " << time(nullptr) - start << " seconds." << std::endl;
+}
+
+// msd_2d <file> <plane> (src/control.cpp:2219-2243)
+void Control::msd_2d(const std::vector<std::string> &args)
+{
+    if (args.size() != 3) throw Error("msd_2d expects an output file and a plane", 0);
+    Trajectory_List *l = target(read_line());
+    Mean_Square_Displacement_2D m(system_.get(), args[2]);
+    std::cout << "\nCalculating mean square displacement.";
+    const time_t start = time(nullptr);
+    m.analyze(l);
+    m.write(args[1]);
+    std::cout << "\nCalculated mean square displacement in " << time(nullptr) - start << " seconds." << std::endl;
+}
+
+// mean_displacement <file> (src/control.cpp:2249-2271)
+void Control::mean_displacement(const std::vector<std::string> &args)
+{
+    if (args.size() != 2) throw Error("mean_displacement expects an output file", 0);
+    Trajectory_List *l = target(read_line());
+    Mean_Displacement m(system_.get());
+    std::cout << "\nCalculating mean displacement.\n";
+    const time_t start = time(nullptr);
+    m.analyze(l);
+    m.write(args[1]);
+    std::cout << "\nCalculated mean displacement in " << time(nullptr) - start << " seconds.";
+}
+
+// vhs <file> <max_range> <n_bins> (src/control.cpp:2517-2550)
+void Control::vhs(const std::vector<std::string> &args)
+{
+    if (args.size() != 4) throw Error("vhs expects an output file, a maximum range and a bin count", 0);
+    Trajectory_List *l = target(read_line());
+    Van_Hove_Self v(system_.get(), std::atoi(args[3].c_str()), (float)std::atof(args[2].c_str()));
+    std::cout << "\nCalculating self part of Van Hove correlation function.";
+    const time_t start = time(nullptr);
+    v.analyze(l);
+    std::cout << "\nCalculated self Van Hove in " << time(nullptr) - start << " seconds.";
+    v.write(args[1]);
 }
 
 void Control::ngp(const std::vector<std::string> &args)
@@ -191,6 +231,9 @@ int Control::run()
             else if (cmd == "isfs") isfs(args);
             else if (cmd == "isf") isf(args);
             else if (cmd == "structure_factor") structure_factor(args);
+            else if (cmd == "msd_2d") msd_2d(args);
+            else if (cmd == "mean_displacement") mean_displacement(args);
+            else if (cmd == "vhs") vhs(args);
             else if (cmd == "print") { std::cout << line.substr(line.find("print") + 5) << std::endl; }
             else throw Error("command " + cmd + " is not supported by this build", 0);
         } catch (const Error &e) {

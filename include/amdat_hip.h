@@ -258,6 +258,23 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
 int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
                   const amdat_pairs *pairs, double *s2, double *s4, double *count);
 
+/* Displacement statistics over the pairs (src/mean_displacement.cpp:163-176,
+ * src/mean_square_displacement_2d.cpp:144-156, src/van_hove_self.cpp:98-102):
+ * per time gap g, FP64 sums over its pairs and the list's atoms of the
+ * displacement components d_c (sum_d[3g + c]) and their squares
+ * (sum_d2[3g + c]; a plane's MSD is the sum of two of them), with
+ * d = (x1 - x0) + (i1 L1 - i0 L0) rebuilt in FP64; count[g] = atom pairs.
+ * With n_bins > 0, hist[g*n_bins + b] counts the atom pairs whose |dr|,
+ * formed as the reference forms it (FP32 unwrapped coordinates
+ * float(x + i L), FP32 difference and length: src/system.cpp:1292-1294,
+ * src/coordinate.h:55), falls in bin b = int(|dr| / (max_range / n_bins))
+ * (src/space_time_correlation_function.cpp:216-230), exactly; n_bins <=
+ * 16384.  The reference keeps these counts in FP32, which stops counting
+ * exactly past 2^24 per bin; here they are 64-bit integers. */
+int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
+                        const amdat_pairs *pairs, int n_bins, float max_range, double *sum_d,
+                        double *sum_d2, unsigned long long *hist, double *count);
+
 /* ---- differentiation (no reference counterpart) ----------------------- */
 
 /* Structure-matching loss on one frame given by device FP32 coordinate

@@ -372,6 +372,37 @@ void orc_msd_ngp_f64(int N, const float *x, const float *y, const float *z,
     }
 }
 
+/* Displacement sums per time gap in FP64 (Mean_Displacement::listkernel,
+ * src/mean_displacement.cpp:163-176; Mean_Square_Displacement_2D::
+ * listkernel, src/mean_square_displacement_2d.cpp:144-156, whose plane MSD
+ * is the sum of two of the squared components): sum_d[3g+c] += d_c,
+ * sum_d2[3g+c] += d_c^2 over the gap's pairs and the atoms. */
+void orc_disp_f64(int N, const float *x, const float *y, const float *z,
+                  const int16_t *ix, const int16_t *iy, const int16_t *iz, const float *box,
+                  const int *atoms, int n_list, int n_pairs, const int *tg, const int *t0,
+                  const int *t1, int ntg, double *sum_d, double *sum_d2, double *count)
+{
+    int cnt = atoms ? n_list : N;
+    for (int i = 0; i < 3 * ntg; i++) { sum_d[i] = 0.0; sum_d2[i] = 0.0; }
+    for (int g = 0; g < ntg; g++) count[g] = 0.0;
+    for (int p = 0; p < n_pairs; p++) {
+        size_t o0 = (size_t)t0[p] * N, o1 = (size_t)t1[p] * N;
+        const float *b0 = box + 3*t0[p], *b1 = box + 3*t1[p];
+        for (int a = 0; a < cnt; a++) {
+            int j = atoms ? atoms[a] : a;
+            double d[3];
+            d[0] = unwrap64(x[o1+j], ix[o1+j], b1[0]) - unwrap64(x[o0+j], ix[o0+j], b0[0]);
+            d[1] = unwrap64(y[o1+j], iy[o1+j], b1[1]) - unwrap64(y[o0+j], iy[o0+j], b0[1]);
+            d[2] = unwrap64(z[o1+j], iz[o1+j], b1[2]) - unwrap64(z[o0+j], iz[o0+j], b0[2]);
+            for (int c = 0; c < 3; c++) {
+                sum_d[3*tg[p] + c] += d[c];
+                sum_d2[3*tg[p] + c] += d[c] * d[c];
+            }
+        }
+        count[tg[p]] += (double)cnt;
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Reference-arithmetic FP32 restatement                               */
 /* ------------------------------------------------------------------ */
@@ -455,6 +486,40 @@ void orc_ref32_sk(int N, const float *x, const float *y, const float *z,
 /* Unwrapped FP32 coordinate as the custom reader forms it
  * (src/system.cpp:1290-1297): float(x + ix*L) computed in double. */
 static inline float unwrap32(float x, int16_t img, float L) { return (float)((double)x + (double)img * (double)L); }
+
+/* Self van Hove counts as the reference bins them: Van_Hove_Self::
+ * listkernel (src/van_hove_self.cpp:98-102) takes the FP32 length of the
+ * difference of the FP32 unwrapped coordinates (src/coordinate.h:55:
+ * sqrt(x*x + y*y + z*z)), and Space_Time_Correlation_Function::bin
+ * (src/space_time_correlation_function.cpp:216-230) counts it in bin
+ * int(distance / bin_size) when 0 <= bin < n_bins, bin_size = max/float(n)
+ * (src/van_hove_self.cpp:56).  hist[g*n_bins + b] += 1 (64-bit counts);
+ * weighting[g] counts every atom pair. */
+void orc_vhs_counts(int N, const float *x, const float *y, const float *z,
+                    const int16_t *ix, const int16_t *iy, const int16_t *iz, const float *box,
+                    const int *atoms, int n_list, int n_pairs, const int *tg, const int *t0,
+                    const int *t1, int ntg, int n_bins, float max_range,
+                    unsigned long long *hist, double *weighting)
+{
+    int cnt = atoms ? n_list : N;
+    float bin_size = max_range / (float)n_bins;
+    for (size_t i = 0; i < (size_t)ntg * n_bins; i++) hist[i] = 0;
+    for (int g = 0; g < ntg; g++) weighting[g] = 0.0;
+    for (int p = 0; p < n_pairs; p++) {
+        size_t o0 = (size_t)t0[p] * N, o1 = (size_t)t1[p] * N;
+        const float *b0 = box + 3*t0[p], *b1 = box + 3*t1[p];
+        for (int a = 0; a < cnt; a++) {
+            int j = atoms ? atoms[a] : a;
+            float dx = unwrap32(x[o1+j], ix[o1+j], b1[0]) - unwrap32(x[o0+j], ix[o0+j], b0[0]);
+            float dy = unwrap32(y[o1+j], iy[o1+j], b1[1]) - unwrap32(y[o0+j], iy[o0+j], b0[1]);
+            float dz = unwrap32(z[o1+j], iz[o1+j], b1[2]) - unwrap32(z[o0+j], iz[o0+j], b0[2]);
+            float distance = sqrtf(dx*dx + dy*dy + dz*dz);
+            int bin = (int)(distance / bin_size);
+            if (bin >= 0 && bin < n_bins) hist[(size_t)tg[p] * n_bins + bin]++;
+            weighting[tg[p]] += 1.0;
+        }
+    }
+}
 
 /* Incoherent_Scattering_Function::listkernel (src/incoherent_scattering_
  * function.cpp:272-299) for one pair: per atom and shell,

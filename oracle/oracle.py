@@ -39,6 +39,9 @@ SIGS = {
     "orc_ref32_fs_pair": (_I, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _I, _I, _PI, _PI, _F, _PF]),
     "orc_ref32_msd_pair": (_I, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _I, _PF, _PF]),
     "orc_sk_loss_grad_f64": (_D, [_I, _PD, _PD, _PD, _I, _PI, _PI, _D, _PD, _PD, _PD, _PD, _PD, _PD]),
+    "orc_disp_f64": (None, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _PI, _PI, _PI, _I, _PD, _PD, _PD]),
+    "orc_vhs_counts": (None, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _PI, _PI, _PI, _I, _I, _F,
+                              C.POINTER(C.c_ulonglong), _PD]),
 }
 
 _lib = None
@@ -208,6 +211,31 @@ def msd_ngp_f64(tr, pairs, ntg, atoms=None):
                           _p(t0, C.c_int), _p(t1, C.c_int), ntg, _p(s2, C.c_double), _p(s4, C.c_double),
                           _p(cnt, C.c_double), _p(msd, C.c_double), _p(ngp, C.c_double))
     return dict(s2=s2, s4=s4, count=cnt, msd=msd, ngp=ngp)
+
+
+def disp_f64(tr, pairs, ntg, atoms=None):
+    """Per gap: sums of d [ntg, 3], of d^2 [ntg, 3] and atom-pair counts."""
+    keep, ptrs = _traj_ptrs(tr)
+    tg, t0, t1 = (_c(v, np.int32) for v in pairs)
+    at = None if atoms is None else _c(atoms, np.int32)
+    sd, sd2, cnt = np.zeros(3 * ntg), np.zeros(3 * ntg), np.zeros(ntg)
+    lib().orc_disp_f64(tr.N, *ptrs, _p(at, C.c_int), 0 if at is None else len(at), len(tg), _p(tg, C.c_int),
+                       _p(t0, C.c_int), _p(t1, C.c_int), ntg, _p(sd, C.c_double), _p(sd2, C.c_double),
+                       _p(cnt, C.c_double))
+    return sd.reshape(ntg, 3), sd2.reshape(ntg, 3), cnt
+
+
+def vhs_counts(tr, pairs, ntg, n_bins, max_range, atoms=None):
+    """Reference-form self van Hove counts [ntg, n_bins] (uint64) and weighting [ntg]."""
+    keep, ptrs = _traj_ptrs(tr)
+    tg, t0, t1 = (_c(v, np.int32) for v in pairs)
+    at = None if atoms is None else _c(atoms, np.int32)
+    hist = np.zeros(ntg * n_bins, np.uint64)
+    w = np.zeros(ntg)
+    lib().orc_vhs_counts(tr.N, *ptrs, _p(at, C.c_int), 0 if at is None else len(at), len(tg), _p(tg, C.c_int),
+                         _p(t0, C.c_int), _p(t1, C.c_int), ntg, n_bins, C.c_float(max_range),
+                         hist.ctypes.data_as(C.POINTER(C.c_ulonglong)), _p(w, C.c_double))
+    return hist.reshape(ntg, n_bins), w
 
 
 # ---- reference FP32 arithmetic ----
