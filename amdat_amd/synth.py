@@ -164,3 +164,31 @@ def fcc_lattice(cells, L, jitter=0.0, n_keep=None, seed=MASTER_SEED + 3):
     (x, y, z), (ix, iy, iz) = _store(sites[None], L)
     box = np.full((1, 3), L, np.float32)
     return Traj(x, y, z, ix, iy, iz, box)
+
+
+def device_random_walk(N, L, T, sigma, seed):
+    """random_walk generated on the GPU frame by frame with torch: FP64
+    positions, stored as frame-major FP32 wrapped coordinates and int16
+    images [T, N] (CUDA tensors, x y z ix iy iz) -- for systems too large to
+    build on the host.  The stream differs from random_walk's."""
+    import torch
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(seed)
+    dev = torch.device("cuda:0")
+    cols = [torch.empty((T, N), dtype=torch.float32, device=dev) for _ in range(3)] + \
+           [torch.empty((T, N), dtype=torch.int16, device=dev) for _ in range(3)]
+    u = torch.empty((N, 3), dtype=torch.float64, device=dev).uniform_(0.0, L, generator=gen)
+    step = torch.empty_like(u)
+    for f in range(T):
+        if f:
+            u += step.normal_(0.0, sigma, generator=gen)
+        img = torch.floor(u / L)
+        w = (u - L * img).to(torch.float32)
+        hit = w >= np.float32(L)
+        w[hit] = 0.0
+        img[hit] += 1.0
+        for c in range(3):
+            cols[c][f] = w[:, c]
+            cols[3 + c][f] = img[:, c].to(torch.int16)
+    torch.cuda.synchronize()
+    return cols
