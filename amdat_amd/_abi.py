@@ -57,6 +57,10 @@ SIGNATURES = {
     "amdat_traj_shape": (_I, [_P, _PI, _PI, _PF]),
     "amdat_list_create": (_I, [_P, _P, _I, _PI, _PP]),
     "amdat_list_free": (_I, [_P, _P]),
+    "amdat_list_create_timed": (_I, [_P, _P, _I, _PI, _PI, _PI, _PP]),
+    "amdat_list_shape": (_I, [_P, _PI, _PI]),
+    "amdat_list_download": (_I, [_P, _P, _PI, _PI, _PI]),
+    "amdat_list_select_displacement": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _F, _F, _PP]),
     "amdat_pairs_upload": (_I, [_P, _I, _PI, _PI, _PI, _I, _PP]),
     "amdat_pairs_free": (_I, [_P, _P]),
     "amdat_rho_compute": (_I, [_P, _P, _P, _P, _I, _I, _F, _PI, _I, _I, _PP]),
@@ -353,6 +357,49 @@ class AtomList(_Handle):
             check(lib().amdat_list_create(ctx.h, traj.h, len(idx), ip(idx), C.byref(h)))
             self.n = len(idx)
         super().__init__(ctx, h)
+
+
+SELECT_FAST, SELECT_SLOW, SELECT_BETWEEN = 0, 1, 2
+
+
+class TimedList(_Handle):
+    """Time-varying atom list: list time c holds indices[off[c]:off[c+1]];
+    conv[t] is the list time of frame t (outside [0, n_times): no atoms)."""
+    _free = "amdat_list_free"
+
+    def __init__(self, ctx, traj, conv, off, indices, _h=None):
+        if _h is None:
+            conv = np.ascontiguousarray(conv, np.int32)
+            off = np.ascontiguousarray(off, np.int32)
+            idx = np.ascontiguousarray(indices, np.int32)
+            assert len(conv) == traj.T
+            h = C.c_void_p()
+            check(lib().amdat_list_create_timed(ctx.h, traj.h, len(off) - 1, ip(conv), ip(off), ip(idx),
+                                                C.byref(h)))
+        else:
+            h = _h
+        super().__init__(ctx, h)
+        self.T = traj.T
+        nt, n = C.c_int(), C.c_int()
+        check(lib().amdat_list_shape(self.h, C.byref(nt), C.byref(n)))
+        self.n_times, self.n = nt.value, n.value
+
+    @classmethod
+    def select_displacement(cls, ctx, traj, in_list, nexp, nes, frt, limit, time_index, mode, lo, hi):
+        """Fast/Slow/Average_Particles selection per exponential block."""
+        h = C.c_void_p()
+        check(lib().amdat_list_select_displacement(ctx.h, traj.h, in_list.h if in_list is not None else None, nexp,
+                                                   nes, frt, limit, time_index, mode, float(lo), float(hi),
+                                                   C.byref(h)))
+        return cls(ctx, traj, None, None, None, _h=h)
+
+    def download(self):
+        """(conv [T], off [n_times+1], indices [n])."""
+        conv = np.zeros(self.T, np.int32)
+        off = np.zeros(self.n_times + 1, np.int32)
+        idx = np.zeros(max(self.n, 1), np.int32)
+        check(lib().amdat_list_download(self.ctx.h, self.h, ip(conv), ip(off), ip(idx)))
+        return conv, off, idx[:self.n]
 
 
 class Pairs(_Handle):

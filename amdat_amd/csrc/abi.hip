@@ -94,9 +94,16 @@ struct amdat_traj {
 };
 
 struct amdat_list {
-    int n = 0;
+    int n = 0;        // entries over all list times
     int n_atoms = 0;
-    DevBuf<int> idx;  // empty = all atoms in order
+    DevBuf<int> idx;  // empty = all atoms in order (static lists only)
+    // Time-varying lists (src/trajectory_list.h:22-30): n_times > 0 lists,
+    // entries off[c] .. off[c+1]-1 of idx at list time c, and the list time
+    // conv[t] of each frame (src/exptime_trajectory_list.cpp:52); a frame
+    // whose list time lies outside [0, n_times) has no atoms.  n_times == 0
+    // is a static list.
+    int n_times = 0;
+    std::vector<int> conv, off;
 };
 
 struct KPlanOwned {
@@ -136,6 +143,7 @@ struct amdat_rho {
     std::vector<int> frames;     // frame index per row
     std::vector<int> frame_row;  // row per frame, -1 absent
     std::vector<int> shell_off;  // relative to the first computed vector
+    std::vector<int> row_n;      // atoms summed into each row (the list at that frame)
     DevBuf<double2> d;
     DevBuf<int> d_shell_off;
 };
@@ -287,6 +295,122 @@ int choose_splits(long blocks_without_split, int n_list)
 
 const int *list_ptr(const amdat_list *l) { return l && l->idx.p ? l->idx.p : nullptr; }
 int list_n(const amdat_traj *tr, const amdat_list *l) { return l ? l->n : tr->dev.N; }
+bool list_timed(const amdat_list *l) { return l && l->n_times > 0; }
+
+// List time of frame t (0 for a static list, -1 where a time-varying list
+// holds no atoms).
+int list_time(const amdat_list *l, int t)
+{
+    if (!list_timed(l)) return 0;
+    const int c = (t >= 0 && t < (int)l->conv.size()) ? l->conv[t] : -1;
+    return (c >= 0 && c < l->n_times) ? c : -1;
+}
+
+// The atoms of a list at one list time: device pointer (nullptr = all atoms
+// in order) and count.
+struct ListSeg {
+    const int *p;
+    int n;
+};
+ListSeg list_seg(const amdat_traj *tr, const amdat_list *l, int c)
+{
+    if (!list_timed(l)) return {list_ptr(l), list_n(tr, l)};
+    if (c < 0) return {nullptr, 0};
+    const int n = l->off[c + 1] - l->off[c];
+    return {n > 0 ? l->idx.p + l->off[c] : nullptr, n};
+}
+ListSeg list_at(const amdat_traj *tr, const amdat_list *l, int t) { return list_seg(tr, l, list_time(l, t)); }
+
+// Pairs regrouped by the list time of their t0, for per-list-time launches:
+// pos[i] is pair i's row in the regrouped order (stable within a list
+// time), seg[g] = {list time, first row, row count} per group, and
+// t0/t1/tg hold the regrouped pairs.
+struct PairGroups {
+    std::vector<int> pos, t0, t1, tg;
+    std::vector<int3> seg;
+};
+PairGroups group_pairs(const amdat_list *l, const amdat_pairs *pairs)
+{
+    PairGroups g;
+    const int n = pairs->n;
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::vector<int> key(n);
+    for (int i = 0; i < n; i++) key[i] = list_time(l, pairs->t0[i]);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+    g.pos.resize(n);
+    g.t0.resize(n);
+    g.t1.resize(n);
+    g.tg.resize(n);
+    for (int r = 0; r < n; r++) {
+        const int i = ord[r];
+        g.pos[i] = r;
+        g.t0[r] = pairs->t0[i];
+        g.t1[r] = pairs->t1[i];
+        g.tg[r] = pairs->tg[i];
+        if (r == 0 || key[i] != key[ord[r - 1]]) g.seg.push_back(make_int3(key[i], r, 0));
+        g.seg.back().z++;
+    }
+    return g;
+}
+
+// Launch plan of a pair-based entry point (Fs, MSD/NGP, displacements):
+// rows [row, row + len) of the per-pair partial buffers sum over the atoms
+// of seg; t0/t1/tg are the device pairs in row order and order maps the
+// reduction's reference-order positions to rows.  A static list is one
+// group over the pairs in their own order; a time-varying list is one group
+// per list time (src/trajectory_list.cpp:202-211 takes the list at the
+// list time of t0).  count[g] receives the atom-pair count of each time gap
+// (src/mean_square_displacement.cpp:157; src/incoherent_scattering_
+// function.cpp:283).
+struct PairLaunch {
+    struct Group {
+        ListSeg seg;
+        int row, len;
+    };
+    std::vector<Group> groups;
+    const int *t0 = nullptr, *t1 = nullptr, *tg = nullptr, *order = nullptr;
+    int max_n = 0;
+    bool any_empty = false;
+    DevBuf<int> d_t0, d_t1, d_tg, d_order;
+};
+int plan_pair_launch(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *l, const amdat_pairs *pairs,
+                     bool need_tg, PairLaunch &pl, double *count)
+{
+    hipStream_t st = stream_of(ctx);
+    for (int g = 0; g < pairs->ntg; g++) count[g] = 0.0;
+    for (int i = 0; i < pairs->n; i++) count[pairs->tg[i]] += (double)list_at(tr, l, pairs->t0[i]).n;
+    if (!list_timed(l)) {
+        const ListSeg sg = list_seg(tr, l, 0);
+        pl.groups.push_back({sg, 0, pairs->n});
+        pl.t0 = pairs->d_t0.p;
+        pl.t1 = pairs->d_t1.p;
+        pl.order = pairs->d_order.p;
+        if (need_tg) HIP_TRY(pl.d_tg.upload(pairs->tg.data(), pairs->tg.size(), st));
+        pl.tg = pl.d_tg.p;
+        pl.max_n = sg.n;
+        pl.any_empty = sg.n == 0;
+        return AMDAT_OK;
+    }
+    PairGroups g = group_pairs(l, pairs);
+    for (const int3 &sg3 : g.seg) {
+        const ListSeg sg = list_seg(tr, l, sg3.x);
+        pl.groups.push_back({sg, sg3.y, sg3.z});
+        pl.max_n = std::max(pl.max_n, sg.n);
+        if (sg.n == 0) pl.any_empty = true;
+    }
+    std::vector<int> ord(pairs->n);
+    for (int q = 0; q < pairs->n; q++) ord[q] = g.pos[pairs->order[q]];
+    HIP_TRY(pl.d_t0.upload(g.t0.data(), g.t0.size(), st));
+    HIP_TRY(pl.d_t1.upload(g.t1.data(), g.t1.size(), st));
+    HIP_TRY(pl.d_order.upload(ord.data(), ord.size(), st));
+    if (need_tg) HIP_TRY(pl.d_tg.upload(g.tg.data(), g.tg.size(), st));
+    pl.t0 = pl.d_t0.p;
+    pl.t1 = pl.d_t1.p;
+    pl.tg = pl.d_tg.p;
+    pl.order = pl.d_order.p;
+    return AMDAT_OK;
+}
 
 }  // namespace
 
@@ -547,6 +671,173 @@ int amdat_list_free(amdat_ctx *, amdat_list *l)
     return AMDAT_OK;
 }
 
+int amdat_list_create_timed(amdat_ctx *ctx, const amdat_traj *tr, int n_times, const int *time_conv,
+                            const int *off, const int *indices, amdat_list **out)
+{
+    clear_error();
+    if (!ctx || !tr || !out || n_times < 1 || !time_conv || !off)
+        return set_error(AMDAT_EINVAL, "amdat_list_create_timed: bad arguments");
+    if (off[0] != 0) return set_error(AMDAT_EINVAL, "amdat_list_create_timed: off[0] must be 0");
+    for (int c = 0; c < n_times; c++)
+        if (off[c + 1] < off[c]) return set_error(AMDAT_EINVAL, "amdat_list_create_timed: offsets decrease");
+    const int n = off[n_times];
+    if (n > 0 && !indices) return set_error(AMDAT_EINVAL, "amdat_list_create_timed: null indices");
+    for (int a = 0; a < n; a++)
+        if (indices[a] < 0 || indices[a] >= tr->dev.N)
+            return set_error(AMDAT_ERANGE, "amdat_list_create_timed: atom index out of range");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto l = std::make_unique<amdat_list>();
+    l->n = n;
+    l->n_atoms = tr->dev.N;
+    l->n_times = n_times;
+    l->conv.assign(time_conv, time_conv + tr->dev.T);
+    l->off.assign(off, off + n_times + 1);
+    if (n > 0) HIP_TRY(l->idx.upload(indices, (size_t)n, stream_of(ctx)));
+    *out = l.release();
+    return AMDAT_OK;
+}
+
+int amdat_list_shape(const amdat_list *l, int *n_times, int *n_entries)
+{
+    if (!l) return set_error(AMDAT_EINVAL, "amdat_list_shape: null list");
+    if (n_times) *n_times = l->n_times;
+    if (n_entries) *n_entries = l->n;
+    return AMDAT_OK;
+}
+
+int amdat_list_download(amdat_ctx *ctx, const amdat_list *l, int *time_conv, int *off, int *indices)
+{
+    clear_error();
+    if (!ctx || !l) return set_error(AMDAT_EINVAL, "amdat_list_download: null argument");
+    if (list_timed(l)) {
+        if (time_conv) std::copy(l->conv.begin(), l->conv.end(), time_conv);
+        if (off) std::copy(l->off.begin(), l->off.end(), off);
+    }
+    if (indices && l->n > 0) {
+        if (l->idx.p) {
+            HIP_TRY(hipSetDevice(ctx->device));
+            hipStream_t st = stream_of(ctx);
+            HIP_TRY(hipMemcpyAsync(indices, l->idx.p, sizeof(int) * (size_t)l->n, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+        } else {
+            std::iota(indices, indices + l->n, 0);
+        }
+    }
+    return AMDAT_OK;
+}
+
+int amdat_list_select_displacement(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *in, int n_exponentials,
+                                   int n_exponential_steps, int frt, int limit, int time_index, int mode, float lo,
+                                   float hi, amdat_list **out)
+{
+    clear_error();
+    if (!ctx || !tr || !out) return set_error(AMDAT_EINVAL, "amdat_list_select_displacement: null argument");
+    const int nexp = n_exponentials, nes = n_exponential_steps, f = frt ? 1 : 0;
+    if (nexp < 1 || nes < 1 || limit < 0)
+        return set_error(AMDAT_EINVAL, "amdat_list_select_displacement: bad time scheme");
+    if (mode < AMDAT_SELECT_FAST || mode > AMDAT_SELECT_BETWEEN)
+        return set_error(AMDAT_EINVAL, "amdat_list_select_displacement: unknown mode");
+    if (in && in->n_atoms != tr->dev.N)
+        return set_error(AMDAT_EINVAL, "amdat_list_select_displacement: list built for another trajectory size");
+    const long T_scheme = (long)nes * nexp + 1;
+    if (T_scheme > tr->dev.T)
+        return set_error(AMDAT_ERANGE, "amdat_list_select_displacement: trajectory shorter than the time scheme");
+    // Pairs of System::displacement_list(analysis, timegap, false)
+    // (src/system.cpp:4349-4398), as Fast_Particles::analyze calls it
+    // (src/fast_particles.cpp:117-122).
+    const int ntg = nes + nexp - 1 + (1 - f);
+    const int g = time_index;
+    std::vector<int> p0, p1;
+    if (g >= 0 && g < nes) {
+        int count = 0;
+        for (int b = 0; b < nexp; b++) {
+            p0.push_back(nes * b + f);
+            p1.push_back(nes * b + f + g);
+            if (++count == limit) break;
+        }
+    } else if (g >= nes && g < ntg - 1 + f) {
+        const int btg = g - nes + 1;
+        int count = 0;
+        for (int b = 0; b < nexp - btg; b++) {
+            p0.push_back(nes * b + f);
+            p1.push_back(nes * b + f + nes * btg);
+            if (++count == limit) break;
+        }
+    } else if (g == ntg - 1 + f) {
+        p0.push_back(0);
+        p1.push_back((int)T_scheme - 1);
+    } else {
+        return set_error(AMDAT_ERANGE, "Requested timegap out of range.");
+    }
+    const int nq = (int)p0.size();
+    // Block of each pair (src/fast_particles.cpp:133) and its input atoms
+    // (the input list at the list time of t0).
+    std::vector<int> blk(nq);
+    std::vector<int4> seg(nq);
+    int max_n = 0;
+    for (int q = 0; q < nq; q++) {
+        blk[q] = (int)(((float)p0[q] - (float)f) / (float)nes);
+        if (blk[q] < 0 || blk[q] >= nexp)
+            return set_error(AMDAT_ERANGE, "amdat_list_select_displacement: pair outside the exponential blocks");
+        const int c = list_time(in, p0[q]);
+        int first = -1, n = 0;
+        if (!list_timed(in)) {
+            n = list_n(tr, in);
+            first = in && in->idx.p ? 0 : -1;
+        } else if (c >= 0) {
+            first = in->off[c];
+            n = in->off[c + 1] - in->off[c];
+        }
+        seg[q] = make_int4(p0[q], p1[q], first, n);
+        max_n = std::max(max_n, n);
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = stream_of(ctx);
+    const int n_tiles = (max_n + kBlock - 1) / kBlock;
+    std::vector<int> cnt((size_t)nq * n_tiles, 0), base((size_t)nq * n_tiles, 0);
+    const int *in_idx = in ? in->idx.p : nullptr;
+    DevBuf<int4> d_seg;
+    DevBuf<int> d_cnt, d_base;
+    if (n_tiles > 0) {
+        HIP_TRY(d_seg.upload(seg.data(), seg.size(), st));
+        HIP_TRY(d_cnt.alloc(cnt.size()));
+        HIP_TRY(launch_select(tr->dev, d_seg.p, nq, in_idx, mode, lo, hi, n_tiles, d_cnt.p, nullptr, nullptr, st));
+        HIP_TRY(hipMemcpyAsync(cnt.data(), d_cnt.p, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    // Output order: blocks in order, pairs of one block in pair order,
+    // atoms in input order (addtrajectory appends, src/trajectory_list.cpp:
+    // 509-514).
+    std::vector<int> off(nexp + 1, 0);
+    long total = 0;
+    for (int b = 0; b < nexp; b++) {
+        for (int q = 0; q < nq; q++) {
+            if (blk[q] != b) continue;
+            for (int t = 0; t < n_tiles; t++) {
+                base[(size_t)q * n_tiles + t] = (int)total;
+                total += cnt[(size_t)q * n_tiles + t];
+            }
+        }
+        off[b + 1] = (int)total;
+    }
+    auto l = std::make_unique<amdat_list>();
+    l->n = (int)total;
+    l->n_atoms = tr->dev.N;
+    l->n_times = nexp;
+    l->off = off;
+    l->conv.resize(tr->dev.T);
+    // Exptime_Trajectory_List time conversion (src/exptime_trajectory_list.cpp:87-91)
+    for (int t = 0; t < tr->dev.T; t++) l->conv[t] = (int)((float)(t - f) / (float)nes);
+    if (total > 0) {
+        HIP_TRY(l->idx.alloc((size_t)total));
+        HIP_TRY(d_base.upload(base.data(), base.size(), st));
+        HIP_TRY(launch_select(tr->dev, d_seg.p, nq, in_idx, mode, lo, hi, n_tiles, nullptr, d_base.p, l->idx.p, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    *out = l.release();
+    return AMDAT_OK;
+}
+
 int amdat_pairs_upload(amdat_ctx *ctx, int n_pairs, const int *tg, const int *t0, const int *t1, int n_timegaps,
                        amdat_pairs **out)
 {
@@ -629,30 +920,54 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
     r->frames = fr;
     r->frame_row.assign(T, -1);
     for (int i = 0; i < r->n_frames; i++) r->frame_row[fr[i]] = i;
+    r->row_n.resize(r->n_frames);
+    for (int i = 0; i < r->n_frames; i++) r->row_n[i] = list_at(tr, list, fr[i]).n;
     r->shell_off.resize(r->n_shells + 1);
     for (int s = 0; s <= r->n_shells; s++) r->shell_off[s] = sh->off[first_shell + s] - sh->off[first_shell];
     hipStream_t st = stream_of(ctx);
     HIP_TRY(r->d_shell_off.upload(r->shell_off.data(), r->shell_off.size(), st));
     HIP_TRY(r->d.alloc((size_t)r->n_frames * kp.K));
-    if (nl > 0 && kp.K > 0) {
+    if (kp.K > 0) {
         DevBuf<int> dfr;
         HIP_TRY(dfr.upload(fr.data(), fr.size(), st));
-        const int n_splits = choose_splits((long)kp.n_chunks * r->n_frames, nl);
         const double inv_ell = 1.0 / (double)length_scale;
-        // Launch in frame batches so a launch stays within grid limits.
+        // Consecutive rows with one list time share a launch (a static list
+        // is one run); launches hold at most fb frames (grid limit).
         const int fb = 65535;
-        DevBuf<double2> part;
-        if (n_splits > 1) HIP_TRY(part.alloc((size_t)std::min(fb, r->n_frames) * n_splits * kp.K));
-        for (int f0 = 0; f0 < r->n_frames; f0 += fb) {
-            const int nf = std::min(fb, r->n_frames - f0);
-            double2 *dst = r->d.p + (size_t)f0 * kp.K;
-            HIP_TRY(launch_rho(precision, tr->dev, list_ptr(list), nl, dfr.p + f0, nf, kp, inv_ell, n_splits,
-                               n_splits > 1 ? part.p : dst, st));
-            if (n_splits > 1) HIP_TRY(launch_split_reduce2(part.p, nf, n_splits, kp.K, dst, st));
+        struct Run {
+            int row, len;
+            ListSeg seg;
+            int splits;
+        };
+        std::vector<Run> runs;
+        for (int i = 0; i < r->n_frames;) {
+            const int c = list_time(list, fr[i]);
+            int j = i + 1;
+            while (j < r->n_frames && list_time(list, fr[j]) == c) j++;
+            const ListSeg sg = list_seg(tr, list, c);
+            runs.push_back({i, j - i, sg, sg.n > 0 ? choose_splits((long)kp.n_chunks * (j - i), sg.n) : 1});
+            i = j;
         }
-        HIP_TRY(hipStreamSynchronize(st));
-    } else if (kp.K > 0) {
-        HIP_TRY(hipMemsetAsync(r->d.p, 0, r->d.n * sizeof(double2), st));
+        size_t part_n = 0;
+        for (const Run &u : runs)
+            if (u.seg.n > 0 && u.splits > 1)
+                part_n = std::max(part_n, (size_t)std::min(fb, u.len) * u.splits * kp.K);
+        DevBuf<double2> part;
+        if (part_n) HIP_TRY(part.alloc(part_n));
+        for (const Run &u : runs) {
+            double2 *rows = r->d.p + (size_t)u.row * kp.K;
+            if (u.seg.n == 0) {
+                HIP_TRY(hipMemsetAsync(rows, 0, sizeof(double2) * (size_t)u.len * kp.K, st));
+                continue;
+            }
+            for (int f0 = 0; f0 < u.len; f0 += fb) {
+                const int nf = std::min(fb, u.len - f0);
+                double2 *dst = rows + (size_t)f0 * kp.K;
+                HIP_TRY(launch_rho(precision, tr->dev, u.seg.p, u.seg.n, dfr.p + u.row + f0, nf, kp, inv_ell,
+                                   u.splits, u.splits > 1 ? part.p : dst, st));
+                if (u.splits > 1) HIP_TRY(launch_split_reduce2(part.p, nf, u.splits, kp.K, dst, st));
+            }
+        }
         HIP_TRY(hipStreamSynchronize(st));
     }
     *out = r.release();
@@ -693,7 +1008,13 @@ int amdat_sk(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, double *S
     const int nf = r1->n_frames;
     std::vector<int> rows(nf), goff = {0, nf};
     std::iota(rows.begin(), rows.end(), 0);
-    const double norm = (double)nf * (double)r1->n_list;
+    double norm = 0.0;
+    for (int i = 0; i < nf; i++) norm += (double)r1->row_n[i];
+    if (norm == 0.0 || r1->K == 0) {
+        // no atoms over the frames: S = 0 (src/structure_factor.cpp:275-285)
+        std::fill(S, S + r1->n_shells, 0.0);
+        return AMDAT_OK;
+    }
     DevBuf<int> d_rows, d_goff;
     DevBuf<double> d_norm, d_out;
     HIP_TRY(d_rows.upload(rows.data(), rows.size(), st));
@@ -727,7 +1048,7 @@ int amdat_fkt(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const am
         row0[q] = r1->frame_row[pairs->t0[i]];
         row1[q] = r2->frame_row[pairs->t1[i]];
         if (row0[q] < 0 || row1[q] < 0) return set_error(AMDAT_ERANGE, "amdat_fkt: a pair uses a frame the density lacks");
-        norm[pairs->tg[i]] += (double)r1->n_list;
+        norm[pairs->tg[i]] += (double)r1->row_n[row0[q]];
     }
     for (int g = 0; g < ntg; g++) count[g] = norm[g];
     if (pairs->n == 0) {
@@ -765,17 +1086,18 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
     rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, 6.0);
     if (rc) return rc;
     const KPlan &kp = plan->kp;
-    const int nl = list_n(tr, list);
     const int ntg = pairs->ntg, nsh = last_shell - first_shell + 1;
     std::vector<double> norm(ntg, 0.0);
-    for (int i = 0; i < pairs->n; i++) norm[pairs->tg[i]] += (double)nl;
+    PairLaunch pl;
+    rc = plan_pair_launch(ctx, tr, list, pairs, false, pl, norm.data());
+    if (rc) return rc;
     for (int g = 0; g < ntg; g++) count[g] = norm[g];
-    if (pairs->n == 0 || nl == 0 || kp.K == 0) {
+    if (pairs->n == 0 || pl.max_n == 0 || kp.K == 0) {
         std::fill(Fs, Fs + (size_t)ntg * nsh, 0.0);
         return AMDAT_OK;
     }
     hipStream_t st = stream_of(ctx);
-    const int n_splits = choose_splits((long)kp.n_chunks * pairs->n, nl);
+    const int n_splits = choose_splits((long)kp.n_chunks * pairs->n, pl.max_n);
     DevBuf<double> V, d_norm, d_out;
     DevBuf<int> d_soff;
     std::vector<int> soff(nsh + 1);
@@ -783,16 +1105,20 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
     HIP_TRY(d_soff.upload(soff.data(), soff.size(), st));
     HIP_TRY(d_norm.upload(norm.data(), norm.size(), st));
     HIP_TRY(V.alloc((size_t)pairs->n * n_splits * kp.K));
+    if (pl.any_empty) HIP_TRY(hipMemsetAsync(V.p, 0, V.n * sizeof(double), st));
     const double inv_ell = 1.0 / (double)length_scale;
     const int pb = 65535;
-    for (int p0 = 0; p0 < pairs->n; p0 += pb) {
-        const int np = std::min(pb, pairs->n - p0);
-        HIP_TRY(launch_fs(precision, tr->dev, list_ptr(list), nl, pairs->d_t0.p + p0, pairs->d_t1.p + p0, np, kp,
-                          inv_ell, n_splits, V.p + (size_t)p0 * n_splits * kp.K, st));
+    for (const PairLaunch::Group &gr : pl.groups) {
+        if (gr.seg.n == 0) continue;
+        for (int p0 = gr.row; p0 < gr.row + gr.len; p0 += pb) {
+            const int np = std::min(pb, gr.row + gr.len - p0);
+            HIP_TRY(launch_fs(precision, tr->dev, gr.seg.p, gr.seg.n, pl.t0 + p0, pl.t1 + p0, np, kp, inv_ell,
+                              n_splits, V.p + (size_t)p0 * n_splits * kp.K, st));
+        }
     }
     HIP_TRY(d_out.alloc((size_t)ntg * nsh));
-    HIP_TRY(launch_fs_shell(V.p, n_splits, kp.K, pairs->d_group_off.p, pairs->d_order.p, ntg, d_soff.p, nsh,
-                            d_norm.p, d_out.p, st));
+    HIP_TRY(launch_fs_shell(V.p, n_splits, kp.K, pairs->d_group_off.p, pl.order, ntg, d_soff.p, nsh, d_norm.p,
+                            d_out.p, st));
     HIP_TRY(hipMemcpyAsync(Fs, d_out.p, sizeof(double) * ntg * nsh, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
@@ -806,28 +1132,32 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
     int rc = check_pairs_frames(pairs, tr->dev.T);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
-    const int nl = list_n(tr, list);
     const int ntg = pairs->ntg;
-    for (int g = 0; g < ntg; g++) count[g] = 0.0;
-    for (int i = 0; i < pairs->n; i++) count[pairs->tg[i]] += (double)nl;
-    if (pairs->n == 0 || nl == 0) {
+    PairLaunch pl;
+    rc = plan_pair_launch(ctx, tr, list, pairs, false, pl, count);
+    if (rc) return rc;
+    if (pairs->n == 0 || pl.max_n == 0) {
         std::fill(s2, s2 + ntg, 0.0);
         std::fill(s4, s4 + ntg, 0.0);
         return AMDAT_OK;
     }
     hipStream_t st = stream_of(ctx);
-    const int nb = std::max(1, std::min(64, (nl + kBlock * 4 - 1) / (kBlock * 4)));
+    const int nb = std::max(1, std::min(64, (pl.max_n + kBlock * 4 - 1) / (kBlock * 4)));
     DevBuf<double> part, d2, d4;
     HIP_TRY(part.alloc((size_t)pairs->n * nb * 2));
+    if (pl.any_empty) HIP_TRY(hipMemsetAsync(part.p, 0, part.n * sizeof(double), st));
     const int pb = 65535;
-    for (int p0 = 0; p0 < pairs->n; p0 += pb) {
-        const int np = std::min(pb, pairs->n - p0);
-        HIP_TRY(launch_msd(tr->dev, list_ptr(list), nl, pairs->d_t0.p + p0, pairs->d_t1.p + p0, np, nb,
-                           part.p + (size_t)p0 * nb * 2, st));
+    for (const PairLaunch::Group &gr : pl.groups) {
+        if (gr.seg.n == 0) continue;
+        for (int p0 = gr.row; p0 < gr.row + gr.len; p0 += pb) {
+            const int np = std::min(pb, gr.row + gr.len - p0);
+            HIP_TRY(launch_msd(tr->dev, gr.seg.p, gr.seg.n, pl.t0 + p0, pl.t1 + p0, np, nb,
+                               part.p + (size_t)p0 * nb * 2, st));
+        }
     }
     HIP_TRY(d2.alloc(ntg));
     HIP_TRY(d4.alloc(ntg));
-    HIP_TRY(launch_msd_reduce(part.p, nb, pairs->d_group_off.p, pairs->d_order.p, ntg, d2.p, d4.p, st));
+    HIP_TRY(launch_msd_reduce(part.p, nb, pairs->d_group_off.p, pl.order, ntg, d2.p, d4.p, st));
     HIP_TRY(hipMemcpyAsync(s2, d2.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(s4, d4.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -846,34 +1176,36 @@ int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
     int rc = check_pairs_frames(pairs, tr->dev.T);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
-    const int nl = list_n(tr, list);
     const int ntg = pairs->ntg;
-    for (int g = 0; g < ntg; g++) count[g] = 0.0;
-    for (int i = 0; i < pairs->n; i++) count[pairs->tg[i]] += (double)nl;
     std::fill(sum_d, sum_d + 3 * (size_t)ntg, 0.0);
     std::fill(sum_d2, sum_d2 + 3 * (size_t)ntg, 0.0);
     if (n_bins > 0) std::fill(hist, hist + (size_t)ntg * n_bins, 0ull);
-    if (pairs->n == 0 || nl == 0) return AMDAT_OK;
+    PairLaunch pl;
+    rc = plan_pair_launch(ctx, tr, list, pairs, true, pl, count);
+    if (rc) return rc;
+    if (pairs->n == 0 || pl.max_n == 0) return AMDAT_OK;
     hipStream_t st = stream_of(ctx);
-    const int nb = std::max(1, std::min(64, (nl + kBlock * 4 - 1) / (kBlock * 4)));
+    const int nb = std::max(1, std::min(64, (pl.max_n + kBlock * 4 - 1) / (kBlock * 4)));
     const float bin_size = n_bins > 0 ? max_range / (float)n_bins : 1.0f;  // src/space_time_correlation_function.cpp
     DevBuf<double> part, out;
     DevBuf<unsigned long long> dh;
-    DevBuf<int> dtg;
     HIP_TRY(part.alloc((size_t)pairs->n * nb * 6));
-    HIP_TRY(dtg.upload(pairs->tg.data(), pairs->tg.size(), st));
+    if (pl.any_empty) HIP_TRY(hipMemsetAsync(part.p, 0, part.n * sizeof(double), st));
     if (n_bins > 0) {
         HIP_TRY(dh.alloc((size_t)ntg * n_bins));
         HIP_TRY(hipMemsetAsync(dh.p, 0, dh.n * sizeof(unsigned long long), st));
     }
     const int pb = 65535;
-    for (int p0 = 0; p0 < pairs->n; p0 += pb) {
-        const int np = std::min(pb, pairs->n - p0);
-        HIP_TRY(launch_disp(tr->dev, list_ptr(list), nl, pairs->d_t0.p + p0, pairs->d_t1.p + p0, dtg.p + p0, np, nb,
-                            n_bins, bin_size, part.p + (size_t)p0 * nb * 6, dh.p, st));
+    for (const PairLaunch::Group &gr : pl.groups) {
+        if (gr.seg.n == 0) continue;
+        for (int p0 = gr.row; p0 < gr.row + gr.len; p0 += pb) {
+            const int np = std::min(pb, gr.row + gr.len - p0);
+            HIP_TRY(launch_disp(tr->dev, gr.seg.p, gr.seg.n, pl.t0 + p0, pl.t1 + p0, pl.tg + p0, np, nb, n_bins,
+                                bin_size, part.p + (size_t)p0 * nb * 6, dh.p, st));
+        }
     }
     HIP_TRY(out.alloc((size_t)ntg * 6));
-    HIP_TRY(launch_disp_reduce(part.p, nb, pairs->d_group_off.p, pairs->d_order.p, ntg, out.p, st));
+    HIP_TRY(launch_disp_reduce(part.p, nb, pairs->d_group_off.p, pl.order, ntg, out.p, st));
     std::vector<double> h((size_t)ntg * 6);
     HIP_TRY(hipMemcpyAsync(h.data(), out.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
     if (n_bins > 0)
@@ -989,6 +1321,8 @@ int amdat_batch_create(amdat_ctx *ctx, int n_traj, const amdat_traj *const *traj
             return set_error(AMDAT_EINVAL, "amdat_batch_create: trajectories differ in frame count");
         if (l && l->n_atoms != t->dev.N)
             return set_error(AMDAT_EINVAL, "amdat_batch_create: list built for another trajectory size");
+        if (list_timed(l))
+            return set_error(AMDAT_EINVAL, "amdat_batch_create: the batched form takes static lists only");
         b->trajs.push_back(t);
         b->lists.push_back(l);
     }

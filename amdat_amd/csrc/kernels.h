@@ -103,6 +103,11 @@ hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int
                        int n_pairs, int n_blocks, int n_bins, float bin_size, double *partial,
                        unsigned long long *hist, hipStream_t st);
 // Per group: the six sums of its pairs' partials (pairs in order[]).
+// Displacement selection (Fast/Slow/Average_Particles): with tile_base ==
+// nullptr, per-block counts into tile_count[n_seg][n_tiles]; otherwise the
+// selected atoms, in input order, into out at tile_base[q][tile] + rank.
+hipError_t launch_select(const TrajDev &tr, const int4 *seg, int n_seg, const int *in_idx, int mode, float lo,
+                         float hi, int n_tiles, int *tile_count, const int *tile_base, int *out, hipStream_t st);
 hipError_t launch_disp_reduce(const double *partial, int n_blocks, const int *group_off, const int *order,
                               int n_groups, double *out, hipStream_t st);
 // Per group: sums of the partials of its pairs.

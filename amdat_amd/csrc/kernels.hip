@@ -291,6 +291,90 @@ __global__ __launch_bounds__(kBlock) void disp_kernel(
     }
 }
 
+// Membership test of Fast_Particles, Slow_Particles and Average_Particles
+// (src/fast_particles.cpp:127-139, src/slow_particles.cpp:100-113,
+// src/average_particles.cpp:103-116) for atom j between frames t0 and t1
+// (element offsets o0, o1; boxes B0, B1).  The squared displacement is
+// formed as the reference forms it: pow(distance, 2) stored as float, with
+// distance = sqrtf((dx dx + dy dy) + dz dz) of the FP32 difference of
+// unwrapped FP32 coordinates u = float(x + i L).  The double square of a
+// float is exact, so its rounding to float equals the FP32 product.
+// mode 0: d2 > lo; 1: d2 < hi; 2: lo <= d2 <= hi.
+__device__ __forceinline__ bool displacement_selected(const float *const (&src)[3], const int16_t *const (&img)[3],
+                                                      size_t o0, size_t o1, const float *B0, const float *B1, int j,
+                                                      int mode, float lo, float hi)
+{
+#pragma clang fp contract(off)
+    float u[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double x0 = src[c][o0 + j], x1 = src[c][o1 + j];
+        const double w0 = (double)img[c][o0 + j] * (double)B0[c], w1 = (double)img[c][o1 + j] * (double)B1[c];
+        u[c] = (float)(x1 + w1) - (float)(x0 + w0);
+    }
+    const float len = sqrtf((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+    const float d2 = len * len;
+    if (mode == 0) return d2 > lo;
+    if (mode == 1) return d2 < hi;
+    return d2 <= hi && d2 >= lo;
+}
+
+// Selection pass 1.  Pair q (blockIdx.y) has seg[q] = {t0, t1, first entry
+// in the input list (-1: all atoms in order), entries}; one thread per
+// entry.  Writes the number of selected entries of each block of kBlock
+// entries to tile_count[q * n_tiles + block].
+__global__ __launch_bounds__(kBlock) void select_count_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+    const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
+    const float *__restrict__ box, int N, const int4 *__restrict__ seg, const int *__restrict__ in_idx, int mode,
+    float lo, float hi, int n_tiles, int *__restrict__ tile_count)
+{
+    const int q = blockIdx.y;
+    const int4 sg = seg[q];
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    const float *src[3] = {X, Y, Z};
+    const int16_t *img[3] = {IX, IY, IZ};
+    bool sel = false;
+    if (a < sg.w) {
+        const int j = sg.z >= 0 ? in_idx[sg.z + a] : a;
+        sel = displacement_selected(src, img, (size_t)sg.x * N, (size_t)sg.y * N, box + 3 * sg.x, box + 3 * sg.y, j,
+                                    mode, lo, hi);
+    }
+    const int c = __syncthreads_count(sel);
+    if (threadIdx.x == 0) tile_count[(size_t)q * n_tiles + blockIdx.x] = c;
+}
+
+// Selection pass 2: the same test; selected atoms are written in input
+// order to out[tile_base[q * n_tiles + block] + rank in the block].
+__global__ __launch_bounds__(kBlock) void select_write_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+    const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
+    const float *__restrict__ box, int N, const int4 *__restrict__ seg, const int *__restrict__ in_idx, int mode,
+    float lo, float hi, int n_tiles, const int *__restrict__ tile_base, int *__restrict__ out)
+{
+    __shared__ int wsum[kBlock / 64];
+    const int q = blockIdx.y;
+    const int4 sg = seg[q];
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    const float *src[3] = {X, Y, Z};
+    const int16_t *img[3] = {IX, IY, IZ};
+    bool sel = false;
+    int j = 0;
+    if (a < sg.w) {
+        j = sg.z >= 0 ? in_idx[sg.z + a] : a;
+        sel = displacement_selected(src, img, (size_t)sg.x * N, (size_t)sg.y * N, box + 3 * sg.x, box + 3 * sg.y, j,
+                                    mode, lo, hi);
+    }
+    const unsigned long long m = __ballot(sel);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int base = tile_base[(size_t)q * n_tiles + blockIdx.x];
+    for (int i = 0; i < w; i++) base += wsum[i];
+    if (sel) out[base + below] = j;
+}
+
 // Per time gap: the six sums of its pairs' partials, pairs in reference order.
 __global__ __launch_bounds__(kBlock) void disp_reduce_kernel(const double *__restrict__ partial, int n_blocks,
                                                               const int *__restrict__ group_off,
@@ -639,6 +723,20 @@ hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int
     hipLaunchKernelGGL(disp_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), sizeof(unsigned int) * (size_t)n_bins, st,
                        tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, tg, n_bins, bin_size,
                        partial, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(const TrajDev &tr, const int4 *seg, int n_seg, const int *in_idx, int mode, float lo,
+                         float hi, int n_tiles, int *tile_count, const int *tile_base, int *out, hipStream_t st)
+{
+    if (n_seg <= 0 || n_tiles <= 0) return hipSuccess;
+    (void)hipGetLastError();
+    if (!tile_base)
+        hipLaunchKernelGGL(select_count_kernel, dim3(n_tiles, n_seg), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
+                           tr.iy, tr.iz, tr.box, tr.N, seg, in_idx, mode, lo, hi, n_tiles, tile_count);
+    else
+        hipLaunchKernelGGL(select_write_kernel, dim3(n_tiles, n_seg), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
+                           tr.iy, tr.iz, tr.box, tr.N, seg, in_idx, mode, lo, hi, n_tiles, tile_base, out);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 //   Incoherent_Scattering_Function  src/incoherent_scattering_function.h
 //   Mean_Square_Displacement        src/mean_square_displacement.h
 //   Non_Gaussian_Parameter          src/non_gaussian_parameter.h
+//   Fast_Particles lists            src/fast_particles.h (time-varying)
 //   Control                         src/control.h (the commands above)
 // The arithmetic runs on the device; results are FP64 there and are
 // stored as float here, as the reference stores them, before writing.
@@ -66,22 +67,37 @@ class Coordinate {
 class System;
 class Trajectory_List;
 
-// Static selection of atoms (Static_Trajectory_List semantics: the same
-// atoms at every frame, in the reference's species -> molecule -> type ->
-// index order).
+// Selection of atoms.  Static (Static_Trajectory_List: the same atoms at
+// every frame, in the reference's species -> molecule -> type -> index
+// order) or time-varying (Exptime_Trajectory_List and the Fast_Particles
+// lists built on it: one list per exponential block, the list of frame t
+// at list time int((t - frt) / n_exponential_steps);
+// src/exptime_trajectory_list.cpp:29-56).
 class Trajectory_List {
   public:
     Trajectory_List(System *sys, std::vector<int> atoms);
+    // Takes ownership of a time-varying device list.
+    Trajectory_List(System *sys, amdat_list *timed);
     ~Trajectory_List();
     Trajectory_List(const Trajectory_List &) = delete;
     Trajectory_List &operator=(const Trajectory_List &) = delete;
-    int show_n_trajectories(int timeii) const { (void)timeii; return (int)atoms_.size(); }
+    // Atoms at frame timeii (src/trajectory_list.h:58).
+    int show_n_trajectories(int timeii) const;
+    bool time_varying() const { return n_times_ > 0; }
+    // Every list time's atoms, concatenated in list-time order.
     const std::vector<int> &atoms() const { return atoms_; }
     amdat_list *handle() const { return list_; }
+    // Counts per list time: Trajectory_List::write_count
+    // (src/trajectory_list.cpp:231-249) for a static list,
+    // Exptime_Trajectory_List::write_count (src/exptime_trajectory_list.
+    // cpp:93-114: block start time and count) for a time-varying one.
+    void write_count(const std::string &filename) const;
 
   private:
     System *system_;
     std::vector<int> atoms_;
+    int n_times_ = 0;
+    std::vector<int> conv_, off_;
     amdat_list *list_ = nullptr;
 };
 
@@ -106,6 +122,7 @@ class System {
     int show_n_atoms() const { return n_atoms_; }
     float show_exp_base() const { return exp_base_; }
     const float *show_timelist() const { return timelist_.data(); }
+    float show_time(int timeii) const { return timelist_.at(timeii); }
     Coordinate min_box_dimensions() const;
     Coordinate size(int timeii) const;
     void set_limit(int limit) { displacement_limit_ = limit; }
@@ -198,7 +215,7 @@ class Wave_Density {
     void copy_from(const Wave_Density &other);
     double show_density_re(int time, int wavenumberindex, int vectorindex) const;
     double show_density_im(int time, int wavenumberindex, int vectorindex) const;
-    int show_n_atoms(int timeii) const { (void)timeii; return n_atoms_; }
+    int show_n_atoms(int timeii) const { return n_atoms_.at(timeii); }
     int show_first() const { return first_; }
     int show_last() const { return last_; }
     const Wave_Vectors *show_wavevectors() const { return wv_; }
@@ -210,7 +227,8 @@ class Wave_Density {
     void fetch() const;
     System *system_;
     const Wave_Vectors *wv_;
-    int first_, last_, n_atoms_ = 0;
+    int first_, last_;
+    std::vector<int> n_atoms_;  // atoms of the list at each frame
     amdat_rho *rho_ = nullptr;
     const Wave_Density *shared_ = nullptr;
     mutable std::vector<double> host_;
@@ -368,6 +386,7 @@ class Control {
     void isfs(const std::vector<std::string> &args);
     void isf(const std::vector<std::string> &args);
     void structure_factor(const std::vector<std::string> &args);
+    void find_fast_fixedthreshold(const std::vector<std::string> &args);
     Trajectory_List *target(const std::string &line);
 
     std::vector<std::string> lines_;

@@ -135,7 +135,8 @@ void Wave_Density::analyze(Trajectory_List *t_list)
     rho_ = nullptr;
     shared_ = nullptr;
     host_.clear();
-    n_atoms_ = t_list->show_n_trajectories(0);
+    n_atoms_.resize(system_->show_n_timesteps());
+    for (int t = 0; t < system_->show_n_timesteps(); t++) n_atoms_[t] = t_list->show_n_trajectories(t);
     check(amdat_rho_compute(device_context(), system_->traj(), t_list->handle(), wv_->handle(), first_, last_,
                             wv_->show_length_scale(), nullptr, 0, AMDAT_PREC_FP32C, &rho_),
           "wave density");

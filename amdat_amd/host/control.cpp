@@ -5,7 +5,7 @@
 // their target lines); command syntax: src/control.cpp:1860-1916 (limit,
 // create_list), 2163-2210 (msd), 2655-2758 (structure_factor), 3107-3192
 // (isf), 3198-3238 (isfs), 3345-3370 (ngp), 2219-2271 (msd_2d,
-// mean_displacement), 2517-2550 (vhs).
+// mean_displacement), 2517-2550 (vhs), 3466-3496 (find_fast_fixedthreshold).
 #include <cstdlib>
 #include <ctime>
 #include <fstream>
@@ -197,6 +197,41 @@ void Control::structure_factor(const std::vector<std::string> &args)
     sf.write(args[1]);
 }
 
+// find_fast_fixedthreshold <list> <count file> <time index> <threshold>,
+// then the target line (src/control.cpp:3466-3496): the atoms whose squared
+// displacement over the time index exceeds the threshold, per exponential
+// block (Fast_Particles, src/fast_particles.cpp:117-139); the list is
+// stored under <list> and its counts are written.
+void Control::find_fast_fixedthreshold(const std::vector<std::string> &args)
+{
+    if (args.size() != 5) throw Error("Incorrect number of arguments for find_fast_fixedthreshold.", 0);
+    const std::string listname = args[1], filename = args[2];
+    const int time_index = std::atoi(args[3].c_str());
+    const float threshold = (float)std::atof(args[4].c_str());
+    const std::string runline = read_line();
+    std::cout << "\n" << runline;
+    std::cout << "\nIdentifying 'fast' particles.";
+    Trajectory_List *in = target(runline);
+    amdat_list *out = nullptr;
+    System *s = system_.get();
+    int rc = amdat_list_select_displacement(device_context(), s->traj(), in->handle(), s->show_n_exponentials(),
+                                            s->show_n_exponential_steps(), s->show_frt(), s->show_limit(), time_index,
+                                            AMDAT_SELECT_FAST, threshold, 0.0f, &out);
+    if (rc < 0) throw Error(std::string("fast particles: ") + amdat_last_error(), 0);
+    auto l = std::make_unique<Trajectory_List>(s, out);
+    l->write_count(filename);
+    // Control::add_trajectorylist (src/control.cpp:1417-1429): an existing
+    // name is kept
+    for (auto &p : lists_)
+        if (p.first == listname) {
+            std::cout << "\nWarning: trajectory_list " << listname
+                      << " not created because a trajectory_list with this name already exists. Replacement of a "
+                         "trajectory_list requires that you first delete the existing list with the same name.\n";
+            return;
+        }
+    lists_.emplace_back(listname, std::move(l));
+}
+
 int Control::run()
 {
     // System block: type, trajectory type, file, time scheme, species,
@@ -234,6 +269,7 @@ int Control::run()
             else if (cmd == "msd_2d") msd_2d(args);
             else if (cmd == "mean_displacement") mean_displacement(args);
             else if (cmd == "vhs") vhs(args);
+            else if (cmd == "find_fast_fixedthreshold") find_fast_fixedthreshold(args);
             else if (cmd == "print") { std::cout << line.substr(line.find("print") + 5) << std::endl; }
             else throw Error("command " + cmd + " is not supported by this build", 0);
         } catch (const Error &e) {

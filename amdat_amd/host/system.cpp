@@ -55,7 +55,49 @@ Trajectory_List::Trajectory_List(System *sys, std::vector<int> atoms) : system_(
           "trajectory list");
 }
 
+Trajectory_List::Trajectory_List(System *sys, amdat_list *timed) : system_(sys), list_(timed)
+{
+    int n = 0;
+    check(amdat_list_shape(timed, &n_times_, &n), "trajectory list");
+    atoms_.resize(n);
+    if (n_times_ > 0) {
+        conv_.resize(sys->show_n_timesteps());
+        off_.resize(n_times_ + 1);
+    }
+    check(amdat_list_download(device_context(), timed, conv_.empty() ? nullptr : conv_.data(),
+                              off_.empty() ? nullptr : off_.data(), atoms_.empty() ? nullptr : atoms_.data()),
+          "trajectory list");
+}
+
 Trajectory_List::~Trajectory_List() { amdat_list_free(device_context(), list_); }
+
+int Trajectory_List::show_n_trajectories(int timeii) const
+{
+    if (n_times_ == 0) return (int)atoms_.size();
+    if (timeii < 0 || timeii >= (int)conv_.size()) return 0;
+    const int c = conv_[timeii];
+    return (c >= 0 && c < n_times_) ? off_[c + 1] - off_[c] : 0;
+}
+
+void Trajectory_List::write_count(const std::string &filename) const
+{
+    const int nt = n_times_ > 0 ? n_times_ : 1;
+    std::vector<int> counts(nt);
+    for (int c = 0; c < nt; c++) counts[c] = n_times_ > 0 ? off_[c + 1] - off_[c] : (int)atoms_.size();
+    double avg = 0.0;
+    for (int c = 0; c < nt; c++) avg += double(counts[c]) / double(nt);
+    std::ofstream out(filename);
+    out << "Average_trajectories:\t" << avg << "\n";
+    out << "Count_List\n";
+    for (int c = 0; c < nt; c++) {
+        if (n_times_ > 0) {
+            const int t = c * system_->show_n_exponential_steps() + system_->show_frt();
+            out << system_->show_time(t) << "\t" << counts[c] << "\n";
+        } else {
+            out << "\t" << counts[c] << "\n";
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // System

@@ -195,6 +195,45 @@ int amdat_list_create(amdat_ctx *ctx, const amdat_traj *tr, int n, const int *in
                       amdat_list **out);
 int amdat_list_free(amdat_ctx *ctx, amdat_list *l);
 
+/* Time-varying atom selection (Trajectory_List with n_times list times,
+ * src/trajectory_list.h:22-30; Exptime_Trajectory_List, src/exptime_
+ * trajectory_list.cpp:29-56): list time c holds indices[off[c] ..
+ * off[c+1]-1] in list order; time_conv[t] (n_frames entries) is the list
+ * time of frame t, and a frame whose list time lies outside [0, n_times)
+ * has no atoms.  Every entry point taking an amdat_list takes such a list:
+ * rho and S(k) use the list at each frame, Fs, MSD/NGP and the displacement
+ * statistics the list at the list time of each pair's t0
+ * (src/trajectory_list.cpp:191-211), and the counts follow.  The batched
+ * entry points take static lists only. */
+int amdat_list_create_timed(amdat_ctx *ctx, const amdat_traj *tr, int n_times, const int *time_conv,
+                            const int *off, const int *indices, amdat_list **out);
+/* n_times (0 for a static list) and total entries. */
+int amdat_list_shape(const amdat_list *l, int *n_times, int *n_entries);
+/* Copy a list to the host: time_conv[n_frames] and off[n_times+1] (time-
+ * varying lists; ignored for static ones) and indices[n_entries]; any
+ * output may be NULL. */
+int amdat_list_download(amdat_ctx *ctx, const amdat_list *l, int *time_conv, int *off, int *indices);
+
+/* Atoms selected by squared displacement at one time gap, per exponential
+ * block: Fast_Particles (src/fast_particles.cpp:117-139; the
+ * find_fast_fixedthreshold command, src/control.cpp:3466-3496),
+ * Slow_Particles (src/slow_particles.cpp:93-113) and Average_Particles
+ * (src/average_particles.cpp:95-116).  Pairs are those of the time gap as
+ * System::displacement_list(analysis, time_index, false) lists them
+ * (src/system.cpp:4349-4398, with limit, 0 = none); for each, the atoms of
+ * `in` (NULL = all atoms) at the list time of t0 whose squared displacement
+ * d2 = pow(distance, 2), formed in FP32 as the reference forms it, passes
+ * the test are appended, in list order, to the list of block
+ * int((t0 - frt) / n_exponential_steps).  The result is a time-varying
+ * list with n_exponentials list times and the exponential time conversion.
+ * FAST: d2 > lo; SLOW: d2 < hi; BETWEEN: lo <= d2 <= hi. */
+#define AMDAT_SELECT_FAST 0
+#define AMDAT_SELECT_SLOW 1
+#define AMDAT_SELECT_BETWEEN 2
+int amdat_list_select_displacement(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *in, int n_exponentials,
+                                   int n_exponential_steps, int frt, int limit, int time_index, int mode, float lo,
+                                   float hi, amdat_list **out);
+
 /* ---- pairs --------------------------------------------------------- */
 
 /* Upload (tg, t0, t1) pairs as produced by amdat_ts_pairs. */

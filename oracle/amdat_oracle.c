@@ -39,6 +39,11 @@
  *   orc_msd_ngp_f64     MSD(0) = 0, single-atom closed forms, image-count
  *                       rebuild equal to an explicit unwrap, Gaussian
  *                       walk MSD = 3 s^2 n.
+ *   orc_select_displacement  constructed trajectories with exactly
+ *                       representable displacements (membership known in
+ *                       closed form per block, image crossings, the squared
+ *                       threshold), list order, limit, cross-block and last
+ *                       gaps, the list-time conversion.
  *   orc_ref32_*         against the FP64 functions within the FP32 error
  *                       the reference itself carries.  Bit-level parity
  *                       with the reference's binary: parity unpinned (the
@@ -519,6 +524,102 @@ void orc_vhs_counts(int N, const float *x, const float *y, const float *z,
             weighting[tg[p]] += 1.0;
         }
     }
+}
+
+/* Fast_Particles / Slow_Particles / Average_Particles (src/fast_particles.
+ * cpp:117-139, src/slow_particles.cpp:93-113, src/average_particles.cpp:
+ * 95-116).  Pairs of System::displacement_list(analysis, time_index, false)
+ * (src/system.cpp:4349-4398); for each, the atoms of the input list at the
+ * list time of t0 (in_n_times == 0: static list in_idx[in_n], or all atoms
+ * when in_idx is NULL; otherwise in_conv[t] and in_off/in_idx) are tested
+ * with square_displacement = pow(distance, 2) stored as float, distance =
+ * sqrtf of the FP32 squared length of the difference of FP32 unwrapped
+ * coordinates (src/trajectory.cpp:326-330, src/coordinate.h:55), and
+ * appended to block int((float(t0) - float(frt)) / float(nes)) when
+ * mode 0: d2 > lo (Fast), 1: d2 < hi (Slow), 2: d2 <= hi && d2 >= lo
+ * (Average).  Output: off[nexp+1], idx (cap entries), conv[T] =
+ * int(float(t - frt) / float(nes)) (src/exptime_trajectory_list.cpp:87-91).
+ * Returns the entry count, or -1 for a time gap out of range. */
+long orc_select_displacement(int N, int T, const float *x, const float *y, const float *z,
+                             const int16_t *ix, const int16_t *iy, const int16_t *iz, const float *box,
+                             int in_n_times, const int *in_conv, const int *in_off, const int *in_idx, int in_n,
+                             int nexp, int nes, int frt, int limit, int time_index, int mode, float lo, float hi,
+                             int *off, int *idx, long cap, int *conv)
+{
+    int f = frt ? 1 : 0;
+    int ntg = nes + nexp - 1 + (1 - f);
+    int g = time_index;
+    int np = 0, *p0 = malloc(sizeof(int) * (nexp + 1)), *p1 = malloc(sizeof(int) * (nexp + 1));
+    if (g >= 0 && g < nes) {
+        int count = 0;
+        for (int b = 0; b < nexp; b++) {
+            p0[np] = nes * b + f; p1[np] = nes * b + f + g; np++;
+            count++;
+            if (count == limit) break;
+        }
+    } else if (g >= nes && g < ntg - 1 + f) {
+        int btg = g - nes + 1, count = 0;
+        for (int e = 0; e < 1; e++) {
+            for (int b = 0; b < nexp - btg; b++) {
+                p0[np] = nes * b + e + f; p1[np] = p0[np] + nes * btg; np++;
+                count++;
+                if (count == limit) break;
+            }
+            if (count == limit) break;
+        }
+    } else if (g == ntg - 1 + f) {
+        p0[np] = 0; p1[np] = nes * nexp; np++;
+    } else {
+        free(p0); free(p1);
+        return -1;
+    }
+    /* per-block lists, appended in pair order */
+    long *bn = calloc(nexp, sizeof(long));
+    int **bl = calloc(nexp, sizeof(int *));
+    for (int q = 0; q < np; q++) {
+        int t0 = p0[q], t1 = p1[q];
+        int expindex = (int)(((float)t0 - (float)f) / (float)nes);
+        const int *atoms = in_idx;
+        int cnt;
+        if (in_n_times == 0) {
+            cnt = in_idx ? in_n : N;
+        } else {
+            int c = in_conv[t0];
+            if (c < 0 || c >= in_n_times) { cnt = 0; atoms = in_idx; }
+            else { atoms = in_idx + in_off[c]; cnt = in_off[c + 1] - in_off[c]; }
+        }
+        size_t o0 = (size_t)t0 * N, o1 = (size_t)t1 * N;
+        const float *b0 = box + 3*t0, *b1 = box + 3*t1;
+        for (int a = 0; a < cnt; a++) {
+            int j = atoms ? atoms[a] : a;
+            float dx = unwrap32(x[o1+j], ix[o1+j], b1[0]) - unwrap32(x[o0+j], ix[o0+j], b0[0]);
+            float dy = unwrap32(y[o1+j], iy[o1+j], b1[1]) - unwrap32(y[o0+j], iy[o0+j], b0[1]);
+            float dz = unwrap32(z[o1+j], iz[o1+j], b1[2]) - unwrap32(z[o0+j], iz[o0+j], b0[2]);
+            float distance = sqrtf(dx*dx + dy*dy + dz*dz);
+            float square_displacement = (float)pow((double)distance, 2.0);
+            int keep;
+            if (mode == 0) keep = square_displacement > lo;
+            else if (mode == 1) keep = square_displacement < hi;
+            else keep = square_displacement <= hi && square_displacement >= lo;
+            if (keep) {
+                bl[expindex] = realloc(bl[expindex], sizeof(int) * (bn[expindex] + 1));
+                bl[expindex][bn[expindex]++] = j;
+            }
+        }
+    }
+    long total = 0;
+    off[0] = 0;
+    for (int b = 0; b < nexp; b++) {
+        for (long i = 0; i < bn[b]; i++) {
+            if (total < cap) idx[total] = bl[b][i];
+            total++;
+        }
+        off[b + 1] = (int)total;
+        free(bl[b]);
+    }
+    for (int t = 0; t < T; t++) conv[t] = (int)((float)(t - f) / (float)nes);
+    free(bl); free(bn); free(p0); free(p1);
+    return total;
 }
 
 /* Incoherent_Scattering_Function::listkernel (src/incoherent_scattering_

@@ -238,6 +238,32 @@ def vhs_counts(tr, pairs, ntg, n_bins, max_range, atoms=None):
     return hist.reshape(ntg, n_bins), w
 
 
+def select_displacement(tr, nexp, nes, frt, limit, time_index, mode, lo, hi, in_list=None):
+    """Fast (mode 0) / Slow (1) / Average (2) particle lists per block.
+    in_list: None (all atoms), an index array (static list) or a tuple
+    (conv, off, idx) (time-varying list).  Returns (conv, off, idx)."""
+    keep, ptrs = _traj_ptrs(tr)
+    n_times, conv_in, off_in, idx_in, n_in = 0, None, None, None, 0
+    if isinstance(in_list, tuple):
+        conv_in, off_in, idx_in = (_c(v, np.int32) for v in in_list)
+        n_times = len(off_in) - 1
+    elif in_list is not None:
+        idx_in = _c(in_list, np.int32)
+        n_in = len(idx_in)
+    cap = tr.N * nexp + 1
+    off = np.zeros(nexp + 1, np.int32)
+    idx = np.zeros(cap, np.int32)
+    conv = np.zeros(tr.T, np.int32)
+    f = lib().orc_select_displacement
+    f.restype = C.c_long
+    n = f(tr.N, tr.T, *ptrs, n_times, _p(conv_in, C.c_int), _p(off_in, C.c_int), _p(idx_in, C.c_int), n_in, nexp,
+          nes, frt, limit, time_index, mode, C.c_float(lo), C.c_float(hi), _p(off, C.c_int), _p(idx, C.c_int),
+          C.c_long(cap), _p(conv, C.c_int))
+    if n < 0:
+        raise ValueError("time gap out of range")
+    return conv, off, idx[:n]
+
+
 # ---- reference FP32 arithmetic ----
 def ref32_rho(x, y, z, nvec, length_scale, atoms=None):
     x, y, z = (_c(a, np.float32) for a in (x, y, z))

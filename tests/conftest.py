@@ -75,3 +75,25 @@ def write_plane_files(dirpath, qvec2d=None, qvec1d=None, bins=range(2, 101)):
                 fh.write("   %.7e   %.7e   %.7e\r\n" % tuple(qvec1d[b - 1] / 2.0))
         stems["1d"] = os.path.join(d, "qvector")
     return stems
+
+
+def atoms_at(lst, t):
+    """Atoms of a time-varying list (conv, off, idx) at frame t."""
+    conv, off, idx = lst
+    c = conv[t]
+    if c < 0 or c >= len(off) - 1:
+        return idx[:0]
+    return idx[off[c]:off[c + 1]]
+
+
+def compose_pairs(lst, pairs, fn):
+    """For a time-varying list: call fn(sub_pairs, atoms) once per list time
+    of the pairs' t0 (skipping empty lists) and yield the results."""
+    tg, t0, t1 = (np.asarray(v) for v in pairs)
+    keys = np.array([lst[0][t] if 0 <= lst[0][t] < len(lst[1]) - 1 else -1 for t in t0])
+    for c in np.unique(keys):
+        sel = keys == c
+        at = atoms_at(lst, int(t0[sel][0]))
+        if len(at) == 0:
+            continue
+        yield fn((tg[sel], t0[sel], t1[sel]), at)
