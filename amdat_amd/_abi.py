@@ -48,6 +48,7 @@ SIGNATURES = {
     "amdat_shells_generate": (_I, [_I, _I, _I, C.c_uint64, _PI, _PI, _I, _PI]),
     "amdat_shells_upload": (_I, [_P, _I, _PI, _PI, _PP]),
     "amdat_shells_free": (_I, [_P, _P]),
+    "amdat_shells_plan_hint": (_I, [_P, _I, _I]),
     "amdat_traj_upload": (_I, [_P, _I, _I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PP]),
     "amdat_traj_wrap_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _PF, _PP]),
     "amdat_traj_free": (_I, [_P, _P]),
@@ -269,6 +270,10 @@ class Shells(_Handle):
         super().__init__(ctx, h)
         self.off = off
         self.nvec = nv.reshape(-1, 3)
+
+    def plan_hint(self, R=0, pair=0):
+        """Fix vectors per thread and pairing of the phase kernels (R = 0: automatic)."""
+        check(lib().amdat_shells_plan_hint(self.h, R, pair))
 
 
 class Trajectory(_Handle):

@@ -22,14 +22,30 @@ void clear_error();
 // permuted so the 32 lanes of a half-wave read y and z entries in distinct
 // LDS bank pairs where possible.  Chunks are 512 consecutive units; unused
 // slots carry (nx of their unit, 0, 0) and destination -1.
+// Table geometry shared with the device code (Geom<MM> in phase_device.h):
+// the half-width MM serving a largest |component| M, and the entry offsets
+// of the y and z sections in a table row.
+constexpr int table_half_width(int M)
+{
+    return M <= 24 ? 24 : (M <= 50 ? 50 : (M <= 100 ? 100 : (M <= 127 ? 127 : -1)));
+}
+constexpr int table_oy(int MM) { return 3 * MM + 1; }
+constexpr int table_oz(int MM) { return 4 * MM + 2; }
+
+// kpack bit 24: the slot's roles of y and z are swapped (a paired slot
+// whose pair shares nz rather than ny).
+constexpr uint32_t kPackSwap = 1u << 24;
+
 struct PlanLayout {
     int R = 0, KB = 0, n_chunks = 0;
     std::vector<uint32_t> pack;  // [n_chunks][R][512]: (nx+128) | (ny+128)<<8 | nz<<16
     std::vector<int> dst;        // same layout: destination index | bit 30 (conjugate), or -1
     std::vector<int> range;      // [n_chunks][4]: nx lo, nx hi, max |ny| | max nz << 16, units with vectors
     long conflicts = 0;          // extra LDS passes summed over (chunk, slot, half-wave, axis)
+    bool pair = false;           // slots (2p, 2p+1) of a thread share ny, or nz (kPackSwap)
 };
-PlanLayout plan_layout(const int *nvec, int K, int R);
+// MM: table half-width of the kernels that read the plan (bank model only).
+PlanLayout plan_layout(const int *nvec, int K, int R, bool pair, int MM);
 
 // ---- trajectory ingestion (host_io.cpp) ----
 struct MappedFile {
@@ -57,8 +73,10 @@ int unwrap_images(int N, int T, const float *L, const float *x, const float *y, 
 }  // namespace amdat
 
 extern "C" {
-// Test hook: the layout of plan_layout for K vectors nvec[K][3].  With
+// Test hook: the layout of plan_layout for K vectors nvec[K][3] (pair != 0:
+// slots 2p and 2p+1 of a thread share ny, or nz where kPackSwap is set).
+// With
 // pack == NULL only *n_chunks is set.  Returns 0 or a negative status.
-int amdat_internal_plan_layout(const int *nvec, int K, int R, uint32_t *pack, int *dst, int *range,
+int amdat_internal_plan_layout(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst, int *range,
                                int *n_chunks, long *conflicts);
 }

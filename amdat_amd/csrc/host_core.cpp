@@ -285,6 +285,16 @@ constexpr int kBankPairs = 32;     // 64 banks of 4 bytes = 32 eight-byte entrie
 
 struct Vec {
     int nx, ny, nz, d;
+    bool sw = false;  // paired on nz: the shared read is z, the own read y
+};
+
+// Table entries a slot reads besides x: the one its pair shares (y, or z
+// when swapped) and its own (z, or y).  Their bank pairs follow the entry
+// index: every lane of a read addresses the same atom row.
+struct Entries {
+    int oy, oz;
+    int shared(const Vec &v) const { return v.sw ? oz + v.nz : oy + v.ny; }
+    int own(const Vec &v) const { return v.sw ? oy + v.ny : oz + v.nz; }
 };
 
 // Extra LDS passes of one read instruction: the largest number of distinct
@@ -301,13 +311,62 @@ long passes_minus_one(const std::vector<int> &vals)
     return worst > 0 ? worst - 1 : 0;
 }
 
+// Slot order of one half-wave of units (kLanesPerPass consecutive units):
+// greedy placement that keeps reads of one slot by the half-wave's lanes in
+// distinct bank pairs where it can.  Unpaired layouts place single slots
+// (each reads y and z); paired layouts place pairs of slots (2p, 2p + 1)
+// that share ny, of which only slot 2p reads y, in either orientation.
+void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, bool pair, const Entries &E)
+{
+    std::vector<std::array<std::vector<int>, kBankPairs>> by(R), bz(R);
+    auto add = [](std::array<std::vector<int>, kBankPairs> &b, int v) {
+        auto &s = b[((v % kBankPairs) + kBankPairs) % kBankPairs];
+        if (std::find(s.begin(), s.end(), v) == s.end()) s.push_back(v);
+    };
+    auto cost1 = [](const std::array<std::vector<int>, kBankPairs> &b, int v) {
+        const auto &s = b[((v % kBankPairs) + kBankPairs) % kBankPairs];
+        return std::find(s.begin(), s.end(), v) == s.end() ? (int)s.size() : 0;
+    };
+    const int S = pair ? 2 : 1, G = R / S;
+    for (size_t u = h0; u < h0 + kLanesPerPass && u < units.size(); u++) {
+        std::vector<Vec> &un = units[u];
+        std::vector<Vec> placed(R);
+        std::vector<char> used_v(G, 0), used_p(G, 0);
+        for (int k = 0; k < G; k++) {
+            int bv = -1, bp = -1, bo = 0, bc = 1 << 30;
+            for (int i = 0; i < G; i++) {
+                if (used_v[i]) continue;
+                for (int p = 0; p < G; p++) {
+                    if (used_p[p]) continue;
+                    for (int o = 0; o < S; o++) {
+                        const Vec &a = un[S * i + o];
+                        int c = cost1(by[S * p], E.shared(a)) + cost1(bz[S * p], E.own(a));
+                        if (pair) c += cost1(bz[S * p + 1], E.own(un[S * i + 1 - o]));
+                        if (c < bc) { bc = c; bv = i; bp = p; bo = o; }
+                    }
+                }
+            }
+            used_v[bv] = used_p[bp] = 1;
+            for (int q = 0; q < S; q++) {
+                const Vec &v = un[S * bv + (q == 0 ? bo : 1 - bo)];
+                placed[S * bp + q] = v;
+                if (q == 0) add(by[S * bp], E.shared(v));
+                add(bz[S * bp + q], E.own(v));
+            }
+        }
+        un.swap(placed);
+    }
+}
+
 }  // namespace
 
-PlanLayout plan_layout(const int *nvec, int K, int R)
+PlanLayout plan_layout(const int *nvec, int K, int R, bool pair, int MM)
 {
+    const Entries E{table_oy(MM), table_oz(MM)};
     PlanLayout L;
     L.R = R;
     L.KB = 512 * R;
+    L.pair = pair;
     std::vector<Vec> vs(K);
     for (int v = 0; v < K; v++) {
         int nx = nvec[3 * v], ny = nvec[3 * v + 1], nz = nvec[3 * v + 2], d = v;
@@ -319,15 +378,46 @@ PlanLayout plan_layout(const int *nvec, int K, int R)
         if (a.ny != b.ny) return a.ny < b.ny;
         return a.nz < b.nz;
     });
-    // thread units: per nx group, stride-dealt
+    // Thread units: per nx group, groups of S slots dealt with stride over
+    // ceil(groups / (R / S)) units.  Paired (S = 2): two vectors with one
+    // ny; then, of the vectors left, two with one nz (swapped roles); then
+    // one vector and a padding slot sharing its entry.
+    const int S = pair ? 2 : 1, G = R / S;
     std::vector<std::vector<Vec>> units;
     for (int g0 = 0; g0 < K;) {
         int g1 = g0;
         while (g1 < K && vs[g1].nx == vs[g0].nx) g1++;
-        const int cnt = g1 - g0, U = (cnt + R - 1) / R;
+        std::vector<std::array<Vec, 2>> grp;
+        std::vector<Vec> left;
+        for (int i = g0; i < g1;) {
+            if (!pair) {
+                grp.push_back({vs[i], Vec{vs[i].nx, 0, 0, -1}});
+                i += 1;
+            } else if (i + 1 < g1 && vs[i + 1].ny == vs[i].ny) {
+                grp.push_back({vs[i], vs[i + 1]});
+                i += 2;
+            } else {
+                left.push_back(vs[i]);
+                i += 1;
+            }
+        }
+        std::stable_sort(left.begin(), left.end(), [](const Vec &a, const Vec &b) { return a.nz < b.nz; });
+        for (size_t i = 0; i < left.size();) {
+            if (i + 1 < left.size() && left[i + 1].nz == left[i].nz) {
+                Vec a = left[i], b = left[i + 1];
+                a.sw = b.sw = true;
+                grp.push_back({a, b});
+                i += 2;
+            } else {
+                grp.push_back({left[i], Vec{left[i].nx, left[i].ny, 0, -1}});
+                i += 1;
+            }
+        }
+        const int cnt = (int)grp.size(), U = (cnt + G - 1) / G;
         for (int u = 0; u < U; u++) {
             std::vector<Vec> un;
-            for (int q = u; q < cnt; q += U) un.push_back(vs[g0 + q]);
+            for (int q = u; q < cnt; q += U)
+                for (int k = 0; k < S; k++) un.push_back(grp[q][k]);
             while ((int)un.size() < R) un.push_back({vs[g0].nx, 0, 0, -1});
             units.push_back(un);
         }
@@ -339,43 +429,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R)
         const int nx = units.empty() ? 0 : units.back()[0].nx;
         units.push_back(std::vector<Vec>(R, Vec{nx, 0, 0, -1}));
     }
-    // slot permutation per unit, greedy over each half-wave of units
-    for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) {
-        // per slot r: values placed in each bank pair, for y and for z
-        std::vector<std::array<std::vector<int>, kBankPairs>> by(R), bz(R);
-        for (size_t u = h0; u < h0 + kLanesPerPass && u < units.size(); u++) {
-            std::vector<Vec> &un = units[u];
-            std::vector<Vec> placed(R);
-            std::vector<char> used_v(R, 0), used_r(R, 0);
-            auto cost = [&](const Vec &v, int r) {
-                auto &sy = by[r][((v.ny % kBankPairs) + kBankPairs) % kBankPairs];
-                auto &sz = bz[r][v.nz % kBankPairs];
-                int c = 0;
-                if (std::find(sy.begin(), sy.end(), v.ny) == sy.end()) c += (int)sy.size();
-                if (std::find(sz.begin(), sz.end(), v.nz) == sz.end()) c += (int)sz.size();
-                return c;
-            };
-            for (int k = 0; k < R; k++) {
-                int bv = -1, br = -1, bc = 1 << 30;
-                for (int i = 0; i < R; i++) {
-                    if (used_v[i]) continue;
-                    for (int r = 0; r < R; r++) {
-                        if (used_r[r]) continue;
-                        const int c = cost(un[i], r);
-                        if (c < bc) { bc = c; bv = i; br = r; }
-                    }
-                }
-                used_v[bv] = used_r[br] = 1;
-                placed[br] = un[bv];
-                const Vec &v = un[bv];
-                auto &sy = by[br][((v.ny % kBankPairs) + kBankPairs) % kBankPairs];
-                if (std::find(sy.begin(), sy.end(), v.ny) == sy.end()) sy.push_back(v.ny);
-                auto &sz = bz[br][v.nz % kBankPairs];
-                if (std::find(sz.begin(), sz.end(), v.nz) == sz.end()) sz.push_back(v.nz);
-            }
-            un.swap(placed);
-        }
-    }
+    for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) order_half_wave(units, h0, R, pair, E);
     // emit
     const size_t n = (size_t)L.n_chunks * L.KB;
     L.pack.resize(n);
@@ -388,7 +442,8 @@ PlanLayout plan_layout(const int *nvec, int K, int R)
             for (int r = 0; r < R; r++) {
                 const Vec &v = un[r];
                 const size_t pos = (size_t)c * L.KB + (size_t)r * 512 + t;
-                L.pack[pos] = (uint32_t)(v.nx + 128) | ((uint32_t)(v.ny + 128) << 8) | ((uint32_t)v.nz << 16);
+                L.pack[pos] = (uint32_t)(v.nx + 128) | ((uint32_t)(v.ny + 128) << 8) | ((uint32_t)v.nz << 16) |
+                              (v.sw ? kPackSwap : 0u);
                 L.dst[pos] = v.d;
                 xlo = std::min(xlo, v.nx);
                 xhi = std::max(xhi, v.nx);
@@ -406,11 +461,12 @@ PlanLayout plan_layout(const int *nvec, int K, int R)
                 std::vector<int> ys, zs, xs;
                 for (int t = h; t < h + kLanesPerPass; t++) {
                     const Vec &v = units[(size_t)c * 512 + t][r];
-                    ys.push_back(v.ny);
-                    zs.push_back(v.nz);
+                    ys.push_back(E.shared(v));
+                    zs.push_back(E.own(v));
                     if (r == 0) xs.push_back(v.nx);
                 }
-                L.conflicts += passes_minus_one(ys) + passes_minus_one(zs);
+                if (!pair || r % 2 == 0) L.conflicts += passes_minus_one(ys);
+                L.conflicts += passes_minus_one(zs);
                 if (r == 0) L.conflicts += passes_minus_one(xs);
             }
     }
@@ -419,11 +475,14 @@ PlanLayout plan_layout(const int *nvec, int K, int R)
 
 }  // namespace amdat
 
-extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, uint32_t *pack, int *dst, int *range,
-                                          int *n_chunks, long *conflicts)
+extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst,
+                                          int *range, int *n_chunks, long *conflicts)
 {
     if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks) return AMDAT_EINVAL;
-    amdat::PlanLayout L = amdat::plan_layout(nvec, K, R);
+    int M = 0;
+    for (int i = 0; i < 3 * K; i++) M = std::max(M, std::abs(nvec[i]));
+    if (table_half_width(M) < 0) return AMDAT_ERANGE;
+    amdat::PlanLayout L = amdat::plan_layout(nvec, K, R, pair != 0, table_half_width(M));
     *n_chunks = L.n_chunks;
     if (conflicts) *conflicts = L.conflicts;
     if (pack) {

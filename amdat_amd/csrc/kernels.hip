@@ -5,6 +5,7 @@
 // The shared tile machinery of the phase kernels is in phase_device.h.
 #include <algorithm>
 
+#include "abi_internal.h"
 #include "kernels.h"
 #include "phase_device.h"
 
@@ -12,8 +13,9 @@ namespace amdat {
 
 namespace {
 
-// rho_k(t), compensated FP32.  grid = (chunks, frames, splits).
-template <int R, int MM>
+// rho_k(t), compensated FP32.  grid = (chunks, frames, splits).  PAIR:
+// the plan pairs slots sharing ny.
+template <int R, int MM, bool PAIR>
 __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
     const int *__restrict__ list, int n_list, const int *__restrict__ frames,
@@ -25,14 +27,14 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
     float hr[R], lr[R], hi[R], li[R];
-    rho_block<R, MM>(X, Y, Z, list, (size_t)frames[fslot] * (size_t)N, a_begin, a_end, inv_ell,
+    rho_block<R, MM, PAIR>(X, Y, Z, list, (size_t)frames[fslot] * (size_t)N, a_begin, a_end, inv_ell,
                      kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
     rho_write<R>(kdst + (size_t)chunk * KB, hr, lr, hi, li, out + ((size_t)fslot * n_splits + split) * (size_t)K);
 }
 
 // Fs per (pair, vector), compensated FP32: sum over the split's atoms of
 // cos(k.dr).  grid = (chunks, pairs, splits).
-template <int R, int MM>
+template <int R, int MM, bool PAIR>
 __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
@@ -47,7 +49,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
     float hs[R], ls[R];
-    fs_block<R, MM>(X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N, a_begin,
+    fs_block<R, MM, PAIR>(X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N, a_begin,
                     a_end, inv_ell, kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, hs, ls);
     fs_write<R>(kdst + (size_t)chunk * KB, hs, ls, out + ((size_t)pslot * n_splits + split) * (size_t)K);
 }
@@ -535,13 +537,13 @@ __global__ __launch_bounds__(kBlock) void sk_adjoint_kernel(
     }
 }
 
-template <int R, int MM>
+template <int R, int MM, bool PAIR>
 hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, const int *frames, int n_frames,
                             const KPlan &kp, double inv_ell, int n_splits, double2 *out, hipStream_t st)
 {
     const int per_split = (n_list + n_splits - 1) / n_splits;
     const size_t lds = Geom<MM>::tab_bytes;
-    auto kfn = rho_c32_kernel<R, MM>;
+    auto kfn = rho_c32_kernel<R, MM, PAIR>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     dim3 grid(kp.n_chunks, n_frames, n_splits);
@@ -552,14 +554,14 @@ hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, cons
     return hipGetLastError();
 }
 
-template <int R, int MM>
+template <int R, int MM, bool PAIR>
 hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
                            int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
                            hipStream_t st)
 {
     const int per_split = (n_list + n_splits - 1) / n_splits;
     const size_t lds = Geom<MM>::tab_bytes;
-    auto kfn = fs_c32_kernel<R, MM>;
+    auto kfn = fs_c32_kernel<R, MM, PAIR>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     dim3 grid(kp.n_chunks, n_pairs, n_splits);
@@ -571,16 +573,42 @@ hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const
     return hipGetLastError();
 }
 
+template <int R, bool PAIR>
+hipError_t dispatch_rho_c32_p(int MM, const TrajDev &tr, const int *list, int n_list, const int *frames,
+                              int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,
+                              hipStream_t st)
+{
+    switch (MM) {
+    case 24: return launch_rho_c32_t<R, 24, PAIR>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 50: return launch_rho_c32_t<R, 50, PAIR>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 100: return launch_rho_c32_t<R, 100, PAIR>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 127: return launch_rho_c32_t<R, 127, PAIR>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Paired kernels exist for R = 4 and 8; any plan also runs on the unpaired
+// kernel (every slot carries its full vector).
 template <int R>
 hipError_t dispatch_rho_c32(int MM, const TrajDev &tr, const int *list, int n_list, const int *frames,
                             int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,
                             hipStream_t st)
 {
+    if constexpr (R >= 4)
+        if (kp.pair) return dispatch_rho_c32_p<R, true>(MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    return dispatch_rho_c32_p<R, false>(MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+}
+
+template <int R, bool PAIR>
+hipError_t dispatch_fs_c32_p(int MM, const TrajDev &tr, const int *list, int n_list, const int *t0,
+                             const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits,
+                             double *out, hipStream_t st)
+{
     switch (MM) {
-    case 24: return launch_rho_c32_t<R, 24>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
-    case 50: return launch_rho_c32_t<R, 50>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
-    case 100: return launch_rho_c32_t<R, 100>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
-    case 127: return launch_rho_c32_t<R, 127>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 24: return launch_fs_c32_t<R, 24, PAIR>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 50: return launch_fs_c32_t<R, 50, PAIR>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 100: return launch_fs_c32_t<R, 100, PAIR>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 127: return launch_fs_c32_t<R, 127, PAIR>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -590,25 +618,19 @@ hipError_t dispatch_fs_c32(int MM, const TrajDev &tr, const int *list, int n_lis
                            const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits,
                            double *out, hipStream_t st)
 {
-    switch (MM) {
-    case 24: return launch_fs_c32_t<R, 24>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 50: return launch_fs_c32_t<R, 50>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 100: return launch_fs_c32_t<R, 100>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 127: return launch_fs_c32_t<R, 127>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    default: return hipErrorInvalidValue;
-    }
+    if constexpr (R >= 4)
+        if (kp.pair) return dispatch_fs_c32_p<R, true>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    return dispatch_fs_c32_p<R, false>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
 }
 
 }  // namespace
 
-int plan_table_width(int max_component)
-{
-    if (max_component <= 24) return 24;
-    if (max_component <= 50) return 50;
-    if (max_component <= 100) return 100;
-    if (max_component <= 127) return 127;
-    return -1;
-}
+int plan_table_width(int max_component) { return table_half_width(max_component); }
+
+static_assert(Geom<24>::OY == table_oy(24) && Geom<24>::OZ == table_oz(24), "table geometry");
+static_assert(Geom<50>::OY == table_oy(50) && Geom<50>::OZ == table_oz(50), "table geometry");
+static_assert(Geom<100>::OY == table_oy(100) && Geom<100>::OZ == table_oz(100), "table geometry");
+static_assert(Geom<127>::OY == table_oy(127) && Geom<127>::OZ == table_oz(127), "table geometry");
 
 hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_list, const int *frames,
                       int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,

@@ -123,12 +123,17 @@ __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uin
     }
 }
 
+// kpack: (nx + 128) | (ny + 128) << 8 | nz << 16, bit 24 = roles of y and z
+// swapped (see below).
 // Hot loop over the atom tiles of one block, shared by the single-trajectory
 // and batched phase kernels.  FS = false: rho (complex sums into h0/l0 = Re,
 // h1/l1 = Im); FS = true: Fs (real part only, into h0/l0).  uf(a, c) gives
 // the fixed-point fraction of list position a on axis c.  The R vectors of
-// a thread share nx: e_x(nx) is read once per atom.
-template <int R, int MM, bool FS, typename UF>
+// a thread share nx: e_x(nx) is read once per atom.  PAIR (paired plans):
+// slots 2p and 2p+1 also share ny (or nz, with y and z swapped), so
+// q = e_x e_y is read and formed once per pair and each slot adds q e_z:
+// 6 VALU per update for rho instead of 8, 4 for Fs instead of 6.
+template <int R, int MM, bool FS, bool PAIR, typename UF>
 __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
                                             float2 *tab, float2 *anc, uint32_t *sU, int *sValid, const UF uf,
                                             float (&h0)[R], float (&l0)[R], float (&h1)[R], float (&l1)[R])
@@ -141,10 +146,14 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int nx, ny, nz;
-        unpack_k(kp[r * kPhaseBlock + tid], nx, ny, nz);
+        const uint32_t k = kp[r * kPhaseBlock + tid];
+        unpack_k(k, nx, ny, nz);
         if (r == 0) px = tab + Gm::OX + nx;
-        py[r] = tab + Gm::OY + ny;
-        pz[r] = tab + Gm::OZ + nz;
+        // py: the factor a pair shares (e_y, or e_z for a pair sharing nz),
+        // pz: the slot's own factor
+        const bool sw = (k >> 24) & 1u;
+        py[r] = sw ? tab + Gm::OZ + nz : tab + Gm::OY + ny;
+        pz[r] = sw ? tab + Gm::OY + ny : tab + Gm::OZ + nz;
         h0[r] = l0[r] = h1[r] = l1[r] = 0.0f;
     }
     // Groups of FT tiles; the FP32 partials of a group are folded into the
@@ -172,26 +181,32 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             // Software pipeline over the flattened (atom, vector) sequence:
             // the y and z reads of update u + kDepth, and the x read of the
             // next atom, are issued before update u is computed.
+            // PAIR: only the even slot of a pair reads e_y (R is even).
             float2 by[kDepth], bz[kDepth];
 #pragma unroll
             for (int u = 0; u < kDepth; ++u) {
-                by[u] = py[u % R][(u / R) * Gm::W];
+                if (!PAIR || u % 2 == 0) by[u] = py[u % R][(u / R) * Gm::W];
+                else by[u] = make_float2(0.0f, 0.0f);
                 bz[u] = pz[u % R][(u / R) * Gm::W];
             }
             float2 ex = px[0], ex_next = ex;
+            float qr = 0.0f, qi = 0.0f;
 #pragma clang loop unroll(full)
             for (int u = 0; u < Gm::A * R; ++u) {
                 const int a = u / R, r = u % R, slot = u % kDepth;
+                const bool lead = !PAIR || r % 2 == 0;  // this update forms q = e_x e_y
                 const float2 ey = by[slot], ez = bz[slot];
                 if (u + kDepth < Gm::A * R) {
                     const int v = u + kDepth;
-                    by[slot] = py[v % R][(v / R) * Gm::W];
+                    if (!PAIR || v % 2 == 0) by[slot] = py[v % R][(v / R) * Gm::W];
                     bz[slot] = pz[v % R][(v / R) * Gm::W];
                 }
                 if (r == 0 && a + 1 < Gm::A) ex_next = px[(a + 1) * Gm::W];
                 __builtin_amdgcn_sched_barrier(0);
-                const float qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
-                const float qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                if (lead) {
+                    qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+                    qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                }
                 p0[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, p0[r]));
                 if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
                 __builtin_amdgcn_sched_barrier(0);
@@ -248,7 +263,7 @@ struct DispFrac {
 };
 
 // rho over one block's atoms: compensated sums (hr + lr, hi + li) per slot.
-template <int R, int MM>
+template <int R, int MM, bool PAIR = false>
 __device__ __forceinline__ void rho_block(const float *__restrict__ X, const float *__restrict__ Y,
                                           const float *__restrict__ Z, const int *__restrict__ list, size_t foff,
                                           int a_begin, int a_end, double inv_ell, const uint32_t *__restrict__ kp,
@@ -256,7 +271,7 @@ __device__ __forceinline__ void rho_block(const float *__restrict__ X, const flo
                                           float (&lr)[R], float (&hi)[R], float (&li)[R])
 {
     const CoordFrac uf{X, Y, Z, list, foff, inv_ell};
-    phase_tiles<R, MM, false>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
+    phase_tiles<R, MM, false, PAIR>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
 }
 
 template <int R>
@@ -276,7 +291,7 @@ __device__ __forceinline__ void rho_write(const int *__restrict__ kd, const floa
 }
 
 // Fs over one block's atoms for the pair (o0, o1): compensated sum of cos(k.dr).
-template <int R, int MM>
+template <int R, int MM, bool PAIR = false>
 __device__ __forceinline__ void fs_block(const float *__restrict__ X, const float *__restrict__ Y,
                                          const float *__restrict__ Z, const int16_t *__restrict__ IX,
                                          const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
@@ -287,7 +302,7 @@ __device__ __forceinline__ void fs_block(const float *__restrict__ X, const floa
 {
     const DispFrac uf{X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell};
     float u1[R], u2[R];
-    phase_tiles<R, MM, true>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
+    phase_tiles<R, MM, true, PAIR>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
 }
 
 template <int R>

@@ -135,6 +135,12 @@ int amdat_shells_generate(int first_bin, int n_bins, int max_per_shell, uint64_t
 /* Upload a shell table to the device. */
 int amdat_shells_upload(amdat_ctx *ctx, int n_shells, const int *shell_off, const int *nvec,
                         amdat_shells **out);
+/* Tuning override, no reference counterpart (the reference has no device
+ * plans): fix the vectors per thread R (2, 4 or 8) and the pairing of slots
+ * that share ny (pair = 1, R = 4 or 8) for the single-trajectory phase
+ * kernels run on this table; R = 0 restores the automatic choice (lowest
+ * modelled cost).  Results do not depend on it beyond FP32 rounding. */
+int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair);
 int amdat_shells_free(amdat_ctx *ctx, amdat_shells *sh);
 
 /* ---- trajectory and atom lists ------------------------------------- */

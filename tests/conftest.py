@@ -97,3 +97,10 @@ def compose_pairs(lst, pairs, fn):
         if len(at) == 0:
             continue
         yield fn((tg[sel], t0[sel], t1[sel]), at)
+
+
+def rho_rel(got, ref, n_atoms):
+    """Per-vector error of rho: |d rho| / max(|rho|, sqrt(N)), the relative
+    error where the sum is coherent (|rho| above sqrt(N)) and the error on
+    the sqrt(N) scale of a sum of N random phases otherwise."""
+    return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), np.sqrt(n_atoms))))

@@ -12,7 +12,7 @@ from amdat_amd import _abi
 from tests.conftest import ref_shells
 
 
-def layout(nvec, R):
+def layout(nvec, R, pair=0):
     L = _abi.lib()
     fn = L.amdat_internal_plan_layout
     fn.restype = C.c_int
@@ -20,16 +20,17 @@ def layout(nvec, R):
     K = len(nv) // 3
     nc, cf = C.c_int(), C.c_long()
     P = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
-    assert fn(P(nv, C.c_int), K, R, None, None, None, C.byref(nc), C.byref(cf)) == 0
+    assert fn(P(nv, C.c_int), K, R, pair, None, None, None, C.byref(nc), C.byref(cf)) == 0
     n, KB = nc.value, 512 * R
     pack = np.zeros(n * KB, np.uint32)
     dst = np.zeros(n * KB, np.int32)
     rng = np.zeros(n * 4, np.int32)
-    assert fn(P(nv, C.c_int), K, R, P(pack, C.c_uint32), P(dst, C.c_int), P(rng, C.c_int), C.byref(nc),
+    assert fn(P(nv, C.c_int), K, R, pair, P(pack, C.c_uint32), P(dst, C.c_int), P(rng, C.c_int), C.byref(nc),
               C.byref(cf)) == 0
     nx = (pack & 0xFF).astype(np.int64) - 128
     ny = ((pack >> 8) & 0xFF).astype(np.int64) - 128
     nz = ((pack >> 16) & 0xFF).astype(np.int64)
+    layout.swap = ((pack >> 24) & 1).astype(bool).reshape(n, R, 512)
     return n, nx.reshape(n, R, 512), ny.reshape(n, R, 512), nz.reshape(n, R, 512), dst.reshape(n, R, 512), \
         rng.reshape(n, 4), cf.value
 
@@ -90,3 +91,45 @@ def test_layout_lowers_bank_conflicts(qvec):
                 if r == 0:
                     total += extra(nx[c, 0, h:h + 32])
     assert conflicts < total
+
+
+@pytest.mark.parametrize("bins,R", [((2, 100), 8), ((2, 100), 4), ((40, 41), 8), ((2, 3), 4), ((95, 100), 4)])
+def test_paired_layout_invariants(qvec, bins, R):
+    """Paired plans: the same invariants, and slots 2p, 2p+1 of a thread
+    share ny, or nz with both slots flagged as swapped (one shared factor per
+    pair); chunk count from pairing on ny first, then on nz."""
+    off, nvec = ref_shells(qvec, *bins)
+    K = len(nvec)
+    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R, pair=1)
+    sw = layout.swap
+    canon = np.where(nvec[:, 2:3] < 0, -nvec, nvec)
+    n_units = 0
+    for x in np.unique(canon[:, 0]):
+        sel = canon[canon[:, 0] == x]
+        ys, cy = np.unique(sel[:, 1], return_counts=True)
+        groups = int(np.sum(cy // 2))
+        left = [sel[sel[:, 1] == y][-1] for y, c in zip(ys, cy) if c % 2]
+        if left:
+            groups += int(np.sum(-(-np.unique(np.array(left)[:, 2], return_counts=True)[1] // 2)))
+        n_units += -(-groups // (R // 2))
+    assert n == max(1, -(-n_units // 512))
+    assert np.array_equal(sw[:, 0::2, :], sw[:, 1::2, :])       # both slots of a pair agree
+    if bins == (2, 100):
+        assert np.any(sw)                                         # nz pairs occur
+    shared_y = ny[:, 0::2, :] == ny[:, 1::2, :]
+    shared_z = nz[:, 0::2, :] == nz[:, 1::2, :]
+    assert np.all(np.where(sw[:, 0::2, :], shared_z, shared_y))
+    real = dst >= 0
+    idx = dst[real] & ~(1 << 30)
+    assert np.array_equal(np.sort(idx), np.arange(K))
+    flip = (dst[real] & (1 << 30)) != 0
+    got = np.stack([nx[real], ny[real], nz[real]], 1)
+    assert np.array_equal(got, nvec[idx] * np.where(flip, -1, 1)[:, None])
+    assert np.all(nz >= 0) and np.all(nx == nx[:, :1, :])
+    xlo, xhi = rng[:, 0][:, None, None], rng[:, 1][:, None, None]
+    my, mz = (rng[:, 2] & 0xFFFF)[:, None, None], (rng[:, 2] >> 16)[:, None, None]
+    assert np.all((nx >= xlo) & (nx <= xhi) & (np.abs(ny) <= my) & (nz <= mz))
+    act = rng[:, 3]
+    for c in range(n):
+        unit_real = real[c].any(axis=0)
+        assert np.all(unit_real[:act[c]]) and not np.any(unit_real[act[c]:])
