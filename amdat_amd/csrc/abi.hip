@@ -126,8 +126,8 @@ struct amdat_shells {
     uint64_t serial = next_serial();
     int n_shells = 0;
     std::vector<int> off, nvec;
-    std::map<std::tuple<int, int, int, int>, std::unique_ptr<KPlanOwned>> plans;  // (first, last, R, pair)
-    std::map<std::tuple<int, int, int>, std::pair<int, int>> choice;               // (first, last, c) -> (R, pair)
+    std::map<std::tuple<int, int, int, int>, std::unique_ptr<KPlanOwned>> plans;  // (first, last, R, mode)
+    std::map<std::tuple<int, int, int>, std::pair<int, int>> choice;               // (first, last, c) -> (R, mode)
     int hint_R = 0, hint_pair = 0;  // amdat_shells_plan_hint (0: automatic)
 };
 
@@ -239,18 +239,18 @@ int shell_range_M(const amdat_shells *sh, int first, int last)
     return M;
 }
 
-// Modelled cost of a layout: every (atom, chunk) costs KB padded updates
-// at c VALU ops per lane, c - 2 for a paired layout (q = e_x e_y formed
-// once per pair), plus the fixed part of choose_R.
+// Modelled cost of a layout per atom: every slot of a wave that runs the
+// sums costs c VALU ops per lane, c - 2 in paired waves (q = e_x e_y formed
+// once per pair), and every chunk the fixed part of choose_R.
 double layout_cost(const PlanLayout &L, double c)
 {
-    return (double)L.n_chunks * ((double)L.KB * (L.pair ? c - 2.0 : c) + 12000.0);
+    return (double)L.n_chunks * 12000.0 + (double)L.paired_slots * (c - 2.0) + (double)L.unpaired_slots * c;
 }
 
 // R_force > 0 fixes the vectors per thread and an unpaired layout (batched
 // launches need one R and run unpaired kernels).  Otherwise the plan is the
 // cheapest by layout_cost of R = 2, 4, 8 unpaired and, when allow_pair
-// (callers whose kernels use pairs), R = 4, 8 paired.
+// (callers whose kernels use pairs), R = 4, 8 paired (mode 1).
 // c: VALU ops per update of the kernel the plan serves (see choose_R).
 int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned **out, int R_force = 0,
                double c = 9.0, bool allow_pair = true)
@@ -262,37 +262,42 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
         for (int c = 0; c < 3; c++) M = std::max(M, std::abs(sh->nvec[3 * v + c]));
     if (M > 127) return set_error(AMDAT_ERANGE, "k-vector component beyond +-127");
     int R = R_force > 0 ? R_force : choose_R(K, M, c);
-    bool pair = false;
+    int mode = 0;  // plan_layout mode: 0 unpaired, 1 paired with padding, 2 paired and unpaired waves
     PlanLayout lay;
     bool have_layout = false;
     if (R_force <= 0 && sh->hint_R > 0) {
         R = sh->hint_R;
-        pair = sh->hint_pair != 0 && allow_pair;
+        mode = allow_pair ? sh->hint_pair : 0;
     } else if (R_force <= 0 && K > 0) {
         const auto ckey = std::make_tuple(first, last, (int)(c * 16.0) * 2 + (allow_pair ? 1 : 0));
         auto ch = sh->choice.find(ckey);
         if (ch != sh->choice.end()) {
             R = ch->second.first;
-            pair = ch->second.second != 0;
+            mode = ch->second.second;
         } else {
             double best = 1e300;
+            // Mode 2 is left to amdat_shells_plan_hint: its unpaired waves
+            // gather the partnerless vectors of many nx groups, whose table
+            // reads spread over the banks (64,000-atom workload: rho 542 ms
+            // in mode 2 against 464 ms in mode 1, although it issues fewer
+            // VALU ops).
             const int cand[5][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}};
             for (const auto &cd : cand) {
                 if (cd[1] && !allow_pair) continue;
-                PlanLayout L = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, cd[0], cd[1] != 0, plan_table_width(M));
+                PlanLayout L = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, cd[0], cd[1], plan_table_width(M));
                 const double cost = layout_cost(L, c);
                 if (cost < best * 0.98) {
                     best = cost;
                     lay = std::move(L);
                     have_layout = true;
                     R = cd[0];
-                    pair = cd[1] != 0;
+                    mode = cd[1];
                 }
             }
-            sh->choice[ckey] = {R, pair ? 1 : 0};
+            sh->choice[ckey] = {R, mode};
         }
     }
-    auto key = std::make_tuple(first, last, R, pair ? 1 : 0);
+    auto key = std::make_tuple(first, last, R, mode);
     auto it = sh->plans.find(key);
     if (it != sh->plans.end()) { *out = it->second.get(); return AMDAT_OK; }
 
@@ -307,13 +312,13 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
         sh->plans[key] = std::move(P);
         return AMDAT_OK;
     }
-    if (!have_layout) lay = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, R, pair, plan_table_width(M));
+    if (!have_layout) lay = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, R, mode, plan_table_width(M));
     kp.K = K;
     kp.R = R;
     kp.KB = lay.KB;
     kp.n_chunks = lay.n_chunks;
     kp.MM = plan_table_width(M);
-    kp.pair = pair ? 1 : 0;
+    kp.pair = mode != 0 ? 1 : 0;
     hipStream_t st = stream_of(ctx);
     HIP_TRY(P->kpack.upload(lay.pack.data(), lay.pack.size(), st));
     HIP_TRY(P->kdst.upload(lay.dst.data(), lay.dst.size(), st));
@@ -521,9 +526,10 @@ int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair)
     clear_error();
     if (!sh) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: null table");
     if (R != 0 && R != 2 && R != 4 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: R must be 0, 2, 4 or 8");
+    if (pair < 0 || pair > 2) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pair must be 0, 1 or 2");
     if (pair && R != 4 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pairs need R = 4 or 8");
     sh->hint_R = R;
-    sh->hint_pair = pair ? 1 : 0;
+    sh->hint_pair = pair;
     return AMDAT_OK;
 }
 

@@ -40,12 +40,16 @@ struct PlanLayout {
     int R = 0, KB = 0, n_chunks = 0;
     std::vector<uint32_t> pack;  // [n_chunks][R][512]: (nx+128) | (ny+128)<<8 | nz<<16
     std::vector<int> dst;        // same layout: destination index | bit 30 (conjugate), or -1
-    std::vector<int> range;      // [n_chunks][4]: nx lo, nx hi, max |ny| | max nz << 16, units with vectors
+    std::vector<int> range;      // [n_chunks][4]: nx lo, nx hi, max |ny| | max nz << 16,
+                                 // units with vectors | first unpaired unit << 16
     long conflicts = 0;          // extra LDS passes summed over (chunk, slot, half-wave, axis)
     bool pair = false;           // slots (2p, 2p+1) of a thread share ny, or nz (kPackSwap)
+    long paired_slots = 0, unpaired_slots = 0;  // slots of waves that run the sums, by kind
 };
-// MM: table half-width of the kernels that read the plan (bank model only).
-PlanLayout plan_layout(const int *nvec, int K, int R, bool pair, int MM);
+// mode 0: unpaired; 1: paired, vectors without a partner padded; 2: paired,
+// vectors without a partner in unpaired waves.  MM: table half-width of the
+// kernels that read the plan (bank model only).
+PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM);
 
 // ---- trajectory ingestion (host_io.cpp) ----
 struct MappedFile {
@@ -73,8 +77,9 @@ int unwrap_images(int N, int T, const float *L, const float *x, const float *y, 
 }  // namespace amdat
 
 extern "C" {
-// Test hook: the layout of plan_layout for K vectors nvec[K][3] (pair != 0:
-// slots 2p and 2p+1 of a thread share ny, or nz where kPackSwap is set).
+// Test hook: the layout of plan_layout for K vectors nvec[K][3] (pair = the
+// layout mode; paired slots 2p and 2p+1 share ny, or nz where kPackSwap is
+// set).
 // With
 // pack == NULL only *n_chunks is set.  Returns 0 or a negative status.
 int amdat_internal_plan_layout(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst, int *range,

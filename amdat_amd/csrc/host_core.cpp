@@ -360,9 +360,10 @@ void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, boo
 
 }  // namespace
 
-PlanLayout plan_layout(const int *nvec, int K, int R, bool pair, int MM)
+PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
 {
     const Entries E{table_oy(MM), table_oz(MM)};
+    const bool pair = mode != 0, mixed = mode == 2;
     PlanLayout L;
     L.R = R;
     L.KB = 512 * R;
@@ -378,22 +379,34 @@ PlanLayout plan_layout(const int *nvec, int K, int R, bool pair, int MM)
         if (a.ny != b.ny) return a.ny < b.ny;
         return a.nz < b.nz;
     });
-    // Thread units: per nx group, groups of S slots dealt with stride over
-    // ceil(groups / (R / S)) units.  Paired (S = 2): two vectors with one
-    // ny; then, of the vectors left, two with one nz (swapped roles); then
-    // one vector and a padding slot sharing its entry.
-    const int S = pair ? 2 : 1, G = R / S;
-    std::vector<std::vector<Vec>> units;
+    // Thread units.  Unpaired (mode 0): per nx group, R vectors per unit,
+    // dealt with stride over ceil(count / R) units.  Paired: per nx group,
+    // pairs of two vectors with one ny, then (of the vectors left) two with
+    // one nz (swapped roles), dealt R / 2 pairs per unit; each vector left
+    // over either forms a pair with a padding slot (mode 1) or goes to
+    // unpaired units (mode 2).  All paired units come first, padded to whole
+    // waves, so every wave is either paired or unpaired.
+    const int G = pair ? R / 2 : R;
+    std::vector<std::vector<Vec>> units_p, units_s;
+    auto deal = [&](const std::vector<std::array<Vec, 2>> &grp, int per_unit, int S, int nx,
+                    std::vector<std::vector<Vec>> &dst) {
+        const int cnt = (int)grp.size(), U = (cnt + per_unit - 1) / per_unit;
+        for (int u = 0; u < U; u++) {
+            std::vector<Vec> un;
+            for (int q = u; q < cnt; q += U)
+                for (int k = 0; k < S; k++) un.push_back(grp[q][k]);
+            while ((int)un.size() < R) un.push_back({nx, 0, 0, -1});
+            dst.push_back(un);
+        }
+    };
     for (int g0 = 0; g0 < K;) {
         int g1 = g0;
         while (g1 < K && vs[g1].nx == vs[g0].nx) g1++;
-        std::vector<std::array<Vec, 2>> grp;
+        const int nx = vs[g0].nx;
+        std::vector<std::array<Vec, 2>> grp, single;
         std::vector<Vec> left;
         for (int i = g0; i < g1;) {
-            if (!pair) {
-                grp.push_back({vs[i], Vec{vs[i].nx, 0, 0, -1}});
-                i += 1;
-            } else if (i + 1 < g1 && vs[i + 1].ny == vs[i].ny) {
+            if (pair && i + 1 < g1 && vs[i + 1].ny == vs[i].ny) {
                 grp.push_back({vs[i], vs[i + 1]});
                 i += 2;
             } else {
@@ -401,35 +414,38 @@ PlanLayout plan_layout(const int *nvec, int K, int R, bool pair, int MM)
                 i += 1;
             }
         }
-        std::stable_sort(left.begin(), left.end(), [](const Vec &a, const Vec &b) { return a.nz < b.nz; });
+        if (pair) std::stable_sort(left.begin(), left.end(), [](const Vec &a, const Vec &b) { return a.nz < b.nz; });
         for (size_t i = 0; i < left.size();) {
-            if (i + 1 < left.size() && left[i + 1].nz == left[i].nz) {
+            if (pair && i + 1 < left.size() && left[i + 1].nz == left[i].nz) {
                 Vec a = left[i], b = left[i + 1];
                 a.sw = b.sw = true;
                 grp.push_back({a, b});
                 i += 2;
+            } else if (pair && !mixed) {
+                grp.push_back({left[i], Vec{nx, left[i].ny, 0, -1}});
+                i += 1;
             } else {
-                grp.push_back({left[i], Vec{left[i].nx, left[i].ny, 0, -1}});
+                single.push_back({left[i], Vec{nx, 0, 0, -1}});
                 i += 1;
             }
         }
-        const int cnt = (int)grp.size(), U = (cnt + G - 1) / G;
-        for (int u = 0; u < U; u++) {
-            std::vector<Vec> un;
-            for (int q = u; q < cnt; q += U)
-                for (int k = 0; k < S; k++) un.push_back(grp[q][k]);
-            while ((int)un.size() < R) un.push_back({vs[g0].nx, 0, 0, -1});
-            units.push_back(un);
-        }
+        if (pair) deal(grp, G, 2, nx, units_p);
+        deal(single, R, 1, nx, units_s);
         g0 = g1;
     }
+    if (!units_s.empty())
+        while (units_p.size() % 64) units_p.push_back(std::vector<Vec>(R, Vec{units_p.back()[0].nx, 0, 0, -1}));
+    const int n_paired = (int)units_p.size();
+    std::vector<std::vector<Vec>> units = std::move(units_p);
+    units.insert(units.end(), units_s.begin(), units_s.end());
     const int n_units = (int)units.size();
     L.n_chunks = std::max(1, (n_units + 511) / 512);
     while ((int)units.size() < L.n_chunks * 512) {
         const int nx = units.empty() ? 0 : units.back()[0].nx;
         units.push_back(std::vector<Vec>(R, Vec{nx, 0, 0, -1}));
     }
-    for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) order_half_wave(units, h0, R, pair, E);
+    for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass)
+        order_half_wave(units, h0, R, (int)h0 < n_paired, E);
     // emit
     const size_t n = (size_t)L.n_chunks * L.KB;
     L.pack.resize(n);
@@ -454,7 +470,11 @@ PlanLayout plan_layout(const int *nvec, int K, int R, bool pair, int MM)
         L.range[4 * c] = xlo;
         L.range[4 * c + 1] = xhi;
         L.range[4 * c + 2] = my | (mz << 16);
-        L.range[4 * c + 3] = std::min(512, n_units - c * 512);
+        const int n_act = std::min(512, n_units - c * 512);
+        const int first_unpaired = std::max(0, std::min(512, n_paired - c * 512));
+        L.range[4 * c + 3] = n_act | (first_unpaired << 16);
+        for (int w = 0; w * 64 < n_act; w++)
+            (w * 64 < first_unpaired ? L.paired_slots : L.unpaired_slots) += 64L * R;
         // conflict estimate of the chunk's reads
         for (int r = 0; r < R; r++)
             for (int h = 0; h < 512; h += kLanesPerPass) {
@@ -465,7 +485,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, bool pair, int MM)
                     zs.push_back(E.own(v));
                     if (r == 0) xs.push_back(v.nx);
                 }
-                if (!pair || r % 2 == 0) L.conflicts += passes_minus_one(ys);
+                if (c * 512 + h >= n_paired || r % 2 == 0) L.conflicts += passes_minus_one(ys);
                 L.conflicts += passes_minus_one(zs);
                 if (r == 0) L.conflicts += passes_minus_one(xs);
             }
@@ -482,7 +502,7 @@ extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, int pai
     int M = 0;
     for (int i = 0; i < 3 * K; i++) M = std::max(M, std::abs(nvec[i]));
     if (table_half_width(M) < 0) return AMDAT_ERANGE;
-    amdat::PlanLayout L = amdat::plan_layout(nvec, K, R, pair != 0, table_half_width(M));
+    amdat::PlanLayout L = amdat::plan_layout(nvec, K, R, pair, table_half_width(M));
     *n_chunks = L.n_chunks;
     if (conflicts) *conflicts = L.conflicts;
     if (pack) {

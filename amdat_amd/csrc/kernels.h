@@ -29,7 +29,8 @@ struct KPlan {
     int MM = 0;        // table half-width the kernels were built for (max |n_c| <= MM)
     uint32_t *kpack = nullptr;  // [n_chunks*KB] (nx+128) | (ny+128)<<8 | nz<<16
     int *kdst = nullptr;        // [n_chunks*KB] destination index, -1 padding, bit 30 = conjugate
-    int4 *range = nullptr;      // [n_chunks] (nx lo, nx hi, max|ny| | max nz << 16, units with vectors)
+    int4 *range = nullptr;      // [n_chunks] (nx lo, nx hi, max|ny| | max nz << 16,
+                                //  units with vectors | first unpaired unit << 16)
     int pair = 0;               // 1: slots 2p and 2p+1 of a thread share ny (e_x e_y formed once per pair)
 };
 

@@ -123,6 +123,49 @@ __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uin
     }
 }
 
+// One tile of the hot loop: A atoms x R slots, accumulated into the FP32
+// partials p0 (Re, or Fs) and p1 (Im).  Software pipeline over the
+// flattened (atom, slot) sequence: the reads of update u + kDepth, and the
+// x read of the next atom, are issued before update u is computed.
+// PAIRED: slots 2p and 2p+1 share the factor behind py, so only the even
+// slot reads it and forms q = e_x e_y (R is even).
+template <int R, int MM, bool FS, bool PAIRED>
+__device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (&py)[R], const float2 *const (&pz)[R],
+                                         float (&p0)[R], float (&p1)[R])
+{
+    using Gm = Geom<MM>;
+    float2 by[kDepth], bz[kDepth];
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) {
+        if (!PAIRED || u % 2 == 0) by[u] = py[u % R][(u / R) * Gm::W];
+        else by[u] = make_float2(0.0f, 0.0f);
+        bz[u] = pz[u % R][(u / R) * Gm::W];
+    }
+    float2 ex = px[0], ex_next = ex;
+    float qr = 0.0f, qi = 0.0f;
+#pragma clang loop unroll(full)
+    for (int u = 0; u < Gm::A * R; ++u) {
+        const int a = u / R, r = u % R, slot = u % kDepth;
+        const bool lead = !PAIRED || r % 2 == 0;  // this update forms q = e_x e_y
+        const float2 ey = by[slot], ez = bz[slot];
+        if (u + kDepth < Gm::A * R) {
+            const int v = u + kDepth;
+            if (!PAIRED || v % 2 == 0) by[slot] = py[v % R][(v / R) * Gm::W];
+            bz[slot] = pz[v % R][(v / R) * Gm::W];
+        }
+        if (r == 0 && a + 1 < Gm::A) ex_next = px[(a + 1) * Gm::W];
+        __builtin_amdgcn_sched_barrier(0);
+        if (lead) {
+            qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+            qi = fmaf(ex.x, ey.y, ex.y * ey.x);
+        }
+        p0[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, p0[r]));
+        if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
+        __builtin_amdgcn_sched_barrier(0);
+        if (r == R - 1) ex = ex_next;
+    }
+}
+
 // kpack: (nx + 128) | (ny + 128) << 8 | nz << 16, bit 24 = roles of y and z
 // swapped (see below).
 // Hot loop over the atom tiles of one block, shared by the single-trajectory
@@ -130,9 +173,10 @@ __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uin
 // h1/l1 = Im); FS = true: Fs (real part only, into h0/l0).  uf(a, c) gives
 // the fixed-point fraction of list position a on axis c.  The R vectors of
 // a thread share nx: e_x(nx) is read once per atom.  PAIR (paired plans):
-// slots 2p and 2p+1 also share ny (or nz, with y and z swapped), so
-// q = e_x e_y is read and formed once per pair and each slot adds q e_z:
-// 6 VALU per update for rho instead of 8, 4 for Fs instead of 6.
+// in the waves that hold pairs, slots 2p and 2p+1 also share ny (or nz,
+// with y and z swapped), so q = e_x e_y is read and formed once per pair and
+// each slot adds q e_z: 6 VALU per update for rho instead of 8, 4 for Fs
+// instead of 6; the plan's other waves run unpaired.
 template <int R, int MM, bool FS, bool PAIR, typename UF>
 __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
                                             float2 *tab, float2 *anc, uint32_t *sU, int *sValid, const UF uf,
@@ -140,8 +184,11 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
 {
     using Gm = Geom<MM>;
     const int tid = threadIdx.x;
-    // waves whose units all lie past the chunk's last real unit skip the sums
-    const bool active = __builtin_amdgcn_readfirstlane(tid >> 6) * 64 < rng.w;
+    // waves whose units all lie past the chunk's last real unit skip the
+    // sums; in paired plans, waves before the first unpaired unit hold pairs
+    const int wave0 = __builtin_amdgcn_readfirstlane(tid >> 6) * 64;
+    const bool active = wave0 < (rng.w & 0xFFFF);
+    const bool paired = wave0 < (rng.w >> 16);
     const float2 *px = nullptr, *py[R], *pz[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -178,39 +225,11 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             build_tables<MM>(tab, anc, sU, sValid, rng);
             __syncthreads();
             if (!active) continue;
-            // Software pipeline over the flattened (atom, vector) sequence:
-            // the y and z reads of update u + kDepth, and the x read of the
-            // next atom, are issued before update u is computed.
-            // PAIR: only the even slot of a pair reads e_y (R is even).
-            float2 by[kDepth], bz[kDepth];
-#pragma unroll
-            for (int u = 0; u < kDepth; ++u) {
-                if (!PAIR || u % 2 == 0) by[u] = py[u % R][(u / R) * Gm::W];
-                else by[u] = make_float2(0.0f, 0.0f);
-                bz[u] = pz[u % R][(u / R) * Gm::W];
-            }
-            float2 ex = px[0], ex_next = ex;
-            float qr = 0.0f, qi = 0.0f;
-#pragma clang loop unroll(full)
-            for (int u = 0; u < Gm::A * R; ++u) {
-                const int a = u / R, r = u % R, slot = u % kDepth;
-                const bool lead = !PAIR || r % 2 == 0;  // this update forms q = e_x e_y
-                const float2 ey = by[slot], ez = bz[slot];
-                if (u + kDepth < Gm::A * R) {
-                    const int v = u + kDepth;
-                    if (!PAIR || v % 2 == 0) by[slot] = py[v % R][(v / R) * Gm::W];
-                    bz[slot] = pz[v % R][(v / R) * Gm::W];
-                }
-                if (r == 0 && a + 1 < Gm::A) ex_next = px[(a + 1) * Gm::W];
-                __builtin_amdgcn_sched_barrier(0);
-                if (lead) {
-                    qr = fmaf(ex.x, ey.x, -ex.y * ey.y);
-                    qi = fmaf(ex.x, ey.y, ex.y * ey.x);
-                }
-                p0[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, p0[r]));
-                if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
-                __builtin_amdgcn_sched_barrier(0);
-                if (r == R - 1) ex = ex_next;
+            if constexpr (PAIR) {
+                if (paired) hot_tile<R, MM, FS, true>(px, py, pz, p0, p1);
+                else hot_tile<R, MM, FS, false>(px, py, pz, p0, p1);
+            } else {
+                hot_tile<R, MM, FS, false>(px, py, pz, p0, p1);
             }
         }
 #pragma unroll

@@ -57,7 +57,8 @@ def test_layout_invariants(qvec, bins, R):
     xlo, xhi = rng[:, 0][:, None, None], rng[:, 1][:, None, None]
     my, mz = (rng[:, 2] & 0xFFFF)[:, None, None], (rng[:, 2] >> 16)[:, None, None]
     assert np.all((nx >= xlo) & (nx <= xhi) & (np.abs(ny) <= my) & (nz <= mz))
-    act = rng[:, 3]
+    act = rng[:, 3] & 0xFFFF
+    assert np.all(rng[:, 3] >> 16 == 0)                          # unpaired plan: no paired waves
     for c in range(n):
         unit_real = real[c].any(axis=0)                          # [512]
         assert np.all(unit_real[:act[c]]) and not np.any(unit_real[act[c]:])
@@ -93,32 +94,50 @@ def test_layout_lowers_bank_conflicts(qvec):
     assert conflicts < total
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("bins,R", [((2, 100), 8), ((2, 100), 4), ((40, 41), 8), ((2, 3), 4), ((95, 100), 4)])
-def test_paired_layout_invariants(qvec, bins, R):
-    """Paired plans: the same invariants, and slots 2p, 2p+1 of a thread
-    share ny, or nz with both slots flagged as swapped (one shared factor per
-    pair); chunk count from pairing on ny first, then on nz."""
+def test_paired_layout_invariants(qvec, bins, R, mode):
+    """Paired plans: the same invariants; units before the first unpaired
+    unit (whole waves) hold pairs sharing ny, or nz with both slots flagged
+    as swapped, and the rest hold single vectors; chunk count from pairing
+    on ny first, then on nz, and the vectors left padded into pairs (mode 1)
+    or R per unpaired unit (mode 2)."""
     off, nvec = ref_shells(qvec, *bins)
     K = len(nvec)
-    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R, pair=1)
+    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R, pair=mode)
     sw = layout.swap
     canon = np.where(nvec[:, 2:3] < 0, -nvec, nvec)
-    n_units = 0
+    up = us = 0
     for x in np.unique(canon[:, 0]):
         sel = canon[canon[:, 0] == x]
+        sel = sel[np.lexsort((sel[:, 2], sel[:, 1]))]               # (ny, nz) order: the odd one is the last
         ys, cy = np.unique(sel[:, 1], return_counts=True)
         groups = int(np.sum(cy // 2))
         left = [sel[sel[:, 1] == y][-1] for y, c in zip(ys, cy) if c % 2]
+        singles = 0
         if left:
-            groups += int(np.sum(-(-np.unique(np.array(left)[:, 2], return_counts=True)[1] // 2)))
-        n_units += -(-groups // (R // 2))
-    assert n == max(1, -(-n_units // 512))
+            cz = np.unique(np.array(left)[:, 2], return_counts=True)[1]
+            groups += int(np.sum(cz // 2))
+            singles = int(np.sum(cz % 2))
+        if mode == 1:
+            up += -(-(groups + singles) // (R // 2))
+        else:
+            up += -(-groups // (R // 2))
+            us += -(-singles // R)
+    n_paired = up if us == 0 else -(-up // 64) * 64
+    assert n == max(1, -(-(n_paired + us) // 512))
+    first = rng[:, 3] >> 16
+    unit = np.arange(n)[:, None] * 512 + np.arange(512)[None, :]  # [n, 512] global unit index
+    assert np.array_equal(first, np.clip(n_paired - np.arange(n) * 512, 0, 512))
+    is_p = unit < n_paired
     assert np.array_equal(sw[:, 0::2, :], sw[:, 1::2, :])       # both slots of a pair agree
     if bins == (2, 100):
         assert np.any(sw)                                         # nz pairs occur
     shared_y = ny[:, 0::2, :] == ny[:, 1::2, :]
     shared_z = nz[:, 0::2, :] == nz[:, 1::2, :]
-    assert np.all(np.where(sw[:, 0::2, :], shared_z, shared_y))
+    ok = np.where(sw[:, 0::2, :], shared_z, shared_y)
+    assert np.all(ok | ~is_p[:, None, :])                          # paired units share a factor
+    assert not np.any(sw & ~is_p[:, None, :])                     # single units are never swapped
     real = dst >= 0
     idx = dst[real] & ~(1 << 30)
     assert np.array_equal(np.sort(idx), np.arange(K))
@@ -129,7 +148,7 @@ def test_paired_layout_invariants(qvec, bins, R):
     xlo, xhi = rng[:, 0][:, None, None], rng[:, 1][:, None, None]
     my, mz = (rng[:, 2] & 0xFFFF)[:, None, None], (rng[:, 2] >> 16)[:, None, None]
     assert np.all((nx >= xlo) & (nx <= xhi) & (np.abs(ny) <= my) & (nz <= mz))
-    act = rng[:, 3]
+    act = rng[:, 3] & 0xFFFF
     for c in range(n):
         unit_real = real[c].any(axis=0)
-        assert np.all(unit_real[:act[c]]) and not np.any(unit_real[act[c]:])
+        assert not np.any(unit_real[act[c]:])
