@@ -81,7 +81,7 @@ SIGNATURES = {
     "amdat_batch_msd_ngp": (_I, [_P, _P, _P, _PD, _PD, _PD]),
     "amdat_batch_run": (_I, [_P, _P, _PF, _P, _I, _I, _PI, _I, _P, _PI, _PI, _P, _I, _I,
                              _PD, _PD, _PD, _PD, _PD, _PD, _PD]),
-    "amdat_batch_job_info": (_I, [_P, _PI, _PI, _PI, _PI]),
+    "amdat_batch_job_info": (_I, [_P, _PI, _PI, _PI, _PI, _PI]),
 }
 
 RUN_GRAPH = 1
@@ -600,6 +600,6 @@ class Batch(_Handle):
         return out
 
     def job_info(self):
-        rs, rf, ni, g = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        check(lib().amdat_batch_job_info(self.h, C.byref(rs), C.byref(rf), C.byref(ni), C.byref(g)))
-        return dict(RS=rs.value, RF=rf.value, n_items=ni.value, graph=bool(g.value))
+        rs, rf, ps, ni, g = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().amdat_batch_job_info(self.h, C.byref(rs), C.byref(rf), C.byref(ps), C.byref(ni), C.byref(g)))
+        return dict(RS=rs.value, RF=rf.value, paired=bool(ps.value), n_items=ni.value, graph=bool(g.value))

@@ -2,6 +2,7 @@
 // device-resident trajectories, shell tables, pair tables and densities,
 // and the entry points that launch the kernels of kernels.hip.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -176,6 +177,7 @@ struct JobParams {
 struct JobCache {
     JobParams prm;
     int RS = 0, RF = 0, MM = 0;
+    int PS = 0;  // the S(k) plan is paired (RS = 8)
     int n = 0, ntg = 0, nsh_sk = 0, G = 0, max_nsh = 0;
     std::vector<int> out_off;              // Fs offsets per trajectory
     std::vector<double> sk_count, fs_count, msd_count;
@@ -204,7 +206,7 @@ struct amdat_batch {
     std::vector<const amdat_list *> lists;
     DevBuf<int> counter;  // work-queue counter of the persistent launches
     std::unique_ptr<JobCache> job;
-    std::vector<std::pair<JobParams, std::pair<int, int>>> tuned;  // autotuned (RS, RF) per job
+    std::vector<std::pair<JobParams, std::array<int, 3>>> tuned;  // autotuned (RS, RF, PS) per job
 };
 
 namespace {
@@ -247,13 +249,13 @@ double layout_cost(const PlanLayout &L, double c)
     return (double)L.n_chunks * 12000.0 + (double)L.paired_slots * (c - 2.0) + (double)L.unpaired_slots * c;
 }
 
-// R_force > 0 fixes the vectors per thread and an unpaired layout (batched
-// launches need one R and run unpaired kernels).  Otherwise the plan is the
+// R_force > 0 fixes the vectors per thread and the layout mode to
+// mode_force (batched launches need one R).  Otherwise the plan is the
 // cheapest by layout_cost of R = 2, 4, 8 unpaired and, when allow_pair
 // (callers whose kernels use pairs), R = 4, 8 paired (mode 1).
 // c: VALU ops per update of the kernel the plan serves (see choose_R).
 int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned **out, int R_force = 0,
-               double c = 9.0, bool allow_pair = true)
+               double c = 9.0, bool allow_pair = true, int mode_force = 0)
 {
     const int v0 = sh->off[first], v1 = sh->off[last + 1];
     const int K = v1 - v0;
@@ -262,7 +264,8 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
         for (int c = 0; c < 3; c++) M = std::max(M, std::abs(sh->nvec[3 * v + c]));
     if (M > 127) return set_error(AMDAT_ERANGE, "k-vector component beyond +-127");
     int R = R_force > 0 ? R_force : choose_R(K, M, c);
-    int mode = 0;  // plan_layout mode: 0 unpaired, 1 paired with padding, 2 paired and unpaired waves
+    int mode = R_force > 0 ? mode_force : 0;  // plan_layout mode: 0 unpaired, 1 paired with padding,
+                                              // 2 paired and unpaired waves
     PlanLayout lay;
     bool have_layout = false;
     if (R_force <= 0 && sh->hint_R > 0) {
@@ -1664,9 +1667,9 @@ hipError_t enqueue_job_sequence(JobCache &jc, hipStream_t st)
 }
 
 // Plan a job for the trajectories with use[b] != 0 (others get no items)
-// with R = RS for S(k) items and RF for Fs items.
+// with R = RS for S(k) items (paired plan when PS) and RF for Fs items.
 int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std::vector<char> &use, int RS, int RF,
-             JobCache &jc)
+             int PS, JobCache &jc)
 {
     const int n = batch->n;
     hipStream_t st = stream_of(ctx);
@@ -1674,6 +1677,7 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
     jc.n = n;
     jc.RS = RS;
     jc.RF = RF;
+    jc.PS = PS;
     jc.ntg = prm.pairs ? prm.pairs->ntg : 0;
     std::vector<int> nl(n);
     for (int b = 0; b < n; b++) nl[b] = list_n(batch->trajs[b], batch->lists[b]);
@@ -1685,7 +1689,7 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
     if (prm.sk_sh) {
         amdat_shells *sh = const_cast<amdat_shells *>(prm.sk_sh);
         KPlanOwned *plan = nullptr;
-        int rc = build_plan(ctx, sh, prm.sk_first, prm.sk_last, &plan, RS);
+        int rc = build_plan(ctx, sh, prm.sk_first, prm.sk_last, &plan, RS, 9.0, true, PS);
         if (rc) return rc;
         skp = plan->kp;
         M = std::max(M, shell_range_M(sh, prm.sk_first, prm.sk_last));
@@ -1834,7 +1838,7 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
     jc.job.items = jc.items.p;
     jc.job.n_items = (int)items.size();
     jc.job.counter = batch->counter.p;
-    HIP_TRY(prepare_job(RS, RF, jc.MM, std::max(1, jc.job.n_items), &jc.L));
+    HIP_TRY(prepare_job(RS, RF, jc.MM, PS, std::max(1, jc.job.n_items), &jc.L));
     return AMDAT_OK;
 }
 
@@ -1857,10 +1861,11 @@ int time_job(amdat_ctx *ctx, JobCache &jc, float *ms)
     return AMDAT_OK;
 }
 
-// Autotuner: time R in {2, 4, 8} for the S(k) items and, separately, for
-// the Fs items, on a sample of about an eighth of the trajectories (every
-// k-th by size), and keep the fastest of each.
-int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, int *RS, int *RF)
+// Autotuner: time R in {2, 4, 8} and R = 8 paired for the S(k) items and,
+// separately, R in {2, 4, 8} for the Fs items, on a sample of about an
+// eighth of the trajectories (every k-th by size), and keep the fastest of
+// each.
+int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, int *RS, int *RF, int *PS)
 {
     const int n = batch->n;
     std::vector<int> idx(n);
@@ -1871,23 +1876,31 @@ int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, int *
     std::vector<char> use(n, 0);
     const int step = std::max(1, n / 8);
     for (int i = 0; i < n; i += step) use[idx[i]] = 1;
-    static const int Rs[] = {2, 4, 8};
+    static const int cand[4][2] = {{2, 0}, {4, 0}, {8, 0}, {8, 1}};
     for (int part = 0; part < 2; part++) {
         JobParams p = prm;
         p.msd = false;
         if (part == 0) { p.fs_sh = nullptr; if (!p.sk_sh) continue; }
         else { p.sk_sh = nullptr; if (!p.fs_sh) continue; }
         float best = 1e30f;
-        for (int R : Rs) {
+        for (const auto &cd : cand) {
+            if (part == 1 && cd[1]) continue;
+            const int R = cd[0];
             JobCache jc;
-            int rc = plan_job(ctx, batch, p, use, part == 0 ? R : *RS, part == 0 ? *RF : R, jc);
+            int rc = plan_job(ctx, batch, p, use, part == 0 ? R : *RS, part == 0 ? *RF : R, part == 0 ? cd[1] : *PS,
+                              jc);
             if (rc) return rc;
             float ms = 0.0f;
             rc = time_job(ctx, jc, &ms);
             if (rc) return rc;
             if (ms < best) {
                 best = ms;
-                (part == 0 ? *RS : *RF) = R;
+                if (part == 0) {
+                    *RS = R;
+                    *PS = cd[1];
+                } else {
+                    *RF = R;
+                }
             }
         }
     }
@@ -1961,10 +1974,16 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
     if (!batch->job || !(batch->job->prm == prm) || (use_graph && !batch->job->exec)) {
         batch->job.reset();
         // vectors per thread: autotuned, or from the cost model
-        int RS = 0, RF = 0;
+        int RS = 0, RF = 0, PS = 0;
         if (sk_shells) {
             const int K = sk_shells->off[sk_last + 1] - sk_shells->off[sk_first];
-            RS = choose_R(std::max(K, 1), shell_range_M(sk_shells, sk_first, sk_last));
+            const int M = shell_range_M(sk_shells, sk_first, sk_last);
+            RS = choose_R(std::max(K, 1), M);
+            if (RS == 8 && K > 0 && plan_table_width(M) > 0) {
+                const int *nv = sk_shells->nvec.data() + 3 * (size_t)sk_shells->off[sk_first];
+                const int MMs = plan_table_width(M);
+                PS = layout_cost(plan_layout(nv, K, 8, 1, MMs), 9.0) < layout_cost(plan_layout(nv, K, 8, 0, MMs), 9.0);
+            }
         }
         if (fs_shells) RF = choose_R(std::max(Kfs, 1), Mfs, 6.0);
         if (!RS) RS = RF ? RF : 2;
@@ -1972,16 +1991,16 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
         if (flags & AMDAT_RUN_TUNE) {
             bool found = false;
             for (auto &t : batch->tuned)
-                if (t.first == prm) { RS = t.second.first; RF = t.second.second; found = true; }
+                if (t.first == prm) { RS = t.second[0]; RF = t.second[1]; PS = t.second[2]; found = true; }
             if (!found) {
-                int rc = autotune_job(ctx, batch, prm, &RS, &RF);
+                int rc = autotune_job(ctx, batch, prm, &RS, &RF, &PS);
                 if (rc) return rc;
-                batch->tuned.push_back({prm, {RS, RF}});
+                batch->tuned.push_back({prm, {RS, RF, PS}});
             }
         }
         auto jc = std::make_unique<JobCache>();
         std::vector<char> use(n, 1);
-        int rc = plan_job(ctx, batch, prm, use, RS, RF, *jc);
+        int rc = plan_job(ctx, batch, prm, use, RS, RF, PS, *jc);
         if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(user));
         if (use_graph) {
@@ -2034,11 +2053,12 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
     return AMDAT_OK;
 }
 
-int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *n_items, int *graph)
+int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *paired, int *n_items, int *graph)
 {
     if (!batch || !batch->job) return set_error(AMDAT_EINVAL, "amdat_batch_job_info: no job has run");
     if (RS) *RS = batch->job->RS;
     if (RF) *RF = batch->job->RF;
+    if (paired) *paired = batch->job->PS;
     if (n_items) *n_items = batch->job->job.n_items;
     if (graph) *graph = batch->job->exec != nullptr;
     return AMDAT_OK;

@@ -14,7 +14,7 @@ namespace {
 // row [g][v] of t.out (FP64; each slot is owned by one thread of one item,
 // so the sum order is fixed).  All trajectories share one k-plan: the same
 // integer vectors, k = 2 pi n / l_b with each trajectory's own l_b.
-template <int R, int MM>
+template <int R, int MM, bool PAIR = false>
 __device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, const int *__restrict__ frames,
                                         int n_frames, int group_size, const KPlan &kp, float2 *tab, float2 *anc,
                                         uint32_t *sU, int *sValid)
@@ -24,7 +24,7 @@ __device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, co
     double *row = t.out + (size_t)g * kp.K;
     for (int f = f_begin; f < f_end; ++f) {
         float hr[R], lr[R], hi[R], li[R];
-        rho_block<R, MM>(t.x, t.y, t.z, t.list, (size_t)frames[f] * (size_t)t.N, 0, t.n_list, t.inv_ell,
+        rho_block<R, MM, PAIR>(t.x, t.y, t.z, t.list, (size_t)frames[f] * (size_t)t.N, 0, t.n_list, t.inv_ell,
                          kp.kpack + (size_t)chunk * kp.KB, kp.range[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -159,7 +159,8 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_batch_kernel(const BatchTra
 
 // One analysis job over a batch in a single persistent launch: S(k), Fs and
 // MSD/NGP items (item.w = 0, 1, 2) drawn from one queue, costliest first.
-template <int RS, int RF, int MM>
+// PS: the S(k) plan is paired.
+template <int RS, int RF, int MM, bool PS>
 __global__ __launch_bounds__(kPhaseBlock, 4) void job_kernel(BatchJob j)
 {
     AMDAT_PHASE_SHARED(MM);
@@ -170,8 +171,8 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void job_kernel(BatchJob j)
         if (q >= j.n_items) break;
         const int4 it = j.items[q];
         if (it.w == 0)
-            sk_item<RS, MM>(j.sk_trajs[it.x], it.y, it.z, j.frames, j.n_frames, j.group_size, j.sk, tab, anc, sU,
-                            sValid);
+            sk_item<RS, MM, PS>(j.sk_trajs[it.x], it.y, it.z, j.frames, j.n_frames, j.group_size, j.sk, tab, anc,
+                                sU, sValid);
         else if (it.w == 1)
             fs_item<RF, MM>(j.fs_trajs[it.x], it.y, it.z, j.t0, j.t1, j.group_off, j.order, j.fs, tab, anc, sU,
                             sValid);
@@ -350,25 +351,25 @@ hipError_t launch_fs_batch_t(const BatchTraj *trajs, const int4 *items, int n_it
     default: return hipErrorInvalidValue;                      \
     }
 
-template <int RS, int RF>
+template <int RS, int RF, bool PS>
 const void *job_fn_mm(int MM)
 {
     switch (MM) {
-    case 24: return (const void *)job_kernel<RS, RF, 24>;
-    case 50: return (const void *)job_kernel<RS, RF, 50>;
-    case 100: return (const void *)job_kernel<RS, RF, 100>;
-    case 127: return (const void *)job_kernel<RS, RF, 127>;
+    case 24: return (const void *)job_kernel<RS, RF, 24, PS>;
+    case 50: return (const void *)job_kernel<RS, RF, 50, PS>;
+    case 100: return (const void *)job_kernel<RS, RF, 100, PS>;
+    case 127: return (const void *)job_kernel<RS, RF, 127, PS>;
     default: return nullptr;
     }
 }
 
-template <int RS>
+template <int RS, bool PS>
 const void *job_fn_rf(int RF, int MM)
 {
     switch (RF) {
-    case 2: return job_fn_mm<RS, 2>(MM);
-    case 4: return job_fn_mm<RS, 4>(MM);
-    case 8: return job_fn_mm<RS, 8>(MM);
+    case 2: return job_fn_mm<RS, 2, PS>(MM);
+    case 4: return job_fn_mm<RS, 4, PS>(MM);
+    case 8: return job_fn_mm<RS, 8, PS>(MM);
     default: return nullptr;
     }
 }
@@ -386,13 +387,13 @@ size_t tab_bytes_of(int MM)
 
 }  // namespace
 
-hipError_t prepare_job(int RS, int RF, int MM, int n_items, JobLaunch *out)
+hipError_t prepare_job(int RS, int RF, int MM, int PS, int n_items, JobLaunch *out)
 {
     const void *fn = nullptr;
     switch (RS) {
-    case 2: fn = job_fn_rf<2>(RF, MM); break;
-    case 4: fn = job_fn_rf<4>(RF, MM); break;
-    case 8: fn = job_fn_rf<8>(RF, MM); break;
+    case 2: fn = PS ? nullptr : job_fn_rf<2, false>(RF, MM); break;
+    case 4: fn = PS ? nullptr : job_fn_rf<4, false>(RF, MM); break;
+    case 8: fn = PS ? job_fn_rf<8, true>(RF, MM) : job_fn_rf<8, false>(RF, MM); break;
     default: break;
     }
     if (!fn) return hipErrorInvalidValue;

@@ -134,7 +134,8 @@ hipError_t launch_fs_batch(const BatchTraj *trajs, const int4 *items, int n_item
 // Job kernel for R = RS (S(k) items), RF (Fs items), table width MM:
 // function attributes and the persistent grid for n_items.  Call outside
 // stream capture; enqueue_job (counter reset + launch) may be captured.
-hipError_t prepare_job(int RS, int RF, int MM, int n_items, JobLaunch *out);
+// PS = 1: the S(k) plan is paired (RS = 8 only).
+hipError_t prepare_job(int RS, int RF, int MM, int PS, int n_items, JobLaunch *out);
 hipError_t enqueue_job(const JobLaunch &L, const BatchJob &j, hipStream_t st);
 // MSD items (trajectory b, pair p): partial[2*(b*n_pairs + p)] = sum r^2, [+1] = sum r^4.
 hipError_t launch_msd_batch(const BatchTraj *trajs, const int2 *items, int n_items, const int *t0, const int *t1,

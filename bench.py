@@ -160,6 +160,30 @@ def bench_batch(args):
     value = updates * args.steps / wall
     tf = (8.0 * U_S + 4.0 * U_Fs) / (ms * 1e-3) / 1e12
     S = out["S"]
+    fp64 = None
+    if not args.no_fp64:
+        # The same job on the plain-FP64 path, one trajectory at a time (FP64
+        # phase, FP64 sincos, FP64 sums): its wall time and the batched
+        # outputs' error against it.
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        S64 = np.zeros_like(S)
+        Fs64 = []
+        for i, m in enumerate(members):
+            r = _abi.Rho(ctx, m, None, sh, 0, sk_last, np.float32(Ls[i]), precision=_abi.PREC_FP64)
+            S64[i] = _abi.sk(ctx, r)
+            r.close()
+            f, _ = _abi.fs(ctx, m, None, sh, int(fs_first[i]), int(fs_last[i]), np.float32(Ls[i]), P,
+                           precision=_abi.PREC_FP64)
+            Fs64.append(f)
+            _abi.msd_ngp(ctx, m, None, P)
+        torch.cuda.synchronize()
+        ms64 = 1e3 * (time.perf_counter() - t)
+        err_S = float(np.max(np.abs(S - S64) / np.abs(S64)))
+        err_Fs = float(max(np.max(np.abs(a - b)) for a, b in zip(out["Fs"], Fs64)))
+        fp64 = {"ms_per_step": ms64, "speedup": ms64 / ms, "rel_err": {"S": err_S, "Fs": err_Fs}}
+        log("plain FP64, one trajectory at a time: %.1f ms (%.1fx); errors S %.2e Fs %.2e"
+            % (ms64, ms64 / ms, err_S, err_Fs))
     result = {
         "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -176,6 +200,10 @@ def bench_batch(args):
         "job": info,
         "check": {"S_mean_high_k": float(np.mean(S[:, -10:]))},
     }
+    if fp64 is not None:
+        result["fp64_ms_per_step"] = fp64["ms_per_step"]
+        result["speedup_vs_fp64"] = fp64["speedup"]
+        result["rel_err_vs_fp64"] = fp64["rel_err"]
     print(json.dumps(result), flush=True)
 
 

@@ -404,9 +404,10 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
                     const amdat_pairs *pairs, int want_msd, int flags, double *S, double *S_count,
                     double *Fs, double *Fs_count, double *s2, double *s4, double *msd_count);
 
-/* The last job's vectors per thread for S(k) (RS) and Fs (RF) items, its
- * number of work items, and whether it runs as a captured graph. */
-int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *n_items, int *graph);
+/* The last job's vectors per thread for S(k) (RS) and Fs (RF) items,
+ * whether its S(k) plan pairs slots sharing ny or nz, its number of work
+ * items, and whether it runs as a captured graph. */
+int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *paired, int *n_items, int *graph);
 
 #ifdef __cplusplus
 }

@@ -96,7 +96,9 @@ def test_batch_sk_frames_and_single_path(ctx, members, sk_shells):
         assert rel(S[i], S64, np.abs(S64)) < TOL
         r = _abi.Rho(ctx, devs[i], lists[i], sh, 2, 9, np.float32(t.L), frames=frames)
         S1 = _abi.sk(ctx, r)
-        assert rel(S[i], S1, np.abs(S1)) < 1e-9
+        # the single-trajectory path may pair slots (another FP32 product
+        # order): agreement to FP32 rounding, not bit for bit
+        assert rel(S[i], S1, np.abs(S1)) < 1e-7
 
 
 @pytest.mark.parametrize("fb", [0, 1])
@@ -129,7 +131,8 @@ def test_batch_fs(ctx, members, qvec, fb):
     # one member through the single-trajectory entry point
     i = 3
     F1, _ = _abi.fs(ctx, devs[i], lists[i], sh, first[i], last[i], np.float32(trs[i].L), P)
-    assert rel(Fs[i], F1, 1.0) < 1e-9
+    # the single-trajectory path may pair slots: agreement to FP32 rounding
+    assert rel(Fs[i], F1, 1.0) < 1e-7
 
 
 def test_batch_msd_ngp(ctx, members):
@@ -258,3 +261,20 @@ def test_batch_job_parts_and_errors(ctx, members, sk_shells, job_args):
         b.run(a["ell"], msd=True)                         # MSD needs pairs
     with pytest.raises(_abi.AmdatError):
         b.run([-1.0] * len(trs), pairs=a["P"], msd=True)  # bad length scale
+
+
+def test_batch_job_paired_sk_plan(ctx, members, qvec):
+    """S(k) on 60 shells (bins 2..61, 14,543 vectors): the cost model pairs
+    the S(k) plan (R = 8, slots sharing ny or nz); results against the
+    oracle, including the member with an atom subset."""
+    trs, devs, lists, atoms, b = members
+    off, n = ref_shells(qvec, 2, 61)
+    sh = _abi.Shells(ctx, off, n)
+    o = b.run([t.L for t in trs], sk=(sh, 0, len(off) - 2), graph=True)
+    info = b.job_info()
+    print("paired S(k) job", info)
+    assert info["RS"] == 8 and info["paired"]
+    for i in (0, 5, 17):
+        t = trs[i]
+        S64 = _sk_oracle(t, n, off, np.arange(t.T), atoms[i])
+        assert rel(o["S"][i], S64, np.abs(S64)) < TOL
