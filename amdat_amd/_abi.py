@@ -73,6 +73,8 @@ SIGNATURES = {
     "amdat_fs": (_I, [_P, _P, _P, _P, _I, _I, _F, _P, _I, _PD, _PD]),
     "amdat_msd_ngp": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
     "amdat_displacements": (_I, [_P, _P, _P, _P, _I, _F, _PD, _PD, C.POINTER(C.c_ulonglong), _PD]),
+    "amdat_displacement_dist": (_I, [_P, _P, _P, _P, _I, _F, C.POINTER(C.c_ulonglong), _PD, _PD, _PD]),
+    "amdat_ts_gap_pairs": (_I, [_I, _I, _I, _I, _I, _PI, _PI, _PI, _I, _PI]),
     "amdat_sk_loss_grad": (_I, [_P, _I, _P, _P, _P, _P, _I, _I, _F, _PD, _PD, _PD, _PD, _P, _P, _P]),
     "amdat_batch_create": (_I, [_P, _I, _PP, _PP, _PP]),
     "amdat_batch_free": (_I, [_P, _P]),
@@ -603,3 +605,23 @@ class Batch(_Handle):
         rs, rf, ps, ni, g = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(lib().amdat_batch_job_info(self.h, C.byref(rs), C.byref(rf), C.byref(ps), C.byref(ni), C.byref(g)))
         return dict(RS=rs.value, RF=rf.value, paired=bool(ps.value), n_items=ni.value, graph=bool(g.value))
+
+
+def ts_gap_pairs(nexp, nes, frt, limit, time_index):
+    """Pairs (tg, t0, t1) of one time gap, first frame of each block only."""
+    n = C.c_int()
+    check(lib().amdat_ts_gap_pairs(nexp, nes, frt, limit, time_index, None, None, None, 0, C.byref(n)))
+    tg, t0, t1 = (np.zeros(max(n.value, 1), np.int32) for _ in range(3))
+    check(lib().amdat_ts_gap_pairs(nexp, nes, frt, limit, time_index, ip(tg), ip(t0), ip(t1), n.value, C.byref(n)))
+    return tg[:n.value], t0[:n.value], t1[:n.value]
+
+
+def displacement_dist(ctx, traj, alist, pairs, n_bins, max_value):
+    """Squared-displacement distribution: (counts [n_bins+1] uint64,
+    sum |dr|^2, sum |dr|^4, atom-pair count)."""
+    counts = np.zeros(n_bins + 1, np.uint64)
+    s2, s4, c = C.c_double(), C.c_double(), C.c_double()
+    check(lib().amdat_displacement_dist(ctx.h, traj.h, alist.h if alist is not None else None, pairs.h, n_bins,
+                                        float(max_value), counts.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+                                        C.byref(s2), C.byref(s4), C.byref(c)))
+    return counts, s2.value, s4.value, c.value

@@ -243,7 +243,10 @@ int shell_range_M(const amdat_shells *sh, int first, int last)
 
 // Modelled cost of a layout per atom: every slot of a wave that runs the
 // sums costs c VALU ops per lane, c - 2 in paired waves (q = e_x e_y formed
-// once per pair), and every chunk the fixed part of choose_R.
+// once per pair), counted in whole SIMD rounds of 4 waves per chunk, and
+// every chunk the fixed part of choose_R.  (Scale-form Fs, 1,500 vectors in
+// one chunk: R = 8 paired, 5 active waves, ran 11.4 s against 10.1 s for
+// R = 4 unpaired with 7.)
 double layout_cost(const PlanLayout &L, double c)
 {
     return (double)L.n_chunks * 12000.0 + (double)L.paired_slots * (c - 2.0) + (double)L.unpaired_slots * c;
@@ -806,34 +809,14 @@ int amdat_list_select_displacement(amdat_ctx *ctx, const amdat_traj *tr, const a
     const long T_scheme = (long)nes * nexp + 1;
     if (T_scheme > tr->dev.T)
         return set_error(AMDAT_ERANGE, "amdat_list_select_displacement: trajectory shorter than the time scheme");
-    // Pairs of System::displacement_list(analysis, timegap, false)
-    // (src/system.cpp:4349-4398), as Fast_Particles::analyze calls it
-    // (src/fast_particles.cpp:117-122).
-    const int ntg = nes + nexp - 1 + (1 - f);
-    const int g = time_index;
-    std::vector<int> p0, p1;
-    if (g >= 0 && g < nes) {
-        int count = 0;
-        for (int b = 0; b < nexp; b++) {
-            p0.push_back(nes * b + f);
-            p1.push_back(nes * b + f + g);
-            if (++count == limit) break;
-        }
-    } else if (g >= nes && g < ntg - 1 + f) {
-        const int btg = g - nes + 1;
-        int count = 0;
-        for (int b = 0; b < nexp - btg; b++) {
-            p0.push_back(nes * b + f);
-            p1.push_back(nes * b + f + nes * btg);
-            if (++count == limit) break;
-        }
-    } else if (g == ntg - 1 + f) {
-        p0.push_back(0);
-        p1.push_back((int)T_scheme - 1);
-    } else {
-        return set_error(AMDAT_ERANGE, "Requested timegap out of range.");
-    }
-    const int nq = (int)p0.size();
+    // Pairs of System::displacement_list(analysis, timegap, false), as
+    // Fast_Particles::analyze calls it (src/fast_particles.cpp:117-122).
+    int nq = 0;
+    int rc = amdat_ts_gap_pairs(nexp, nes, f, limit, time_index, nullptr, nullptr, nullptr, 0, &nq);
+    if (rc) return rc;
+    std::vector<int> gtg(nq), p0(nq), p1(nq);
+    rc = amdat_ts_gap_pairs(nexp, nes, f, limit, time_index, gtg.data(), p0.data(), p1.data(), nq, &nq);
+    if (rc) return rc;
     // Block of each pair (src/fast_particles.cpp:133) and its input atoms
     // (the input list at the list time of t0).
     std::vector<int> blk(nq);
@@ -1228,15 +1211,16 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
     return AMDAT_OK;
 }
 
-int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
-                        int n_bins, float max_range, double *sum_d, double *sum_d2, unsigned long long *hist,
-                        double *count)
+}  // extern "C"
+
+namespace {
+
+// Displacement sums per time gap and, with n_bins > 0, a histogram per time
+// gap (hist_mode 0: |dr| below n_bins; 1: |dr|^2 with overflow bin).
+int displacements_impl(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                       int n_bins, int hist_mode, float bin_size, double *sum_d, double *sum_d2,
+                       unsigned long long *hist, double *count)
 {
-    clear_error();
-    if (!ctx || !tr || !pairs || !sum_d || !sum_d2 || !count)
-        return set_error(AMDAT_EINVAL, "amdat_displacements: null argument");
-    if (n_bins < 0 || n_bins > 16384 || (n_bins > 0 && (!hist || !(max_range > 0.0f))))
-        return set_error(AMDAT_EINVAL, "amdat_displacements: bins must be 0..16384 with a histogram and max_range > 0");
     int rc = check_pairs_frames(pairs, tr->dev.T);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
@@ -1250,7 +1234,6 @@ int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
     if (pairs->n == 0 || pl.max_n == 0) return AMDAT_OK;
     hipStream_t st = stream_of(ctx);
     const int nb = std::max(1, std::min(64, (pl.max_n + kBlock * 4 - 1) / (kBlock * 4)));
-    const float bin_size = n_bins > 0 ? max_range / (float)n_bins : 1.0f;  // src/space_time_correlation_function.cpp
     DevBuf<double> part, out;
     DevBuf<unsigned long long> dh;
     HIP_TRY(part.alloc((size_t)pairs->n * nb * 6));
@@ -1265,7 +1248,7 @@ int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
         for (int p0 = gr.row; p0 < gr.row + gr.len; p0 += pb) {
             const int np = std::min(pb, gr.row + gr.len - p0);
             HIP_TRY(launch_disp(tr->dev, gr.seg.p, gr.seg.n, pl.t0 + p0, pl.t1 + p0, pl.tg + p0, np, nb, n_bins,
-                                bin_size, part.p + (size_t)p0 * nb * 6, dh.p, st));
+                                hist_mode, bin_size, part.p + (size_t)p0 * nb * 6, dh.p, st));
         }
     }
     HIP_TRY(out.alloc((size_t)ntg * 6));
@@ -1280,6 +1263,53 @@ int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
             sum_d[3 * g + c] = h[6 * (size_t)g + c];
             sum_d2[3 * g + c] = h[6 * (size_t)g + 3 + c];
         }
+    return AMDAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                        int n_bins, float max_range, double *sum_d, double *sum_d2, unsigned long long *hist,
+                        double *count)
+{
+    clear_error();
+    if (!ctx || !tr || !pairs || !sum_d || !sum_d2 || !count)
+        return set_error(AMDAT_EINVAL, "amdat_displacements: null argument");
+    if (n_bins < 0 || n_bins > 16384 || (n_bins > 0 && (!hist || !(max_range > 0.0f))))
+        return set_error(AMDAT_EINVAL, "amdat_displacements: bins must be 0..16384 with a histogram and max_range > 0");
+    // bin size of Space_Time_Correlation_Function (src/van_hove_self.cpp:56)
+    const float bin_size = n_bins > 0 ? max_range / (float)n_bins : 1.0f;
+    return displacements_impl(ctx, tr, list, pairs, n_bins, 0, bin_size, sum_d, sum_d2, hist, count);
+}
+
+int amdat_displacement_dist(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                            int n_bins, float max_value, unsigned long long *counts, double *sum_d2,
+                            double *sum_d4, double *count)
+{
+    clear_error();
+    if (!ctx || !tr || !pairs || !counts || !sum_d2 || !sum_d4 || !count)
+        return set_error(AMDAT_EINVAL, "amdat_displacement_dist: null argument");
+    if (n_bins < 1 || n_bins > 16383 || !(max_value > 0.0f))
+        return set_error(AMDAT_EINVAL, "amdat_displacement_dist: bins must be 1..16383 and max_value > 0");
+    const int ntg = pairs->ntg, nb = n_bins + 1;
+    // binsize = maxvalue / bins (src/debyewaller_dist.cpp:20)
+    const float bin_size = max_value / (float)n_bins;
+    std::vector<double> sd(3 * (size_t)ntg), sd2(3 * (size_t)ntg), cnt(ntg), s2(ntg), s4(ntg), c2(ntg);
+    std::vector<unsigned long long> h((size_t)ntg * nb);
+    int rc = displacements_impl(ctx, tr, list, pairs, nb, 1, bin_size, sd.data(), sd2.data(), h.data(), cnt.data());
+    if (rc) return rc;
+    rc = amdat_msd_ngp(ctx, tr, list, pairs, s2.data(), s4.data(), c2.data());
+    if (rc) return rc;
+    std::fill(counts, counts + nb, 0ull);
+    *sum_d2 = *sum_d4 = *count = 0.0;
+    for (int g = 0; g < ntg; g++) {
+        for (int b = 0; b < nb; b++) counts[b] += h[(size_t)g * nb + b];
+        *sum_d2 += s2[g];
+        *sum_d4 += s4[g];
+        *count += cnt[g];
+    }
     return AMDAT_OK;
 }
 

@@ -44,7 +44,8 @@ struct PlanLayout {
                                  // units with vectors | first unpaired unit << 16
     long conflicts = 0;          // extra LDS passes summed over (chunk, slot, half-wave, axis)
     bool pair = false;           // slots (2p, 2p+1) of a thread share ny, or nz (kPackSwap)
-    long paired_slots = 0, unpaired_slots = 0;  // slots of waves that run the sums, by kind
+    long paired_slots = 0, unpaired_slots = 0;  // slots of waves that run the sums, by kind, each chunk's
+                                                // rounded up to whole SIMD rounds (4 waves)
 };
 // mode 0: unpaired; 1: paired, vectors without a partner padded; 2: paired,
 // vectors without a partner in unpaired waves.  MM: table half-width of the

@@ -111,6 +111,47 @@ int amdat_ts_pairs(int nexp, int nes, int frt, int fullblock, int limit, int *tg
     return AMDAT_OK;
 }
 
+// System::displacement_list(analysis, timegap, false), src/system.cpp:
+// 4349-4398: the pairs of one time gap with the first time of each block
+// only (in-block gaps: every block; cross-block gaps: blocks that reach;
+// the last gap: frames 0 and n_timesteps - 1, which with frt = 1 is time
+// gap n_timegaps, one past the full enumeration).  limit > 0 keeps the
+// first limit pairs.
+int amdat_ts_gap_pairs(int nexp, int nes, int frt, int limit, int time_index, int *tg, int *t0, int *t1, int cap,
+                       int *n_pairs)
+{
+    if (nexp < 1 || nes < 1 || limit < 0) return set_error(AMDAT_EINVAL, "amdat_ts_gap_pairs: bad time scheme");
+    if (cap > 0 && (!tg || !t0 || !t1)) return set_error(AMDAT_EINVAL, "amdat_ts_gap_pairs: null output");
+    const int f = frt ? 1 : 0;
+    const int ntg = nes + nexp - 1 + (1 - f);
+    const int g = time_index;
+    int n = 0;
+    auto emit = [&](int gg, int a, int b) {
+        if (n < cap) { tg[n] = gg; t0[n] = a; t1[n] = b; }
+        ++n;
+    };
+    if (g >= 0 && g < nes) {
+        int count = 0;
+        for (int b = 0; b < nexp; b++) {
+            emit(g, nes * b + f, nes * b + f + g);
+            if (++count == limit) break;
+        }
+    } else if (g >= nes && g < ntg - 1 + f) {
+        const int btg = g - nes + 1;
+        int count = 0;
+        for (int b = 0; b < nexp - btg; b++) {
+            emit(g, nes * b + f, nes * b + f + nes * btg);
+            if (++count == limit) break;
+        }
+    } else if (g == ntg - 1 + f) {
+        emit(ntg - 1, 0, nes * nexp);
+    } else {
+        return set_error(AMDAT_ERANGE, "Requested timegap out of range.");
+    }
+    if (n_pairs) *n_pairs = n;
+    return AMDAT_OK;
+}
+
 // System::timegap_weighting, src/system.cpp:4599-4628 (cross-block entries
 // carry the reference's +1).
 int amdat_ts_weighting(int nexp, int nes, int frt, int fullblock, int limit, int *w, int cap)
@@ -473,8 +514,15 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
         const int n_act = std::min(512, n_units - c * 512);
         const int first_unpaired = std::max(0, std::min(512, n_paired - c * 512));
         L.range[4 * c + 3] = n_act | (first_unpaired << 16);
-        for (int w = 0; w * 64 < n_act; w++)
-            (w * 64 < first_unpaired ? L.paired_slots : L.unpaired_slots) += 64L * R;
+        // Work of the waves that run the sums, in slots.  A block's 8 waves
+        // share 4 SIMDs two by two, so a chunk with a active waves takes as
+        // long as ceil(a / 4) * 4 of them.
+        const int a = (n_act + 63) / 64, ap = std::min(a, (first_unpaired + 63) / 64);
+        if (a > 0) {
+            const double scale = (double)(((a + 3) / 4) * 4) / a;
+            L.paired_slots += (long)(ap * 64.0 * R * scale);
+            L.unpaired_slots += (long)((a - ap) * 64.0 * R * scale);
+        }
         // conflict estimate of the chunk's reads
         for (int r = 0; r < R; r++)
             for (int h = 0; h < 512; h += kLanesPerPass) {

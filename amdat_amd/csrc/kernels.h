@@ -99,10 +99,12 @@ hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *grou
 hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int *t0,
                       const int *t1, int n_pairs, int n_blocks, double *partial, hipStream_t st);
 // Per (pair, atom block): FP64 sums of d_c and d_c^2 into partial[(p*n_blocks
-// + block)*6 + k]; with n_bins > 0 the reference-form |dr| counts are added to
-// hist[tg[p]*n_bins + bin] (bin = int(|dr| / bin_size)).
+// + block)*6 + k]; with n_bins > 0 the reference-form counts are added to
+// hist[tg[p]*n_bins + bin]: hist_mode 0, bin = int(|dr| / bin_size) below
+// n_bins; hist_mode 1, bin = int(|dr|^2 / bin_size), the last bin taking
+// the overflow.
 hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, const int *tg,
-                       int n_pairs, int n_blocks, int n_bins, float bin_size, double *partial,
+                       int n_pairs, int n_blocks, int n_bins, int hist_mode, float bin_size, double *partial,
                        unsigned long long *hist, hipStream_t st);
 // Per group: the six sums of its pairs' partials (pairs in order[]).
 // Displacement selection (Fast/Slow/Average_Particles): with tile_base ==

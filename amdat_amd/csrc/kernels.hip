@@ -239,17 +239,20 @@ __global__ __launch_bounds__(kBlock) void msd_kernel(
 // Displacement statistics per (pair, atom block): FP64 sums of d_c and
 // d_c^2 (c = x, y, z) from the FP64 rebuild d = (x1 - x0) + (i1 L1 - i0 L0),
 // into partial[(p * blocks + block) * 6 + k] (k = dx, dy, dz, dx^2, dy^2,
-// dz^2).  With n_bins > 0 also the self van Hove counts: |dr| as the
-// reference forms it (unwrapped FP32 u = float(x + i L), FP32 difference,
-// sqrtf((dx dx + dy dy) + dz dz)), bin = int(|dr| / bin_size), counted in
-// LDS and added to hist[TG[p] * n_bins + bin] (integer atomics: the counts
-// do not depend on the order).
+// dz^2).  With n_bins > 0 also a histogram of |dr| as the reference forms
+// it (unwrapped FP32 u = float(x + i L), FP32 difference,
+// sqrtf((dx dx + dy dy) + dz dz)), counted in LDS and added to
+// hist[TG[p] * n_bins + bin] (integer atomics: the counts do not depend on
+// the order).  hist_mode 0 (self van Hove): bin = int(|dr| / bin_size),
+// counted when below n_bins; hist_mode 1 (DebyeWaller_Dist,
+// Displacement_Distribution): bin = int(d2 / bin_size) of the FP32 square
+// d2 = |dr| |dr|, the last bin taking the overflow.
 __global__ __launch_bounds__(kBlock) void disp_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
     const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int *__restrict__ T0,
-    const int *__restrict__ T1, const int *__restrict__ TG, int n_bins, float bin_size, double *__restrict__ partial,
-    unsigned long long *__restrict__ hist)
+    const int *__restrict__ T1, const int *__restrict__ TG, int n_bins, int hist_mode, float bin_size,
+    double *__restrict__ partial, unsigned long long *__restrict__ hist)
 {
 #pragma clang fp contract(off)
     extern __shared__ unsigned int sh_hist[];
@@ -277,8 +280,14 @@ __global__ __launch_bounds__(kBlock) void disp_kernel(
         }
         if (n_bins > 0) {
             const float len = sqrtf((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
-            const int bin = (int)(len / bin_size);
-            if (bin >= 0 && bin < n_bins) atomicAdd(&sh_hist[bin], 1u);
+            if (hist_mode == 0) {
+                const int bin = (int)(len / bin_size);
+                if (bin >= 0 && bin < n_bins) atomicAdd(&sh_hist[bin], 1u);
+            } else {
+                const float d2 = len * len;
+                const int bin = min((int)(d2 / bin_size), n_bins - 1);
+                if (bin >= 0) atomicAdd(&sh_hist[bin], 1u);
+            }
         }
     }
 #pragma unroll
@@ -737,14 +746,14 @@ hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int 
 }
 
 hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, const int *tg,
-                       int n_pairs, int n_blocks, int n_bins, float bin_size, double *partial,
+                       int n_pairs, int n_blocks, int n_bins, int hist_mode, float bin_size, double *partial,
                        unsigned long long *hist, hipStream_t st)
 {
     if (n_pairs <= 0) return hipSuccess;
     (void)hipGetLastError();
     hipLaunchKernelGGL(disp_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), sizeof(unsigned int) * (size_t)n_bins, st,
-                       tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, tg, n_bins, bin_size,
-                       partial, hist);
+                       tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, tg, n_bins, hist_mode,
+                       bin_size, partial, hist);
     return hipGetLastError();
 }
 

@@ -133,6 +133,9 @@ class System {
     // src/system.cpp:3763-3784
     std::vector<float> displacement_times() const;
     // (tg, t0, t1) in the order of src/system.cpp:4090-4179, as a device table.
+    // Pairs of one time gap, first frame of each block only
+    // (src/system.cpp:4349-4398), as a device table.
+    amdat_pairs *gap_pairs(int time_index) const;
     amdat_pairs *displacement_pairs(bool fullblock, std::vector<int> *tg = nullptr,
                                     std::vector<int> *t0 = nullptr) const;
 
@@ -337,6 +340,36 @@ class Mean_Displacement {
     std::vector<Coordinate> md_;
 };
 
+// src/displacement_distribution.cpp: distribution of the squared
+// displacement at one time gap (the reference ignores `power` there too),
+// the last bin taking the overflow; mean and variance of the squared
+// displacement.  DebyeWaller_Dist (src/debyewaller_dist.cpp) computes the
+// same and writes another header.
+class Displacement_Distribution {
+  public:
+    Displacement_Distribution(System *sys, float power, int bins, float maxvalue, float t);
+    virtual ~Displacement_Distribution() = default;
+    void analyze(Trajectory_List *t_list);
+    virtual void write(const std::string &filename) const;
+    float show_mean() const { return mean_; }
+    float show_variance() const { return variance_; }
+    float show(int bin) const { return distribution_.at(bin); }
+
+  protected:
+    System *system_;
+    float power_;
+    int n_bins_, time_index_;
+    float max_value_, binsize_, time_;
+    float mean_ = 0, variance_ = 0;
+    std::vector<float> distribution_;
+};
+
+class DebyeWaller_Dist : public Displacement_Distribution {
+  public:
+    DebyeWaller_Dist(System *sys, int bins, float maxvalue, float t);
+    void write(const std::string &filename) const override;
+};
+
 // src/van_hove_self.cpp + src/space_time_correlation_function.cpp: self
 // part of the van Hove function, counts of |dr| per bin over atom pairs /
 // (weighting * shell volume).
@@ -383,6 +416,8 @@ class Control {
     void msd_2d(const std::vector<std::string> &args);
     void mean_displacement(const std::vector<std::string> &args);
     void vhs(const std::vector<std::string> &args);
+    void u2dist(const std::vector<std::string> &args);
+    void displacement_dist(const std::vector<std::string> &args);
     void isfs(const std::vector<std::string> &args);
     void isf(const std::vector<std::string> &args);
     void structure_factor(const std::vector<std::string> &args);

@@ -493,6 +493,68 @@ void Mean_Displacement::write(const std::string &filename) const
             << "\n";
 }
 
+// ---------------------------------------------------------------------------
+// Displacement_Distribution (src/displacement_distribution.cpp:19-141),
+// DebyeWaller_Dist (src/debyewaller_dist.cpp:14-141)
+
+Displacement_Distribution::Displacement_Distribution(System *sys, float power, int bins, float maxvalue, float t)
+    : system_(sys), power_(power), n_bins_(bins + 1), time_index_((int)t), max_value_(maxvalue)
+{
+    if (bins < 1 || bins > 16383) throw Error("distribution: number of bins must be 1..16383", 0);
+    binsize_ = maxvalue / bins;
+    const std::vector<float> times = sys->displacement_times();
+    if (time_index_ < 0 || time_index_ >= (int)times.size()) throw Error("Requested timegap out of range.", 0);
+    time_ = times[time_index_];
+    distribution_.assign(n_bins_, 0.0f);
+}
+
+void Displacement_Distribution::analyze(Trajectory_List *t_list)
+{
+    amdat_pairs *p = system_->gap_pairs(time_index_);
+    std::vector<unsigned long long> counts(n_bins_);
+    double s2 = 0, s4 = 0, c = 0;
+    int rc = amdat_displacement_dist(device_context(), system_->traj(), t_list->handle(), p, n_bins_ - 1,
+                                     max_value_, counts.data(), &s2, &s4, &c);
+    amdat_pairs_free(device_context(), p);
+    check(rc, "displacement distribution");
+    // postprocess_list: mean and distribution per atom (the 1/weighting
+    // factors of the reference cancel); an empty selection gives 0/0 as there
+    const double mean = s2 / c;
+    mean_ = (float)mean;
+    variance_ = (float)(s4 / c - mean * mean);
+    for (int b = 0; b < n_bins_; b++) distribution_[b] = (float)((double)counts[b] / c);
+}
+
+void Displacement_Distribution::write(const std::string &filename) const
+{
+    std::ofstream out(filename);
+    std::cout << "\nWriting distribution of particle displacement magnitudes to the " << power_ << " power to file "
+              << filename << ".";
+    out << "Particle displacement distribution to the " << power_ << " power created by AMDAT v." << kSemver << "\n";
+    out << "Last bin includes overflow.";
+    out << "time = " << time_ << "\n";
+    out << "Mean\t" << mean_ << "\n";
+    out << "Variance\t" << variance_ << "\n\n";
+    for (int b = 0; b < n_bins_; b++) out << (float(b) + 0.5) * binsize_ << "\t" << distribution_[b] << "\n";
+}
+
+DebyeWaller_Dist::DebyeWaller_Dist(System *sys, int bins, float maxvalue, float t)
+    : Displacement_Distribution(sys, 2.0f, bins, maxvalue, t)
+{
+}
+
+void DebyeWaller_Dist::write(const std::string &filename) const
+{
+    std::ofstream out(filename);
+    std::cout << "\nWriting distribution of mean-square dispalcements to file " << filename << ".";
+    out << "MSD distribution created by AMDAT v." << kSemver << "\n";
+    out << "MSD time = " << time_ << "\n";
+    out << "Mean_stiffness\t" << mean_ << "\n";
+    out << "Variance\t" << variance_ << "\n\n";
+    for (int b = 0; b < n_bins_; b++) out << (float(b) + 0.5) * binsize_ << "\t" << distribution_[b] << "\n";
+    out << "Last bin includes overflow.";
+}
+
 Van_Hove_Self::Van_Hove_Self(System *sys, int bin_count, float value_max)
     : system_(sys), n_bins_(bin_count), max_value_(value_max)
 {

@@ -5,7 +5,8 @@
 // their target lines); command syntax: src/control.cpp:1860-1916 (limit,
 // create_list), 2163-2210 (msd), 2655-2758 (structure_factor), 3107-3192
 // (isf), 3198-3238 (isfs), 3345-3370 (ngp), 2219-2271 (msd_2d,
-// mean_displacement), 2517-2550 (vhs), 3466-3496 (find_fast_fixedthreshold).
+// mean_displacement), 2517-2550 (vhs), 3466-3496 (find_fast_fixedthreshold),
+// 3247-3273 (u2dist), 3312-3339 (displacement_dist).
 #include <cstdlib>
 #include <ctime>
 #include <fstream>
@@ -118,6 +119,31 @@ void Control::vhs(const std::vector<std::string> &args)
     v.analyze(l);
     std::cout << "\nCalculated self Van Hove in " << time(nullptr) - start << " seconds.";
     v.write(args[1]);
+}
+
+// u2dist <file> <n_bins> <maxvalue> <time index> (src/control.cpp:3247-3273)
+void Control::u2dist(const std::vector<std::string> &args)
+{
+    if (args.size() != 5) throw Error("Incorrect number of arguments for u2dist.", 0);
+    const std::string runline = read_line();
+    std::cout << "\n" << runline;
+    DebyeWaller_Dist d(system_.get(), std::atoi(args[2].c_str()), (float)std::atof(args[3].c_str()),
+                       (float)std::atoi(args[4].c_str()));
+    d.analyze(target(runline));
+    d.write(args[1]);
+}
+
+// displacement_dist <file> <power> <n_bins> <maxvalue> <time index>
+// (src/control.cpp:3312-3339)
+void Control::displacement_dist(const std::vector<std::string> &args)
+{
+    if (args.size() != 6) throw Error("Incorrect number of arguments for displacement_dist.", 0);
+    const std::string runline = read_line();
+    std::cout << "\n" << runline;
+    Displacement_Distribution d(system_.get(), (float)std::atof(args[2].c_str()), std::atoi(args[3].c_str()),
+                                (float)std::atof(args[4].c_str()), (float)std::atoi(args[5].c_str()));
+    d.analyze(target(runline));
+    d.write(args[1]);
 }
 
 void Control::ngp(const std::vector<std::string> &args)
@@ -270,6 +296,8 @@ int Control::run()
             else if (cmd == "mean_displacement") mean_displacement(args);
             else if (cmd == "vhs") vhs(args);
             else if (cmd == "find_fast_fixedthreshold") find_fast_fixedthreshold(args);
+            else if (cmd == "u2dist") u2dist(args);
+            else if (cmd == "displacement_dist") displacement_dist(args);
             else if (cmd == "print") { std::cout << line.substr(line.find("print") + 5) << std::endl; }
             else throw Error("command " + cmd + " is not supported by this build", 0);
         } catch (const Error &e) {

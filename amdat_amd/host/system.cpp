@@ -289,6 +289,21 @@ std::vector<float> System::displacement_times() const
     return out;
 }
 
+amdat_pairs *System::gap_pairs(int time_index) const
+{
+    int n = 0;
+    int rc = amdat_ts_gap_pairs(n_exponentials_, n_exponential_steps_, frt_, displacement_limit_, time_index, nullptr,
+                                nullptr, nullptr, 0, &n);
+    if (rc < 0) throw Error(amdat_last_error(), 0);
+    std::vector<int> tg(n), t0(n), t1(n);
+    check(amdat_ts_gap_pairs(n_exponentials_, n_exponential_steps_, frt_, displacement_limit_, time_index, tg.data(),
+                             t0.data(), t1.data(), n, &n),
+          "pairs");
+    amdat_pairs *p = nullptr;
+    check(amdat_pairs_upload(device_context(), n, tg.data(), t0.data(), t1.data(), n_timegaps_, &p), "pair upload");
+    return p;
+}
+
 amdat_pairs *System::displacement_pairs(bool fullblock, std::vector<int> *tg_out, std::vector<int> *t0_out) const
 {
     int n = 0;

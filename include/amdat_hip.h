@@ -99,6 +99,14 @@ int amdat_ts_displacement_times(int n_exponentials, int n_exponential_steps, int
  * *n_pairs receives the full count. */
 int amdat_ts_pairs(int n_exponentials, int n_exponential_steps, int frt, int fullblock,
                    int limit, int *tg, int *t0, int *t1, int cap, int *n_pairs);
+/* Pairs of one time gap, System::displacement_list(analysis, timegap,
+ * false) (src/system.cpp:4349-4398): the first frame of each block only;
+ * the last gap is frames 0 and n_timesteps-1 (time gap n_timegaps-1, or
+ * requested as n_timegaps when frt = 1).  limit as in amdat_ts_pairs.  A
+ * time gap out of range is AMDAT_ERANGE ("Requested timegap out of
+ * range.").  Writes at most cap pairs; *n_pairs receives the count. */
+int amdat_ts_gap_pairs(int n_exponentials, int n_exponential_steps, int frt, int limit, int time_index,
+                       int *tg, int *t0, int *t1, int cap, int *n_pairs);
 /* System::timegap_weighting(fullblock) (src/system.cpp:4599-4628),
  * including its +1 on cross-block gaps.  out[n_timegaps]. */
 int amdat_ts_weighting(int n_exponentials, int n_exponential_steps, int frt, int fullblock,
@@ -320,6 +328,24 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
 int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
                         const amdat_pairs *pairs, int n_bins, float max_range, double *sum_d,
                         double *sum_d2, unsigned long long *hist, double *count);
+
+/* Distribution of squared displacements at one time gap: DebyeWaller_Dist
+ * (the u2dist command, src/debyewaller_dist.cpp:47-95, src/control.cpp:
+ * 3247-3273) and Displacement_Distribution (displacement_dist, src/
+ * displacement_distribution.cpp:50-98, src/control.cpp:3312-3339; its
+ * power argument is not used there either).  Pass the pairs of the time
+ * gap (amdat_ts_gap_pairs).  counts[n_bins + 1] receives, over all pairs
+ * and the list's atoms, the number whose squared displacement d2 =
+ * pow(distance, 2) stored as float (the FP32 length of the FP32 unwrapped
+ * difference) falls in bin int(d2 / (max_value / n_bins)), the last bin
+ * taking the overflow, exactly; sum_d2 and sum_d4 receive FP64 sums of
+ * |dr|^2 and |dr|^4 (FP64 rebuild) and count the atom pairs: the mean is
+ * sum_d2 / count, the variance sum_d4 / count - mean^2, and bin b's value
+ * counts[b] / count (the reference's 1/weighting factors cancel).
+ * n_bins <= 16383. */
+int amdat_displacement_dist(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                            int n_bins, float max_value, unsigned long long *counts, double *sum_d2,
+                            double *sum_d4, double *count);
 
 /* ---- differentiation (no reference counterpart) ----------------------- */
 

@@ -39,6 +39,9 @@
  *   orc_msd_ngp_f64     MSD(0) = 0, single-atom closed forms, image-count
  *                       rebuild equal to an explicit unwrap, Gaussian
  *                       walk MSD = 3 s^2 n.
+ *   orc_displacement_dist  constructed trajectories with exactly
+ *                       representable squared displacements (bin counts
+ *                       in closed form, bin edges, the overflow bin).
  *   orc_select_displacement  constructed trajectories with exactly
  *                       representable displacements (membership known in
  *                       closed form per block, image crossings, the squared
@@ -522,6 +525,45 @@ void orc_vhs_counts(int N, const float *x, const float *y, const float *z,
             int bin = (int)(distance / bin_size);
             if (bin >= 0 && bin < n_bins) hist[(size_t)tg[p] * n_bins + bin]++;
             weighting[tg[p]] += 1.0;
+        }
+    }
+}
+
+/* DebyeWaller_Dist / Displacement_Distribution (src/debyewaller_dist.cpp:
+ * 47-95, src/displacement_distribution.cpp:50-98) over given pairs: per
+ * pair, every atom of the list (all atoms when atoms is NULL) adds
+ * msd = pow(distance, 2) stored as float, with distance the FP32 length of
+ * the FP32 unwrapped difference, to bin int(msd / binsize), binsize =
+ * max_value / n_bins in FP32, bins >= n_bins + 1 clamped to the last
+ * (overflow) bin.  counts[n_bins + 1]; sum_d2, sum_d4 in FP64 of the FP32
+ * msd; *count = atom pairs.  The reference accumulates distribution, mean
+ * and square_term in FP32 with 1/weighting factors that cancel. */
+void orc_displacement_dist(int N, const float *x, const float *y, const float *z,
+                           const int16_t *ix, const int16_t *iy, const int16_t *iz, const float *box,
+                           const int *atoms, int n_list, int n_pairs, const int *t0, const int *t1,
+                           int n_bins, float max_value, unsigned long long *counts, double *sum_d2,
+                           double *sum_d4, double *count)
+{
+    int cnt = atoms ? n_list : N;
+    int nb = n_bins + 1;
+    float binsize = max_value / (float)n_bins;
+    for (int b = 0; b < nb; b++) counts[b] = 0;
+    *sum_d2 = *sum_d4 = *count = 0.0;
+    for (int p = 0; p < n_pairs; p++) {
+        size_t o0 = (size_t)t0[p] * N, o1 = (size_t)t1[p] * N;
+        const float *b0 = box + 3*t0[p], *b1 = box + 3*t1[p];
+        for (int a = 0; a < cnt; a++) {
+            int j = atoms ? atoms[a] : a;
+            float dx = unwrap32(x[o1+j], ix[o1+j], b1[0]) - unwrap32(x[o0+j], ix[o0+j], b0[0]);
+            float dy = unwrap32(y[o1+j], iy[o1+j], b1[1]) - unwrap32(y[o0+j], iy[o0+j], b0[1]);
+            float dz = unwrap32(z[o1+j], iz[o1+j], b1[2]) - unwrap32(z[o0+j], iz[o0+j], b0[2]);
+            float msd = (float)pow((double)sqrtf(dx*dx + dy*dy + dz*dz), 2.0);
+            int index = (int)(msd / binsize);
+            if (index >= nb) index = nb - 1;
+            counts[index]++;
+            *sum_d2 += (double)msd;
+            *sum_d4 += (double)msd * (double)msd;
+            *count += 1.0;
         }
     }
 }

@@ -238,6 +238,21 @@ def vhs_counts(tr, pairs, ntg, n_bins, max_range, atoms=None):
     return hist.reshape(ntg, n_bins), w
 
 
+def displacement_dist(tr, pairs, n_bins, max_value, atoms=None):
+    """Reference-form squared-displacement histogram over the pairs:
+    (counts [n_bins+1], sum d2, sum d2^2, count), d2 the FP32 msd."""
+    keep, ptrs = _traj_ptrs(tr)
+    tg, t0, t1 = (_c(v, np.int32) for v in pairs)
+    at = None if atoms is None else _c(atoms, np.int32)
+    counts = np.zeros(n_bins + 1, np.uint64)
+    s2, s4, c = C.c_double(), C.c_double(), C.c_double()
+    lib().orc_displacement_dist(tr.N, *ptrs, _p(at, C.c_int), 0 if at is None else len(at), len(t0),
+                                _p(t0, C.c_int), _p(t1, C.c_int), n_bins, C.c_float(max_value),
+                                counts.ctypes.data_as(C.POINTER(C.c_ulonglong)), C.byref(s2), C.byref(s4),
+                                C.byref(c))
+    return counts, s2.value, s4.value, c.value
+
+
 def select_displacement(tr, nexp, nes, frt, limit, time_index, mode, lo, hi, in_list=None):
     """Fast (mode 0) / Slow (1) / Average (2) particle lists per block.
     in_list: None (all atoms), an index array (static list) or a tuple

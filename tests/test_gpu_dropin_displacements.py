@@ -1,6 +1,6 @@
-"""The drop-in driver's msd_2d, mean_displacement and vhs commands (the
-reference's file formats) against the oracle, to the 6 significant digits
-the writers print."""
+"""The drop-in driver's msd_2d, mean_displacement, vhs, u2dist and
+displacement_dist commands (the reference's file formats) against the
+oracle, to the 6 significant digits the writers print."""
 import os
 import subprocess
 
@@ -45,6 +45,10 @@ list all
 mean_displacement md.dat
 list all
 vhs vhs.dat 3.0 60
+list all
+u2dist u2.dat 30 1.5 6
+list all
+displacement_dist dd.dat 4 20 0.8 2
 list all
 """
     (d / "in.amdat").write_text(script)
@@ -94,3 +98,44 @@ def test_vhs(run):
     assert np.allclose(centers, bs / 2 + np.arange(60) * bs, rtol=1e-6)
     assert rows.shape == (ntg, 61)
     assert np.allclose(rows[:, 1:], want, rtol=RTOL, atol=1e-12)
+
+
+def _dist_rows(path):
+    lines = open(path).read().split("\n")
+    return lines
+
+
+def _gap_pairs(g):
+    tg, t0, t1 = orc.ts_pairs(4, 5)
+    sel = tg == g
+    return tg[sel], t0[sel], t1[sel]
+
+
+@pytest.mark.parametrize("name,g,bins,maxv", [("u2.dat", 6, 30, 1.5), ("dd.dat", 2, 20, 0.8)])
+def test_squared_displacement_distributions(run, name, g, bins, maxv):
+    d, tr = run
+    counts, s2, s4, c = orc.displacement_dist(tr, _gap_pairs(g), bins, maxv)
+    mean = s2 / c
+    var = s4 / c - mean * mean
+    lines = _dist_rows(d / name)
+    tl = orc.ts_timelist(4, 5, np.float32(1.3), 0, 1.0)
+    time = orc.ts_displacement_times(4, 5, 0, tl)[g]
+    if name == "u2.dat":
+        assert lines[0].startswith("MSD distribution created by AMDAT v.")
+        assert np.isclose(float(lines[1].split(" = ")[1]), time, rtol=RTOL)
+        assert lines[2].startswith("Mean_stiffness\t") and lines[3].startswith("Variance\t")
+        body, tail = lines[5:5 + bins + 1], lines[5 + bins + 1]
+        assert tail == "Last bin includes overflow."
+    else:
+        assert lines[0] == "Particle displacement distribution to the 4 power created by AMDAT v." + lines[0].split("v.")[1]
+        assert lines[1].startswith("Last bin includes overflow.time = ")
+        assert np.isclose(float(lines[1].split(" = ")[1]), time, rtol=RTOL)
+        assert lines[2].startswith("Mean\t") and lines[3].startswith("Variance\t")
+        body = lines[5:5 + bins + 1]
+    assert np.isclose(float(lines[2].split("\t")[1]), mean, rtol=RTOL)
+    assert np.isclose(float(lines[3].split("\t")[1]), var, rtol=1e-4)
+    rows = np.array([[float(v) for v in l.split("\t")] for l in body])
+    binsize = np.float32(maxv) / np.float32(bins)
+    assert np.allclose(rows[:, 0], (np.arange(bins + 1) + 0.5) * binsize, rtol=RTOL)
+    assert np.allclose(rows[:, 1], counts / c, rtol=RTOL, atol=1e-9)
+    assert counts[-1] > 0                                        # the overflow bin is in the file

@@ -163,3 +163,34 @@ def test_frt_pairs():
     check(orc.select_displacement(tr, NEXP, NES, 1, 0, 6, 0, 0.3, 0.0), *want)
     with pytest.raises(ValueError):
         orc.select_displacement(tr, NEXP, NES, 1, 0, 7, 0, 0.3, 0.0)
+
+
+@pytest.mark.parametrize("g", [2, 4, 6])
+def test_displacement_dist(g):
+    """Squared-displacement histogram (u2dist / displacement_dist): with
+    Delta^2 exact multiples of 1/16 and bin size 1/4 (max 4, 16 bins + the
+    overflow bin), every count follows from the construction, bin edges
+    included (Delta^2 = 0.25 falls in bin 1)."""
+    tr, s = constructed()
+    if g < NES:
+        pairs = [(NES * b, NES * b + g) for b in range(NEXP)]
+    elif g < NES + NEXP - 1:
+        btg = g - NES + 1
+        pairs = [(NES * b, NES * b + NES * btg) for b in range(NEXP - btg)]
+    else:
+        pairs = [(0, T - 1)]
+    t0 = np.array([p[0] for p in pairs], np.int32)
+    t1 = np.array([p[1] for p in pairs], np.int32)
+    counts, s2, s4, c = orc.displacement_dist(tr, (np.full(len(t0), g, np.int32), t0, t1), 16, 4.0)
+    want = np.zeros(17, np.uint64)
+    d2 = np.array([(s[j, b] - s[j, a]) ** 2 for a, b in pairs for j in range(tr.N)])
+    for v in d2:
+        want[min(int(v / 0.25), 16)] += 1
+    assert np.array_equal(counts, want)
+    assert want[16] > 0                                  # the overflow bin is exercised
+    assert c == len(d2) and s2 == d2.sum() and s4 == (d2 ** 2).sum()
+    # a subset in a given order counts only its atoms
+    sub = np.array([3, 7, 1], np.int32)
+    counts, s2, s4, c = orc.displacement_dist(tr, (np.full(len(t0), g, np.int32), t0, t1), 16, 4.0, atoms=sub)
+    d2s = np.array([(s[j, b] - s[j, a]) ** 2 for a, b in pairs for j in sub])
+    assert c == len(d2s) and s2 == d2s.sum() and counts.sum() == len(d2s)
