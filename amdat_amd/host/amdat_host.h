@@ -379,12 +379,47 @@ class Van_Hove_Self {
     void analyze(Trajectory_List *t_list);
     void write(const std::string &filename) const;
     float show(int t, int b) const { return corr_[(size_t)t * n_bins_ + b]; }
+    int show_n_bins() const { return n_bins_; }
+    float show_bin_size() const { return bin_size_; }
 
   private:
     System *system_;
     int n_bins_;
     float max_value_, bin_size_;
     std::vector<float> timetable_, corr_;
+};
+
+class Non_Gaussian_Parameter;
+
+// src/gaussian_comparison.cpp: the self van Hove at one time gap against
+// its Gaussian approximation of the same MSD (both times the shell
+// volume); the crossings give the slow and fast boundaries (squared
+// displacements) and the fractions.  Host arithmetic on the device-computed
+// van Hove and MSD, in the reference's types and order.
+class Gaussian_Comparison {
+  public:
+    // time index: the non-Gaussian parameter's maximum (src/gaussian_comparison.cpp:126-149)
+    Gaussian_Comparison(System *sys, const Non_Gaussian_Parameter *ngp, const Van_Hove_Self *vhs,
+                        const Mean_Square_Displacement *msd);
+    // given time index (src/gaussian_comparison.cpp:160-185)
+    Gaussian_Comparison(System *sys, int time_in, const Van_Hove_Self *vhs, const Mean_Square_Displacement *msd);
+    void write(const std::string &filename) const;
+    int show_time_index() const { return time_index_; }
+    float show_fastboundary() const { return fastboundary_; }
+    float show_slowboundary() const { return slowboundary_; }
+    float show_slowfraction() const { return slowfraction_; }
+    float show_fastfraction() const { return fastfraction_; }
+
+  private:
+    void init(const Van_Hove_Self *vhs, const Mean_Square_Displacement *msd);
+    System *system_;
+    int n_bins_ = 0, time_index_ = 0;
+    float bin_size_ = 0, mean_square_displacement_ = 0;
+    std::vector<float> gaussian_approx_, van_hove_;
+    int slowboundary_index_ = 0, fastboundary_index_ = 0;
+    float slowboundary_ = 0, fastboundary_ = 0;
+    float slowfraction_ = 0, fastfraction_ = 0, gaussian_slowfraction_ = 0, gaussian_fastfraction_ = 0;
+    bool errorstate_ = false;
 };
 
 class Non_Gaussian_Parameter {
@@ -422,12 +457,17 @@ class Control {
     void isf(const std::vector<std::string> &args);
     void structure_factor(const std::vector<std::string> &args);
     void find_fast_fixedthreshold(const std::vector<std::string> &args);
+    void gaussian_comparison(const std::vector<std::string> &args);
+    void find_fast(const std::vector<std::string> &args);
+    void add_list(const std::string &name, std::unique_ptr<Trajectory_List> l);
     Trajectory_List *target(const std::string &line);
 
     std::vector<std::string> lines_;
     size_t cursor_ = 0;
     std::unique_ptr<System> system_;
     std::vector<std::pair<std::string, std::unique_ptr<Trajectory_List>>> lists_;
+    std::unique_ptr<Van_Hove_Self> vhs_;               // the last vhs command's result
+    std::unique_ptr<Gaussian_Comparison> gaussian_;    // the last gaussian_comparison
 };
 
 std::vector<std::string> tokenize(const std::string &line);

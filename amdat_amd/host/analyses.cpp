@@ -494,6 +494,102 @@ void Mean_Displacement::write(const std::string &filename) const
 }
 
 // ---------------------------------------------------------------------------
+// Gaussian_Comparison (src/gaussian_comparison.cpp:100-332)
+
+namespace {
+constexpr double kGcPi = 3.14159265;  // src/gaussian_comparison.cpp:14
+}
+
+Gaussian_Comparison::Gaussian_Comparison(System *sys, const Non_Gaussian_Parameter *ngp, const Van_Hove_Self *vhs,
+                                         const Mean_Square_Displacement *msd)
+    : system_(sys)
+{
+    time_index_ = ngp->max();
+    init(vhs, msd);
+}
+
+Gaussian_Comparison::Gaussian_Comparison(System *sys, int time_in, const Van_Hove_Self *vhs,
+                                         const Mean_Square_Displacement *msd)
+    : system_(sys)
+{
+    time_index_ = time_in;
+    init(vhs, msd);
+}
+
+void Gaussian_Comparison::init(const Van_Hove_Self *vhs, const Mean_Square_Displacement *msd)
+{
+    if (time_index_ < 0 || time_index_ >= msd->show_n_times())
+        throw Error("gaussian comparison: time index outside the time gaps", 0);
+    n_bins_ = vhs->show_n_bins();
+    bin_size_ = vhs->show_bin_size();
+    mean_square_displacement_ = msd->show(time_index_);
+    gaussian_approx_.assign(n_bins_, 0.0f);
+    van_hove_.assign(n_bins_, 0.0f);
+    // scale_van_hoves (src/gaussian_comparison.cpp:190-206)
+    const float alpha = 3.0 / (2.0 * mean_square_displacement_);
+    for (int b = 0; b < n_bins_; b++) {
+        const float shellvolume = 4.0 / 3.0 * kGcPi *
+                                  (std::pow((float(b) + 1.0) * bin_size_, 3.0) - std::pow(float(b) * bin_size_, 3.0));
+        van_hove_[b] = vhs->show(time_index_, b) * shellvolume;
+        gaussian_approx_[b] = std::pow(alpha / kGcPi, 3.0 / 2.0) *
+                              std::exp(-alpha * std::pow((float(b) + .5) * bin_size_, 2.0)) * shellvolume;
+    }
+    // find_crossovers (src/gaussian_comparison.cpp:211-249)
+    for (int b = 1; b < n_bins_; b++)
+        if (gaussian_approx_[b] >= van_hove_[b]) {
+            slowboundary_index_ = b - 1;
+            break;
+        }
+    for (int b = slowboundary_index_ + 1; b < n_bins_; b++)
+        if (gaussian_approx_[b] <= van_hove_[b]) {
+            fastboundary_index_ = b;
+            break;
+        }
+    if (fastboundary_index_ == 0) {
+        std::cout << "\nWarning: program cannot identify two crossing points between self Van Hove and Gaussian "
+                     "approximation self Van Hove.  Fractions of slow and fast particles will be incorrect.\n";
+        errorstate_ = true;
+    }
+    slowboundary_ = std::pow((float(slowboundary_index_) + .5) * bin_size_, 2);
+    fastboundary_ = std::pow((float(fastboundary_index_) - .5) * bin_size_, 2);
+    // determine_fractions (src/gaussian_comparison.cpp:256-272)
+    for (int b = 1; b <= slowboundary_index_; b++) {
+        slowfraction_ += van_hove_[b];
+        gaussian_slowfraction_ += gaussian_approx_[b];
+    }
+    fastfraction_ = 1.0 - slowfraction_;
+    gaussian_fastfraction_ = 1.0 - gaussian_slowfraction_;
+    for (int b = slowboundary_index_ + 1; b < fastboundary_index_; b++) {
+        fastfraction_ -= van_hove_[b];
+        gaussian_fastfraction_ -= gaussian_approx_[b];
+    }
+}
+
+void Gaussian_Comparison::write(const std::string &filename) const
+{
+    std::cout << "\nWriting fractions of slow and fast particles and comparison between gaussian and actual self van "
+                 "hove to file "
+              << filename << ".";
+    std::ofstream out(filename);
+    if (errorstate_)
+        out << "Warning: program cannot identify two crossing points between self Van Hove and Gaussian "
+               "approximation self Van Hove.  Fractions of slow and fast particles will be incorrect.\n";
+    out << "Gaussian comparison data created by AMDAT v." << kSemver << "\n";
+    out << "t*\n" << system_->displacement_times()[time_index_] << "\n";
+    out << "Slow_Cutoff: " << std::pow(slowboundary_, 0.5) << "\n";
+    out << "Fast_Cutoff: " << std::pow(fastboundary_, 0.5) << "\n";
+    out << "\t" << "Slow_Fraction" << "\t" << "Fast_Fraction\n";
+    out << "Gaussian" << "\t" << gaussian_slowfraction_ << "\t" << gaussian_fastfraction_ << "\n";
+    out << "Actual" << "\t" << slowfraction_ << "\t" << fastfraction_ << "\n";
+    out << "Excess" << "\t" << slowfraction_ - gaussian_slowfraction_ << "\t" << fastfraction_ - gaussian_fastfraction_
+        << "\n\n";
+    out << "Self_Van_Hove*shellvolume\n";
+    out << "radius\tgaussian\tactual\n";
+    for (int b = 0; b < n_bins_; b++)
+        out << (b + .5) * bin_size_ << "\t" << gaussian_approx_[b] << "\t" << van_hove_[b] << "\n";
+}
+
+// ---------------------------------------------------------------------------
 // Displacement_Distribution (src/displacement_distribution.cpp:19-141),
 // DebyeWaller_Dist (src/debyewaller_dist.cpp:14-141)
 

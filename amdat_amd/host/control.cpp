@@ -6,7 +6,8 @@
 // create_list), 2163-2210 (msd), 2655-2758 (structure_factor), 3107-3192
 // (isf), 3198-3238 (isfs), 3345-3370 (ngp), 2219-2271 (msd_2d,
 // mean_displacement), 2517-2550 (vhs), 3466-3496 (find_fast_fixedthreshold),
-// 3247-3273 (u2dist), 3312-3339 (displacement_dist).
+// 3247-3273 (u2dist), 3312-3339 (displacement_dist), 3377-3420
+// (gaussian_comparison), 3430-3458 (find_fast).
 #include <cstdlib>
 #include <ctime>
 #include <fstream>
@@ -113,12 +114,80 @@ void Control::vhs(const std::vector<std::string> &args)
 {
     if (args.size() != 4) throw Error("vhs expects an output file, a maximum range and a bin count", 0);
     Trajectory_List *l = target(read_line());
-    Van_Hove_Self v(system_.get(), std::atoi(args[3].c_str()), (float)std::atof(args[2].c_str()));
+    auto v = std::make_unique<Van_Hove_Self>(system_.get(), std::atoi(args[3].c_str()),
+                                             (float)std::atof(args[2].c_str()));
     std::cout << "\nCalculating self part of Van Hove correlation function.";
     const time_t start = time(nullptr);
-    v.analyze(l);
+    v->analyze(l);
     std::cout << "\nCalculated self Van Hove in " << time(nullptr) - start << " seconds.";
-    v.write(args[1]);
+    v->write(args[1]);
+    vhs_ = std::move(v);  // kept for gaussian_comparison (src/control.cpp:2541-2546)
+}
+
+// gaussian_comparison <file> [time index], then the target line
+// (src/control.cpp:3377-3420): MSD and NGP of the target, then the last
+// vhs against its Gaussian approximation at the NGP's maximum or the given
+// time index.
+void Control::gaussian_comparison(const std::vector<std::string> &args)
+{
+    if (!vhs_)
+        throw Error("self van hove not yet defined.  Self van hove must be calculated prior to compare_gaussian "
+                    "calculation.",
+                    0);
+    if (args.size() != 2 && args.size() != 3)
+        throw Error("Incorrect number of arguments for gaussian comparison method.", 0);
+    const std::string runline = read_line();
+    std::cout << "\n" << runline;
+    Trajectory_List *l = target(runline);
+    Mean_Square_Displacement msd(system_.get());
+    msd.analyze(l);
+    std::cout << "\nCalculating non-Gaussian parameter";
+    Non_Gaussian_Parameter ngpar(system_.get(), &msd);
+    ngpar.analyze(l);
+    std::cout << "\nComparing self Van Hove with Gaussian approximation.";
+    if (args.size() == 2)
+        gaussian_ = std::make_unique<Gaussian_Comparison>(system_.get(), &ngpar, vhs_.get(), &msd);
+    else
+        gaussian_ = std::make_unique<Gaussian_Comparison>(system_.get(), std::atoi(args[2].c_str()), vhs_.get(), &msd);
+    gaussian_->write(args[1]);
+}
+
+// find_fast <list> <count file>, then the target line (src/control.cpp:
+// 3430-3458): Fast_Particles with the last Gaussian comparison's time index
+// and fast boundary (src/fast_particles.cpp:38-56).
+void Control::find_fast(const std::vector<std::string> &args)
+{
+    if (args.size() < 3) throw Error("find_fast expects a list name and a count file", 0);
+    const std::string runline = read_line();
+    std::cout << "\n" << runline;
+    if (!gaussian_)
+        throw Error("A guassian comparison analysis myst be defined prior to calculating fast particle list.", 0);
+    std::cout << "\nIdentifying 'fast' particles.";
+    Trajectory_List *in = target(runline);
+    System *s = system_.get();
+    amdat_list *out = nullptr;
+    int rc = amdat_list_select_displacement(device_context(), s->traj(), in->handle(), s->show_n_exponentials(),
+                                            s->show_n_exponential_steps(), s->show_frt(), s->show_limit(),
+                                            gaussian_->show_time_index(), AMDAT_SELECT_FAST,
+                                            gaussian_->show_fastboundary(), 0.0f, &out);
+    if (rc < 0) throw Error(std::string("fast particles: ") + amdat_last_error(), 0);
+    auto l = std::make_unique<Trajectory_List>(s, out);
+    l->write_count(args[2]);
+    add_list(args[1], std::move(l));
+}
+
+// Control::add_trajectorylist (src/control.cpp:1417-1429): an existing name
+// is kept.
+void Control::add_list(const std::string &name, std::unique_ptr<Trajectory_List> l)
+{
+    for (auto &p : lists_)
+        if (p.first == name) {
+            std::cout << "\nWarning: trajectory_list " << name
+                      << " not created because a trajectory_list with this name already exists. Replacement of a "
+                         "trajectory_list requires that you first delete the existing list with the same name.\n";
+            return;
+        }
+    lists_.emplace_back(name, std::move(l));
 }
 
 // u2dist <file> <n_bins> <maxvalue> <time index> (src/control.cpp:3247-3273)
@@ -246,16 +315,7 @@ void Control::find_fast_fixedthreshold(const std::vector<std::string> &args)
     if (rc < 0) throw Error(std::string("fast particles: ") + amdat_last_error(), 0);
     auto l = std::make_unique<Trajectory_List>(s, out);
     l->write_count(filename);
-    // Control::add_trajectorylist (src/control.cpp:1417-1429): an existing
-    // name is kept
-    for (auto &p : lists_)
-        if (p.first == listname) {
-            std::cout << "\nWarning: trajectory_list " << listname
-                      << " not created because a trajectory_list with this name already exists. Replacement of a "
-                         "trajectory_list requires that you first delete the existing list with the same name.\n";
-            return;
-        }
-    lists_.emplace_back(listname, std::move(l));
+    add_list(listname, std::move(l));
 }
 
 int Control::run()
@@ -297,6 +357,8 @@ int Control::run()
             else if (cmd == "vhs") vhs(args);
             else if (cmd == "find_fast_fixedthreshold") find_fast_fixedthreshold(args);
             else if (cmd == "u2dist") u2dist(args);
+            else if (cmd == "gaussian_comparison") gaussian_comparison(args);
+            else if (cmd == "find_fast") find_fast(args);
             else if (cmd == "displacement_dist") displacement_dist(args);
             else if (cmd == "print") { std::cout << line.substr(line.find("print") + 5) << std::endl; }
             else throw Error("command " + cmd + " is not supported by this build", 0);
