@@ -56,6 +56,8 @@ SIGNATURES = {
     "amdat_read_xyz": (_I, [C.c_char_p, _I, _I, _PF, _I, _PF, _PF, _PF, _PS, _PS, _PS, _PI]),
     "amdat_traj_read": (_I, [_P, C.c_char_p, C.c_char_p, _I, _I, _PF, _I, _PI, _PP]),
     "amdat_traj_shape": (_I, [_P, _PI, _PI, _PF]),
+    "amdat_traj_bounds": (_I, [_P, _PF]),
+    "amdat_bins_assign": (_I, [_P, _P, _I, _I, _I, _PF, _PF, _PI]),
     "amdat_list_create": (_I, [_P, _P, _I, _PI, _PP]),
     "amdat_list_free": (_I, [_P, _P]),
     "amdat_list_create_timed": (_I, [_P, _P, _I, _PI, _PI, _PI, _PP]),
@@ -625,3 +627,20 @@ def displacement_dist(ctx, traj, alist, pairs, n_bins, max_value):
                                         float(max_value), counts.ctypes.data_as(C.POINTER(C.c_ulonglong)),
                                         C.byref(s2), C.byref(s4), C.byref(c)))
     return counts, s2.value, s4.value, c.value
+
+
+def bins_assign(ctx, traj, nbins, lo, hi):
+    """Cell of every (frame, atom) in a rectangular grid of nbins = (nx, ny, nz)
+    over (lo, hi) [T, 3] per frame: int32 [T, N], -1 outside."""
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    cell = np.zeros((traj.T, traj.N), np.int32)
+    check(lib().amdat_bins_assign(ctx.h, traj.h, int(nbins[0]), int(nbins[1]), int(nbins[2]), fp(lo), fp(hi),
+                                  ip(cell)))
+    return cell
+
+
+def traj_bounds(traj):
+    lo = np.zeros((traj.T, 3), np.float32)
+    check(lib().amdat_traj_bounds(traj.h, fp(lo)))
+    return lo

@@ -90,6 +90,7 @@ struct amdat_ctx {
 struct amdat_traj {
     TrajDev dev;
     std::vector<float> box;  // host copy [T][3]
+    std::vector<float> lo;   // box lower bounds [T][3] (custom files; zeros otherwise)
     DevBuf<float> x, y, z, dbox;
     DevBuf<int16_t> ix, iy, iz;
 };
@@ -550,6 +551,7 @@ static int traj_finish(amdat_ctx *ctx, amdat_traj *t, int n_atoms, int n_frames,
     t->dev.N = n_atoms;
     t->dev.T = n_frames;
     t->box.assign(box, box + 3 * (size_t)n_frames);
+    if (t->lo.size() != t->box.size()) t->lo.assign(t->box.size(), 0.0f);
     HIP_TRY(t->dbox.upload(t->box.data(), t->box.size(), stream_of(ctx)));
     t->dev.box = t->dbox.p;
     return AMDAT_OK;
@@ -605,6 +607,34 @@ int amdat_traj_shape(const amdat_traj *tr, int *n_atoms, int *n_frames, float *b
     if (n_atoms) *n_atoms = tr->dev.N;
     if (n_frames) *n_frames = tr->dev.T;
     if (box) std::copy(tr->box.begin(), tr->box.end(), box);
+    return AMDAT_OK;
+}
+
+int amdat_traj_bounds(const amdat_traj *tr, float *lo)
+{
+    if (!tr || !lo) return set_error(AMDAT_EINVAL, "amdat_traj_bounds: null argument");
+    std::copy(tr->lo.begin(), tr->lo.end(), lo);
+    return AMDAT_OK;
+}
+
+int amdat_bins_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+                      const float *hi, int *cell)
+{
+    clear_error();
+    if (!ctx || !tr || !lo || !hi || !cell) return set_error(AMDAT_EINVAL, "amdat_bins_assign: null argument");
+    if (n_xbins < 1 || n_ybins < 1 || n_zbins < 1 || (long)n_xbins * n_ybins * n_zbins > (1L << 30))
+        return set_error(AMDAT_EINVAL, "amdat_bins_assign: bin counts must be >= 1");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = stream_of(ctx);
+    const int N = tr->dev.N, T = tr->dev.T;
+    DevBuf<float> dlo, dhi;
+    DevBuf<int> dc;
+    HIP_TRY(dlo.upload(lo, 3 * (size_t)T, st));
+    HIP_TRY(dhi.upload(hi, 3 * (size_t)T, st));
+    HIP_TRY(dc.alloc((size_t)N * T));
+    HIP_TRY(launch_bins_assign(tr->dev, n_xbins, n_ybins, n_zbins, dlo.p, dhi.p, dc.p, st));
+    HIP_TRY(hipMemcpyAsync(cell, dc.p, sizeof(int) * (size_t)N * T, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
 }
 
@@ -664,7 +694,7 @@ int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_
         }
     } evg{done};
     hipStream_t st = stream_of(ctx);
-    std::vector<float> box((size_t)3 * T), prev_xyz;
+    std::vector<float> box((size_t)3 * T), lo((size_t)3 * T, 0.0f), prev_xyz;
     std::vector<int16_t> prev_img;
     for (int f0 = 0, c = 0; f0 < T; f0 += CF, c++) {
         const int f1 = std::min(T, f0 + CF), nf = f1 - f0, b = c & 1;
@@ -673,7 +703,7 @@ int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_
         int16_t *six = (int16_t *)(sz + cn), *siy = six + cn, *siz = siy + cn;
         if (fmt == "custom") {
             rc = parse_custom(f, off, f0, f1, N, threads, sx, sy, sz, six, siy, siz, box.data() + 3 * (size_t)f0,
-                              nullptr, f0 == 0 ? types : nullptr);
+                              lo.data() + 3 * (size_t)f0, f0 == 0 ? types : nullptr);
         } else {
             rc = parse_xyz(f, off, f0, f1, N, threads, sx, sy, sz, f0 == 0 ? types : nullptr);
             if (!rc)
@@ -708,6 +738,7 @@ int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_
     HIP_TRY(hipStreamSynchronize(st));
     t->dev.x = t->x.p; t->dev.y = t->y.p; t->dev.z = t->z.p;
     t->dev.ix = t->ix.p; t->dev.iy = t->iy.p; t->dev.iz = t->iz.p;
+    t->lo = lo;
     rc = traj_finish(ctx, t.get(), N, T, box.data());
     if (rc) return rc;
     *out = t.release();

@@ -107,6 +107,9 @@ hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int
                        int n_pairs, int n_blocks, int n_bins, int hist_mode, float bin_size, double *partial,
                        unsigned long long *hist, hipStream_t st);
 // Per group: the six sums of its pairs' partials (pairs in order[]).
+// Rectangular bin of every (frame, atom) into cell[T][N] (-1 outside).
+hipError_t launch_bins_assign(const TrajDev &tr, int nx, int ny, int nz, const float *lo, const float *hi, int *cell,
+                              hipStream_t st);
 // Displacement selection (Fast/Slow/Average_Particles): with tile_base ==
 // nullptr, per-block counts into tile_count[n_seg][n_tiles]; otherwise the
 // selected atoms, in input order, into out at tile_base[q][tile] + rank.

@@ -330,6 +330,36 @@ __device__ __forceinline__ bool displacement_selected(const float *const (&src)[
     return d2 <= hi && d2 >= lo;
 }
 
+// Rectangular bins (Trajectory_List_Bins::assign_bins, src/trajectory_list_
+// bins.cpp:949-1096): atom j of frame t (blockIdx.y) lies in the bin of its
+// stored coordinate when strictly inside (lo, hi) of the frame on every
+// axis, with index int((c - lo + 10 eps) / (hi - lo + 20 eps) * n) (FP32
+// differences, the rest in double, eps = FLT_EPSILON), as the reference
+// forms it; cell[t N + j] = (x ny + y) nz + z, or -1 outside.
+__global__ __launch_bounds__(kBlock) void bins_assign_kernel(const float *__restrict__ X, const float *__restrict__ Y,
+                                                              const float *__restrict__ Z, int N, int nx, int ny,
+                                                              int nz, const float *__restrict__ lo,
+                                                              const float *__restrict__ hi, int *__restrict__ cell)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.y;
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= N) return;
+    const size_t o = (size_t)t * N + j;
+    const float c[3] = {X[o], Y[o], Z[o]};
+    const int n[3] = {nx, ny, nz};
+    const double eps = (double)1.19209290e-07f;
+    int b[3];
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float l = lo[3 * t + k], h = hi[3 * t + k];
+        inside = inside && c[k] < h && c[k] > l;
+        b[k] = inside ? (int)(((double)(c[k] - l) + eps * 10.0) / ((double)(h - l) + eps * 20.0) * (double)(float)n[k]) : 0;
+    }
+    cell[o] = inside ? (b[0] * ny + b[1]) * nz + b[2] : -1;
+}
+
 // Selection pass 1.  Pair q (blockIdx.y) has seg[q] = {t0, t1, first entry
 // in the input list (-1: all atoms in order), entries}; one thread per
 // entry.  Writes the number of selected entries of each block of kBlock
@@ -754,6 +784,15 @@ hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int
     hipLaunchKernelGGL(disp_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), sizeof(unsigned int) * (size_t)n_bins, st,
                        tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, tg, n_bins, hist_mode,
                        bin_size, partial, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_bins_assign(const TrajDev &tr, int nx, int ny, int nz, const float *lo, const float *hi, int *cell,
+                              hipStream_t st)
+{
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(bins_assign_kernel, dim3((tr.N + kBlock - 1) / kBlock, tr.T), dim3(kBlock), 0, st, tr.x, tr.y,
+                       tr.z, tr.N, nx, ny, nz, lo, hi, cell);
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -86,6 +87,8 @@ class Trajectory_List {
     bool time_varying() const { return n_times_ > 0; }
     // Every list time's atoms, concatenated in list-time order.
     const std::vector<int> &atoms() const { return atoms_; }
+    // The atoms at frame timeii, in list order.
+    std::vector<int> atoms_at(int timeii) const;
     amdat_list *handle() const { return list_; }
     // Counts per list time: Trajectory_List::write_count
     // (src/trajectory_list.cpp:231-249) for a static list,
@@ -99,6 +102,31 @@ class Trajectory_List {
     int n_times_ = 0;
     std::vector<int> conv_, off_;
     amdat_list *list_ = nullptr;
+};
+
+// Rectangular spatial bins of all atoms at every frame
+// (Trajectory_List_Bins, src/trajectory_list_bins.cpp: "all" 207-279,
+// "region" 280-394, assign_bins 949-1096), computed on the device.
+class Trajectory_List_Bins {
+  public:
+    // all: the whole box of every frame
+    Trajectory_List_Bins(System *sys, int xbins, int ybins, int zbins);
+    // region: bounds 0 or outside the box fall back to the box
+    Trajectory_List_Bins(System *sys, int xbins, int ybins, int zbins, float x_lo, float x_hi, float y_lo,
+                         float y_hi, float z_lo, float z_hi);
+    int show_n_xbins() const { return n_[0]; }
+    int show_n_ybins() const { return n_[1]; }
+    int show_n_zbins() const { return n_[2]; }
+    // The atoms of list l that lie in bin (x, y, z), frame by frame, in atom
+    // order (Trajectory_List::operator&&, src/trajectory_list.cpp:518-):
+    // a time-varying list with one list time per frame.
+    std::unique_ptr<Trajectory_List> intersect(const Trajectory_List *l, int xii, int yii, int zii) const;
+
+  private:
+    void assign(const std::vector<float> &lo, const std::vector<float> &hi);
+    System *system_;
+    int n_[3];
+    std::vector<int> cell_;  // [T][N], -1 outside
 };
 
 class System {
@@ -125,6 +153,9 @@ class System {
     float show_time(int timeii) const { return timelist_.at(timeii); }
     Coordinate min_box_dimensions() const;
     Coordinate size(int timeii) const;
+    // Box lower bounds [T][3] (System::boundaries(t)[0], src/system.cpp);
+    // the upper bounds are these plus size(t).
+    std::vector<float> box_lo() const;
     void set_limit(int limit) { displacement_limit_ = limit; }
     int show_limit() const { return displacement_limit_; }
 
@@ -461,6 +492,13 @@ class Control {
     void find_fast(const std::vector<std::string> &args);
     void add_list(const std::string &name, std::unique_ptr<Trajectory_List> l);
     Trajectory_List *target(const std::string &line);
+    // Run fn(file suffix, list) for a "list <name>" target (suffix "") or,
+    // for a "bin_list <bins> <list> [persistence]" target, once per bin
+    // with the suffix Bin_Static_Analysis::write forms (src/bin_static_
+    // analysis.h:114-140).
+    void for_each_target(const std::string &line,
+                         const std::function<void(const std::string &, Trajectory_List *)> &fn);
+    void create_bin_list(const std::vector<std::string> &args);
 
     std::vector<std::string> lines_;
     size_t cursor_ = 0;
@@ -468,6 +506,7 @@ class Control {
     std::vector<std::pair<std::string, std::unique_ptr<Trajectory_List>>> lists_;
     std::unique_ptr<Van_Hove_Self> vhs_;               // the last vhs command's result
     std::unique_ptr<Gaussian_Comparison> gaussian_;    // the last gaussian_comparison
+    std::vector<std::pair<std::string, std::unique_ptr<Trajectory_List_Bins>>> bin_lists_;
 };
 
 std::vector<std::string> tokenize(const std::string &line);

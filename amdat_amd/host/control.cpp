@@ -7,8 +7,10 @@
 // (isf), 3198-3238 (isfs), 3345-3370 (ngp), 2219-2271 (msd_2d,
 // mean_displacement), 2517-2550 (vhs), 3466-3496 (find_fast_fixedthreshold),
 // 3247-3273 (u2dist), 3312-3339 (displacement_dist), 3377-3420
-// (gaussian_comparison), 3430-3458 (find_fast).
+// (gaussian_comparison), 3430-3458 (find_fast), 3841-4003 (create_bin_list),
+// and bin_list targets (src/control.h:269-391, Bin_Static_Analysis).
 #include <cstdlib>
+#include <functional>
 #include <ctime>
 #include <fstream>
 #include <iostream>
@@ -54,6 +56,84 @@ Trajectory_List *Control::target(const std::string &line)
     throw Error("Trajectory list '" + t[1] + "' not found.", 0);
 }
 
+// Bin_Static_Analysis::write (src/bin_static_analysis.h:123-137) forms
+// each bin's index strings with one stringstream per axis that is rewound
+// (seekp(0)) but never cleared, so a shorter index overwrites the first
+// characters of a longer earlier one ("02" after "12").  Kept for identical
+// file names.
+namespace {
+struct RewoundIndex {
+    std::string buf;
+    const std::string &put(int v)
+    {
+        const std::string s = std::to_string(v);
+        if (s.size() > buf.size()) buf.resize(s.size());
+        buf.replace(0, s.size(), s);
+        return buf;
+    }
+};
+}  // namespace
+
+void Control::for_each_target(const std::string &line,
+                              const std::function<void(const std::string &, Trajectory_List *)> &fn)
+{
+    auto t = tokenize(line);
+    if (!t.empty() && t[0] == "bin_list") {
+        // src/control.h:340-380: bin_list <bin list> <list> [persistence]
+        if (t.size() < 3) throw Error("Insufficient number of arguments in bin_list target line.", 0);
+        if (t.size() == 4 && std::atoi(t[3].c_str()) != 0)
+            throw Error("bin_list persistence (Bin_Dynamics_Analysis) is not supported by this build", 1);
+        const Trajectory_List_Bins *bins = nullptr;
+        for (auto &p : bin_lists_)
+            if (p.first == t[1]) bins = p.second.get();
+        if (!bins) throw Error("Bin list '" + t[1] + "' not found.", 0);
+        Trajectory_List *l = target("list " + t[2]);
+        RewoundIndex sx, sy, sz;
+        for (int x = 0; x < bins->show_n_xbins(); x++)
+            for (int y = 0; y < bins->show_n_ybins(); y++)
+                for (int z = 0; z < bins->show_n_zbins(); z++) {
+                    std::cout << "Bin (" << x << "," << y << "," << z << ")" << "\r";
+                    std::cout.flush();
+                    auto bl = bins->intersect(l, x, y, z);
+                    const std::string suffix =
+                        "_bin_" + sx.put(x) + "_" + sy.put(y) + "_" + sz.put(z) + ".bindata";
+                    fn(suffix, bl.get());
+                }
+        std::cout << std::endl;
+        return;
+    }
+    fn("", target(line));
+}
+
+// create_bin_list <name>, then "all <nx> <ny> <nz>" or "region <nx> <ny>
+// <nz> <xlo> <xhi> <ylo> <yhi> <zlo> <zhi>" (src/control.cpp:3841-4003).
+void Control::create_bin_list(const std::vector<std::string> &args)
+{
+    if (args.size() != 2) throw Error("create_bin_list expects one list name", 0);
+    std::cout << "\nCreating Binned Trajectory List";
+    auto a = tokenize(read_line());
+    if (a.empty()) throw Error("No set type found.", 0);
+    std::unique_ptr<Trajectory_List_Bins> b;
+    auto I = [&](int k) { return std::atoi(a[k].c_str()); };
+    auto F = [&](int k) { return (float)std::atof(a[k].c_str()); };
+    if (a[0] == "all") {
+        if (a.size() != 4) throw Error("Incorrect number of arguments for create_bin_list all.", 0);
+        b = std::make_unique<Trajectory_List_Bins>(system_.get(), I(1), I(2), I(3));
+    } else if (a[0] == "region") {
+        if (a.size() != 10) throw Error("Incorrect number of arguments for create_bin_list region.", 0);
+        b = std::make_unique<Trajectory_List_Bins>(system_.get(), I(1), I(2), I(3), F(4), F(5), F(6), F(7), F(8),
+                                                   F(9));
+    } else {
+        throw Error("bin_list of type " + a[0] + " is not supported by this build (all, region)", 0);
+    }
+    for (auto &p : bin_lists_)
+        if (p.first == args[1]) {
+            p.second = std::move(b);
+            return;
+        }
+    bin_lists_.emplace_back(args[1], std::move(b));
+}
+
 static std::vector<std::string> strip_store_flag(std::vector<std::string> a)
 {
     std::vector<std::string> out;
@@ -74,12 +154,14 @@ void Control::create_list(const std::vector<std::string> &args)
 void Control::msd(const std::vector<std::string> &args)
 {
     if (args.size() != 2) throw Error("msd expects an output file", 0);
-    Trajectory_List *l = target(read_line());
-    Mean_Square_Displacement m(system_.get());
+    const std::string runline = read_line();
     std::cout << "\nCalculating mean square displacement.\n";
     const time_t start = time(nullptr);
-    m.analyze(l);
-    m.write(args[1]);
+    for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
+        Mean_Square_Displacement m(system_.get());
+        m.analyze(l);
+        m.write(args[1] + suffix);
+    });
     std::cout << "\nCalculated mean square displacement in " << time(nullptr) - start << " seconds." << std::endl;
 }
 
@@ -87,12 +169,14 @@ void Control::msd(const std::vector<std::string> &args)
 void Control::msd_2d(const std::vector<std::string> &args)
 {
     if (args.size() != 3) throw Error("msd_2d expects an output file and a plane", 0);
-    Trajectory_List *l = target(read_line());
-    Mean_Square_Displacement_2D m(system_.get(), args[2]);
+    const std::string runline = read_line();
     std::cout << "\nCalculating mean square displacement.";
     const time_t start = time(nullptr);
-    m.analyze(l);
-    m.write(args[1]);
+    for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
+        Mean_Square_Displacement_2D m(system_.get(), args[2]);
+        m.analyze(l);
+        m.write(args[1] + suffix);
+    });
     std::cout << "\nCalculated mean square displacement in " << time(nullptr) - start << " seconds." << std::endl;
 }
 
@@ -100,12 +184,14 @@ void Control::msd_2d(const std::vector<std::string> &args)
 void Control::mean_displacement(const std::vector<std::string> &args)
 {
     if (args.size() != 2) throw Error("mean_displacement expects an output file", 0);
-    Trajectory_List *l = target(read_line());
-    Mean_Displacement m(system_.get());
+    const std::string runline = read_line();
     std::cout << "\nCalculating mean displacement.\n";
     const time_t start = time(nullptr);
-    m.analyze(l);
-    m.write(args[1]);
+    for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
+        Mean_Displacement m(system_.get());
+        m.analyze(l);
+        m.write(args[1] + suffix);
+    });
     std::cout << "\nCalculated mean displacement in " << time(nullptr) - start << " seconds.";
 }
 
@@ -234,12 +320,14 @@ void Control::isfs(const std::vector<std::string> &args)
     const int inner = std::atoi(args[2].c_str()), outer = std::atoi(args[3].c_str());
     const float ell = (float)std::atof(args[5].c_str());
     const bool fb = args.size() == 7 ? std::atoi(args[6].c_str()) != 0 : false;
-    Trajectory_List *l = target(read_line());
+    const std::string runline = read_line();
     const time_t start = time(nullptr);
     Wave_Vectors wv(system_.get(), args[4], ell);
-    Incoherent_Scattering_Function f(system_.get(), &wv, inner, outer, fb);
-    f.analyze(l);
-    f.write(args[1]);
+    for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
+        Incoherent_Scattering_Function f(system_.get(), &wv, inner, outer, fb);
+        f.analyze(l);
+        f.write(args[1] + suffix);
+    });
     std::cout << "\nSelf Intermediate Scattering Function calculated in " << time(nullptr) - start
               << " seconds.\n";
 }
@@ -278,9 +366,23 @@ void Control::structure_factor(const std::vector<std::string> &args)
     const int ts = args.size() == 6 ? std::atoi(args[5].c_str()) : 0;
     if (ell < 0 && ts < -1) ell = system_->size(-ts - 2).min();
     Wave_Vectors wv(system_.get(), args[3], ell);
-    Structure_Factor sf(system_.get(), &wv, ts);
-    Trajectory_List *l1 = target(read_line());
+    const std::string runline = read_line();
     const time_t start = time(nullptr);
+    const auto rt = tokenize(runline);
+    if (!rt.empty() && rt[0] == "bin_list") {
+        // src/control.cpp:2730-2744: binning works with symmetric S(k) only
+        if (args[2] != "symmetric") throw Error("Binning currently only functions with symmetric S(k).", 0);
+        std::cout << "\nCalculating structure factor.";
+        for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
+            Structure_Factor sf(system_.get(), &wv, ts);
+            sf.analyze(l);
+            sf.write(args[1] + suffix);
+        });
+        std::cout << "\nCalculated structure factor in " << time(nullptr) - start << " seconds.";
+        return;
+    }
+    Structure_Factor sf(system_.get(), &wv, ts);
+    Trajectory_List *l1 = target(runline);
     if (args[2] == "symmetric") {
         sf.analyze(l1);
     } else if (args[2] == "asymmetric") {
@@ -358,6 +460,7 @@ int Control::run()
             else if (cmd == "find_fast_fixedthreshold") find_fast_fixedthreshold(args);
             else if (cmd == "u2dist") u2dist(args);
             else if (cmd == "gaussian_comparison") gaussian_comparison(args);
+            else if (cmd == "create_bin_list") create_bin_list(args);
             else if (cmd == "find_fast") find_fast(args);
             else if (cmd == "displacement_dist") displacement_dist(args);
             else if (cmd == "print") { std::cout << line.substr(line.find("print") + 5) << std::endl; }

@@ -79,6 +79,83 @@ int Trajectory_List::show_n_trajectories(int timeii) const
     return (c >= 0 && c < n_times_) ? off_[c + 1] - off_[c] : 0;
 }
 
+std::vector<int> Trajectory_List::atoms_at(int timeii) const
+{
+    if (n_times_ == 0) return atoms_;
+    if (timeii < 0 || timeii >= (int)conv_.size()) return {};
+    const int c = conv_[timeii];
+    if (c < 0 || c >= n_times_) return {};
+    return std::vector<int>(atoms_.begin() + off_[c], atoms_.begin() + off_[c + 1]);
+}
+
+// ---------------------------------------------------------------------------
+// Trajectory_List_Bins
+
+Trajectory_List_Bins::Trajectory_List_Bins(System *sys, int xbins, int ybins, int zbins) : system_(sys), n_{xbins, ybins, zbins}
+{
+    const int T = sys->show_n_timesteps();
+    std::vector<float> lo = sys->box_lo(), hi(3 * (size_t)T);
+    for (int t = 0; t < T; t++) {
+        const Coordinate L = sys->size(t);
+        hi[3 * t] = lo[3 * t] + L.show_x();
+        hi[3 * t + 1] = lo[3 * t + 1] + L.show_y();
+        hi[3 * t + 2] = lo[3 * t + 2] + L.show_z();
+    }
+    assign(lo, hi);
+}
+
+Trajectory_List_Bins::Trajectory_List_Bins(System *sys, int xbins, int ybins, int zbins, float x_lo, float x_hi,
+                                           float y_lo, float y_hi, float z_lo, float z_hi)
+    : system_(sys), n_{xbins, ybins, zbins}
+{
+    // src/trajectory_list_bins.cpp:309-356: a bound of 0, or beyond the
+    // box, is the box boundary of the frame
+    const int T = sys->show_n_timesteps();
+    const std::vector<float> blo = sys->box_lo();
+    std::vector<float> lo(3 * (size_t)T), hi(3 * (size_t)T);
+    const float ulo[3] = {x_lo, y_lo, z_lo}, uhi[3] = {x_hi, y_hi, z_hi};
+    for (int t = 0; t < T; t++) {
+        const Coordinate L = sys->size(t);
+        const float len[3] = {L.show_x(), L.show_y(), L.show_z()};
+        for (int c = 0; c < 3; c++) {
+            const float bl = blo[3 * t + c], bh = blo[3 * t + c] + len[c];
+            lo[3 * t + c] = (ulo[c] == 0 || ulo[c] < bl) ? bl : ulo[c];
+            hi[3 * t + c] = (uhi[c] == 0 || uhi[c] > bh) ? bh : uhi[c];
+        }
+    }
+    assign(lo, hi);
+}
+
+void Trajectory_List_Bins::assign(const std::vector<float> &lo, const std::vector<float> &hi)
+{
+    if (n_[0] < 1 || n_[1] < 1 || n_[2] < 1) throw Error("bin counts must be at least 1", 0);
+    cell_.resize((size_t)system_->show_n_timesteps() * system_->show_n_atoms());
+    check(amdat_bins_assign(device_context(), system_->traj(), n_[0], n_[1], n_[2], lo.data(), hi.data(),
+                            cell_.data()),
+          "bins");
+}
+
+std::unique_ptr<Trajectory_List> Trajectory_List_Bins::intersect(const Trajectory_List *l, int xii, int yii,
+                                                                 int zii) const
+{
+    const int T = system_->show_n_timesteps(), N = system_->show_n_atoms();
+    const int bin = (xii * n_[1] + yii) * n_[2] + zii;
+    std::vector<int> conv(T), off(T + 1, 0), idx;
+    for (int t = 0; t < T; t++) {
+        conv[t] = t;
+        std::vector<int> a = l->atoms_at(t);
+        std::sort(a.begin(), a.end());
+        for (int j : a)
+            if (cell_[(size_t)t * N + j] == bin) idx.push_back(j);
+        off[t + 1] = (int)idx.size();
+    }
+    amdat_list *h = nullptr;
+    check(amdat_list_create_timed(device_context(), system_->traj(), T, conv.data(), off.data(),
+                                  idx.empty() ? nullptr : idx.data(), &h),
+          "bin list");
+    return std::make_unique<Trajectory_List>(system_, h);
+}
+
 void Trajectory_List::write_count(const std::string &filename) const
 {
     const int nt = n_times_ > 0 ? n_times_ : 1;
@@ -270,6 +347,13 @@ Coordinate System::min_box_dimensions() const
 }
 
 Coordinate System::size(int t) const { return Coordinate(box_[3 * t], box_[3 * t + 1], box_[3 * t + 2]); }
+
+std::vector<float> System::box_lo() const
+{
+    std::vector<float> lo(3 * (size_t)n_timesteps_, 0.0f);
+    check(amdat_traj_bounds(traj_, lo.data()), "box bounds");
+    return lo;
+}
 
 std::vector<int> System::timegap_weighting(bool fullblock) const
 {

@@ -203,6 +203,22 @@ int amdat_read_xyz(const char *path, int n_atoms, int n_frames, const float *box
 int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_atoms, int n_frames,
                     const float *box3, int threads, int *types, amdat_traj **out);
 
+/* Box lower bounds [n_frames][3] of a trajectory: the bounds lines of a
+ * custom file (amdat_traj_read), zeros for xyz files and uploads.  With the
+ * box lengths (amdat_traj_shape) they give System::boundaries
+ * (src/system.cpp). */
+int amdat_traj_bounds(const amdat_traj *tr, float *lo);
+
+/* Rectangular bins of Trajectory_List_Bins (src/trajectory_list_bins.cpp:
+ * 207-279 "all", 280-394 "region", assign_bins 949-1096): for every frame
+ * t and atom j, cell[t*n_atoms + j] = (x*n_ybins + y)*n_zbins + z for the
+ * atom's stored coordinate c strictly inside (lo, hi) of frame t on every
+ * axis (lo, hi: [n_frames][3]), with x = int((c - lo + 10 FLT_EPSILON) /
+ * (hi - lo + 20 FLT_EPSILON) * n_xbins) formed as the reference forms it,
+ * or -1 outside.  cell is host memory [n_frames][n_atoms]. */
+int amdat_bins_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+                      const float *hi, int *cell);
+
 /* Static atom selection (Static_Trajectory_List, src/static_trajectory_
  * list.cpp:59-101): atom indices in list order.  indices == NULL with
  * n == n_atoms of the trajectory means "all atoms in order". */

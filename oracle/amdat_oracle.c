@@ -39,6 +39,9 @@
  *   orc_msd_ngp_f64     MSD(0) = 0, single-atom closed forms, image-count
  *                       rebuild equal to an explicit unwrap, Gaussian
  *                       walk MSD = 3 s^2 n.
+ *   orc_bins_assign     interior points map to floor((c - lo)/w), bin
+ *                       boundaries go to the upper bin, box faces and
+ *                       points outside the region are excluded.
  *   orc_displacement_dist  constructed trajectories with exactly
  *                       representable squared displacements (bin counts
  *                       in closed form, bin edges, the overflow bin).
@@ -527,6 +530,33 @@ void orc_vhs_counts(int N, const float *x, const float *y, const float *z,
             weighting[tg[p]] += 1.0;
         }
     }
+}
+
+/* Trajectory_List_Bins::assign_bins (src/trajectory_list_bins.cpp:
+ * 949-1096): cell[t*N + j] = (xi*ny + yi)*nz + zi when the stored
+ * coordinate is strictly inside (lo, hi) of frame t on every axis, with
+ * xi = int((x - lo + FLT_EPSILON*10.0) / (hi - lo + FLT_EPSILON*20.0) *
+ * float(nx)) in the reference's types (FP32 differences, double
+ * arithmetic after), else -1. */
+void orc_bins_assign(int N, int T, const float *x, const float *y, const float *z, int nx, int ny, int nz,
+                     const float *lo, const float *hi, int *cell)
+{
+    const float feps = 1.19209290e-07f;
+    int n[3] = {nx, ny, nz};
+    for (int t = 0; t < T; t++)
+        for (int j = 0; j < N; j++) {
+            size_t o = (size_t)t * N + j;
+            float c[3] = {x[o], y[o], z[o]};
+            int inside = 1, b[3] = {0, 0, 0};
+            for (int k = 0; k < 3; k++)
+                if (!(c[k] < hi[3*t + k] && c[k] > lo[3*t + k])) inside = 0;
+            if (inside)
+                for (int k = 0; k < 3; k++) {
+                    float l = lo[3*t + k], h = hi[3*t + k];
+                    b[k] = (int)(((double)(c[k] - l) + feps * 10.0) / ((double)(h - l) + feps * 20.0) * (double)(float)n[k]);
+                }
+            cell[o] = inside ? (b[0] * ny + b[1]) * nz + b[2] : -1;
+        }
 }
 
 /* DebyeWaller_Dist / Displacement_Distribution (src/debyewaller_dist.cpp:

@@ -238,6 +238,17 @@ def vhs_counts(tr, pairs, ntg, n_bins, max_range, atoms=None):
     return hist.reshape(ntg, n_bins), w
 
 
+def bins_assign(tr, nbins, lo, hi):
+    """Rectangular bin of every (frame, atom): int32 [T, N], -1 outside."""
+    x, y, z = (_c(a, np.float32) for a in (tr.x, tr.y, tr.z))
+    lo = _c(lo, np.float32)
+    hi = _c(hi, np.float32)
+    cell = np.zeros((tr.T, tr.N), np.int32)
+    lib().orc_bins_assign(tr.N, tr.T, _p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), int(nbins[0]),
+                          int(nbins[1]), int(nbins[2]), _p(lo, C.c_float), _p(hi, C.c_float), _p(cell, C.c_int))
+    return cell
+
+
 def displacement_dist(tr, pairs, n_bins, max_value, atoms=None):
     """Reference-form squared-displacement histogram over the pairs:
     (counts [n_bins+1], sum d2, sum d2^2, count), d2 the FP32 msd."""

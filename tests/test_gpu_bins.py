@@ -1,0 +1,162 @@
+"""Rectangular bins (create_bin_list all/region, bin_list targets) on the
+device: cell assignment bit-exact with the oracle, box bounds read from a
+custom file, and the drop-in's per-bin msd, isfs and structure_factor files
+(Bin_Static_Analysis names, quirk included) against the oracle composed
+bin by bin.  Plain form N = 1,000 in a box [-4.5, 4.91)^3, exponential 5 20."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from amdat_amd import _abi, synth
+from oracle import oracle as orc
+from tests.conftest import ROOT, atoms_at, compose_pairs, ref_shells
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 2e-5
+NEXP, NES = 5, 20
+NTG = NES + NEXP
+LO = -4.5
+
+
+def shifted():
+    tr = synth.plain_config()
+    lo = np.float32(LO)
+    return synth.Traj((tr.x + lo).astype(np.float32), (tr.y + lo).astype(np.float32),
+                      (tr.z + lo).astype(np.float32), tr.ix, tr.iy, tr.iz, tr.box)
+
+
+def bounds(tr):
+    lo = np.full((tr.T, 3), np.float32(LO), np.float32)
+    return lo, (lo + tr.box).astype(np.float32)
+
+
+def test_cells_match_oracle(ctx):
+    print("case: bins_assign plain form", flush=True)
+    tr = shifted()
+    d = _abi.Trajectory(ctx, *tr.arrays)
+    lo, hi = bounds(tr)
+    for nb in [(2, 2, 2), (3, 1, 4), (7, 5, 3)]:
+        got = _abi.bins_assign(ctx, d, nb, lo, hi)
+        want = orc.bins_assign(tr, nb, lo, hi)
+        assert np.array_equal(got, want)
+        assert np.all(got >= 0)                                   # every atom is inside its box
+    # a region: part of the atoms fall outside
+    rlo = lo.copy()
+    rhi = hi.copy()
+    rlo[:, 0], rhi[:, 1] = -1.0, 2.0
+    got = _abi.bins_assign(ctx, d, (4, 4, 4), rlo, rhi)
+    assert np.array_equal(got, orc.bins_assign(tr, (4, 4, 4), rlo, rhi))
+    assert 0 < np.mean(got < 0) < 1
+
+
+def bin_lists(tr, cells, nb):
+    out = {}
+    for b in range(int(np.prod(nb))):
+        conv = np.arange(tr.T, dtype=np.int32)
+        off, idx = [0], []
+        for t in range(tr.T):
+            a = np.nonzero(cells[t] == b)[0]
+            idx.extend(a)
+            off.append(len(idx))
+        out[b] = (conv, np.array(off, np.int32), np.array(idx, np.int32))
+    return out
+
+
+@pytest.fixture(scope="module")
+def run(tmp_path_factory, qvec):
+    d = tmp_path_factory.mktemp("bins")
+    off, n = qvec
+    qdir = d / "qvectors3d"
+    qdir.mkdir()
+    for b in range(2, 101):
+        with open(qdir / ("qvector.%03d" % b), "w", newline="") as fh:
+            for v in n[off[b - 1]:off[b]]:
+                fh.write("  %.7e   %.7e   %.7e\r\n" % tuple(v))
+    tr = shifted()
+    synth.write_lammps_custom(tr, str(d / "traj.dump"), box_lo=LO)
+    script = """system
+custom
+traj.dump
+exponential %d %d 1.3 0 0 1
+A 800 B 200
+1 2
+1 0
+0 1
+
+create_list all
+all
+create_bin_list grid
+all 2 2 2
+msd msd.dat
+bin_list grid all
+isfs isfs.dat 20 20 xyz 9.41 0
+bin_list grid all
+structure_factor sk.dat symmetric xyz 0 -1
+bin_list grid all
+""" % (NEXP, NES)
+    (d / "in.amdat").write_text(script)
+    env = dict(os.environ, AMDAT_WV3D=str(qdir / "qvector"))
+    print("case: amdat_hip bin_list script, N=1000 T=101, 2x2x2 bins", flush=True)
+    p = subprocess.run([os.path.join(ROOT, "amdat_amd", "bin", "amdat_hip"), "-i", "in.amdat"], cwd=str(d), env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-3000:] + p.stderr[-3000:]
+    lo, hi = bounds(tr)
+    cells = orc.bins_assign(tr, (2, 2, 2), lo, hi)
+    return d, tr, bin_lists(tr, cells, (2, 2, 2))
+
+
+def name(stem, b):
+    x, y, z = b // 4, (b // 2) % 2, b % 2
+    return "%s_bin_%d_%d_%d.bindata" % (stem, x, y, z)
+
+
+def test_bin_msd_files(run):
+    d, tr, lists = run
+    pairs = orc.ts_pairs(NEXP, NES)
+    for b, lst in lists.items():
+        s2, cnt = np.zeros(NTG), np.zeros(NTG)
+        for m in compose_pairs(lst, pairs, lambda p, at: orc.msd_ngp_f64(tr, p, NTG, atoms=at)):
+            s2 += m["s2"]
+            cnt += m["count"]
+        rows = np.array([[float(v) for v in l.split("\t")] for l in
+                         (d / name("msd.dat", b)).read_text().split("\n")[1:] if l.strip()])
+        ok = cnt > 0
+        assert np.allclose(rows[1:, 1][ok[1:]], (s2 / np.where(ok, cnt, 1))[1:][ok[1:]], rtol=RTOL)
+
+
+def test_bin_isfs_and_sk_files(run, qvec):
+    d, tr, lists = run
+    pairs = orc.ts_pairs(NEXP, NES)
+    o22, n22 = ref_shells(qvec, 22, 22)
+    offA, nA = ref_shells(qvec, 2, 100)
+    ell = float(np.float32(9.41))
+    for b, lst in lists.items():
+        SF, SC = np.zeros((NTG, 1)), np.zeros(NTG)
+        for f, c in compose_pairs(lst, pairs, lambda p, at: orc.fs_f64(tr, p, NTG, o22, n22, ell, at)):
+            SF += f * c[:, None]
+            SC += c
+        lines = (d / name("isfs.dat", b)).read_text().split("\n")
+        rows = np.array([[float(v) for v in l.strip("\t").split("\t")] for l in lines[2:] if l.strip()])
+        ok = SC > 0
+        assert np.allclose(rows[ok, 1], SF[ok, 0] / SC[ok], rtol=RTOL, atol=1e-6)
+    # S(k) for two bins (every frame, the bin's atoms at that frame)
+    L = float(np.float32(9.41))
+    for b in (0, 7):
+        lst = lists[b]
+        natoms = np.array([len(atoms_at(lst, t)) for t in range(tr.T)])
+        rho = np.stack([orc.rho_f64(tr.x[t], tr.y[t], tr.z[t], nA, L, atoms_at(lst, t)) if natoms[t] else
+                        np.zeros(len(nA), complex) for t in range(tr.T)])
+        S = orc.sk_f64(rho, rho, natoms, offA)
+        lines = (d / name("sk.dat", b)).read_text().split("\n")
+        rows = np.array([[float(v) for v in l.split("\t")] for l in lines[2:] if l.strip()])
+        assert np.allclose(rows[:, 6], S, rtol=RTOL)
+
+
+def test_bounds_from_custom_file(ctx, run):
+    d, tr, lists = run
+    t, _ = _abi.Trajectory.read(ctx, "custom", str(d / "traj.dump"), tr.N, tr.T)
+    lo = _abi.traj_bounds(t)
+    assert np.all(lo == np.float32(LO))
