@@ -58,22 +58,40 @@ def build_workload(small):
 
 
 def cpu_baseline(tr, shF, shS, L, ell_fs, n_sample, pairs):
-    """Reference-arithmetic FP32 restatement (oracle, one core) on a sample:
-    rho for one frame and Fs for one pair over n_sample atoms, all vectors."""
+    """Reference-arithmetic FP32 restatement (oracle) on a sample: rho for
+    one frame and Fs for one pair over n_sample atoms, all vectors.  One core
+    (the reference's hot path is serial) is the reported value; the same
+    restatement on all host cores (threads over vectors and shells, same
+    results) and the FP64 oracle on all host cores are timed beside it."""
     from oracle import oracle as orc
-    orc.set_threads(1)
     atoms = np.arange(n_sample, dtype=np.int32)
-    t = time.perf_counter()
-    orc.ref32_rho(tr.x[500 % tr.T], tr.y[500 % tr.T], tr.z[500 % tr.T], shF[1], np.float32(L), atoms=atoms)
     tg, t0, t1, ntg = pairs
     sel = (np.array([tg[-2]]), np.array([t0[-2]]), np.array([t1[-2]]))
-    orc.ref32_fs(tr, sel, ntg, shS[0], shS[1], np.float32(ell_fs), atoms=atoms)
-    dt = time.perf_counter() - t
+    f = 500 % tr.T
     upd = n_sample * len(shF[1]) + n_sample * len(shS[1])
+
+    def run(threads, fp64=False):
+        orc.set_threads(threads)
+        t = time.perf_counter()
+        if fp64:
+            orc.rho_f64(tr.x[f], tr.y[f], tr.z[f], shF[1], float(np.float32(L)), atoms=atoms)
+            orc.fs_f64(tr, sel, ntg, shS[0], shS[1], float(np.float32(ell_fs)), atoms=atoms)
+        else:
+            orc.ref32_rho(tr.x[f], tr.y[f], tr.z[f], shF[1], np.float32(L), atoms=atoms)
+            orc.ref32_fs(tr, sel, ntg, shS[0], shS[1], np.float32(ell_fs), atoms=atoms)
+        return time.perf_counter() - t
+
+    dt = run(1)
+    cores = max(1, min(16, os.cpu_count() or 1))
+    dt_all = run(cores)
+    dt64 = run(cores, fp64=True)
+    orc.set_threads(1)
     return dict(value=upd / dt, unit="updates/s", cores=1, kind="port",
                 sample="rho_k one frame + Fs one time-origin pair, %d atoms x %d/%d vectors, "
                        "reference FP32 arithmetic (cosf/sinf per update), %.1f s" % (
-                           n_sample, len(shF[1]), len(shS[1]), dt))
+                           n_sample, len(shF[1]), len(shS[1]), dt),
+                all_cores={"value": upd / dt_all, "cores": cores, "kind": "port"},
+                fp64_all_cores={"value": upd / dt64, "cores": cores, "kind": "FP64 oracle"})
 
 
 def make_batch_members(ctx, n_traj, n_min, n_max, nexp, nes, seed):
@@ -437,7 +455,10 @@ def main():
         n_sample = 600 if args.small else 20000   # about 10 s of one core
         result["cpu_baseline"] = cpu_baseline(tr, (offF, nF), (offS, nS), cfg["L"], cfg["ell_fs"], n_sample,
                                               (tg, t0, t1, ntg))
-        log("cpu baseline %.3e updates/s" % result["cpu_baseline"]["value"])
+        cb = result["cpu_baseline"]
+        log("cpu baseline %.3e updates/s (1 core); %d cores: %.3e; FP64 oracle on %d cores: %.3e"
+            % (cb["value"], cb["all_cores"]["cores"], cb["all_cores"]["value"], cb["fp64_all_cores"]["cores"],
+               cb["fp64_all_cores"]["value"]))
     print(json.dumps(result), flush=True)
 
 

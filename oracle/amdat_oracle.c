@@ -438,15 +438,20 @@ void orc_ref32_rho(int N, const float *x, const float *y, const float *z,
     float spacing = orc_wave_spacing(length_scale);
     float *kk = (float *)malloc(sizeof(float) * 3 * (size_t)(K > 0 ? K : 1));
     for (int v = 0; v < K; v++) kvec32(nvec + 3*v, spacing, kk + 3*v);
-    for (int v = 0; v < 2*K; v++) rho[v] = 0.0;
-    for (int a = 0; a < cnt; a++) {
-        int j = atoms ? atoms[a] : a;
-        float cx = x[j], cy = y[j], cz = z[j];
-        for (int v = 0; v < K; v++) {
-            float kr = kk[3*v] * cx + kk[3*v+1] * cy + kk[3*v+2] * cz;
-            rho[2*v] += (double)cosf(kr);
-            rho[2*v+1] += (double)sinf(kr);
+    /* Each vector's sum runs over the atoms in list order, as in the
+     * reference's atom-outer loop; threads (orc_set_threads) take vectors,
+     * so the results do not depend on the thread count. */
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int v = 0; v < K; v++) {
+        double re = 0.0, im = 0.0;
+        for (int a = 0; a < cnt; a++) {
+            int j = atoms ? atoms[a] : a;
+            float kr = kk[3*v] * x[j] + kk[3*v+1] * y[j] + kk[3*v+2] * z[j];
+            re += (double)cosf(kr);
+            im += (double)sinf(kr);
         }
+        rho[2*v] = re;
+        rho[2*v+1] = im;
     }
     free(kk);
 }
@@ -711,13 +716,16 @@ int orc_ref32_fs_pair(int N, const float *x, const float *y, const float *z,
     for (int v = 0; v < K; v++) kvec32(nvec + 3*v, spacing, kk + 3*v);
     size_t o0 = (size_t)t0 * N, o1 = (size_t)t1 * N;
     const float *b0 = box + 3*t0, *b1 = box + 3*t1;
-    for (int a = 0; a < cnt; a++) {
-        int j = atoms ? atoms[a] : a;
-        float dx = unwrap32(x[o1+j], ix[o1+j], b1[0]) - unwrap32(x[o0+j], ix[o0+j], b0[0]);
-        float dy = unwrap32(y[o1+j], iy[o1+j], b1[1]) - unwrap32(y[o0+j], iy[o0+j], b0[1]);
-        float dz = unwrap32(z[o1+j], iz[o1+j], b1[2]) - unwrap32(z[o0+j], iz[o0+j], b0[2]);
-        for (int s = 0; s < n_shells; s++) {
-            int H = shell_off[s+1] - shell_off[s];
+    /* Each shell's sum runs over the atoms in list order, as in the
+     * reference's atom-outer loop; threads take shells. */
+#pragma omp parallel for schedule(dynamic) num_threads(g_threads)
+    for (int s = 0; s < n_shells; s++) {
+        int H = shell_off[s+1] - shell_off[s];
+        for (int a = 0; a < cnt; a++) {
+            int j = atoms ? atoms[a] : a;
+            float dx = unwrap32(x[o1+j], ix[o1+j], b1[0]) - unwrap32(x[o0+j], ix[o0+j], b0[0]);
+            float dy = unwrap32(y[o1+j], iy[o1+j], b1[1]) - unwrap32(y[o0+j], iy[o0+j], b0[1]);
+            float dz = unwrap32(z[o1+j], iz[o1+j], b1[2]) - unwrap32(z[o0+j], iz[o0+j], b0[2]);
             double t = 0.0;
             for (int v = shell_off[s]; v < shell_off[s+1]; v++) {
                 float kr = kk[3*v] * dx + kk[3*v+1] * dy + kk[3*v+2] * dz;

@@ -330,7 +330,8 @@ def ref32_fs(tr, pairs, ntg, shell_off, nvec, length_scale, atoms=None):
         corr[g] = row
         for _ in range(n):
             natoms[g] = np.float32(natoms[g] + np.float32(1.0))
-    return corr / natoms[:, None], natoms
+    with np.errstate(invalid="ignore", divide="ignore"):   # time gaps without pairs: 0/0 as in the reference
+        return corr / natoms[:, None], natoms
 
 
 # ---- S(k) misfit loss and gradient ----
