@@ -61,6 +61,7 @@ SIGNATURES = {
     "amdat_list_create": (_I, [_P, _P, _I, _PI, _PP]),
     "amdat_list_free": (_I, [_P, _P]),
     "amdat_list_create_timed": (_I, [_P, _P, _I, _PI, _PI, _PI, _PP]),
+    "amdat_list_create_paired": (_I, [_P, _P, _I, _PI, _PI, _PI, _PI, _PP]),
     "amdat_list_shape": (_I, [_P, _PI, _PI]),
     "amdat_list_download": (_I, [_P, _P, _PI, _PI, _PI]),
     "amdat_list_select_displacement": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _F, _F, _PP]),
@@ -409,6 +410,22 @@ class TimedList(_Handle):
         idx = np.zeros(max(self.n, 1), np.int32)
         check(lib().amdat_list_download(self.ctx.h, self.h, ip(conv), ip(off), ip(idx)))
         return conv, off, idx[:self.n]
+
+
+class PairList(_Handle):
+    """Pair-keyed atom list: the atoms indices[off[c]:off[c+1]] apply to
+    the frame pair (t0[c], t1[c]); pairs without a key have no atoms."""
+    _free = "amdat_list_free"
+
+    def __init__(self, ctx, traj, t0, t1, off, indices):
+        t0, t1, off, idx = (np.ascontiguousarray(a, np.int32) for a in (t0, t1, off, indices))
+        assert len(t0) == len(t1) == len(off) - 1
+        h = C.c_void_p()
+        check(lib().amdat_list_create_paired(ctx.h, traj.h, len(t0), ip(t0), ip(t1), ip(off), ip(idx), C.byref(h)))
+        super().__init__(ctx, h)
+        nt, n = C.c_int(), C.c_int()
+        check(lib().amdat_list_shape(self.h, C.byref(nt), C.byref(n)))
+        self.n_keys, self.n = nt.value, n.value
 
 
 class Pairs(_Handle):

@@ -12,6 +12,7 @@
 #include <numeric>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/amdat_hip.h"
@@ -106,6 +107,11 @@ struct amdat_list {
     // is a static list.
     int n_times = 0;
     std::vector<int> conv, off;
+    // Pair-keyed lists (src/bin_dynamics_analysis.h:120-141): list time c
+    // belongs to the frame pair (t0, t1) that pkey maps to c, and conv is
+    // empty; a pair with no key has no atoms.
+    bool paired = false;
+    std::unordered_map<int64_t, int> pkey;
 };
 
 struct KPlanOwned {
@@ -377,6 +383,22 @@ ListSeg list_seg(const amdat_traj *tr, const amdat_list *l, int c)
 }
 ListSeg list_at(const amdat_traj *tr, const amdat_list *l, int t) { return list_seg(tr, l, list_time(l, t)); }
 
+// List time of the frame pair (t0, t1): the pair's key for a pair-keyed
+// list, else the list time of t0 (src/trajectory_list.cpp:202-211).
+int64_t pair_key(int t0, int t1) { return ((int64_t)t0 << 32) | (uint32_t)t1; }
+int pair_time(const amdat_list *l, int t0, int t1)
+{
+    if (!l || !l->paired) return list_time(l, t0);
+    const auto it = l->pkey.find(pair_key(t0, t1));
+    return it == l->pkey.end() ? -1 : it->second;
+}
+ListSeg list_at_pair(const amdat_traj *tr, const amdat_list *l, int t0, int t1)
+{
+    return list_seg(tr, l, pair_time(l, t0, t1));
+}
+// Entry points over frames take lists per frame: static or time-varying.
+bool frame_list_ok(const amdat_list *l) { return !l || !l->paired; }
+
 // Pairs regrouped by the list time of their t0, for per-list-time launches:
 // pos[i] is pair i's row in the regrouped order (stable within a list
 // time), seg[g] = {list time, first row, row count} per group, and
@@ -392,7 +414,7 @@ PairGroups group_pairs(const amdat_list *l, const amdat_pairs *pairs)
     std::vector<int> ord(n);
     std::iota(ord.begin(), ord.end(), 0);
     std::vector<int> key(n);
-    for (int i = 0; i < n; i++) key[i] = list_time(l, pairs->t0[i]);
+    for (int i = 0; i < n; i++) key[i] = pair_time(l, pairs->t0[i], pairs->t1[i]);
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
     g.pos.resize(n);
     g.t0.resize(n);
@@ -416,7 +438,7 @@ PairGroups group_pairs(const amdat_list *l, const amdat_pairs *pairs)
 // reduction's reference-order positions to rows.  A static list is one
 // group over the pairs in their own order; a time-varying list is one group
 // per list time (src/trajectory_list.cpp:202-211 takes the list at the
-// list time of t0).  count[g] receives the atom-pair count of each time gap
+// list time of t0), a pair-keyed list one group per pair.  count[g] receives the atom-pair count of each time gap
 // (src/mean_square_displacement.cpp:157; src/incoherent_scattering_
 // function.cpp:283).
 struct PairLaunch {
@@ -435,7 +457,8 @@ int plan_pair_launch(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *l, 
 {
     hipStream_t st = stream_of(ctx);
     for (int g = 0; g < pairs->ntg; g++) count[g] = 0.0;
-    for (int i = 0; i < pairs->n; i++) count[pairs->tg[i]] += (double)list_at(tr, l, pairs->t0[i]).n;
+    for (int i = 0; i < pairs->n; i++)
+        count[pairs->tg[i]] += (double)list_at_pair(tr, l, pairs->t0[i], pairs->t1[i]).n;
     if (!list_timed(l)) {
         const ListSeg sg = list_seg(tr, l, 0);
         pl.groups.push_back({sg, 0, pairs->n});
@@ -795,6 +818,37 @@ int amdat_list_create_timed(amdat_ctx *ctx, const amdat_traj *tr, int n_times, c
     return AMDAT_OK;
 }
 
+int amdat_list_create_paired(amdat_ctx *ctx, const amdat_traj *tr, int n_keys, const int *t0, const int *t1,
+                             const int *off, const int *indices, amdat_list **out)
+{
+    clear_error();
+    if (!ctx || !tr || !out || n_keys < 1 || !t0 || !t1 || !off)
+        return set_error(AMDAT_EINVAL, "amdat_list_create_paired: bad arguments");
+    if (off[0] != 0) return set_error(AMDAT_EINVAL, "amdat_list_create_paired: off[0] must be 0");
+    auto l = std::make_unique<amdat_list>();
+    for (int c = 0; c < n_keys; c++) {
+        if (off[c + 1] < off[c]) return set_error(AMDAT_EINVAL, "amdat_list_create_paired: offsets decrease");
+        if (t0[c] < 0 || t0[c] >= tr->dev.T || t1[c] < 0 || t1[c] >= tr->dev.T)
+            return set_error(AMDAT_ERANGE, "amdat_list_create_paired: frame out of range");
+        if (!l->pkey.emplace(pair_key(t0[c], t1[c]), c).second)
+            return set_error(AMDAT_EINVAL, "amdat_list_create_paired: frame pair given twice");
+    }
+    const int n = off[n_keys];
+    if (n > 0 && !indices) return set_error(AMDAT_EINVAL, "amdat_list_create_paired: null indices");
+    for (int a = 0; a < n; a++)
+        if (indices[a] < 0 || indices[a] >= tr->dev.N)
+            return set_error(AMDAT_ERANGE, "amdat_list_create_paired: atom index out of range");
+    HIP_TRY(hipSetDevice(ctx->device));
+    l->n = n;
+    l->n_atoms = tr->dev.N;
+    l->n_times = n_keys;
+    l->paired = true;
+    l->off.assign(off, off + n_keys + 1);
+    if (n > 0) HIP_TRY(l->idx.upload(indices, (size_t)n, stream_of(ctx)));
+    *out = l.release();
+    return AMDAT_OK;
+}
+
 int amdat_list_shape(const amdat_list *l, int *n_times, int *n_entries)
 {
     if (!l) return set_error(AMDAT_EINVAL, "amdat_list_shape: null list");
@@ -808,7 +862,7 @@ int amdat_list_download(amdat_ctx *ctx, const amdat_list *l, int *time_conv, int
     clear_error();
     if (!ctx || !l) return set_error(AMDAT_EINVAL, "amdat_list_download: null argument");
     if (list_timed(l)) {
-        if (time_conv) std::copy(l->conv.begin(), l->conv.end(), time_conv);
+        if (time_conv && !l->paired) std::copy(l->conv.begin(), l->conv.end(), time_conv);
         if (off) std::copy(l->off.begin(), l->off.end(), off);
     }
     if (indices && l->n > 0) {
@@ -837,6 +891,8 @@ int amdat_list_select_displacement(amdat_ctx *ctx, const amdat_traj *tr, const a
         return set_error(AMDAT_EINVAL, "amdat_list_select_displacement: unknown mode");
     if (in && in->n_atoms != tr->dev.N)
         return set_error(AMDAT_EINVAL, "amdat_list_select_displacement: list built for another trajectory size");
+    if (!frame_list_ok(in))
+        return set_error(AMDAT_EINVAL, "amdat_list_select_displacement: a pair-keyed list applies to frame pairs only");
     const long T_scheme = (long)nes * nexp + 1;
     if (T_scheme > tr->dev.T)
         return set_error(AMDAT_ERANGE, "amdat_list_select_displacement: trajectory shorter than the time scheme");
@@ -969,6 +1025,8 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
     if (first_shell < 0 || last_shell < first_shell || last_shell >= sh->n_shells)
         return set_error(AMDAT_ERANGE, "amdat_rho_compute: shell range outside the table");
     if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_rho_compute: length scale must be > 0");
+    if (!frame_list_ok(list))
+        return set_error(AMDAT_EINVAL, "amdat_rho_compute: a pair-keyed list applies to frame pairs only");
     if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64 && precision != AMDAT_PREC_FP32X)
         return set_error(AMDAT_EINVAL, "amdat_rho_compute: unknown precision");
     HIP_TRY(hipSetDevice(ctx->device));

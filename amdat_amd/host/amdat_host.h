@@ -79,12 +79,17 @@ class Trajectory_List {
     Trajectory_List(System *sys, std::vector<int> atoms);
     // Takes ownership of a time-varying device list.
     Trajectory_List(System *sys, amdat_list *timed);
+    // Takes ownership of a pair-keyed device list (amdat_list_create_paired):
+    // the analyses over frame pairs take it; it has no atoms per frame.
+    struct PairKeyed {};
+    Trajectory_List(System *sys, amdat_list *paired, PairKeyed);
     ~Trajectory_List();
     Trajectory_List(const Trajectory_List &) = delete;
     Trajectory_List &operator=(const Trajectory_List &) = delete;
     // Atoms at frame timeii (src/trajectory_list.h:58).
     int show_n_trajectories(int timeii) const;
     bool time_varying() const { return n_times_ > 0; }
+    bool pair_keyed() const { return pair_keyed_; }
     // Every list time's atoms, concatenated in list-time order.
     const std::vector<int> &atoms() const { return atoms_; }
     // The atoms at frame timeii, in list order.
@@ -100,6 +105,7 @@ class Trajectory_List {
     System *system_;
     std::vector<int> atoms_;
     int n_times_ = 0;
+    bool pair_keyed_ = false;
     std::vector<int> conv_, off_;
     amdat_list *list_ = nullptr;
 };
@@ -121,6 +127,12 @@ class Trajectory_List_Bins {
     // order (Trajectory_List::operator&&, src/trajectory_list.cpp:518-):
     // a time-varying list with one list time per frame.
     std::unique_ptr<Trajectory_List> intersect(const Trajectory_List *l, int xii, int yii, int zii) const;
+    // Per bin, in bin order ((x * ny + y) * nz + z), the pair-keyed list of
+    // Bin_Dynamics_Analysis (src/bin_dynamics_analysis.h:120-141): for
+    // each frame pair of System::displacement_list(analysis) (fullblock 0,
+    // with limit), the atoms of l at t0 that lie in the bin at t0 and at t1,
+    // in atom order.
+    std::vector<std::unique_ptr<Trajectory_List>> intersect_pairs(const Trajectory_List *l) const;
 
   private:
     void assign(const std::vector<float> &lo, const std::vector<float> &hi);
@@ -169,6 +181,9 @@ class System {
     amdat_pairs *gap_pairs(int time_index) const;
     amdat_pairs *displacement_pairs(bool fullblock, std::vector<int> *tg = nullptr,
                                     std::vector<int> *t0 = nullptr) const;
+    // The same pairs on the host only.
+    void displacement_pair_list(bool fullblock, std::vector<int> &tg, std::vector<int> &t0,
+                                std::vector<int> &t1) const;
 
     // Atom selections (create_list keywords all / type_system / species /
     // type_species); indices in the reference's list order.
@@ -495,9 +510,13 @@ class Control {
     // Run fn(file suffix, list) for a "list <name>" target (suffix "") or,
     // for a "bin_list <bins> <list> [persistence]" target, once per bin
     // with the suffix Bin_Static_Analysis::write forms (src/bin_static_
-    // analysis.h:114-140).
+    // analysis.h:114-140).  For a dynamic analysis with persistence 1
+    // (src/control.h:353-366) each bin's list is pair-keyed, as
+    // Bin_Dynamics_Analysis forms it, and the suffix is its write's
+    // (src/bin_dynamics_analysis.h:145-175).
     void for_each_target(const std::string &line,
-                         const std::function<void(const std::string &, Trajectory_List *)> &fn);
+                         const std::function<void(const std::string &, Trajectory_List *)> &fn,
+                         bool dynamic = false);
     void create_bin_list(const std::vector<std::string> &args);
 
     std::vector<std::string> lines_;

@@ -75,29 +75,37 @@ struct RewoundIndex {
 }  // namespace
 
 void Control::for_each_target(const std::string &line,
-                              const std::function<void(const std::string &, Trajectory_List *)> &fn)
+                              const std::function<void(const std::string &, Trajectory_List *)> &fn,
+                              bool dynamic)
 {
     auto t = tokenize(line);
     if (!t.empty() && t[0] == "bin_list") {
-        // src/control.h:340-380: bin_list <bin list> <list> [persistence]
+        // src/control.h:340-380: bin_list <bin list> <list> [persistence];
+        // persistence counts for the dynamic analyses only (src/control.h:362)
         if (t.size() < 3) throw Error("Insufficient number of arguments in bin_list target line.", 0);
-        if (t.size() == 4 && std::atoi(t[3].c_str()) != 0)
-            throw Error("bin_list persistence (Bin_Dynamics_Analysis) is not supported by this build", 1);
+        const bool persist = dynamic && t.size() == 4 && std::atoi(t[3].c_str()) != 0;
         const Trajectory_List_Bins *bins = nullptr;
         for (auto &p : bin_lists_)
             if (p.first == t[1]) bins = p.second.get();
         if (!bins) throw Error("Bin list '" + t[1] + "' not found.", 0);
         Trajectory_List *l = target("list " + t[2]);
+        std::vector<std::unique_ptr<Trajectory_List>> pair_lists;
+        if (persist) pair_lists = bins->intersect_pairs(l);
+        const char *sep = persist ? "." : "_";
         RewoundIndex sx, sy, sz;
         for (int x = 0; x < bins->show_n_xbins(); x++)
             for (int y = 0; y < bins->show_n_ybins(); y++)
                 for (int z = 0; z < bins->show_n_zbins(); z++) {
                     std::cout << "Bin (" << x << "," << y << "," << z << ")" << "\r";
                     std::cout.flush();
-                    auto bl = bins->intersect(l, x, y, z);
                     const std::string suffix =
-                        "_bin_" + sx.put(x) + "_" + sy.put(y) + "_" + sz.put(z) + ".bindata";
-                    fn(suffix, bl.get());
+                        "_bin_" + sx.put(x) + sep + sy.put(y) + sep + sz.put(z) + ".bindata";
+                    if (persist) {
+                        fn(suffix, pair_lists[(x * bins->show_n_ybins() + y) * bins->show_n_zbins() + z].get());
+                    } else {
+                        auto bl = bins->intersect(l, x, y, z);
+                        fn(suffix, bl.get());
+                    }
                 }
         std::cout << std::endl;
         return;
@@ -157,11 +165,14 @@ void Control::msd(const std::vector<std::string> &args)
     const std::string runline = read_line();
     std::cout << "\nCalculating mean square displacement.\n";
     const time_t start = time(nullptr);
-    for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
-        Mean_Square_Displacement m(system_.get());
-        m.analyze(l);
-        m.write(args[1] + suffix);
-    });
+    for_each_target(
+        runline,
+        [&](const std::string &suffix, Trajectory_List *l) {
+            Mean_Square_Displacement m(system_.get());
+            m.analyze(l);
+            m.write(args[1] + suffix);
+        },
+        true);
     std::cout << "\nCalculated mean square displacement in " << time(nullptr) - start << " seconds." << std::endl;
 }
 
@@ -172,11 +183,14 @@ void Control::msd_2d(const std::vector<std::string> &args)
     const std::string runline = read_line();
     std::cout << "\nCalculating mean square displacement.";
     const time_t start = time(nullptr);
-    for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
-        Mean_Square_Displacement_2D m(system_.get(), args[2]);
-        m.analyze(l);
-        m.write(args[1] + suffix);
-    });
+    for_each_target(
+        runline,
+        [&](const std::string &suffix, Trajectory_List *l) {
+            Mean_Square_Displacement_2D m(system_.get(), args[2]);
+            m.analyze(l);
+            m.write(args[1] + suffix);
+        },
+        true);
     std::cout << "\nCalculated mean square displacement in " << time(nullptr) - start << " seconds." << std::endl;
 }
 
@@ -187,11 +201,14 @@ void Control::mean_displacement(const std::vector<std::string> &args)
     const std::string runline = read_line();
     std::cout << "\nCalculating mean displacement.\n";
     const time_t start = time(nullptr);
-    for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
-        Mean_Displacement m(system_.get());
-        m.analyze(l);
-        m.write(args[1] + suffix);
-    });
+    for_each_target(
+        runline,
+        [&](const std::string &suffix, Trajectory_List *l) {
+            Mean_Displacement m(system_.get());
+            m.analyze(l);
+            m.write(args[1] + suffix);
+        },
+        true);
     std::cout << "\nCalculated mean displacement in " << time(nullptr) - start << " seconds.";
 }
 
@@ -323,11 +340,16 @@ void Control::isfs(const std::vector<std::string> &args)
     const std::string runline = read_line();
     const time_t start = time(nullptr);
     Wave_Vectors wv(system_.get(), args[4], ell);
-    for_each_target(runline, [&](const std::string &suffix, Trajectory_List *l) {
-        Incoherent_Scattering_Function f(system_.get(), &wv, inner, outer, fb);
-        f.analyze(l);
-        f.write(args[1] + suffix);
-    });
+    for_each_target(
+        runline,
+        [&](const std::string &suffix, Trajectory_List *l) {
+            // Bin_Dynamics_Analysis::analyze loops the pairs of
+            // displacement_list(this), fullblock 0 (src/bin_dynamics_analysis.h:99)
+            Incoherent_Scattering_Function f(system_.get(), &wv, inner, outer, l->pair_keyed() ? false : fb);
+            f.analyze(l);
+            f.write(args[1] + suffix);
+        },
+        true);
     std::cout << "\nSelf Intermediate Scattering Function calculated in " << time(nullptr) - start
               << " seconds.\n";
 }

@@ -69,10 +69,16 @@ Trajectory_List::Trajectory_List(System *sys, amdat_list *timed) : system_(sys),
           "trajectory list");
 }
 
+Trajectory_List::Trajectory_List(System *sys, amdat_list *paired, PairKeyed)
+    : system_(sys), pair_keyed_(true), list_(paired)
+{
+}
+
 Trajectory_List::~Trajectory_List() { amdat_list_free(device_context(), list_); }
 
 int Trajectory_List::show_n_trajectories(int timeii) const
 {
+    if (pair_keyed_) throw Error("a pair-keyed list has no atoms per frame", 0);
     if (n_times_ == 0) return (int)atoms_.size();
     if (timeii < 0 || timeii >= (int)conv_.size()) return 0;
     const int c = conv_[timeii];
@@ -81,6 +87,7 @@ int Trajectory_List::show_n_trajectories(int timeii) const
 
 std::vector<int> Trajectory_List::atoms_at(int timeii) const
 {
+    if (pair_keyed_) throw Error("a pair-keyed list has no atoms per frame", 0);
     if (n_times_ == 0) return atoms_;
     if (timeii < 0 || timeii >= (int)conv_.size()) return {};
     const int c = conv_[timeii];
@@ -154,6 +161,42 @@ std::unique_ptr<Trajectory_List> Trajectory_List_Bins::intersect(const Trajector
                                   idx.empty() ? nullptr : idx.data(), &h),
           "bin list");
     return std::make_unique<Trajectory_List>(system_, h);
+}
+
+std::vector<std::unique_ptr<Trajectory_List>> Trajectory_List_Bins::intersect_pairs(const Trajectory_List *l) const
+{
+    const int N = system_->show_n_atoms();
+    const int nb = n_[0] * n_[1] * n_[2];
+    std::vector<int> tg, t0, t1;
+    system_->displacement_pair_list(false, tg, t0, t1);
+    // One key per distinct frame pair; membership depends on the frames only.
+    std::vector<std::pair<int, int>> keys;
+    for (size_t i = 0; i < t0.size(); i++) keys.emplace_back(t0[i], t1[i]);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int nk = (int)keys.size();
+    std::vector<std::vector<int>> idx(nb), off(nb, std::vector<int>(nk + 1, 0));
+    for (int k = 0; k < nk; k++) {
+        const int a = keys[k].first, b = keys[k].second;
+        std::vector<int> at = l->atoms_at(a);
+        std::sort(at.begin(), at.end());
+        const int *ca = cell_.data() + (size_t)a * N, *cb = cell_.data() + (size_t)b * N;
+        for (int j : at)
+            if (ca[j] >= 0 && ca[j] == cb[j]) idx[ca[j]].push_back(j);
+        for (int c = 0; c < nb; c++) off[c][k + 1] = (int)idx[c].size();
+    }
+    std::vector<int> k0(nk), k1(nk);
+    for (int k = 0; k < nk; k++) k0[k] = keys[k].first, k1[k] = keys[k].second;
+    std::vector<std::unique_ptr<Trajectory_List>> out;
+    for (int c = 0; c < nb; c++) {
+        amdat_list *h = nullptr;
+        check(amdat_list_create_paired(device_context(), system_->traj(), nk, k0.data(), k1.data(), off[c].data(),
+                                       idx[c].empty() ? nullptr : idx[c].data(), &h),
+              "bin list");
+        out.push_back(std::make_unique<Trajectory_List>(system_, h, Trajectory_List::PairKeyed{}));
+        std::vector<int>().swap(idx[c]);
+    }
+    return out;
 }
 
 void Trajectory_List::write_count(const std::string &filename) const
@@ -386,6 +429,21 @@ amdat_pairs *System::gap_pairs(int time_index) const
     amdat_pairs *p = nullptr;
     check(amdat_pairs_upload(device_context(), n, tg.data(), t0.data(), t1.data(), n_timegaps_, &p), "pair upload");
     return p;
+}
+
+void System::displacement_pair_list(bool fullblock, std::vector<int> &tg, std::vector<int> &t0,
+                                    std::vector<int> &t1) const
+{
+    int n = 0;
+    check(amdat_ts_pairs(n_exponentials_, n_exponential_steps_, frt_, fullblock, displacement_limit_, nullptr, nullptr,
+                         nullptr, 0, &n),
+          "pairs");
+    tg.resize(n);
+    t0.resize(n);
+    t1.resize(n);
+    check(amdat_ts_pairs(n_exponentials_, n_exponential_steps_, frt_, fullblock, displacement_limit_, tg.data(),
+                         t0.data(), t1.data(), n, &n),
+          "pairs");
 }
 
 amdat_pairs *System::displacement_pairs(bool fullblock, std::vector<int> *tg_out, std::vector<int> *t0_out) const

@@ -238,11 +238,24 @@ int amdat_list_free(amdat_ctx *ctx, amdat_list *l);
  * entry points take static lists only. */
 int amdat_list_create_timed(amdat_ctx *ctx, const amdat_traj *tr, int n_times, const int *time_conv,
                             const int *off, const int *indices, amdat_list **out);
-/* n_times (0 for a static list) and total entries. */
+/* Pair-keyed atom selection: the atoms of key c, indices[off[c] ..
+ * off[c+1]-1], apply to the frame pair (t0[c], t1[c]); a frame pair with
+ * no key has no atoms, and no pair may be given twice.  It replaces the
+ * per-pair list of Bin_Dynamics_Analysis::list_displacementkernel
+ * (src/bin_dynamics_analysis.h:120-141: the list's atoms at t0 that lie
+ * in one bin at both t0 and t1, Trajectory_List::time_intersection,
+ * src/trajectory_list.cpp:252-277), handed to each analysis's bin_hook.
+ * Fs, MSD/NGP and the displacement statistics take it, with the counts per
+ * time gap summed over the pairs' own lists; rho, S(k), the displacement
+ * selection and the batched entry points reject it (AMDAT_EINVAL). */
+int amdat_list_create_paired(amdat_ctx *ctx, const amdat_traj *tr, int n_keys, const int *t0, const int *t1,
+                             const int *off, const int *indices, amdat_list **out);
+/* n_times (0 for a static list, the key count for a pair-keyed one) and
+ * total entries. */
 int amdat_list_shape(const amdat_list *l, int *n_times, int *n_entries);
-/* Copy a list to the host: time_conv[n_frames] and off[n_times+1] (time-
- * varying lists; ignored for static ones) and indices[n_entries]; any
- * output may be NULL. */
+/* Copy a list to the host: time_conv[n_frames] (time-varying lists) and
+ * off[n_times+1] (time-varying and pair-keyed lists; both ignored for
+ * static ones) and indices[n_entries]; any output may be NULL. */
 int amdat_list_download(amdat_ctx *ctx, const amdat_list *l, int *time_conv, int *off, int *indices);
 
 /* Atoms selected by squared displacement at one time gap, per exponential

@@ -2,7 +2,10 @@
 device: cell assignment bit-exact with the oracle, box bounds read from a
 custom file, and the drop-in's per-bin msd, isfs and structure_factor files
 (Bin_Static_Analysis names, quirk included) against the oracle composed
-bin by bin.  Plain form N = 1,000 in a box [-4.5, 4.91)^3, exponential 5 20."""
+bin by bin.  The persistence form (Bin_Dynamics_Analysis: per frame pair,
+the atoms in the bin at both frames) through pair-keyed lists, at the C ABI
+and in the drop-in's files, against the oracle composed pair by pair.
+Plain form N = 1,000 in a box [-4.5, 4.91)^3, exponential 5 20."""
 import os
 import subprocess
 
@@ -65,6 +68,89 @@ def bin_lists(tr, cells, nb):
     return out
 
 
+def pair_bin_lists(cells, nb, pairs):
+    """Per bin: over the distinct frame pairs (t0, t1), the atoms in the bin
+    at t0 and at t1 (atom order), as (t0, t1, off, indices)."""
+    _, t0, t1 = pairs
+    keys = sorted(set(zip(np.asarray(t0).tolist(), np.asarray(t1).tolist())))
+    k0 = np.array([k[0] for k in keys], np.int32)
+    k1 = np.array([k[1] for k in keys], np.int32)
+    out = {}
+    for b in range(int(np.prod(nb))):
+        off, idx = [0], []
+        for a, c in keys:
+            idx.extend(np.nonzero((cells[a] == b) & (cells[c] == b))[0])
+            off.append(len(idx))
+        out[b] = (k0, k1, np.array(off, np.int32), np.array(idx, np.int32))
+    return out
+
+
+def per_pair(plist, pairs, fn):
+    """fn(one pair, its atoms) for every pair with atoms."""
+    k0, k1, off, idx = plist
+    key = {(a, c): i for i, (a, c) in enumerate(zip(k0.tolist(), k1.tolist()))}
+    for g, a, c in zip(*(np.asarray(v).tolist() for v in pairs)):
+        i = key[(a, c)]
+        if off[i + 1] > off[i]:
+            yield fn((np.array([g]), np.array([a]), np.array([c])), idx[off[i]:off[i + 1]])
+
+
+def test_pair_keyed_lists(ctx, qvec):
+    print("case: pair-keyed bin lists at the C ABI, 2x2x2 bins", flush=True)
+    tr = shifted()
+    d = _abi.Trajectory(ctx, *tr.arrays)
+    lo, hi = bounds(tr)
+    cells = orc.bins_assign(tr, (2, 2, 2), lo, hi)
+    pairs = orc.ts_pairs(NEXP, NES)
+    P = _abi.Pairs(ctx, *pairs, NTG)
+    o22, n22 = ref_shells(qvec, 22, 22)
+    sh = _abi.Shells(ctx, o22, n22)
+    ell = np.float32(9.41)
+    plists = pair_bin_lists(cells, (2, 2, 2), pairs)
+    static = bin_lists(tr, cells, (2, 2, 2))
+    for b in (0, 5):
+        pl = _abi.PairList(ctx, d, *plists[b])
+        assert pl.n_keys == len(plists[b][0]) and pl.n == len(plists[b][3])
+        s2, s4, cnt = _abi.msd_ngp(ctx, d, pl, P)
+        R2, R4, RC = np.zeros(NTG), np.zeros(NTG), np.zeros(NTG)
+        for m in per_pair(plists[b], pairs, lambda p, at: orc.msd_ngp_f64(tr, p, NTG, atoms=at)):
+            R2 += m["s2"]
+            R4 += m["s4"]
+            RC += m["count"]
+        assert np.array_equal(cnt, RC)
+        assert np.allclose(s2, R2, rtol=1e-12, atol=1e-12) and np.allclose(s4, R4, rtol=1e-12, atol=1e-12)
+        # atoms that stay in the bin are a subset of those in it at t0; at lag 0 they are the same
+        sc = np.zeros(NTG)
+        for m in compose_pairs(static[b], pairs, lambda p, at: orc.msd_ngp_f64(tr, p, NTG, atoms=at)):
+            sc += m["count"]
+        assert np.all(cnt <= sc) and cnt[0] == sc[0] and np.any(cnt < sc)
+        F, fc = _abi.fs(ctx, d, pl, sh, 0, 0, ell, P)
+        SF, SC = np.zeros((NTG, 1)), np.zeros(NTG)
+        for f, c in per_pair(plists[b], pairs, lambda p, at: orc.fs_f64(tr, p, NTG, o22, n22, float(ell), at)):
+            SF += f * c[:, None]
+            SC += c
+        assert np.array_equal(fc, SC)
+        ok = SC > 0
+        assert np.max(np.abs(F[ok] - SF[ok] / SC[ok, None])) < 1e-6
+        sd, sd2, dc, hist = _abi.displacements(ctx, d, pl, P, n_bins=40, max_range=1.0)
+        H, D2 = np.zeros((NTG, 40), np.uint64), np.zeros((NTG, 3))
+        for h, _ in per_pair(plists[b], pairs, lambda p, at: orc.vhs_counts(tr, p, NTG, 40, 1.0, at)):
+            H += h
+        for _, b2, _ in per_pair(plists[b], pairs, lambda p, at: orc.disp_f64(tr, p, NTG, at)):
+            D2 += b2
+        assert np.array_equal(hist, H) and np.array_equal(dc, RC)
+        assert np.allclose(sd2, D2, rtol=1e-12, atol=1e-12)
+        # frame entry points reject a pair-keyed list
+        with pytest.raises(_abi.AmdatError):
+            _abi.Rho(ctx, d, pl, sh, 0, 0, ell)
+        pl.close()
+    k0, k1, off, idx = plists[0]
+    with pytest.raises(_abi.AmdatError):                       # a frame pair given twice
+        _abi.PairList(ctx, d, np.r_[k0, k0[:1]], np.r_[k1, k1[:1]], np.r_[off, off[-1]], idx)
+    with pytest.raises(_abi.AmdatError):                       # frame out of range
+        _abi.PairList(ctx, d, np.r_[k0[:-1], tr.T], k1, off, idx)
+
+
 @pytest.fixture(scope="module")
 def run(tmp_path_factory, qvec):
     d = tmp_path_factory.mktemp("bins")
@@ -96,6 +182,12 @@ isfs isfs.dat 20 20 xyz 9.41 0
 bin_list grid all
 structure_factor sk.dat symmetric xyz 0 -1
 bin_list grid all
+msd msdp.dat
+bin_list grid all 1
+isfs isfsp.dat 20 20 xyz 9.41 1
+bin_list grid all 1
+structure_factor skp.dat symmetric xyz 0 -1
+bin_list grid all 1
 """ % (NEXP, NES)
     (d / "in.amdat").write_text(script)
     env = dict(os.environ, AMDAT_WV3D=str(qdir / "qvector"))
@@ -105,16 +197,16 @@ bin_list grid all
     assert p.returncode == 0, p.stdout[-3000:] + p.stderr[-3000:]
     lo, hi = bounds(tr)
     cells = orc.bins_assign(tr, (2, 2, 2), lo, hi)
-    return d, tr, bin_lists(tr, cells, (2, 2, 2))
+    return d, tr, bin_lists(tr, cells, (2, 2, 2)), cells
 
 
-def name(stem, b):
+def name(stem, b, sep="_"):
     x, y, z = b // 4, (b // 2) % 2, b % 2
-    return "%s_bin_%d_%d_%d.bindata" % (stem, x, y, z)
+    return "%s_bin_%d%s%d%s%d.bindata" % (stem, x, sep, y, sep, z)
 
 
 def test_bin_msd_files(run):
-    d, tr, lists = run
+    d, tr, lists, _ = run
     pairs = orc.ts_pairs(NEXP, NES)
     for b, lst in lists.items():
         s2, cnt = np.zeros(NTG), np.zeros(NTG)
@@ -128,7 +220,7 @@ def test_bin_msd_files(run):
 
 
 def test_bin_isfs_and_sk_files(run, qvec):
-    d, tr, lists = run
+    d, tr, lists, _ = run
     pairs = orc.ts_pairs(NEXP, NES)
     o22, n22 = ref_shells(qvec, 22, 22)
     offA, nA = ref_shells(qvec, 2, 100)
@@ -156,7 +248,37 @@ def test_bin_isfs_and_sk_files(run, qvec):
 
 
 def test_bounds_from_custom_file(ctx, run):
-    d, tr, lists = run
+    d, tr, lists, _ = run
     t, _ = _abi.Trajectory.read(ctx, "custom", str(d / "traj.dump"), tr.N, tr.T)
     lo = _abi.traj_bounds(t)
     assert np.all(lo == np.float32(LO))
+
+
+def test_bin_persistence_files(run, qvec):
+    """bin_list <bins> <list> 1: msd and isfs per bin over the atoms that
+    stay in the bin (fullblock 0 even where isfs asks for 1); Bin_Dynamics_
+    Analysis names; structure_factor is not dynamic and ignores the flag."""
+    d, tr, lists, cells = run
+    pairs = orc.ts_pairs(NEXP, NES)
+    plists = pair_bin_lists(cells, (2, 2, 2), pairs)
+    o22, n22 = ref_shells(qvec, 22, 22)
+    ell = float(np.float32(9.41))
+    for b, pl in plists.items():
+        s2, cnt = np.zeros(NTG), np.zeros(NTG)
+        SF, SC = np.zeros((NTG, 1)), np.zeros(NTG)
+        for m in per_pair(pl, pairs, lambda p, at: orc.msd_ngp_f64(tr, p, NTG, atoms=at)):
+            s2 += m["s2"]
+            cnt += m["count"]
+        for f, c in per_pair(pl, pairs, lambda p, at: orc.fs_f64(tr, p, NTG, o22, n22, ell, at)):
+            SF += f * c[:, None]
+            SC += c
+        rows = np.array([[float(v) for v in l.split("\t")] for l in
+                         (d / name("msdp.dat", b, ".")).read_text().split("\n")[1:] if l.strip()])
+        ok = cnt > 0
+        assert ok[0] and np.any(ok[1:])
+        assert np.allclose(rows[1:, 1][ok[1:]], (s2 / np.where(ok, cnt, 1))[1:][ok[1:]], rtol=RTOL)
+        lines = (d / name("isfsp.dat", b, ".")).read_text().split("\n")
+        rows = np.array([[float(v) for v in l.strip("\t").split("\t")] for l in lines[2:] if l.strip()])
+        ok = SC > 0
+        assert np.allclose(rows[ok, 1], SF[ok, 0] / SC[ok], rtol=RTOL, atol=1e-6)
+        assert (d / name("skp.dat", b)).read_text() == (d / name("sk.dat", b)).read_text()
