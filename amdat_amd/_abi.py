@@ -58,6 +58,7 @@ SIGNATURES = {
     "amdat_traj_shape": (_I, [_P, _PI, _PI, _PF]),
     "amdat_traj_bounds": (_I, [_P, _PF]),
     "amdat_bins_assign": (_I, [_P, _P, _I, _I, _I, _PF, _PF, _PI]),
+    "amdat_bins_distance": (_I, [_P, _P, _P, _I, _PF, _I, _I, _P, _F, _I, _PI]),
     "amdat_list_create": (_I, [_P, _P, _I, _PI, _PP]),
     "amdat_list_free": (_I, [_P, _P]),
     "amdat_list_create_timed": (_I, [_P, _P, _I, _PI, _PI, _PI, _PP]),
@@ -654,6 +655,22 @@ def bins_assign(ctx, traj, nbins, lo, hi):
     cell = np.zeros((traj.T, traj.N), np.int32)
     check(lib().amdat_bins_assign(ctx.h, traj.h, int(nbins[0]), int(nbins[1]), int(nbins[2]), fp(lo), fp(hi),
                                   ip(cell)))
+    return cell
+
+
+DIST_POINT, DIST_PLANE, DIST_LIST = 0, 1, 2
+
+
+def bins_distance(ctx, traj, binned, kind, thickness, n_bins, point=None, axis=0, direction=0, cluster=None):
+    """Distance bin of every (frame, atom) of `binned` (None = all atoms):
+    int32 [T, N], -1 where not binned.  kind DIST_POINT (point = (x, y, z)),
+    DIST_PLANE (point = (position,), axis 0-2, direction 0 average / 1 above
+    / 2 below) or DIST_LIST (distance to the atoms of `cluster`)."""
+    pt = None if point is None else np.ascontiguousarray(np.resize(np.asarray(point, np.float32), 3), np.float32)
+    cell = np.zeros((traj.T, traj.N), np.int32)
+    check(lib().amdat_bins_distance(ctx.h, traj.h, binned.h if binned is not None else None, kind, fp(pt), axis,
+                                    direction, cluster.h if cluster is not None else None, float(thickness),
+                                    int(n_bins), ip(cell)))
     return cell
 
 

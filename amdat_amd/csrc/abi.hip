@@ -672,6 +672,79 @@ struct PinnedBuf {
 };
 }  // namespace
 
+// Per frame, the entries of a frame list: {first entry of idx, count}; the
+// index array (null = all atoms in order) and the largest count.
+static const int *frame_segs(const amdat_traj *tr, const amdat_list *l, std::vector<int2> &seg, int &max_n)
+{
+    const int T = tr->dev.T;
+    seg.assign(T, make_int2(0, 0));
+    max_n = 0;
+    for (int t = 0; t < T; t++) {
+        if (!list_timed(l)) {
+            seg[t] = make_int2(0, list_n(tr, l));
+        } else {
+            const int c = list_time(l, t);
+            if (c >= 0) seg[t] = make_int2(l->off[c], l->off[c + 1] - l->off[c]);
+        }
+        max_n = std::max(max_n, seg[t].y);
+    }
+    return list_ptr(l);
+}
+
+int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+                        const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
+                        int n_bins, int *cell)
+{
+    clear_error();
+    if (!ctx || !tr || !cell) return set_error(AMDAT_EINVAL, "amdat_bins_distance: null argument");
+    if (kind < AMDAT_DIST_POINT || kind > AMDAT_DIST_LIST)
+        return set_error(AMDAT_EINVAL, "amdat_bins_distance: unknown kind");
+    if (n_bins < 1 || !(thickness > 0.0f))
+        return set_error(AMDAT_EINVAL, "amdat_bins_distance: need n_bins >= 1 and thickness > 0");
+    if (!frame_list_ok(binned) || !frame_list_ok(cluster))
+        return set_error(AMDAT_EINVAL, "amdat_bins_distance: a pair-keyed list applies to frame pairs only");
+    if ((binned && binned->n_atoms != tr->dev.N) || (cluster && cluster->n_atoms != tr->dev.N))
+        return set_error(AMDAT_EINVAL, "amdat_bins_distance: list built for another trajectory size");
+    const int N = tr->dev.N, T = tr->dev.T;
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    if (kind == AMDAT_DIST_POINT) {
+        if (!point) return set_error(AMDAT_EINVAL, "amdat_bins_distance: null point");
+        p[0] = point[0], p[1] = point[1], p[2] = point[2];
+        // src/trajectory_list_bins.cpp:1358-1362 (Coordinate::operator>)
+        for (int t = 0; t < T; t++)
+            if (p[0] > tr->box[3 * t] || p[1] > tr->box[3 * t + 1] || p[2] > tr->box[3 * t + 2])
+                return set_error(AMDAT_ERANGE, "amdat_bins_distance: point chosen does not lie within the box dimensions");
+    } else if (kind == AMDAT_DIST_PLANE) {
+        if (!point || axis < 0 || axis > 2 || direction < 0 || direction > 2)
+            return set_error(AMDAT_EINVAL, "amdat_bins_distance: plane needs a position, axis 0-2 and direction 0-2");
+        p[0] = point[0];
+        for (int t = 0; t < T; t++)
+            if (p[0] > tr->box[3 * t + axis])
+                return set_error(AMDAT_ERANGE, "amdat_bins_distance: plane chosen does not lie within the box dimensions");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = stream_of(ctx);
+    std::vector<int2> seg, cseg;
+    int max_n = 0, max_c = 0;
+    const int *idx = frame_segs(tr, binned, seg, max_n);
+    DevBuf<int2> dseg, dcseg;
+    DevBuf<int> dc;
+    HIP_TRY(dseg.upload(seg.data(), seg.size(), st));
+    HIP_TRY(dc.alloc((size_t)N * T));
+    HIP_TRY(hipMemsetAsync(dc.p, 0xFF, sizeof(int) * (size_t)N * T, st));
+    if (kind == AMDAT_DIST_LIST) {
+        const int *cidx = frame_segs(tr, cluster, cseg, max_c);
+        HIP_TRY(dcseg.upload(cseg.data(), cseg.size(), st));
+        HIP_TRY(launch_cluster_cells(tr->dev, dseg.p, idx, max_n, dcseg.p, cidx, thickness, n_bins, dc.p, st));
+    } else {
+        HIP_TRY(launch_distance_cells(tr->dev, dseg.p, idx, max_n, kind, axis, direction, p, thickness, n_bins, dc.p,
+                                      st));
+    }
+    HIP_TRY(hipMemcpyAsync(cell, dc.p, sizeof(int) * (size_t)N * T, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
 int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_atoms, int n_frames,
                     const float *box3, int threads, int *types, amdat_traj **out)
 {

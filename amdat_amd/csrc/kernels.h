@@ -110,6 +110,14 @@ hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int
 // Rectangular bin of every (frame, atom) into cell[T][N] (-1 outside).
 hipError_t launch_bins_assign(const TrajDev &tr, int nx, int ny, int nz, const float *lo, const float *hi, int *cell,
                               hipStream_t st);
+// Distance bins: point (kind 0; point = {x, y, z}) or plane (kind 1; axis,
+// dir 0 average / 1 above / 2 below, point[0] = position); seg[t] = {first
+// entry of idx, count} of the binned list at frame t (idx null = all atoms).
+hipError_t launch_distance_cells(const TrajDev &tr, const int2 *seg, const int *idx, int max_n, int kind, int axis,
+                                 int dir, const float *point, float w, int n_bins, int *cell, hipStream_t st);
+// Distance bins from the atoms of a second list (cseg, cidx as seg, idx).
+hipError_t launch_cluster_cells(const TrajDev &tr, const int2 *seg, const int *idx, int max_n, const int2 *cseg,
+                                const int *cidx, float w, int n_bins, int *cell, hipStream_t st);
 // Displacement selection (Fast/Slow/Average_Particles): with tile_base ==
 // nullptr, per-block counts into tile_count[n_seg][n_tiles]; otherwise the
 // selected atoms, in input order, into out at tile_base[q][tile] + rank.

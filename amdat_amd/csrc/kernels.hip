@@ -360,6 +360,113 @@ __global__ __launch_bounds__(kBlock) void bins_assign_kernel(const float *__rest
     cell[o] = inside ? (b[0] * ny + b[1]) * nz + b[2] : -1;
 }
 
+// Distance bins, point and plane forms (Trajectory_List_Bins::assign_bins_
+// distance_point, src/trajectory_list_bins.cpp:1321-1488, and assign_bins_
+// distance_plane, :1489-1774).  Frame t = blockIdx.y; entry a of the binned
+// list at t (seg[t] = {first entry of idx, count}; idx null = all atoms in
+// order); L = the frame's box lengths.  FP32 in the reference's order:
+//   point p: d = r - p, d -= L * float(int(d / (L * 0.5f))) per axis,
+//            distance sqrt((dx dx + dy dy) + dz dz);
+//   plane on axis k at p: d = r_k - p; average |L * int(d / double(L * 0.5))
+//            - d| (that quotient is a double one); above d < 0 ? L + d : d;
+//            below d < 0 ? |d| : L - d.
+// bin = int(distance / w), stored in cell[t N + j] when 0 <= bin < n_bins
+// (cells not stored keep the caller's -1).
+__global__ __launch_bounds__(kBlock) void distance_cells_kernel(const float *__restrict__ X,
+                                                                 const float *__restrict__ Y,
+                                                                 const float *__restrict__ Z,
+                                                                 const float *__restrict__ box, int N,
+                                                                 const int2 *__restrict__ seg,
+                                                                 const int *__restrict__ idx, int kind, int axis,
+                                                                 int dir, float px, float py, float pz, float w,
+                                                                 int n_bins, int *__restrict__ cell)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.y;
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    const int2 sg = seg[t];
+    if (a >= sg.y) return;
+    const int j = idx ? idx[sg.x + a] : a;
+    const size_t o = (size_t)t * N + j;
+    const float r[3] = {X[o], Y[o], Z[o]};
+    const float L[3] = {box[3 * t], box[3 * t + 1], box[3 * t + 2]};
+    float len;
+    if (kind == 0) {
+        const float p[3] = {px, py, pz};
+        float d[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            d[k] = r[k] - p[k];
+            d[k] = d[k] - L[k] * (float)(int)(d[k] / (L[k] * 0.5f));
+        }
+        len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    } else {
+        const float Lk = axis == 0 ? L[0] : (axis == 1 ? L[1] : L[2]);
+        const float d = (axis == 0 ? r[0] : (axis == 1 ? r[1] : r[2])) - px;
+        if (dir == 0) len = fabsf(Lk * (float)(int)((double)d / ((double)Lk * 0.5)) - d);
+        else if (dir == 1) len = d < 0.0f ? Lk + d : d;
+        else len = d < 0.0f ? fabsf(d) : Lk - d;
+    }
+    const int b = (int)(len / w);
+    if (b >= 0 && b < n_bins) cell[o] = b;
+}
+
+// Distance bins from a second list (Trajectory_List_Bins::assign_bins_
+// distance_clusters, src/trajectory_list_bins.cpp:1097-1320): entry a of the
+// binned list at frame t (blockIdx.y) takes the smallest distance to the
+// atoms of the cluster list at t (cseg, cidx as seg, idx), starting from the
+// largest box length of the frame.  Each difference takes its shortest image
+// with the box of frame 0 (Coordinate::vector_unwrapped, src/coordinate.cpp:
+// 98-124: d > L/2 -> d - L, d < -L/2 -> d + L); FP32 as in the reference.
+// The cluster list is staged through LDS, kBlock atoms at a time.
+__global__ __launch_bounds__(kBlock) void cluster_cells_kernel(
+    const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
+    const float *__restrict__ box, int N, const int2 *__restrict__ seg, const int *__restrict__ idx,
+    const int2 *__restrict__ cseg, const int *__restrict__ cidx, float w, int n_bins, int *__restrict__ cell)
+{
+#pragma clang fp contract(off)
+    __shared__ float sx[kBlock], sy[kBlock], sz[kBlock];
+    const int t = blockIdx.y;
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    const int2 sg = seg[t], cs = cseg[t];
+    const bool active = a < sg.y;
+    const size_t ft = (size_t)t * N;
+    int j = 0;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    float len = box[3 * t];
+    if (box[3 * t + 1] > len) len = box[3 * t + 1];
+    if (box[3 * t + 2] > len) len = box[3 * t + 2];
+    if (active) {
+        j = idx ? idx[sg.x + a] : a;
+        cx = X[ft + j], cy = Y[ft + j], cz = Z[ft + j];
+    }
+    const float bx = box[0], by = box[1], bz = box[2];
+    const float hx = bx / 2.0f, hy = by / 2.0f, hz = bz / 2.0f;
+    for (int base = 0; base < cs.y; base += kBlock) {
+        const int c = base + threadIdx.x;
+        if (c < cs.y) {
+            const int k = cidx ? cidx[cs.x + c] : c;
+            sx[threadIdx.x] = X[ft + k], sy[threadIdx.x] = Y[ft + k], sz[threadIdx.x] = Z[ft + k];
+        }
+        __syncthreads();
+        const int m = min(kBlock, cs.y - base);
+        if (active) {
+            for (int i = 0; i < m; i++) {
+                float dx = cx - sx[i], dy = cy - sy[i], dz = cz - sz[i];
+                dx = dx > hx ? dx - bx : (dx < -hx ? dx + bx : dx);
+                dy = dy > hy ? dy - by : (dy < -hy ? dy + by : dy);
+                dz = dz > hz ? dz - bz : (dz < -hz ? dz + bz : dz);
+                const float l = sqrtf((dx * dx + dy * dy) + dz * dz);
+                len = l < len ? l : len;
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    const int b = (int)(len / w);
+    if (b < n_bins) cell[ft + j] = b;
+}
+
 // Selection pass 1.  Pair q (blockIdx.y) has seg[q] = {t0, t1, first entry
 // in the input list (-1: all atoms in order), entries}; one thread per
 // entry.  Writes the number of selected entries of each block of kBlock
@@ -793,6 +900,27 @@ hipError_t launch_bins_assign(const TrajDev &tr, int nx, int ny, int nz, const f
     (void)hipGetLastError();
     hipLaunchKernelGGL(bins_assign_kernel, dim3((tr.N + kBlock - 1) / kBlock, tr.T), dim3(kBlock), 0, st, tr.x, tr.y,
                        tr.z, tr.N, nx, ny, nz, lo, hi, cell);
+    return hipGetLastError();
+}
+
+hipError_t launch_distance_cells(const TrajDev &tr, const int2 *seg, const int *idx, int max_n, int kind, int axis,
+                                 int dir, const float *point, float w, int n_bins, int *cell, hipStream_t st)
+{
+    if (max_n <= 0) return hipSuccess;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(distance_cells_kernel, dim3((max_n + kBlock - 1) / kBlock, tr.T), dim3(kBlock), 0, st, tr.x,
+                       tr.y, tr.z, tr.box, tr.N, seg, idx, kind, axis, dir, point[0], point[1], point[2], w, n_bins,
+                       cell);
+    return hipGetLastError();
+}
+
+hipError_t launch_cluster_cells(const TrajDev &tr, const int2 *seg, const int *idx, int max_n, const int2 *cseg,
+                                const int *cidx, float w, int n_bins, int *cell, hipStream_t st)
+{
+    if (max_n <= 0) return hipSuccess;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(cluster_cells_kernel, dim3((max_n + kBlock - 1) / kBlock, tr.T), dim3(kBlock), 0, st, tr.x,
+                       tr.y, tr.z, tr.box, tr.N, seg, idx, cseg, cidx, w, n_bins, cell);
     return hipGetLastError();
 }
 

@@ -120,6 +120,16 @@ class Trajectory_List_Bins {
     // region: bounds 0 or outside the box fall back to the box
     Trajectory_List_Bins(System *sys, int xbins, int ybins, int zbins, float x_lo, float x_hi, float y_lo,
                          float y_hi, float z_lo, float z_hi);
+    // Distance bins of the atoms of `binned`: from the nearest atom of
+    // `cluster` (src/trajectory_list_bins.cpp:461-559), from a plane x, y or
+    // z at `position` with direction average, above or below (:560-765), from
+    // a point (:766-889).  n_bins is capped at int(reach / thickness) + 1.
+    Trajectory_List_Bins(System *sys, float thickness, int n_bins, const Trajectory_List *binned,
+                         const Trajectory_List *cluster);
+    Trajectory_List_Bins(System *sys, float thickness, int n_bins, const Trajectory_List *binned,
+                         const std::string &plane, float position, const std::string &direction);
+    Trajectory_List_Bins(System *sys, float thickness, int n_bins, const Trajectory_List *binned,
+                         const Coordinate &point);
     int show_n_xbins() const { return n_[0]; }
     int show_n_ybins() const { return n_[1]; }
     int show_n_zbins() const { return n_[2]; }
@@ -136,6 +146,8 @@ class Trajectory_List_Bins {
 
   private:
     void assign(const std::vector<float> &lo, const std::vector<float> &hi);
+    void assign_distance(int kind, const float *point, int axis, int direction, const Trajectory_List *binned,
+                         const Trajectory_List *cluster, float thickness, int n_bins);
     System *system_;
     int n_[3];
     std::vector<int> cell_;  // [T][N], -1 outside

@@ -131,8 +131,26 @@ void Control::create_bin_list(const std::vector<std::string> &args)
         if (a.size() != 10) throw Error("Incorrect number of arguments for create_bin_list region.", 0);
         b = std::make_unique<Trajectory_List_Bins>(system_.get(), I(1), I(2), I(3), F(4), F(5), F(6), F(7), F(8),
                                                    F(9));
+    } else if (a[0] == "distance" && a.size() > 1) {
+        // src/control.cpp:3915-3985
+        auto L = [&](int k) { return target("list " + a[k]); };
+        if (a[1] == "trajectory") {
+            // distance trajectory <list to bin> <list to measure from> <thickness> <n_bins>
+            if (a.size() != 6) throw Error("Incorrect number of arguments for create_bin_list distance trajectory.", 0);
+            b = std::make_unique<Trajectory_List_Bins>(system_.get(), F(4), I(5), L(2), L(3));
+        } else if (a[1] == "point") {
+            // distance point <list to bin> <x> <y> <z> <thickness> <n_bins>
+            if (a.size() != 8) throw Error("Incorrect number of arguments for create_bin_list distance point.", 0);
+            b = std::make_unique<Trajectory_List_Bins>(system_.get(), F(6), I(7), L(2), Coordinate(F(3), F(4), F(5)));
+        } else if (a[1] == "plane") {
+            // distance plane <direction> <list to bin> <plane> <position> <thickness> <n_bins>
+            if (a.size() != 8) throw Error("Incorrect number of arguments for create_bin_list distance plane.", 0);
+            b = std::make_unique<Trajectory_List_Bins>(system_.get(), F(6), I(7), L(3), a[4], F(5), a[2]);
+        } else {
+            throw Error("distance binning type " + a[1] + " not recognized, use trajectory, point or plane.", 0);
+        }
     } else {
-        throw Error("bin_list of type " + a[0] + " is not supported by this build (all, region)", 0);
+        throw Error("bin_list of type " + a[0] + " invalid", 0);
     }
     for (auto &p : bin_lists_)
         if (p.first == args[1]) {

@@ -142,6 +142,83 @@ void Trajectory_List_Bins::assign(const std::vector<float> &lo, const std::vecto
           "bins");
 }
 
+// The bin cap of the point and cluster forms (src/trajectory_list_bins.cpp:
+// 487-517, 792-822): the largest half box diagonal over the frames, formed
+// with pow in double and kept as float, gives int(length / thickness) + 1.
+static int distance_bin_cap(const System *sys, float thickness, int n_bins)
+{
+    float length = 0;
+    for (int t = 0; t < sys->show_n_timesteps(); t++) {
+        const Coordinate L = sys->size(t);
+        const float xl = L.show_x() / 2, yl = L.show_y() / 2, zl = L.show_z() / 2;
+        const double d = std::pow(std::pow(xl, 2) + std::pow(yl, 2) + std::pow(zl, 2), .5);
+        if (d > length) length = (float)d;
+    }
+    const int max_bins = int(length / thickness) + 1;
+    if (n_bins < max_bins) return n_bins;
+    std::cout << std::endl << "system too small for " << n_bins << " bins. using " << max_bins << " instead." << std::endl;
+    return max_bins;
+}
+
+Trajectory_List_Bins::Trajectory_List_Bins(System *sys, float thickness, int n_bins, const Trajectory_List *binned,
+                                           const Trajectory_List *cluster)
+    : system_(sys), n_{1, 1, 1}
+{
+    n_[2] = distance_bin_cap(sys, thickness, n_bins);
+    assign_distance(AMDAT_DIST_LIST, nullptr, 0, 0, binned, cluster, thickness, n_[2]);
+}
+
+Trajectory_List_Bins::Trajectory_List_Bins(System *sys, float thickness, int n_bins, const Trajectory_List *binned,
+                                           const Coordinate &point)
+    : system_(sys), n_{1, 1, 1}
+{
+    n_[2] = distance_bin_cap(sys, thickness, n_bins);
+    const float p[3] = {point.show_x(), point.show_y(), point.show_z()};
+    assign_distance(AMDAT_DIST_POINT, p, 0, 0, binned, nullptr, thickness, n_[2]);
+}
+
+Trajectory_List_Bins::Trajectory_List_Bins(System *sys, float thickness, int n_bins, const Trajectory_List *binned,
+                                           const std::string &plane, float position, const std::string &direction)
+    : system_(sys), n_{1, 1, 1}
+{
+    // src/trajectory_list_bins.cpp:585-700: the reach is the last frame's
+    // half box (average) or whole box (above, below) along the plane normal
+    int dir;
+    if (direction == "average") dir = 0;
+    else if (direction == "above") dir = 1;
+    else if (direction == "below") dir = 2;
+    else throw Error("could not recognize direction " + direction + ". valid choices are above, below or average.", 0);
+    int axis;
+    if (plane == "x") axis = 0;
+    else if (plane == "y") axis = 1;
+    else if (plane == "z") axis = 2;
+    else throw Error("plane " + plane + " not recognized, plane must be x, y, or z.", 0);
+    const Coordinate L = sys->size(sys->show_n_timesteps() - 1);
+    const float Lk = axis == 0 ? L.show_x() : (axis == 1 ? L.show_y() : L.show_z());
+    const float reach = dir == 0 ? Lk / 2 : Lk;
+    const int max_bins = int(reach / thickness) + 1;
+    int nb = n_bins;
+    if (n_bins >= max_bins) {
+        std::cout << std::endl << "system too small for " << n_bins << " bins. Using " << max_bins << " instead."
+                  << std::endl;
+        nb = max_bins;
+    }
+    n_[axis] = nb;
+    const float p[3] = {position, 0.0f, 0.0f};
+    assign_distance(AMDAT_DIST_PLANE, p, axis, dir, binned, nullptr, thickness, nb);
+}
+
+void Trajectory_List_Bins::assign_distance(int kind, const float *point, int axis, int direction,
+                                           const Trajectory_List *binned, const Trajectory_List *cluster,
+                                           float thickness, int n_bins)
+{
+    cell_.resize((size_t)system_->show_n_timesteps() * system_->show_n_atoms());
+    check(amdat_bins_distance(device_context(), system_->traj(), binned ? binned->handle() : nullptr, kind, point,
+                              axis, direction, cluster ? cluster->handle() : nullptr, thickness, n_bins,
+                              cell_.data()),
+          "distance bins");
+}
+
 std::unique_ptr<Trajectory_List> Trajectory_List_Bins::intersect(const Trajectory_List *l, int xii, int yii,
                                                                  int zii) const
 {

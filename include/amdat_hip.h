@@ -219,6 +219,30 @@ int amdat_traj_bounds(const amdat_traj *tr, float *lo);
 int amdat_bins_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
                       const float *hi, int *cell);
 
+/* Distance bins of Trajectory_List_Bins (create_bin_list distance ...,
+ * src/control.cpp:3915-3985): for every frame t and every atom j of
+ * `binned` at t (NULL = all atoms), cell[t*n_atoms + j] = int(distance /
+ * thickness) when below n_bins, else -1 (also for atoms not in `binned`).
+ * Distances in FP32 as the reference forms them, L the frame's box lengths:
+ *  AMDAT_DIST_POINT (src/trajectory_list_bins.cpp:1321-1488): point[3];
+ *    d = r - point, d -= L * trunc(d / (L/2)) per axis, |d|;
+ *  AMDAT_DIST_PLANE (:1489-1774): axis 0/1/2, point[0] = position,
+ *    direction 0 average (|L * trunc(d / (L/2)) - d|), 1 above (d < 0 ?
+ *    L + d : d), 2 below (d < 0 ? |d| : L - d), d = r[axis] - position;
+ *  AMDAT_DIST_LIST (:1097-1320): the smallest distance to the atoms of
+ *    `cluster` at t (NULL = all atoms), shortest images in the box of frame
+ *    0, starting from the frame's largest box length.
+ * A point or plane position beyond a frame's box length is AMDAT_ERANGE, as
+ * the reference stops there.  n_bins is the bin count the caller settled
+ * (the constructors cap it at int(reach / thickness) + 1).  cell is host
+ * memory [n_frames][n_atoms]. */
+#define AMDAT_DIST_POINT 0
+#define AMDAT_DIST_PLANE 1
+#define AMDAT_DIST_LIST 2
+int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+                        const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
+                        int n_bins, int *cell);
+
 /* Static atom selection (Static_Trajectory_List, src/static_trajectory_
  * list.cpp:59-101): atom indices in list order.  indices == NULL with
  * n == n_atoms of the trajectory means "all atoms in order". */

@@ -42,6 +42,10 @@
  *   orc_bins_assign     interior points map to floor((c - lo)/w), bin
  *                       boundaries go to the upper bin, box faces and
  *                       points outside the region are excluded.
+ *   orc_bins_distance   FP64 minimum-image distances (point, plane in each
+ *                       direction, nearest atom of a second list by brute
+ *                       force) away from bin edges, the above/below
+ *                       complement L, the reach cap, list membership.
  *   orc_displacement_dist  constructed trajectories with exactly
  *                       representable squared displacements (bin counts
  *                       in closed form, bin edges, the overflow bin).
@@ -562,6 +566,67 @@ void orc_bins_assign(int N, int T, const float *x, const float *y, const float *
                 }
             cell[o] = inside ? (b[0] * ny + b[1]) * nz + b[2] : -1;
         }
+}
+
+/* Distance bins of Trajectory_List_Bins: point (kind 0,
+ * assign_bins_distance_point, src/trajectory_list_bins.cpp:1321-1488),
+ * plane (kind 1, assign_bins_distance_plane, :1489-1774) and distance to a
+ * second list (kind 2, assign_bins_distance_clusters, :1097-1320).  The
+ * binned atoms of frame t are idx[off[t] .. off[t+1]-1], the second list's
+ * cidx[coff[t] .. coff[t+1]-1].  FP32 in the reference's types and order:
+ * Coordinate arithmetic is per component in float, box_size*.5 is a float
+ * product for the point form (Coordinate * float) and a double one for the
+ * plane form (float * double literal), Coordinate::length is
+ * sqrt((x*x + y*y) + z*z) in float, vector_unwrapped (src/coordinate.cpp:
+ * 98-124) uses the box of frame 0.  cell[t*N + j] = int(length / w) when
+ * that is in [0, n_bins), else -1. */
+void orc_bins_distance(int N, int T, const float *x, const float *y, const float *z, const float *box, int kind,
+                       const float *point, int axis, int dir, const int *off, const int *idx, const int *coff,
+                       const int *cidx, float w, int n_bins, int *cell)
+{
+    for (size_t o = 0; o < (size_t)N * T; o++) cell[o] = -1;
+    for (int t = 0; t < T; t++) {
+        const float *L = box + 3 * t;
+        const float *r[3] = {x + (size_t)t * N, y + (size_t)t * N, z + (size_t)t * N};
+        for (int a = off[t]; a < off[t + 1]; a++) {
+            const int j = idx[a];
+            float len = 0.0f;
+            if (kind == 0) {
+                float d[3];
+                for (int k = 0; k < 3; k++) {
+                    d[k] = r[k][j] - point[k];
+                    const float h = L[k] * 0.5f;
+                    const float q = d[k] / h;
+                    const float im = (float)(int)q;
+                    d[k] -= L[k] * im;
+                }
+                len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            } else if (kind == 1) {
+                const float Lk = L[axis];
+                const float d = r[axis][j] - point[0];
+                if (dir == 0) len = fabsf(Lk * (float)(int)((double)d / ((double)Lk * .5)) - d);
+                else if (dir == 1) len = d < 0 ? Lk + d : d;
+                else len = d < 0 ? fabsf(d) : Lk - d;
+            } else {
+                len = L[0];
+                if (L[1] > len) len = L[1];
+                if (L[2] > len) len = L[2];
+                for (int c = coff[t]; c < coff[t + 1]; c++) {
+                    const int k = cidx[c];
+                    float d[3];
+                    for (int m = 0; m < 3; m++) {
+                        d[m] = r[m][j] - r[m][k];
+                        if (d[m] > box[m] / 2) d[m] = d[m] - box[m];
+                        else if (d[m] < -box[m] / 2) d[m] = d[m] + box[m];
+                    }
+                    const float l = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    if (l < len) len = l;
+                }
+            }
+            const int b = (int)(len / w);
+            if (b >= 0 && b < n_bins) cell[(size_t)t * N + j] = b;
+        }
+    }
 }
 
 /* DebyeWaller_Dist / Displacement_Distribution (src/debyewaller_dist.cpp:

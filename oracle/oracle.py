@@ -40,6 +40,7 @@ SIGS = {
     "orc_ref32_msd_pair": (_I, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _I, _PF, _PF]),
     "orc_sk_loss_grad_f64": (_D, [_I, _PD, _PD, _PD, _I, _PI, _PI, _D, _PD, _PD, _PD, _PD, _PD, _PD]),
     "orc_disp_f64": (None, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _PI, _PI, _PI, _I, _PD, _PD, _PD]),
+    "orc_bins_distance": (None, [_I, _I, _PF, _PF, _PF, _PF, _I, _PF, _I, _I, _PI, _PI, _PI, _PI, _F, _I, _PI]),
     "orc_vhs_counts": (None, [_I, _PF, _PF, _PF, _PS, _PS, _PS, _PF, _PI, _I, _I, _PI, _PI, _PI, _I, _I, _F,
                               C.POINTER(C.c_ulonglong), _PD]),
 }
@@ -246,6 +247,32 @@ def bins_assign(tr, nbins, lo, hi):
     cell = np.zeros((tr.T, tr.N), np.int32)
     lib().orc_bins_assign(tr.N, tr.T, _p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), int(nbins[0]),
                           int(nbins[1]), int(nbins[2]), _p(lo, C.c_float), _p(hi, C.c_float), _p(cell, C.c_int))
+    return cell
+
+
+def _frame_lists(tr, lst):
+    """(off [T+1], idx) of a list per frame: None = all atoms, an index
+    array = the same atoms at every frame, a callable t -> atoms."""
+    per = [np.arange(tr.N) if lst is None else (lst(t) if callable(lst) else lst) for t in range(tr.T)]
+    off = np.zeros(tr.T + 1, np.int32)
+    off[1:] = np.cumsum([len(a) for a in per])
+    idx = np.concatenate([np.asarray(a, np.int32) for a in per]) if off[-1] else np.zeros(1, np.int32)
+    return off, _c(idx, np.int32)
+
+
+def bins_distance(tr, kind, thickness, n_bins, point=None, axis=0, direction=0, binned=None, cluster=None):
+    """Distance bin of every (frame, atom) in the reference's FP32 form:
+    int32 [T, N], -1 where not binned.  kind 0 point, 1 plane (point =
+    (position,), direction 0 average / 1 above / 2 below), 2 nearest atom
+    of `cluster`.  binned / cluster: None, an index array or t -> atoms."""
+    x, y, z, box = (_c(a, np.float32) for a in (tr.x, tr.y, tr.z, tr.box))
+    pt = _c(np.resize(np.asarray(point if point is not None else [0.0], np.float32), 3), np.float32)
+    off, idx = _frame_lists(tr, binned)
+    coff, cidx = _frame_lists(tr, cluster)
+    cell = np.zeros((tr.T, tr.N), np.int32)
+    lib().orc_bins_distance(tr.N, tr.T, _p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), _p(box, C.c_float),
+                            kind, _p(pt, C.c_float), axis, direction, _p(off, C.c_int), _p(idx, C.c_int),
+                            _p(coff, C.c_int), _p(cidx, C.c_int), C.c_float(thickness), n_bins, _p(cell, C.c_int))
     return cell
 
 

@@ -5,7 +5,10 @@ custom file, and the drop-in's per-bin msd, isfs and structure_factor files
 bin by bin.  The persistence form (Bin_Dynamics_Analysis: per frame pair,
 the atoms in the bin at both frames) through pair-keyed lists, at the C ABI
 and in the drop-in's files, against the oracle composed pair by pair.
-Plain form N = 1,000 in a box [-4.5, 4.91)^3, exponential 5 20."""
+Distance bins (point, plane, nearest atom of a second list): device cells
+bit-exact with the oracle, errors, and the drop-in's create_bin_list
+distance forms.  Plain form N = 1,000 in a box [-4.5, 4.91)^3,
+exponential 5 20."""
 import os
 import subprocess
 
@@ -53,6 +56,42 @@ def test_cells_match_oracle(ctx):
     got = _abi.bins_assign(ctx, d, (4, 4, 4), rlo, rhi)
     assert np.array_equal(got, orc.bins_assign(tr, (4, 4, 4), rlo, rhi))
     assert 0 < np.mean(got < 0) < 1
+
+
+def test_distance_cells_match_oracle(ctx):
+    print("case: bins_distance point/plane/list, plain form", flush=True)
+    tr = synth.plain_config()
+    d = _abi.Trajectory(ctx, *tr.arrays)
+    sub = np.arange(0, tr.N, 3)
+    per = {t: np.arange(t % 5, tr.N, 4) for t in range(tr.T)}
+    conv = np.arange(tr.T, dtype=np.int32)
+    toff = np.r_[0, np.cumsum([len(per[t]) for t in range(tr.T)])].astype(np.int32)
+    timed = _abi.TimedList(ctx, d, conv, toff, np.concatenate([per[t] for t in range(tr.T)]))
+    static = _abi.AtomList(ctx, d, sub)
+    lists = [(None, None), (static, sub), (timed, lambda t: per[t])]
+    for dl, ol in lists:
+        for p, nb in [((4.0, 4.0, 4.0), 100), ((0.5, 9.0, 2.25), 5)]:
+            got = _abi.bins_distance(ctx, d, dl, _abi.DIST_POINT, 0.37, nb, point=p)
+            assert np.array_equal(got, orc.bins_distance(tr, 0, 0.37, nb, point=p, binned=ol))
+        for axis in range(3):
+            for k in range(3):
+                got = _abi.bins_distance(ctx, d, dl, _abi.DIST_PLANE, 0.29, 40, point=(3.3,), axis=axis, direction=k)
+                want = orc.bins_distance(tr, 1, 0.29, 40, point=(3.3,), axis=axis, direction=k, binned=ol)
+                assert np.array_equal(got, want), (axis, k)
+    cl = np.arange(800, 1000)
+    cluster = _abi.AtomList(ctx, d, cl)
+    for dl, ol in lists:
+        got = _abi.bins_distance(ctx, d, dl, _abi.DIST_LIST, 0.2, 8, cluster=cluster)
+        want = orc.bins_distance(tr, 2, 0.2, 8, binned=ol, cluster=cl)
+        assert np.array_equal(got, want)
+        assert np.all(got[:, 800:][got[:, 800:] >= 0] == 0)
+    got = _abi.bins_distance(ctx, d, None, _abi.DIST_LIST, 0.2, 8, cluster=timed)
+    assert np.array_equal(got, orc.bins_distance(tr, 2, 0.2, 8, cluster=lambda t: per[t]))
+    for kw in [dict(kind=_abi.DIST_POINT, point=(10.0, 1.0, 1.0)),            # beyond the box length
+               dict(kind=_abi.DIST_PLANE, point=(9.5,), axis=1),
+               dict(kind=3, point=(1.0, 1.0, 1.0))]:
+        with pytest.raises(_abi.AmdatError):
+            _abi.bins_distance(ctx, d, None, thickness=0.3, n_bins=4, **kw)
 
 
 def bin_lists(tr, cells, nb):
@@ -188,6 +227,20 @@ isfs isfsp.dat 20 20 xyz 9.41 1
 bin_list grid all 1
 structure_factor skp.dat symmetric xyz 0 -1
 bin_list grid all 1
+create_list bsp
+species B
+create_bin_list dpt
+distance point all 0.5 0.5 0.5 1.0 4
+create_bin_list dpl
+distance plane above all z 0.2 1.5 3
+create_bin_list dtr
+distance trajectory all bsp 0.4 3
+msd msd_dpt.dat
+bin_list dpt all
+msd msd_dpl.dat
+bin_list dpl all 1
+isfs isfs_dtr.dat 20 20 xyz 9.41 0
+bin_list dtr all
 """ % (NEXP, NES)
     (d / "in.amdat").write_text(script)
     env = dict(os.environ, AMDAT_WV3D=str(qdir / "qvector"))
@@ -282,3 +335,44 @@ def test_bin_persistence_files(run, qvec):
         ok = SC > 0
         assert np.allclose(rows[ok, 1], SF[ok, 0] / SC[ok], rtol=RTOL, atol=1e-6)
         assert (d / name("skp.dat", b)).read_text() == (d / name("sk.dat", b)).read_text()
+
+
+def test_distance_bin_files(run, qvec):
+    """create_bin_list distance point / plane above / trajectory: the bin
+    count capped as the reference caps it (4, 3, 3 here, all below the cap),
+    cells as the oracle forms them; msd per bin (static and persistence) and
+    isfs per bin against the oracle composed bin by bin."""
+    d, tr, _, _ = run
+    pairs = orc.ts_pairs(NEXP, NES)
+    bsp = np.arange(800, 1000)
+    cases = [("msd_dpt.dat", orc.bins_distance(tr, 0, 1.0, 4, point=(0.5, 0.5, 0.5)), (1, 1, 4), False),
+             ("msd_dpl.dat", orc.bins_distance(tr, 1, 1.5, 3, point=(0.2,), axis=2, direction=1), (1, 1, 3), True)]
+    for stem, cells, nb, persist in cases:
+        assert cells.max() == nb[2] - 1
+        lists = pair_bin_lists(cells, nb, pairs) if persist else bin_lists(tr, cells, nb)
+        for b, lst in lists.items():
+            s2, cnt = np.zeros(NTG), np.zeros(NTG)
+            it = per_pair(lst, pairs, lambda p, at: orc.msd_ngp_f64(tr, p, NTG, atoms=at)) if persist else \
+                compose_pairs(lst, pairs, lambda p, at: orc.msd_ngp_f64(tr, p, NTG, atoms=at))
+            for m in it:
+                s2 += m["s2"]
+                cnt += m["count"]
+            rows = np.array([[float(v) for v in l.split("\t")] for l in
+                             (d / ("%s_bin_0%s0%s%d.bindata" % (stem, *(".." if persist else "__"), b))).read_text()
+                             .split("\n")[1:] if l.strip()])
+            ok = cnt > 0
+            assert ok[0]
+            assert np.allclose(rows[1:, 1][ok[1:]], (s2 / np.where(ok, cnt, 1))[1:][ok[1:]], rtol=RTOL)
+    cells = orc.bins_distance(tr, 2, 0.4, 3, cluster=bsp)
+    assert np.all(cells[:, 800:] == 0)
+    o22, n22 = ref_shells(qvec, 22, 22)
+    ell = float(np.float32(9.41))
+    for b, lst in bin_lists(tr, cells, (1, 1, 3)).items():
+        SF, SC = np.zeros((NTG, 1)), np.zeros(NTG)
+        for f, c in compose_pairs(lst, pairs, lambda p, at: orc.fs_f64(tr, p, NTG, o22, n22, ell, at)):
+            SF += f * c[:, None]
+            SC += c
+        lines = (d / ("isfs_dtr.dat_bin_0_0_%d.bindata" % b)).read_text().split("\n")
+        rows = np.array([[float(v) for v in l.strip("\t").split("\t")] for l in lines[2:] if l.strip()])
+        ok = SC > 0
+        assert ok[0] and np.allclose(rows[ok, 1], SF[ok, 0] / SC[ok], rtol=RTOL, atol=1e-6)
