@@ -29,8 +29,10 @@ constexpr int table_half_width(int M)
 {
     return M <= 24 ? 24 : (M <= 50 ? 50 : (M <= 100 ? 100 : (M <= 127 ? 127 : -1)));
 }
-constexpr int table_oy(int MM) { return 3 * MM + 1; }
-constexpr int table_oz(int MM) { return 4 * MM + 2; }
+// Sections hold H = 8 floor(MM/8) + 7 >= MM entries on each side.
+constexpr int table_h(int MM) { return 8 * (MM / 8) + 7; }
+constexpr int table_oy(int MM) { return 3 * table_h(MM) + 1; }
+constexpr int table_oz(int MM) { return 4 * table_h(MM) + 2; }
 
 // kpack bit 24: the slot's roles of y and z are swapped (a paired slot
 // whose pair shares nz rather than ny).

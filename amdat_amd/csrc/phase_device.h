@@ -37,19 +37,24 @@ constexpr int kWaves = kPhaseBlock / 64;
 // Prefetch depth (updates in flight) of the hot loops.
 constexpr int kDepth = 4;
 
-// Table geometry for half-width MM: row of W float2 per atom,
-// x at [0, 2MM], y at [2MM+1, 4MM+1], z (m >= 0 only) at [4MM+2, 5MM+2].
-// A atoms per tile: (A-1)*W*8 stays below 64 KiB so every read is one
-// ds_read_b64 with an immediate offset.
+// Table geometry for half-width MM: the sections are padded to whole
+// groups of 8 entries, H = 8 floor(MM/8) + 7 >= MM, so the table build
+// writes every entry of a group unconditionally.  Row of W float2 per atom:
+// x at [0, 2H], y at [2H+1, 4H+1], z (m >= 0 only) at [4H+2, 5H+2].
+// A atoms per tile: the reads of atom a sit a*W*8 bytes past the lane's
+// base, an immediate offset while below 64 KiB.  A = 32 when at most the
+// last two atoms of a tile fall past that reach (the compiler rebases their
+// reads), else 16 when all of them fit, else 8.
 template <int MM>
 struct Geom {
+    static constexpr int H = 8 * (MM / 8) + 7;
     // at least 257 float2 per row: reads of neighbouring atoms then never
     // pair into one ds_read2 (8-bit offsets), which keeps register use low
-    static constexpr int W = (5 * MM + 3) > 257 ? (5 * MM + 3) : 257;
-    static constexpr int OX = MM;
-    static constexpr int OY = 2 * MM + 1 + MM;
-    static constexpr int OZ = 2 * (2 * MM + 1);
-    static constexpr int A = (31 * W * 8 < 65536) ? 32 : ((15 * W * 8 < 65536) ? 16 : 8);
+    static constexpr int W = (5 * H + 3) > 257 ? (5 * H + 3) : 257;
+    static constexpr int OX = H;
+    static constexpr int OY = 3 * H + 1;
+    static constexpr int OZ = 4 * H + 2;
+    static constexpr int A = (29 * W * 8 < 65536) ? 32 : ((15 * W * 8 < 65536) ? 16 : 8);
     static constexpr int LOG2A = A == 32 ? 5 : (A == 16 ? 4 : 3);
     static constexpr size_t tab_bytes = (size_t)A * W * sizeof(float2);
 };
@@ -78,7 +83,8 @@ __device__ __forceinline__ int range_mz(int4 r) { return r.z >> 16; }
 // [-my, my], z in [0, mz].  Phase 1: exact small anchors e(d), d < 8, into
 // anc[d][3A].  Phase 2: one work item per (axis, j, atom), atom fastest,
 // computes the exact anchor e(8j) and the entries e(8j + d) = e(8j) e(d),
-// writing each at +m and its conjugate at -m where the range holds them.
+// writing all eight at +m and, on x and y, their conjugates at -m (the
+// sections are padded to whole groups, so no store needs a range test).
 // Neighbouring lanes work on neighbouring atoms, whose rows start W
 // entries apart, so their stores and anchor reads fall in distinct banks.
 template <int MM>
@@ -101,25 +107,26 @@ __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uin
     const int nj = njx + njy + njz;
     for (int it = tid; it < nj << Gm::LOG2A; it += kPhaseBlock) {
         const int i = it & (Gm::A - 1), k = it >> Gm::LOG2A;
-        int c, j, lo, hi, off;
-        if (k < njx) {
-            c = 0; j = jx0 + k; lo = xlo; hi = xhi; off = Gm::OX;
-        } else if (k < njx + njy) {
-            c = 1; j = k - njx; lo = -my; hi = my; off = Gm::OY;
-        } else {
-            c = 2; j = k - njx - njy; lo = 0; hi = mz; off = Gm::OZ;
-        }
+        const int c = k < njx ? 0 : (k < njx + njy ? 1 : 2);
+        const int j = c == 0 ? jx0 + k : (c == 1 ? k - njx : k - njx - njy);
+        const int off = c == 0 ? Gm::OX : (c == 1 ? Gm::OY : Gm::OZ);
         const int row = 3 * i + c;
         float2 a8 = sincos_turns(sU[row] * (uint32_t)(8 * j));
         if (!sValid[i]) a8 = make_float2(0.0f, 0.0f);
-        float2 *dst = tab + i * Gm::W + off;
+        // entries m = 8j + d (d = 0..7) and, on x and y, -m: all inside the
+        // section (8j + 7 <= H), each written by this lane only; e(0) is
+        // written last so it keeps a +0 imaginary part
+        float2 v[8];
 #pragma unroll
-        for (int d = 0; d < 8; ++d) {
-            const int m = 8 * j + d;
-            const float2 v = cmul(a8, anc[d * NR + row]);
-            if (m >= lo && m <= hi) dst[m] = v;
-            if (m > 0 && -m >= lo && -m <= hi) dst[-m] = make_float2(v.x, -v.y);
+        for (int d = 0; d < 8; ++d) v[d] = cmul(a8, anc[d * NR + row]);
+        float2 *up = tab + i * Gm::W + off + 8 * j;
+        if (c != 2) {
+            float2 *dn = tab + i * Gm::W + off - 8 * j - 7;
+#pragma unroll
+            for (int d = 0; d < 8; ++d) dn[7 - d] = make_float2(v[d].x, -v[d].y);
         }
+#pragma unroll
+        for (int d = 0; d < 8; ++d) up[d] = v[d];
     }
 }
 
