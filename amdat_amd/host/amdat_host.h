@@ -21,6 +21,7 @@
 #pragma once
 
 #include <cstdint>
+#include <complex>
 #include <functional>
 #include <memory>
 #include <stdexcept>
@@ -196,6 +197,10 @@ class System {
     // The same pairs on the host only.
     void displacement_pair_list(bool fullblock, std::vector<int> &tg, std::vector<int> &t0,
                                 std::vector<int> &t1) const;
+    // Pairs of one time gap as System::displacement_loop(analysis, traj,
+    // timegap, fullblock) visits them (src/system.cpp:3972-4023), as a
+    // device table.
+    amdat_pairs *timegap_loop_pairs(int timegapii, bool fullblock) const;
 
     // Atom selections (create_list keywords all / type_system / species /
     // type_species); indices in the reference's list order.
@@ -266,33 +271,47 @@ class Wave_Vectors {
 
 class Wave_Density {
   public:
+    // src/wave_density.cpp:46-80: every shell of wv; stf: only the first
+    // frame of each exponential block, t = n_exponential_steps * b (the
+    // static S(q) form, :316-323), other frames hold zero.
+    Wave_Density(System *sys, const Wave_Vectors *wv, bool stf = false);
+    // src/wave_density.cpp:83-124: shells inner .. outer.
     Wave_Density(System *sys, const Wave_Vectors *wv, int inner, int outer);
-    ~Wave_Density();
-    Wave_Density(const Wave_Density &) = delete;
-    Wave_Density &operator=(const Wave_Density &) = delete;
-    // src/wave_density.cpp:313-336: rho_k(t) for every frame.
+    // Copies share the computed device density, which is never modified
+    // after analyze() (a later analyze() or set() gives the object a new
+    // one), so a copy behaves as the reference's deep copy.
+    Wave_Density(const Wave_Density &other) = default;
+    Wave_Density &operator=(const Wave_Density &other) = default;
+    // src/wave_density.cpp:226-299: new range, density cleared.
+    void set(System *sys, const Wave_Vectors *wv);
+    void set(System *sys, const Wave_Vectors *wv, int inner, int outer);
+    // src/wave_density.cpp:313-336: rho_k(t) for every frame (or the block
+    // starts).  The reference adds to a density analysed before; here
+    // analyze() replaces it.
     void analyze(Trajectory_List *t_list);
-    // Deep copy of the computed density (the reference's wd2 = wd1).
-    void copy_from(const Wave_Density &other);
+    // The reference's wd2 = wd1.
+    void copy_from(const Wave_Density &other) { *this = other; }
+    std::complex<double> show_density(int time, int wavenumberindex, int vectorindex) const;
     double show_density_re(int time, int wavenumberindex, int vectorindex) const;
     double show_density_im(int time, int wavenumberindex, int vectorindex) const;
     int show_n_atoms(int timeii) const { return n_atoms_.at(timeii); }
+    // Atom-frames summed into the density (src/wave_density.cpp:339-365).
+    int show_n_atoms_looped() const;
     int show_first() const { return first_; }
     int show_last() const { return last_; }
     const Wave_Vectors *show_wavevectors() const { return wv_; }
     System *show_system() const { return system_; }
-    // Device density (this object's, or the one it was copied from).
-    const amdat_rho *device() const;
+    const amdat_rho *device() const { return rho_.get(); }
 
   private:
     void fetch() const;
     System *system_;
     const Wave_Vectors *wv_;
     int first_, last_;
+    bool sf_ = false;
     std::vector<int> n_atoms_;  // atoms of the list at each frame
-    amdat_rho *rho_ = nullptr;
-    const Wave_Density *shared_ = nullptr;
-    mutable std::vector<double> host_;
+    std::shared_ptr<amdat_rho> rho_;
+    mutable std::shared_ptr<std::vector<double>> host_;
 };
 
 class Structure_Factor {
@@ -314,13 +333,29 @@ class Structure_Factor {
 
 class Intermediate_Scattering_Function {
   public:
-    // src/intermediate_scattering_function.cpp:237-322
+    // src/intermediate_scattering_function.cpp:52-147: every shell of the
+    // densities (which must share wave vectors, system and shell range)
+    Intermediate_Scattering_Function(System *sys, Wave_Density *wd1, Wave_Density *wd2, bool parallel = true);
+    // :149-235: one shell
+    Intermediate_Scattering_Function(System *sys, Wave_Density *wd1, Wave_Density *wd2, int wn,
+                                     bool parallel = true);
+    // :237-322: shells wn1 .. wn2.  Shell wn reads the densities of shell wn
+    // (the reference reads the density of shell wn - wn1 relative to the
+    // density's first shell with the vector count of shell wn, which only
+    // agrees when wn1 is the density's first shell).
     Intermediate_Scattering_Function(System *sys, Wave_Density *wd1, Wave_Density *wd2, int wn1, int wn2,
-                                     bool parallel);
+                                     bool parallel = true);
+    // :324-412: every shell at one time gap only (pairs of System::
+    // displacement_loop(analysis, traj, timegap, fullblock), src/system.cpp:
+    // 3972-4023); time gap 0 gives S(k) over the block starts.  Other rows
+    // stay 0.
+    Intermediate_Scattering_Function(System *sys, int timegapii, Wave_Density *wd1, Wave_Density *wd2,
+                                     bool parallel = true);
     void write(const std::string &filename) const;
     float show(int timeii, int binii) const { return correlation_[timeii][binii]; }
 
   private:
+    void compute(Wave_Density *wd1, Wave_Density *wd2, amdat_pairs *pairs, int only_timegap);
     System *system_;
     const Wave_Vectors *wv_;
     int first_bin_, last_bin_;

@@ -118,7 +118,17 @@ Coordinate Wave_Vectors::show_mean_wavevector(int index) const
 }
 
 // ---------------------------------------------------------------------------
-// Wave_Density (src/wave_density.cpp:273-365)
+// Wave_Density (src/wave_density.cpp:46-365)
+
+static std::shared_ptr<amdat_rho> own_rho(amdat_rho *r)
+{
+    return std::shared_ptr<amdat_rho>(r, [](amdat_rho *p) { amdat_rho_free(device_context(), p); });
+}
+
+Wave_Density::Wave_Density(System *sys, const Wave_Vectors *wv, bool stf)
+    : system_(sys), wv_(wv), first_(0), last_(wv->show_n_wavenumbers() - 1), sf_(stf)
+{
+}
 
 Wave_Density::Wave_Density(System *sys, const Wave_Vectors *wv, int inner, int outer)
     : system_(sys), wv_(wv), first_(inner), last_(outer)
@@ -127,63 +137,92 @@ Wave_Density::Wave_Density(System *sys, const Wave_Vectors *wv, int inner, int o
         throw Error("wavenumber range outside the wave vector table", 0);
 }
 
-Wave_Density::~Wave_Density() { amdat_rho_free(device_context(), rho_); }
+void Wave_Density::set(System *sys, const Wave_Vectors *wv)
+{
+    set(sys, wv, 0, wv->show_n_wavenumbers() - 1);
+}
+
+void Wave_Density::set(System *sys, const Wave_Vectors *wv, int inner, int outer)
+{
+    if (inner < 0 || outer < inner || outer >= wv->show_n_wavenumbers())
+        throw Error("wavenumber range outside the wave vector table", 0);
+    system_ = sys;
+    wv_ = wv;
+    first_ = inner;
+    last_ = outer;
+    n_atoms_.clear();
+    rho_.reset();
+    host_.reset();
+}
 
 void Wave_Density::analyze(Trajectory_List *t_list)
 {
-    amdat_rho_free(device_context(), rho_);
-    rho_ = nullptr;
-    shared_ = nullptr;
-    host_.clear();
-    n_atoms_.resize(system_->show_n_timesteps());
-    for (int t = 0; t < system_->show_n_timesteps(); t++) n_atoms_[t] = t_list->show_n_trajectories(t);
-    check(amdat_rho_compute(device_context(), system_->traj(), t_list->handle(), wv_->handle(), first_, last_,
-                            wv_->show_length_scale(), nullptr, 0, AMDAT_PREC_FP32C, &rho_),
-          "wave density");
+    const int T = system_->show_n_timesteps();
+    n_atoms_.assign(T, 0);
+    host_.reset();
+    amdat_rho *r = nullptr;
+    if (!sf_) {
+        for (int t = 0; t < T; t++) n_atoms_[t] = t_list->show_n_trajectories(t);
+        check(amdat_rho_compute(device_context(), system_->traj(), t_list->handle(), wv_->handle(), first_, last_,
+                                wv_->show_length_scale(), nullptr, 0, AMDAT_PREC_FP32C, &r),
+              "wave density");
+    } else {
+        // the list's atoms at the block starts only: a time-varying list
+        // whose other frames hold no atoms
+        std::vector<int> conv(T, -1), off(1, 0), idx;
+        int c = 0;
+        for (int b = 0; b < system_->show_n_exponentials(); b++) {
+            const int t = system_->show_n_exponential_steps() * b;
+            if (t >= T) break;
+            const std::vector<int> a = t_list->atoms_at(t);
+            n_atoms_[t] = (int)a.size();
+            idx.insert(idx.end(), a.begin(), a.end());
+            off.push_back((int)idx.size());
+            conv[t] = c++;
+        }
+        amdat_list *h = nullptr;
+        check(amdat_list_create_timed(device_context(), system_->traj(), c, conv.data(), off.data(),
+                                      idx.empty() ? nullptr : idx.data(), &h),
+              "wave density list");
+        Trajectory_List starts(system_, h);
+        check(amdat_rho_compute(device_context(), system_->traj(), starts.handle(), wv_->handle(), first_, last_,
+                                wv_->show_length_scale(), nullptr, 0, AMDAT_PREC_FP32C, &r),
+              "wave density");
+    }
+    rho_ = own_rho(r);
 }
 
-void Wave_Density::copy_from(const Wave_Density &o)
+int Wave_Density::show_n_atoms_looped() const
 {
-    if (!o.rho_) throw Error("wave density copied before it was computed", 0);
-    // The reference deep-copies (wd2 = wd1); the density is read-only after
-    // analyze(), so the copy shares the source's device buffer.
-    amdat_rho_free(device_context(), rho_);
-    rho_ = nullptr;
-    n_atoms_ = o.n_atoms_;
-    first_ = o.first_;
-    last_ = o.last_;
-    wv_ = o.wv_;
-    shared_ = &o;
+    int n = 0;
+    for (int v : n_atoms_) n += v;
+    return n;
 }
-
-const amdat_rho *Wave_Density::device() const { return shared_ ? shared_->device() : rho_; }
 
 void Wave_Density::fetch() const
 {
-    if (!host_.empty()) return;
+    if (host_) return;
     const amdat_rho *r = device();
     if (!r) throw Error("wave density not computed", 0);
     int nf = 0, nv = 0;
     amdat_rho_shape(r, &nf, &nv);
-    host_.assign(2 * (size_t)nf * nv, 0.0);
-    check(amdat_rho_download(device_context(), r, host_.data(), host_.size()), "wave density download");
+    auto h = std::make_shared<std::vector<double>>(2 * (size_t)nf * nv, 0.0);
+    check(amdat_rho_download(device_context(), r, h->data(), h->size()), "wave density download");
+    host_ = h;
 }
 
-double Wave_Density::show_density_re(int t, int wn, int v) const
+std::complex<double> Wave_Density::show_density(int t, int wn, int v) const
 {
     fetch();
     const int K = wv_->offsets()[last_ + 1] - wv_->offsets()[first_];
     const int idx = wv_->offsets()[first_ + wn] - wv_->offsets()[first_] + v;
-    return host_[2 * ((size_t)t * K + idx)];
+    const size_t o = 2 * ((size_t)t * K + idx);
+    return {(*host_)[o], (*host_)[o + 1]};
 }
 
-double Wave_Density::show_density_im(int t, int wn, int v) const
-{
-    fetch();
-    const int K = wv_->offsets()[last_ + 1] - wv_->offsets()[first_];
-    const int idx = wv_->offsets()[first_ + wn] - wv_->offsets()[first_] + v;
-    return host_[2 * ((size_t)t * K + idx) + 1];
-}
+double Wave_Density::show_density_re(int t, int wn, int v) const { return show_density(t, wn, v).real(); }
+
+double Wave_Density::show_density_im(int t, int wn, int v) const { return show_density(t, wn, v).imag(); }
 
 // ---------------------------------------------------------------------------
 // Structure_Factor (src/structure_factor.cpp:55-344, src/analysis_onetime.cpp:22-85)
@@ -283,31 +322,84 @@ static void write_correlation(const std::string &filename, const Wave_Vectors *w
 // ---------------------------------------------------------------------------
 // Intermediate_Scattering_Function (src/intermediate_scattering_function.cpp:237-322)
 
+static void check_densities(const Wave_Density *wd1, const Wave_Density *wd2)
+{
+    if (wd1->show_wavevectors() != wd2->show_wavevectors())
+        throw Error("Wave vectors for density distributions do not match!", 0);
+    if (wd1->show_system() != wd2->show_system()) throw Error("Systems for density distributions do not match!", 0);
+    if (!wd1->device() || !wd2->device()) throw Error("wave density not computed", 0);
+}
+
+static void check_same_range(const Wave_Density *wd1, const Wave_Density *wd2)
+{
+    if (wd1->show_first() != wd2->show_first() || wd1->show_last() != wd2->show_last())
+        throw Error("Wave vector bins for density distributions do not match!", 0);
+}
+
+Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, Wave_Density *wd1,
+                                                                   Wave_Density *wd2, bool parallel)
+    : system_(sys), first_bin_(wd1->show_first()), last_bin_(wd1->show_last())
+{
+    check_densities(wd1, wd2);
+    check_same_range(wd1, wd2);
+    amdat_pairs *p = sys->displacement_pairs(parallel);
+    compute(wd1, wd2, p, -1);
+}
+
+Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, Wave_Density *wd1,
+                                                                   Wave_Density *wd2, int wn, bool parallel)
+    : Intermediate_Scattering_Function(sys, wd1, wd2, wn, wn, parallel)
+{
+}
+
 Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, Wave_Density *wd1,
                                                                    Wave_Density *wd2, int wn1, int wn2,
                                                                    bool parallel)
     : system_(sys), first_bin_(wn1), last_bin_(wn2)
 {
-    if (wd1->show_wavevectors() != wd2->show_wavevectors())
-        throw Error("Wave vectors for density distributions do not match!", 0);
-    if (wd1->show_system() != wd2->show_system()) throw Error("Systems for density distributions do not match!", 0);
-    wv_ = wd1->show_wavevectors();
+    check_densities(wd1, wd2);
     if (wn1 < wd1->show_first() || wn1 < wd2->show_first())
         throw Error("requested wavenumber below range of that contained in fourier density object.", 0);
     if (wn1 > wd1->show_last() || wn1 > wd2->show_last() || wn2 > wd1->show_last() || wn2 > wd2->show_last() ||
         wn2 < wn1)
         throw Error("requested wavenumber above range of that contained in fourier density object.", 0);
-    timetable_ = sys->displacement_times();
+    if (wd1->show_first() != wd2->show_first())
+        throw Error("Wave vector bins for density distributions do not match!", 0);
     amdat_pairs *p = sys->displacement_pairs(parallel);
-    const int ntg = sys->show_n_timegaps(), nsh = wn2 - wn1 + 1;
+    compute(wd1, wd2, p, -1);
+}
+
+Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, int timegapii, Wave_Density *wd1,
+                                                                   Wave_Density *wd2, bool parallel)
+    : system_(sys), first_bin_(wd1->show_first()), last_bin_(wd1->show_last())
+{
+    check_densities(wd1, wd2);
+    check_same_range(wd1, wd2);
+    if (timegapii < 0 || timegapii >= sys->show_n_timegaps()) throw Error("Requested timegap out of range.", 0);
+    amdat_pairs *p = sys->timegap_loop_pairs(timegapii, parallel);
+    compute(wd1, wd2, p, timegapii);
+}
+
+// F over the given pairs (amdat_fkt; FP64 sums normalised by the atoms at
+// t0, src/intermediate_scattering_function.cpp:551-566 and 310-314); takes
+// ownership of pairs.  only_timegap >= 0 keeps that row and zeroes the rest.
+void Intermediate_Scattering_Function::compute(Wave_Density *wd1, Wave_Density *wd2, amdat_pairs *p,
+                                               int only_timegap)
+{
+    System *sys = system_;
+    wv_ = wd1->show_wavevectors();
+    timetable_ = sys->displacement_times();
+    const int ntg = sys->show_n_timegaps(), nsh = last_bin_ - first_bin_ + 1;
     std::vector<double> F((size_t)ntg * nsh), cnt(ntg);
-    int rc = amdat_fkt(device_context(), wd1->device(), wd2->device(), p, wn1 - wd1->show_first(),
-                       wn2 - wd1->show_first(), F.data(), cnt.data());
+    int rc = amdat_fkt(device_context(), wd1->device(), wd2->device(), p, first_bin_ - wd1->show_first(),
+                       last_bin_ - wd1->show_first(), F.data(), cnt.data());
     amdat_pairs_free(device_context(), p);
     check(rc, "intermediate scattering function");
-    correlation_.assign(ntg, std::vector<float>(nsh));
-    for (int t = 0; t < ntg; t++)
+    correlation_.assign(ntg, std::vector<float>(nsh, 0.0f));
+    for (int t = 0; t < ntg; t++) {
+        if (only_timegap >= 0 && t != only_timegap) continue;
         for (int s = 0; s < nsh; s++) correlation_[t][s] = (float)F[(size_t)t * nsh + s];
+    }
 }
 
 void Intermediate_Scattering_Function::write(const std::string &filename) const

@@ -523,6 +523,43 @@ void System::displacement_pair_list(bool fullblock, std::vector<int> &tg, std::v
           "pairs");
 }
 
+amdat_pairs *System::timegap_loop_pairs(int timegapii, bool fullblock) const
+{
+    // src/system.cpp:3972-4023, the break test as there: count == limit
+    std::vector<int> tg, t0, t1;
+    const int f = frt_ ? 1 : 0, nes = n_exponential_steps_, nexp = n_exponentials_;
+    int count = 0;
+    auto add = [&](int g, int a, int b) {
+        tg.push_back(g);
+        t0.push_back(a);
+        t1.push_back(b);
+        return ++count == displacement_limit_;
+    };
+    if (timegapii < nes) {
+        for (int b = 0; b < nexp; b++)
+            if (add(timegapii, nes * b + f, nes * b + f + timegapii)) break;
+    } else if (timegapii < n_timegaps_ - 1 + f) {
+        const int bg = timegapii - nes + 1;
+        bool stop = false;
+        for (int e = 0; e < (fullblock ? nes - 1 : 0) + 1 && !stop; e++)
+            for (int b = 0; b < nexp - bg; b++) {
+                const int a = nes * b + e + f;
+                if (add(timegapii, a, a + nes * bg)) {
+                    stop = true;
+                    break;
+                }
+            }
+    } else if (timegapii == n_timegaps_ - 1 + f) {
+        add(n_timegaps_ - 1, 0, n_timesteps_ - 1);
+    } else {
+        throw Error("Requested timegap out of range.", 0);
+    }
+    amdat_pairs *p = nullptr;
+    check(amdat_pairs_upload(device_context(), (int)tg.size(), tg.data(), t0.data(), t1.data(), n_timegaps_, &p),
+          "pair upload");
+    return p;
+}
+
 amdat_pairs *System::displacement_pairs(bool fullblock, std::vector<int> *tg_out, std::vector<int> *t0_out) const
 {
     int n = 0;
