@@ -1555,7 +1555,11 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     DevBuf<int> dn;
     HIP_TRY(dq.upload(q.data(), q.size(), st));
     HIP_TRY(dn.upload(sh->nvec.data() + 3 * (size_t)v0, 3 * (size_t)K, st));
-    HIP_TRY(launch_sk_adjoint(d_x, d_y, d_z, n_atoms, inv_ell, dn.p, dq.p, K, d_gx, d_gy, d_gz, st));
+    const int g_splits = sk_adjoint_splits(n_atoms, K);
+    DevBuf<double> g_part;
+    if (g_splits > 1) HIP_TRY(g_part.alloc((size_t)g_splits * 3 * n_atoms));
+    HIP_TRY(launch_sk_adjoint(d_x, d_y, d_z, n_atoms, inv_ell, dn.p, dq.p, K, g_splits, g_part.p, d_gx, d_gy, d_gz,
+                              st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
 }

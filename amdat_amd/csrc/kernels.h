@@ -165,8 +165,12 @@ hipError_t launch_fs_batch_reduce(const BatchTraj *trajs, int n_traj, const int 
                                   const int *nsh, int max_nsh, int ntg, const double *norm, const int *out_off,
                                   double *out, hipStream_t st);
 
-// Adjoint of the S(k) misfit: g_j = (2 pi/l) sum_v n_v Im(conj(q_v) e^{ik_v.r_j}).
+// Adjoint of the S(k) misfit: g_j = (2 pi/l) sum_v n_v Im(conj(q_v) e^{ik_v.r_j}),
+// the K vectors in n_splits ranges (sk_adjoint_splits); with more than one,
+// part holds n_splits * 3 * N doubles of partial sums.
+int sk_adjoint_splits(int N, int K);
 hipError_t launch_sk_adjoint(const float *x, const float *y, const float *z, int N, double inv_ell, const int *nvec,
-                             const float2 *q, int K, double *gx, double *gy, double *gz, hipStream_t st);
+                             const float2 *q, int K, int n_splits, double *part, double *gx, double *gy, double *gz,
+                             hipStream_t st);
 
 }  // namespace amdat

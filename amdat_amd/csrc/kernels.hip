@@ -654,19 +654,23 @@ __global__ __launch_bounds__(kPhaseBlock) void rho_direct_kernel(
 //   g_j = (2 pi / l) sum_v n_v Im(conj(q_v) e^{i k_v.r_j}),  q_v = c_v rho_v,
 // with the phase recomputed per (atom, vector) from the exact fixed-point
 // fractions (no N x K table is stored).  Lanes are atoms, vectors are
-// wave-uniform; sums are FP64.
+// wave-uniform; sums are FP64.  blockIdx.y = s takes vectors
+// [s * per_split, (s + 1) * per_split); with one split the block writes
+// g = kscale * sum, else the unscaled partial sums go to
+// part[(s * 3 + c) * N + j] for adjoint_reduce_kernel.
 __global__ __launch_bounds__(kBlock) void sk_adjoint_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N, double inv_ell,
-    const int *__restrict__ nvec, const float2 *__restrict__ q, int K, double kscale, double *__restrict__ gx,
-    double *__restrict__ gy, double *__restrict__ gz)
+    const int *__restrict__ nvec, const float2 *__restrict__ q, int K, int per_split, double kscale,
+    double *__restrict__ gx, double *__restrict__ gy, double *__restrict__ gz, double *__restrict__ part)
 {
     const int j = blockIdx.x * kBlock + threadIdx.x;
     const int jj = j < N ? j : N - 1;
     const uint32_t ux = turns_fixed((double)X[jj] * inv_ell);
     const uint32_t uy = turns_fixed((double)Y[jj] * inv_ell);
     const uint32_t uz = turns_fixed((double)Z[jj] * inv_ell);
+    const int v0 = blockIdx.y * per_split, v1 = min(K, v0 + per_split);
     double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int v = 0; v < K; ++v) {
+    for (int v = v0; v < v1; ++v) {
         const int nx = nvec[3 * v], ny = nvec[3 * v + 1], nz = nvec[3 * v + 2];
         const uint32_t P = (uint32_t)nx * ux + (uint32_t)ny * uy + (uint32_t)nz * uz;
         const float2 e = sincos_turns(P);
@@ -676,11 +680,33 @@ __global__ __launch_bounds__(kBlock) void sk_adjoint_kernel(
         ay = fma((double)ny, t, ay);
         az = fma((double)nz, t, az);
     }
-    if (j < N) {
+    if (j >= N) return;
+    if (gridDim.y == 1) {
         gx[j] = ax * kscale;
         gy[j] = ay * kscale;
         gz[j] = az * kscale;
+    } else {
+        const size_t o = (size_t)blockIdx.y * 3 * N + j;
+        part[o] = ax;
+        part[o + N] = ay;
+        part[o + 2 * (size_t)N] = az;
     }
+}
+
+// g_c[j] = kscale * sum over the splits, in split order, of part[(s * 3 + c)
+// * N + j]: the same result on every run.
+__global__ __launch_bounds__(kBlock) void adjoint_reduce_kernel(const double *__restrict__ part, int N,
+                                                                int n_splits, double kscale, double *__restrict__ gx,
+                                                                double *__restrict__ gy, double *__restrict__ gz)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= N) return;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < n_splits; ++s)
+        for (int c = 0; c < 3; ++c) a[c] += part[((size_t)s * 3 + c) * N + j];
+    gx[j] = a[0] * kscale;
+    gy[j] = a[1] * kscale;
+    gz[j] = a[2] * kscale;
 }
 
 template <int R, int MM, bool PAIR>
@@ -957,13 +983,27 @@ hipError_t launch_msd_reduce(const double *partial, int n_blocks, const int *gro
     return hipGetLastError();
 }
 
+int sk_adjoint_splits(int N, int K)
+{
+    // enough blocks for 8 per CU, at least 32 vectors per split
+    const int blocks = (N + kBlock - 1) / kBlock;
+    const int want = (2048 + blocks - 1) / blocks;
+    return std::max(1, std::min(want, K / 32));
+}
+
 hipError_t launch_sk_adjoint(const float *x, const float *y, const float *z, int N, double inv_ell, const int *nvec,
-                             const float2 *q, int K, double *gx, double *gy, double *gz, hipStream_t st)
+                             const float2 *q, int K, int n_splits, double *part, double *gx, double *gy, double *gz,
+                             hipStream_t st)
 {
     const int blocks = (N + kBlock - 1) / kBlock;
+    const int per_split = (K + n_splits - 1) / n_splits;
+    const double kscale = 2.0 * M_PI * inv_ell;
     (void)hipGetLastError();
-    hipLaunchKernelGGL(sk_adjoint_kernel, dim3(blocks), dim3(kBlock), 0, st, x, y, z, N, inv_ell, nvec, q, K,
-                       2.0 * M_PI * inv_ell, gx, gy, gz);
+    hipLaunchKernelGGL(sk_adjoint_kernel, dim3(blocks, n_splits), dim3(kBlock), 0, st, x, y, z, N, inv_ell, nvec, q,
+                       K, per_split, kscale, gx, gy, gz, part);
+    if (n_splits > 1)
+        hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(blocks), dim3(kBlock), 0, st, part, N, n_splits, kscale, gx,
+                           gy, gz);
     return hipGetLastError();
 }
 
