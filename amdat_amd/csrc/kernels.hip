@@ -13,17 +13,19 @@ namespace amdat {
 
 namespace {
 
-// rho_k(t), compensated FP32.  grid = (chunks, frames, splits).  PAIR:
-// the plan pairs slots sharing ny.
+// rho_k(t), compensated FP32.  Work items (chunk, frame slot, split),
+// numbered by xcd_item.  PAIR: the plan pairs slots sharing ny.
 template <int R, int MM, bool PAIR>
 __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
-    const int *__restrict__ list, int n_list, const int *__restrict__ frames,
+    const int *__restrict__ list, int n_list, const int *__restrict__ frames, int n_frames, int n_chunks,
     const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, const int4 *__restrict__ range,
     int K, int KB, double inv_ell, int per_split, int n_splits, double2 *__restrict__ out)
 {
     AMDAT_PHASE_SHARED(MM);
-    const int chunk = blockIdx.x, fslot = blockIdx.y, split = blockIdx.z;
+    int chunk, row;
+    if (!xcd_item(n_chunks, n_frames * n_splits, chunk, row)) return;
+    const int fslot = row % n_frames, split = row / n_frames;
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
     float hr[R], lr[R], hi[R], li[R];
@@ -33,18 +35,20 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
 }
 
 // Fs per (pair, vector), compensated FP32: sum over the split's atoms of
-// cos(k.dr).  grid = (chunks, pairs, splits).
+// cos(k.dr).  Work items (chunk, pair slot, split), numbered by xcd_item.
 template <int R, int MM, bool PAIR>
 __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
     const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
-    const int *__restrict__ T0, const int *__restrict__ T1, const uint32_t *__restrict__ kpack,
-    const int *__restrict__ kdst, const int4 *__restrict__ range, int K, int KB, double inv_ell,
-    int per_split, int n_splits, double *__restrict__ out)
+    const int *__restrict__ T0, const int *__restrict__ T1, int n_pairs, int n_chunks,
+    const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, const int4 *__restrict__ range, int K, int KB,
+    double inv_ell, int per_split, int n_splits, double *__restrict__ out)
 {
     AMDAT_PHASE_SHARED(MM);
-    const int chunk = blockIdx.x, pslot = blockIdx.y, split = blockIdx.z;
+    int chunk, row;
+    if (!xcd_item(n_chunks, n_pairs * n_splits, chunk, row)) return;
+    const int pslot = row % n_pairs, split = row / n_pairs;
     const int t0 = T0[pslot], t1 = T1[pslot];
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
@@ -718,11 +722,13 @@ hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, cons
     auto kfn = rho_c32_kernel<R, MM, PAIR>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    dim3 grid(kp.n_chunks, n_frames, n_splits);
+    const long rows = (long)n_frames * n_splits;
+    dim3 grid((unsigned)(kXcds * kp.n_chunks * ((rows + kXcds - 1) / kXcds)));
     (void)hipGetLastError();
 
     hipLaunchKernelGGL(kfn, grid, dim3(kPhaseBlock), lds, st, tr.x, tr.y, tr.z, tr.N, list, n_list, frames,
-                       kp.kpack, kp.kdst, kp.range, kp.K, kp.KB, inv_ell, per_split, n_splits, out);
+                       n_frames, kp.n_chunks, kp.kpack, kp.kdst, kp.range, kp.K, kp.KB, inv_ell, per_split, n_splits,
+                       out);
     return hipGetLastError();
 }
 
@@ -736,12 +742,13 @@ hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const
     auto kfn = fs_c32_kernel<R, MM, PAIR>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    dim3 grid(kp.n_chunks, n_pairs, n_splits);
+    const long rows = (long)n_pairs * n_splits;
+    dim3 grid((unsigned)(kXcds * kp.n_chunks * ((rows + kXcds - 1) / kXcds)));
     (void)hipGetLastError();
 
     hipLaunchKernelGGL(kfn, grid, dim3(kPhaseBlock), lds, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box,
-                       tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.range, kp.K, kp.KB, inv_ell,
-                       per_split, n_splits, out);
+                       tr.N, list, n_list, t0, t1, n_pairs, kp.n_chunks, kp.kpack, kp.kdst, kp.range, kp.K, kp.KB,
+                       inv_ell, per_split, n_splits, out);
     return hipGetLastError();
 }
 

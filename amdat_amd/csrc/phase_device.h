@@ -342,6 +342,23 @@ __device__ __forceinline__ void fs_write(const int *__restrict__ kd, const float
     }
 }
 
+// Blocks of a launch are dealt to the 8 XCDs round robin by linear block
+// id, and each XCD has its own L2.  The phase kernels number their work
+// items (chunk, row), row = frame or pair + n_rows * split, so that all
+// chunks of one row land on one XCD one after another: the row's
+// coordinates come from HBM once and from that XCD's L2 for its other
+// chunks.  Grid: kXcds * n_chunks * ceil(n_rows_total / kXcds) blocks;
+// false for the ids past the last row (the whole block returns).
+constexpr int kXcds = 8;
+__device__ __forceinline__ bool xcd_item(int n_chunks, int n_rows_total, int &chunk, int &row)
+{
+    const int L = blockIdx.x;
+    const int x = L % kXcds, k = L / kXcds;
+    chunk = k % n_chunks;
+    row = (k / n_chunks) * kXcds + x;
+    return row < n_rows_total;
+}
+
 #define AMDAT_PHASE_SHARED(MM)                         \
     extern __shared__ float2 tab[];                    \
     __shared__ float2 anc[3 * Geom<MM>::A * 8];          \
