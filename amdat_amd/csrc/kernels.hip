@@ -209,21 +209,22 @@ __global__ __launch_bounds__(kBlock) void fs_shell_kernel(
     }
 }
 
-// grid = (atom blocks, pairs).
+// Work items (atom block, pair), numbered by xcd_atom_block.
 __global__ __launch_bounds__(kBlock) void msd_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
     const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
-    const int *__restrict__ T0, const int *__restrict__ T1, double *__restrict__ partial)
+    const int *__restrict__ T0, const int *__restrict__ T1, int n_pairs, int n_blocks, double *__restrict__ partial)
 {
     __shared__ double red[kBlock];
-    const int p = blockIdx.y;
+    int bx, p;
+    if (!xcd_atom_block(n_blocks, n_pairs, bx, p)) return;
     const int t0 = T0[p], t1 = T1[p];
     const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
     const double L0x = box[3 * t0], L0y = box[3 * t0 + 1], L0z = box[3 * t0 + 2];
     const double L1x = box[3 * t1], L1y = box[3 * t1 + 1], L1z = box[3 * t1 + 2];
     double a2 = 0.0, a4 = 0.0;
-    for (int a = blockIdx.x * kBlock + threadIdx.x; a < n_list; a += gridDim.x * kBlock) {
+    for (int a = bx * kBlock + threadIdx.x; a < n_list; a += n_blocks * kBlock) {
         const int j = list ? list[a] : a;
         const double dx = ((double)X[o1 + j] - (double)X[o0 + j]) + ((double)IX[o1 + j] * L1x - (double)IX[o0 + j] * L0x);
         const double dy = ((double)Y[o1 + j] - (double)Y[o0 + j]) + ((double)IY[o1 + j] * L1y - (double)IY[o0 + j] * L0y);
@@ -235,8 +236,8 @@ __global__ __launch_bounds__(kBlock) void msd_kernel(
     a2 = block_sum(a2, red);
     a4 = block_sum(a4, red);
     if (threadIdx.x == 0) {
-        partial[((size_t)p * gridDim.x + blockIdx.x) * 2] = a2;
-        partial[((size_t)p * gridDim.x + blockIdx.x) * 2 + 1] = a4;
+        partial[((size_t)p * n_blocks + bx) * 2] = a2;
+        partial[((size_t)p * n_blocks + bx) * 2 + 1] = a4;
     }
 }
 
@@ -255,13 +256,14 @@ __global__ __launch_bounds__(kBlock) void disp_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
     const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int *__restrict__ T0,
-    const int *__restrict__ T1, const int *__restrict__ TG, int n_bins, int hist_mode, float bin_size,
-    double *__restrict__ partial, unsigned long long *__restrict__ hist)
+    const int *__restrict__ T1, const int *__restrict__ TG, int n_pairs, int n_blocks, int n_bins, int hist_mode,
+    float bin_size, double *__restrict__ partial, unsigned long long *__restrict__ hist)
 {
 #pragma clang fp contract(off)
     extern __shared__ unsigned int sh_hist[];
     __shared__ double red[kBlock];
-    const int p = blockIdx.y;
+    int bx, p;
+    if (!xcd_atom_block(n_blocks, n_pairs, bx, p)) return;
     const int t0 = T0[p], t1 = T1[p];
     const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
     const float *B0 = box + 3 * t0, *B1 = box + 3 * t1;
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void disp_kernel(
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const float *src[3] = {X, Y, Z};
     const int16_t *img[3] = {IX, IY, IZ};
-    for (int a = blockIdx.x * kBlock + threadIdx.x; a < n_list; a += gridDim.x * kBlock) {
+    for (int a = bx * kBlock + threadIdx.x; a < n_list; a += n_blocks * kBlock) {
         const int j = list ? list[a] : a;
         float u[3];
 #pragma unroll
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void disp_kernel(
 #pragma unroll
     for (int k = 0; k < 6; k++) s[k] = block_sum(s[k], red);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 6; k++) partial[((size_t)p * gridDim.x + blockIdx.x) * 6 + k] = s[k];
+        for (int k = 0; k < 6; k++) partial[((size_t)p * n_blocks + bx) * 6 + k] = s[k];
     if (n_bins > 0) {
         __syncthreads();
         unsigned long long *row = hist + (size_t)TG[p] * n_bins;
@@ -909,9 +911,11 @@ hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *grou
 hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
                       int n_pairs, int n_blocks, double *partial, hipStream_t st)
 {
+    if (n_pairs <= 0 || n_blocks <= 0) return hipSuccess;
     (void)hipGetLastError();
-    hipLaunchKernelGGL(msd_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix,
-                       tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, partial);
+    const long grid = (long)kXcds * ((n_blocks + kXcds - 1) / kXcds) * n_pairs;
+    hipLaunchKernelGGL(msd_kernel, dim3((unsigned)grid), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz,
+                       tr.box, tr.N, list, n_list, t0, t1, n_pairs, n_blocks, partial);
     return hipGetLastError();
 }
 
@@ -919,11 +923,12 @@ hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int
                        int n_pairs, int n_blocks, int n_bins, int hist_mode, float bin_size, double *partial,
                        unsigned long long *hist, hipStream_t st)
 {
-    if (n_pairs <= 0) return hipSuccess;
+    if (n_pairs <= 0 || n_blocks <= 0) return hipSuccess;
     (void)hipGetLastError();
-    hipLaunchKernelGGL(disp_kernel, dim3(n_blocks, n_pairs), dim3(kBlock), sizeof(unsigned int) * (size_t)n_bins, st,
-                       tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, tg, n_bins, hist_mode,
-                       bin_size, partial, hist);
+    const long grid = (long)kXcds * ((n_blocks + kXcds - 1) / kXcds) * n_pairs;
+    hipLaunchKernelGGL(disp_kernel, dim3((unsigned)grid), dim3(kBlock), sizeof(unsigned int) * (size_t)n_bins, st,
+                       tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, tg, n_pairs, n_blocks,
+                       n_bins, hist_mode, bin_size, partial, hist);
     return hipGetLastError();
 }
 

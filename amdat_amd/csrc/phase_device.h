@@ -359,6 +359,20 @@ __device__ __forceinline__ bool xcd_item(int n_chunks, int n_rows_total, int &ch
     return row < n_rows_total;
 }
 
+// Pair kernels (MSD, displacement statistics): atom block bx of pair p
+// runs on XCD bx % kXcds, each XCD taking its atom blocks pair after pair,
+// so the frame slices that nearby pairs share come from its L2.  Grid:
+// kXcds * ceil(n_blocks / kXcds) * n_pairs blocks; false for the padding ids.
+__device__ __forceinline__ bool xcd_atom_block(int n_blocks, int n_pairs, int &bx, int &p)
+{
+    const int L = blockIdx.x;
+    const int x = L % kXcds, k = L / kXcds;
+    const int nb8 = (n_blocks + kXcds - 1) / kXcds;
+    bx = (k % nb8) * kXcds + x;
+    p = k / nb8;
+    return bx < n_blocks && p < n_pairs;
+}
+
 #define AMDAT_PHASE_SHARED(MM)                         \
     extern __shared__ float2 tab[];                    \
     __shared__ float2 anc[3 * Geom<MM>::A * 8];          \

@@ -388,6 +388,60 @@ def bench_adjoint(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_extras(args):
+    """The pair-loop statistics, selections and bins on the 64,000-atom melt
+    (T = 1,001, 1,046 pairs): per entry point the call time (synchronous
+    calls, results copied to the host) and the algorithmic HBM bytes of its
+    kernels over that time.  Kernel durations: rocprofv3 --kernel-trace on
+    this workload."""
+    from amdat_amd import _abi
+    cfg, tr, _, _, (tg, t0, t1, ntg) = build_workload(args.small)
+    ctx = _abi.Context(0)
+    d = _abi.Trajectory(ctx, *tr.arrays)
+    P = _abi.Pairs(ctx, tg, t0, t1, ntg)
+    N, T, NP = tr.N, tr.T, len(tg)
+    nexp, nes = cfg["nexp"], cfg["nes"]
+    gtg, g0, g1 = _abi.ts_gap_pairs(nexp, nes, 0, 0, 10)
+    G = _abi.Pairs(ctx, gtg, g0, g1, ntg)
+    lo = np.zeros((T, 3), np.float32)
+    hi = np.full((T, 3), np.float32(cfg["L"]), np.float32)
+    cluster = _abi.AtomList(ctx, d, np.arange(0, N, 64))           # 1,000 atoms
+    Lh = float(np.float32(cfg["L"]) / 2)
+    # algorithmic bytes: 18 B per distinct atom-frame the pairs touch
+    # (FP32 xyz + int16 images), 16 B per atom-frame for the bins
+    fr_all = len(np.unique(np.concatenate([t0, t1])))
+    fr_gap = len(np.unique(np.concatenate([g0, g1])))
+    calls = {
+        "msd_ngp": (lambda: _abi.msd_ngp(ctx, d, None, P), 18.0 * N * fr_all),
+        "displacements (100 van Hove bins)": (lambda: _abi.displacements(ctx, d, None, P, 100, 3.0),
+                                              18.0 * N * fr_all),
+        "displacement_dist (one gap, 10 pairs)": (lambda: _abi.displacement_dist(ctx, d, None, G, 100, 4.0),
+                                                  18.0 * N * fr_gap),
+        "select_displacement (find_fast, one gap)": (
+            lambda: _abi.TimedList.select_displacement(ctx, d, None, nexp, nes, 0, 0, 10, 0, 0.5, 0.0).close(),
+            2 * 18.0 * N * fr_gap),
+        "bins_assign (10x10x10)": (lambda: _abi.bins_assign(ctx, d, (10, 10, 10), lo, hi), 16.0 * N * T),
+        "bins_distance point": (lambda: _abi.bins_distance(ctx, d, None, _abi.DIST_POINT, 0.5, 40,
+                                                           point=(Lh, Lh, Lh)), 16.0 * N * T),
+        "bins_distance list (1,000 atoms)": (lambda: _abi.bins_distance(ctx, d, None, _abi.DIST_LIST, 0.25, 16,
+                                                                        cluster=cluster), 16.0 * N * T),
+    }
+    res = {}
+    for name, (fn, nbytes) in calls.items():
+        fn()
+        ts = []
+        for _ in range(max(1, args.steps)):
+            c0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - c0)
+        t = float(np.median(ts))
+        res[name] = {"call_ms": 1e3 * t, "algorithmic_bytes": nbytes, "GBps_at_call_time": nbytes / t / 1e9}
+        log("%-42s %9.3f ms  %8.1f GB/s" % (name, 1e3 * t, nbytes / t / 1e9))
+    result = {"workload": "pair-loop statistics, selections and bins on the precision-form melt (N=%d, T=%d, "
+                          "%d pairs)" % (N, T, NP), "calls": res, "peak_hbm_GBps": PEAK_HBM_GBS}
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,10 +450,10 @@ def main():
     ap.add_argument("--small", action="store_true", help="reduced workload for a quick check")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", choices=["precision", "batch", "scale", "adjoint"], default="precision",
+    ap.add_argument("--workload", choices=["precision", "batch", "scale", "adjoint", "extras"], default="precision",
                     help="precision: the 64,000-atom melt (default); batch: 256 trajectories in one launch; "
                          "scale: 1,048,576 atoms x 2,001 frames, all time-origin pairs; adjoint: S(k)-misfit "
-                         "loss and gradient, 32,768 atoms")
+                         "loss and gradient, 32,768 atoms; extras: pair-loop statistics, selections and bins")
     ap.add_argument("--no-tune", action="store_true", help="batch workload: skip the autotuner")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the HBM byte counters)")
@@ -426,6 +480,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "extras":
+        bench_extras(args)
         return
 
     import torch
