@@ -86,7 +86,26 @@ struct amdat_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    // Grow-only device scratch (ctx_scratch), for entry points that
+    // synchronise their stream before returning: one call's kernels are done
+    // with it before the next call writes it.
+    DevBuf<char> ws[4];
 };
+
+namespace {
+template <typename T>
+hipError_t ctx_scratch(amdat_ctx *ctx, int slot, size_t count, T **out)
+{
+    DevBuf<char> &b = ctx->ws[slot];
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (b.n < bytes) {
+        hipError_t e = b.alloc(bytes);
+        if (e != hipSuccess) return e;
+    }
+    *out = reinterpret_cast<T *>(b.p);
+    return hipSuccess;
+}
+}  // namespace
 
 struct amdat_traj {
     TrajDev dev;
@@ -137,6 +156,7 @@ struct amdat_shells {
     std::map<std::tuple<int, int, int, int>, std::unique_ptr<KPlanOwned>> plans;  // (first, last, R, mode)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> choice;               // (first, last, c) -> (R, mode)
     int hint_R = 0, hint_pair = 0;  // amdat_shells_plan_hint (0: automatic)
+    DevBuf<int> d_nvec;             // the integer vectors on the device, uploaded on first use
 };
 
 struct amdat_pairs {
@@ -1517,21 +1537,22 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     view.T = 1;
     view.x = d_x; view.y = d_y; view.z = d_z;
     const double inv_ell = 1.0 / (double)length_scale;
-    // forward: rho for the frame (FP64 result)
-    const int zero = 0;
-    DevBuf<int> dfr;
-    HIP_TRY(dfr.upload(&zero, 1, st));
+    // forward: rho for the frame (FP64 result); scratch from the context
+    // (the call ends with a stream synchronisation)
+    int *dfr = nullptr;
+    HIP_TRY(ctx_scratch(ctx, 0, 1, &dfr));
+    HIP_TRY(hipMemsetAsync(dfr, 0, sizeof(int), st));
     const int n_splits = choose_splits((long)kp.n_chunks, n_atoms);
-    DevBuf<double2> part, rho;
-    HIP_TRY(rho.alloc(K));
-    if (n_splits > 1) HIP_TRY(part.alloc((size_t)n_splits * K));
+    double2 *rho = nullptr;
+    HIP_TRY(ctx_scratch(ctx, 1, (size_t)K * (n_splits > 1 ? n_splits + 1 : 1), &rho));
+    double2 *part = rho + K;
     // one frame and few vectors per shell: no averaging over frames, so the
     // direct exact-phase kernel (about 1 ulp per term) is used
-    HIP_TRY(launch_rho(AMDAT_PREC_FP32X, view, nullptr, n_atoms, dfr.p, 1, kp, inv_ell, n_splits,
-                       n_splits > 1 ? part.p : rho.p, st));
-    if (n_splits > 1) HIP_TRY(launch_split_reduce2(part.p, 1, n_splits, K, rho.p, st));
+    HIP_TRY(launch_rho(AMDAT_PREC_FP32X, view, nullptr, n_atoms, dfr, 1, kp, inv_ell, n_splits,
+                       n_splits > 1 ? part : rho, st));
+    if (n_splits > 1) HIP_TRY(launch_split_reduce2(part, 1, n_splits, K, rho, st));
     std::vector<double2> h(K);
-    HIP_TRY(hipMemcpyAsync(h.data(), rho.p, sizeof(double2) * K, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(h.data(), rho, sizeof(double2) * K, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     // S per shell, loss, and the adjoint weights q_v = c_v rho_v (FP64 on the host: K values)
     std::vector<float2> q(K);
@@ -1551,15 +1572,15 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     *loss = L;
     if (!d_gx) return AMDAT_OK;
     // backward: the adjoint kernel recomputes every phase
-    DevBuf<float2> dq;
-    DevBuf<int> dn;
-    HIP_TRY(dq.upload(q.data(), q.size(), st));
-    HIP_TRY(dn.upload(sh->nvec.data() + 3 * (size_t)v0, 3 * (size_t)K, st));
+    float2 *dq = nullptr;
+    HIP_TRY(ctx_scratch(ctx, 2, (size_t)K, &dq));
+    HIP_TRY(hipMemcpyAsync(dq, q.data(), sizeof(float2) * K, hipMemcpyHostToDevice, st));
+    if (!sh->d_nvec.p) HIP_TRY(sh->d_nvec.upload(sh->nvec.data(), sh->nvec.size(), st));
     const int g_splits = sk_adjoint_splits(n_atoms, K);
-    DevBuf<double> g_part;
-    if (g_splits > 1) HIP_TRY(g_part.alloc((size_t)g_splits * 3 * n_atoms));
-    HIP_TRY(launch_sk_adjoint(d_x, d_y, d_z, n_atoms, inv_ell, dn.p, dq.p, K, g_splits, g_part.p, d_gx, d_gy, d_gz,
-                              st));
+    double *g_part = nullptr;
+    if (g_splits > 1) HIP_TRY(ctx_scratch(ctx, 3, (size_t)g_splits * 3 * n_atoms, &g_part));
+    HIP_TRY(launch_sk_adjoint(d_x, d_y, d_z, n_atoms, inv_ell, sh->d_nvec.p + 3 * (size_t)v0, dq, K, g_splits, g_part,
+                              d_gx, d_gy, d_gz, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
 }
