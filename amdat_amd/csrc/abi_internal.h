@@ -43,7 +43,7 @@ struct PlanLayout {
     std::vector<uint32_t> pack;  // [n_chunks][R][512]: (nx+128) | (ny+128)<<8 | nz<<16
     std::vector<int> dst;        // same layout: destination index | bit 30 (conjugate), or -1
     std::vector<int> range;      // [n_chunks][4]: nx lo, nx hi, max |ny| | max nz << 16,
-                                 // units with vectors | first unpaired unit << 16
+                                 // units with vectors | paired-wave mask << 16 (bit w: wave w)
     long conflicts = 0;          // extra LDS passes summed over (chunk, slot, half-wave, axis)
     bool pair = false;           // slots (2p, 2p+1) of a thread share ny, or nz (kPackSwap)
     long paired_slots = 0, unpaired_slots = 0;  // slots of waves that run the sums, by kind, each chunk's

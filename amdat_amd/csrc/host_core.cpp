@@ -425,10 +425,26 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
     // pairs of two vectors with one ny, then (of the vectors left) two with
     // one nz (swapped roles), dealt R / 2 pairs per unit; each vector left
     // over either forms a pair with a padding slot (mode 1) or goes to
-    // unpaired units (mode 2).  All paired units come first, padded to whole
-    // waves, so every wave is either paired or unpaired.
+    // unpaired units (mode 2).  Every wave is either paired or unpaired: in
+    // mode 2 the two kinds fill separate waves of 64 units, emitted in nx
+    // order as they fill, so a chunk's waves of both kinds cover a narrow
+    // nx range.
     const int G = pair ? R / 2 : R;
     std::vector<std::vector<Vec>> units_p, units_s;
+    // mode 2: waves emitted as they fill (is_p: the wave is paired)
+    std::vector<std::vector<Vec>> waved;
+    std::vector<char> wave_p;
+    size_t sent_p = 0, sent_s = 0;
+    auto emit = [&](std::vector<std::vector<Vec>> &src, size_t &sent, bool paired_kind, bool flush) {
+        while (src.size() - sent >= 64 || (flush && src.size() > sent)) {
+            const size_t take = std::min<size_t>(64, src.size() - sent);
+            for (size_t i = 0; i < take; i++) waved.push_back(src[sent + i]);
+            for (size_t i = take; i < 64; i++)
+                waved.push_back(std::vector<Vec>(R, Vec{src[sent + take - 1][0].nx, 0, 0, -1}));
+            wave_p.push_back(paired_kind);
+            sent += take;
+        }
+    };
     auto deal = [&](const std::vector<std::array<Vec, 2>> &grp, int per_unit, int S, int nx,
                     std::vector<std::vector<Vec>> &dst) {
         const int cnt = (int)grp.size(), U = (cnt + per_unit - 1) / per_unit;
@@ -472,21 +488,35 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
         }
         if (pair) deal(grp, G, 2, nx, units_p);
         deal(single, R, 1, nx, units_s);
+        if (mixed) {
+            emit(units_p, sent_p, true, false);
+            emit(units_s, sent_s, false, false);
+        }
         g0 = g1;
     }
-    if (!units_s.empty())
-        while (units_p.size() % 64) units_p.push_back(std::vector<Vec>(R, Vec{units_p.back()[0].nx, 0, 0, -1}));
-    const int n_paired = (int)units_p.size();
-    std::vector<std::vector<Vec>> units = std::move(units_p);
-    units.insert(units.end(), units_s.begin(), units_s.end());
+    // unit_p[u]: unit u belongs to a paired wave
+    std::vector<std::vector<Vec>> units;
+    std::vector<char> unit_p;
+    if (mixed) {
+        emit(units_p, sent_p, true, true);
+        emit(units_s, sent_s, false, true);
+        units = std::move(waved);
+        for (char k : wave_p) unit_p.insert(unit_p.end(), 64, k);
+    } else {
+        units = std::move(units_p);
+        unit_p.assign(units.size(), pair ? 1 : 0);
+        units.insert(units.end(), units_s.begin(), units_s.end());
+        unit_p.resize(units.size(), 0);
+    }
     const int n_units = (int)units.size();
     L.n_chunks = std::max(1, (n_units + 511) / 512);
     while ((int)units.size() < L.n_chunks * 512) {
         const int nx = units.empty() ? 0 : units.back()[0].nx;
         units.push_back(std::vector<Vec>(R, Vec{nx, 0, 0, -1}));
+        unit_p.push_back(0);
     }
     for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass)
-        order_half_wave(units, h0, R, (int)h0 < n_paired, E);
+        order_half_wave(units, h0, R, unit_p[h0] != 0, E);
     // emit
     const size_t n = (size_t)L.n_chunks * L.KB;
     L.pack.resize(n);
@@ -512,12 +542,18 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
         L.range[4 * c + 1] = xhi;
         L.range[4 * c + 2] = my | (mz << 16);
         const int n_act = std::min(512, n_units - c * 512);
-        const int first_unpaired = std::max(0, std::min(512, n_paired - c * 512));
-        L.range[4 * c + 3] = n_act | (first_unpaired << 16);
+        // bit w of the mask: wave w holds pairs
+        int mask = 0, ap = 0;
+        for (int w = 0; w < 8 && w * 64 < n_act; w++)
+            if (unit_p[(size_t)c * 512 + w * 64]) {
+                mask |= 1 << w;
+                ap++;
+            }
+        L.range[4 * c + 3] = n_act | (mask << 16);
         // Work of the waves that run the sums, in slots.  A block's 8 waves
         // share 4 SIMDs two by two, so a chunk with a active waves takes as
         // long as ceil(a / 4) * 4 of them.
-        const int a = (n_act + 63) / 64, ap = std::min(a, (first_unpaired + 63) / 64);
+        const int a = (n_act + 63) / 64;
         if (a > 0) {
             const double scale = (double)(((a + 3) / 4) * 4) / a;
             L.paired_slots += (long)(ap * 64.0 * R * scale);
@@ -533,7 +569,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
                     zs.push_back(E.own(v));
                     if (r == 0) xs.push_back(v.nx);
                 }
-                if (c * 512 + h >= n_paired || r % 2 == 0) L.conflicts += passes_minus_one(ys);
+                if (!unit_p[(size_t)c * 512 + h] || r % 2 == 0) L.conflicts += passes_minus_one(ys);
                 L.conflicts += passes_minus_one(zs);
                 if (r == 0) L.conflicts += passes_minus_one(xs);
             }

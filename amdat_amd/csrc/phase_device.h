@@ -192,10 +192,10 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
     using Gm = Geom<MM>;
     const int tid = threadIdx.x;
     // waves whose units all lie past the chunk's last real unit skip the
-    // sums; in paired plans, waves before the first unpaired unit hold pairs
+    // sums; in paired plans, bit 16 + w of rng.w marks wave w as paired
     const int wave0 = __builtin_amdgcn_readfirstlane(tid >> 6) * 64;
     const bool active = wave0 < (rng.w & 0xFFFF);
-    const bool paired = wave0 < (rng.w >> 16);
+    const bool paired = (rng.w >> (16 + (wave0 >> 6))) & 1;
     const float2 *px = nullptr, *py[R], *pz[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {

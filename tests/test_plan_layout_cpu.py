@@ -97,11 +97,12 @@ def test_layout_lowers_bank_conflicts(qvec):
 @pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("bins,R", [((2, 100), 8), ((2, 100), 4), ((40, 41), 8), ((2, 3), 4), ((95, 100), 4)])
 def test_paired_layout_invariants(qvec, bins, R, mode):
-    """Paired plans: the same invariants; units before the first unpaired
-    unit (whole waves) hold pairs sharing ny, or nz with both slots flagged
-    as swapped, and the rest hold single vectors; chunk count from pairing
-    on ny first, then on nz, and the vectors left padded into pairs (mode 1)
-    or R per unpaired unit (mode 2)."""
+    """Paired plans: the same invariants; the units of the waves the chunk's
+    mask marks as paired hold pairs sharing ny, or nz with both slots
+    flagged as swapped, and the other waves single vectors; chunk count from
+    pairing on ny first, then on nz, and the vectors left padded into pairs
+    (mode 1: every wave paired) or R per unit in unpaired waves of their
+    own (mode 2: paired and unpaired waves of 64 units each, interleaved)."""
     off, nvec = ref_shells(qvec, *bins)
     K = len(nvec)
     n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R, pair=mode)
@@ -124,12 +125,19 @@ def test_paired_layout_invariants(qvec, bins, R, mode):
         else:
             up += -(-groups // (R // 2))
             us += -(-singles // R)
-    n_paired = up if us == 0 else -(-up // 64) * 64
-    assert n == max(1, -(-(n_paired + us) // 512))
-    first = rng[:, 3] >> 16
-    unit = np.arange(n)[:, None] * 512 + np.arange(512)[None, :]  # [n, 512] global unit index
-    assert np.array_equal(first, np.clip(n_paired - np.arange(n) * 512, 0, 512))
-    is_p = unit < n_paired
+    mask = rng[:, 3] >> 16
+    wave_p = (mask[:, None] >> np.arange(8)[None, :]) & 1                # [n, 8]
+    is_p = np.repeat(wave_p, 64, axis=1).astype(bool)                   # [n, 512] per unit
+    act_w = -(-(rng[:, 3] & 0xFFFF) // 64)
+    if mode == 1:
+        assert n == max(1, -(-up // 512))
+        for c in range(n):                                              # every active wave is paired
+            assert np.array_equal(wave_p[c], (np.arange(8) < act_w[c]).astype(int))
+    else:
+        wp, ws = -(-up // 64), -(-us // 64)
+        assert n == max(1, -(-(wp + ws) // 8))
+        assert int(wave_p.sum()) == wp
+        assert np.all(wave_p[np.arange(8)[None, :] >= act_w[:, None]] == 0)
     assert np.array_equal(sw[:, 0::2, :], sw[:, 1::2, :])       # both slots of a pair agree
     if bins == (2, 100):
         assert np.any(sw)                                         # nz pairs occur
