@@ -270,13 +270,15 @@ int shell_range_M(const amdat_shells *sh, int first, int last)
 
 // Modelled cost of a layout per atom: every slot of a wave that runs the
 // sums costs c VALU ops per lane, c - 2 in paired waves (q = e_x e_y formed
-// once per pair), counted in whole SIMD rounds of 4 waves per chunk, and
+// once per pair), c - 3 in quad waves (once per four slots), counted in
+// whole SIMD rounds of 4 waves per chunk, and
 // every chunk the fixed part of choose_R.  (Scale-form Fs, 1,500 vectors in
 // one chunk: R = 8 paired, 5 active waves, ran 11.4 s against 10.1 s for
 // R = 4 unpaired with 7.)
 double layout_cost(const PlanLayout &L, double c)
 {
-    return (double)L.n_chunks * 12000.0 + (double)L.paired_slots * (c - 2.0) + (double)L.unpaired_slots * c;
+    return (double)L.n_chunks * 12000.0 + (double)L.quad_slots * (c - 3.0) + (double)L.paired_slots * (c - 2.0) +
+           (double)L.unpaired_slots * c;
 }
 
 // R_force > 0 fixes the vectors per thread and the layout mode to
@@ -351,7 +353,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
     kp.KB = lay.KB;
     kp.n_chunks = lay.n_chunks;
     kp.MM = plan_table_width(M);
-    kp.pair = mode != 0 ? 1 : 0;
+    kp.pair = mode == 3 ? 2 : (mode != 0 ? 1 : 0);
     hipStream_t st = stream_of(ctx);
     HIP_TRY(P->kpack.upload(lay.pack.data(), lay.pack.size(), st));
     HIP_TRY(P->kdst.upload(lay.dst.data(), lay.dst.size(), st));
@@ -576,7 +578,7 @@ int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair)
     clear_error();
     if (!sh) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: null table");
     if (R != 0 && R != 2 && R != 4 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: R must be 0, 2, 4 or 8");
-    if (pair < 0 || pair > 2) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pair must be 0, 1 or 2");
+    if (pair < 0 || pair > 3) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pair must be 0, 1, 2 or 3");
     if (pair && R != 4 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pairs need R = 4 or 8");
     sh->hint_R = R;
     sh->hint_pair = pair;

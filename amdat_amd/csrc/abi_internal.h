@@ -46,12 +46,15 @@ struct PlanLayout {
                                  // units with vectors | paired-wave mask << 16 (bit w: wave w)
     long conflicts = 0;          // extra LDS passes summed over (chunk, slot, half-wave, axis)
     bool pair = false;           // slots (2p, 2p+1) of a thread share ny, or nz (kPackSwap)
-    long paired_slots = 0, unpaired_slots = 0;  // slots of waves that run the sums, by kind, each chunk's
-                                                // rounded up to whole SIMD rounds (4 waves)
+    bool quad = false;           // mode 3: the mask marks waves whose slots (4q .. 4q+3) share ny
+    long quad_slots = 0, paired_slots = 0, unpaired_slots = 0;  // slots of waves that run the sums, by
+                                                                // kind, each chunk's rounded up to whole
+                                                                // SIMD rounds (4 waves)
 };
 // mode 0: unpaired; 1: paired, vectors without a partner padded; 2: paired,
-// vectors without a partner in unpaired waves.  MM: table half-width of the
-// kernels that read the plan (bank model only).
+// vectors without a partner in unpaired waves; 3: quads sharing ny where a
+// ny class holds four, then pairs as mode 1, in waves of one kind each.  MM:
+// table half-width of the kernels that read the plan (bank model only).
 PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM);
 
 // ---- trajectory ingestion (host_io.cpp) ----

@@ -354,10 +354,11 @@ long passes_minus_one(const std::vector<int> &vals)
 
 // Slot order of one half-wave of units (kLanesPerPass consecutive units):
 // greedy placement that keeps reads of one slot by the half-wave's lanes in
-// distinct bank pairs where it can.  Unpaired layouts place single slots
-// (each reads y and z); paired layouts place pairs of slots (2p, 2p + 1)
-// that share ny, of which only slot 2p reads y, in either orientation.
-void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, bool pair, const Entries &E)
+// distinct bank pairs where it can.  Groups of S slots (S = 1: single
+// slots, each reading y and z; S = 2: pairs (2p, 2p + 1) sharing ny, of
+// which only slot 2p reads y, in either orientation; S = 4: quads sharing
+// ny, read by their first slot) are placed whole.
+void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, int S, const Entries &E)
 {
     std::vector<std::array<std::vector<int>, kBankPairs>> by(R), bz(R);
     auto add = [](std::array<std::vector<int>, kBankPairs> &b, int v) {
@@ -368,7 +369,7 @@ void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, boo
         const auto &s = b[((v % kBankPairs) + kBankPairs) % kBankPairs];
         return std::find(s.begin(), s.end(), v) == s.end() ? (int)s.size() : 0;
     };
-    const int S = pair ? 2 : 1, G = R / S;
+    const int G = R / S, NO = S == 2 ? 2 : 1;  // orientations tried
     for (size_t u = h0; u < h0 + kLanesPerPass && u < units.size(); u++) {
         std::vector<Vec> &un = units[u];
         std::vector<Vec> placed(R);
@@ -379,17 +380,18 @@ void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, boo
                 if (used_v[i]) continue;
                 for (int p = 0; p < G; p++) {
                     if (used_p[p]) continue;
-                    for (int o = 0; o < S; o++) {
+                    for (int o = 0; o < NO; o++) {
                         const Vec &a = un[S * i + o];
                         int c = cost1(by[S * p], E.shared(a)) + cost1(bz[S * p], E.own(a));
-                        if (pair) c += cost1(bz[S * p + 1], E.own(un[S * i + 1 - o]));
+                        if (S == 2) c += cost1(bz[S * p + 1], E.own(un[S * i + 1 - o]));
+                        for (int q = S == 2 ? 2 : 1; q < S; q++) c += cost1(bz[S * p + q], E.own(un[S * i + q]));
                         if (c < bc) { bc = c; bv = i; bp = p; bo = o; }
                     }
                 }
             }
             used_v[bv] = used_p[bp] = 1;
             for (int q = 0; q < S; q++) {
-                const Vec &v = un[S * bv + (q == 0 ? bo : 1 - bo)];
+                const Vec &v = un[S * bv + (q >= 2 ? q : (q == 0 ? bo : (S == 1 ? 0 : 1 - bo)))];
                 placed[S * bp + q] = v;
                 if (q == 0) add(by[S * bp], E.shared(v));
                 add(bz[S * bp + q], E.own(v));
@@ -404,11 +406,12 @@ void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, boo
 PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
 {
     const Entries E{table_oy(MM), table_oz(MM)};
-    const bool pair = mode != 0, mixed = mode == 2;
+    const bool pair = mode != 0, mixed = mode == 2, quad = mode == 3;
     PlanLayout L;
     L.R = R;
     L.KB = 512 * R;
     L.pair = pair;
+    L.quad = quad;
     std::vector<Vec> vs(K);
     for (int v = 0; v < K; v++) {
         int nx = nvec[3 * v], ny = nvec[3 * v + 1], nz = nvec[3 * v + 2], d = v;
@@ -424,28 +427,29 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
     // dealt with stride over ceil(count / R) units.  Paired: per nx group,
     // pairs of two vectors with one ny, then (of the vectors left) two with
     // one nz (swapped roles), dealt R / 2 pairs per unit; each vector left
-    // over either forms a pair with a padding slot (mode 1) or goes to
-    // unpaired units (mode 2).  Every wave is either paired or unpaired: in
-    // mode 2 the two kinds fill separate waves of 64 units, emitted in nx
-    // order as they fill, so a chunk's waves of both kinds cover a narrow
-    // nx range.
-    const int G = pair ? R / 2 : R;
-    std::vector<std::vector<Vec>> units_p, units_s;
-    // mode 2: waves emitted as they fill (is_p: the wave is paired)
+    // over either forms a pair with a padding slot (modes 1, 3) or goes to
+    // unpaired units (mode 2).  Mode 3 first takes quads, four vectors with
+    // one ny, dealt R / 4 per unit.  Every wave holds one kind of unit: in
+    // modes 2 and 3 the kinds fill separate waves of 64 units, emitted in nx
+    // order as they fill, so a chunk's waves of every kind cover a narrow nx
+    // range.
+    std::vector<std::vector<Vec>> units_q, units_p, units_s;
+    // modes 2, 3: waves emitted as they fill, with their group size
     std::vector<std::vector<Vec>> waved;
-    std::vector<char> wave_p;
-    size_t sent_p = 0, sent_s = 0;
-    auto emit = [&](std::vector<std::vector<Vec>> &src, size_t &sent, bool paired_kind, bool flush) {
+    std::vector<char> wave_g;
+    size_t sent_q = 0, sent_p = 0, sent_s = 0;
+    auto emit = [&](std::vector<std::vector<Vec>> &src, size_t &sent, char g, bool flush) {
         while (src.size() - sent >= 64 || (flush && src.size() > sent)) {
             const size_t take = std::min<size_t>(64, src.size() - sent);
             for (size_t i = 0; i < take; i++) waved.push_back(src[sent + i]);
             for (size_t i = take; i < 64; i++)
                 waved.push_back(std::vector<Vec>(R, Vec{src[sent + take - 1][0].nx, 0, 0, -1}));
-            wave_p.push_back(paired_kind);
+            wave_g.push_back(g);
             sent += take;
         }
     };
-    auto deal = [&](const std::vector<std::array<Vec, 2>> &grp, int per_unit, int S, int nx,
+    // deal groups of S vectors, per_unit groups per unit, with stride
+    auto deal = [&](const std::vector<std::array<Vec, 4>> &grp, int per_unit, int S, int nx,
                     std::vector<std::vector<Vec>> &dst) {
         const int cnt = (int)grp.size(), U = (cnt + per_unit - 1) / per_unit;
         for (int u = 0; u < U; u++) {
@@ -456,67 +460,73 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
             dst.push_back(un);
         }
     };
+    const Vec none{0, 0, 0, -1};
     for (int g0 = 0; g0 < K;) {
         int g1 = g0;
         while (g1 < K && vs[g1].nx == vs[g0].nx) g1++;
         const int nx = vs[g0].nx;
-        std::vector<std::array<Vec, 2>> grp, single;
+        std::vector<std::array<Vec, 4>> quads, grp, single;
         std::vector<Vec> left;
         for (int i = g0; i < g1;) {
-            if (pair && i + 1 < g1 && vs[i + 1].ny == vs[i].ny) {
-                grp.push_back({vs[i], vs[i + 1]});
-                i += 2;
-            } else {
-                left.push_back(vs[i]);
-                i += 1;
-            }
+            int j = i;
+            while (j < g1 && vs[j].ny == vs[i].ny) j++;
+            // run [i, j) of one ny: quads (mode 3), then pairs, then one left over
+            int k = i;
+            if (quad)
+                for (; k + 4 <= j; k += 4) quads.push_back({vs[k], vs[k + 1], vs[k + 2], vs[k + 3]});
+            if (pair)
+                for (; k + 2 <= j; k += 2) grp.push_back({vs[k], vs[k + 1], none, none});
+            for (; k < j; k++) left.push_back(vs[k]);
+            i = j;
         }
         if (pair) std::stable_sort(left.begin(), left.end(), [](const Vec &a, const Vec &b) { return a.nz < b.nz; });
         for (size_t i = 0; i < left.size();) {
             if (pair && i + 1 < left.size() && left[i + 1].nz == left[i].nz) {
                 Vec a = left[i], b = left[i + 1];
                 a.sw = b.sw = true;
-                grp.push_back({a, b});
+                grp.push_back({a, b, none, none});
                 i += 2;
             } else if (pair && !mixed) {
-                grp.push_back({left[i], Vec{nx, left[i].ny, 0, -1}});
+                grp.push_back({left[i], Vec{nx, left[i].ny, 0, -1}, none, none});
                 i += 1;
             } else {
-                single.push_back({left[i], Vec{nx, 0, 0, -1}});
+                single.push_back({left[i], none, none, none});
                 i += 1;
             }
         }
-        if (pair) deal(grp, G, 2, nx, units_p);
+        if (quad) deal(quads, R / 4, 4, nx, units_q);
+        if (pair) deal(grp, R / 2, 2, nx, units_p);
         deal(single, R, 1, nx, units_s);
-        if (mixed) {
-            emit(units_p, sent_p, true, false);
-            emit(units_s, sent_s, false, false);
+        if (mixed || quad) {
+            emit(units_q, sent_q, 4, false);
+            emit(units_p, sent_p, 2, false);
+            emit(units_s, sent_s, 1, false);
         }
         g0 = g1;
     }
-    // unit_p[u]: unit u belongs to a paired wave
+    // unit_g[u]: group size of unit u's wave (1 unpaired, 2 pairs, 4 quads)
     std::vector<std::vector<Vec>> units;
-    std::vector<char> unit_p;
-    if (mixed) {
-        emit(units_p, sent_p, true, true);
-        emit(units_s, sent_s, false, true);
+    std::vector<char> unit_g;
+    if (mixed || quad) {
+        emit(units_q, sent_q, 4, true);
+        emit(units_p, sent_p, 2, true);
+        emit(units_s, sent_s, 1, true);
         units = std::move(waved);
-        for (char k : wave_p) unit_p.insert(unit_p.end(), 64, k);
+        for (char g : wave_g) unit_g.insert(unit_g.end(), 64, g);
     } else {
         units = std::move(units_p);
-        unit_p.assign(units.size(), pair ? 1 : 0);
+        unit_g.assign(units.size(), pair ? 2 : 1);
         units.insert(units.end(), units_s.begin(), units_s.end());
-        unit_p.resize(units.size(), 0);
+        unit_g.resize(units.size(), 1);
     }
     const int n_units = (int)units.size();
     L.n_chunks = std::max(1, (n_units + 511) / 512);
     while ((int)units.size() < L.n_chunks * 512) {
         const int nx = units.empty() ? 0 : units.back()[0].nx;
         units.push_back(std::vector<Vec>(R, Vec{nx, 0, 0, -1}));
-        unit_p.push_back(0);
+        unit_g.push_back(1);
     }
-    for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass)
-        order_half_wave(units, h0, R, unit_p[h0] != 0, E);
+    for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) order_half_wave(units, h0, R, unit_g[h0], E);
     // emit
     const size_t n = (size_t)L.n_chunks * L.KB;
     L.pack.resize(n);
@@ -542,13 +552,14 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
         L.range[4 * c + 1] = xhi;
         L.range[4 * c + 2] = my | (mz << 16);
         const int n_act = std::min(512, n_units - c * 512);
-        // bit w of the mask: wave w holds pairs
-        int mask = 0, ap = 0;
-        for (int w = 0; w < 8 && w * 64 < n_act; w++)
-            if (unit_p[(size_t)c * 512 + w * 64]) {
-                mask |= 1 << w;
-                ap++;
-            }
+        // bit w of the mask: wave w holds pairs (modes 1, 2) or quads (mode 3)
+        int mask = 0, aq = 0, ap = 0;
+        for (int w = 0; w < 8 && w * 64 < n_act; w++) {
+            const int g = unit_g[(size_t)c * 512 + w * 64];
+            if (g == 4) aq++;
+            if (g == 2) ap++;
+            if (g == (quad ? 4 : 2)) mask |= 1 << w;
+        }
         L.range[4 * c + 3] = n_act | (mask << 16);
         // Work of the waves that run the sums, in slots.  A block's 8 waves
         // share 4 SIMDs two by two, so a chunk with a active waves takes as
@@ -556,8 +567,9 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
         const int a = (n_act + 63) / 64;
         if (a > 0) {
             const double scale = (double)(((a + 3) / 4) * 4) / a;
+            L.quad_slots += (long)(aq * 64.0 * R * scale);
             L.paired_slots += (long)(ap * 64.0 * R * scale);
-            L.unpaired_slots += (long)((a - ap) * 64.0 * R * scale);
+            L.unpaired_slots += (long)((a - ap - aq) * 64.0 * R * scale);
         }
         // conflict estimate of the chunk's reads
         for (int r = 0; r < R; r++)
@@ -569,7 +581,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
                     zs.push_back(E.own(v));
                     if (r == 0) xs.push_back(v.nx);
                 }
-                if (!unit_p[(size_t)c * 512 + h] || r % 2 == 0) L.conflicts += passes_minus_one(ys);
+                if (r % unit_g[(size_t)c * 512 + h] == 0) L.conflicts += passes_minus_one(ys);
                 L.conflicts += passes_minus_one(zs);
                 if (r == 0) L.conflicts += passes_minus_one(xs);
             }
@@ -582,7 +594,8 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
 extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst,
                                           int *range, int *n_chunks, long *conflicts)
 {
-    if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks) return AMDAT_EINVAL;
+    if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks || pair < 0 || pair > 3) return AMDAT_EINVAL;
+    if (pair && R < 4) return AMDAT_EINVAL;
     int M = 0;
     for (int i = 0; i < 3 * K; i++) M = std::max(M, std::abs(nvec[i]));
     if (table_half_width(M) < 0) return AMDAT_ERANGE;

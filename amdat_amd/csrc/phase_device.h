@@ -134,17 +134,19 @@ __device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uin
 // partials p0 (Re, or Fs) and p1 (Im).  Software pipeline over the
 // flattened (atom, slot) sequence: the reads of update u + kDepth, and the
 // x read of the next atom, are issued before update u is computed.
-// PAIRED: slots 2p and 2p+1 share the factor behind py, so only the even
-// slot reads it and forms q = e_x e_y (R is even).
-template <int R, int MM, bool FS, bool PAIRED>
+// GS slots in a group (1, 2 or 4; R is a multiple of GS) share the factor
+// behind py, so only the group's first slot reads it and forms
+// q = e_x e_y.
+template <int R, int MM, bool FS, int GS>
 __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (&py)[R], const float2 *const (&pz)[R],
                                          float (&p0)[R], float (&p1)[R])
 {
     using Gm = Geom<MM>;
+    static_assert(R % GS == 0, "slots per thread must hold whole groups");
     float2 by[kDepth], bz[kDepth];
 #pragma unroll
     for (int u = 0; u < kDepth; ++u) {
-        if (!PAIRED || u % 2 == 0) by[u] = py[u % R][(u / R) * Gm::W];
+        if (u % GS == 0) by[u] = py[u % R][(u / R) * Gm::W];
         else by[u] = make_float2(0.0f, 0.0f);
         bz[u] = pz[u % R][(u / R) * Gm::W];
     }
@@ -153,11 +155,11 @@ __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (
 #pragma clang loop unroll(full)
     for (int u = 0; u < Gm::A * R; ++u) {
         const int a = u / R, r = u % R, slot = u % kDepth;
-        const bool lead = !PAIRED || r % 2 == 0;  // this update forms q = e_x e_y
+        const bool lead = r % GS == 0;  // this update forms q = e_x e_y
         const float2 ey = by[slot], ez = bz[slot];
         if (u + kDepth < Gm::A * R) {
             const int v = u + kDepth;
-            if (!PAIRED || v % 2 == 0) by[slot] = py[v % R][(v / R) * Gm::W];
+            if (v % GS == 0) by[slot] = py[v % R][(v / R) * Gm::W];
             bz[slot] = pz[v % R][(v / R) * Gm::W];
         }
         if (r == 0 && a + 1 < Gm::A) ex_next = px[(a + 1) * Gm::W];
@@ -183,8 +185,10 @@ __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (
 // in the waves that hold pairs, slots 2p and 2p+1 also share ny (or nz,
 // with y and z swapped), so q = e_x e_y is read and formed once per pair and
 // each slot adds q e_z: 6 VALU per update for rho instead of 8, 4 for Fs
-// instead of 6; the plan's other waves run unpaired.
-template <int R, int MM, bool FS, bool PAIR, typename UF>
+// instead of 6; the plan's other waves run unpaired.  QUAD (quad plans):
+// the waves the mask marks hold quads, slots 4q .. 4q+3 sharing ny (5 VALU
+// per rho update, 3 for Fs), the others pairs.
+template <int R, int MM, bool FS, bool PAIR, bool QUAD, typename UF>
 __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
                                             float2 *tab, float2 *anc, uint32_t *sU, int *sValid, const UF uf,
                                             float (&h0)[R], float (&l0)[R], float (&h1)[R], float (&l1)[R])
@@ -192,7 +196,8 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
     using Gm = Geom<MM>;
     const int tid = threadIdx.x;
     // waves whose units all lie past the chunk's last real unit skip the
-    // sums; in paired plans, bit 16 + w of rng.w marks wave w as paired
+    // sums; in paired plans, bit 16 + w of rng.w marks wave w as paired (as
+    // holding quads in quad plans)
     const int wave0 = __builtin_amdgcn_readfirstlane(tid >> 6) * 64;
     const bool active = wave0 < (rng.w & 0xFFFF);
     const bool paired = (rng.w >> (16 + (wave0 >> 6))) & 1;
@@ -233,10 +238,15 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             __syncthreads();
             if (!active) continue;
             if constexpr (PAIR) {
-                if (paired) hot_tile<R, MM, FS, true>(px, py, pz, p0, p1);
-                else hot_tile<R, MM, FS, false>(px, py, pz, p0, p1);
+                if constexpr (QUAD) {
+                    if (paired) hot_tile<R, MM, FS, 4>(px, py, pz, p0, p1);
+                    else hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
+                } else {
+                    if (paired) hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
+                    else hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
+                }
             } else {
-                hot_tile<R, MM, FS, false>(px, py, pz, p0, p1);
+                hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
             }
         }
 #pragma unroll
@@ -289,7 +299,7 @@ struct DispFrac {
 };
 
 // rho over one block's atoms: compensated sums (hr + lr, hi + li) per slot.
-template <int R, int MM, bool PAIR = false>
+template <int R, int MM, bool PAIR = false, bool QUAD = false>
 __device__ __forceinline__ void rho_block(const float *__restrict__ X, const float *__restrict__ Y,
                                           const float *__restrict__ Z, const int *__restrict__ list, size_t foff,
                                           int a_begin, int a_end, double inv_ell, const uint32_t *__restrict__ kp,
@@ -297,7 +307,7 @@ __device__ __forceinline__ void rho_block(const float *__restrict__ X, const flo
                                           float (&lr)[R], float (&hi)[R], float (&li)[R])
 {
     const CoordFrac uf{X, Y, Z, list, foff, inv_ell};
-    phase_tiles<R, MM, false, PAIR>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
+    phase_tiles<R, MM, false, PAIR, QUAD>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
 }
 
 template <int R>
@@ -317,7 +327,7 @@ __device__ __forceinline__ void rho_write(const int *__restrict__ kd, const floa
 }
 
 // Fs over one block's atoms for the pair (o0, o1): compensated sum of cos(k.dr).
-template <int R, int MM, bool PAIR = false>
+template <int R, int MM, bool PAIR = false, bool QUAD = false>
 __device__ __forceinline__ void fs_block(const float *__restrict__ X, const float *__restrict__ Y,
                                          const float *__restrict__ Z, const int16_t *__restrict__ IX,
                                          const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
@@ -328,7 +338,7 @@ __device__ __forceinline__ void fs_block(const float *__restrict__ X, const floa
 {
     const DispFrac uf{X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell};
     float u1[R], u2[R];
-    phase_tiles<R, MM, true, PAIR>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
+    phase_tiles<R, MM, true, PAIR, QUAD>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
 }
 
 template <int R>

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 orc.set_threads(min(16, os.cpu_count() or 1))
 
 TOL = 1e-6
-PLANS = [(2, 0), (4, 0), (8, 0), (4, 1), (8, 1), (4, 2), (8, 2)]
+PLANS = [(2, 0), (4, 0), (8, 0), (4, 1), (8, 1), (4, 2), (8, 2), (4, 3), (8, 3)]
 
 
 @pytest.fixture(scope="module")
@@ -54,6 +54,6 @@ def test_rho_and_fs_each_plan(ctx, data, R, pair):
 def test_hint_rejects_bad_values(ctx, data):
     tr, d, off, n, *_ = data
     sh = _abi.Shells(ctx, off, n)
-    for R, pair in [(3, 0), (2, 1), (16, 0), (4, 3)]:
+    for R, pair in [(3, 0), (2, 1), (16, 0), (4, 4), (2, 3)]:
         with pytest.raises(_abi.AmdatError):
             sh.plan_hint(R, pair)

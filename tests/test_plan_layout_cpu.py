@@ -160,3 +160,33 @@ def test_paired_layout_invariants(qvec, bins, R, mode):
     for c in range(n):
         unit_real = real[c].any(axis=0)
         assert not np.any(unit_real[act[c]:])
+
+
+@pytest.mark.parametrize("bins,R", [((2, 100), 8), ((2, 100), 4), ((40, 41), 8), ((2, 3), 4)])
+def test_quad_layout_invariants(qvec, bins, R):
+    """Quad plans (mode 3): every vector exactly once with its flip; the
+    waves the mask marks hold units of R/4 quads whose four slots share ny
+    (none swapped), the other waves pairs sharing ny or nz; the share of
+    vectors in quads equals, per nx group, four times the quads its ny
+    classes hold."""
+    off, nvec = ref_shells(qvec, *bins)
+    K = len(nvec)
+    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R, pair=3)
+    sw = layout.swap
+    real = dst >= 0
+    idx = dst[real] & ~(1 << 30)
+    assert np.array_equal(np.sort(idx), np.arange(K))
+    flip = (dst[real] & (1 << 30)) != 0
+    assert np.array_equal(np.stack([nx[real], ny[real], nz[real]], 1), nvec[idx] * np.where(flip, -1, 1)[:, None])
+    assert np.all(nx == nx[:, :1, :])
+    mask = rng[:, 3] >> 16
+    isq = np.repeat(((mask[:, None] >> np.arange(8)[None, :]) & 1).astype(bool), 64, axis=1)    # [n, 512]
+    for q in range(R // 4):
+        g = slice(4 * q, 4 * q + 4)
+        ok = np.all(ny[:, g, :] == ny[:, 4 * q:4 * q + 1, :], axis=1) & ~np.any(sw[:, g, :], axis=1)
+        assert np.all(ok | ~isq)
+    shared = np.where(sw[:, 0::2, :], nz[:, 0::2, :] == nz[:, 1::2, :], ny[:, 0::2, :] == ny[:, 1::2, :])
+    assert np.all(shared | isq[:, None, :])
+    canon = np.where(nvec[:, 2:3] < 0, -nvec, nvec)
+    cls = np.unique(canon[:, :2], axis=0, return_counts=True)[1]
+    assert int((real & isq[:, None, :]).sum()) == int(np.sum(cls // 4) * 4)
