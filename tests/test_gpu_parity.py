@@ -1,6 +1,6 @@
 """Parity of the HIP path (called through the C ABI) with the FP64 oracle.
 
-Gate (north star): S, F, Fs and MSD within 1e-6 relative of FP64, with
+Gate: S, F, Fs and MSD within 1e-6 relative of FP64, with
     rel = max |y - y64| / max(|y64|, s)
     s = |y| for S and MSD, S(k) of the shell for F, 1 for Fs;
 per-vector rho within 1e-6 of max(|rho|, sqrt(N)).  The plain-FP64 device
