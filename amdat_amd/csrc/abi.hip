@@ -421,6 +421,18 @@ ListSeg list_at_pair(const amdat_traj *tr, const amdat_list *l, int t0, int t1)
 // Entry points over frames take lists per frame: static or time-varying.
 bool frame_list_ok(const amdat_list *l) { return !l || !l->paired; }
 
+// Atom blocks per pair of the MSD kernel: 1,024 atoms per block, at most 64.
+int msd_blocks(int n_list) { return std::max(1, std::min(64, (n_list + kBlock * 4 - 1) / (kBlock * 4))); }
+
+// A list indexes the atoms of the trajectory it was built for: one built
+// for another atom count would send indices past this trajectory's frames.
+int check_list_traj(const amdat_list *l, const amdat_traj *tr, const char *fn)
+{
+    if (l && l->n_atoms != tr->dev.N)
+        return set_error(AMDAT_EINVAL, std::string(fn) + ": list built for a trajectory of another size");
+    return AMDAT_OK;
+}
+
 // Pairs regrouped by the list time of their t0, for per-list-time launches:
 // pos[i] is pair i's row in the regrouped order (stable within a list
 // time), seg[g] = {list time, first row, row count} per group, and
@@ -1122,6 +1134,7 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
     if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_rho_compute: length scale must be > 0");
     if (!frame_list_ok(list))
         return set_error(AMDAT_EINVAL, "amdat_rho_compute: a pair-keyed list applies to frame pairs only");
+    if (int rc0 = check_list_traj(list, tr, "amdat_rho_compute")) return rc0;
     if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64 && precision != AMDAT_PREC_FP32X)
         return set_error(AMDAT_EINVAL, "amdat_rho_compute: unknown precision");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -1310,7 +1323,9 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
     if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_fs: length scale must be > 0");
     if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64)
         return set_error(AMDAT_EINVAL, "amdat_fs: unknown precision");
-    int rc = check_pairs_frames(pairs, tr->dev.T);
+    int rc = check_list_traj(list, tr, "amdat_fs");
+    if (rc) return rc;
+    rc = check_pairs_frames(pairs, tr->dev.T);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     KPlanOwned *plan = nullptr;
@@ -1360,7 +1375,9 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
 {
     clear_error();
     if (!ctx || !tr || !pairs || !s2 || !s4 || !count) return set_error(AMDAT_EINVAL, "amdat_msd_ngp: null argument");
-    int rc = check_pairs_frames(pairs, tr->dev.T);
+    int rc = check_list_traj(list, tr, "amdat_msd_ngp");
+    if (rc) return rc;
+    rc = check_pairs_frames(pairs, tr->dev.T);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     const int ntg = pairs->ntg;
@@ -1373,7 +1390,7 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
         return AMDAT_OK;
     }
     hipStream_t st = stream_of(ctx);
-    const int nb = std::max(1, std::min(64, (pl.max_n + kBlock * 4 - 1) / (kBlock * 4)));
+    const int nb = msd_blocks(pl.max_n);
     DevBuf<double> part, d2, d4;
     HIP_TRY(part.alloc((size_t)pairs->n * nb * 2));
     if (pl.any_empty) HIP_TRY(hipMemsetAsync(part.p, 0, part.n * sizeof(double), st));
@@ -1405,7 +1422,9 @@ int displacements_impl(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *l
                        int n_bins, int hist_mode, float bin_size, double *sum_d, double *sum_d2,
                        unsigned long long *hist, double *count)
 {
-    int rc = check_pairs_frames(pairs, tr->dev.T);
+    int rc = check_list_traj(list, tr, "amdat_displacements");
+    if (rc) return rc;
+    rc = check_pairs_frames(pairs, tr->dev.T);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     const int ntg = pairs->ntg;

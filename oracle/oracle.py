@@ -361,6 +361,29 @@ def ref32_fs(tr, pairs, ntg, shell_off, nvec, length_scale, atoms=None):
         return corr / natoms[:, None], natoms
 
 
+def ref32_msd_ngp(tr, pairs, ntg, atoms=None):
+    """Reference-FP32 MSD and the non-Gaussian parameter's fourth-moment sum
+    over the pairs in order (Mean_Square_Displacement::list_displacementkernel,
+    listkernel and postprocess_list, src/mean_square_displacement.cpp:146-180;
+    Non_Gaussian_Parameter's, src/non_gaussian_parameter.cpp:106-143):
+    per time gap FP32 sums of d*d and pow(d, 4), the weighting (atoms per
+    pair, an integer) and msd = sum / float(weighting)."""
+    keep, ptrs = _traj_ptrs(tr)
+    at = None if atoms is None else _c(atoms, np.int32)
+    msd = np.zeros(ntg, np.float32)
+    s4 = np.zeros(ntg, np.float32)
+    w = np.zeros(ntg, np.int64)
+    for g, a, b in zip(*pairs):
+        m = np.array([msd[g]], np.float32)
+        q = np.array([s4[g]], np.float32)
+        n = lib().orc_ref32_msd_pair(tr.N, *ptrs, _p(at, C.c_int), 0 if at is None else len(at), int(a), int(b),
+                                     _p(m, C.c_float), _p(q, C.c_float))
+        msd[g], s4[g] = m[0], q[0]
+        w[g] += n
+    with np.errstate(invalid="ignore", divide="ignore"):   # time gaps without pairs: 0/0 as in the reference
+        return dict(msd=msd / w.astype(np.float32), s2=msd, s4=s4, weighting=w)
+
+
 # ---- S(k) misfit loss and gradient ----
 def sk_loss_grad_f64(pos, shell_off, nvec, ell, w, target, grad=True):
     """pos [N,3] float64 (one frame).  Returns (loss, S[n_shells], grad [N,3] or None)."""

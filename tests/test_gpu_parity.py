@@ -100,6 +100,35 @@ def test_fkt(ctx, rho_pair, plain, shells99):
         assert np.allclose(Fs_, F[:, 10:21], rtol=0, atol=1e-13)
 
 
+def test_fkt_asymmetric(ctx, plain, shells99):
+    """isf ... asymmetric: densities of two lists of unequal size; F(A, B)
+    correlates rho_A(t0) with rho_B(t1) and divides by list A's atom count
+    (src/intermediate_scattering_function.cpp:551-566)."""
+    tr, d = plain
+    off, n, sh = shells99
+    rng = np.random.default_rng(11)
+    A = np.sort(rng.choice(tr.N, 300, replace=False)).astype(np.int32)
+    B = np.setdiff1d(np.arange(tr.N), A).astype(np.int32)
+    LA, LB = _abi.AtomList(ctx, d, A), _abi.AtomList(ctx, d, B)
+    L = float(np.float32(9.41))
+    rA = _abi.Rho(ctx, d, LA, sh, 0, 98, np.float32(9.41))
+    rB = _abi.Rho(ctx, d, LB, sh, 0, 98, np.float32(9.41))
+    refA = orc.rho_f64_traj(tr, n, L, atoms=A)
+    refB = orc.rho_f64_traj(tr, n, L, atoms=B)
+    for fb in (0, 1):
+        tg, t0, t1 = _abi.ts_pairs(2, 3, 0, fb, 0)
+        ntg = _abi.ts_n_timegaps(2, 3, 0)
+        P = _abi.Pairs(ctx, tg, t0, t1, ntg)
+        for r1, r2, ref1, ref2, n1 in ((rA, rB, refA, refB, len(A)), (rB, rA, refB, refA, len(B))):
+            F64, c64 = orc.fkt_f64(ref1, ref2, np.full(tr.T, n1), (tg, t0, t1), ntg, off)
+            S1 = orc.sk_f64(ref1, ref1, np.full(tr.T, n1), off)
+            F, c = _abi.fkt(ctx, r1, r2, P)
+            assert np.array_equal(c, c64)
+            e = rel(F, F64, np.broadcast_to(np.abs(S1), F64.shape))
+            print("asymmetric F rel err fb=%d N1=%d: %.3e" % (fb, n1, e))
+            assert e < TOL
+
+
 @pytest.mark.parametrize("fb", [0, 1])
 def test_fs(ctx, plain, shells99, fb):
     tr, d = plain
@@ -199,6 +228,25 @@ def test_error_codes(ctx, plain, shells99):
     with pytest.raises(_abi.AmdatError) as e:
         _abi.AtomList(ctx, d, np.array([tr.N], np.int32))
     assert e.value.code == -4
+
+
+def test_list_of_another_trajectory_is_rejected(ctx, plain, shells99):
+    """A list built for a larger trajectory holds indices past this one's
+    atoms: every entry point that takes a list refuses it before launching."""
+    tr, d = plain
+    sh = shells99[2]
+    big = _abi.Trajectory(ctx, *synth.random_walk(tr.N + 200, 9.41, tr.T, 0.05, seed=4).arrays)
+    far = _abi.AtomList(ctx, big, np.arange(tr.N, tr.N + 200, dtype=np.int32))
+    tg, t0, t1 = _abi.ts_pairs(2, 3)
+    P = _abi.Pairs(ctx, tg, t0, t1, _abi.ts_n_timegaps(2, 3, 0))
+    calls = [lambda: _abi.Rho(ctx, d, far, sh, 0, 3, np.float32(9.41)),
+             lambda: _abi.fs(ctx, d, far, sh, 0, 3, np.float32(4.0), P),
+             lambda: _abi.msd_ngp(ctx, d, far, P),
+             lambda: _abi.displacements(ctx, d, far, P, 0, 1.0)]
+    for call in calls:
+        with pytest.raises(_abi.AmdatError) as e:
+            call()
+        assert e.value.code == -1 and "another size" in str(e.value)
 
 
 def test_deterministic(ctx, plain, shells99):

@@ -241,3 +241,103 @@ def test_empty_shell_gives_zero():
     Fs_full, _ = orc.fs_f64(tr, (tg, t0, t1), ntg, off_full, n, 5.0)
     Fs_gap, _ = orc.fs_f64(tr, (tg, t0, t1), ntg, off_gap, n, 5.0)
     assert np.all(Fs_gap[:, 1] == 0.0) and np.array_equal(Fs_gap[:, [0, 2]], Fs_full)
+
+
+def _steady(L, T, d_atoms, seed=10):
+    """Atoms moving by their own constant step d_atoms[j] per frame from
+    dyadic starts near the box edge (so that some cross it): every stored
+    value and every unwrapped FP32 coordinate is exact."""
+    rng = np.random.default_rng(seed)
+    N = len(d_atoms)
+    u0 = np.round(rng.uniform(L - 1.0, L, size=(N, 3)) * 64) / 64
+    u = u0[None] + np.arange(T)[:, None, None] * np.asarray(d_atoms, np.float64)[None]
+    (x, y, z), (ix, iy, iz) = synth._store(u, L)
+    return synth.Traj(x, y, z, ix, iy, iz, np.full((T, 3), L, np.float32))
+
+
+def test_ref32_msd_ngp_exact_displacements():
+    """Per-frame steps along Pythagorean directions, |d| in {5/4, 7/4, 3/2,
+    1/2}: |dr| = lag |d| is exact in FP32, and so are d^2, d^4 and their
+    sums at these sizes, so the reference-FP32 MSD and fourth-moment sums
+    equal the closed forms bit for bit (all three components, the image
+    counts, d^2 against d^4, the per-gap weighting)."""
+    steps = np.array([[3, 4, 0], [2, 3, 6], [2, 4, 4], [0, 0, 2]], np.float64) / np.array([4, 4, 4, 4])[:, None]
+    mag = np.array([5 / 4, 7 / 4, 6 / 4, 2 / 4])
+    d_atoms = np.repeat(steps, 3, axis=0)               # 12 atoms
+    m_atoms = np.repeat(mag, 3)
+    tr = _steady(4.0, 5, d_atoms)
+    assert np.abs(tr.ix).max() > 0 and np.abs(tr.iz).max() > 0
+    pairs = orc.ts_pairs(2, 2)
+    ntg = orc.ts_n_timegaps(2, 2)
+    r = orc.ref32_msd_ngp(tr, pairs, ntg)
+    tg, t0, t1 = pairs
+    for g in range(ntg):
+        lags = (t1 - t0)[tg == g]
+        s2 = sum(float(np.sum((l * m_atoms) ** 2)) for l in lags)
+        s4 = sum(float(np.sum((l * m_atoms) ** 4)) for l in lags)
+        assert r["weighting"][g] == len(lags) * 12
+        assert r["s2"][g] == np.float32(s2) and r["s4"][g] == np.float32(s4)
+        assert r["msd"][g] == np.float32(np.float32(s2) / np.float32(len(lags) * 12))
+    # atom subset: only the listed atoms count
+    sub = np.array([0, 4, 11], np.int32)
+    rs = orc.ref32_msd_ngp(tr, pairs, ntg, atoms=sub)
+    g = 1
+    lags = (t1 - t0)[tg == g]
+    assert rs["s2"][g] == np.float32(sum(float(np.sum((l * m_atoms[sub]) ** 2)) for l in lags))
+
+
+def test_ref32_msd_close_to_f64():
+    tr = synth.random_walk(500, 6.0, 7, 0.3, seed=12)
+    pairs = orc.ts_pairs(2, 3)
+    ntg = orc.ts_n_timegaps(2, 3)
+    r32 = orc.ref32_msd_ngp(tr, pairs, ntg)
+    m = orc.msd_ngp_f64(tr, pairs, ntg)
+    assert np.array_equal(r32["weighting"], m["count"])
+    # FP32 sums of ~1e3 terms: a few 1e-6 relative
+    assert np.allclose(r32["msd"][1:], m["msd"][1:], rtol=2e-5)
+    assert np.allclose(r32["s4"][1:], m["s4"][1:], rtol=2e-5)
+    assert r32["msd"][0] == 0.0
+
+
+def test_fkt_asymmetric_decomposition(qvec):
+    """With rho_all = rho_A + rho_B over disjoint lists of unequal size,
+    N_all F(all, all) = N_A [F(A, A) + F(A, B)] + N_B [F(B, A) + F(B, B)]:
+    pins the cross term Re(rho1 conj rho2) and the normalisation by the
+    first list's atom count."""
+    off, n = ref_shells(qvec, 2, 25)
+    tr = synth.random_walk(90, 5.0, 7, 0.2, seed=13)
+    A = np.arange(0, 90, 3).astype(np.int32)            # 30 atoms
+    Bl = np.setdiff1d(np.arange(90), A).astype(np.int32)  # 60 atoms
+    rA = orc.rho_f64_traj(tr, n, 5.0, atoms=A)
+    rB = orc.rho_f64_traj(tr, n, 5.0, atoms=Bl)
+    rall = orc.rho_f64_traj(tr, n, 5.0)
+    assert np.allclose(rA + rB, rall, atol=1e-10)
+    pairs = orc.ts_pairs(2, 3)
+    ntg = orc.ts_n_timegaps(2, 3)
+    T = tr.T
+    F = {}
+    for a, ra, na in (("A", rA, 30), ("B", rB, 60)):
+        for b, rb in (("A", rA), ("B", rB)):
+            F[a + b], cnt = orc.fkt_f64(ra, rb, np.full(T, na), pairs, ntg, off)
+            assert np.allclose(cnt, np.bincount(pairs[0], minlength=ntg) * na)
+    Fall, _ = orc.fkt_f64(rall, rall, np.full(T, 90), pairs, ntg, off)
+    assert np.allclose(90 * Fall, 30 * (F["AA"] + F["AB"]) + 60 * (F["BA"] + F["BB"]), rtol=1e-10, atol=1e-10)
+    assert not np.allclose(F["AB"], F["BA"], atol=1e-3)   # the cross terms differ
+
+
+def test_fkt_asymmetric_roles():
+    """F(rho1, rho2) takes rho1 at each pair's origin t0 and rho2 at t1: with
+    rho2 constant in time the result depends on the pairs only through t0,
+    with rho1 constant only through t1."""
+    T, K = 7, 6
+    rng = np.random.default_rng(14)
+    varying = rng.normal(size=(T, K)) + 1j * rng.normal(size=(T, K))
+    const = np.repeat(rng.normal(size=(1, K)) + 1j * rng.normal(size=(1, K)), T, axis=0)
+    off = np.array([0, 2, 6], np.int32)
+    # four single-pair time gaps: (0 -> 3), (0 -> 5), (2 -> 3), (2 -> 5)
+    pairs = (np.arange(4), np.array([0, 0, 2, 2]), np.array([3, 5, 3, 5]))
+    na = np.full(T, 10)
+    F1, _ = orc.fkt_f64(varying, const, na, pairs, 4, off)
+    assert np.allclose(F1[0], F1[1]) and np.allclose(F1[2], F1[3]) and not np.allclose(F1[0], F1[2])
+    F2, _ = orc.fkt_f64(const, varying, na, pairs, 4, off)
+    assert np.allclose(F2[0], F2[2]) and np.allclose(F2[1], F2[3]) and not np.allclose(F2[0], F2[1])
