@@ -76,6 +76,7 @@ SIGNATURES = {
     "amdat_fkt": (_I, [_P, _P, _P, _P, _I, _I, _PD, _PD]),
     "amdat_fs": (_I, [_P, _P, _P, _P, _I, _I, _F, _P, _I, _PD, _PD]),
     "amdat_msd_ngp": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
+    "amdat_msd_ngp_noalloc": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
     "amdat_displacements": (_I, [_P, _P, _P, _P, _I, _F, _PD, _PD, C.POINTER(C.c_ulonglong), _PD]),
     "amdat_displacement_dist": (_I, [_P, _P, _P, _P, _I, _F, C.POINTER(C.c_ulonglong), _PD, _PD, _PD]),
     "amdat_ts_gap_pairs": (_I, [_I, _I, _I, _I, _I, _PI, _PI, _PI, _I, _PI]),
@@ -489,6 +490,17 @@ def msd_ngp(ctx, traj, alist, pairs):
     cnt = np.zeros(pairs.ntg, np.float64)
     check(lib().amdat_msd_ngp(ctx.h, traj.h, alist.h if alist is not None else None, pairs.h, dp(s2), dp(s4),
                               dp(cnt)))
+    return s2, s4, cnt
+
+
+def msd_ngp_noalloc(ctx, traj, alist, pairs):
+    """msd_ngp through the entry point that allocates nothing (static lists,
+    bounded by the library's per-GPU scratch)."""
+    s2 = np.zeros(pairs.ntg, np.float64)
+    s4 = np.zeros(pairs.ntg, np.float64)
+    cnt = np.zeros(pairs.ntg, np.float64)
+    check(lib().amdat_msd_ngp_noalloc(ctx.h, traj.h, alist.h if alist is not None else None, pairs.h, dp(s2), dp(s4),
+                                      dp(cnt)))
     return s2, s4, cnt
 
 

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <tuple>
@@ -78,6 +79,38 @@ struct DevBuf {
 };
 
 }  // namespace
+
+// Device scratch of amdat_msd_ngp_noalloc: the per-(pair, atom block)
+// partial sums and the per-gap results of one call.  Each GPU the library
+// runs on holds its own copy until the process ends.  Host code never
+// passes this name to a kernel or a copy: it stands for a host-side shadow,
+// not a device address; scratch_address() gives the address on a device.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::unordered_map<int, void *> g_scratch_addr;  // device -> that device's copy of scratch
+
+// Address of `scratch` on `device`, which must be the current device;
+// looked up once per device and kept.
+int scratch_address(int device, void **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) {
+        *out = it->second;
+        return AMDAT_OK;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(AMDAT_EDEVICE, std::string("scratch address lookup: ") + hipGetErrorString(e));
+    g_scratch_addr[device] = p;
+    *out = p;
+    return AMDAT_OK;
+}
+}  // namespace
 }  // namespace amdat
 
 using namespace amdat;
@@ -86,6 +119,7 @@ struct amdat_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    void *scratch = nullptr;  // this device's copy of amdat::scratch
     // Grow-only device scratch (ctx_scratch), for entry points that
     // synchronise their stream before returning: one call's kernels are done
     // with it before the next call writes it.
@@ -540,6 +574,8 @@ int amdat_ctx_create(int device, amdat_ctx **out)
     HIP_TRY(hipSetDevice(device));
     auto c = std::make_unique<amdat_ctx>();
     c->device = device;
+    int rc = scratch_address(device, &c->scratch);
+    if (rc) return rc;
     HIP_TRY(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
     *out = c.release();
     return AMDAT_OK;
@@ -1408,6 +1444,55 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
     HIP_TRY(launch_msd_reduce(part.p, nb, pairs->d_group_off.p, pl.order, ntg, d2.p, d4.p, st));
     HIP_TRY(hipMemcpyAsync(s2, d2.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(s4, d4.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_msd_ngp_noalloc(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                          double *s2, double *s4, double *count)
+{
+    clear_error();
+    if (!ctx || !tr || !pairs || !s2 || !s4 || !count)
+        return set_error(AMDAT_EINVAL, "amdat_msd_ngp_noalloc: null argument");
+    if (list && (list->n_times > 0 || list->paired))
+        return set_error(AMDAT_EINVAL, "amdat_msd_ngp_noalloc: static lists only (a time-varying or pair-keyed "
+                                       "list needs per-call tables: use amdat_msd_ngp)");
+    int rc = check_list_traj(list, tr, "amdat_msd_ngp_noalloc");
+    if (rc) return rc;
+    rc = check_pairs_frames(pairs, tr->dev.T);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int ntg = pairs->ntg, n = list_n(tr, list), np = pairs->n;
+    for (int g = 0; g < ntg; g++) count[g] = 0.0;
+    for (int i = 0; i < np; i++) count[pairs->tg[i]] += (double)n;
+    std::fill(s2, s2 + ntg, 0.0);
+    std::fill(s4, s4 + ntg, 0.0);
+    if (np == 0 || n == 0) return AMDAT_OK;
+    // the same atom blocks as amdat_msd_ngp, so the sums are the same bits
+    const int nb = msd_blocks(n);
+    const size_t n_part = (size_t)np * nb * 2;
+    if ((n_part + 2 * (size_t)ntg) * sizeof(double) > kNoAllocScratchBytes)
+        return set_error(AMDAT_ERANGE, "amdat_msd_ngp_noalloc: " + std::to_string(np) + " pairs x " +
+                                           std::to_string(nb) + " atom blocks need more than the " +
+                                           std::to_string(kNoAllocScratchBytes >> 20) + " MiB scratch");
+    if (!ctx->scratch) {
+        rc = scratch_address(ctx->device, &ctx->scratch);
+        if (rc) return rc;
+    }
+    double *part = static_cast<double *>(ctx->scratch);
+    double *d2 = part + n_part, *d4 = d2 + ntg;
+    hipStream_t st = stream_of(ctx);
+    // every (pair, block) partial is written by launch_msd before the
+    // reduction reads it: nothing needs clearing
+    const int pb = 65535;
+    for (int p0 = 0; p0 < np; p0 += pb) {
+        const int m = std::min(pb, np - p0);
+        HIP_TRY(launch_msd(tr->dev, list_ptr(list), n, pairs->d_t0.p + p0, pairs->d_t1.p + p0, m, nb,
+                           part + (size_t)p0 * nb * 2, st));
+    }
+    HIP_TRY(launch_msd_reduce(part, nb, pairs->d_group_off.p, pairs->d_order.p, ntg, d2, d4, st));
+    HIP_TRY(hipMemcpyAsync(s2, d2, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s4, d4, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
 }

@@ -366,6 +366,20 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
 int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
                   const amdat_pairs *pairs, double *s2, double *s4, double *count);
 
+/* amdat_msd_ngp with no allocation: the same results, bit for bit, for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It calls no hipMalloc, hipFree or
+ * allocating library routine: its partial sums live in a device array the
+ * library holds per GPU (32 MiB), which bounds n_pairs * atom blocks * 16 B
+ * + n_timegaps * 16 B (atom blocks = min(64, ceil(list size / 1024)); a
+ * larger call fails with AMDAT_ERANGE).  Static lists only (AMDAT_EINVAL
+ * otherwise).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two host threads, or two streams) race and must be
+ * serialised by the caller. */
+int amdat_msd_ngp_noalloc(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list,
+                          const amdat_pairs *pairs, double *s2, double *s4, double *count);
+
 /* Displacement statistics over the pairs (src/mean_displacement.cpp:163-176,
  * src/mean_square_displacement_2d.cpp:144-156, src/van_hove_self.cpp:98-102):
  * per time gap g, FP64 sums over its pairs and the list's atoms of the
