@@ -303,16 +303,16 @@ int shell_range_M(const amdat_shells *sh, int first, int last)
 }
 
 // Modelled cost of a layout per atom: every slot of a wave that runs the
-// sums costs c VALU ops per lane, c - 2 in paired waves (q = e_x e_y formed
-// once per pair), c - 3 in quad waves (once per four slots), counted in
-// whole SIMD rounds of 4 waves per chunk, and
-// every chunk the fixed part of choose_R.  (Scale-form Fs, 1,500 vectors in
-// one chunk: R = 8 paired, 5 active waves, ran 11.4 s against 10.1 s for
-// R = 4 unpaired with 7.)
+// sums costs c VALU ops per lane in waves of single slots, c - 4 + 4/GS in
+// waves of groups of GS slots (q = e_x e_y formed once per group: c - 2 for
+// pairs, c - 3 for quads, c - 3.5 for groups of 8), counted in whole SIMD
+// rounds of 4 waves per chunk, and every chunk the fixed part of choose_R.
+// (Scale-form Fs, 1,500 vectors in one chunk: R = 8 paired, 5 active
+// waves, ran 11.4 s against 10.1 s for R = 4 unpaired with 7.)
 double layout_cost(const PlanLayout &L, double c)
 {
-    return (double)L.n_chunks * 12000.0 + (double)L.quad_slots * (c - 3.0) + (double)L.paired_slots * (c - 2.0) +
-           (double)L.unpaired_slots * c;
+    return (double)L.n_chunks * 12000.0 + (double)L.gs8_slots * (c - 3.5) + (double)L.quad_slots * (c - 3.0) +
+           (double)L.paired_slots * (c - 2.0) + (double)L.unpaired_slots * c;
 }
 
 // R_force > 0 fixes the vectors per thread and the layout mode to
@@ -331,7 +331,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
     if (M > 127) return set_error(AMDAT_ERANGE, "k-vector component beyond +-127");
     int R = R_force > 0 ? R_force : choose_R(K, M, c);
     int mode = R_force > 0 ? mode_force : 0;  // plan_layout mode: 0 unpaired, 1 paired with padding,
-                                              // 2 paired and unpaired waves
+                                              // 2 paired and unpaired waves, 3 quads, 4 grouped
     PlanLayout lay;
     bool have_layout = false;
     if (R_force <= 0 && sh->hint_R > 0) {
@@ -350,15 +350,16 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
             // reads spread over the banks (64,000-atom workload: rho 542 ms
             // in mode 2 against 464 ms in mode 1, although it issues fewer
             // VALU ops).
-            const int cand[5][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}};
+            const int cand[7][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {4, 4}, {8, 4}};
             for (const auto &cd : cand) {
                 if (cd[1] && !allow_pair) continue;
-                PlanLayout L = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, cd[0], cd[1], plan_table_width(M));
+                PlanLayout L =
+                    plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, cd[0], cd[1], plan_table_width(M), c, false);
                 const double cost = layout_cost(L, c);
                 if (cost < best * 0.98) {
                     best = cost;
                     lay = std::move(L);
-                    have_layout = true;
+                    have_layout = cd[1] != 4;  // a grouped plan is placed once chosen
                     R = cd[0];
                     mode = cd[1];
                 }
@@ -381,13 +382,13 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
         sh->plans[key] = std::move(P);
         return AMDAT_OK;
     }
-    if (!have_layout) lay = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, R, mode, plan_table_width(M));
+    if (!have_layout) lay = plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, R, mode, plan_table_width(M), c);
     kp.K = K;
     kp.R = R;
     kp.KB = lay.KB;
     kp.n_chunks = lay.n_chunks;
     kp.MM = plan_table_width(M);
-    kp.pair = mode == 3 ? 2 : (mode != 0 ? 1 : 0);
+    kp.pair = mode != 0 ? 1 : 0;
     hipStream_t st = stream_of(ctx);
     HIP_TRY(P->kpack.upload(lay.pack.data(), lay.pack.size(), st));
     HIP_TRY(P->kdst.upload(lay.dst.data(), lay.dst.size(), st));
@@ -626,7 +627,7 @@ int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair)
     clear_error();
     if (!sh) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: null table");
     if (R != 0 && R != 2 && R != 4 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: R must be 0, 2, 4 or 8");
-    if (pair < 0 || pair > 3) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pair must be 0, 1, 2 or 3");
+    if (pair < 0 || pair > 4) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pair must be 0 to 4");
     if (pair && R != 4 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pairs need R = 4 or 8");
     sh->hint_R = R;
     sh->hint_pair = pair;

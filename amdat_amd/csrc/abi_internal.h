@@ -43,19 +43,25 @@ struct PlanLayout {
     std::vector<uint32_t> pack;  // [n_chunks][R][512]: (nx+128) | (ny+128)<<8 | nz<<16
     std::vector<int> dst;        // same layout: destination index | bit 30 (conjugate), or -1
     std::vector<int> range;      // [n_chunks][4]: nx lo, nx hi, max |ny| | max nz << 16,
-                                 // units with vectors | paired-wave mask << 16 (bit w: wave w)
+                                 // units with vectors | group codes << 16 (2 bits per wave:
+                                 // group size 1 << code)
     long conflicts = 0;          // extra LDS passes summed over (chunk, slot, half-wave, axis)
-    bool pair = false;           // slots (2p, 2p+1) of a thread share ny, or nz (kPackSwap)
-    bool quad = false;           // mode 3: the mask marks waves whose slots (4q .. 4q+3) share ny
-    long quad_slots = 0, paired_slots = 0, unpaired_slots = 0;  // slots of waves that run the sums, by
-                                                                // kind, each chunk's rounded up to whole
-                                                                // SIMD rounds (4 waves)
+    bool pair = false;           // grouped waves: a group's slots share ny, or nz (kPackSwap)
+    bool quad = false;           // mode 3
+    long gs8_slots = 0, quad_slots = 0, paired_slots = 0, unpaired_slots = 0;
+                                 // slots of waves that run the sums, by group size 8, 4, 2, 1,
+                                 // each chunk's rounded up to whole SIMD rounds (4 waves)
 };
 // mode 0: unpaired; 1: paired, vectors without a partner padded; 2: paired,
 // vectors without a partner in unpaired waves; 3: quads sharing ny where a
-// ny class holds four, then pairs as mode 1, in waves of one kind each.  MM:
-// table half-width of the kernels that read the plan (bank model only).
-PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM);
+// ny class holds four, then pairs as mode 1, in waves of one kind each;
+// 4: grouped, classes sharing nz or ny cut into groups of 8, 4, 2 and 1
+// slots (cheapest at c VALU ops per unpaired slot: one accumulate of c - 4
+// per slot and one product of 4 per group), waves of one group size each,
+// slots, groups and units placed by a local search against LDS bank-pair
+// clashes (place = false skips that search: the cost figures are the
+// same).  MM: table half-width of the kernels that read the plan.
+PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c = 9.0, bool place = true);
 
 // ---- trajectory ingestion (host_io.cpp) ----
 struct MappedFile {

@@ -14,7 +14,7 @@ namespace {
 // row [g][v] of t.out (FP64; each slot is owned by one thread of one item,
 // so the sum order is fixed).  All trajectories share one k-plan: the same
 // integer vectors, k = 2 pi n / l_b with each trajectory's own l_b.
-template <int R, int MM, bool PAIR = false>
+template <int R, int MM, bool GROUPED = false>
 __device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, const int *__restrict__ frames,
                                         int n_frames, int group_size, const KPlan &kp, float2 *tab, float2 *anc,
                                         uint32_t *sU, int *sValid)
@@ -24,7 +24,7 @@ __device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, co
     double *row = t.out + (size_t)g * kp.K;
     for (int f = f_begin; f < f_end; ++f) {
         float hr[R], lr[R], hi[R], li[R];
-        rho_block<R, MM, PAIR>(t.x, t.y, t.z, t.list, (size_t)frames[f] * (size_t)t.N, 0, t.n_list, t.inv_ell,
+        rho_block<R, MM, GROUPED>(t.x, t.y, t.z, t.list, (size_t)frames[f] * (size_t)t.N, 0, t.n_list, t.inv_ell,
                          kp.kpack + (size_t)chunk * kp.KB, kp.range[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_batch_kernel(const BatchTra
 
 // One analysis job over a batch in a single persistent launch: S(k), Fs and
 // MSD/NGP items (item.w = 0, 1, 2) drawn from one queue, costliest first.
-// PS: the S(k) plan is paired.
+// PS: the S(k) plan carries group codes.
 template <int RS, int RF, int MM, bool PS>
 __global__ __launch_bounds__(kPhaseBlock, 4) void job_kernel(BatchJob j)
 {

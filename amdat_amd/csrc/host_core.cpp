@@ -326,10 +326,10 @@ constexpr int kBankPairs = 32;     // 64 banks of 4 bytes = 32 eight-byte entrie
 
 struct Vec {
     int nx, ny, nz, d;
-    bool sw = false;  // paired on nz: the shared read is z, the own read y
+    bool sw = false;  // the group shares nz: the shared read is z, the own read y
 };
 
-// Table entries a slot reads besides x: the one its pair shares (y, or z
+// Table entries a slot reads besides x: the one its group shares (y, or z
 // when swapped) and its own (z, or y).  Their bank pairs follow the entry
 // index: every lane of a read addresses the same atom row.
 struct Entries {
@@ -337,6 +337,9 @@ struct Entries {
     int shared(const Vec &v) const { return v.sw ? oz + v.nz : oy + v.ny; }
     int own(const Vec &v) const { return v.sw ? oy + v.ny : oz + v.nz; }
 };
+
+// Two-bit group code of a wave whose groups hold GS slots.
+int gs_code(int gs) { return gs >= 8 ? 3 : (gs == 4 ? 2 : (gs == 2 ? 1 : 0)); }
 
 // Extra LDS passes of one read instruction: the largest number of distinct
 // entries falling into one bank pair, minus one, over the lanes of a pass.
@@ -357,7 +360,7 @@ long passes_minus_one(const std::vector<int> &vals)
 // distinct bank pairs where it can.  Groups of S slots (S = 1: single
 // slots, each reading y and z; S = 2: pairs (2p, 2p + 1) sharing ny, of
 // which only slot 2p reads y, in either orientation; S = 4: quads sharing
-// ny, read by their first slot) are placed whole.
+// ny, read by their first slot) are placed whole.  (Modes 1 to 3.)
 void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, int S, const Entries &E)
 {
     std::vector<std::array<std::vector<int>, kBankPairs>> by(R), bz(R);
@@ -401,12 +404,281 @@ void order_half_wave(std::vector<std::vector<Vec>> &units, size_t h0, int R, int
     }
 }
 
+// ---- grouped layout (mode 4) --------------------------------------------
+
+// Greedy cover of one nx class by sub-classes that share nz (the shared
+// read is then z, the own read y) or ny (shared y, own z): repeatedly the
+// largest sub-class left.
+void cover_groups(const std::vector<Vec> &vs, std::vector<std::vector<Vec>> &out)
+{
+    const int n = (int)vs.size();
+    std::vector<std::vector<int>> byz(256), byy(256);
+    std::vector<int> cz(256, 0), cy(256, 0);
+    for (int i = 0; i < n; i++) {
+        byz[vs[i].nz].push_back(i);
+        byy[vs[i].ny + 128].push_back(i);
+        cz[vs[i].nz]++;
+        cy[vs[i].ny + 128]++;
+    }
+    std::vector<char> alive(n, 1);
+    for (int left = n; left > 0;) {
+        const int kz = (int)(std::max_element(cz.begin(), cz.end()) - cz.begin());
+        const int ky = (int)(std::max_element(cy.begin(), cy.end()) - cy.begin());
+        const bool use_z = cz[kz] >= cy[ky];
+        const std::vector<int> &mem = use_z ? byz[kz] : byy[ky];
+        std::vector<Vec> grp;
+        for (int i : mem) {
+            if (!alive[i]) continue;
+            alive[i] = 0;
+            Vec v = vs[i];
+            v.sw = use_z;
+            grp.push_back(v);
+            cz[vs[i].nz]--;
+            cy[vs[i].ny + 128]--;
+            left--;
+        }
+        out.push_back(std::move(grp));
+    }
+}
+
+// Sizes of the pieces a class of m vectors is cut into (powers of two up
+// to gmax; the last piece may carry padding slots), cheapest by
+// cost(piece of s slots) = s * c_mac + c_q: each slot's accumulate plus one
+// product q per piece.
+std::vector<int> piece_sizes(int m, int gmax, double c_mac, double c_q)
+{
+    std::vector<int> out;
+    while (m > 2 * gmax) {
+        out.push_back(gmax);
+        m -= gmax;
+    }
+    std::vector<double> best(m + 1, 1e300);
+    std::vector<int> pick(m + 1, 1);
+    best[0] = 0.0;
+    for (int k = 1; k <= m; k++)
+        for (int s = 1; s <= gmax; s *= 2) {
+            const double c = s * c_mac + c_q + best[k - std::min(s, k)];
+            if (c < best[k]) { best[k] = c; pick[k] = s; }
+        }
+    for (int k = m; k > 0; k -= std::min(pick[k], k)) out.push_back(pick[k]);
+    return out;
+}
+
+// Bank-pair clashes of the reads of half-waves, for the local search below.
+// Read q of a half-wave is the 32 lanes' entry indices of one instruction;
+// its cost is the sum over bank pairs of (distinct entries in the pair)^2
+// (32 when every lane hits its own pair or a broadcast); wildcards (-1:
+// padding slots, set afterwards to an entry another lane reads) cost
+// nothing.
+struct ReadCounters {
+    int n_reads = 0, W = 0;
+    std::vector<int> cnt;  // [n_reads][W]: lanes reading each entry
+    std::vector<int> d;    // [n_reads][32]: distinct entries per bank pair
+    void init(int nr, int w)
+    {
+        n_reads = nr;
+        W = w;
+        cnt.assign((size_t)nr * w, 0);
+        d.assign((size_t)nr * kBankPairs, 0);
+    }
+    // cost change of adding (+1) or removing (-1) one lane's entry a
+    long change(int q, int a, int sgn)
+    {
+        if (a < 0) return 0;
+        int &c = cnt[(size_t)q * W + a];
+        int &b = d[(size_t)q * kBankPairs + a % kBankPairs];
+        long delta = 0;
+        if (sgn > 0) {
+            if (c++ == 0) { delta = 2L * b + 1; b++; }
+        } else {
+            if (--c == 0) { delta = -(2L * b - 1); b--; }
+        }
+        return delta;
+    }
+    // whether entry a of read q shares its bank pair with another entry
+    bool clashes(int q, int a) const { return a >= 0 && d[(size_t)q * kBankPairs + a % kBankPairs] > 1; }
+    int worst(int q) const
+    {
+        int m = 0;
+        for (int b = 0; b < kBankPairs; b++) m = std::max(m, d[(size_t)q * kBankPairs + b]);
+        return std::max(m, 1);
+    }
+};
+
+// Reads of a unit of R slots in groups of GS: own read of slot r is read r,
+// the shared read of group g (by its slot g * GS) is read R + g.
+struct UnitReads {
+    int R, GS;
+    const Entries &E;
+    int own(const Vec &v) const { return v.d < 0 ? -1 : E.own(v); }
+    int shared(const std::vector<Vec> &un, int g) const
+    {
+        for (int i = 0; i < GS; i++)
+            if (un[g * GS + i].d >= 0) return E.shared(un[g * GS + i]);
+        return -1;  // padding group
+    }
+    bool clashes(const ReadCounters &rc, const std::vector<Vec> &un) const
+    {
+        for (int r = 0; r < R; r++)
+            if (rc.clashes(r, own(un[r]))) return true;
+        for (int g = 0; g < R / GS; g++)
+            if (rc.clashes(R + g, shared(un, g))) return true;
+        return false;
+    }
+    long apply(ReadCounters &rc, const std::vector<Vec> &un, int sgn) const
+    {
+        long delta = 0;
+        for (int r = 0; r < R; r++) delta += rc.change(r, own(un[r]), sgn);
+        for (int g = 0; g < R / GS; g++) delta += rc.change(R + g, shared(un, g), sgn);
+        return delta;
+    }
+};
+
+// Local search over the units of one chunk that share a group size: for a
+// unit with a clashing read, swap two slots of one group, two groups of the
+// unit, or the unit with one of another half-wave, keeping each move that
+// does not raise the summed clash cost.  hw[h] lists the unit indices of
+// half-wave h.
+void optimise_reads(std::vector<std::vector<Vec>> &units, const std::vector<std::vector<int>> &hw, int R, int GS,
+                    const Entries &E, int W, uint64_t seed)
+{
+    const int H = (int)hw.size(), G = R / GS;
+    if (H == 0) return;
+    const UnitReads ur{R, GS, E};
+    std::vector<ReadCounters> rc(H);
+    for (int h = 0; h < H; h++) {
+        rc[h].init(R + G, W);
+        for (int u : hw[h]) ur.apply(rc[h], units[u], +1);
+    }
+    std::vector<std::pair<int, int>> pos;  // (half-wave, lane)
+    for (int h = 0; h < H; h++)
+        for (int l = 0; l < (int)hw[h].size(); l++) pos.push_back({h, l});
+    const long moves = 150L * (long)pos.size(), draws = 40L * moves;
+    uint64_t st = seed;
+    long tried = 0;
+    // annealing: a move that raises the cost by delta is kept with
+    // probability exp(-delta / temp), temp falling linearly to 0
+    auto keep = [&](long delta) {
+        if (delta <= 0) return true;
+        const double temp = 2.0 * (1.0 - (double)tried / (double)moves);
+        if (temp <= 0.0) return false;
+        const double u = (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+        return u < std::exp(-(double)delta / temp);
+    };
+    for (long it = 0; it < draws && tried < moves; it++) {
+        const uint64_t rnd = splitmix64(st);
+        const int kind = (int)(rnd % 3);
+        const auto &p1 = pos[(rnd >> 8) % pos.size()];
+        std::vector<Vec> &a = units[hw[p1.first][p1.second]];
+        ReadCounters &ra = rc[p1.first];
+        if (!ur.clashes(ra, a)) continue;
+        tried++;
+        if (kind == 0 && GS > 1) {
+            // two slots of one group: the group's shared read is unchanged
+            const int g = (int)((rnd >> 32) % G), i = (int)((rnd >> 40) % GS), j = (int)((rnd >> 48) % GS);
+            if (i == j) continue;
+            const int ri = g * GS + i, rj = g * GS + j;
+            long d = ra.change(ri, ur.own(a[ri]), -1) + ra.change(rj, ur.own(a[rj]), -1) +
+                     ra.change(ri, ur.own(a[rj]), +1) + ra.change(rj, ur.own(a[ri]), +1);
+            if (keep(d)) {
+                std::swap(a[ri], a[rj]);
+            } else {
+                ra.change(ri, ur.own(a[rj]), -1);
+                ra.change(rj, ur.own(a[ri]), -1);
+                ra.change(ri, ur.own(a[ri]), +1);
+                ra.change(rj, ur.own(a[rj]), +1);
+            }
+        } else if (kind == 1 && G > 1) {
+            // two groups of one unit
+            const int g1 = (int)((rnd >> 32) % G), g2 = (int)((rnd >> 40) % G);
+            if (g1 == g2) continue;
+            long d = ur.apply(ra, a, -1);
+            std::vector<Vec> b = a;
+            for (int i = 0; i < GS; i++) std::swap(b[g1 * GS + i], b[g2 * GS + i]);
+            d += ur.apply(ra, b, +1);
+            if (keep(d)) {
+                a.swap(b);
+            } else {
+                ur.apply(ra, b, -1);
+                ur.apply(ra, a, +1);
+            }
+        } else if (H > 1) {
+            // two units of different half-waves
+            const auto &p2 = pos[(rnd >> 32) % pos.size()];
+            if (p2.first == p1.first) continue;
+            std::vector<Vec> &b = units[hw[p2.first][p2.second]];
+            ReadCounters &rb = rc[p2.first];
+            long d = ur.apply(ra, a, -1) + ur.apply(rb, b, -1) + ur.apply(ra, b, +1) + ur.apply(rb, a, +1);
+            if (keep(d)) {
+                a.swap(b);
+            } else {
+                ur.apply(ra, b, -1);
+                ur.apply(rb, a, -1);
+                ur.apply(ra, a, +1);
+                ur.apply(rb, b, +1);
+            }
+        }
+    }
+}
+
+// Padding slots read what another lane of their half-wave reads in the
+// same instruction (a broadcast): own reads copy the first real lane's own
+// entry, a padding group's lead also its shared entry where the two lie in
+// different sections.  A padding unit takes that lane's nx as well.
+void fill_padding(std::vector<std::vector<Vec>> &units, size_t h0, int R, int GS, const Entries &E)
+{
+    const size_t h1 = std::min(units.size(), h0 + kLanesPerPass);
+    for (int r = 0; r < R; r++) {
+        int ref = -1;
+        for (size_t u = h0; u < h1 && ref < 0; u++)
+            if (units[u][r].d >= 0) ref = (int)u;
+        if (ref < 0) continue;
+        const Vec &src = units[ref][r];
+        for (size_t u = h0; u < h1; u++) {
+            Vec &v = units[u][r];
+            if (v.d >= 0) continue;
+            const bool lead = r % GS == 0;
+            bool lead_pad = lead;
+            for (int i = r - r % GS; i < r - r % GS + GS && lead_pad; i++)
+                if (units[u][i].d >= 0) lead_pad = false;
+            if (!lead || lead_pad) {
+                // a slot that reads no shared entry, or a padding group's
+                // lead: read both of src's entries
+                v.ny = src.ny;
+                v.nz = src.nz;
+                v.sw = src.sw;
+            } else if (v.sw == src.sw) {
+                // a padding lead of a real group keeps the group's shared
+                // entry; its own entry follows src's where on the same axis
+                if (v.sw) v.ny = src.ny;
+                else v.nz = src.nz;
+            }
+        }
+    }
+    // padding units: every slot padding; take the reference lane's nx
+    for (size_t u = h0; u < h1; u++) {
+        bool all_pad = true;
+        for (int r = 0; r < R && all_pad; r++)
+            if (units[u][r].d >= 0) all_pad = false;
+        if (!all_pad) continue;
+        for (size_t w = h0; w < h1; w++) {
+            bool real = false;
+            for (int r = 0; r < R && !real; r++) real = units[w][r].d >= 0;
+            if (real) {
+                for (int r = 0; r < R; r++) units[u][r].nx = units[w][0].nx;
+                break;
+            }
+        }
+    }
+}
+
 }  // namespace
 
-PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
+PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c, bool place)
 {
     const Entries E{table_oy(MM), table_oz(MM)};
-    const bool pair = mode != 0, mixed = mode == 2, quad = mode == 3;
+    const bool pair = mode != 0, mixed = mode == 2, quad = mode == 3, grouped = mode == 4;
+    const int W = std::max(5 * table_h(MM) + 3, 257);
     PlanLayout L;
     L.R = R;
     L.KB = 512 * R;
@@ -429,23 +701,27 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
     // one nz (swapped roles), dealt R / 2 pairs per unit; each vector left
     // over either forms a pair with a padding slot (modes 1, 3) or goes to
     // unpaired units (mode 2).  Mode 3 first takes quads, four vectors with
-    // one ny, dealt R / 4 per unit.  Every wave holds one kind of unit: in
-    // modes 2 and 3 the kinds fill separate waves of 64 units, emitted in nx
-    // order as they fill, so a chunk's waves of every kind cover a narrow nx
+    // one ny, dealt R / 4 per unit.  Grouped (mode 4): per nx group, the
+    // classes of cover_groups cut into pieces of 8, 4, 2 and 1 slots
+    // (piece_sizes), R / GS pieces of one size per unit.  In modes 2 to 4
+    // every wave holds one kind of unit, emitted in nx order as 64 units of
+    // the kind fill, so a chunk's waves of every kind cover a narrow nx
     // range.
+    const int kinds[4] = {8, 4, 2, 1};
+    std::vector<std::vector<Vec>> pend[4];  // per group size 8, 4, 2, 1
+    size_t sent[4] = {0, 0, 0, 0};
     std::vector<std::vector<Vec>> units_q, units_p, units_s;
-    // modes 2, 3: waves emitted as they fill, with their group size
     std::vector<std::vector<Vec>> waved;
     std::vector<char> wave_g;
     size_t sent_q = 0, sent_p = 0, sent_s = 0;
-    auto emit = [&](std::vector<std::vector<Vec>> &src, size_t &sent, char g, bool flush) {
-        while (src.size() - sent >= 64 || (flush && src.size() > sent)) {
-            const size_t take = std::min<size_t>(64, src.size() - sent);
-            for (size_t i = 0; i < take; i++) waved.push_back(src[sent + i]);
+    auto emit = [&](std::vector<std::vector<Vec>> &src, size_t &sent_n, char g, bool flush) {
+        while (src.size() - sent_n >= 64 || (flush && src.size() > sent_n)) {
+            const size_t take = std::min<size_t>(64, src.size() - sent_n);
+            for (size_t i = 0; i < take; i++) waved.push_back(src[sent_n + i]);
             for (size_t i = take; i < 64; i++)
-                waved.push_back(std::vector<Vec>(R, Vec{src[sent + take - 1][0].nx, 0, 0, -1}));
+                waved.push_back(std::vector<Vec>(R, Vec{src[sent_n + take - 1][0].nx, 0, 0, -1}));
             wave_g.push_back(g);
-            sent += take;
+            sent_n += take;
         }
     };
     // deal groups of S vectors, per_unit groups per unit, with stride
@@ -461,10 +737,42 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
         }
     };
     const Vec none{0, 0, 0, -1};
+    const int gmax = std::min(R, 8);
     for (int g0 = 0; g0 < K;) {
         int g1 = g0;
         while (g1 < K && vs[g1].nx == vs[g0].nx) g1++;
         const int nx = vs[g0].nx;
+        if (grouped) {
+            std::vector<std::vector<Vec>> classes;
+            cover_groups(std::vector<Vec>(vs.begin() + g0, vs.begin() + g1), classes);
+            std::vector<std::vector<Vec>> pieces[4];
+            for (const auto &cl : classes) {
+                size_t at = 0;
+                for (int s : piece_sizes((int)cl.size(), gmax, c - 4.0, 4.0)) {
+                    std::vector<Vec> pc;
+                    for (int i = 0; i < s; i++) {
+                        if (at < cl.size()) pc.push_back(cl[at++]);
+                        else pc.push_back(Vec{nx, cl.back().ny, cl.back().nz, -1, cl.back().sw});
+                    }
+                    pieces[s == 8 ? 0 : (s == 4 ? 1 : (s == 2 ? 2 : 3))].push_back(std::move(pc));
+                }
+            }
+            for (int k = 0; k < 4; k++) {
+                const int s = kinds[k], per = R / s;
+                if (s > gmax) continue;
+                for (size_t i = 0; i < pieces[k].size(); i += per) {
+                    std::vector<Vec> un;
+                    for (int j = 0; j < per; j++) {
+                        if (i + j < pieces[k].size()) un.insert(un.end(), pieces[k][i + j].begin(), pieces[k][i + j].end());
+                        else for (int q = 0; q < s; q++) un.push_back(Vec{nx, 0, 0, -1});
+                    }
+                    pend[k].push_back(std::move(un));
+                }
+                emit(pend[k], sent[k], (char)s, false);
+            }
+            g0 = g1;
+            continue;
+        }
         std::vector<std::array<Vec, 4>> quads, grp, single;
         std::vector<Vec> left;
         for (int i = g0; i < g1;) {
@@ -504,13 +812,18 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
         }
         g0 = g1;
     }
-    // unit_g[u]: group size of unit u's wave (1 unpaired, 2 pairs, 4 quads)
+    // unit_g[u]: group size of unit u's wave
     std::vector<std::vector<Vec>> units;
     std::vector<char> unit_g;
+    if (grouped) {
+        for (int k = 0; k < 4; k++) emit(pend[k], sent[k], (char)kinds[k], true);
+    }
     if (mixed || quad) {
         emit(units_q, sent_q, 4, true);
         emit(units_p, sent_p, 2, true);
         emit(units_s, sent_s, 1, true);
+    }
+    if (grouped || mixed || quad) {
         units = std::move(waved);
         for (char g : wave_g) unit_g.insert(unit_g.end(), 64, g);
     } else {
@@ -526,19 +839,39 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
         units.push_back(std::vector<Vec>(R, Vec{nx, 0, 0, -1}));
         unit_g.push_back(1);
     }
-    for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) order_half_wave(units, h0, R, unit_g[h0], E);
+    if (grouped && place) {
+        // per chunk and group size: local search over the half-waves of the
+        // chunk's real units, then padding slots pointed at broadcasts
+        for (int c = 0; c < L.n_chunks; c++) {
+            const int n_act = std::min(512, n_units - c * 512);
+            for (int gs = 1; gs <= 8; gs *= 2) {
+                std::vector<std::vector<int>> hw;
+                for (int h = 0; h < n_act; h += kLanesPerPass) {
+                    const size_t u0 = (size_t)c * 512 + h;
+                    if (unit_g[u0] != gs) continue;
+                    std::vector<int> lanes;
+                    for (int l = 0; l < kLanesPerPass && h + l < n_act; l++) lanes.push_back((int)u0 + l);
+                    hw.push_back(std::move(lanes));
+                }
+                optimise_reads(units, hw, R, gs, E, W, 0x9E3779B97F4A7C15ull * (uint64_t)(c * 16 + gs + 1));
+            }
+        }
+        for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) fill_padding(units, h0, R, unit_g[h0], E);
+    } else if (!grouped) {
+        for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) order_half_wave(units, h0, R, unit_g[h0], E);
+    }
     // emit
     const size_t n = (size_t)L.n_chunks * L.KB;
     L.pack.resize(n);
     L.dst.resize(n);
     L.range.assign((size_t)L.n_chunks * 4, 0);
-    for (int c = 0; c < L.n_chunks; c++) {
+    for (int ch = 0; ch < L.n_chunks; ch++) {
         int xlo = 1 << 30, xhi = -(1 << 30), my = 0, mz = 0;
         for (int t = 0; t < 512; t++) {
-            const std::vector<Vec> &un = units[(size_t)c * 512 + t];
+            const std::vector<Vec> &un = units[(size_t)ch * 512 + t];
             for (int r = 0; r < R; r++) {
                 const Vec &v = un[r];
-                const size_t pos = (size_t)c * L.KB + (size_t)r * 512 + t;
+                const size_t pos = (size_t)ch * L.KB + (size_t)r * 512 + t;
                 L.pack[pos] = (uint32_t)(v.nx + 128) | ((uint32_t)(v.ny + 128) << 8) | ((uint32_t)v.nz << 16) |
                               (v.sw ? kPackSwap : 0u);
                 L.dst[pos] = v.d;
@@ -548,40 +881,42 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
                 mz = std::max(mz, v.nz);
             }
         }
-        L.range[4 * c] = xlo;
-        L.range[4 * c + 1] = xhi;
-        L.range[4 * c + 2] = my | (mz << 16);
-        const int n_act = std::min(512, n_units - c * 512);
-        // bit w of the mask: wave w holds pairs (modes 1, 2) or quads (mode 3)
-        int mask = 0, aq = 0, ap = 0;
+        L.range[4 * ch] = xlo;
+        L.range[4 * ch + 1] = xhi;
+        L.range[4 * ch + 2] = my | (mz << 16);
+        const int n_act = std::min(512, n_units - ch * 512);
+        // two bits per wave: its group code (group size 1 << code)
+        unsigned codes = 0;
+        int a_by_g[4] = {0, 0, 0, 0};  // active waves by code
         for (int w = 0; w < 8 && w * 64 < n_act; w++) {
-            const int g = unit_g[(size_t)c * 512 + w * 64];
-            if (g == 4) aq++;
-            if (g == 2) ap++;
-            if (g == (quad ? 4 : 2)) mask |= 1 << w;
+            const int g = unit_g[(size_t)ch * 512 + w * 64];
+            const int cd = pair ? gs_code(g) : 0;
+            codes |= (unsigned)cd << (2 * w);
+            a_by_g[cd]++;
         }
-        L.range[4 * c + 3] = n_act | (mask << 16);
+        L.range[4 * ch + 3] = (int)((unsigned)n_act | (codes << 16));
         // Work of the waves that run the sums, in slots.  A block's 8 waves
         // share 4 SIMDs two by two, so a chunk with a active waves takes as
         // long as ceil(a / 4) * 4 of them.
         const int a = (n_act + 63) / 64;
         if (a > 0) {
             const double scale = (double)(((a + 3) / 4) * 4) / a;
-            L.quad_slots += (long)(aq * 64.0 * R * scale);
-            L.paired_slots += (long)(ap * 64.0 * R * scale);
-            L.unpaired_slots += (long)((a - ap - aq) * 64.0 * R * scale);
+            L.gs8_slots += (long)(a_by_g[3] * 64.0 * R * scale);
+            L.quad_slots += (long)(a_by_g[2] * 64.0 * R * scale);
+            L.paired_slots += (long)(a_by_g[1] * 64.0 * R * scale);
+            L.unpaired_slots += (long)(a_by_g[0] * 64.0 * R * scale);
         }
         // conflict estimate of the chunk's reads
         for (int r = 0; r < R; r++)
             for (int h = 0; h < 512; h += kLanesPerPass) {
                 std::vector<int> ys, zs, xs;
                 for (int t = h; t < h + kLanesPerPass; t++) {
-                    const Vec &v = units[(size_t)c * 512 + t][r];
+                    const Vec &v = units[(size_t)ch * 512 + t][r];
                     ys.push_back(E.shared(v));
                     zs.push_back(E.own(v));
                     if (r == 0) xs.push_back(v.nx);
                 }
-                if (r % unit_g[(size_t)c * 512 + h] == 0) L.conflicts += passes_minus_one(ys);
+                if (r % unit_g[(size_t)ch * 512 + h] == 0) L.conflicts += passes_minus_one(ys);
                 L.conflicts += passes_minus_one(zs);
                 if (r == 0) L.conflicts += passes_minus_one(xs);
             }
@@ -594,7 +929,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM)
 extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst,
                                           int *range, int *n_chunks, long *conflicts)
 {
-    if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks || pair < 0 || pair > 3) return AMDAT_EINVAL;
+    if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks || pair < 0 || pair > 4) return AMDAT_EINVAL;
     if (pair && R < 4) return AMDAT_EINVAL;
     int M = 0;
     for (int i = 0; i < 3 * K; i++) M = std::max(M, std::abs(nvec[i]));

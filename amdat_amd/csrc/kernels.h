@@ -30,8 +30,8 @@ struct KPlan {
     uint32_t *kpack = nullptr;  // [n_chunks*KB] (nx+128) | (ny+128)<<8 | nz<<16
     int *kdst = nullptr;        // [n_chunks*KB] destination index, -1 padding, bit 30 = conjugate
     int4 *range = nullptr;      // [n_chunks] (nx lo, nx hi, max|ny| | max nz << 16,
-                                //  units with vectors | paired-wave mask << 16, bit w = wave w)
-    int pair = 0;               // 1: slots 2p and 2p+1 of a thread share ny (e_x e_y formed once per pair)
+                                //  units with vectors | group codes << 16, 2 bits per wave)
+    int pair = 0;               // 1: grouped plan, wave w's GS = 1 << code(w) slots share e_x e_y
 };
 
 // One trajectory of a batch (device-side table entry).

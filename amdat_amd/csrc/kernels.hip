@@ -14,8 +14,8 @@ namespace amdat {
 namespace {
 
 // rho_k(t), compensated FP32.  Work items (chunk, frame slot, split),
-// numbered by xcd_item.  PAIR: the plan pairs slots sharing ny.
-template <int R, int MM, bool PAIR, bool QUAD>
+// numbered by xcd_item.  GROUPED: the plan's waves carry group codes.
+template <int R, int MM, bool GROUPED>
 __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z, int N,
     const int *__restrict__ list, int n_list, const int *__restrict__ frames, int n_frames, int n_chunks,
@@ -29,14 +29,14 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
     float hr[R], lr[R], hi[R], li[R];
-    rho_block<R, MM, PAIR, QUAD>(X, Y, Z, list, (size_t)frames[fslot] * (size_t)N, a_begin, a_end, inv_ell,
+    rho_block<R, MM, GROUPED>(X, Y, Z, list, (size_t)frames[fslot] * (size_t)N, a_begin, a_end, inv_ell,
                      kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
     rho_write<R>(kdst + (size_t)chunk * KB, hr, lr, hi, li, out + ((size_t)fslot * n_splits + split) * (size_t)K);
 }
 
 // Fs per (pair, vector), compensated FP32: sum over the split's atoms of
 // cos(k.dr).  Work items (chunk, pair slot, split), numbered by xcd_item.
-template <int R, int MM, bool PAIR, bool QUAD>
+template <int R, int MM, bool GROUPED>
 __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
     float hs[R], ls[R];
-    fs_block<R, MM, PAIR, QUAD>(X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N, a_begin,
+    fs_block<R, MM, GROUPED>(X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N, a_begin,
                     a_end, inv_ell, kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, hs, ls);
     fs_write<R>(kdst + (size_t)chunk * KB, hs, ls, out + ((size_t)pslot * n_splits + split) * (size_t)K);
 }
@@ -715,13 +715,13 @@ __global__ __launch_bounds__(kBlock) void adjoint_reduce_kernel(const double *__
     gz[j] = a[2] * kscale;
 }
 
-template <int R, int MM, bool PAIR, bool QUAD>
+template <int R, int MM, bool GROUPED>
 hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, const int *frames, int n_frames,
                             const KPlan &kp, double inv_ell, int n_splits, double2 *out, hipStream_t st)
 {
     const int per_split = (n_list + n_splits - 1) / n_splits;
     const size_t lds = Geom<MM>::tab_bytes;
-    auto kfn = rho_c32_kernel<R, MM, PAIR, QUAD>;
+    auto kfn = rho_c32_kernel<R, MM, GROUPED>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const long rows = (long)n_frames * n_splits;
@@ -734,14 +734,14 @@ hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, cons
     return hipGetLastError();
 }
 
-template <int R, int MM, bool PAIR, bool QUAD>
+template <int R, int MM, bool GROUPED>
 hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
                            int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
                            hipStream_t st)
 {
     const int per_split = (n_list + n_splits - 1) / n_splits;
     const size_t lds = Geom<MM>::tab_bytes;
-    auto kfn = fs_c32_kernel<R, MM, PAIR, QUAD>;
+    auto kfn = fs_c32_kernel<R, MM, GROUPED>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const long rows = (long)n_pairs * n_splits;
@@ -754,46 +754,44 @@ hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const
     return hipGetLastError();
 }
 
-template <int R, bool PAIR, bool QUAD>
+template <int R, bool GROUPED>
 hipError_t dispatch_rho_c32_p(int MM, const TrajDev &tr, const int *list, int n_list, const int *frames,
                               int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,
                               hipStream_t st)
 {
     switch (MM) {
-    case 24: return launch_rho_c32_t<R, 24, PAIR, QUAD>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
-    case 50: return launch_rho_c32_t<R, 50, PAIR, QUAD>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
-    case 100: return launch_rho_c32_t<R, 100, PAIR, QUAD>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
-    case 127: return launch_rho_c32_t<R, 127, PAIR, QUAD>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 24: return launch_rho_c32_t<R, 24, GROUPED>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 50: return launch_rho_c32_t<R, 50, GROUPED>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 100: return launch_rho_c32_t<R, 100, GROUPED>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    case 127: return launch_rho_c32_t<R, 127, GROUPED>(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-// Paired kernels exist for R = 4 and 8; any plan also runs on the unpaired
-// kernel (every slot carries its full vector).
+// Grouped kernels exist for R = 4 and 8; an ungrouped plan runs on the
+// kernel without group codes (every slot carries its full vector).
 template <int R>
 hipError_t dispatch_rho_c32(int MM, const TrajDev &tr, const int *list, int n_list, const int *frames,
                             int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,
                             hipStream_t st)
 {
     if constexpr (R >= 4) {
-        if (kp.pair == 2)
-            return dispatch_rho_c32_p<R, true, true>(MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
         if (kp.pair)
-            return dispatch_rho_c32_p<R, true, false>(MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+            return dispatch_rho_c32_p<R, true>(MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
     }
-    return dispatch_rho_c32_p<R, false, false>(MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
+    return dispatch_rho_c32_p<R, false>(MM, tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
 }
 
-template <int R, bool PAIR, bool QUAD>
+template <int R, bool GROUPED>
 hipError_t dispatch_fs_c32_p(int MM, const TrajDev &tr, const int *list, int n_list, const int *t0,
                              const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits,
                              double *out, hipStream_t st)
 {
     switch (MM) {
-    case 24: return launch_fs_c32_t<R, 24, PAIR, QUAD>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 50: return launch_fs_c32_t<R, 50, PAIR, QUAD>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 100: return launch_fs_c32_t<R, 100, PAIR, QUAD>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 127: return launch_fs_c32_t<R, 127, PAIR, QUAD>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 24: return launch_fs_c32_t<R, 24, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 50: return launch_fs_c32_t<R, 50, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 100: return launch_fs_c32_t<R, 100, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 127: return launch_fs_c32_t<R, 127, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -804,12 +802,10 @@ hipError_t dispatch_fs_c32(int MM, const TrajDev &tr, const int *list, int n_lis
                            double *out, hipStream_t st)
 {
     if constexpr (R >= 4) {
-        if (kp.pair == 2)
-            return dispatch_fs_c32_p<R, true, true>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
         if (kp.pair)
-            return dispatch_fs_c32_p<R, true, false>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+            return dispatch_fs_c32_p<R, true>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     }
-    return dispatch_fs_c32_p<R, false, false>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    return dispatch_fs_c32_p<R, false>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
 }
 
 }  // namespace

@@ -21,8 +21,8 @@
 //     unrolled), and two complex products,
 //         exp(i k.r) = e_x(nx) e_y(ny) e_z(nz),
 //     accumulated with FMA into an FP32 partial per tile; tile partials are
-//     folded into a TwoSum-compensated FP32 (hi, lo) pair; the block writes
-//     hi + lo as FP64.
+//     folded into a compensated FP32 (hi, lo) pair (Fast2Sum); the block
+//     writes hi + lo as FP64.
 // The plain-FP64 form evaluates exp(i k.r) with FP64 sincos per update and
 // accumulates in FP64; it is the reference the FP32 form is measured against.
 #pragma once
@@ -34,7 +34,9 @@ namespace amdat {
 namespace {
 
 constexpr int kWaves = kPhaseBlock / 64;
-// Prefetch depth (updates in flight) of the hot loops.
+// Prefetch depth (updates in flight) of the hot loops.  (Fs with 8 in
+// flight ran no faster on the 64,000-atom workload and spills in its
+// unpaired 32-atom-tile variant.)
 constexpr int kDepth = 4;
 
 // Table geometry for half-width MM: the sections are padded to whole
@@ -73,9 +75,13 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 
 // Per-chunk range descriptor (int4): x = lowest nx, y = highest nx,
 // z = max |ny| | max nz << 16, w = thread units holding vectors (waves past
-// them skip the hot loop).
+// them skip the hot loop) | group codes << 16: two bits per wave w at
+// 16 + 2w, group size GS = 1 << code (slots g*GS .. g*GS + GS - 1 of a
+// thread share the lead slot's e_x e_y, or e_x e_z).
 __device__ __forceinline__ int range_my(int4 r) { return r.z & 0xFFFF; }
 __device__ __forceinline__ int range_mz(int4 r) { return r.z >> 16; }
+__device__ __forceinline__ int range_units(int4 r) { return r.w & 0xFFFF; }
+__device__ __forceinline__ int range_gs_code(int4 r, int wave) { return (int)(((unsigned)r.w >> (16 + 2 * wave)) & 3u); }
 
 // Build one tile of tables from the fixed-point fractions sU[3A] (row
 // 3i+c = atom i, axis c; invalid atoms carry sValid[i] == 0 and get zero
@@ -181,14 +187,12 @@ __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (
 // and batched phase kernels.  FS = false: rho (complex sums into h0/l0 = Re,
 // h1/l1 = Im); FS = true: Fs (real part only, into h0/l0).  uf(a, c) gives
 // the fixed-point fraction of list position a on axis c.  The R vectors of
-// a thread share nx: e_x(nx) is read once per atom.  PAIR (paired plans):
-// in the waves that hold pairs, slots 2p and 2p+1 also share ny (or nz,
-// with y and z swapped), so q = e_x e_y is read and formed once per pair and
-// each slot adds q e_z: 6 VALU per update for rho instead of 8, 4 for Fs
-// instead of 6; the plan's other waves run unpaired.  QUAD (quad plans):
-// the waves the mask marks hold quads, slots 4q .. 4q+3 sharing ny (5 VALU
-// per rho update, 3 for Fs), the others pairs.
-template <int R, int MM, bool FS, bool PAIR, bool QUAD, typename UF>
+// a thread share nx: e_x(nx) is read once per atom.  GROUPED plans: each
+// wave's group code gives its group size GS; the GS slots of a group also
+// share ny (or nz, with the roles of y and z swapped), so q = e_x e_y is
+// read and formed once per group and each slot adds q e_z: rho costs
+// 4 + 4/GS VALU per update (8 for GS = 1, 4.5 for GS = 8), Fs 2 + 4/GS.
+template <int R, int MM, bool FS, bool GROUPED, typename UF>
 __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
                                             float2 *tab, float2 *anc, uint32_t *sU, int *sValid, const UF uf,
                                             float (&h0)[R], float (&l0)[R], float (&h1)[R], float (&l1)[R])
@@ -196,11 +200,10 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
     using Gm = Geom<MM>;
     const int tid = threadIdx.x;
     // waves whose units all lie past the chunk's last real unit skip the
-    // sums; in paired plans, bit 16 + w of rng.w marks wave w as paired (as
-    // holding quads in quad plans)
+    // sums; grouped plans give each wave its group size
     const int wave0 = __builtin_amdgcn_readfirstlane(tid >> 6) * 64;
-    const bool active = wave0 < (rng.w & 0xFFFF);
-    const bool paired = (rng.w >> (16 + (wave0 >> 6))) & 1;
+    const bool active = wave0 < range_units(rng);
+    const int gs_code = GROUPED ? range_gs_code(rng, wave0 >> 6) : 0;
     const float2 *px = nullptr, *py[R], *pz[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -237,22 +240,29 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             build_tables<MM>(tab, anc, sU, sValid, rng);
             __syncthreads();
             if (!active) continue;
-            if constexpr (PAIR) {
-                if constexpr (QUAD) {
-                    if (paired) hot_tile<R, MM, FS, 4>(px, py, pz, p0, p1);
-                    else hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
-                } else {
-                    if (paired) hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
-                    else hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
+            if constexpr (GROUPED) {
+                if constexpr (R >= 8) {
+                    if (gs_code == 3) {
+                        hot_tile<R, MM, FS, 8>(px, py, pz, p0, p1);
+                        continue;
+                    }
                 }
+                if constexpr (R >= 4) {
+                    if (gs_code == 2) {
+                        hot_tile<R, MM, FS, 4>(px, py, pz, p0, p1);
+                        continue;
+                    }
+                }
+                if (gs_code == 1) hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
+                else hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
             } else {
                 hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
             }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            two_sum_acc(h0[r], l0[r], p0[r]);
-            if constexpr (!FS) two_sum_acc(h1[r], l1[r], p1[r]);
+            fast_two_sum_acc(h0[r], l0[r], p0[r]);
+            if constexpr (!FS) fast_two_sum_acc(h1[r], l1[r], p1[r]);
         }
     }
 }
@@ -299,7 +309,7 @@ struct DispFrac {
 };
 
 // rho over one block's atoms: compensated sums (hr + lr, hi + li) per slot.
-template <int R, int MM, bool PAIR = false, bool QUAD = false>
+template <int R, int MM, bool GROUPED = false>
 __device__ __forceinline__ void rho_block(const float *__restrict__ X, const float *__restrict__ Y,
                                           const float *__restrict__ Z, const int *__restrict__ list, size_t foff,
                                           int a_begin, int a_end, double inv_ell, const uint32_t *__restrict__ kp,
@@ -307,7 +317,7 @@ __device__ __forceinline__ void rho_block(const float *__restrict__ X, const flo
                                           float (&lr)[R], float (&hi)[R], float (&li)[R])
 {
     const CoordFrac uf{X, Y, Z, list, foff, inv_ell};
-    phase_tiles<R, MM, false, PAIR, QUAD>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
+    phase_tiles<R, MM, false, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
 }
 
 template <int R>
@@ -327,7 +337,7 @@ __device__ __forceinline__ void rho_write(const int *__restrict__ kd, const floa
 }
 
 // Fs over one block's atoms for the pair (o0, o1): compensated sum of cos(k.dr).
-template <int R, int MM, bool PAIR = false, bool QUAD = false>
+template <int R, int MM, bool GROUPED = false>
 __device__ __forceinline__ void fs_block(const float *__restrict__ X, const float *__restrict__ Y,
                                          const float *__restrict__ Z, const int16_t *__restrict__ IX,
                                          const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
@@ -338,7 +348,7 @@ __device__ __forceinline__ void fs_block(const float *__restrict__ X, const floa
 {
     const DispFrac uf{X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell};
     float u1[R], u2[R];
-    phase_tiles<R, MM, true, PAIR, QUAD>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
+    phase_tiles<R, MM, true, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
 }
 
 template <int R>

@@ -84,4 +84,17 @@ AMDAT_HD inline void two_sum_acc(float &hi, float &lo, float x)
     lo += e;
 }
 
+// Compensated accumulation by Dekker's Fast2Sum: hi + x = s + e exactly
+// when |hi| >= |x| (the usual case when tile partials are folded into a
+// running sum), in 4 operations against TwoSum's 7; a step with |hi| < |x|
+// (the first folds, or the sum passing through zero) keeps the error of one
+// plain FP32 addition.
+AMDAT_HD inline void fast_two_sum_acc(float &hi, float &lo, float x)
+{
+    float s = hi + x;
+    float z = s - hi;
+    lo += x - z;
+    hi = s;
+}
+
 }  // namespace amdat

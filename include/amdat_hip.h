@@ -147,7 +147,9 @@ int amdat_shells_upload(amdat_ctx *ctx, int n_shells, const int *shell_off, cons
  * plans): fix the vectors per thread R (2, 4 or 8) and the pairing of slots
  * that share ny or nz (R = 4 or 8; pair = 1: vectors without a partner are
  * padded, 2: they run in unpaired waves, 3: quads of four vectors sharing
- * ny first, then pairs as 1) for the single-trajectory phase kernels run on
+ * ny first, then pairs as 1, 4: groups of 8, 4, 2 or 1 vectors sharing nz
+ * or ny, placed against LDS bank clashes) for the single-trajectory phase
+ * kernels run on
  * this table; R = 0 restores the automatic choice (lowest modelled cost).
  * Results do not depend on it beyond FP32 rounding. */
 int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair);

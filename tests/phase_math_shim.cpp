@@ -22,4 +22,11 @@ void shim_two_sum(const float *x, int n, float *hi, float *lo)
     *hi = h;
     *lo = l;
 }
+void shim_fast_two_sum(const float *x, int n, float *hi, float *lo)
+{
+    float h = 0.0f, l = 0.0f;
+    for (int i = 0; i < n; i++) amdat::fast_two_sum_acc(h, l, x[i]);
+    *hi = h;
+    *lo = l;
+}
 }

@@ -12,6 +12,13 @@ from amdat_amd import _abi
 from tests.conftest import ref_shells
 
 
+def wave_codes(rng):
+    """Group code of each chunk's 8 waves (2 bits per wave above bit 16 of
+    the range word; group size 1 << code)."""
+    w = rng[:, 3].astype(np.int64) & 0xFFFFFFFF
+    return (w[:, None] >> (16 + 2 * np.arange(8))[None, :]) & 3
+
+
 def layout(nvec, R, pair=0):
     L = _abi.lib()
     fn = L.amdat_internal_plan_layout
@@ -97,8 +104,8 @@ def test_layout_lowers_bank_conflicts(qvec):
 @pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("bins,R", [((2, 100), 8), ((2, 100), 4), ((40, 41), 8), ((2, 3), 4), ((95, 100), 4)])
 def test_paired_layout_invariants(qvec, bins, R, mode):
-    """Paired plans: the same invariants; the units of the waves the chunk's
-    mask marks as paired hold pairs sharing ny, or nz with both slots
+    """Paired plans: the same invariants; the units of the waves whose group
+    code is 1 (pairs) hold pairs sharing ny, or nz with both slots
     flagged as swapped, and the other waves single vectors; chunk count from
     pairing on ny first, then on nz, and the vectors left padded into pairs
     (mode 1: every wave paired) or R per unit in unpaired waves of their
@@ -125,8 +132,8 @@ def test_paired_layout_invariants(qvec, bins, R, mode):
         else:
             up += -(-groups // (R // 2))
             us += -(-singles // R)
-    mask = rng[:, 3] >> 16
-    wave_p = (mask[:, None] >> np.arange(8)[None, :]) & 1                # [n, 8]
+    wave_p = (wave_codes(rng) == 1).astype(int)                          # [n, 8]: waves of pairs
+    assert np.all(wave_codes(rng) <= 1)
     is_p = np.repeat(wave_p, 64, axis=1).astype(bool)                   # [n, 512] per unit
     act_w = -(-(rng[:, 3] & 0xFFFF) // 64)
     if mode == 1:
@@ -179,8 +186,9 @@ def test_quad_layout_invariants(qvec, bins, R):
     flip = (dst[real] & (1 << 30)) != 0
     assert np.array_equal(np.stack([nx[real], ny[real], nz[real]], 1), nvec[idx] * np.where(flip, -1, 1)[:, None])
     assert np.all(nx == nx[:, :1, :])
-    mask = rng[:, 3] >> 16
-    isq = np.repeat(((mask[:, None] >> np.arange(8)[None, :]) & 1).astype(bool), 64, axis=1)    # [n, 512]
+    codes = wave_codes(rng)
+    assert np.all(codes[codes != 2] <= 1)                        # quads (code 2) and pairs
+    isq = np.repeat(codes == 2, 64, axis=1)                      # [n, 512]
     for q in range(R // 4):
         g = slice(4 * q, 4 * q + 4)
         ok = np.all(ny[:, g, :] == ny[:, 4 * q:4 * q + 1, :], axis=1) & ~np.any(sw[:, g, :], axis=1)
@@ -190,3 +198,55 @@ def test_quad_layout_invariants(qvec, bins, R):
     canon = np.where(nvec[:, 2:3] < 0, -nvec, nvec)
     cls = np.unique(canon[:, :2], axis=0, return_counts=True)[1]
     assert int((real & isq[:, None, :]).sum()) == int(np.sum(cls // 4) * 4)
+
+
+@pytest.mark.parametrize("bins,R", [((2, 100), 8), ((2, 100), 4), ((40, 41), 8), ((2, 3), 4), ((95, 100), 8)])
+def test_grouped_layout_invariants(qvec, bins, R):
+    """Grouped plans (mode 4): every vector exactly once with its flip; a
+    thread's slots share nx; in a wave of group code g the slots of each
+    group of 1 << g share the lead slot's roles and shared component (nz
+    when swapped, else ny), which the kernel's product q is formed from;
+    codes fit R; ranges cover every slot; units past the active count hold
+    no vector."""
+    off, nvec = ref_shells(qvec, *bins)
+    K = len(nvec)
+    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R, pair=4)
+    sw = layout.swap
+    real = dst >= 0
+    idx = dst[real] & ~(1 << 30)
+    assert np.array_equal(np.sort(idx), np.arange(K))
+    flip = (dst[real] & (1 << 30)) != 0
+    assert np.array_equal(np.stack([nx[real], ny[real], nz[real]], 1), nvec[idx] * np.where(flip, -1, 1)[:, None])
+    assert np.all(nz >= 0) and np.all(nx == nx[:, :1, :])
+    codes = wave_codes(rng)
+    assert codes.max() <= {4: 2, 8: 3}[R]
+    act = rng[:, 3] & 0xFFFF
+    key = np.where(sw, nz, ny)
+    for c in range(n):
+        for w in range(8):
+            if w * 64 >= act[c]:
+                assert codes[c, w] == 0
+                continue
+            gs = 1 << codes[c, w]
+            lanes = slice(w * 64, w * 64 + 64)
+            for g0 in range(0, R, gs):
+                lead_sw, lead_key = sw[c, g0, lanes], key[c, g0, lanes]
+                for r in range(g0, g0 + gs):
+                    m = real[c, r, lanes]
+                    assert np.all(sw[c, r, lanes][m] == lead_sw[m])
+                    assert np.all(key[c, r, lanes][m] == lead_key[m])
+    xlo, xhi = rng[:, 0][:, None, None], rng[:, 1][:, None, None]
+    my, mz = (rng[:, 2] & 0xFFFF)[:, None, None], (rng[:, 2] >> 16)[:, None, None]
+    assert np.all((nx >= xlo) & (nx <= xhi) & (np.abs(ny) <= my) & (nz <= mz))
+    for c in range(n):
+        unit_real = real[c].any(axis=0)
+        assert not np.any(unit_real[act[c]:])
+
+
+def test_grouped_layout_fewer_clashes_than_pairs(qvec):
+    """On the reference shells 2..100, the grouped placement leaves fewer
+    extra LDS passes than the paired layout."""
+    off, nvec = ref_shells(qvec, 2, 100)
+    c4 = layout(nvec, 8, pair=4)[-1]
+    c1 = layout(nvec, 8, pair=1)[-1]
+    assert c4 < c1
