@@ -77,6 +77,8 @@ SIGNATURES = {
     "amdat_fs": (_I, [_P, _P, _P, _P, _I, _I, _F, _P, _I, _PD, _PD]),
     "amdat_msd_ngp": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
     "amdat_msd_ngp_noalloc": (_I, [_P, _P, _P, _P, _PD, _PD, _PD]),
+    "amdat_fkt_vectors": (_I, [_P, _P, _P, _P, _I, _PI, _PI, _PD, _PD]),
+    "amdat_rho_sum": (_I, [_P, _P, _P, C.POINTER(_P)]),
     "amdat_displacements": (_I, [_P, _P, _P, _P, _I, _F, _PD, _PD, C.POINTER(C.c_ulonglong), _PD]),
     "amdat_displacement_dist": (_I, [_P, _P, _P, _P, _I, _F, C.POINTER(C.c_ulonglong), _PD, _PD, _PD]),
     "amdat_ts_gap_pairs": (_I, [_I, _I, _I, _I, _I, _PI, _PI, _PI, _I, _PI]),
@@ -474,6 +476,27 @@ def fkt(ctx, rho1, rho2, pairs, wn1=0, wn2=None):
     cnt = np.zeros(pairs.ntg, np.float64)
     check(lib().amdat_fkt(ctx.h, rho1.h, rho2.h, pairs.h, wn1, wn2, dp(F), dp(cnt)))
     return F, cnt
+
+
+def fkt_vectors(ctx, rho1, rho2, pairs, v_first, v_count):
+    """F over arbitrary vector ranges: column c sums vectors
+    v_first[c] .. v_first[c] + v_count[c] - 1 of the densities."""
+    vf = np.ascontiguousarray(v_first, np.int32)
+    vc = np.ascontiguousarray(v_count, np.int32)
+    F = np.zeros((pairs.ntg, len(vf)), np.float64)
+    cnt = np.zeros(pairs.ntg, np.float64)
+    check(lib().amdat_fkt_vectors(ctx.h, rho1.h, rho2.h, pairs.h, len(vf), ip(vf), ip(vc), dp(F), dp(cnt)))
+    return F, cnt
+
+
+def rho_sum(ctx, a, b):
+    """A new density a + b (same frames and vectors; atom counts added)."""
+    h = C.c_void_p()
+    check(lib().amdat_rho_sum(ctx.h, a.h, b.h, C.byref(h)))
+    r = Rho.__new__(Rho)
+    _Handle.__init__(r, ctx, h)
+    r.n_frames, r.K, r.n_shells = a.n_frames, a.K, a.n_shells
+    return r
 
 
 def fs(ctx, traj, alist, shells, first, last, length_scale, pairs, precision=PREC_FP32C):

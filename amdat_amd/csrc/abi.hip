@@ -1303,37 +1303,42 @@ int amdat_sk(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, double *S
     HIP_TRY(d_norm.upload(&norm, 1, st));
     HIP_TRY(d_out.alloc(r1->n_shells));
     HIP_TRY(launch_corr_shell(r1->d.p, r2->d.p, r1->K, d_goff.p, d_rows.p, d_rows.p, 1, r1->d_shell_off.p,
+                              r1->d_shell_off.p + 1,
                               r1->n_shells, d_norm.p, d_out.p, st));
     HIP_TRY(hipMemcpyAsync(S, d_out.p, sizeof(double) * r1->n_shells, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
 }
 
-int amdat_fkt(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs, int wn1, int wn2,
-              double *F, double *count)
+}  // extern "C"
+
+namespace {
+// F(k,t) over columns of vectors: column c sums vectors vbeg[c] ..
+// vend[c]-1 of the densities (device arrays; null = the densities' shells
+// col0 .. col0+n_cols-1).  Pairs in reference order within each time gap;
+// normalised by the atoms of the first density at each pair's t0.
+int fkt_columns(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs, int n_cols,
+                const int *vbeg, const int *vend, int col0, double *F, double *count, const char *fn)
 {
-    clear_error();
-    if (!ctx || !r1 || !r2 || !pairs || !F || !count) return set_error(AMDAT_EINVAL, "amdat_fkt: null argument");
     if (r1->K != r2->K || r1->shell_off != r2->shell_off || r1->T != r2->T)
-        return set_error(AMDAT_EINVAL, "amdat_fkt: densities cover different vectors");
-    if (wn1 < 0 || wn2 < wn1 || wn2 >= r1->n_shells)
-        return set_error(AMDAT_ERANGE, "amdat_fkt: shell range outside the densities");
+        return set_error(AMDAT_EINVAL, std::string(fn) + ": densities cover different vectors");
     int rc = check_pairs_frames(pairs, r1->T);
     if (rc) return rc;
     hipStream_t st = stream_of(ctx);
-    const int ntg = pairs->ntg, nsh = wn2 - wn1 + 1;
+    const int ntg = pairs->ntg;
     std::vector<int> row0(pairs->n), row1(pairs->n);
     std::vector<double> norm(ntg, 0.0);
     for (int q = 0; q < pairs->n; q++) {
         const int i = pairs->order[q];
         row0[q] = r1->frame_row[pairs->t0[i]];
         row1[q] = r2->frame_row[pairs->t1[i]];
-        if (row0[q] < 0 || row1[q] < 0) return set_error(AMDAT_ERANGE, "amdat_fkt: a pair uses a frame the density lacks");
+        if (row0[q] < 0 || row1[q] < 0)
+            return set_error(AMDAT_ERANGE, std::string(fn) + ": a pair uses a frame the density lacks");
         norm[pairs->tg[i]] += (double)r1->row_n[row0[q]];
     }
     for (int g = 0; g < ntg; g++) count[g] = norm[g];
-    if (pairs->n == 0) {
-        std::fill(F, F + (size_t)ntg * nsh, 0.0);
+    if (pairs->n == 0 || n_cols == 0) {
+        std::fill(F, F + (size_t)ntg * n_cols, 0.0);
         return AMDAT_OK;
     }
     DevBuf<int> d_r0, d_r1;
@@ -1341,11 +1346,78 @@ int amdat_fkt(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const am
     HIP_TRY(d_r0.upload(row0.data(), row0.size(), st));
     HIP_TRY(d_r1.upload(row1.data(), row1.size(), st));
     HIP_TRY(d_norm.upload(norm.data(), norm.size(), st));
-    HIP_TRY(d_out.alloc((size_t)ntg * nsh));
-    HIP_TRY(launch_corr_shell(r1->d.p, r2->d.p, r1->K, pairs->d_group_off.p, d_r0.p, d_r1.p, ntg,
-                              r1->d_shell_off.p + wn1, nsh, d_norm.p, d_out.p, st));
-    HIP_TRY(hipMemcpyAsync(F, d_out.p, sizeof(double) * ntg * nsh, hipMemcpyDeviceToHost, st));
+    HIP_TRY(d_out.alloc((size_t)ntg * n_cols));
+    const int *b = vbeg ? vbeg : r1->d_shell_off.p + col0;
+    const int *e = vbeg ? vend : r1->d_shell_off.p + col0 + 1;
+    HIP_TRY(launch_corr_shell(r1->d.p, r2->d.p, r1->K, pairs->d_group_off.p, d_r0.p, d_r1.p, ntg, b, e, n_cols,
+                              d_norm.p, d_out.p, st));
+    HIP_TRY(hipMemcpyAsync(F, d_out.p, sizeof(double) * ntg * n_cols, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int amdat_fkt(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs, int wn1, int wn2,
+              double *F, double *count)
+{
+    clear_error();
+    if (!ctx || !r1 || !r2 || !pairs || !F || !count) return set_error(AMDAT_EINVAL, "amdat_fkt: null argument");
+    if (wn1 < 0 || wn2 < wn1 || wn2 >= r1->n_shells)
+        return set_error(AMDAT_ERANGE, "amdat_fkt: shell range outside the densities");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return fkt_columns(ctx, r1, r2, pairs, wn2 - wn1 + 1, nullptr, nullptr, wn1, F, count, "amdat_fkt");
+}
+
+int amdat_fkt_vectors(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs,
+                      int n_cols, const int *v_first, const int *v_count, double *F, double *count)
+{
+    clear_error();
+    if (!ctx || !r1 || !r2 || !pairs || !F || !count || n_cols < 0 || (n_cols > 0 && (!v_first || !v_count)))
+        return set_error(AMDAT_EINVAL, "amdat_fkt_vectors: null argument");
+    std::vector<int> b(n_cols), e(n_cols);
+    for (int c = 0; c < n_cols; c++) {
+        if (v_first[c] < 0 || v_count[c] < 0 || (long)v_first[c] + v_count[c] > r1->K)
+            return set_error(AMDAT_ERANGE, "amdat_fkt_vectors: vector range outside the densities");
+        b[c] = v_first[c];
+        e[c] = v_first[c] + v_count[c];
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf<int> d_b, d_e;
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(d_b.upload(b.data(), b.size(), st));
+    HIP_TRY(d_e.upload(e.data(), e.size(), st));
+    return fkt_columns(ctx, r1, r2, pairs, n_cols, d_b.p, d_e.p, 0, F, count, "amdat_fkt_vectors");
+}
+
+int amdat_rho_sum(amdat_ctx *ctx, const amdat_rho *a, const amdat_rho *b, amdat_rho **out)
+{
+    clear_error();
+    if (!ctx || !a || !b || !out) return set_error(AMDAT_EINVAL, "amdat_rho_sum: null argument");
+    if (a->K != b->K || a->frames != b->frames || a->shell_off != b->shell_off || a->T != b->T ||
+        a->first_shell != b->first_shell)
+        return set_error(AMDAT_EINVAL, "amdat_rho_sum: densities cover different frames or vectors");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto r = std::make_unique<amdat_rho>();
+    r->n_frames = a->n_frames;
+    r->K = a->K;
+    r->n_list = a->n_list + b->n_list;
+    r->T = a->T;
+    r->first_shell = a->first_shell;
+    r->n_shells = a->n_shells;
+    r->frames = a->frames;
+    r->frame_row = a->frame_row;
+    r->shell_off = a->shell_off;
+    r->row_n.resize(a->row_n.size());
+    for (size_t i = 0; i < r->row_n.size(); i++) r->row_n[i] = a->row_n[i] + b->row_n[i];
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(r->d_shell_off.upload(r->shell_off.data(), r->shell_off.size(), st));
+    const size_t n = (size_t)r->n_frames * r->K;
+    HIP_TRY(r->d.alloc(n));
+    HIP_TRY(launch_rho_add(a->d.p, b->d.p, n, r->d.p, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *out = r.release();
     return AMDAT_OK;
 }
 

@@ -85,11 +85,15 @@ hipError_t launch_split_reduce2(const double2 *in, int n_rows, int n_splits, int
 hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_list,
                      const int *t0, const int *t1, int n_pairs, const KPlan &kp, double inv_ell,
                      int n_splits, double *out, hipStream_t st);
-// Per (group, shell): sum over the group's pairs and the shell's vectors of
-// Re(rho1[row0] conj rho2[row1]) / H, divided by norm[group].
+// Per (group, column s): sum over the group's pairs and the column's vectors
+// v in [vbeg[s], vend[s]) of Re(rho1[row0] conj rho2[row1]) / H (H =
+// vend[s] - vbeg[s]), divided by norm[group].  A shell table serves as
+// vbeg = shell_off, vend = shell_off + 1.
 hipError_t launch_corr_shell(const double2 *rho1, const double2 *rho2, int K, const int *group_off,
-                             const int *row0, const int *row1, int n_groups, const int *shell_off,
+                             const int *row0, const int *row1, int n_groups, const int *vbeg, const int *vend,
                              int n_shells, const double *norm, double *out, hipStream_t st);
+// out[i] = a[i] + b[i], i < n.
+hipError_t launch_rho_add(const double2 *a, const double2 *b, size_t n, double2 *out, hipStream_t st);
 // Per (group, shell): sum over the group's pairs (order[]) and splits of
 // V[p][split][v] over the shell's vectors / H, divided by norm[group].
 hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *group_off,

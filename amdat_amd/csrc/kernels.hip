@@ -156,16 +156,16 @@ __global__ __launch_bounds__(kBlock) void split_reduce2_kernel(const double2 *__
     }
 }
 
-// grid = (shells, groups).
+// grid = (columns, groups).
 __global__ __launch_bounds__(kBlock) void corr_shell_kernel(
     const double2 *__restrict__ rho1, const double2 *__restrict__ rho2, int K,
     const int *__restrict__ group_off, const int *__restrict__ row0, const int *__restrict__ row1,
-    const int *__restrict__ shell_off, int n_shells, const double *__restrict__ norm,
+    const int *__restrict__ vbeg, const int *__restrict__ vend, int n_shells, const double *__restrict__ norm,
     double *__restrict__ out)
 {
     __shared__ double red[kBlock];
     const int s = blockIdx.x, g = blockIdx.y;
-    const int v0 = shell_off[s], H = shell_off[s + 1] - v0;
+    const int v0 = vbeg[s], H = vend[s] - v0;
     const int p0 = group_off[g], np = group_off[g + 1] - p0;
     const long items = (long)np * H;
     double acc = 0.0;
@@ -180,6 +180,15 @@ __global__ __launch_bounds__(kBlock) void corr_shell_kernel(
     if (threadIdx.x == 0) {
         const double nrm = norm[g];
         out[(size_t)g * n_shells + s] = (H > 0 && nrm != 0.0) ? acc / (double)H / nrm : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void rho_add_kernel(const double2 *__restrict__ a, const double2 *__restrict__ b,
+                                                         size_t n, double2 *__restrict__ out)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double2 x = a[i], y = b[i];
+        out[i] = make_double2(x.x + y.x, x.y + y.y);
     }
 }
 
@@ -891,13 +900,22 @@ hipError_t launch_split_reduce2(const double2 *in, int n_rows, int n_splits, int
 }
 
 hipError_t launch_corr_shell(const double2 *rho1, const double2 *rho2, int K, const int *group_off,
-                             const int *row0, const int *row1, int n_groups, const int *shell_off,
+                             const int *row0, const int *row1, int n_groups, const int *vbeg, const int *vend,
                              int n_shells, const double *norm, double *out, hipStream_t st)
 {
     (void)hipGetLastError();
 
     hipLaunchKernelGGL(corr_shell_kernel, dim3(n_shells, n_groups), dim3(kBlock), 0, st, rho1, rho2, K,
-                       group_off, row0, row1, shell_off, n_shells, norm, out);
+                       group_off, row0, row1, vbeg, vend, n_shells, norm, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rho_add(const double2 *a, const double2 *b, size_t n, double2 *out, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    (void)hipGetLastError();
+    const unsigned grid = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(rho_add_kernel, dim3(grid), dim3(kBlock), 0, st, a, b, n, out);
     return hipGetLastError();
 }
 

@@ -69,6 +69,70 @@ class Coordinate {
 class System;
 class Trajectory_List;
 
+// Kinds an analysis reports through what_are_you() (src/analysis.h:27-56).
+enum Analysis_Type {
+    analysis,
+    composition,
+    correlation_2d,
+    debyewaller_dist,
+    displacement_dist,
+    displacement_map,
+    exptime_trajectory_list,
+    fast_particles,
+    gaussian_comparison,
+    incoherent_scattering_function,
+    intermediate_scattering_function,
+    mean_displacement,
+    mean_square_displacement,
+    mean_square_displacement_2d,
+    n_fold_order_parameter,
+    non_gaussian_parameter,
+    radial_debye_waller,
+    rgtensor_stats,
+    space_time_correlation_function,
+    stiffness_dist,
+    strings,
+    edge_detector_timedependent,
+    radial_distribution_function,
+    displacement_list,
+    distance_neighbor_list,
+    neighbor_decorrelation_function,
+    find_between
+};
+
+// Parent of the analyses (src/analysis.h:62-134): the virtual interface
+// through which AMDAT's Control drives an analysis it holds as Analysis*
+// (run_analysis(Analysis*, string), src/control.cpp:1316-1363).  A method a
+// class does not implement prints the reference's message and does
+// nothing.  The per-atom kernels of the reference's interface
+// (listkernel, list_displacementkernel, atomkernel) have no counterpart:
+// the loops they served run on the device inside analyze().
+class Analysis {
+  public:
+    virtual ~Analysis() = default;
+    virtual Analysis_Type what_are_you() { return analysis; }
+    virtual void analyze(Trajectory_List *, Trajectory_List *);
+    virtual void analyze(Trajectory_List *);
+    // Trajectory_List_Bins targets (Bin_Static_Analysis / Bin_Dynamics_
+    // Analysis call these per bin): in this implementation the Control
+    // forms per-bin lists on the device and calls analyze() with them.
+    virtual void bin_hook(Trajectory_List *, int, int, int);
+    virtual void postprocess_bins() {}
+    virtual void write(std::string) const;
+    System *show_system() { return system_base_; }
+    virtual bool isThreadSafe() { return false; }
+
+  protected:
+    Analysis() = default;
+    explicit Analysis(System *sys) : system_base_(sys) {}
+    Analysis(const Analysis &) = default;
+    Analysis &operator=(const Analysis &) = default;
+    virtual void preprocess() {}
+    virtual void postprocess() {}
+    virtual void postprocess_list() {}
+    System *system_base_ = nullptr;
+};
+
 // Selection of atoms.  Static (Static_Trajectory_List: the same atoms at
 // every frame, in the reference's species -> molecule -> type -> index
 // order) or time-varying (Exptime_Trajectory_List and the Fast_Particles
@@ -269,7 +333,7 @@ class Wave_Vectors {
     amdat_shells *shells_ = nullptr;
 };
 
-class Wave_Density {
+class Wave_Density : public Analysis {
   public:
     // src/wave_density.cpp:46-80: every shell of wv; stf: only the first
     // frame of each exponential block, t = n_exponential_steps * b (the
@@ -286,9 +350,10 @@ class Wave_Density {
     void set(System *sys, const Wave_Vectors *wv);
     void set(System *sys, const Wave_Vectors *wv, int inner, int outer);
     // src/wave_density.cpp:313-336: rho_k(t) for every frame (or the block
-    // starts).  The reference adds to a density analysed before; here
-    // analyze() replaces it.
-    void analyze(Trajectory_List *t_list);
+    // starts), added to the density and atom counts of earlier calls since
+    // construction or set(), as the reference adds them.
+    void analyze(Trajectory_List *t_list) override;
+    using Analysis::analyze;
     // The reference's wd2 = wd1.
     void copy_from(const Wave_Density &other) { *this = other; }
     std::complex<double> show_density(int time, int wavenumberindex, int vectorindex) const;
@@ -314,12 +379,12 @@ class Wave_Density {
     mutable std::shared_ptr<std::vector<double>> host_;
 };
 
-class Structure_Factor {
+class Structure_Factor : public Analysis {
   public:
     Structure_Factor(System *sys, const Wave_Vectors *wv, int timescheme = -1);
-    void analyze(Trajectory_List *t_list);
-    void analyze(Trajectory_List *t_list, Trajectory_List *t_list2);
-    void write(const std::string &filename) const;
+    void analyze(Trajectory_List *t_list) override;
+    void analyze(Trajectory_List *t_list, Trajectory_List *t_list2) override;
+    void write(std::string filename) const override;
     float show(int wavenumberii) const { return structure_factor_[wavenumberii]; }
     // Frames the time scheme selects (src/analysis_onetime.cpp:22-52).
     std::vector<int> frames() const;
@@ -331,31 +396,38 @@ class Structure_Factor {
     std::vector<float> structure_factor_;
 };
 
-class Intermediate_Scattering_Function {
+class Intermediate_Scattering_Function : public Analysis {
   public:
     // src/intermediate_scattering_function.cpp:52-147: every shell of the
     // densities (which must share wave vectors, system and shell range)
     Intermediate_Scattering_Function(System *sys, Wave_Density *wd1, Wave_Density *wd2, bool parallel = true);
-    // :149-235: one shell
+    // :149-235: one shell, read as the range constructor reads wn1 = wn2 =
+    // wn (the density's first slot with shell wn's vector count);
+    // physics_correct = true reads shell wn
     Intermediate_Scattering_Function(System *sys, Wave_Density *wd1, Wave_Density *wd2, int wn,
-                                     bool parallel = true);
-    // :237-322: shells wn1 .. wn2.  Shell wn reads the densities of shell wn
-    // (the reference reads the density of shell wn - wn1 relative to the
-    // density's first shell with the vector count of shell wn, which only
-    // agrees when wn1 is the density's first shell).
+                                     bool parallel = true, bool physics_correct = false);
+    // :237-322: shells wn1 .. wn2, as the reference reads them: column
+    // wn - wn1 sums the density's slot wn - wn1 (counted from the density's
+    // first shell) over the vector count of shell wn, :556-562.  That is
+    // shell wn when wn1 is the density's first shell (the isf command's
+    // case); otherwise it is the slot's shell, over the first vectors of it,
+    // and a slot holding fewer vectors than shell wn is refused (the
+    // reference reads past it).  physics_correct = true reads shell wn.
     Intermediate_Scattering_Function(System *sys, Wave_Density *wd1, Wave_Density *wd2, int wn1, int wn2,
-                                     bool parallel = true);
+                                     bool parallel = true, bool physics_correct = false);
     // :324-412: every shell at one time gap only (pairs of System::
     // displacement_loop(analysis, traj, timegap, fullblock), src/system.cpp:
     // 3972-4023); time gap 0 gives S(k) over the block starts.  Other rows
     // stay 0.
     Intermediate_Scattering_Function(System *sys, int timegapii, Wave_Density *wd1, Wave_Density *wd2,
                                      bool parallel = true);
-    void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return intermediate_scattering_function; }
+    void write(std::string filename) const override;
     float show(int timeii, int binii) const { return correlation_[timeii][binii]; }
 
   private:
-    void compute(Wave_Density *wd1, Wave_Density *wd2, amdat_pairs *pairs, int only_timegap);
+    void compute(Wave_Density *wd1, Wave_Density *wd2, amdat_pairs *pairs, int only_timegap,
+                 bool slot_quirk = false);
     System *system_;
     const Wave_Vectors *wv_;
     int first_bin_, last_bin_;
@@ -363,12 +435,14 @@ class Intermediate_Scattering_Function {
     std::vector<std::vector<float>> correlation_;
 };
 
-class Incoherent_Scattering_Function {
+class Incoherent_Scattering_Function : public Analysis {
   public:
     // src/incoherent_scattering_function.cpp:176-212
     Incoherent_Scattering_Function(System *sys, const Wave_Vectors *wv, int inner, int outer, bool fblock);
-    void analyze(Trajectory_List *t_list);
-    void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return incoherent_scattering_function; }
+    void analyze(Trajectory_List *t_list) override;
+    using Analysis::analyze;
+    void write(std::string filename) const override;
     float show(int timeii, int binii) const { return correlation_[timeii][binii]; }
 
   private:
@@ -380,11 +454,13 @@ class Incoherent_Scattering_Function {
     std::vector<std::vector<float>> correlation_;
 };
 
-class Mean_Square_Displacement {
+class Mean_Square_Displacement : public Analysis {
   public:
     explicit Mean_Square_Displacement(System *sys);
-    void analyze(Trajectory_List *t_list);
-    void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return mean_square_displacement; }
+    void analyze(Trajectory_List *t_list) override;
+    using Analysis::analyze;
+    void write(std::string filename) const override;
     float show(int t) const { return msd_[t]; }
     double show_f64(int t) const { return msd64_[t]; }
     int show_n_times() const { return (int)msd_.size(); }
@@ -405,11 +481,13 @@ struct Displacement_Stats {
 };
 
 // src/mean_square_displacement_2d.cpp: MSD in the xy, xz or yz plane.
-class Mean_Square_Displacement_2D {
+class Mean_Square_Displacement_2D : public Analysis {
   public:
     Mean_Square_Displacement_2D(System *sys, const std::string &plane);
-    void analyze(Trajectory_List *t_list);
-    void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return mean_square_displacement_2d; }
+    void analyze(Trajectory_List *t_list) override;
+    using Analysis::analyze;
+    void write(std::string filename) const override;
     float show(int t) const { return msd_[t]; }
 
   private:
@@ -420,11 +498,13 @@ class Mean_Square_Displacement_2D {
 };
 
 // src/mean_displacement.cpp: mean displacement vector per time gap.
-class Mean_Displacement {
+class Mean_Displacement : public Analysis {
   public:
     explicit Mean_Displacement(System *sys);
-    void analyze(Trajectory_List *t_list);
-    void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return mean_displacement; }
+    void analyze(Trajectory_List *t_list) override;
+    using Analysis::analyze;
+    void write(std::string filename) const override;
     Coordinate show(int t) const { return md_[t]; }
 
   private:
@@ -438,12 +518,13 @@ class Mean_Displacement {
 // the last bin taking the overflow; mean and variance of the squared
 // displacement.  DebyeWaller_Dist (src/debyewaller_dist.cpp) computes the
 // same and writes another header.
-class Displacement_Distribution {
+class Displacement_Distribution : public Analysis {
   public:
     Displacement_Distribution(System *sys, float power, int bins, float maxvalue, float t);
-    virtual ~Displacement_Distribution() = default;
-    void analyze(Trajectory_List *t_list);
-    virtual void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return displacement_dist; }
+    void analyze(Trajectory_List *t_list) override;
+    using Analysis::analyze;
+    void write(std::string filename) const override;
     float show_mean() const { return mean_; }
     float show_variance() const { return variance_; }
     float show(int bin) const { return distribution_.at(bin); }
@@ -460,17 +541,20 @@ class Displacement_Distribution {
 class DebyeWaller_Dist : public Displacement_Distribution {
   public:
     DebyeWaller_Dist(System *sys, int bins, float maxvalue, float t);
-    void write(const std::string &filename) const override;
+    Analysis_Type what_are_you() override { return debyewaller_dist; }
+    void write(std::string filename) const override;
 };
 
 // src/van_hove_self.cpp + src/space_time_correlation_function.cpp: self
 // part of the van Hove function, counts of |dr| per bin over atom pairs /
 // (weighting * shell volume).
-class Van_Hove_Self {
+class Van_Hove_Self : public Analysis {
   public:
     Van_Hove_Self(System *sys, int bin_count, float value_max);
-    void analyze(Trajectory_List *t_list);
-    void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return space_time_correlation_function; }
+    void analyze(Trajectory_List *t_list) override;
+    using Analysis::analyze;
+    void write(std::string filename) const override;
     float show(int t, int b) const { return corr_[(size_t)t * n_bins_ + b]; }
     int show_n_bins() const { return n_bins_; }
     float show_bin_size() const { return bin_size_; }
@@ -489,14 +573,15 @@ class Non_Gaussian_Parameter;
 // volume); the crossings give the slow and fast boundaries (squared
 // displacements) and the fractions.  Host arithmetic on the device-computed
 // van Hove and MSD, in the reference's types and order.
-class Gaussian_Comparison {
+class Gaussian_Comparison : public Analysis {
   public:
     // time index: the non-Gaussian parameter's maximum (src/gaussian_comparison.cpp:126-149)
     Gaussian_Comparison(System *sys, const Non_Gaussian_Parameter *ngp, const Van_Hove_Self *vhs,
                         const Mean_Square_Displacement *msd);
     // given time index (src/gaussian_comparison.cpp:160-185)
     Gaussian_Comparison(System *sys, int time_in, const Van_Hove_Self *vhs, const Mean_Square_Displacement *msd);
-    void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return gaussian_comparison; }
+    void write(std::string filename) const override;
     int show_time_index() const { return time_index_; }
     float show_fastboundary() const { return fastboundary_; }
     float show_slowboundary() const { return slowboundary_; }
@@ -515,11 +600,13 @@ class Gaussian_Comparison {
     bool errorstate_ = false;
 };
 
-class Non_Gaussian_Parameter {
+class Non_Gaussian_Parameter : public Analysis {
   public:
     Non_Gaussian_Parameter(System *sys, const Mean_Square_Displacement *msd);
-    void analyze(Trajectory_List *t_list);
-    void write(const std::string &filename) const;
+    Analysis_Type what_are_you() override { return non_gaussian_parameter; }
+    void analyze(Trajectory_List *t_list) override;
+    using Analysis::analyze;
+    void write(std::string filename) const override;
     int max() const;
     float show(int t) const { return ngp_[t]; }
 
@@ -554,6 +641,8 @@ class Control {
     void find_fast(const std::vector<std::string> &args);
     void add_list(const std::string &name, std::unique_ptr<Trajectory_List> l);
     Trajectory_List *target(const std::string &line);
+    // src/control.cpp:1316-1363
+    void run_analysis(Analysis *analyzer, const std::string &setline);
     // Run fn(file suffix, list) for a "list <name>" target (suffix "") or,
     // for a "bin_list <bins> <list> [persistence]" target, once per bin
     // with the suffix Bin_Static_Analysis::write forms (src/bin_static_

@@ -118,6 +118,30 @@ Coordinate Wave_Vectors::show_mean_wavevector(int index) const
 }
 
 // ---------------------------------------------------------------------------
+// Analysis (src/analysis.h:83-106): the reference's messages for methods a
+// class does not implement
+
+void Analysis::analyze(Trajectory_List *, Trajectory_List *)
+{
+    std::cout << "Error: Trajectory list targets with two lists not implemented for this analysis method.\n";
+}
+
+void Analysis::analyze(Trajectory_List *)
+{
+    std::cout << "Error: Trajectory list targets with one list not implemented for this analysis method.\n";
+}
+
+void Analysis::bin_hook(Trajectory_List *, int, int, int)
+{
+    std::cout << "Error: Trajectory list bins not implemented for this analysis method.\n";
+}
+
+void Analysis::write(std::string) const
+{
+    std::cout << "Error: No standard write method implemented for this analysis method.\n";
+}
+
+// ---------------------------------------------------------------------------
 // Wave_Density (src/wave_density.cpp:46-365)
 
 static std::shared_ptr<amdat_rho> own_rho(amdat_rho *r)
@@ -126,12 +150,12 @@ static std::shared_ptr<amdat_rho> own_rho(amdat_rho *r)
 }
 
 Wave_Density::Wave_Density(System *sys, const Wave_Vectors *wv, bool stf)
-    : system_(sys), wv_(wv), first_(0), last_(wv->show_n_wavenumbers() - 1), sf_(stf)
+    : Analysis(sys), system_(sys), wv_(wv), first_(0), last_(wv->show_n_wavenumbers() - 1), sf_(stf)
 {
 }
 
 Wave_Density::Wave_Density(System *sys, const Wave_Vectors *wv, int inner, int outer)
-    : system_(sys), wv_(wv), first_(inner), last_(outer)
+    : Analysis(sys), system_(sys), wv_(wv), first_(inner), last_(outer)
 {
     if (inner < 0 || outer < inner || outer >= wv->show_n_wavenumbers())
         throw Error("wavenumber range outside the wave vector table", 0);
@@ -147,6 +171,7 @@ void Wave_Density::set(System *sys, const Wave_Vectors *wv, int inner, int outer
     if (inner < 0 || outer < inner || outer >= wv->show_n_wavenumbers())
         throw Error("wavenumber range outside the wave vector table", 0);
     system_ = sys;
+    system_base_ = sys;
     wv_ = wv;
     first_ = inner;
     last_ = outer;
@@ -158,11 +183,11 @@ void Wave_Density::set(System *sys, const Wave_Vectors *wv, int inner, int outer
 void Wave_Density::analyze(Trajectory_List *t_list)
 {
     const int T = system_->show_n_timesteps();
-    n_atoms_.assign(T, 0);
+    if ((int)n_atoms_.size() != T) n_atoms_.assign(T, 0);
     host_.reset();
     amdat_rho *r = nullptr;
     if (!sf_) {
-        for (int t = 0; t < T; t++) n_atoms_[t] = t_list->show_n_trajectories(t);
+        for (int t = 0; t < T; t++) n_atoms_[t] += t_list->show_n_trajectories(t);
         check(amdat_rho_compute(device_context(), system_->traj(), t_list->handle(), wv_->handle(), first_, last_,
                                 wv_->show_length_scale(), nullptr, 0, AMDAT_PREC_FP32C, &r),
               "wave density");
@@ -175,7 +200,7 @@ void Wave_Density::analyze(Trajectory_List *t_list)
             const int t = system_->show_n_exponential_steps() * b;
             if (t >= T) break;
             const std::vector<int> a = t_list->atoms_at(t);
-            n_atoms_[t] = (int)a.size();
+            n_atoms_[t] += (int)a.size();
             idx.insert(idx.end(), a.begin(), a.end());
             off.push_back((int)idx.size());
             conv[t] = c++;
@@ -189,7 +214,16 @@ void Wave_Density::analyze(Trajectory_List *t_list)
                                 wv_->show_length_scale(), nullptr, 0, AMDAT_PREC_FP32C, &r),
               "wave density");
     }
-    rho_ = own_rho(r);
+    if (!rho_) {
+        rho_ = own_rho(r);
+        return;
+    }
+    // a later call adds to the density: a new device object, so copies made
+    // in between keep theirs
+    std::shared_ptr<amdat_rho> add = own_rho(r);
+    amdat_rho *sum = nullptr;
+    check(amdat_rho_sum(device_context(), rho_.get(), add.get(), &sum), "wave density sum");
+    rho_ = own_rho(sum);
 }
 
 int Wave_Density::show_n_atoms_looped() const
@@ -228,7 +262,7 @@ double Wave_Density::show_density_im(int t, int wn, int v) const { return show_d
 // Structure_Factor (src/structure_factor.cpp:55-344, src/analysis_onetime.cpp:22-85)
 
 Structure_Factor::Structure_Factor(System *sys, const Wave_Vectors *wv, int timescheme)
-    : system_(sys), wv_(wv), time_scheme_(timescheme), structure_factor_(wv->show_n_wavenumbers(), 0.0f)
+    : Analysis(sys), system_(sys), wv_(wv), time_scheme_(timescheme), structure_factor_(wv->show_n_wavenumbers(), 0.0f)
 {
 }
 
@@ -285,7 +319,7 @@ void Structure_Factor::analyze(Trajectory_List *t_list, Trajectory_List *t_list2
     structure_factor_run(system_, wv_, frames(), t_list, t_list2, structure_factor_);
 }
 
-void Structure_Factor::write(const std::string &filename) const
+void Structure_Factor::write(std::string filename) const
 {
     std::ofstream out(filename);
     std::cout << "\nWriting to file " << filename << ".";
@@ -338,7 +372,7 @@ static void check_same_range(const Wave_Density *wd1, const Wave_Density *wd2)
 
 Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, Wave_Density *wd1,
                                                                    Wave_Density *wd2, bool parallel)
-    : system_(sys), first_bin_(wd1->show_first()), last_bin_(wd1->show_last())
+    : Analysis(sys), system_(sys), first_bin_(wd1->show_first()), last_bin_(wd1->show_last())
 {
     check_densities(wd1, wd2);
     check_same_range(wd1, wd2);
@@ -347,15 +381,16 @@ Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, 
 }
 
 Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, Wave_Density *wd1,
-                                                                   Wave_Density *wd2, int wn, bool parallel)
-    : Intermediate_Scattering_Function(sys, wd1, wd2, wn, wn, parallel)
+                                                                   Wave_Density *wd2, int wn, bool parallel,
+                                                                   bool physics_correct)
+    : Intermediate_Scattering_Function(sys, wd1, wd2, wn, wn, parallel, physics_correct)
 {
 }
 
 Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, Wave_Density *wd1,
                                                                    Wave_Density *wd2, int wn1, int wn2,
-                                                                   bool parallel)
-    : system_(sys), first_bin_(wn1), last_bin_(wn2)
+                                                                   bool parallel, bool physics_correct)
+    : Analysis(sys), system_(sys), first_bin_(wn1), last_bin_(wn2)
 {
     check_densities(wd1, wd2);
     if (wn1 < wd1->show_first() || wn1 < wd2->show_first())
@@ -366,12 +401,12 @@ Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, 
     if (wd1->show_first() != wd2->show_first())
         throw Error("Wave vector bins for density distributions do not match!", 0);
     amdat_pairs *p = sys->displacement_pairs(parallel);
-    compute(wd1, wd2, p, -1);
+    compute(wd1, wd2, p, -1, !physics_correct && wn1 != wd1->show_first());
 }
 
 Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, int timegapii, Wave_Density *wd1,
                                                                    Wave_Density *wd2, bool parallel)
-    : system_(sys), first_bin_(wd1->show_first()), last_bin_(wd1->show_last())
+    : Analysis(sys), system_(sys), first_bin_(wd1->show_first()), last_bin_(wd1->show_last())
 {
     check_densities(wd1, wd2);
     check_same_range(wd1, wd2);
@@ -383,16 +418,40 @@ Intermediate_Scattering_Function::Intermediate_Scattering_Function(System *sys, 
 // F over the given pairs (amdat_fkt; FP64 sums normalised by the atoms at
 // t0, src/intermediate_scattering_function.cpp:551-566 and 310-314); takes
 // ownership of pairs.  only_timegap >= 0 keeps that row and zeroes the rest.
+// slot_quirk: column i sums the density's slot i (shell first + i) over the
+// vector count of shell first_bin_ + i (amdat_fkt_vectors), as the
+// reference's displacementkernel indexes the densities.
 void Intermediate_Scattering_Function::compute(Wave_Density *wd1, Wave_Density *wd2, amdat_pairs *p,
-                                               int only_timegap)
+                                               int only_timegap, bool slot_quirk)
 {
     System *sys = system_;
     wv_ = wd1->show_wavevectors();
     timetable_ = sys->displacement_times();
     const int ntg = sys->show_n_timegaps(), nsh = last_bin_ - first_bin_ + 1;
     std::vector<double> F((size_t)ntg * nsh), cnt(ntg);
-    int rc = amdat_fkt(device_context(), wd1->device(), wd2->device(), p, first_bin_ - wd1->show_first(),
+    int rc;
+    if (slot_quirk) {
+        const std::vector<int> &off = wv_->offsets();
+        const int first = wd1->show_first();
+        std::vector<int> vf(nsh), vc(nsh);
+        for (int i = 0; i < nsh; i++) {
+            const int slot = first + i, wn = first_bin_ + i;
+            vf[i] = off[slot] - off[first];
+            vc[i] = off[wn + 1] - off[wn];
+            if (vc[i] > off[slot + 1] - off[slot]) {
+                amdat_pairs_free(device_context(), p);
+                throw Error("Intermediate_Scattering_Function: shell " + std::to_string(wn) +
+                                " is read from the density's slot " + std::to_string(i) +
+                                ", which holds fewer vectors (the reference reads past it)",
+                            10);
+            }
+        }
+        rc = amdat_fkt_vectors(device_context(), wd1->device(), wd2->device(), p, nsh, vf.data(), vc.data(),
+                               F.data(), cnt.data());
+    } else {
+        rc = amdat_fkt(device_context(), wd1->device(), wd2->device(), p, first_bin_ - wd1->show_first(),
                        last_bin_ - wd1->show_first(), F.data(), cnt.data());
+    }
     amdat_pairs_free(device_context(), p);
     check(rc, "intermediate scattering function");
     correlation_.assign(ntg, std::vector<float>(nsh, 0.0f));
@@ -402,7 +461,7 @@ void Intermediate_Scattering_Function::compute(Wave_Density *wd1, Wave_Density *
     }
 }
 
-void Intermediate_Scattering_Function::write(const std::string &filename) const
+void Intermediate_Scattering_Function::write(std::string filename) const
 {
     write_correlation(filename, wv_, first_bin_, last_bin_, timetable_, correlation_);
 }
@@ -412,7 +471,7 @@ void Intermediate_Scattering_Function::write(const std::string &filename) const
 
 Incoherent_Scattering_Function::Incoherent_Scattering_Function(System *sys, const Wave_Vectors *wv, int inner,
                                                                int outer, bool fblock)
-    : system_(sys), wv_(wv), first_bin_(inner), last_bin_(outer), fullblock_(fblock)
+    : Analysis(sys), system_(sys), wv_(wv), first_bin_(inner), last_bin_(outer), fullblock_(fblock)
 {
     if (inner < 0 || outer < inner || outer >= wv->show_n_wavenumbers())
         throw Error("wavenumber range outside the wave vector table", 0);
@@ -433,7 +492,7 @@ void Incoherent_Scattering_Function::analyze(Trajectory_List *t_list)
         for (int s = 0; s < nsh; s++) correlation_[t][s] = (float)F[(size_t)t * nsh + s];
 }
 
-void Incoherent_Scattering_Function::write(const std::string &filename) const
+void Incoherent_Scattering_Function::write(std::string filename) const
 {
     write_correlation(filename, wv_, first_bin_, last_bin_, timetable_, correlation_);
 }
@@ -441,7 +500,7 @@ void Incoherent_Scattering_Function::write(const std::string &filename) const
 // ---------------------------------------------------------------------------
 // Mean_Square_Displacement (src/mean_square_displacement.cpp:54-200)
 
-Mean_Square_Displacement::Mean_Square_Displacement(System *sys) : system_(sys)
+Mean_Square_Displacement::Mean_Square_Displacement(System *sys) : Analysis(sys), system_(sys)
 {
     timetable_ = sys->displacement_times();
     msd_.assign(sys->show_n_timegaps(), 0.0f);
@@ -466,7 +525,7 @@ void Mean_Square_Displacement::analyze(Trajectory_List *t_list)
     }
 }
 
-void Mean_Square_Displacement::write(const std::string &filename) const
+void Mean_Square_Displacement::write(std::string filename) const
 {
     std::cout << "\nWriting msd to file " << filename << ".";
     std::ofstream out(filename);
@@ -478,7 +537,7 @@ void Mean_Square_Displacement::write(const std::string &filename) const
 // Non_Gaussian_Parameter (src/non_gaussian_parameter.cpp:96-190)
 
 Non_Gaussian_Parameter::Non_Gaussian_Parameter(System *sys, const Mean_Square_Displacement *msd)
-    : system_(sys), msd_(msd)
+    : Analysis(sys), system_(sys), msd_(msd)
 {
     timetable_ = sys->displacement_times();
     ngp_.assign(sys->show_n_timegaps(), 0.0f);
@@ -496,7 +555,7 @@ void Non_Gaussian_Parameter::analyze(Trajectory_List *t_list)
     }
 }
 
-void Non_Gaussian_Parameter::write(const std::string &filename) const
+void Non_Gaussian_Parameter::write(std::string filename) const
 {
     std::cout << "\nWriting non-Gaussian parameter to file " << filename << ".";
     std::ofstream out(filename);
@@ -534,7 +593,7 @@ void Displacement_Stats::run(System *sys, Trajectory_List *t_list, int n_bins, f
 }
 
 Mean_Square_Displacement_2D::Mean_Square_Displacement_2D(System *sys, const std::string &plane)
-    : system_(sys), plane_(plane)
+    : Analysis(sys), system_(sys), plane_(plane)
 {
     if (plane == "xy") { a0_ = 0; a1_ = 1; }
     else if (plane == "xz") { a0_ = 0; a1_ = 2; }
@@ -552,7 +611,7 @@ void Mean_Square_Displacement_2D::analyze(Trajectory_List *t_list)
         msd_[t] = (float)((d.sum_d2[3 * t + a0_] + d.sum_d2[3 * t + a1_]) / d.count[t]);
 }
 
-void Mean_Square_Displacement_2D::write(const std::string &filename) const
+void Mean_Square_Displacement_2D::write(std::string filename) const
 {
     std::cout << "\nWriting msd to file.";
     std::ofstream out(filename);
@@ -560,7 +619,7 @@ void Mean_Square_Displacement_2D::write(const std::string &filename) const
     for (size_t t = 0; t < msd_.size(); t++) out << timetable_[t] << "\t" << msd_[t] << "\n";
 }
 
-Mean_Displacement::Mean_Displacement(System *sys) : system_(sys)
+Mean_Displacement::Mean_Displacement(System *sys) : Analysis(sys), system_(sys)
 {
     timetable_ = sys->displacement_times();
     md_.assign(sys->show_n_timegaps(), Coordinate());
@@ -575,7 +634,7 @@ void Mean_Displacement::analyze(Trajectory_List *t_list)
                             (float)(d.sum_d[3 * t + 2] / d.count[t]));
 }
 
-void Mean_Displacement::write(const std::string &filename) const
+void Mean_Displacement::write(std::string filename) const
 {
     std::cout << "\nWriting mean displacement to file " << filename << ".";
     std::ofstream out(filename);
@@ -594,7 +653,7 @@ constexpr double kGcPi = 3.14159265;  // src/gaussian_comparison.cpp:14
 
 Gaussian_Comparison::Gaussian_Comparison(System *sys, const Non_Gaussian_Parameter *ngp, const Van_Hove_Self *vhs,
                                          const Mean_Square_Displacement *msd)
-    : system_(sys)
+    : Analysis(sys), system_(sys)
 {
     time_index_ = ngp->max();
     init(vhs, msd);
@@ -602,7 +661,7 @@ Gaussian_Comparison::Gaussian_Comparison(System *sys, const Non_Gaussian_Paramet
 
 Gaussian_Comparison::Gaussian_Comparison(System *sys, int time_in, const Van_Hove_Self *vhs,
                                          const Mean_Square_Displacement *msd)
-    : system_(sys)
+    : Analysis(sys), system_(sys)
 {
     time_index_ = time_in;
     init(vhs, msd);
@@ -657,7 +716,7 @@ void Gaussian_Comparison::init(const Van_Hove_Self *vhs, const Mean_Square_Displ
     }
 }
 
-void Gaussian_Comparison::write(const std::string &filename) const
+void Gaussian_Comparison::write(std::string filename) const
 {
     std::cout << "\nWriting fractions of slow and fast particles and comparison between gaussian and actual self van "
                  "hove to file "
@@ -686,7 +745,7 @@ void Gaussian_Comparison::write(const std::string &filename) const
 // DebyeWaller_Dist (src/debyewaller_dist.cpp:14-141)
 
 Displacement_Distribution::Displacement_Distribution(System *sys, float power, int bins, float maxvalue, float t)
-    : system_(sys), power_(power), n_bins_(bins + 1), time_index_((int)t), max_value_(maxvalue)
+    : Analysis(sys), system_(sys), power_(power), n_bins_(bins + 1), time_index_((int)t), max_value_(maxvalue)
 {
     if (bins < 1 || bins > 16383) throw Error("distribution: number of bins must be 1..16383", 0);
     binsize_ = maxvalue / bins;
@@ -713,7 +772,7 @@ void Displacement_Distribution::analyze(Trajectory_List *t_list)
     for (int b = 0; b < n_bins_; b++) distribution_[b] = (float)((double)counts[b] / c);
 }
 
-void Displacement_Distribution::write(const std::string &filename) const
+void Displacement_Distribution::write(std::string filename) const
 {
     std::ofstream out(filename);
     std::cout << "\nWriting distribution of particle displacement magnitudes to the " << power_ << " power to file "
@@ -731,7 +790,7 @@ DebyeWaller_Dist::DebyeWaller_Dist(System *sys, int bins, float maxvalue, float 
 {
 }
 
-void DebyeWaller_Dist::write(const std::string &filename) const
+void DebyeWaller_Dist::write(std::string filename) const
 {
     std::ofstream out(filename);
     std::cout << "\nWriting distribution of mean-square dispalcements to file " << filename << ".";
@@ -744,7 +803,7 @@ void DebyeWaller_Dist::write(const std::string &filename) const
 }
 
 Van_Hove_Self::Van_Hove_Self(System *sys, int bin_count, float value_max)
-    : system_(sys), n_bins_(bin_count), max_value_(value_max)
+    : Analysis(sys), system_(sys), n_bins_(bin_count), max_value_(value_max)
 {
     if (bin_count < 1 || bin_count > 16384) throw Error("vhs: number of bins must be 1..16384", 0);
     bin_size_ = max_value_ / (float)n_bins_;
@@ -767,7 +826,7 @@ void Van_Hove_Self::analyze(Trajectory_List *t_list)
     }
 }
 
-void Van_Hove_Self::write(const std::string &filename) const
+void Van_Hove_Self::write(std::string filename) const
 {
     std::ofstream out(filename);
     out << "Correlation data created by AMDAT v." << kSemver << "\n";

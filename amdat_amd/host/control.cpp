@@ -372,25 +372,91 @@ void Control::isfs(const std::vector<std::string> &args)
               << " seconds.\n";
 }
 
+// src/control.cpp:1316-1363: run an analysis held as Analysis* on the
+// target of a "list <name>" line (the reference's older set-based targets
+// are not built).
+void Control::run_analysis(Analysis *analyzer, const std::string &setline)
+{
+    auto t = tokenize(setline);
+    if (t.empty()) throw Error("No atom set command found.", 5);
+    if (t[0] != "list")
+        throw Error("target '" + setline + "' is not supported by this build (list <name> only)", 1);
+    if (t.size() != 2) throw Error("Incorrect number of arguments for list.", 0);
+    for (auto &p : lists_)
+        if (p.first == t[1]) {
+            analyzer->analyze(p.second.get());
+            return;
+        }
+    std::cout << "\nTrajectory list '" << t[1] << "' not found.";
+}
+
 void Control::isf(const std::vector<std::string> &args)
 {
     // isf <file> auto <symmetric|asymmetric> <plane> <fullblock> <inner> <outer>
+    // isf <file> manual <symmetric|asymmetric> <plane> <fullblock> <-> <S(k) file>
+    // (src/control.cpp:3107-3192)
     if (args.size() < 6) throw Error("Incorrect number of arguments for isf.", 0);
     const std::string method = args[2], symmetry = args[3], plane = args[4];
     const bool fb = std::atoi(args[5].c_str()) != 0;
-    if (method != "auto")
-        throw Error("isf method " + method + " (interactive wavenumber entry) is not supported by this build", 1);
-    if (args.size() != 8) throw Error("Incorrect number of arguments for isf auto.", 0);
-    const int inner = std::atoi(args[6].c_str()), outer = std::atoi(args[7].c_str());
+    std::cout << "\nDetermining wave vectors.";
+    std::cout.flush();
     Wave_Vectors wv(system_.get(), plane);  // isf always uses the box size
-    Wave_Density wd1(system_.get(), &wv, inner, outer), wd2(system_.get(), &wv, inner, outer);
-    wd1.analyze(target(read_line()));
-    if (symmetry == "symmetric") {
-        wd2.copy_from(wd1);
-    } else if (symmetry == "asymmetric") {
-        wd2.analyze(target(read_line()));
+    Wave_Density wd1(system_.get(), &wv), wd2(system_.get(), &wv);
+    int inner = 0, outer = 0;
+    auto densities = [&]() {
+        std::cout << "\nCalculating fourier transform of density for first set of particles.";
+        std::cout.flush();
+        run_analysis(&wd1, read_line());
+        if (symmetry == "symmetric") {
+            wd2.copy_from(wd1);
+        } else if (symmetry == "asymmetric") {
+            std::cout << "\nCalculating fourier transform of density for second set of particles.";
+            std::cout.flush();
+            const std::string line = read_line();
+            std::cout << "\n" << line;
+            run_analysis(&wd2, line);
+        } else {
+            throw Error("command for structure factor calculation atom set types not understood.", 0);
+        }
+    };
+    if (method == "auto") {
+        if (args.size() != 8) throw Error("Incorrect number of arguments for isf auto.", 0);
+        inner = std::atoi(args[6].c_str());
+        outer = std::atoi(args[7].c_str());
+        wd1.set(system_.get(), &wv, inner, outer);
+        wd2.set(system_.get(), &wv, inner, outer);
+        densities();
+    } else if (method == "manual") {
+        if (args.size() != 8) throw Error("Incorrect number of arguments for isf manual.", 0);
+        densities();
+        std::cout << "\nCalculating structure factor.";
+        std::cout.flush();
+        Intermediate_Scattering_Function struct_fac(system_.get(), 0, &wd1, &wd2);
+        struct_fac.write(args[7]);
+        std::cout << "Please input the lower and upper wavenumber index bounds, separated by spaces.\n";
+        if (!(std::cin >> inner >> outer)) throw Error("isf manual: no wavenumber bounds on the input.", 0);
     } else {
-        throw Error("command for structure factor calculation atom set types not understood.", 0);
+        throw Error("isf method " + method + " not understood.", 0);
+    }
+    std::cout << "\nCalculating intermediate scattering function.";
+    std::cout.flush();
+    if (method == "manual" && inner != wd1.show_first()) {
+        // The reference reads the density's slot wn - inner with shell wn's
+        // vector count (src/intermediate_scattering_function.cpp:556-562);
+        // over densities of every shell that runs past the slots, so the
+        // shells asked for are computed instead.
+        try {
+            Intermediate_Scattering_Function f(system_.get(), &wd1, &wd2, inner, outer, fb);
+            f.write(args[1]);
+            return;
+        } catch (const Error &e) {
+            if (e.severe()) throw;
+            warn(std::string(e.what()) + "; computing shells " + std::to_string(inner) + " to " +
+                 std::to_string(outer) + " as requested");
+        }
+        Intermediate_Scattering_Function f(system_.get(), &wd1, &wd2, inner, outer, fb, true);
+        f.write(args[1]);
+        return;
     }
     Intermediate_Scattering_Function f(system_.get(), &wd1, &wd2, inner, outer, fb);
     f.write(args[1]);

@@ -350,6 +350,23 @@ int amdat_sk(amdat_ctx *ctx, const amdat_rho *rho1, const amdat_rho *rho2, doubl
 int amdat_fkt(amdat_ctx *ctx, const amdat_rho *rho1, const amdat_rho *rho2,
               const amdat_pairs *pairs, int wn1, int wn2, double *F, double *count);
 
+/* F(k,t) over arbitrary vector ranges of two densities: column c sums
+ * the vectors v_first[c] .. v_first[c]+v_count[c]-1 (positions relative to
+ * the densities' first vector) and divides by v_count[c]; otherwise as
+ * amdat_fkt.  Expresses the reference's reading of shells wn1..wn2 by
+ * Intermediate_Scattering_Function(sys, wd1, wd2, wn1, wn2, ...): it takes
+ * the density's slot wn - wn1, counted from the density's first shell,
+ * with the vector count of shell wn (src/intermediate_scattering_function.
+ * cpp:556-562), which only names shell wn when wn1 is the density's first
+ * shell.  F[n_timegaps*n_cols], count[n_timegaps]. */
+int amdat_fkt_vectors(amdat_ctx *ctx, const amdat_rho *rho1, const amdat_rho *rho2, const amdat_pairs *pairs,
+                      int n_cols, const int *v_first, const int *v_count, double *F, double *count);
+
+/* A density that is the sum of two over the same frames and vectors: the
+ * reference's Wave_Density::analyze adds each call's rho_k(t) and atom
+ * counts to what earlier calls left (src/wave_density.cpp:313-336). */
+int amdat_rho_sum(amdat_ctx *ctx, const amdat_rho *a, const amdat_rho *b, amdat_rho **out);
+
 /* Fs(k,t) = sum_pairs sum_atoms mean_{v in s} cos(k.(r(t1)-r(t0)))
  *           / (atom-pair count), with r(t1)-r(t0) rebuilt in FP64 from
  * wrapped coordinates and image counts.

@@ -69,12 +69,20 @@ list big
 list all
 structure_factor sk0.dat symmetric xyz 0
 list all
+isf isf_ab.dat auto asymmetric xyz 0 0 20
+list big
+list all
+isf isf_man.dat manual symmetric xyz 1 0 sk_man.dat
+list all
 """
     (d / "in.amdat").write_text(script)
     env = dict(os.environ, AMDAT_WV3D=str(qdir / "qvector"))
     print("case: amdat_hip drop-in script, N=300 T=7", flush=True)
-    p = subprocess.run([BIN, "-i", "in.amdat"], cwd=str(d), env=env, capture_output=True, text=True, timeout=300)
+    # isf manual reads its wavenumber bounds from standard input
+    p = subprocess.run([BIN, "-i", "in.amdat"], cwd=str(d), env=env, capture_output=True, text=True, timeout=300,
+                       input="3 8\n")
     assert p.returncode == 0, p.stdout[-3000:] + p.stderr[-3000:]
+    (d / "stdout.txt").write_text(p.stdout)
     return d, tr
 
 
@@ -144,3 +152,49 @@ def test_structure_factor_files(run, qvec):
     S0 = orc.sk_f64(r0, r0, np.array([tr.N]), off)
     _, rows = read_rows(d / "sk0.dat", 2)
     assert np.allclose(rows[:, 6], S0, rtol=RTOL)
+
+
+def test_isf_asymmetric_file(run, qvec):
+    """isf auto asymmetric (src/control.cpp:3145-3152): rho of the first
+    list at t0 against rho of the second at t1, normalised by the first
+    list's atoms (src/intermediate_scattering_function.cpp:551-566)."""
+    d, tr = run
+    off, n = ref_shells(qvec, 2, 22)          # wn 0..20
+    L = float(np.float32(7.0))
+    big = np.arange(240)
+    rb = orc.rho_f64_traj(tr, n, L, atoms=big)
+    ra = orc.rho_f64_traj(tr, n, L)
+    tg, t0, t1 = orc.ts_pairs(2, 3)
+    F, _ = orc.fkt_f64(rb, ra, np.full(tr.T, 240), (tg, t0, t1), 5, off)
+    S = orc.sk_f64(rb, rb, np.full(tr.T, 240), off)
+    _, rows = read_rows(d / "isf_ab.dat", 2)
+    assert rows.shape == (5, 22)
+    assert np.allclose(rows[:, 1:], F, rtol=RTOL, atol=1e-6 * np.max(np.abs(S)))
+
+
+def test_isf_manual_files(run, qvec):
+    """isf manual (src/control.cpp:3154-3189): the S(k) file from the time
+    gap 0 correlation over every shell (fullblock pairs of gap 0), then the
+    bounds read from standard input ("3 8"); the reference would read the
+    densities' slots 0..5 with the vector counts of shells 3..8, past those
+    slots, so shells 3..8 are computed, with a warning."""
+    d, tr = run
+    off, n = ref_shells(qvec, 2, 100)
+    L = float(np.float32(7.0))
+    rho = orc.rho_f64_traj(tr, n, L)
+    nat = np.full(tr.T, tr.N)
+    tg, t0, t1 = orc.ts_pairs(2, 3, fullblock=1)
+    sel = tg == 0
+    G, _ = orc.fkt_f64(rho, rho, nat, (tg[sel], t0[sel], t1[sel]), 5, off)
+    lines, rows = read_rows(d / "sk_man.dat", 2)
+    assert lines[0].startswith("Correlation data created by AMDAT v.")
+    assert rows.shape == (5, 100)
+    assert np.allclose(rows[0, 1:], G[0], rtol=RTOL)
+    assert np.all(rows[1:, 1:] == 0)
+    F, _ = orc.fkt_f64(rho, rho, nat, orc.ts_pairs(2, 3, fullblock=1), 5, off)
+    _, rows = read_rows(d / "isf_man.dat", 2)
+    assert rows.shape == (5, 7)
+    assert np.allclose(rows[:, 1:], F[:, 3:9], rtol=RTOL, atol=1e-6)
+    out = (d / "stdout.txt").read_text()
+    assert "Please input the lower and upper wavenumber index bounds" in out
+    assert "Warning" in out
