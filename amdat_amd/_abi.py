@@ -91,7 +91,17 @@ SIGNATURES = {
     "amdat_batch_run": (_I, [_P, _P, _PF, _P, _I, _I, _PI, _I, _P, _PI, _PI, _P, _I, _I,
                              _PD, _PD, _PD, _PD, _PD, _PD, _PD]),
     "amdat_batch_job_info": (_I, [_P, _PI, _PI, _PI, _PI, _PI]),
+    "amdat_batch_job_settings": (_I, [_P, _P]),
+    "amdat_batch_use_settings": (_I, [_P, _P]),
 }
+
+
+class JobSettings(C.Structure):
+    """amdat_job_settings (include/amdat_hip.h)."""
+    _fields_ = [(n, C.c_int) for n in ("sk_R", "sk_plan_mode", "sk_frames_per_item", "sk_frame_groups", "fs_R",
+                                       "fs_pairs_per_item", "fs_atom_parts", "fs_pair_chunks", "atoms_per_tile",
+                                       "table_half_width", "n_items", "grid", "graph", "tuned")] + \
+               [("model_ms", C.c_float), ("tuned_ms", C.c_float)]
 
 RUN_GRAPH = 1
 RUN_TUNE = 2
@@ -660,6 +670,29 @@ class Batch(_Handle):
         rs, rf, ps, ni, g = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(lib().amdat_batch_job_info(self.h, C.byref(rs), C.byref(rf), C.byref(ps), C.byref(ni), C.byref(g)))
         return dict(RS=rs.value, RF=rf.value, paired=bool(ps.value), n_items=ni.value, graph=bool(g.value))
+
+    def job_settings(self):
+        """Every launch setting of the last job (amdat_batch_job_settings)."""
+        st = JobSettings()
+        check(lib().amdat_batch_job_settings(self.h, C.byref(st)))
+        out = {n: getattr(st, n) for n, _ in JobSettings._fields_}
+        out["graph"], out["tuned"] = bool(out["graph"]), bool(out["tuned"])
+        return out
+
+    def use_settings(self, **kw):
+        """Fix launch settings of the next jobs (amdat_batch_use_settings):
+        sk_R, sk_plan_mode, sk_frames_per_item, fs_R, fs_pairs_per_item,
+        fs_atom_parts (omitted: the cost model's).  No keywords: back to the
+        cost model and tuner."""
+        if not kw:
+            check(lib().amdat_batch_use_settings(self.h, None))
+            return
+        st = JobSettings()
+        for k, v in kw.items():
+            if k not in ("sk_R", "sk_plan_mode", "sk_frames_per_item", "fs_R", "fs_pairs_per_item", "fs_atom_parts"):
+                raise TypeError("unknown setting %r" % k)
+            setattr(st, k, int(v))
+        check(lib().amdat_batch_use_settings(self.h, C.byref(st)))
 
 
 def ts_gap_pairs(nexp, nes, frt, limit, time_index):

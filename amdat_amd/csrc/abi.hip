@@ -169,6 +169,7 @@ struct amdat_list {
 
 struct KPlanOwned {
     KPlan kp;
+    int mode = 0;  // plan_layout mode the plan was built in
     DevBuf<uint32_t> kpack;
     DevBuf<int> kdst;
     DevBuf<int4> range;
@@ -233,13 +234,32 @@ struct JobParams {
     }
 };
 
+// Launch settings of a batch job (the autotuner's knobs).
+struct JobChoice {
+    int RS = 0;  // S(k): vectors per thread
+    int SM = 0;  // S(k): plan mode (0 unpaired, 1 paired, 4 grouped)
+    int FG = 0;  // S(k): frames per item (0: from the item target)
+    int RF = 0;  // Fs: vectors per thread
+    int PC = 0;  // Fs: pairs per item (0: all pairs of a time gap)
+    int AF = 1;  // Fs: atom parts per pair chunk
+};
+
+// What the autotuner kept for one job, with its timings of the cost-model
+// settings and of the kept ones (whole job, all trajectories).
+struct TuneResult {
+    JobChoice ch;
+    float model_ms = 0.0f, tuned_ms = 0.0f;
+};
+
 // A planned job: device tables, queue, outputs and (optionally) the
 // instantiated graph of its launch sequence.
 struct JobCache {
     JobParams prm;
-    int RS = 0, RF = 0, MM = 0;
-    int PS = 0;  // the S(k) plan is paired (RS = 8)
-    int n = 0, ntg = 0, nsh_sk = 0, G = 0, max_nsh = 0;
+    JobChoice ch;  // as planned (FG resolved)
+    int MM = 0;
+    bool tuned = false;
+    float model_ms = 0.0f, tuned_ms = 0.0f;
+    int n = 0, ntg = 0, nsh_sk = 0, G = 0, max_nsh = 0, n_pc = 0;
     std::vector<int> out_off;              // Fs offsets per trajectory
     std::vector<double> sk_count, fs_count, msd_count;
     std::vector<char> msd_item;            // [n*ntg]: an MSD item wrote (b, g)
@@ -247,7 +267,7 @@ struct JobCache {
     DevBuf<int4> items;
     DevBuf<int> frames, sk_soff, fs_soff, fs_base, fs_nsh, fs_oo;
     DevBuf<uint32_t> fs_kpack;
-    DevBuf<int> fs_kdst;
+    DevBuf<int> fs_kdst, fs_pcoff, fs_rowoff;
     DevBuf<int4> fs_range;
     DevBuf<double> sk_rows, fs_rows, sk_norm, fs_norm, S, Fs, s2, s4;
     BatchJob job;
@@ -267,7 +287,9 @@ struct amdat_batch {
     std::vector<const amdat_list *> lists;
     DevBuf<int> counter;  // work-queue counter of the persistent launches
     std::unique_ptr<JobCache> job;
-    std::vector<std::pair<JobParams, std::array<int, 3>>> tuned;  // autotuned (RS, RF, PS) per job
+    std::vector<std::pair<JobParams, TuneResult>> tuned;  // autotuned settings per job
+    bool forced = false;  // amdat_batch_use_settings
+    JobChoice force;      // zero fields: the cost model's
 };
 
 namespace {
@@ -373,6 +395,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
 
     auto P = std::make_unique<KPlanOwned>();
     KPlan &kp = P->kp;
+    P->mode = mode;
     if (K == 0) {
         // empty shells (the 2D tables have some): no chunks, every sum is 0
         kp.R = R;
@@ -1689,8 +1712,9 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
         return set_error(AMDAT_ERANGE, "amdat_sk_loss_grad: shell range outside the table");
     if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_sk_loss_grad: length scale must be > 0");
     HIP_TRY(hipSetDevice(ctx->device));
+    // the plan the single-trajectory S(k) path chooses (grouped slots allowed)
     KPlanOwned *plan = nullptr;
-    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, 9.0, false);
+    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, 9.0, true);
     if (rc) return rc;
     const KPlan &kp = plan->kp;
     const int K = kp.K, nsh = last_shell - first_shell + 1, v0 = sh->off[first_shell];
@@ -1847,8 +1871,9 @@ int amdat_batch_sk(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
         std::iota(fr.begin(), fr.end(), 0);
     }
     HIP_TRY(hipSetDevice(ctx->device));
+    // the plan the single-trajectory S(k) path chooses (grouped slots allowed)
     KPlanOwned *plan = nullptr;
-    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, 9.0, false);
+    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, 9.0, true);
     if (rc) return rc;
     const KPlan &kp = plan->kp;
     const int K = kp.K, nsh = last_shell - first_shell + 1, F = (int)fr.size();
@@ -1997,8 +2022,8 @@ int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
     HIP_TRY(d_out.alloc((size_t)out_off[n]));
     HIP_TRY(launch_fs_batch(d_bt.p, d_items.p, (int)items.size(), batch->counter.p, pairs->d_t0.p, pairs->d_t1.p,
                             pairs->d_group_off.p, pairs->d_order.p, cat, st));
-    HIP_TRY(launch_fs_batch_reduce(d_bt.p, n, d_soff.p, d_base.p, d_nsh.p, max_nsh, ntg, d_norm.p, d_oo.p, d_out.p,
-                                   st));
+    HIP_TRY(launch_fs_batch_reduce(d_bt.p, n, d_soff.p, d_base.p, d_nsh.p, max_nsh, ntg, nullptr, d_norm.p, d_oo.p,
+                                   d_out.p, st));
     HIP_TRY(hipMemcpyAsync(Fs, d_out.p, sizeof(double) * out_off[n], hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
@@ -2058,22 +2083,49 @@ hipError_t enqueue_job_sequence(JobCache &jc, hipStream_t st)
                                    st);
     if (e == hipSuccess && jc.max_nsh > 0)
         e = launch_fs_batch_reduce(jc.fs_bt.p, jc.n, jc.fs_soff.p, jc.fs_base.p, jc.fs_nsh.p, jc.max_nsh, jc.ntg,
-                                   jc.fs_norm.p, jc.fs_oo.p, jc.Fs.p, st);
+                                   jc.fs_rowoff.p, jc.fs_norm.p, jc.fs_oo.p, jc.Fs.p, st);
     return e;
 }
 
+// Frames per S(k) item when the settings leave it open: enough items
+// (trajectories x chunks x frame groups) to balance the queue, about
+// AMDAT_BATCH_ITEMS (default 16384).
+int default_frames_per_item(int n_traj, int n_chunks, int F)
+{
+    long target = 16384;
+    if (const char *e = std::getenv("AMDAT_BATCH_ITEMS")) target = std::max(1L, std::atol(e));
+    const long units = std::max(1L, (long)n_traj * n_chunks);
+    const int G = (int)std::max(1L, std::min((long)F, (target + units - 1) / units));
+    return (F + G - 1) / G;
+}
+
+// Pair chunks of at most PC pairs (0: whole gaps) over the pairs' reference
+// order: offsets into order[] per chunk, and per gap its first chunk.
+void pair_chunks(const amdat_pairs *pairs, int PC, std::vector<int> &pc_off, std::vector<int> &gap_pc)
+{
+    const int ntg = pairs->ntg;
+    pc_off.assign(1, 0);
+    gap_pc.assign(ntg + 1, 0);
+    for (int g = 0; g < ntg; g++) {
+        gap_pc[g] = (int)pc_off.size() - 1;
+        const int b = pairs->group_off[g], e = pairs->group_off[g + 1];
+        const int step = PC > 0 ? PC : std::max(1, e - b);
+        for (int q = b; q < e; q += step) pc_off.push_back(std::min(e, q + step));
+    }
+    gap_pc[ntg] = (int)pc_off.size() - 1;
+}
+
 // Plan a job for the trajectories with use[b] != 0 (others get no items)
-// with R = RS for S(k) items (paired plan when PS) and RF for Fs items.
-int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std::vector<char> &use, int RS, int RF,
-             int PS, JobCache &jc)
+// with the launch settings ch.
+int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std::vector<char> &use,
+             const JobChoice &ch, JobCache &jc)
 {
     const int n = batch->n;
     hipStream_t st = stream_of(ctx);
     jc.prm = prm;
     jc.n = n;
-    jc.RS = RS;
-    jc.RF = RF;
-    jc.PS = PS;
+    jc.ch = ch;
+    jc.ch.AF = std::max(1, ch.AF);
     jc.ntg = prm.pairs ? prm.pairs->ntg : 0;
     std::vector<int> nl(n);
     for (int b = 0; b < n; b++) nl[b] = list_n(batch->trajs[b], batch->lists[b]);
@@ -2085,18 +2137,15 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
     if (prm.sk_sh) {
         amdat_shells *sh = const_cast<amdat_shells *>(prm.sk_sh);
         KPlanOwned *plan = nullptr;
-        int rc = build_plan(ctx, sh, prm.sk_first, prm.sk_last, &plan, RS, 9.0, true, PS);
+        int rc = build_plan(ctx, sh, prm.sk_first, prm.sk_last, &plan, ch.RS, 9.0, true, ch.SM);
         if (rc) return rc;
         skp = plan->kp;
         M = std::max(M, shell_range_M(sh, prm.sk_first, prm.sk_last));
         const int F = (int)prm.frames.size(), K = skp.K;
         jc.nsh_sk = prm.sk_last - prm.sk_first + 1;
-        long target = 16384;
-        if (const char *e = std::getenv("AMDAT_BATCH_ITEMS")) target = std::max(1L, std::atol(e));
-        const long units = std::max(1L, (long)n * skp.n_chunks);
-        int G = (int)std::max(1L, std::min((long)F, (target + units - 1) / units));
-        const int FG = (F + G - 1) / G;
-        G = (F + FG - 1) / FG;
+        const int FG = ch.FG > 0 ? std::min(ch.FG, F) : default_frames_per_item(n, skp.n_chunks, F);
+        const int G = (F + FG - 1) / FG;
+        jc.ch.FG = FG;
         jc.G = G;
         HIP_TRY(jc.sk_rows.alloc((size_t)n * G * K));
         std::vector<BatchTraj> bt(n);
@@ -2134,13 +2183,13 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
         std::vector<KPlanOwned *> plans(n);
         std::vector<int> chunk_base(n + 1, 0), row_off(n + 1, 0);
         for (int b = 0; b < n; b++) {
-            int rc = build_plan(ctx, sh, prm.fs_first[b], prm.fs_last[b], &plans[b], RF);
+            int rc = build_plan(ctx, sh, prm.fs_first[b], prm.fs_last[b], &plans[b], ch.RF);
             if (rc) return rc;
             M = std::max(M, shell_range_M(sh, prm.fs_first[b], prm.fs_last[b]));
             chunk_base[b + 1] = chunk_base[b] + plans[b]->kp.n_chunks;
         }
-        fsp.R = RF;
-        fsp.KB = kPhaseBlock * RF;
+        fsp.R = ch.RF;
+        fsp.KB = kPhaseBlock * ch.RF;
         fsp.n_chunks = chunk_base[n];
         HIP_TRY(jc.fs_kpack.alloc((size_t)fsp.n_chunks * fsp.KB));
         HIP_TRY(jc.fs_kdst.alloc((size_t)fsp.n_chunks * fsp.KB));
@@ -2157,11 +2206,19 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
         fsp.kpack = jc.fs_kpack.p;
         fsp.kdst = jc.fs_kdst.p;
         fsp.range = jc.fs_range.p;
+        // rows: pair chunk x atom part, a gap's rows consecutive
+        const int AF = jc.ch.AF;
+        std::vector<int> pc_off, gap_pc;
+        pair_chunks(pairs, ch.PC, pc_off, gap_pc);
+        jc.n_pc = (int)pc_off.size() - 1;
+        std::vector<int> gap_rows(ntg + 1);
+        for (int g = 0; g <= ntg; g++) gap_rows[g] = gap_pc[g] * AF;
+        const int n_rows = jc.n_pc * AF;
         std::vector<int> soff, base(n), nsh(n);
         jc.out_off.assign(n + 1, 0);
         std::vector<double> norm((size_t)n * ntg, 0.0);
         for (int b = 0; b < n; b++) {
-            row_off[b + 1] = row_off[b] + ntg * plans[b]->kp.K;
+            row_off[b + 1] = row_off[b] + n_rows * plans[b]->kp.K;
             nsh[b] = prm.fs_last[b] - prm.fs_first[b] + 1;
             jc.max_nsh = std::max(jc.max_nsh, nsh[b]);
             base[b] = (int)soff.size();
@@ -2176,13 +2233,18 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
             bt[b] = batch_entry(batch, b, 1.0 / (double)prm.ell[b], plans[b]->kp.K, jc.fs_rows.p + row_off[b]);
             if (!use[b] || nl[b] == 0) continue;
             for (int c = 0; c < plans[b]->kp.n_chunks; c++)
-                for (int g = 0; g < ntg; g++) {
-                    const int np = pairs->group_off[g + 1] - pairs->group_off[g];
-                    if (np == 0) continue;
-                    items.push_back(make_int4(b, chunk_base[b] + c, g, 1));
-                    cost.push_back((double)nl[b] * np * fsp.KB);
-                }
+                for (int q = 0; q < jc.n_pc; q++)
+                    for (int a = 0; a < AF; a++) {
+                        const int np = pc_off[q + 1] - pc_off[q];
+                        const long na = ((long)(a + 1) * nl[b]) / AF - ((long)a * nl[b]) / AF;
+                        items.push_back(make_int4(b, chunk_base[b] + c, q * AF + a, 1));
+                        cost.push_back((double)na * np * fsp.KB);
+                    }
         }
+        HIP_TRY(jc.fs_pcoff.upload(pc_off.data(), pc_off.size(), st));
+        HIP_TRY(jc.fs_rowoff.upload(gap_rows.data(), gap_rows.size(), st));
+        jc.job.fs_pc = jc.fs_pcoff.p;
+        jc.job.fs_split = AF;
         HIP_TRY(jc.fs_bt.upload(bt.data(), bt.size(), st));
         HIP_TRY(jc.fs_soff.upload(soff.data(), soff.size(), st));
         HIP_TRY(jc.fs_base.upload(base.data(), base.size(), st));
@@ -2234,7 +2296,7 @@ int plan_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const std
     jc.job.items = jc.items.p;
     jc.job.n_items = (int)items.size();
     jc.job.counter = batch->counter.p;
-    HIP_TRY(prepare_job(RS, RF, jc.MM, PS, std::max(1, jc.job.n_items), &jc.L));
+    HIP_TRY(prepare_job(jc.ch.RS, jc.ch.RF, jc.MM, jc.ch.SM != 0, std::max(1, jc.job.n_items), &jc.L));
     return AMDAT_OK;
 }
 
@@ -2257,11 +2319,31 @@ int time_job(amdat_ctx *ctx, JobCache &jc, float *ms)
     return AMDAT_OK;
 }
 
-// Autotuner: time R in {2, 4, 8} and R = 8 paired for the S(k) items and,
-// separately, R in {2, 4, 8} for the Fs items, on a sample of about an
-// eighth of the trajectories (every k-th by size), and keep the fastest of
-// each.
-int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, int *RS, int *RF, int *PS)
+// Bytes of the Fs rows of a job with pair chunks of PC pairs and AF atom
+// parts (FP64, every trajectory).
+double fs_rows_bytes(const amdat_batch *batch, const JobParams &prm, int PC, int AF)
+{
+    if (!prm.fs_sh || !prm.pairs) return 0.0;
+    std::vector<int> pc_off, gap_pc;
+    pair_chunks(prm.pairs, PC, pc_off, gap_pc);
+    double k = 0.0;
+    for (int b = 0; b < batch->n; b++) k += prm.fs_sh->off[prm.fs_last[b] + 1] - prm.fs_sh->off[prm.fs_first[b]];
+    return 8.0 * k * (double)(pc_off.size() - 1) * AF;
+}
+
+// Autotuner.  Starts from the cost model's settings `model` and keeps, for
+// the S(k) items and the Fs items separately, the fastest of:
+//  1. vectors per thread and plan mode: S(k) R = 2, 4, 8 unpaired, R = 4, 8
+//     paired and grouped; Fs R = 2, 4, 8.  Timed on about an eighth of the
+//     trajectories (every k-th by size): per-update speed.
+//  2. granularity, timed on every trajectory (the queue's balance depends
+//     on the whole job): S(k) frames per item x1/2, x1, x2; Fs pairs per
+//     item (whole gaps, 2, 1), then atom parts per pair chunk (1, 2, 4).
+// Each part is timed alone (S(k)-only or Fs-only jobs).  Finally the whole
+// job (with MSD) is timed with the model's and the tuned settings and the
+// faster is kept, so the tuned launch is never slower than the model's as
+// measured here.  Every timing is the second of two runs.
+int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const JobChoice &model, TuneResult *res)
 {
     const int n = batch->n;
     std::vector<int> idx(n);
@@ -2269,36 +2351,106 @@ int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, int *
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
         return list_n(batch->trajs[a], batch->lists[a]) > list_n(batch->trajs[b], batch->lists[b]);
     });
-    std::vector<char> use(n, 0);
+    std::vector<char> sample(n, 0), all(n, 1);
     const int step = std::max(1, n / 8);
-    for (int i = 0; i < n; i += step) use[idx[i]] = 1;
-    static const int cand[4][2] = {{2, 0}, {4, 0}, {8, 0}, {8, 1}};
-    for (int part = 0; part < 2; part++) {
+    for (int i = 0; i < n; i += step) sample[idx[i]] = 1;
+    auto time_of = [&](const JobParams &p, const std::vector<char> &use, const JobChoice &c, float *ms,
+                       int *fg_out) -> int {
+        JobCache jc;
+        int rc = plan_job(ctx, batch, p, use, c, jc);
+        if (rc) return rc;
+        if (fg_out) *fg_out = jc.ch.FG;
+        return time_job(ctx, jc, ms);
+    };
+    JobChoice best = model;
+    const double max_rows = 8.0 * (1ull << 30);  // Fs rows of a candidate: at most 8 GiB
+    if (prm.sk_sh) {
         JobParams p = prm;
+        p.fs_sh = nullptr;
         p.msd = false;
-        if (part == 0) { p.fs_sh = nullptr; if (!p.sk_sh) continue; }
-        else { p.sk_sh = nullptr; if (!p.fs_sh) continue; }
-        float best = 1e30f;
+        static const int cand[7][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {4, 4}, {8, 4}};
+        float t_best = 1e30f;
         for (const auto &cd : cand) {
-            if (part == 1 && cd[1]) continue;
-            const int R = cd[0];
-            JobCache jc;
-            int rc = plan_job(ctx, batch, p, use, part == 0 ? R : *RS, part == 0 ? *RF : R, part == 0 ? cd[1] : *PS,
-                              jc);
-            if (rc) return rc;
+            JobChoice c = best;
+            c.RS = cd[0];
+            c.SM = cd[1];
+            c.FG = 0;
             float ms = 0.0f;
-            rc = time_job(ctx, jc, &ms);
+            int rc = time_of(p, sample, c, &ms, nullptr);
             if (rc) return rc;
-            if (ms < best) {
-                best = ms;
-                if (part == 0) {
-                    *RS = R;
-                    *PS = cd[1];
-                } else {
-                    *RF = R;
-                }
-            }
+            if (ms < t_best) { t_best = ms; best.RS = c.RS; best.SM = c.SM; }
         }
+        // frames per item around the default for the chosen plan
+        JobChoice c = best;
+        c.FG = 0;
+        int fg0 = 0;
+        float ms = 0.0f;
+        int rc = time_of(p, all, c, &ms, &fg0);
+        if (rc) return rc;
+        t_best = ms;
+        best.FG = fg0;
+        for (int fg : {fg0 / 2, fg0 * 2}) {
+            if (fg < 1 || fg == fg0 || fg > (int)p.frames.size()) continue;
+            c.FG = fg;
+            rc = time_of(p, all, c, &ms, nullptr);
+            if (rc) return rc;
+            if (ms < t_best) { t_best = ms; best.FG = fg; }
+        }
+    }
+    if (prm.fs_sh) {
+        JobParams p = prm;
+        p.sk_sh = nullptr;
+        p.msd = false;
+        float t_best = 1e30f;
+        for (int R : {2, 4, 8}) {
+            JobChoice c = best;
+            c.RF = R;
+            c.PC = 0;
+            c.AF = 1;
+            float ms = 0.0f;
+            int rc = time_of(p, sample, c, &ms, nullptr);
+            if (rc) return rc;
+            if (ms < t_best) { t_best = ms; best.RF = R; }
+        }
+        best.PC = 0;
+        best.AF = 1;
+        int max_np = 0;
+        for (int g = 0; g < prm.pairs->ntg; g++)
+            max_np = std::max(max_np, prm.pairs->group_off[g + 1] - prm.pairs->group_off[g]);
+        t_best = 1e30f;
+        for (int pc : {0, 2, 1}) {
+            if (pc > 0 && pc >= max_np) continue;
+            if (fs_rows_bytes(batch, p, pc, 1) > max_rows) continue;
+            JobChoice c = best;
+            c.PC = pc;
+            float ms = 0.0f;
+            int rc = time_of(p, all, c, &ms, nullptr);
+            if (rc) return rc;
+            if (ms < t_best) { t_best = ms; best.PC = pc; }
+        }
+        for (int af : {2, 4}) {
+            if (fs_rows_bytes(batch, p, best.PC, af) > max_rows) continue;
+            JobChoice c = best;
+            c.AF = af;
+            float ms = 0.0f;
+            int rc = time_of(p, all, c, &ms, nullptr);
+            if (rc) return rc;
+            if (ms < t_best) { t_best = ms; best.AF = af; }
+        }
+    }
+    // the whole job: model against tuned
+    float t_model = 0.0f, t_tuned = 0.0f;
+    int rc = time_of(prm, all, model, &t_model, nullptr);
+    if (rc) return rc;
+    rc = time_of(prm, all, best, &t_tuned, nullptr);
+    if (rc) return rc;
+    res->model_ms = t_model;
+    if (t_tuned <= t_model) {
+        res->ch = best;
+        res->tuned_ms = t_tuned;
+    } else {
+        res->ch = model;
+        res->tuned_ms = t_model;
     }
     return AMDAT_OK;
 }
@@ -2367,37 +2519,50 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
     HIP_TRY(hipSetDevice(ctx->device));
     const bool use_graph = (flags & AMDAT_RUN_GRAPH) != 0;
     hipStream_t user = stream_of(ctx);
-    if (!batch->job || !(batch->job->prm == prm) || (use_graph && !batch->job->exec)) {
+    // settings from the tuner only when asked for (and not fixed by the caller)
+    const bool tune = (flags & AMDAT_RUN_TUNE) != 0 && !batch->forced;
+    if (!batch->job || !(batch->job->prm == prm) || (use_graph && !batch->job->exec) || batch->job->tuned != tune) {
         batch->job.reset();
-        // vectors per thread: autotuned, or from the cost model
-        int RS = 0, RF = 0, PS = 0;
+        // launch settings: from the cost model, or autotuned from there
+        JobChoice model;
         if (sk_shells) {
-            const int K = sk_shells->off[sk_last + 1] - sk_shells->off[sk_first];
-            const int M = shell_range_M(sk_shells, sk_first, sk_last);
-            RS = choose_R(std::max(K, 1), M);
-            if (RS == 8 && K > 0 && plan_table_width(M) > 0) {
-                const int *nv = sk_shells->nvec.data() + 3 * (size_t)sk_shells->off[sk_first];
-                const int MMs = plan_table_width(M);
-                PS = layout_cost(plan_layout(nv, K, 8, 1, MMs), 9.0) < layout_cost(plan_layout(nv, K, 8, 0, MMs), 9.0);
-            }
+            // the plan the single-trajectory S(k) path would choose (cost model
+            // over R and plan mode); kept in the table's plan cache
+            KPlanOwned *plan = nullptr;
+            int rc = build_plan(ctx, const_cast<amdat_shells *>(sk_shells), sk_first, sk_last, &plan, 0, 9.0, true);
+            if (rc) return rc;
+            model.RS = plan->kp.R;
+            model.SM = plan->mode;
+            if (model.RS == 2 && model.SM != 0) model.SM = 0;  // no grouped R = 2 job kernel
         }
-        if (fs_shells) RF = choose_R(std::max(Kfs, 1), Mfs, 6.0);
-        if (!RS) RS = RF ? RF : 2;
-        if (!RF) RF = RS;
-        if (flags & AMDAT_RUN_TUNE) {
+        if (fs_shells) model.RF = choose_R(std::max(Kfs, 1), Mfs, 6.0);
+        if (!model.RS) model.RS = model.RF ? model.RF : 2;
+        if (!model.RF) model.RF = model.RS;
+        TuneResult tr{model, 0.0f, 0.0f};
+        if (batch->forced) {
+            const JobChoice &f = batch->force;
+            if (f.RS) { tr.ch.RS = f.RS; tr.ch.SM = f.SM; }
+            if (f.FG) tr.ch.FG = f.FG;
+            if (f.RF) tr.ch.RF = f.RF;
+            tr.ch.PC = f.PC;
+            tr.ch.AF = std::max(1, f.AF);
+        } else if (tune) {
             bool found = false;
             for (auto &t : batch->tuned)
-                if (t.first == prm) { RS = t.second[0]; RF = t.second[1]; PS = t.second[2]; found = true; }
+                if (t.first == prm) { tr = t.second; found = true; }
             if (!found) {
-                int rc = autotune_job(ctx, batch, prm, &RS, &RF, &PS);
+                int rc = autotune_job(ctx, batch, prm, model, &tr);
                 if (rc) return rc;
-                batch->tuned.push_back({prm, {RS, RF, PS}});
+                batch->tuned.push_back({prm, tr});
             }
         }
         auto jc = std::make_unique<JobCache>();
         std::vector<char> use(n, 1);
-        int rc = plan_job(ctx, batch, prm, use, RS, RF, PS, *jc);
+        int rc = plan_job(ctx, batch, prm, use, tr.ch, *jc);
         if (rc) return rc;
+        jc->tuned = tune;
+        jc->model_ms = tr.model_ms;
+        jc->tuned_ms = tr.tuned_ms;
         HIP_TRY(hipStreamSynchronize(user));
         if (use_graph) {
             // capture on the context's own stream (the caller's may be the null stream)
@@ -2452,11 +2617,67 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
 int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *paired, int *n_items, int *graph)
 {
     if (!batch || !batch->job) return set_error(AMDAT_EINVAL, "amdat_batch_job_info: no job has run");
-    if (RS) *RS = batch->job->RS;
-    if (RF) *RF = batch->job->RF;
-    if (paired) *paired = batch->job->PS;
+    if (RS) *RS = batch->job->ch.RS;
+    if (RF) *RF = batch->job->ch.RF;
+    if (paired) *paired = batch->job->ch.SM != 0;
     if (n_items) *n_items = batch->job->job.n_items;
     if (graph) *graph = batch->job->exec != nullptr;
+    return AMDAT_OK;
+}
+
+int amdat_batch_use_settings(amdat_batch *batch, const amdat_job_settings *s)
+{
+    clear_error();
+    if (!batch) return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: null batch");
+    if (!s) {
+        batch->forced = false;
+        batch->force = JobChoice{};
+        batch->job.reset();
+        return AMDAT_OK;
+    }
+    auto r_ok = [](int R) { return R == 0 || R == 2 || R == 4 || R == 8; };
+    if (!r_ok(s->sk_R) || !r_ok(s->fs_R))
+        return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: vectors per thread must be 2, 4 or 8");
+    if (s->sk_plan_mode != 0 && s->sk_plan_mode != 1 && s->sk_plan_mode != 4)
+        return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: S(k) plan mode must be 0, 1 or 4");
+    if (s->sk_plan_mode != 0 && s->sk_R != 4 && s->sk_R != 8)
+        return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: paired and grouped S(k) plans need R = 4 or 8");
+    if (s->sk_frames_per_item < 0 || s->fs_pairs_per_item < 0 || s->fs_atom_parts < 0 || s->fs_atom_parts > 64)
+        return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: frames, pairs >= 0 and atom parts in 0..64");
+    JobChoice f;
+    f.RS = s->sk_R;
+    f.SM = s->sk_R ? s->sk_plan_mode : 0;
+    f.FG = s->sk_frames_per_item;
+    f.RF = s->fs_R;
+    f.PC = s->fs_pairs_per_item;
+    f.AF = std::max(1, s->fs_atom_parts);
+    batch->forced = true;
+    batch->force = f;
+    batch->job.reset();
+    return AMDAT_OK;
+}
+
+int amdat_batch_job_settings(const amdat_batch *batch, amdat_job_settings *out)
+{
+    if (!batch || !batch->job) return set_error(AMDAT_EINVAL, "amdat_batch_job_settings: no job has run");
+    if (!out) return set_error(AMDAT_EINVAL, "amdat_batch_job_settings: null argument");
+    const JobCache &jc = *batch->job;
+    out->sk_R = jc.prm.sk_sh ? jc.ch.RS : 0;
+    out->sk_plan_mode = jc.prm.sk_sh ? jc.ch.SM : 0;
+    out->sk_frames_per_item = jc.prm.sk_sh ? jc.ch.FG : 0;
+    out->sk_frame_groups = jc.prm.sk_sh ? jc.G : 0;
+    out->fs_R = jc.prm.fs_sh ? jc.ch.RF : 0;
+    out->fs_pairs_per_item = jc.prm.fs_sh ? jc.ch.PC : 0;
+    out->fs_atom_parts = jc.prm.fs_sh ? jc.ch.AF : 0;
+    out->fs_pair_chunks = jc.prm.fs_sh ? jc.n_pc : 0;
+    out->atoms_per_tile = table_atoms_per_tile(jc.MM);
+    out->table_half_width = jc.MM;
+    out->n_items = jc.job.n_items;
+    out->grid = jc.L.grid;
+    out->graph = jc.exec != nullptr;
+    out->tuned = jc.tuned;
+    out->model_ms = jc.model_ms;
+    out->tuned_ms = jc.tuned_ms;
     return AMDAT_OK;
 }
 

@@ -2,6 +2,7 @@
 // S(k), Fs and MSD/NGP work items of a batch, drawn from one device queue.
 #include <algorithm>
 
+#include "batch_items.h"
 #include "kernels.h"
 #include "phase_device.h"
 
@@ -9,120 +10,8 @@ namespace amdat {
 
 namespace {
 
-// S(k) item (trajectory b, chunk, frame group g): for each frame of the
-// group, rho over all of the trajectory's atoms, and |rho_v|^2 added into
-// row [g][v] of t.out (FP64; each slot is owned by one thread of one item,
-// so the sum order is fixed).  All trajectories share one k-plan: the same
-// integer vectors, k = 2 pi n / l_b with each trajectory's own l_b.
-template <int R, int MM, bool GROUPED = false>
-__device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, const int *__restrict__ frames,
-                                        int n_frames, int group_size, const KPlan &kp, float2 *tab, float2 *anc,
-                                        uint32_t *sU, int *sValid)
-{
-    const int f_begin = g * group_size, f_end = min(n_frames, f_begin + group_size);
-    const int *kd = kp.kdst + (size_t)chunk * kp.KB;
-    double *row = t.out + (size_t)g * kp.K;
-    for (int f = f_begin; f < f_end; ++f) {
-        float hr[R], lr[R], hi[R], li[R];
-        rho_block<R, MM, GROUPED>(t.x, t.y, t.z, t.list, (size_t)frames[f] * (size_t)t.N, 0, t.n_list, t.inv_ell,
-                         kp.kpack + (size_t)chunk * kp.KB, kp.range[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int d = kd[r * kPhaseBlock + threadIdx.x];
-            if (d >= 0) {
-                const double re = (double)hr[r] + (double)lr[r];
-                const double im = (double)hi[r] + (double)li[r];
-                double *slot = row + (d & ~(1 << 30));
-                const double prev = f == f_begin ? 0.0 : *slot;
-                *slot = prev + (re * re + im * im);
-            }
-        }
-    }
-}
-
-// Fs item (trajectory b, global chunk of the concatenated per-trajectory
-// plans, time gap g): the gap's pairs in reference order, each pair's sum
-// over atoms of cos(k.dr) added into row [g][v] of t.out (FP64, fixed order).
-template <int R, int MM>
-__device__ __forceinline__ void fs_item(const BatchTraj &t, int chunk, int g, const int *__restrict__ T0,
-                                        const int *__restrict__ T1, const int *__restrict__ group_off,
-                                        const int *__restrict__ order, const KPlan &kp, float2 *tab, float2 *anc,
-                                        uint32_t *sU, int *sValid)
-{
-    const int *kd = kp.kdst + (size_t)chunk * kp.KB;
-    double *row = t.out + (size_t)g * t.K;
-    const int p_begin = group_off[g], p_end = group_off[g + 1];
-    for (int pq = p_begin; pq < p_end; ++pq) {
-        const int p = order[pq];
-        const int t0 = T0[p], t1 = T1[p];
-        float hs[R], ls[R];
-        fs_block<R, MM>(t.x, t.y, t.z, t.ix, t.iy, t.iz, t.box + 3 * t0, t.box + 3 * t1, t.list, (size_t)t0 * t.N,
-                        (size_t)t1 * t.N, 0, t.n_list, t.inv_ell, kp.kpack + (size_t)chunk * kp.KB,
-                        kp.range[chunk], tab, anc, sU, sValid, hs, ls);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int d = kd[r * kPhaseBlock + threadIdx.x];
-            if (d >= 0) {
-                double *slot = row + (d & ~(1 << 30));
-                const double prev = pq == p_begin ? 0.0 : *slot;
-                *slot = prev + ((double)hs[r] + (double)ls[r]);
-            }
-        }
-    }
-}
-
-// Sum over a wavefront by lane exchanges, then over the block's waves in
-// wave order (deterministic); red holds kWaves doubles.  Every thread gets
-// the result.
-__device__ __forceinline__ double block_sum_waves(double v, double *red)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();  // red is free
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < kWaves; ++i) s += red[i];
-    return s;
-}
-
-// MSD item (trajectory b, time gap g): FP64 sums of |dr|^2 and |dr|^4 over
-// the gap's pairs (reference order) and the trajectory's atoms, written to
-// s2/s4[b*ntg + g].
-__device__ __forceinline__ void msd_item(const BatchTraj &t, int b, int g, int ntg, const int *__restrict__ T0,
-                                         const int *__restrict__ T1, const int *__restrict__ group_off,
-                                         const int *__restrict__ order, double *s2, double *s4, double *red)
-{
-    double acc2 = 0.0, acc4 = 0.0;
-    for (int pq = group_off[g]; pq < group_off[g + 1]; ++pq) {
-        const int p = order[pq];
-        const int t0 = T0[p], t1 = T1[p];
-        const size_t o0 = (size_t)t0 * (size_t)t.N, o1 = (size_t)t1 * (size_t)t.N;
-        const double L0x = t.box[3 * t0], L0y = t.box[3 * t0 + 1], L0z = t.box[3 * t0 + 2];
-        const double L1x = t.box[3 * t1], L1y = t.box[3 * t1 + 1], L1z = t.box[3 * t1 + 2];
-        double a2 = 0.0, a4 = 0.0;
-        for (int a = threadIdx.x; a < t.n_list; a += kPhaseBlock) {
-            const int j = t.list ? t.list[a] : a;
-            const double dx = ((double)t.x[o1 + j] - (double)t.x[o0 + j]) + ((double)t.ix[o1 + j] * L1x - (double)t.ix[o0 + j] * L0x);
-            const double dy = ((double)t.y[o1 + j] - (double)t.y[o0 + j]) + ((double)t.iy[o1 + j] * L1y - (double)t.iy[o0 + j] * L0y);
-            const double dz = ((double)t.z[o1 + j] - (double)t.z[o0 + j]) + ((double)t.iz[o1 + j] * L1z - (double)t.iz[o0 + j] * L0z);
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            a2 += r2;
-            a4 += r2 * r2;
-        }
-        acc2 += block_sum_waves(a2, red);
-        acc4 += block_sum_waves(a4, red);
-    }
-    if (threadIdx.x == 0) {
-        s2[(size_t)b * ntg + g] = acc2;
-        s4[(size_t)b * ntg + g] = acc4;
-    }
-}
-
 // Batched S(k), persistent: items (trajectory, chunk, frame group, -).
-template <int R, int MM>
+template <int R, int MM, bool GROUPED>
 __global__ __launch_bounds__(kPhaseBlock, 4) void sk_batch_kernel(const BatchTraj *__restrict__ trajs,
                                                                  const int4 *__restrict__ items, int n_items,
                                                                  int *counter, const int *__restrict__ frames,
@@ -134,11 +23,12 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void sk_batch_kernel(const BatchTra
         const int q = next_item(counter, &s_item);
         if (q >= n_items) break;
         const int4 it = items[q];
-        sk_item<R, MM>(trajs[it.x], it.y, it.z, frames, n_frames, group_size, kp, tab, anc, sU, sValid);
+        sk_item<R, MM, GROUPED>(trajs[it.x], it.y, it.z, frames, n_frames, group_size, kp, tab, anc, sU, sValid);
     }
 }
 
-// Batched Fs, persistent: items (trajectory, global chunk, time gap, -).
+// Batched Fs, persistent: items (trajectory, global chunk, time gap, -);
+// one row per gap.
 template <int R, int MM>
 __global__ __launch_bounds__(kPhaseBlock, 4) void fs_batch_kernel(const BatchTraj *__restrict__ trajs,
                                                                  const int4 *__restrict__ items, int n_items,
@@ -153,31 +43,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_batch_kernel(const BatchTra
         const int q = next_item(counter, &s_item);
         if (q >= n_items) break;
         const int4 it = items[q];
-        fs_item<R, MM>(trajs[it.x], it.y, it.z, T0, T1, group_off, order, kp, tab, anc, sU, sValid);
-    }
-}
-
-// One analysis job over a batch in a single persistent launch: S(k), Fs and
-// MSD/NGP items (item.w = 0, 1, 2) drawn from one queue, costliest first.
-// PS: the S(k) plan carries group codes.
-template <int RS, int RF, int MM, bool PS>
-__global__ __launch_bounds__(kPhaseBlock, 4) void job_kernel(BatchJob j)
-{
-    AMDAT_PHASE_SHARED(MM);
-    __shared__ int s_item;
-    __shared__ double red[kWaves];
-    for (;;) {
-        const int q = next_item(j.counter, &s_item);
-        if (q >= j.n_items) break;
-        const int4 it = j.items[q];
-        if (it.w == 0)
-            sk_item<RS, MM, PS>(j.sk_trajs[it.x], it.y, it.z, j.frames, j.n_frames, j.group_size, j.sk, tab, anc,
-                                sU, sValid);
-        else if (it.w == 1)
-            fs_item<RF, MM>(j.fs_trajs[it.x], it.y, it.z, j.t0, j.t1, j.group_off, j.order, j.fs, tab, anc, sU,
-                            sValid);
-        else
-            msd_item(j.msd_trajs[it.x], it.x, it.y, j.ntg, j.t0, j.t1, j.group_off, j.order, j.s2, j.s4, red);
+        fs_item<R, MM>(trajs[it.x], it.y, it.z, 1, T0, T1, group_off, order, kp, tab, anc, sU, sValid);
     }
 }
 
@@ -258,11 +124,13 @@ __global__ __launch_bounds__(kBlock) void sk_batch_reduce_kernel(const BatchTraj
 }
 
 // grid = (max shells, time gaps, trajectories) over ragged per-trajectory
-// shells: out[out_off[b] + g*nsh[b] + s] = sum_{v in s} V_b[g][v] / H / norm[b*ntg + g].
+// shells: out[out_off[b] + g*nsh[b] + s] = sum over the gap's rows
+// r in [row_off[g], row_off[g+1]) (row_off null: r = g) and v in s of
+// V_b[r][v], / H / norm[b*ntg + g].
 __global__ __launch_bounds__(kBlock) void fs_batch_reduce_kernel(
     const BatchTraj *__restrict__ trajs, const int *__restrict__ shell_off, const int *__restrict__ shell_base,
-    const int *__restrict__ nsh, int ntg, const double *__restrict__ norm, const int *__restrict__ out_off,
-    double *__restrict__ out)
+    const int *__restrict__ nsh, int ntg, const int *__restrict__ row_off, const double *__restrict__ norm,
+    const int *__restrict__ out_off, double *__restrict__ out)
 {
     __shared__ double red[kBlock];
     const int s = blockIdx.x, g = blockIdx.y, b = blockIdx.z;
@@ -271,18 +139,14 @@ __global__ __launch_bounds__(kBlock) void fs_batch_reduce_kernel(
     const int *so = shell_off + shell_base[b];
     const int v0 = so[s], H = so[s + 1] - v0;
     const double nrm = norm[(size_t)b * ntg + g];
+    const int r0 = row_off ? row_off[g] : g, r1 = row_off ? row_off[g + 1] : g + 1;
     double acc = 0.0;
     if (nrm != 0.0)
-        for (int v = threadIdx.x; v < H; v += kBlock) acc += t.out[(size_t)g * t.K + v0 + v];
+        for (int r = r0; r < r1; ++r)
+            for (int v = threadIdx.x; v < H; v += kBlock) acc += t.out[(size_t)r * t.K + v0 + v];
     acc = block_sum(acc, red);
     if (threadIdx.x == 0)
         out[out_off[b] + (size_t)g * nsh[b] + s] = (H > 0 && nrm != 0.0) ? acc / (double)H / nrm : 0.0;
-}
-
-__global__ __launch_bounds__(64) void fill_int_kernel(int *p, int n, int v)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
 }
 
 }  // namespace
@@ -303,19 +167,12 @@ hipError_t persistent_grid(F kfn, size_t lds, int n_items, int *grid)
     return hipSuccess;
 }
 
-hipError_t reset_counter(int *counter, hipStream_t st)
-{
-    (void)hipGetLastError();
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, counter, 1, 0);
-    return hipGetLastError();
-}
-
-template <int R, int MM>
-hipError_t launch_sk_batch_t(const BatchTraj *trajs, const int4 *items, int n_items, int *counter, const int *frames,
+template <int R, int MM, bool GROUPED>
+hipError_t launch_sk_batch_g(const BatchTraj *trajs, const int4 *items, int n_items, int *counter, const int *frames,
                              int n_frames, int group_size, const KPlan &kp, hipStream_t st)
 {
     const size_t lds = Geom<MM>::tab_bytes;
-    auto kfn = sk_batch_kernel<R, MM>;
+    auto kfn = sk_batch_kernel<R, MM, GROUPED>;
     hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int grid = 0;
     if (e == hipSuccess) e = persistent_grid(kfn, lds, n_items, &grid);
@@ -324,6 +181,21 @@ hipError_t launch_sk_batch_t(const BatchTraj *trajs, const int4 *items, int n_it
     hipLaunchKernelGGL(kfn, dim3(grid), dim3(kPhaseBlock), lds, st, trajs, items, n_items, counter, frames, n_frames,
                        group_size, kp);
     return hipGetLastError();
+}
+
+// grouped plans (kp.pair) exist for R = 4 and 8
+template <int R, int MM>
+hipError_t launch_sk_batch_t(const BatchTraj *trajs, const int4 *items, int n_items, int *counter, const int *frames,
+                             int n_frames, int group_size, const KPlan &kp, hipStream_t st)
+{
+    if constexpr (R >= 4) {
+        if (kp.pair)
+            return launch_sk_batch_g<R, MM, true>(trajs, items, n_items, counter, frames, n_frames, group_size, kp,
+                                                  st);
+    } else {
+        if (kp.pair) return hipErrorInvalidValue;
+    }
+    return launch_sk_batch_g<R, MM, false>(trajs, items, n_items, counter, frames, n_frames, group_size, kp, st);
 }
 
 template <int R, int MM>
@@ -351,74 +223,7 @@ hipError_t launch_fs_batch_t(const BatchTraj *trajs, const int4 *items, int n_it
     default: return hipErrorInvalidValue;                      \
     }
 
-template <int RS, int RF, bool PS>
-const void *job_fn_mm(int MM)
-{
-    switch (MM) {
-    case 24: return (const void *)job_kernel<RS, RF, 24, PS>;
-    case 50: return (const void *)job_kernel<RS, RF, 50, PS>;
-    case 100: return (const void *)job_kernel<RS, RF, 100, PS>;
-    case 127: return (const void *)job_kernel<RS, RF, 127, PS>;
-    default: return nullptr;
-    }
-}
-
-template <int RS, bool PS>
-const void *job_fn_rf(int RF, int MM)
-{
-    switch (RF) {
-    case 2: return job_fn_mm<RS, 2, PS>(MM);
-    case 4: return job_fn_mm<RS, 4, PS>(MM);
-    case 8: return job_fn_mm<RS, 8, PS>(MM);
-    default: return nullptr;
-    }
-}
-
-size_t tab_bytes_of(int MM)
-{
-    switch (MM) {
-    case 24: return Geom<24>::tab_bytes;
-    case 50: return Geom<50>::tab_bytes;
-    case 100: return Geom<100>::tab_bytes;
-    case 127: return Geom<127>::tab_bytes;
-    default: return 0;
-    }
-}
-
 }  // namespace
-
-hipError_t prepare_job(int RS, int RF, int MM, int PS, int n_items, JobLaunch *out)
-{
-    const void *fn = nullptr;
-    switch (RS) {
-    case 2: fn = PS ? nullptr : job_fn_rf<2, false>(RF, MM); break;
-    case 4: fn = PS ? nullptr : job_fn_rf<4, false>(RF, MM); break;
-    case 8: fn = PS ? job_fn_rf<8, true>(RF, MM) : job_fn_rf<8, false>(RF, MM); break;
-    default: break;
-    }
-    if (!fn) return hipErrorInvalidValue;
-    const size_t lds = tab_bytes_of(MM);
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    int dev = 0, n_cu = 0, per_cu = 0;
-    if (e == hipSuccess) e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kPhaseBlock, lds);
-    if (e != hipSuccess) return e;
-    out->fn = fn;
-    out->lds = lds;
-    out->grid = std::max(1, std::min(n_items, std::max(1, per_cu) * n_cu));
-    return hipSuccess;
-}
-
-hipError_t enqueue_job(const JobLaunch &L, const BatchJob &j, hipStream_t st)
-{
-    if (j.n_items <= 0) return hipSuccess;
-    hipError_t e = reset_counter(j.counter, st);
-    if (e != hipSuccess) return e;
-    BatchJob arg = j;
-    void *args[] = {&arg};
-    return hipLaunchKernel(L.fn, dim3(L.grid), dim3(kPhaseBlock), args, L.lds, st);
-}
 
 hipError_t launch_sk_batch(const BatchTraj *trajs, const int4 *items, int n_items, int *counter, const int *frames,
                            int n_frames, int group_size, const KPlan &kp, hipStream_t st)
@@ -472,12 +277,12 @@ hipError_t launch_sk_batch_reduce(const BatchTraj *trajs, int n_traj, int n_grou
 }
 
 hipError_t launch_fs_batch_reduce(const BatchTraj *trajs, int n_traj, const int *shell_off, const int *shell_base,
-                                  const int *nsh, int max_nsh, int ntg, const double *norm, const int *out_off,
-                                  double *out, hipStream_t st)
+                                  const int *nsh, int max_nsh, int ntg, const int *row_off, const double *norm,
+                                  const int *out_off, double *out, hipStream_t st)
 {
     (void)hipGetLastError();
     hipLaunchKernelGGL(fs_batch_reduce_kernel, dim3(max_nsh, ntg, n_traj), dim3(kBlock), 0, st, trajs, shell_off,
-                       shell_base, nsh, ntg, norm, out_off, out);
+                       shell_base, nsh, ntg, row_off, norm, out_off, out);
     return hipGetLastError();
 }
 

@@ -47,8 +47,10 @@ struct BatchTraj {
 };
 
 // One analysis job over a batch (device view): S(k), Fs and MSD/NGP items in
-// one queue.  item = (trajectory, chunk, frame group or gap, kind) with kind
-// 0 = S(k), 1 = Fs; MSD items are (trajectory, gap, -, 2).
+// one queue.  item = (trajectory, chunk, row, kind): kind 0 = S(k), row =
+// frame group; kind 1 = Fs, row = pair chunk * fs_split + atom part (pair
+// chunk c holds the pairs order[fs_pc[c] .. fs_pc[c+1]) of one gap); MSD
+// items are (trajectory, gap, -, 2).
 struct BatchJob {
     const int4 *items = nullptr;
     int n_items = 0;
@@ -58,6 +60,8 @@ struct BatchJob {
     int n_frames = 0, group_size = 1;
     KPlan sk, fs;  // fs: per-trajectory plans concatenated
     const int *t0 = nullptr, *t1 = nullptr, *group_off = nullptr, *order = nullptr;
+    const int *fs_pc = nullptr;  // pair-chunk offsets into order
+    int fs_split = 1;            // atom parts per Fs pair chunk
     int ntg = 0;
     double *s2 = nullptr, *s4 = nullptr;  // [trajectory][gap]
 };
@@ -151,9 +155,12 @@ hipError_t launch_fs_batch(const BatchTraj *trajs, const int4 *items, int n_item
 // Job kernel for R = RS (S(k) items), RF (Fs items), table width MM:
 // function attributes and the persistent grid for n_items.  Call outside
 // stream capture; enqueue_job (counter reset + launch) may be captured.
-// PS = 1: the S(k) plan is paired (RS = 8 only).
-hipError_t prepare_job(int RS, int RF, int MM, int PS, int n_items, JobLaunch *out);
+// PS: the S(k) plan carries group codes (RS = 4 or 8).
+hipError_t prepare_job(int RS, int RF, int MM, bool PS, int n_items, JobLaunch *out);
 hipError_t enqueue_job(const JobLaunch &L, const BatchJob &j, hipStream_t st);
+// Atoms per tile of the table kernels built for table half-width MM (fixed
+// at compile time by the LDS budget of two 512-thread blocks per CU).
+int table_atoms_per_tile(int MM);
 // MSD items (trajectory b, pair p): partial[2*(b*n_pairs + p)] = sum r^2, [+1] = sum r^4.
 hipError_t launch_msd_batch(const BatchTraj *trajs, const int2 *items, int n_items, const int *t0, const int *t1,
                             int n_pairs, double *partial, hipStream_t st);
@@ -164,10 +171,12 @@ hipError_t launch_msd_batch_reduce(const double *partial, int n_pairs, const int
 hipError_t launch_sk_batch_reduce(const BatchTraj *trajs, int n_traj, int n_groups, int K, const int *shell_off,
                                   int n_shells, const double *norm, double *S, hipStream_t st);
 // Fs per (trajectory, time gap, shell) over ragged per-trajectory shells:
-// out[out_off[b] + g*nsh[b] + s] = sum_{v in s} rows_b[g][v] / H / norm[b*ntg + g].
+// out[out_off[b] + g*nsh[b] + s] = sum_{r in gap g's rows} sum_{v in s}
+// rows_b[r][v] / H / norm[b*ntg + g]; gap g's rows are [row_off[g],
+// row_off[g+1]) (row_off null: the single row g).
 hipError_t launch_fs_batch_reduce(const BatchTraj *trajs, int n_traj, const int *shell_off, const int *shell_base,
-                                  const int *nsh, int max_nsh, int ntg, const double *norm, const int *out_off,
-                                  double *out, hipStream_t st);
+                                  const int *nsh, int max_nsh, int ntg, const int *row_off, const double *norm,
+                                  const int *out_off, double *out, hipStream_t st);
 
 // Adjoint of the S(k) misfit: g_j = (2 pi/l) sum_v n_v Im(conj(q_v) e^{ik_v.r_j}),
 // the K vectors in n_splits ranges (sk_adjoint_splits); with more than one,

@@ -132,7 +132,8 @@ def bench_batch(args):
     for i in range(max(1, args.warmup)):
         run()
         log("warmup %d done (%.1fs; plan, tuning and capture on the first)" % (i, time.time() - t))
-    info = b.job_info()
+    info = b.job_settings()
+    log("job settings: %s" % info)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for i in range(args.steps):

@@ -502,10 +502,21 @@ int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *p
  * want_msd == 0.  Arguments and outputs are those of amdat_batch_sk,
  * amdat_batch_fs and amdat_batch_msd_ngp (pairs serves Fs and MSD).
  * flags: AMDAT_RUN_GRAPH captures the launch sequence in a hipGraph on the
- * first call and replays it while the arguments stay the same;
- * AMDAT_RUN_TUNE picks the vectors per thread of the S(k) and the Fs items
- * (k-vectors per LDS tile: 1024, 2048 or 4096) by timing each on about an
- * eighth of the trajectories, once per argument set.  The batch keeps the
+ * first call and replays it while the arguments stay the same.  Without
+ * AMDAT_RUN_TUNE the launch settings come from a cost model: the S(k) plan
+ * (vectors per thread, slot grouping) the single-trajectory path would
+ * choose, frames per S(k) item for about 16,384 items, Fs vectors per thread
+ * by the same model, whole time gaps per Fs item, no atom parts.
+ * AMDAT_RUN_TUNE times, once per argument set and separately for the S(k)
+ * and the Fs items: vectors per thread and plan mode (k-vectors per LDS
+ * tile: 1024, 2048 or 4096) on about an eighth of the trajectories, then on
+ * all of them the S(k) frames per item (x1/2, x1, x2 the model's), the Fs
+ * pairs per item (whole gaps, 2, 1) and Fs atom parts (1, 2, 4); it keeps
+ * the tuned settings only when the whole job runs no slower with them than
+ * with the model's (amdat_batch_job_settings reports both timings).  S(k)
+ * items never split a frame's atoms (|rho|^2 is not additive over atom
+ * parts); the atoms per LDS tile are fixed per table width at compile time
+ * (the largest count whose tables keep two blocks per CU).  The batch keeps the
  * planned job (device tables, queue, graph) until another job replaces it
  * or the batch is freed: free the batch before the shells and pairs its
  * last job used. */
@@ -521,6 +532,36 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
  * whether its S(k) plan pairs slots sharing ny or nz, its number of work
  * items, and whether it runs as a captured graph. */
 int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *paired, int *n_items, int *graph);
+
+/* Every launch setting of the last job (no counterpart in the reference,
+ * which has no batched launch): S(k) vectors per thread, plan mode (0
+ * unpaired, 1 paired, 4 grouped in 8, 4 or 2 slots), frames per item and
+ * frame groups; Fs vectors per thread, pairs per item (0: whole time
+ * gaps), atom parts per pair chunk and pair chunks; atoms per LDS tile and
+ * table half-width; work items and resident blocks of the persistent
+ * launch; graph replay; whether the settings came from the autotuner, with
+ * the tuner's whole-job timings (ms, all trajectories) of the cost-model
+ * settings and of the kept ones (0 when not tuned).  Zero for parts the
+ * job does not compute. */
+typedef struct amdat_job_settings {
+    int sk_R, sk_plan_mode, sk_frames_per_item, sk_frame_groups;
+    int fs_R, fs_pairs_per_item, fs_atom_parts, fs_pair_chunks;
+    int atoms_per_tile, table_half_width, n_items, grid;
+    int graph, tuned;
+    float model_ms, tuned_ms;
+} amdat_job_settings;
+int amdat_batch_job_settings(const amdat_batch *batch, amdat_job_settings *out);
+
+/* Fix the launch settings of the batch's next jobs (no counterpart in the
+ * reference): sk_R with sk_plan_mode, sk_frames_per_item, fs_R,
+ * fs_pairs_per_item and fs_atom_parts of *s; a zero field keeps the cost
+ * model's choice (fs_pairs_per_item 0: whole time gaps), the other fields
+ * are ignored, and AMDAT_RUN_TUNE is then ignored.  s == NULL returns to
+ * the cost model and tuner.  For settings read back from
+ * amdat_batch_job_settings of a tuned run, or for tests of each item form.
+ * EINVAL: R not 2, 4 or 8; plan mode not 0, 1 or 4 (1 and 4 need R = 4 or
+ * 8); negative counts; more than 64 atom parts. */
+int amdat_batch_use_settings(amdat_batch *batch, const amdat_job_settings *s);
 
 #ifdef __cplusplus
 }

@@ -229,8 +229,14 @@ def test_batch_job_tuned_against_oracle(ctx, members, sk_shells, job_args):
     o = b.run(a["ell"], sk=(sh, 0, len(off) - 2), fs=(a["fsh"], a["first"], a["last"]), pairs=a["P"], msd=True,
               graph=True, tune=True)
     info = b.job_info()
-    print("tuned", info)
+    st = b.job_settings()
+    print("tuned", info, st)
     assert info["RS"] in (2, 4, 8) and info["RF"] in (2, 4, 8)
+    assert st["tuned"] and st["graph"]
+    assert 0.0 < st["tuned_ms"] <= st["model_ms"]
+    assert st["sk_plan_mode"] in (0, 1, 4) and 1 <= st["sk_frames_per_item"] <= trs[0].T
+    assert st["fs_pairs_per_item"] in (0, 1, 2) and st["fs_atom_parts"] in (1, 2, 4)
+    assert st["atoms_per_tile"] in (8, 16, 32)
     tg, t0, t1 = _abi.ts_pairs(NEXP, NES, 0, 0, 0)
     ntg = _abi.ts_n_timegaps(NEXP, NES, 0)
     qoff, qn = ref_shells(qvec_cache(), 2, 100)
@@ -278,3 +284,68 @@ def test_batch_job_paired_sk_plan(ctx, members, qvec):
         t = trs[i]
         S64 = _sk_oracle(t, n, off, np.arange(t.T), atoms[i])
         assert rel(o["S"][i], S64, np.abs(S64)) < TOL
+
+
+ITEM_FORMS = [
+    dict(sk_R=8, sk_plan_mode=4, sk_frames_per_item=7, fs_R=4, fs_pairs_per_item=1, fs_atom_parts=3),
+    dict(sk_R=4, sk_plan_mode=4, sk_frames_per_item=101, fs_R=2, fs_pairs_per_item=2, fs_atom_parts=2),
+    dict(sk_R=4, sk_plan_mode=1, fs_R=8, fs_atom_parts=64),
+    dict(sk_R=2, fs_pairs_per_item=3),
+]
+
+
+@pytest.mark.parametrize("form", range(len(ITEM_FORMS)))
+def test_batch_job_item_forms(ctx, members, sk_shells, job_args, form):
+    """Each item form the autotuner can choose (plan mode, frames per S(k)
+    item, pairs per Fs item, atom parts per pair chunk; 64 parts leave the
+    smallest members' parts empty) against the separate batched calls (to
+    FP32 rounding: tile boundaries and product orders differ) and members
+    0, 5 (atom subset) and 17 against the oracle."""
+    trs, devs, lists, atoms, b = members
+    a = job_args
+    off, n, sh = sk_shells
+    kw = ITEM_FORMS[form]
+    S, Sc, Fs, Fc, s2, s4, mc = _separate(b, sk_shells, a)
+    b.use_settings(**kw)
+    try:
+        o = b.run(a["ell"], sk=(sh, 0, len(off) - 2), fs=(a["fsh"], a["first"], a["last"]), pairs=a["P"],
+                  msd=True, graph=True, tune=True)
+        st = b.job_settings()
+    finally:
+        b.use_settings()
+    print("form", kw, "->", st)
+    assert not st["tuned"]
+    assert st["sk_R"] == kw["sk_R"] and st["sk_plan_mode"] == kw.get("sk_plan_mode", 0)
+    if "sk_frames_per_item" in kw:
+        assert st["sk_frames_per_item"] == kw["sk_frames_per_item"]
+        assert st["sk_frame_groups"] == -(-trs[0].T // kw["sk_frames_per_item"])
+    assert st["fs_R"] == kw.get("fs_R", st["fs_R"]) and st["fs_atom_parts"] == kw.get("fs_atom_parts", 1)
+    assert st["fs_pairs_per_item"] == kw.get("fs_pairs_per_item", 0)
+    assert np.array_equal(o["S_count"], Sc) and np.array_equal(o["Fs_count"], Fc)
+    full = Sc > 0
+    assert rel(o["S"][full], S[full], np.abs(S[full])) < 1e-7
+    assert np.all(o["S"][~full] == 0.0)
+    for i in range(len(trs)):
+        assert rel(o["Fs"][i], Fs[i], 1.0) < 1e-7
+    assert np.all(o["Fs"][9] == 0.0)
+    _msd_close(o["s2"], s2, mc)
+    tg, t0, t1 = _abi.ts_pairs(NEXP, NES, 0, 0, 0)
+    ntg = _abi.ts_n_timegaps(NEXP, NES, 0)
+    qoff, qn = ref_shells(qvec_cache(), 2, 100)
+    for i in (0, 5, 17):
+        t = trs[i]
+        S64 = _sk_oracle(t, n, off, np.arange(t.T), atoms[i])
+        assert rel(o["S"][i], S64, np.abs(S64)) < TOL
+        so = qoff[a["first"][i]:a["last"][i] + 2] - qoff[a["first"][i]]
+        nv = qn[qoff[a["first"][i]]:qoff[a["last"][i] + 1]]
+        F64, _ = orc.fs_f64(t, (tg, t0, t1), ntg, so, nv, t.L, atoms=atoms[i])
+        assert rel(o["Fs"][i], F64, 1.0) < TOL
+
+
+def test_batch_use_settings_rejects(ctx, members):
+    b = members[4]
+    for kw in (dict(sk_R=3), dict(fs_R=16), dict(sk_R=2, sk_plan_mode=4), dict(sk_R=8, sk_plan_mode=2),
+               dict(fs_atom_parts=65), dict(fs_pairs_per_item=-1), dict(sk_frames_per_item=-2)):
+        with pytest.raises(_abi.AmdatError):
+            b.use_settings(**kw)
+    b.use_settings()

@@ -42,17 +42,20 @@ def c3(ctx):
     print("case: batched job, tuned, graph capture and two replays", flush=True)
     outs = [b.run(Ls, graph=True, tune=True, **args) for _ in range(3)]
     info = b.job_info()
-    print("job", info, flush=True)
+    st = b.job_settings()
+    print("job", info, st, flush=True)
     yield dict(members=members, cols=cols, Ls=Ls, off=off, nv=nv, sk_last=sk_last, fs_first=fs_first,
-               fs_last=fs_last, sh=sh, P=P, pairs=(tg, t0, t1), ntg=ntg, b=b, outs=outs, info=info)
+               fs_last=fs_last, sh=sh, P=P, pairs=(tg, t0, t1), ntg=ntg, b=b, outs=outs, info=info, st=st)
     del b
     del cols
 
 
 def test_c3_job_shape_and_replays(c3):
-    outs, info = c3["outs"], c3["info"]
+    outs, info, st = c3["outs"], c3["info"], c3["st"]
     assert len(c3["members"]) == 256
     assert info["graph"]
+    # the tuner keeps its settings only when the whole job ran no slower
+    assert st["tuned"] and 0.0 < st["tuned_ms"] <= st["model_ms"]
     Ns = np.array([m.N for m in c3["members"]], np.float64)
     assert np.array_equal(outs[0]["S_count"], Ns * 201)
     assert np.array_equal(outs[0]["msd_count"], Ns[:, None] * np.bincount(c3["pairs"][0], minlength=c3["ntg"]))
