@@ -16,6 +16,7 @@ AMDAT_OK = 0
 PREC_FP32C = 0
 PREC_FP64 = 1
 PREC_FP32X = 2
+PREC_FP64T = 3
 
 _ERRNAMES = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ERANGE", -5: "EIO"}
 

@@ -1195,7 +1195,8 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
     if (!frame_list_ok(list))
         return set_error(AMDAT_EINVAL, "amdat_rho_compute: a pair-keyed list applies to frame pairs only");
     if (int rc0 = check_list_traj(list, tr, "amdat_rho_compute")) return rc0;
-    if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64 && precision != AMDAT_PREC_FP32X)
+    if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64 && precision != AMDAT_PREC_FP32X &&
+        precision != AMDAT_PREC_FP64T)
         return set_error(AMDAT_EINVAL, "amdat_rho_compute: unknown precision");
     HIP_TRY(hipSetDevice(ctx->device));
     const int T = tr->dev.T;
@@ -1453,7 +1454,7 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
     if (first_shell < 0 || last_shell < first_shell || last_shell >= sh->n_shells)
         return set_error(AMDAT_ERANGE, "amdat_fs: shell range outside the table");
     if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_fs: length scale must be > 0");
-    if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64)
+    if (precision != AMDAT_PREC_FP32C && precision != AMDAT_PREC_FP64 && precision != AMDAT_PREC_FP64T)
         return set_error(AMDAT_EINVAL, "amdat_fs: unknown precision");
     int rc = check_list_traj(list, tr, "amdat_fs");
     if (rc) return rc;

@@ -78,10 +78,16 @@ int plan_table_width(int max_component);
 
 // rho_k(t) for frames[n_frames] (device); out rows [frame slot][split][K].
 // precision: 0 table products (compensated FP32), 1 plain FP64,
-// 2 direct exact-phase FP32 evaluation (compensated sums).
+// 2 direct exact-phase FP32 evaluation (compensated sums), 3 table
+// products in FP64 (the plan and tables of 0, FP64 entries and sums).
 hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_list,
                       const int *frames, int n_frames, const KPlan &kp, double inv_ell,
                       int n_splits, double2 *out, hipStream_t st);
+// Tabled-FP64 rho and Fs (precision 3 of launch_rho and launch_fs).
+hipError_t launch_rho_t64(const TrajDev &tr, const int *list, int n_list, const int *frames, int n_frames,
+                          const KPlan &kp, double inv_ell, int n_splits, double2 *out, hipStream_t st);
+hipError_t launch_fs_t64(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, int n_pairs,
+                         const KPlan &kp, double inv_ell, int n_splits, double *out, hipStream_t st);
 // Sum splits: out[f][k] = sum_s in[f][s][k].
 hipError_t launch_split_reduce2(const double2 *in, int n_rows, int n_splits, int K, double2 *out,
                                 hipStream_t st);

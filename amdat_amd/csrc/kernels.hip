@@ -830,6 +830,7 @@ hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_l
                       int n_frames, const KPlan &kp, double inv_ell, int n_splits, double2 *out,
                       hipStream_t st)
 {
+    if (precision == 3) return launch_rho_t64(tr, list, n_list, frames, n_frames, kp, inv_ell, n_splits, out, st);
     if (precision == 2) {
         const int per_split = (n_list + n_splits - 1) / n_splits;
         dim3 grid(kp.n_chunks, n_frames, n_splits);
@@ -867,6 +868,7 @@ hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_li
                      const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
                      hipStream_t st)
 {
+    if (precision == 3) return launch_fs_t64(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     if (precision == 1) {
         const int per_split = (n_list + n_splits - 1) / n_splits;
         const double kscale = 2.0 * M_PI * inv_ell;

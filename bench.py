@@ -570,6 +570,27 @@ def main():
         result["fp64_ms_per_step"] = fp64_ms
         result["speedup_vs_fp64"] = fp64_ms / ms_per_step
         log("plain FP64 step %.1f ms; errors %s" % (fp64_ms, err))
+        # The tabled-FP64 form: the same plan and phase tables as the timed
+        # FP32 form with FP64 entries, products and sums (the strongest FP64
+        # form here); one untimed run first, then one timed step.
+        step(_abi.PREC_FP64T)
+        t64 = {"rho": [], "fs": []}
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        tab = step(_abi.PREC_FP64T, t64)
+        torch.cuda.synchronize()
+        tab_ms = 1e3 * (time.perf_counter() - t)
+        tab_err = {
+            "S": float(np.max(np.abs(tab["S"] - S64) / np.abs(S64))),
+            "F": float(np.max(np.abs(tab["F"] - ref["F"]) / np.maximum(np.abs(ref["F"]), np.abs(S64)[None]))),
+            "Fs": float(np.max(np.abs(tab["Fs"] - ref["Fs"]) / np.maximum(np.abs(ref["Fs"]), 1.0))),
+        }
+        result["fp64_tabled_ms_per_step"] = tab_ms
+        result["speedup_vs_fp64_tabled"] = tab_ms / ms_per_step
+        result["fp64_tabled_launch_ms"] = {"rho": t64["rho"][0], "fs": t64["fs"][0]}
+        result["fp64_tabled_rel_err_vs_fp64"] = tab_err
+        log("tabled FP64 step %.1f ms (rho %.1f ms, Fs %.1f ms): FP32 form %.2fx faster; vs plain FP64 %s"
+            % (tab_ms, t64["rho"][0], t64["fs"][0], tab_ms / ms_per_step, tab_err))
 
     if not args.no_cpu:
         n_sample = 600 if args.small else 20000   # about 10 s of one core

@@ -49,6 +49,7 @@ extern "C" {
 #define AMDAT_PREC_FP32C 0 /* FP32 phase tables + compensated FP32 sums, FP64 reductions */
 #define AMDAT_PREC_FP64 1  /* plain FP64: FP64 phase, FP64 sin/cos, FP64 sums */
 #define AMDAT_PREC_FP32X 2 /* rho only: exact phase and one FP32 sincos per update, compensated sums */
+#define AMDAT_PREC_FP64T 3 /* tabled FP64: the plan and phase tables of FP32C with FP64 entries and sums */
 
 typedef struct amdat_ctx amdat_ctx;
 typedef struct amdat_traj amdat_traj;

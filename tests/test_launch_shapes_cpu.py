@@ -10,7 +10,7 @@ from tests.conftest import ROOT
 def test_launch_block_sizes_match_bounds():
     src = "\n".join(open(f).read() for f in sorted(glob.glob(os.path.join(ROOT, "amdat_amd", "csrc", "*.hip"))))
     bounds = {}
-    for m in re.finditer(r"__launch_bounds__\((\w+)(?:,\s*\d+)?\)\s+void\s+(\w+)\(", src):
+    for m in re.finditer(r"__launch_bounds__\((\w+)(?:,[^\n]*?)?\)\s+void\s+(\w+)\(", src):
         bounds[m.group(2)] = m.group(1)
     assert len(bounds) >= 10
     launches = re.findall(r"hipLaunchKernelGGL\((\w+)(?:<[^>]*>)?,\s*[^,]+(?:\([^)]*\))?,\s*dim3\((\w+)\)", src)
