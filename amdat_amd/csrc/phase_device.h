@@ -83,28 +83,68 @@ __device__ __forceinline__ int range_mz(int4 r) { return r.z >> 16; }
 __device__ __forceinline__ int range_units(int4 r) { return r.w & 0xFFFF; }
 __device__ __forceinline__ int range_gs_code(int4 r, int wave) { return (int)(((unsigned)r.w >> (16 + 2 * wave)) & 3u); }
 
-// Build one tile of tables from the fixed-point fractions sU[3A] (row
-// 3i+c = atom i, axis c; invalid atoms carry sValid[i] == 0 and get zero
-// rows), only over the entries the chunk reads: x in [xlo, xhi], y in
-// [-my, my], z in [0, mz].  Phase 1: exact small anchors e(d), d < 8, into
-// anc[d][3A].  Phase 2: one work item per (axis, j, atom), atom fastest,
-// computes the exact anchor e(8j) and the entries e(8j + d) = e(8j) e(d),
-// writing all eight at +m and, on x and y, their conjugates at -m (the
-// sections are padded to whole groups, so no store needs a range test).
-// Neighbouring lanes work on neighbouring atoms, whose rows start W
-// entries apart, so their stores and anchor reads fall in distinct banks.
+// Anchor stage of a tile: the fixed-point fractions sU[3A] (row 3i+c =
+// atom i, axis c; invalid atoms carry sValid[i] == 0 and u = 0) and the
+// exact small anchors e(d), d < 8, into anc[d][3A].  Row r is handled by
+// LANES consecutive lanes, each converting the row's coordinate (loaded one
+// tile ahead by fetch_tile) and computing PER anchors d = lane_in_row * PER
+// + q.  Needs no barrier before it: it runs after the previous tile's group
+// stage has read anc and sU.
 template <int MM>
-__device__ __forceinline__ void build_tables(float2 *tab, float2 *anc, const uint32_t *sU, const int *sValid, int4 rng)
+struct AnchorLayout {
+    static constexpr int NR = 3 * Geom<MM>::A;                 // table rows of a tile
+    static constexpr int PER = (NR * 8 <= kPhaseBlock) ? 1 : 2;  // anchors per lane
+    static constexpr int LANES = 8 / PER;                        // lanes per row
+    static constexpr int N = NR * LANES;                         // lanes with work
+    static_assert(N <= kPhaseBlock, "one anchor item per lane");
+};
+
+// Issue the loads of the tile starting at a0 (the lanes of the anchor
+// stage, one row each); the values are consumed by tile_anchors after the
+// current tile's hot loop, so their latency is hidden behind it.
+template <int MM, typename UF>
+__device__ __forceinline__ void fetch_tile(const UF &uf, int a0, int a_end, typename UF::Raw &raw)
+{
+    using AL = AnchorLayout<MM>;
+    const int e = threadIdx.x, row = e / AL::LANES, i = row / 3, c = row - 3 * (row / 3);
+    if (e < AL::N && a0 + i < a_end) raw = uf.load(a0 + i, c);
+}
+
+template <int MM, typename UF>
+__device__ __forceinline__ void tile_anchors(const UF &uf, int a0, int a_end, const typename UF::Raw &raw,
+                                             float2 *anc, uint32_t *sU, int *sValid)
+{
+    using AL = AnchorLayout<MM>;
+    const int e = threadIdx.x;
+    if (e >= AL::N) return;
+    const int row = e / AL::LANES, q0 = (e - row * AL::LANES) * AL::PER;
+    const int i = row / 3, c = row - 3 * (row / 3);
+    const bool valid = a0 + i < a_end;
+    const uint32_t u = valid ? uf.convert(raw, c) : 0u;
+#pragma unroll
+    for (int q = 0; q < AL::PER; ++q) anc[(q0 + q) * AL::NR + row] = sincos_turns(u * (uint32_t)(q0 + q));
+    if (q0 == 0) {
+        sU[row] = u;
+        if (c == 0) sValid[i] = valid;
+    }
+}
+
+// Group stage of a tile, over the entries the chunk reads only: x in
+// [xlo, xhi], y in [-my, my], z in [0, mz].  One work item per (axis, j,
+// atom), atom fastest, computes the exact anchor e(8j) and the entries
+// e(8j + d) = e(8j) e(d), writing all eight at +m and, on x and y, their
+// conjugates at -m (the sections are padded to whole groups, so no store
+// needs a range test).  Neighbouring lanes work on neighbouring atoms,
+// whose rows start W entries apart, so their stores and anchor reads fall
+// in distinct banks.  Runs after a barrier that follows tile_anchors.
+template <int MM>
+__device__ __forceinline__ void build_groups(float2 *tab, const float2 *anc, const uint32_t *sU, const int *sValid,
+                                             int4 rng)
 {
     using Gm = Geom<MM>;
     constexpr int NR = 3 * Gm::A;  // table rows of a tile
     const int tid = threadIdx.x;
     const int xlo = rng.x, xhi = rng.y, my = range_my(rng), mz = range_mz(rng);
-    for (int e = tid; e < NR * 8; e += kPhaseBlock) {
-        const int d = e / NR, row = e - d * NR;
-        anc[e] = sincos_turns(sU[row] * (uint32_t)d);
-    }
-    __syncthreads();
     // |m| needed on x: [xa, xb]
     const int xa = xlo >= 0 ? xlo : (xhi <= 0 ? -xhi : 0);
     const int xb = max(abs(xlo), abs(xhi));
@@ -218,6 +258,13 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         pz[r] = sw ? tab + Gm::OY + ny : tab + Gm::OZ + nz;
         h0[r] = l0[r] = h1[r] = l1[r] = 0.0f;
     }
+    // Tile pipeline, two barriers per tile: the coordinates of tile t+1 are
+    // loaded before tile t's hot loop and its anchors computed after it, so
+    // neither the load latency nor the anchor stage waits at a barrier.
+    typename UF::Raw raw{};
+    fetch_tile<MM>(uf, a_begin, a_end, raw);
+    tile_anchors<MM>(uf, a_begin, a_end, raw, anc, sU, sValid);
+    if (a_begin + Gm::A < a_end) fetch_tile<MM>(uf, a_begin + Gm::A, a_end, raw);
     // Groups of FT tiles; the FP32 partials of a group are folded into the
     // compensated sums once, after its last tile: every 32 atoms for rho
     // (per-vector sums are outputs), every 64 for Fs (averaged over pairs
@@ -229,34 +276,36 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         for (int r = 0; r < R; ++r) p0[r] = p1[r] = 0.0f;
         const int g_end = min(a_end, g0 + FT * Gm::A);
         for (int a0 = g0; a0 < g_end; a0 += Gm::A) {
-            __syncthreads();
-            if (tid < 3 * Gm::A) {
-                const int i = tid / 3, c = tid - 3 * (tid / 3);
-                const int a = a0 + i;
-                sU[tid] = a < a_end ? uf(a, c) : 0u;
-                if (c == 0) sValid[i] = a < a_end;
+            __syncthreads();  // anchors of tile a0 written; the previous tile's reads of tab done
+            build_groups<MM>(tab, anc, sU, sValid, rng);
+            __syncthreads();  // tables of tile a0 complete
+            if (active) {
+                if constexpr (GROUPED) {
+                    bool done = false;
+                    if constexpr (R >= 8) {
+                        if (gs_code == 3) {
+                            hot_tile<R, MM, FS, 8>(px, py, pz, p0, p1);
+                            done = true;
+                        }
+                    }
+                    if constexpr (R >= 4) {
+                        if (!done && gs_code == 2) {
+                            hot_tile<R, MM, FS, 4>(px, py, pz, p0, p1);
+                            done = true;
+                        }
+                    }
+                    if (!done) {
+                        if (gs_code == 1) hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
+                        else hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
+                    }
+                } else {
+                    hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
+                }
             }
-            __syncthreads();
-            build_tables<MM>(tab, anc, sU, sValid, rng);
-            __syncthreads();
-            if (!active) continue;
-            if constexpr (GROUPED) {
-                if constexpr (R >= 8) {
-                    if (gs_code == 3) {
-                        hot_tile<R, MM, FS, 8>(px, py, pz, p0, p1);
-                        continue;
-                    }
-                }
-                if constexpr (R >= 4) {
-                    if (gs_code == 2) {
-                        hot_tile<R, MM, FS, 4>(px, py, pz, p0, p1);
-                        continue;
-                    }
-                }
-                if (gs_code == 1) hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
-                else hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
-            } else {
-                hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
+            const int an = a0 + Gm::A;  // block-uniform
+            if (an < a_end) {
+                tile_anchors<MM>(uf, an, a_end, raw, anc, sU, sValid);
+                if (an + Gm::A < a_end) fetch_tile<MM>(uf, an + Gm::A, a_end, raw);
             }
         }
 #pragma unroll
@@ -267,43 +316,58 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
     }
 }
 
-// Fixed-point fraction of a coordinate (rho).
+// Fixed-point fraction of a coordinate (rho): load() fetches the FP32
+// coordinate of list position a on axis c, convert() forms the fraction.
 struct CoordFrac {
     const float *X, *Y, *Z;
     const int *list;
     size_t foff;
     double inv_ell;
-    __device__ __forceinline__ uint32_t operator()(int a, int c) const
+    struct Raw {
+        float v;
+    };
+    __device__ __forceinline__ Raw load(int a, int c) const
     {
         const size_t o = foff + (list ? list[a] : a);
-        const float v = c == 0 ? X[o] : (c == 1 ? Y[o] : Z[o]);
-        return turns_fixed((double)v * inv_ell);
+        return Raw{c == 0 ? X[o] : (c == 1 ? Y[o] : Z[o])};
     }
+    __device__ __forceinline__ uint32_t convert(const Raw &r, int) const { return turns_fixed((double)r.v * inv_ell); }
 };
 
 // Fixed-point fraction of a displacement dr = (x1 + i1*L1) - (x0 + i0*L0),
-// rebuilt exactly in FP64 (Fs).
+// rebuilt exactly in FP64 (Fs); the pair's box lengths L0/L1 are loaded at
+// construction.
 struct DispFrac {
     const float *X, *Y, *Z;
     const int16_t *IX, *IY, *IZ;
-    const float *L0, *L1;
     const int *list;
     size_t o0, o1;
     double inv_ell;
-    __device__ __forceinline__ uint32_t operator()(int a, int c) const
+    float l0x, l0y, l0z, l1x, l1y, l1z;
+    struct Raw {
+        float x0, x1;
+        uint32_t i01;  // i0 in the low 16 bits, i1 in the high
+    };
+    __device__ DispFrac(const float *X_, const float *Y_, const float *Z_, const int16_t *IX_, const int16_t *IY_,
+                        const int16_t *IZ_, const float *L0, const float *L1, const int *list_, size_t o0_,
+                        size_t o1_, double inv_ell_)
+        : X(X_), Y(Y_), Z(Z_), IX(IX_), IY(IY_), IZ(IZ_), list(list_), o0(o0_), o1(o1_), inv_ell(inv_ell_),
+          l0x(L0[0]), l0y(L0[1]), l0z(L0[2]), l1x(L1[0]), l1y(L1[1]), l1z(L1[2])
+    {
+    }
+    __device__ __forceinline__ Raw load(int a, int c) const
     {
         const int j = list ? list[a] : a;
         const size_t p0 = o0 + j, p1 = o1 + j;
-        float x0, x1, l0, l1;
-        int i0, i1;
-        if (c == 0) {
-            x0 = X[p0], x1 = X[p1], i0 = IX[p0], i1 = IX[p1], l0 = L0[0], l1 = L1[0];
-        } else if (c == 1) {
-            x0 = Y[p0], x1 = Y[p1], i0 = IY[p0], i1 = IY[p1], l0 = L0[1], l1 = L1[1];
-        } else {
-            x0 = Z[p0], x1 = Z[p1], i0 = IZ[p0], i1 = IZ[p1], l0 = L0[2], l1 = L1[2];
-        }
-        const double d = ((double)x1 - (double)x0) + ((double)i1 * (double)l1 - (double)i0 * (double)l0);
+        const float *P = c == 0 ? X : (c == 1 ? Y : Z);
+        const int16_t *I = c == 0 ? IX : (c == 1 ? IY : IZ);
+        return Raw{P[p0], P[p1], (uint32_t)(uint16_t)I[p0] | ((uint32_t)(uint16_t)I[p1] << 16)};
+    }
+    __device__ __forceinline__ uint32_t convert(const Raw &r, int c) const
+    {
+        const float l0 = c == 0 ? l0x : (c == 1 ? l0y : l0z), l1 = c == 0 ? l1x : (c == 1 ? l1y : l1z);
+        const int i0 = (int16_t)(r.i01 & 0xFFFFu), i1 = (int16_t)(r.i01 >> 16);
+        const double d = ((double)r.x1 - (double)r.x0) + ((double)i1 * (double)l1 - (double)i0 * (double)l0);
         return turns_fixed(d * inv_ell);
     }
 };
@@ -346,7 +410,7 @@ __device__ __forceinline__ void fs_block(const float *__restrict__ X, const floa
                                          double inv_ell, const uint32_t *__restrict__ kp, int4 rng, float2 *tab,
                                          float2 *anc, uint32_t *sU, int *sValid, float (&hs)[R], float (&ls)[R])
 {
-    const DispFrac uf{X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell};
+    const DispFrac uf(X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell);
     float u1[R], u2[R];
     phase_tiles<R, MM, true, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
 }
