@@ -26,19 +26,16 @@ __device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, co
     const int *kd = kp.kdst + (size_t)chunk * kp.KB;
     double *row = t.out + (size_t)g * kp.K;
     for (int f = f_begin; f < f_end; ++f) {
-        float hr[R], lr[R], hi[R], li[R];
+        double re[R], im[R];
         rho_block<R, MM, GROUPED>(t.x, t.y, t.z, t.list, (size_t)frames[f] * (size_t)t.N, 0, t.n_list, t.inv_ell,
-                                  kp.kpack + (size_t)chunk * kp.KB, kp.range[chunk], tab, anc, sU, sValid, hr, lr,
-                                  hi, li);
+                                  kp.kpack + (size_t)chunk * kp.KB, kp.range[chunk], tab, anc, sU, sValid, re, im);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int d = kd[r * kPhaseBlock + threadIdx.x];
             if (d >= 0) {
-                const double re = (double)hr[r] + (double)lr[r];
-                const double im = (double)hi[r] + (double)li[r];
                 double *slot = row + (d & ~(1 << 30));
                 const double prev = f == f_begin ? 0.0 : *slot;
-                *slot = prev + (re * re + im * im);
+                *slot = prev + (re[r] * re[r] + im[r] * im[r]);
             }
         }
     }
@@ -66,17 +63,17 @@ __device__ __forceinline__ void fs_item(const BatchTraj &t, int chunk, int r, in
     for (int pq = p_begin; pq < p_end; ++pq) {
         const int p = order[pq];
         const int t0 = T0[p], t1 = T1[p];
-        float hs[R], ls[R];
+        double sum[R];
         fs_block<R, MM>(t.x, t.y, t.z, t.ix, t.iy, t.iz, t.box + 3 * t0, t.box + 3 * t1, t.list, (size_t)t0 * t.N,
                         (size_t)t1 * t.N, a_begin, a_end, t.inv_ell, kp.kpack + (size_t)chunk * kp.KB,
-                        kp.range[chunk], tab, anc, sU, sValid, hs, ls);
+                        kp.range[chunk], tab, anc, sU, sValid, sum);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int d = kd[q * kPhaseBlock + threadIdx.x];
             if (d >= 0) {
                 double *slot = row + (d & ~(1 << 30));
                 const double prev = pq == p_begin ? 0.0 : *slot;
-                *slot = prev + ((double)hs[q] + (double)ls[q]);
+                *slot = prev + sum[q];
             }
         }
     }

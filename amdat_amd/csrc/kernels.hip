@@ -13,7 +13,7 @@ namespace amdat {
 
 namespace {
 
-// rho_k(t), compensated FP32.  Work items (chunk, frame slot, split),
+// rho_k(t), FP32 tables and tile partials, FP64 sums.  Work items (chunk, frame slot, split),
 // numbered by xcd_item.  GROUPED: the plan's waves carry group codes.
 template <int R, int MM, bool GROUPED>
 __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
@@ -28,10 +28,10 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const int fslot = row % n_frames, split = row / n_frames;
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
-    float hr[R], lr[R], hi[R], li[R];
+    double re[R], im[R];
     rho_block<R, MM, GROUPED>(X, Y, Z, list, (size_t)frames[fslot] * (size_t)N, a_begin, a_end, inv_ell,
-                     kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, hr, lr, hi, li);
-    rho_write<R>(kdst + (size_t)chunk * KB, hr, lr, hi, li, out + ((size_t)fslot * n_splits + split) * (size_t)K);
+                              kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, re, im);
+    rho_write<R>(kdst + (size_t)chunk * KB, re, im, out + ((size_t)fslot * n_splits + split) * (size_t)K);
 }
 
 // Fs per (pair, vector), compensated FP32: sum over the split's atoms of
@@ -52,10 +52,11 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const int t0 = T0[pslot], t1 = T1[pslot];
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
-    float hs[R], ls[R];
-    fs_block<R, MM, GROUPED>(X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N, a_begin,
-                    a_end, inv_ell, kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, hs, ls);
-    fs_write<R>(kdst + (size_t)chunk * KB, hs, ls, out + ((size_t)pslot * n_splits + split) * (size_t)K);
+    double sum[R];
+    fs_block<R, MM, GROUPED>(X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N,
+                             a_begin, a_end, inv_ell, kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid,
+                             sum);
+    fs_write<R>(kdst + (size_t)chunk * KB, sum, out + ((size_t)pslot * n_splits + split) * (size_t)K);
 }
 
 // Plain FP64 rho: FP64 phase, FP64 sincos, FP64 sums.

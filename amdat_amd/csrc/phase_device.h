@@ -2,7 +2,7 @@
 // batch_kernels.hip): table geometry, table build, the tile loop and the
 // per-atom phase-fraction functors.
 //
-// Phase kernels (rho, Fs), compensated-FP32 form:
+// Phase kernels (rho, Fs), FP32-table form (AMDAT_PREC_FP32C):
 //   * a block of 512 threads owns one chunk of KB = 512*R k-vectors (R per
 //     thread, kept in registers as LDS addresses) and one frame (rho) or
 //     time-origin pair (Fs), and streams the atoms of its split through LDS
@@ -20,9 +20,9 @@
 //     per-lane base and a compile-time offset (the tile loop is fully
 //     unrolled), and two complex products,
 //         exp(i k.r) = e_x(nx) e_y(ny) e_z(nz),
-//     accumulated with FMA into an FP32 partial per tile; tile partials are
-//     folded into a compensated FP32 (hi, lo) pair (Fast2Sum); the block
-//     writes hi + lo as FP64.
+//     accumulated with FMA into an FP32 partial over 32 atoms (rho) or 64
+//     (Fs); the partials are added into an FP64 sum per slot (one convert
+//     and one FP64 add), which the block writes.
 // The plain-FP64 form evaluates exp(i k.r) with FP64 sincos per update and
 // accumulates in FP64; it is the reference the FP32 form is measured against.
 #pragma once
@@ -235,7 +235,7 @@ __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (
 template <int R, int MM, bool FS, bool GROUPED, typename UF>
 __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
                                             float2 *tab, float2 *anc, uint32_t *sU, int *sValid, const UF uf,
-                                            float (&h0)[R], float (&l0)[R], float (&h1)[R], float (&l1)[R])
+                                            double (&s0)[R], double (&s1)[R])
 {
     using Gm = Geom<MM>;
     const int tid = threadIdx.x;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         const bool sw = (k >> 24) & 1u;
         py[r] = sw ? tab + Gm::OZ + nz : tab + Gm::OY + ny;
         pz[r] = sw ? tab + Gm::OY + ny : tab + Gm::OZ + nz;
-        h0[r] = l0[r] = h1[r] = l1[r] = 0.0f;
+        s0[r] = s1[r] = 0.0;
     }
     // Tile pipeline, two barriers per tile: the coordinates of tile t+1 are
     // loaded before tile t's hot loop and its anchors computed after it, so
@@ -265,8 +265,8 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
     fetch_tile<MM>(uf, a_begin, a_end, raw);
     tile_anchors<MM>(uf, a_begin, a_end, raw, anc, sU, sValid);
     if (a_begin + Gm::A < a_end) fetch_tile<MM>(uf, a_begin + Gm::A, a_end, raw);
-    // Groups of FT tiles; the FP32 partials of a group are folded into the
-    // compensated sums once, after its last tile: every 32 atoms for rho
+    // Groups of FT tiles; the FP32 partials of a group are added into the
+    // FP64 sums once, after its last tile: every 32 atoms for rho
     // (per-vector sums are outputs), every 64 for Fs (averaged over pairs
     // and shells).
     constexpr int FT = (FS ? 64 : 32) / Gm::A > 0 ? (FS ? 64 : 32) / Gm::A : 1;
@@ -310,8 +310,8 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            fast_two_sum_acc(h0[r], l0[r], p0[r]);
-            if constexpr (!FS) fast_two_sum_acc(h1[r], l1[r], p1[r]);
+            s0[r] += (double)p0[r];
+            if constexpr (!FS) s1[r] += (double)p1[r];
         }
     }
 }
@@ -372,35 +372,30 @@ struct DispFrac {
     }
 };
 
-// rho over one block's atoms: compensated sums (hr + lr, hi + li) per slot.
+// rho over one block's atoms: FP64 sums (re, im) per slot.
 template <int R, int MM, bool GROUPED = false>
 __device__ __forceinline__ void rho_block(const float *__restrict__ X, const float *__restrict__ Y,
                                           const float *__restrict__ Z, const int *__restrict__ list, size_t foff,
                                           int a_begin, int a_end, double inv_ell, const uint32_t *__restrict__ kp,
-                                          int4 rng, float2 *tab, float2 *anc, uint32_t *sU, int *sValid, float (&hr)[R],
-                                          float (&lr)[R], float (&hi)[R], float (&li)[R])
+                                          int4 rng, float2 *tab, float2 *anc, uint32_t *sU, int *sValid,
+                                          double (&re)[R], double (&im)[R])
 {
     const CoordFrac uf{X, Y, Z, list, foff, inv_ell};
-    phase_tiles<R, MM, false, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hr, lr, hi, li);
+    phase_tiles<R, MM, false, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, re, im);
 }
 
 template <int R>
-__device__ __forceinline__ void rho_write(const int *__restrict__ kd, const float (&hr)[R], const float (&lr)[R],
-                                          const float (&hi)[R], const float (&li)[R], double2 *__restrict__ dst)
+__device__ __forceinline__ void rho_write(const int *__restrict__ kd, const double (&re)[R], const double (&im)[R],
+                                          double2 *__restrict__ dst)
 {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int d = kd[r * kPhaseBlock + threadIdx.x];
-        if (d >= 0) {
-            const double re = (double)hr[r] + (double)lr[r];
-            double im = (double)hi[r] + (double)li[r];
-            if (d & (1 << 30)) im = -im;
-            dst[d & ~(1 << 30)] = make_double2(re, im);
-        }
+        if (d >= 0) dst[d & ~(1 << 30)] = make_double2(re[r], (d & (1 << 30)) ? -im[r] : im[r]);
     }
 }
 
-// Fs over one block's atoms for the pair (o0, o1): compensated sum of cos(k.dr).
+// Fs over one block's atoms for the pair (o0, o1): FP64 sum of cos(k.dr).
 template <int R, int MM, bool GROUPED = false>
 __device__ __forceinline__ void fs_block(const float *__restrict__ X, const float *__restrict__ Y,
                                          const float *__restrict__ Z, const int16_t *__restrict__ IX,
@@ -408,21 +403,20 @@ __device__ __forceinline__ void fs_block(const float *__restrict__ X, const floa
                                          const float *__restrict__ L0, const float *__restrict__ L1,
                                          const int *__restrict__ list, size_t o0, size_t o1, int a_begin, int a_end,
                                          double inv_ell, const uint32_t *__restrict__ kp, int4 rng, float2 *tab,
-                                         float2 *anc, uint32_t *sU, int *sValid, float (&hs)[R], float (&ls)[R])
+                                         float2 *anc, uint32_t *sU, int *sValid, double (&s)[R])
 {
     const DispFrac uf(X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell);
-    float u1[R], u2[R];
-    phase_tiles<R, MM, true, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, hs, ls, u1, u2);
+    double unused[R];
+    phase_tiles<R, MM, true, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, s, unused);
 }
 
 template <int R>
-__device__ __forceinline__ void fs_write(const int *__restrict__ kd, const float (&hs)[R], const float (&ls)[R],
-                                         double *__restrict__ dst)
+__device__ __forceinline__ void fs_write(const int *__restrict__ kd, const double (&s)[R], double *__restrict__ dst)
 {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int d = kd[r * kPhaseBlock + threadIdx.x];
-        if (d >= 0) dst[d & ~(1 << 30)] = (double)hs[r] + (double)ls[r];
+        if (d >= 0) dst[d & ~(1 << 30)] = s[r];
     }
 }
 
