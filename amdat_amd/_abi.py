@@ -60,6 +60,12 @@ SIGNATURES = {
     "amdat_traj_bounds": (_I, [_P, _PF]),
     "amdat_bins_assign": (_I, [_P, _P, _I, _I, _I, _PF, _PF, _PI]),
     "amdat_bins_distance": (_I, [_P, _P, _P, _I, _PF, _I, _I, _P, _F, _I, _PI]),
+    "amdat_cells_assign": (_I, [_P, _P, _I, _I, _I, _PF, _PF, _PP]),
+    "amdat_cells_distance": (_I, [_P, _P, _P, _I, _PF, _I, _I, _P, _F, _I, _PP]),
+    "amdat_cells_download": (_I, [_P, _P, _PI]),
+    "amdat_cells_free": (_I, [_P, _P]),
+    "amdat_cells_bin_lists": (_I, [_P, _P, _P, _I, _I, _PP]),
+    "amdat_cells_pair_lists": (_I, [_P, _P, _P, _I, _PI, _PI, _I, _I, _PP]),
     "amdat_list_create": (_I, [_P, _P, _I, _PI, _PP]),
     "amdat_list_free": (_I, [_P, _P]),
     "amdat_list_create_timed": (_I, [_P, _P, _I, _PI, _PI, _PI, _PP]),
@@ -432,15 +438,78 @@ class PairList(_Handle):
     the frame pair (t0[c], t1[c]); pairs without a key have no atoms."""
     _free = "amdat_list_free"
 
-    def __init__(self, ctx, traj, t0, t1, off, indices):
-        t0, t1, off, idx = (np.ascontiguousarray(a, np.int32) for a in (t0, t1, off, indices))
-        assert len(t0) == len(t1) == len(off) - 1
-        h = C.c_void_p()
-        check(lib().amdat_list_create_paired(ctx.h, traj.h, len(t0), ip(t0), ip(t1), ip(off), ip(idx), C.byref(h)))
+    def __init__(self, ctx, traj, t0, t1, off, indices, _h=None):
+        if _h is None:
+            t0, t1, off, idx = (np.ascontiguousarray(a, np.int32) for a in (t0, t1, off, indices))
+            assert len(t0) == len(t1) == len(off) - 1
+            h = C.c_void_p()
+            check(lib().amdat_list_create_paired(ctx.h, traj.h, len(t0), ip(t0), ip(t1), ip(off), ip(idx),
+                                                 C.byref(h)))
+        else:
+            h = _h
         super().__init__(ctx, h)
         nt, n = C.c_int(), C.c_int()
         check(lib().amdat_list_shape(self.h, C.byref(nt), C.byref(n)))
         self.n_keys, self.n = nt.value, n.value
+
+    def download(self):
+        """(off [n_keys+1], indices [n])."""
+        off = np.zeros(self.n_keys + 1, np.int32)
+        idx = np.zeros(max(self.n, 1), np.int32)
+        check(lib().amdat_list_download(self.ctx.h, self.h, None, ip(off), ip(idx)))
+        return off, idx[:self.n]
+
+
+class Cells(_Handle):
+    """A cell table [T][N] kept on the device (amdat_cells_assign /
+    amdat_cells_distance), and the per-bin lists formed from it."""
+    _free = "amdat_cells_free"
+
+    def __init__(self, ctx, traj, h, n_cells):
+        super().__init__(ctx, h)
+        self.traj, self.n_cells = traj, n_cells
+
+    @classmethod
+    def assign(cls, ctx, traj, nbins, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        h = C.c_void_p()
+        check(lib().amdat_cells_assign(ctx.h, traj.h, int(nbins[0]), int(nbins[1]), int(nbins[2]), fp(lo), fp(hi),
+                                       C.byref(h)))
+        return cls(ctx, traj, h, int(nbins[0]) * int(nbins[1]) * int(nbins[2]))
+
+    @classmethod
+    def distance(cls, ctx, traj, binned, kind, thickness, n_bins, point=None, axis=0, direction=0, cluster=None):
+        pt = np.zeros(3, np.float32)
+        if point is not None:
+            pt[:len(point)] = point
+        h = C.c_void_p()
+        check(lib().amdat_cells_distance(ctx.h, traj.h, binned.h if binned is not None else None, kind, fp(pt), axis,
+                                         direction, cluster.h if cluster is not None else None, float(thickness),
+                                         int(n_bins), C.byref(h)))
+        return cls(ctx, traj, h, int(n_bins))
+
+    def download(self):
+        cell = np.zeros((self.traj.T, self.traj.N), np.int32)
+        check(lib().amdat_cells_download(self.ctx.h, self.h, ip(cell)))
+        return cell
+
+    def bin_lists(self, alist=None, first=0, n=None):
+        """TimedList per bin first .. first+n-1 (one list time per frame)."""
+        n = self.n_cells - first if n is None else n
+        hs = (C.c_void_p * n)()
+        check(lib().amdat_cells_bin_lists(self.ctx.h, self.h, alist.h if alist is not None else None, first, n,
+                                          C.cast(hs, _PP)))
+        return [TimedList(self.ctx, self.traj, None, None, None, _h=C.c_void_p(h)) for h in hs]
+
+    def pair_lists(self, t0, t1, alist=None, first=0, n=None):
+        """PairList per bin, keys (t0[k], t1[k])."""
+        n = self.n_cells - first if n is None else n
+        t0, t1 = np.ascontiguousarray(t0, np.int32), np.ascontiguousarray(t1, np.int32)
+        hs = (C.c_void_p * n)()
+        check(lib().amdat_cells_pair_lists(self.ctx.h, self.h, alist.h if alist is not None else None, len(t0),
+                                           ip(t0), ip(t1), first, n, C.cast(hs, _PP)))
+        return [PairList(self.ctx, self.traj, None, None, None, None, _h=C.c_void_p(h)) for h in hs]
 
 
 class Pairs(_Handle):

@@ -167,6 +167,12 @@ struct amdat_list {
     std::unordered_map<int64_t, int> pkey;
 };
 
+// A cell table [T][N] kept on the device (-1: the atom is in no bin).
+struct amdat_cells {
+    int N = 0, T = 0, n_cells = 0;
+    DevBuf<int> cell;
+};
+
 struct KPlanOwned {
     KPlan kp;
     int mode = 0;  // plan_layout mode the plan was built in
@@ -495,42 +501,16 @@ int check_list_traj(const amdat_list *l, const amdat_traj *tr, const char *fn)
 // pos[i] is pair i's row in the regrouped order (stable within a list
 // time), seg[g] = {list time, first row, row count} per group, and
 // t0/t1/tg hold the regrouped pairs.
-struct PairGroups {
-    std::vector<int> pos, t0, t1, tg;
-    std::vector<int3> seg;
-};
-PairGroups group_pairs(const amdat_list *l, const amdat_pairs *pairs)
-{
-    PairGroups g;
-    const int n = pairs->n;
-    std::vector<int> ord(n);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::vector<int> key(n);
-    for (int i = 0; i < n; i++) key[i] = pair_time(l, pairs->t0[i], pairs->t1[i]);
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
-    g.pos.resize(n);
-    g.t0.resize(n);
-    g.t1.resize(n);
-    g.tg.resize(n);
-    for (int r = 0; r < n; r++) {
-        const int i = ord[r];
-        g.pos[i] = r;
-        g.t0[r] = pairs->t0[i];
-        g.t1[r] = pairs->t1[i];
-        g.tg[r] = pairs->tg[i];
-        if (r == 0 || key[i] != key[ord[r - 1]]) g.seg.push_back(make_int3(key[i], r, 0));
-        g.seg.back().z++;
-    }
-    return g;
-}
-
 // Launch plan of a pair-based entry point (Fs, MSD/NGP, displacements):
 // rows [row, row + len) of the per-pair partial buffers sum over the atoms
 // of seg; t0/t1/tg are the device pairs in row order and order maps the
 // reduction's reference-order positions to rows.  A static list is one
-// group over the pairs in their own order; a time-varying list is one group
-// per list time (src/trajectory_list.cpp:202-211 takes the list at the
-// list time of t0), a pair-keyed list one group per pair.  count[g] receives the atom-pair count of each time gap
+// group over the pairs in their own order.  A time-varying list (the list
+// at the list time of t0, src/trajectory_list.cpp:202-211) or a pair-keyed
+// list is one group as well, over the list's entries, with a per-pair
+// segment table: pair p takes entries [pseg[p].x, pseg[p].x + pseg[p].y)
+// (one launch for every pair, whatever its list time or key).  count[g]
+// receives the atom-pair count of each time gap
 // (src/mean_square_displacement.cpp:157; src/incoherent_scattering_
 // function.cpp:283).
 struct PairLaunch {
@@ -540,9 +520,11 @@ struct PairLaunch {
     };
     std::vector<Group> groups;
     const int *t0 = nullptr, *t1 = nullptr, *tg = nullptr, *order = nullptr;
+    const int2 *pseg = nullptr;
     int max_n = 0;
     bool any_empty = false;
-    DevBuf<int> d_t0, d_t1, d_tg, d_order;
+    DevBuf<int> d_tg;
+    DevBuf<int2> d_pseg;
 };
 int plan_pair_launch(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *l, const amdat_pairs *pairs,
                      bool need_tg, PairLaunch &pl, double *count)
@@ -551,35 +533,28 @@ int plan_pair_launch(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *l, 
     for (int g = 0; g < pairs->ntg; g++) count[g] = 0.0;
     for (int i = 0; i < pairs->n; i++)
         count[pairs->tg[i]] += (double)list_at_pair(tr, l, pairs->t0[i], pairs->t1[i]).n;
+    pl.t0 = pairs->d_t0.p;
+    pl.t1 = pairs->d_t1.p;
+    pl.order = pairs->d_order.p;
+    if (need_tg) HIP_TRY(pl.d_tg.upload(pairs->tg.data(), pairs->tg.size(), st));
+    pl.tg = pl.d_tg.p;
     if (!list_timed(l)) {
         const ListSeg sg = list_seg(tr, l, 0);
         pl.groups.push_back({sg, 0, pairs->n});
-        pl.t0 = pairs->d_t0.p;
-        pl.t1 = pairs->d_t1.p;
-        pl.order = pairs->d_order.p;
-        if (need_tg) HIP_TRY(pl.d_tg.upload(pairs->tg.data(), pairs->tg.size(), st));
-        pl.tg = pl.d_tg.p;
         pl.max_n = sg.n;
         pl.any_empty = sg.n == 0;
         return AMDAT_OK;
     }
-    PairGroups g = group_pairs(l, pairs);
-    for (const int3 &sg3 : g.seg) {
-        const ListSeg sg = list_seg(tr, l, sg3.x);
-        pl.groups.push_back({sg, sg3.y, sg3.z});
-        pl.max_n = std::max(pl.max_n, sg.n);
-        if (sg.n == 0) pl.any_empty = true;
+    std::vector<int2> seg(pairs->n, make_int2(0, 0));
+    for (int i = 0; i < pairs->n; i++) {
+        const int c = pair_time(l, pairs->t0[i], pairs->t1[i]);
+        if (c >= 0 && c < l->n_times) seg[i] = make_int2(l->off[c], l->off[c + 1] - l->off[c]);
+        pl.max_n = std::max(pl.max_n, seg[i].y);
+        if (seg[i].y == 0) pl.any_empty = true;
     }
-    std::vector<int> ord(pairs->n);
-    for (int q = 0; q < pairs->n; q++) ord[q] = g.pos[pairs->order[q]];
-    HIP_TRY(pl.d_t0.upload(g.t0.data(), g.t0.size(), st));
-    HIP_TRY(pl.d_t1.upload(g.t1.data(), g.t1.size(), st));
-    HIP_TRY(pl.d_order.upload(ord.data(), ord.size(), st));
-    if (need_tg) HIP_TRY(pl.d_tg.upload(g.tg.data(), g.tg.size(), st));
-    pl.t0 = pl.d_t0.p;
-    pl.t1 = pl.d_t1.p;
-    pl.tg = pl.d_tg.p;
-    pl.order = pl.d_order.p;
+    HIP_TRY(pl.d_pseg.upload(seg.data(), seg.size(), st));
+    pl.pseg = pl.d_pseg.p;
+    pl.groups.push_back({ListSeg{l->idx.p, pl.max_n}, 0, pairs->n});
     return AMDAT_OK;
 }
 
@@ -734,26 +709,63 @@ int amdat_traj_bounds(const amdat_traj *tr, float *lo)
     return AMDAT_OK;
 }
 
-int amdat_bins_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
-                      const float *hi, int *cell)
+}  // extern "C"
+
+namespace {
+
+// Rectangular cells into dc ([T][N] on the device).
+int cells_rect(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+               const float *hi, DevBuf<int> &dc, const char *fn)
 {
-    clear_error();
-    if (!ctx || !tr || !lo || !hi || !cell) return set_error(AMDAT_EINVAL, "amdat_bins_assign: null argument");
+    if (!ctx || !tr || !lo || !hi) return set_error(AMDAT_EINVAL, std::string(fn) + ": null argument");
     if (n_xbins < 1 || n_ybins < 1 || n_zbins < 1 || (long)n_xbins * n_ybins * n_zbins > (1L << 30))
-        return set_error(AMDAT_EINVAL, "amdat_bins_assign: bin counts must be >= 1");
+        return set_error(AMDAT_EINVAL, std::string(fn) + ": bin counts must be >= 1");
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t st = stream_of(ctx);
     const int N = tr->dev.N, T = tr->dev.T;
     DevBuf<float> dlo, dhi;
-    DevBuf<int> dc;
     HIP_TRY(dlo.upload(lo, 3 * (size_t)T, st));
     HIP_TRY(dhi.upload(hi, 3 * (size_t)T, st));
     HIP_TRY(dc.alloc((size_t)N * T));
     HIP_TRY(launch_bins_assign(tr->dev, n_xbins, n_ybins, n_zbins, dlo.p, dhi.p, dc.p, st));
-    HIP_TRY(hipMemcpyAsync(cell, dc.p, sizeof(int) * (size_t)N * T, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int amdat_bins_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+                      const float *hi, int *cell)
+{
+    clear_error();
+    if (!cell) return set_error(AMDAT_EINVAL, "amdat_bins_assign: null argument");
+    DevBuf<int> dc;
+    int rc = cells_rect(ctx, tr, n_xbins, n_ybins, n_zbins, lo, hi, dc, "amdat_bins_assign");
+    if (rc) return rc;
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(hipMemcpyAsync(cell, dc.p, sizeof(int) * dc.n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_cells_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+                       const float *hi, amdat_cells **out)
+{
+    clear_error();
+    if (!out) return set_error(AMDAT_EINVAL, "amdat_cells_assign: null output");
+    auto c = std::make_unique<amdat_cells>();
+    int rc = cells_rect(ctx, tr, n_xbins, n_ybins, n_zbins, lo, hi, c->cell, "amdat_cells_assign");
+    if (rc) return rc;
+    c->N = tr->dev.N;
+    c->T = tr->dev.T;
+    c->n_cells = n_xbins * n_ybins * n_zbins;
+    *out = c.release();
+    return AMDAT_OK;
+}
+
+}  // extern "C"
 
 namespace {
 // Page-locked host buffer (hipHostMalloc), freed on scope exit.
@@ -785,36 +797,39 @@ static const int *frame_segs(const amdat_traj *tr, const amdat_list *l, std::vec
     return list_ptr(l);
 }
 
-int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
-                        const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
-                        int n_bins, int *cell)
+namespace {
+
+// Distance cells into dc ([T][N] on the device); messages name fn.
+int cells_dist(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind, const float *point,
+               int axis, int direction, const amdat_list *cluster, float thickness, int n_bins, DevBuf<int> &dc,
+               const char *fn)
 {
-    clear_error();
-    if (!ctx || !tr || !cell) return set_error(AMDAT_EINVAL, "amdat_bins_distance: null argument");
+    const std::string F(fn);
+    if (!ctx || !tr) return set_error(AMDAT_EINVAL, F + ": null argument");
     if (kind < AMDAT_DIST_POINT || kind > AMDAT_DIST_LIST)
-        return set_error(AMDAT_EINVAL, "amdat_bins_distance: unknown kind");
+        return set_error(AMDAT_EINVAL, F + ": unknown kind");
     if (n_bins < 1 || !(thickness > 0.0f))
-        return set_error(AMDAT_EINVAL, "amdat_bins_distance: need n_bins >= 1 and thickness > 0");
+        return set_error(AMDAT_EINVAL, F + ": need n_bins >= 1 and thickness > 0");
     if (!frame_list_ok(binned) || !frame_list_ok(cluster))
-        return set_error(AMDAT_EINVAL, "amdat_bins_distance: a pair-keyed list applies to frame pairs only");
+        return set_error(AMDAT_EINVAL, F + ": a pair-keyed list applies to frame pairs only");
     if ((binned && binned->n_atoms != tr->dev.N) || (cluster && cluster->n_atoms != tr->dev.N))
-        return set_error(AMDAT_EINVAL, "amdat_bins_distance: list built for another trajectory size");
+        return set_error(AMDAT_EINVAL, F + ": list built for another trajectory size");
     const int N = tr->dev.N, T = tr->dev.T;
     float p[3] = {0.0f, 0.0f, 0.0f};
     if (kind == AMDAT_DIST_POINT) {
-        if (!point) return set_error(AMDAT_EINVAL, "amdat_bins_distance: null point");
+        if (!point) return set_error(AMDAT_EINVAL, F + ": null point");
         p[0] = point[0], p[1] = point[1], p[2] = point[2];
         // src/trajectory_list_bins.cpp:1358-1362 (Coordinate::operator>)
         for (int t = 0; t < T; t++)
             if (p[0] > tr->box[3 * t] || p[1] > tr->box[3 * t + 1] || p[2] > tr->box[3 * t + 2])
-                return set_error(AMDAT_ERANGE, "amdat_bins_distance: point chosen does not lie within the box dimensions");
+                return set_error(AMDAT_ERANGE, F + ": point chosen does not lie within the box dimensions");
     } else if (kind == AMDAT_DIST_PLANE) {
         if (!point || axis < 0 || axis > 2 || direction < 0 || direction > 2)
-            return set_error(AMDAT_EINVAL, "amdat_bins_distance: plane needs a position, axis 0-2 and direction 0-2");
+            return set_error(AMDAT_EINVAL, F + ": plane needs a position, axis 0-2 and direction 0-2");
         p[0] = point[0];
         for (int t = 0; t < T; t++)
             if (p[0] > tr->box[3 * t + axis])
-                return set_error(AMDAT_ERANGE, "amdat_bins_distance: plane chosen does not lie within the box dimensions");
+                return set_error(AMDAT_ERANGE, F + ": plane chosen does not lie within the box dimensions");
     }
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t st = stream_of(ctx);
@@ -822,7 +837,6 @@ int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
     int max_n = 0, max_c = 0;
     const int *idx = frame_segs(tr, binned, seg, max_n);
     DevBuf<int2> dseg, dcseg;
-    DevBuf<int> dc;
     HIP_TRY(dseg.upload(seg.data(), seg.size(), st));
     HIP_TRY(dc.alloc((size_t)N * T));
     HIP_TRY(hipMemsetAsync(dc.p, 0xFF, sizeof(int) * (size_t)N * T, st));
@@ -834,8 +848,202 @@ int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
         HIP_TRY(launch_distance_cells(tr->dev, dseg.p, idx, max_n, kind, axis, direction, p, thickness, n_bins, dc.p,
                                       st));
     }
-    HIP_TRY(hipMemcpyAsync(cell, dc.p, sizeof(int) * (size_t)N * T, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+                        const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
+                        int n_bins, int *cell)
+{
+    clear_error();
+    if (!cell) return set_error(AMDAT_EINVAL, "amdat_bins_distance: null argument");
+    DevBuf<int> dc;
+    int rc = cells_dist(ctx, tr, binned, kind, point, axis, direction, cluster, thickness, n_bins, dc,
+                        "amdat_bins_distance");
+    if (rc) return rc;
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(hipMemcpyAsync(cell, dc.p, sizeof(int) * dc.n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_cells_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+                         const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
+                         int n_bins, amdat_cells **out)
+{
+    clear_error();
+    if (!out) return set_error(AMDAT_EINVAL, "amdat_cells_distance: null output");
+    auto c = std::make_unique<amdat_cells>();
+    int rc = cells_dist(ctx, tr, binned, kind, point, axis, direction, cluster, thickness, n_bins, c->cell,
+                        "amdat_cells_distance");
+    if (rc) return rc;
+    c->N = tr->dev.N;
+    c->T = tr->dev.T;
+    c->n_cells = n_bins;
+    *out = c.release();
+    return AMDAT_OK;
+}
+
+int amdat_cells_download(amdat_ctx *ctx, const amdat_cells *c, int *cell)
+{
+    clear_error();
+    if (!ctx || !c || !cell) return set_error(AMDAT_EINVAL, "amdat_cells_download: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = stream_of(ctx);
+    HIP_TRY(hipMemcpyAsync(cell, c->cell.p, sizeof(int) * c->cell.n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return AMDAT_OK;
+}
+
+int amdat_cells_free(amdat_ctx *ctx, amdat_cells *c)
+{
+    if (ctx && c) (void)hipSetDevice(ctx->device);
+    delete c;
+    return AMDAT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Per-bin lists of `rows` (int4: frame of the cell, frame whose cell must
+// match or -1, row of the list mask) for bins [first, first + n_bins) of the
+// table: out[b]->off[r .. r+1] and idx receive the row's atoms of bin
+// first + b in ascending order.  l gives the mask: NULL = every atom.
+int cells_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, std::vector<int4> &rows, int first,
+                int n_bins, std::vector<std::unique_ptr<amdat_list>> &out)
+{
+    hipStream_t st = stream_of(ctx);
+    const int N = c->N, nr = (int)rows.size();
+    // list mask rows: static list 0 / every atom -2 / no atom -1; time-varying list: its list time
+    DevBuf<uint8_t> mask;
+    const bool explicit_idx = l && l->n > 0 && l->idx.p;
+    if (explicit_idx) {
+        const int nt = list_timed(l) ? l->n_times : 1;
+        std::vector<int> off = list_timed(l) ? l->off : std::vector<int>{0, l->n};
+        DevBuf<int> doff;
+        HIP_TRY(doff.upload(off.data(), off.size(), st));
+        HIP_TRY(mask.alloc((size_t)nt * N));
+        HIP_TRY(hipMemsetAsync(mask.p, 0, mask.n, st));
+        HIP_TRY(launch_list_mask(l->idx.p, doff.p, nt, l->n, N, mask.p, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    for (int4 &r : rows) {
+        if (!l) { r.z = -2; continue; }
+        if (!list_timed(l)) { r.z = l->n == 0 ? -1 : (l->idx.p ? 0 : -2); continue; }
+        const int ct = list_time(l, r.x);
+        r.z = (ct < 0 || l->off[ct + 1] == l->off[ct]) ? -1 : ct;
+    }
+    DevBuf<int4> drows;
+    HIP_TRY(drows.upload(rows.data(), rows.size(), st));
+    out.clear();
+    for (int b = 0; b < n_bins; b++) {
+        out.push_back(std::make_unique<amdat_list>());
+        out.back()->n_atoms = N;
+        out.back()->off.assign(nr + 1, 0);
+    }
+    std::vector<int> cnt, pos;
+    std::vector<int *> ptr;
+    for (int w0 = 0; w0 < n_bins; w0 += kBinWindow) {
+        const int nw = std::min(kBinWindow, n_bins - w0);
+        DevBuf<int> dcnt, dpos;
+        HIP_TRY(dcnt.alloc((size_t)nw * nr));
+        HIP_TRY(hipMemsetAsync(dcnt.p, 0, sizeof(int) * dcnt.n, st));
+        HIP_TRY(launch_bin_count(c->cell.p, N, drows.p, nr, mask.p, first + w0, nw, dcnt.p, st));
+        cnt.resize((size_t)nw * nr);
+        HIP_TRY(hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        pos.resize(cnt.size());
+        ptr.assign(nw, nullptr);
+        for (int w = 0; w < nw; w++) {
+            amdat_list &L = *out[w0 + w];
+            for (int r = 0; r < nr; r++) {
+                pos[(size_t)w * nr + r] = L.off[r];
+                L.off[r + 1] = L.off[r] + cnt[(size_t)w * nr + r];
+            }
+            L.n = L.off[nr];
+            if (L.n > 0) HIP_TRY(L.idx.alloc((size_t)L.n));
+            ptr[w] = L.idx.p;
+        }
+        DevBuf<int *> dptr;
+        HIP_TRY(dpos.upload(pos.data(), pos.size(), st));
+        HIP_TRY(dptr.upload(ptr.data(), ptr.size(), st));
+        HIP_TRY(launch_bin_write(c->cell.p, N, drows.p, nr, mask.p, first + w0, nw, dpos.p, dptr.p, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return AMDAT_OK;
+}
+
+int cells_args(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int first_bin, int n_bins,
+               amdat_list **lists, const char *fn)
+{
+    const std::string F(fn);
+    if (!ctx || !c || !lists) return set_error(AMDAT_EINVAL, F + ": null argument");
+    if (n_bins < 1 || first_bin < 0 || first_bin + n_bins > c->n_cells)
+        return set_error(AMDAT_ERANGE, F + ": bin range outside the table");
+    if (!frame_list_ok(l)) return set_error(AMDAT_EINVAL, F + ": a pair-keyed list applies to frame pairs only");
+    if (l && l->n_atoms != c->N) return set_error(AMDAT_EINVAL, F + ": list built for another trajectory size");
+    if (l && list_timed(l) && (int)l->conv.size() != c->T)
+        return set_error(AMDAT_EINVAL, F + ": list built for a trajectory of another length");
+    return AMDAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amdat_cells_bin_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int first_bin, int n_bins,
+                          amdat_list **lists)
+{
+    clear_error();
+    int rc = cells_args(ctx, c, l, first_bin, n_bins, lists, "amdat_cells_bin_lists");
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<int4> rows(c->T);
+    for (int t = 0; t < c->T; t++) rows[t] = make_int4(t, -1, 0, 0);
+    std::vector<std::unique_ptr<amdat_list>> out;
+    rc = cells_lists(ctx, c, l, rows, first_bin, n_bins, out);
+    if (rc) return rc;
+    for (int b = 0; b < n_bins; b++) {
+        out[b]->n_times = c->T;
+        out[b]->conv.resize(c->T);
+        std::iota(out[b]->conv.begin(), out[b]->conv.end(), 0);
+        lists[b] = out[b].release();
+    }
+    return AMDAT_OK;
+}
+
+int amdat_cells_pair_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int n_keys, const int *t0,
+                           const int *t1, int first_bin, int n_bins, amdat_list **lists)
+{
+    clear_error();
+    int rc = cells_args(ctx, c, l, first_bin, n_bins, lists, "amdat_cells_pair_lists");
+    if (rc) return rc;
+    if (n_keys < 1 || !t0 || !t1) return set_error(AMDAT_EINVAL, "amdat_cells_pair_lists: bad frame pairs");
+    std::unordered_map<int64_t, int> pkey;
+    std::vector<int4> rows(n_keys);
+    for (int k = 0; k < n_keys; k++) {
+        if (t0[k] < 0 || t0[k] >= c->T || t1[k] < 0 || t1[k] >= c->T)
+            return set_error(AMDAT_ERANGE, "amdat_cells_pair_lists: frame out of range");
+        if (!pkey.emplace(pair_key(t0[k], t1[k]), k).second)
+            return set_error(AMDAT_EINVAL, "amdat_cells_pair_lists: frame pair given twice");
+        rows[k] = make_int4(t0[k], t1[k], 0, 0);
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<std::unique_ptr<amdat_list>> out;
+    rc = cells_lists(ctx, c, l, rows, first_bin, n_bins, out);
+    if (rc) return rc;
+    for (int b = 0; b < n_bins; b++) {
+        out[b]->n_times = n_keys;
+        out[b]->paired = true;
+        out[b]->pkey = pkey;
+        lists[b] = out[b].release();
+    }
     return AMDAT_OK;
 }
 
@@ -1491,7 +1699,8 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
         if (gr.seg.n == 0) continue;
         for (int p0 = gr.row; p0 < gr.row + gr.len; p0 += pb) {
             const int np = std::min(pb, gr.row + gr.len - p0);
-            HIP_TRY(launch_fs(precision, tr->dev, gr.seg.p, gr.seg.n, pl.t0 + p0, pl.t1 + p0, np, kp, inv_ell,
+            HIP_TRY(launch_fs(precision, tr->dev, gr.seg.p, gr.seg.n, pl.pseg ? pl.pseg + p0 : nullptr, pl.t0 + p0,
+                              pl.t1 + p0, np, kp, inv_ell,
                               n_splits, V.p + (size_t)p0 * n_splits * kp.K, st));
         }
     }
@@ -1532,7 +1741,8 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
         if (gr.seg.n == 0) continue;
         for (int p0 = gr.row; p0 < gr.row + gr.len; p0 += pb) {
             const int np = std::min(pb, gr.row + gr.len - p0);
-            HIP_TRY(launch_msd(tr->dev, gr.seg.p, gr.seg.n, pl.t0 + p0, pl.t1 + p0, np, nb,
+            HIP_TRY(launch_msd(tr->dev, gr.seg.p, gr.seg.n, pl.pseg ? pl.pseg + p0 : nullptr, pl.t0 + p0, pl.t1 + p0,
+                               np, nb,
                                part.p + (size_t)p0 * nb * 2, st));
         }
     }
@@ -1584,7 +1794,7 @@ int amdat_msd_ngp_noalloc(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list
     const int pb = 65535;
     for (int p0 = 0; p0 < np; p0 += pb) {
         const int m = std::min(pb, np - p0);
-        HIP_TRY(launch_msd(tr->dev, list_ptr(list), n, pairs->d_t0.p + p0, pairs->d_t1.p + p0, m, nb,
+        HIP_TRY(launch_msd(tr->dev, list_ptr(list), n, nullptr, pairs->d_t0.p + p0, pairs->d_t1.p + p0, m, nb,
                            part + (size_t)p0 * nb * 2, st));
     }
     HIP_TRY(launch_msd_reduce(part, nb, pairs->d_group_off.p, pairs->d_order.p, ntg, d2, d4, st));
@@ -1632,7 +1842,8 @@ int displacements_impl(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *l
         if (gr.seg.n == 0) continue;
         for (int p0 = gr.row; p0 < gr.row + gr.len; p0 += pb) {
             const int np = std::min(pb, gr.row + gr.len - p0);
-            HIP_TRY(launch_disp(tr->dev, gr.seg.p, gr.seg.n, pl.t0 + p0, pl.t1 + p0, pl.tg + p0, np, nb, n_bins,
+            HIP_TRY(launch_disp(tr->dev, gr.seg.p, gr.seg.n, pl.pseg ? pl.pseg + p0 : nullptr, pl.t0 + p0, pl.t1 + p0,
+                                pl.tg + p0, np, nb, n_bins,
                                 hist_mode, bin_size, part.p + (size_t)p0 * nb * 6, dh.p, st));
         }
     }

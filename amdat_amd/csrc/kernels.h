@@ -83,16 +83,34 @@ int plan_table_width(int max_component);
 hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_list,
                       const int *frames, int n_frames, const KPlan &kp, double inv_ell,
                       int n_splits, double2 *out, hipStream_t st);
+// Per-bin lists from a cell table (bin_lists.hip).  Rows: int4 (frame of
+// the cell, second frame whose cell must match or -1, list mask row: -2
+// every atom, -1 none).  Bins are processed in windows [w0, w0 + nw) of at
+// most kBinWindow bins.
+constexpr int kBinWindow = 2048;
+// mask[c * N + idx[e]] = 1 for the entries of list time c (off on the device).
+hipError_t launch_list_mask(const int *idx, const int *off, int n_times, int n, int N, uint8_t *mask, hipStream_t st);
+// cnt[w * n_rows + r] += the row's atoms in window bin w (cnt zeroed by the caller).
+hipError_t launch_bin_count(const int *cell, int N, const int4 *rows, int n_rows, const uint8_t *mask, int w0,
+                            int nw, int *cnt, hipStream_t st);
+// The row's atoms in window bin w, ascending, to dst[w][pos[w * n_rows + r] ...].
+hipError_t launch_bin_write(const int *cell, int N, const int4 *rows, int n_rows, const uint8_t *mask, int w0,
+                            int nw, const int *pos, int *const *dst, hipStream_t st);
 // Tabled-FP64 rho and Fs (precision 3 of launch_rho and launch_fs).
 hipError_t launch_rho_t64(const TrajDev &tr, const int *list, int n_list, const int *frames, int n_frames,
                           const KPlan &kp, double inv_ell, int n_splits, double2 *out, hipStream_t st);
-hipError_t launch_fs_t64(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, int n_pairs,
-                         const KPlan &kp, double inv_ell, int n_splits, double *out, hipStream_t st);
+hipError_t launch_fs_t64(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0,
+                         const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
+                         hipStream_t st);
 // Sum splits: out[f][k] = sum_s in[f][s][k].
 hipError_t launch_split_reduce2(const double2 *in, int n_rows, int n_splits, int K, double2 *out,
                                 hipStream_t st);
-// Fs per (pair, vector): out[p][split][K] = sum_atoms cos(k.dr).
-hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_list,
+// Fs per (pair, vector): out[p][split][K] = sum_atoms cos(k.dr).  The pair
+// kernels (launch_fs, launch_msd, launch_disp) take every pair's atoms from
+// list[0 .. n_list) (list null = all atoms in order), or, with seg non-null,
+// pair p's own atoms list[seg[p].x .. seg[p].x + seg[p].y) (n_list is then
+// the largest count, which sets the atom splits).
+hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_list, const int2 *seg,
                      const int *t0, const int *t1, int n_pairs, const KPlan &kp, double inv_ell,
                      int n_splits, double *out, hipStream_t st);
 // Per (group, column s): sum over the group's pairs and the column's vectors
@@ -110,16 +128,16 @@ hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *grou
                            const int *order, int n_groups, const int *shell_off, int n_shells,
                            const double *norm, double *out, hipStream_t st);
 // Per (pair, atom block): FP64 sums of |dr|^2 and |dr|^4.
-hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int *t0,
+hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0,
                       const int *t1, int n_pairs, int n_blocks, double *partial, hipStream_t st);
 // Per (pair, atom block): FP64 sums of d_c and d_c^2 into partial[(p*n_blocks
 // + block)*6 + k]; with n_bins > 0 the reference-form counts are added to
 // hist[tg[p]*n_bins + bin]: hist_mode 0, bin = int(|dr| / bin_size) below
 // n_bins; hist_mode 1, bin = int(|dr|^2 / bin_size), the last bin taking
 // the overflow.
-hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, const int *tg,
-                       int n_pairs, int n_blocks, int n_bins, int hist_mode, float bin_size, double *partial,
-                       unsigned long long *hist, hipStream_t st);
+hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0, const int *t1,
+                       const int *tg, int n_pairs, int n_blocks, int n_bins, int hist_mode, float bin_size,
+                       double *partial, unsigned long long *hist, hipStream_t st);
 // Per group: the six sums of its pairs' partials (pairs in order[]).
 // Rectangular bin of every (frame, atom) into cell[T][N] (-1 outside).
 hipError_t launch_bins_assign(const TrajDev &tr, int nx, int ny, int nz, const float *lo, const float *hi, int *cell,

@@ -40,7 +40,7 @@ template <int R, int MM, bool GROUPED>
 __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
-    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int2 *__restrict__ seg,
     const int *__restrict__ T0, const int *__restrict__ T1, int n_pairs, int n_chunks,
     const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, const int4 *__restrict__ range, int K, int KB,
     double inv_ell, int per_split, int n_splits, double *__restrict__ out)
@@ -50,6 +50,11 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     if (!xcd_item(n_chunks, n_pairs * n_splits, chunk, row)) return;
     const int pslot = row % n_pairs, split = row / n_pairs;
     const int t0 = T0[pslot], t1 = T1[pslot];
+    if (seg) {  // the pair's own atoms: list[seg.x .. seg.x + seg.y)
+        const int2 sg = seg[pslot];
+        list += sg.x;
+        n_list = sg.y;
+    }
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
     double sum[R];
@@ -104,7 +109,7 @@ template <int R>
 __global__ __launch_bounds__(kPhaseBlock) void fs_f64_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
-    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int2 *__restrict__ seg,
     const int *__restrict__ T0, const int *__restrict__ T1, const uint32_t *__restrict__ kpack,
     const int *__restrict__ kdst, int K, int KB, double kscale, int per_split, int n_splits,
     double *__restrict__ out)
@@ -112,6 +117,11 @@ __global__ __launch_bounds__(kPhaseBlock) void fs_f64_kernel(
     const int chunk = blockIdx.x, pslot = blockIdx.y, split = blockIdx.z;
     const int tid = threadIdx.x;
     const int t0 = T0[pslot], t1 = T1[pslot];
+    if (seg) {
+        const int2 sg = seg[pslot];
+        list += sg.x;
+        n_list = sg.y;
+    }
     const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
     const double L0x = box[3 * t0], L0y = box[3 * t0 + 1], L0z = box[3 * t0 + 2];
     const double L1x = box[3 * t1], L1y = box[3 * t1 + 1], L1z = box[3 * t1 + 2];
@@ -223,13 +233,18 @@ __global__ __launch_bounds__(kBlock) void fs_shell_kernel(
 __global__ __launch_bounds__(kBlock) void msd_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
-    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int2 *__restrict__ seg,
     const int *__restrict__ T0, const int *__restrict__ T1, int n_pairs, int n_blocks, double *__restrict__ partial)
 {
     __shared__ double red[kBlock];
     int bx, p;
     if (!xcd_atom_block(n_blocks, n_pairs, bx, p)) return;
     const int t0 = T0[p], t1 = T1[p];
+    if (seg) {  // the pair's own atoms: list[seg.x .. seg.x + seg.y)
+        const int2 sg = seg[p];
+        list += sg.x;
+        n_list = sg.y;
+    }
     const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
     const double L0x = box[3 * t0], L0y = box[3 * t0 + 1], L0z = box[3 * t0 + 2];
     const double L1x = box[3 * t1], L1y = box[3 * t1 + 1], L1z = box[3 * t1 + 2];
@@ -265,9 +280,9 @@ __global__ __launch_bounds__(kBlock) void msd_kernel(
 __global__ __launch_bounds__(kBlock) void disp_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
-    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int *__restrict__ T0,
-    const int *__restrict__ T1, const int *__restrict__ TG, int n_pairs, int n_blocks, int n_bins, int hist_mode,
-    float bin_size, double *__restrict__ partial, unsigned long long *__restrict__ hist)
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int2 *__restrict__ seg,
+    const int *__restrict__ T0, const int *__restrict__ T1, const int *__restrict__ TG, int n_pairs, int n_blocks,
+    int n_bins, int hist_mode, float bin_size, double *__restrict__ partial, unsigned long long *__restrict__ hist)
 {
 #pragma clang fp contract(off)
     extern __shared__ unsigned int sh_hist[];
@@ -275,6 +290,11 @@ __global__ __launch_bounds__(kBlock) void disp_kernel(
     int bx, p;
     if (!xcd_atom_block(n_blocks, n_pairs, bx, p)) return;
     const int t0 = T0[p], t1 = T1[p];
+    if (seg) {  // the pair's own atoms: list[seg.x .. seg.x + seg.y)
+        const int2 sg = seg[p];
+        list += sg.x;
+        n_list = sg.y;
+    }
     const size_t o0 = (size_t)t0 * (size_t)N, o1 = (size_t)t1 * (size_t)N;
     const float *B0 = box + 3 * t0, *B1 = box + 3 * t1;
     for (int b = threadIdx.x; b < n_bins; b += kBlock) sh_hist[b] = 0u;
@@ -745,8 +765,8 @@ hipError_t launch_rho_c32_t(const TrajDev &tr, const int *list, int n_list, cons
 }
 
 template <int R, int MM, bool GROUPED>
-hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
-                           int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
+hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0,
+                           const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
                            hipStream_t st)
 {
     const int per_split = (n_list + n_splits - 1) / n_splits;
@@ -759,8 +779,8 @@ hipError_t launch_fs_c32_t(const TrajDev &tr, const int *list, int n_list, const
     (void)hipGetLastError();
 
     hipLaunchKernelGGL(kfn, grid, dim3(kPhaseBlock), lds, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box,
-                       tr.N, list, n_list, t0, t1, n_pairs, kp.n_chunks, kp.kpack, kp.kdst, kp.range, kp.K, kp.KB,
-                       inv_ell, per_split, n_splits, out);
+                       tr.N, list, n_list, seg, t0, t1, n_pairs, kp.n_chunks, kp.kpack, kp.kdst, kp.range, kp.K,
+                       kp.KB, inv_ell, per_split, n_splits, out);
     return hipGetLastError();
 }
 
@@ -793,29 +813,29 @@ hipError_t dispatch_rho_c32(int MM, const TrajDev &tr, const int *list, int n_li
 }
 
 template <int R, bool GROUPED>
-hipError_t dispatch_fs_c32_p(int MM, const TrajDev &tr, const int *list, int n_list, const int *t0,
-                             const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits,
-                             double *out, hipStream_t st)
+hipError_t dispatch_fs_c32_p(int MM, const TrajDev &tr, const int *list, int n_list, const int2 *seg,
+                             const int *t0, const int *t1, int n_pairs, const KPlan &kp, double inv_ell,
+                             int n_splits, double *out, hipStream_t st)
 {
     switch (MM) {
-    case 24: return launch_fs_c32_t<R, 24, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 50: return launch_fs_c32_t<R, 50, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 100: return launch_fs_c32_t<R, 100, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 127: return launch_fs_c32_t<R, 127, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 24: return launch_fs_c32_t<R, 24, GROUPED>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 50: return launch_fs_c32_t<R, 50, GROUPED>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 100: return launch_fs_c32_t<R, 100, GROUPED>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 127: return launch_fs_c32_t<R, 127, GROUPED>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <int R>
-hipError_t dispatch_fs_c32(int MM, const TrajDev &tr, const int *list, int n_list, const int *t0,
-                           const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits,
-                           double *out, hipStream_t st)
+hipError_t dispatch_fs_c32(int MM, const TrajDev &tr, const int *list, int n_list, const int2 *seg,
+                           const int *t0, const int *t1, int n_pairs, const KPlan &kp, double inv_ell,
+                           int n_splits, double *out, hipStream_t st)
 {
     if constexpr (R >= 4) {
         if (kp.pair)
-            return dispatch_fs_c32_p<R, true>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+            return dispatch_fs_c32_p<R, true>(MM, tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     }
-    return dispatch_fs_c32_p<R, false>(MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    return dispatch_fs_c32_p<R, false>(MM, tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
 }
 
 }  // namespace
@@ -865,28 +885,28 @@ hipError_t launch_rho(int precision, const TrajDev &tr, const int *list, int n_l
     }
 }
 
-hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_list, const int *t0,
+hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0,
                      const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
                      hipStream_t st)
 {
-    if (precision == 3) return launch_fs_t64(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    if (precision == 3) return launch_fs_t64(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     if (precision == 1) {
         const int per_split = (n_list + n_splits - 1) / n_splits;
         const double kscale = 2.0 * M_PI * inv_ell;
         dim3 grid(kp.n_chunks, n_pairs, n_splits);
         (void)hipGetLastError();
         switch (kp.R) {
-        case 4: hipLaunchKernelGGL(fs_f64_kernel<4>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
-        case 8: hipLaunchKernelGGL(fs_f64_kernel<8>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
-        case 2: hipLaunchKernelGGL(fs_f64_kernel<2>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 4: hipLaunchKernelGGL(fs_f64_kernel<4>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, seg, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 8: hipLaunchKernelGGL(fs_f64_kernel<8>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, seg, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
+        case 2: hipLaunchKernelGGL(fs_f64_kernel<2>, grid, dim3(kPhaseBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, seg, t0, t1, kp.kpack, kp.kdst, kp.K, kp.KB, kscale, per_split, n_splits, out); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
     }
     switch (kp.R) {
-    case 4: return dispatch_fs_c32<4>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 8: return dispatch_fs_c32<8>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 2: return dispatch_fs_c32<2>(kp.MM, tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 4: return dispatch_fs_c32<4>(kp.MM, tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 8: return dispatch_fs_c32<8>(kp.MM, tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 2: return dispatch_fs_c32<2>(kp.MM, tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -933,27 +953,27 @@ hipError_t launch_fs_shell(const double *V, int n_splits, int K, const int *grou
     return hipGetLastError();
 }
 
-hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
+hipError_t launch_msd(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0, const int *t1,
                       int n_pairs, int n_blocks, double *partial, hipStream_t st)
 {
     if (n_pairs <= 0 || n_blocks <= 0) return hipSuccess;
     (void)hipGetLastError();
     const long grid = (long)kXcds * ((n_blocks + kXcds - 1) / kXcds) * n_pairs;
     hipLaunchKernelGGL(msd_kernel, dim3((unsigned)grid), dim3(kBlock), 0, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz,
-                       tr.box, tr.N, list, n_list, t0, t1, n_pairs, n_blocks, partial);
+                       tr.box, tr.N, list, n_list, seg, t0, t1, n_pairs, n_blocks, partial);
     return hipGetLastError();
 }
 
-hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, const int *tg,
-                       int n_pairs, int n_blocks, int n_bins, int hist_mode, float bin_size, double *partial,
-                       unsigned long long *hist, hipStream_t st)
+hipError_t launch_disp(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0, const int *t1,
+                       const int *tg, int n_pairs, int n_blocks, int n_bins, int hist_mode, float bin_size,
+                       double *partial, unsigned long long *hist, hipStream_t st)
 {
     if (n_pairs <= 0 || n_blocks <= 0) return hipSuccess;
     (void)hipGetLastError();
     const long grid = (long)kXcds * ((n_blocks + kXcds - 1) / kXcds) * n_pairs;
     hipLaunchKernelGGL(disp_kernel, dim3((unsigned)grid), dim3(kBlock), sizeof(unsigned int) * (size_t)n_bins, st,
-                       tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, t0, t1, tg, n_pairs, n_blocks,
-                       n_bins, hist_mode, bin_size, partial, hist);
+                       tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box, tr.N, list, n_list, seg, t0, t1, tg, n_pairs,
+                       n_blocks, n_bins, hist_mode, bin_size, partial, hist);
     return hipGetLastError();
 }
 

@@ -282,7 +282,7 @@ template <int R, int MM, bool GROUPED>
 __global__ __launch_bounds__(kPhaseBlock, 4) void fs_t64_kernel(
     const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ Z,
     const int16_t *__restrict__ IX, const int16_t *__restrict__ IY, const int16_t *__restrict__ IZ,
-    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list,
+    const float *__restrict__ box, int N, const int *__restrict__ list, int n_list, const int2 *__restrict__ seg,
     const int *__restrict__ T0, const int *__restrict__ T1, int n_pairs, int n_chunks,
     const uint32_t *__restrict__ kpack, const int *__restrict__ kdst, const int4 *__restrict__ range, int K, int KB,
     double inv_ell, int per_split, int n_splits, double *__restrict__ out)
@@ -292,6 +292,11 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_t64_kernel(
     if (!xcd_item(n_chunks, n_pairs * n_splits, chunk, row)) return;
     const int pslot = row % n_pairs, split = row / n_pairs;
     const int t0 = T0[pslot], t1 = T1[pslot];
+    if (seg) {  // the pair's own atoms: list[seg.x .. seg.x + seg.y)
+        const int2 sg = seg[pslot];
+        list += sg.x;
+        n_list = sg.y;
+    }
     const int a_begin = split * per_split;
     const int a_end = min(n_list, a_begin + per_split);
     const DispFrac64 uf{X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N,
@@ -327,8 +332,9 @@ hipError_t launch_rho_t64_t(const TrajDev &tr, const int *list, int n_list, cons
 }
 
 template <int R, int MM, bool GROUPED>
-hipError_t launch_fs_t64_t(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1,
-                           int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out, hipStream_t st)
+hipError_t launch_fs_t64_t(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0,
+                           const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
+                           hipStream_t st)
 {
     const int per_split = (n_list + n_splits - 1) / n_splits;
     const size_t lds = Geom64<MM>::tab_bytes;
@@ -339,8 +345,8 @@ hipError_t launch_fs_t64_t(const TrajDev &tr, const int *list, int n_list, const
     dim3 grid((unsigned)(kXcds * kp.n_chunks * ((rows + kXcds - 1) / kXcds)));
     (void)hipGetLastError();
     hipLaunchKernelGGL(kfn, grid, dim3(kPhaseBlock), lds, st, tr.x, tr.y, tr.z, tr.ix, tr.iy, tr.iz, tr.box,
-                       tr.N, list, n_list, t0, t1, n_pairs, kp.n_chunks, kp.kpack, kp.kdst, kp.range, kp.K, kp.KB,
-                       inv_ell, per_split, n_splits, out);
+                       tr.N, list, n_list, seg, t0, t1, n_pairs, kp.n_chunks, kp.kpack, kp.kdst, kp.range, kp.K,
+                       kp.KB, inv_ell, per_split, n_splits, out);
     return hipGetLastError();
 }
 
@@ -358,14 +364,14 @@ hipError_t rho_t64_mm(const TrajDev &tr, const int *list, int n_list, const int 
 }
 
 template <int R, bool GROUPED>
-hipError_t fs_t64_mm(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, int n_pairs,
-                     const KPlan &kp, double inv_ell, int n_splits, double *out, hipStream_t st)
+hipError_t fs_t64_mm(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0, const int *t1,
+                     int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out, hipStream_t st)
 {
     switch (kp.MM) {
-    case 24: return launch_fs_t64_t<R, 24, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 50: return launch_fs_t64_t<R, 50, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 100: return launch_fs_t64_t<R, 100, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-    case 127: return launch_fs_t64_t<R, 127, GROUPED>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 24: return launch_fs_t64_t<R, 24, GROUPED>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 50: return launch_fs_t64_t<R, 50, GROUPED>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 100: return launch_fs_t64_t<R, 100, GROUPED>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+    case 127: return launch_fs_t64_t<R, 127, GROUPED>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -389,19 +395,20 @@ hipError_t launch_rho_t64(const TrajDev &tr, const int *list, int n_list, const 
     }
 }
 
-hipError_t launch_fs_t64(const TrajDev &tr, const int *list, int n_list, const int *t0, const int *t1, int n_pairs,
-                         const KPlan &kp, double inv_ell, int n_splits, double *out, hipStream_t st)
+hipError_t launch_fs_t64(const TrajDev &tr, const int *list, int n_list, const int2 *seg, const int *t0,
+                         const int *t1, int n_pairs, const KPlan &kp, double inv_ell, int n_splits, double *out,
+                         hipStream_t st)
 {
     switch (kp.R) {
     case 2:
         if (kp.pair) return hipErrorInvalidValue;
-        return fs_t64_mm<2, false>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+        return fs_t64_mm<2, false>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     case 4:
-        if (kp.pair) return fs_t64_mm<4, true>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-        return fs_t64_mm<4, false>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+        if (kp.pair) return fs_t64_mm<4, true>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+        return fs_t64_mm<4, false>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     case 8:
-        if (kp.pair) return fs_t64_mm<8, true>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
-        return fs_t64_mm<8, false>(tr, list, n_list, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+        if (kp.pair) return fs_t64_mm<8, true>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
+        return fs_t64_mm<8, false>(tr, list, n_list, seg, t0, t1, n_pairs, kp, inv_ell, n_splits, out, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -155,8 +155,13 @@ class Trajectory_List {
     int show_n_trajectories(int timeii) const;
     bool time_varying() const { return n_times_ > 0; }
     bool pair_keyed() const { return pair_keyed_; }
-    // Every list time's atoms, concatenated in list-time order.
-    const std::vector<int> &atoms() const { return atoms_; }
+    // Every list time's atoms, concatenated in list-time order (a device
+    // list's atoms are copied to the host on first use).
+    const std::vector<int> &atoms() const
+    {
+        load_atoms();
+        return atoms_;
+    }
     // The atoms at frame timeii, in list order.
     std::vector<int> atoms_at(int timeii) const;
     amdat_list *handle() const { return list_; }
@@ -167,8 +172,11 @@ class Trajectory_List {
     void write_count(const std::string &filename) const;
 
   private:
+    void load_atoms() const;
     System *system_;
-    std::vector<int> atoms_;
+    mutable std::vector<int> atoms_;
+    mutable bool atoms_loaded_ = true;
+    int n_entries_ = 0;
     int n_times_ = 0;
     bool pair_keyed_ = false;
     std::vector<int> conv_, off_;
@@ -198,10 +206,15 @@ class Trajectory_List_Bins {
     int show_n_xbins() const { return n_[0]; }
     int show_n_ybins() const { return n_[1]; }
     int show_n_zbins() const { return n_[2]; }
+    ~Trajectory_List_Bins();
+    Trajectory_List_Bins(const Trajectory_List_Bins &) = delete;
+    Trajectory_List_Bins &operator=(const Trajectory_List_Bins &) = delete;
     // The atoms of list l that lie in bin (x, y, z), frame by frame, in atom
     // order (Trajectory_List::operator&&, src/trajectory_list.cpp:518-):
-    // a time-varying list with one list time per frame.
+    // a time-varying list with one list time per frame, formed on the device.
     std::unique_ptr<Trajectory_List> intersect(const Trajectory_List *l, int xii, int yii, int zii) const;
+    // intersect for every bin, in bin order ((x * ny + y) * nz + z).
+    std::vector<std::unique_ptr<Trajectory_List>> intersect_all(const Trajectory_List *l) const;
     // Per bin, in bin order ((x * ny + y) * nz + z), the pair-keyed list of
     // Bin_Dynamics_Analysis (src/bin_dynamics_analysis.h:120-141): for
     // each frame pair of System::displacement_list(analysis) (fullblock 0,
@@ -215,7 +228,7 @@ class Trajectory_List_Bins {
                          const Trajectory_List *cluster, float thickness, int n_bins);
     System *system_;
     int n_[3];
-    std::vector<int> cell_;  // [T][N], -1 outside
+    amdat_cells *cells_ = nullptr;  // device cell table [T][N], -1 outside
 };
 
 class System {

@@ -89,8 +89,9 @@ void Control::for_each_target(const std::string &line,
             if (p.first == t[1]) bins = p.second.get();
         if (!bins) throw Error("Bin list '" + t[1] + "' not found.", 0);
         Trajectory_List *l = target("list " + t[2]);
-        std::vector<std::unique_ptr<Trajectory_List>> pair_lists;
-        if (persist) pair_lists = bins->intersect_pairs(l);
+        // every bin's list at once, formed on the device
+        std::vector<std::unique_ptr<Trajectory_List>> bin_lists =
+            persist ? bins->intersect_pairs(l) : bins->intersect_all(l);
         const char *sep = persist ? "." : "_";
         RewoundIndex sx, sy, sz;
         for (int x = 0; x < bins->show_n_xbins(); x++)
@@ -100,12 +101,7 @@ void Control::for_each_target(const std::string &line,
                     std::cout.flush();
                     const std::string suffix =
                         "_bin_" + sx.put(x) + sep + sy.put(y) + sep + sz.put(z) + ".bindata";
-                    if (persist) {
-                        fn(suffix, pair_lists[(x * bins->show_n_ybins() + y) * bins->show_n_zbins() + z].get());
-                    } else {
-                        auto bl = bins->intersect(l, x, y, z);
-                        fn(suffix, bl.get());
-                    }
+                    fn(suffix, bin_lists[(x * bins->show_n_ybins() + y) * bins->show_n_zbins() + z].get());
                 }
         std::cout << std::endl;
         return;

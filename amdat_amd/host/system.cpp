@@ -57,16 +57,26 @@ Trajectory_List::Trajectory_List(System *sys, std::vector<int> atoms) : system_(
 
 Trajectory_List::Trajectory_List(System *sys, amdat_list *timed) : system_(sys), list_(timed)
 {
-    int n = 0;
-    check(amdat_list_shape(timed, &n_times_, &n), "trajectory list");
-    atoms_.resize(n);
+    check(amdat_list_shape(timed, &n_times_, &n_entries_), "trajectory list");
     if (n_times_ > 0) {
         conv_.resize(sys->show_n_timesteps());
         off_.resize(n_times_ + 1);
+        check(amdat_list_download(device_context(), timed, conv_.data(), off_.data(), nullptr), "trajectory list");
+        atoms_loaded_ = false;  // the atoms stay on the device until asked for
+    } else {
+        atoms_.resize(n_entries_);
+        check(amdat_list_download(device_context(), timed, nullptr, nullptr, atoms_.empty() ? nullptr : atoms_.data()),
+              "trajectory list");
     }
-    check(amdat_list_download(device_context(), timed, conv_.empty() ? nullptr : conv_.data(),
-                              off_.empty() ? nullptr : off_.data(), atoms_.empty() ? nullptr : atoms_.data()),
+}
+
+void Trajectory_List::load_atoms() const
+{
+    if (atoms_loaded_) return;
+    atoms_.resize(n_entries_);
+    check(amdat_list_download(device_context(), list_, nullptr, nullptr, atoms_.empty() ? nullptr : atoms_.data()),
           "trajectory list");
+    atoms_loaded_ = true;
 }
 
 Trajectory_List::Trajectory_List(System *sys, amdat_list *paired, PairKeyed)
@@ -92,6 +102,7 @@ std::vector<int> Trajectory_List::atoms_at(int timeii) const
     if (timeii < 0 || timeii >= (int)conv_.size()) return {};
     const int c = conv_[timeii];
     if (c < 0 || c >= n_times_) return {};
+    load_atoms();
     return std::vector<int>(atoms_.begin() + off_[c], atoms_.begin() + off_[c + 1]);
 }
 
@@ -136,11 +147,11 @@ Trajectory_List_Bins::Trajectory_List_Bins(System *sys, int xbins, int ybins, in
 void Trajectory_List_Bins::assign(const std::vector<float> &lo, const std::vector<float> &hi)
 {
     if (n_[0] < 1 || n_[1] < 1 || n_[2] < 1) throw Error("bin counts must be at least 1", 0);
-    cell_.resize((size_t)system_->show_n_timesteps() * system_->show_n_atoms());
-    check(amdat_bins_assign(device_context(), system_->traj(), n_[0], n_[1], n_[2], lo.data(), hi.data(),
-                            cell_.data()),
+    check(amdat_cells_assign(device_context(), system_->traj(), n_[0], n_[1], n_[2], lo.data(), hi.data(), &cells_),
           "bins");
 }
+
+Trajectory_List_Bins::~Trajectory_List_Bins() { amdat_cells_free(device_context(), cells_); }
 
 // The bin cap of the point and cluster forms (src/trajectory_list_bins.cpp:
 // 487-517, 792-822): the largest half box diagonal over the frames, formed
@@ -212,37 +223,32 @@ void Trajectory_List_Bins::assign_distance(int kind, const float *point, int axi
                                            const Trajectory_List *binned, const Trajectory_List *cluster,
                                            float thickness, int n_bins)
 {
-    cell_.resize((size_t)system_->show_n_timesteps() * system_->show_n_atoms());
-    check(amdat_bins_distance(device_context(), system_->traj(), binned ? binned->handle() : nullptr, kind, point,
-                              axis, direction, cluster ? cluster->handle() : nullptr, thickness, n_bins,
-                              cell_.data()),
+    check(amdat_cells_distance(device_context(), system_->traj(), binned ? binned->handle() : nullptr, kind, point,
+                               axis, direction, cluster ? cluster->handle() : nullptr, thickness, n_bins, &cells_),
           "distance bins");
 }
 
 std::unique_ptr<Trajectory_List> Trajectory_List_Bins::intersect(const Trajectory_List *l, int xii, int yii,
                                                                  int zii) const
 {
-    const int T = system_->show_n_timesteps(), N = system_->show_n_atoms();
     const int bin = (xii * n_[1] + yii) * n_[2] + zii;
-    std::vector<int> conv(T), off(T + 1, 0), idx;
-    for (int t = 0; t < T; t++) {
-        conv[t] = t;
-        std::vector<int> a = l->atoms_at(t);
-        std::sort(a.begin(), a.end());
-        for (int j : a)
-            if (cell_[(size_t)t * N + j] == bin) idx.push_back(j);
-        off[t + 1] = (int)idx.size();
-    }
     amdat_list *h = nullptr;
-    check(amdat_list_create_timed(device_context(), system_->traj(), T, conv.data(), off.data(),
-                                  idx.empty() ? nullptr : idx.data(), &h),
-          "bin list");
+    check(amdat_cells_bin_lists(device_context(), cells_, l ? l->handle() : nullptr, bin, 1, &h), "bin list");
     return std::make_unique<Trajectory_List>(system_, h);
+}
+
+std::vector<std::unique_ptr<Trajectory_List>> Trajectory_List_Bins::intersect_all(const Trajectory_List *l) const
+{
+    const int nb = n_[0] * n_[1] * n_[2];
+    std::vector<amdat_list *> h(nb, nullptr);
+    check(amdat_cells_bin_lists(device_context(), cells_, l ? l->handle() : nullptr, 0, nb, h.data()), "bin list");
+    std::vector<std::unique_ptr<Trajectory_List>> out;
+    for (int c = 0; c < nb; c++) out.push_back(std::make_unique<Trajectory_List>(system_, h[c]));
+    return out;
 }
 
 std::vector<std::unique_ptr<Trajectory_List>> Trajectory_List_Bins::intersect_pairs(const Trajectory_List *l) const
 {
-    const int N = system_->show_n_atoms();
     const int nb = n_[0] * n_[1] * n_[2];
     std::vector<int> tg, t0, t1;
     system_->displacement_pair_list(false, tg, t0, t1);
@@ -252,27 +258,15 @@ std::vector<std::unique_ptr<Trajectory_List>> Trajectory_List_Bins::intersect_pa
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     const int nk = (int)keys.size();
-    std::vector<std::vector<int>> idx(nb), off(nb, std::vector<int>(nk + 1, 0));
-    for (int k = 0; k < nk; k++) {
-        const int a = keys[k].first, b = keys[k].second;
-        std::vector<int> at = l->atoms_at(a);
-        std::sort(at.begin(), at.end());
-        const int *ca = cell_.data() + (size_t)a * N, *cb = cell_.data() + (size_t)b * N;
-        for (int j : at)
-            if (ca[j] >= 0 && ca[j] == cb[j]) idx[ca[j]].push_back(j);
-        for (int c = 0; c < nb; c++) off[c][k + 1] = (int)idx[c].size();
-    }
     std::vector<int> k0(nk), k1(nk);
     for (int k = 0; k < nk; k++) k0[k] = keys[k].first, k1[k] = keys[k].second;
+    std::vector<amdat_list *> h(nb, nullptr);
+    check(amdat_cells_pair_lists(device_context(), cells_, l ? l->handle() : nullptr, nk, k0.data(), k1.data(), 0,
+                                 nb, h.data()),
+          "bin list");
     std::vector<std::unique_ptr<Trajectory_List>> out;
-    for (int c = 0; c < nb; c++) {
-        amdat_list *h = nullptr;
-        check(amdat_list_create_paired(device_context(), system_->traj(), nk, k0.data(), k1.data(), off[c].data(),
-                                       idx[c].empty() ? nullptr : idx[c].data(), &h),
-              "bin list");
-        out.push_back(std::make_unique<Trajectory_List>(system_, h, Trajectory_List::PairKeyed{}));
-        std::vector<int>().swap(idx[c]);
-    }
+    for (int c = 0; c < nb; c++)
+        out.push_back(std::make_unique<Trajectory_List>(system_, h[c], Trajectory_List::PairKeyed{}));
     return out;
 }
 

@@ -247,6 +247,38 @@ int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
                         const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
                         int n_bins, int *cell);
 
+/* Cell tables kept on the device: the cells of amdat_bins_assign
+ * (amdat_cells_assign, n_xbins*n_ybins*n_zbins bins) and of
+ * amdat_bins_distance (amdat_cells_distance, n_bins bins), same arguments,
+ * for forming per-bin lists without copying the [n_frames][n_atoms] table
+ * to the host.  amdat_cells_download copies it ([n_frames][n_atoms]). */
+typedef struct amdat_cells amdat_cells;
+int amdat_cells_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins,
+                       const float *lo, const float *hi, amdat_cells **out);
+int amdat_cells_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+                         const float *point, int axis, int direction, const amdat_list *cluster,
+                         float thickness, int n_bins, amdat_cells **out);
+int amdat_cells_download(amdat_ctx *ctx, const amdat_cells *cells, int *cell);
+int amdat_cells_free(amdat_ctx *ctx, amdat_cells *cells);
+
+/* Per-bin lists formed on the device, lists[b] for bin first_bin + b
+ * (b < n_bins):
+ *  amdat_cells_bin_lists: Trajectory_List_Bins intersected with list l
+ *    (Trajectory_List::operator&&, src/trajectory_list.cpp:518-; l NULL =
+ *    all atoms): a time-varying list with one list time per frame holding
+ *    the atoms of l at that frame that lie in the bin, in ascending index
+ *    order;
+ *  amdat_cells_pair_lists: the persistence form of Bin_Dynamics_Analysis
+ *    (src/bin_dynamics_analysis.h:120-141): a pair-keyed list (see
+ *    amdat_list_create_paired) with key k = (t0[k], t1[k]) holding the
+ *    atoms of l at t0 that lie in the bin at t0 and at t1, ascending.
+ * Each list is freed with amdat_list_free.  ERANGE: bin range outside the
+ * table or frame out of range; EINVAL: pair-keyed l, a pair given twice. */
+int amdat_cells_bin_lists(amdat_ctx *ctx, const amdat_cells *cells, const amdat_list *l, int first_bin,
+                          int n_bins, amdat_list **lists);
+int amdat_cells_pair_lists(amdat_ctx *ctx, const amdat_cells *cells, const amdat_list *l, int n_keys,
+                           const int *t0, const int *t1, int first_bin, int n_bins, amdat_list **lists);
+
 /* Static atom selection (Static_Trajectory_List, src/static_trajectory_
  * list.cpp:59-101): atom indices in list order.  indices == NULL with
  * n == n_atoms of the trajectory means "all atoms in order". */

@@ -134,6 +134,92 @@ def per_pair(plist, pairs, fn):
             yield fn((np.array([g]), np.array([a]), np.array([c])), idx[off[i]:off[i + 1]])
 
 
+def _list_atoms(kind, t, sub, per):
+    """Atoms of the test list at frame t, ascending (None: every atom)."""
+    if kind == "all":
+        return None
+    if kind == "static":
+        return np.sort(sub)
+    return np.sort(per[t]) if t in per else np.zeros(0, np.int64)
+
+
+def _host_bin_lists(cell, nbins, kind, sub, per):
+    T, N = cell.shape
+    out = []
+    for b in range(nbins):
+        off, idx = [0], []
+        for t in range(T):
+            a = _list_atoms(kind, t, sub, per)
+            a = np.nonzero(cell[t] == b)[0] if a is None else a[cell[t][a] == b]
+            idx.extend(a.tolist())
+            off.append(len(idx))
+        out.append((np.array(off, np.int32), np.array(idx, np.int32)))
+    return out
+
+
+def test_device_bin_lists_match_host(ctx):
+    """amdat_cells_bin_lists / amdat_cells_pair_lists (lists formed on the
+    device) against the same lists formed from the downloaded cell table:
+    rectangular and distance cells; every atom, a static subset in shuffled
+    list order, a time-varying list (frames without atoms included); a bin
+    range in the middle of the table."""
+    print("case: device-formed bin lists vs host formation, plain form", flush=True)
+    tr = shifted()
+    d = _abi.Trajectory(ctx, *tr.arrays)
+    lo, hi = bounds(tr)
+    rng = np.random.default_rng(7)
+    sub = rng.permutation(tr.N)[:371].astype(np.int32)      # list order is not index order
+    per = {t: rng.permutation(tr.N)[:rng.integers(0, 300)] for t in range(tr.T) if t % 7 != 3}
+    present = [t for t in range(tr.T) if t in per]
+    lt = {t: i for i, t in enumerate(present)}
+    conv = np.array([lt.get(t, -1) for t in range(tr.T)], np.int32)
+    toff = np.r_[0, np.cumsum([len(per[t]) for t in present])].astype(np.int32)
+    timed = _abi.TimedList(ctx, d, conv, toff, np.concatenate([per[t] for t in present]).astype(np.int32))
+    static = _abi.AtomList(ctx, d, sub)
+    tg, t0, t1 = _abi.ts_pairs(NEXP, NES, 0, 0, 0)
+    keys = sorted(set(zip(t0.tolist(), t1.tolist())))
+    k0 = np.array([k[0] for k in keys], np.int32)
+    k1 = np.array([k[1] for k in keys], np.int32)
+    tables = [("rect", _abi.Cells.assign(ctx, d, (3, 2, 2), lo, hi)),
+              ("dist", _abi.Cells.distance(ctx, d, None, _abi.DIST_POINT, 0.45, 9, point=(4.0, 4.0, 4.0)))]
+    for name, cells in tables:
+        cell = cells.download()
+        if name == "rect":
+            assert np.array_equal(cell, _abi.bins_assign(ctx, d, (3, 2, 2), lo, hi))
+        nb = cells.n_cells
+        for kind, dl in (("all", None), ("static", static), ("timed", timed)):
+            want = _host_bin_lists(cell, nb, kind, sub, per)
+            got = cells.bin_lists(dl)
+            assert len(got) == nb
+            for b in range(nb):
+                conv_b, off_b, idx_b = got[b].download()
+                assert np.array_equal(conv_b, np.arange(tr.T))
+                assert np.array_equal(off_b, want[b][0]), (name, kind, b)
+                assert np.array_equal(idx_b, want[b][1]), (name, kind, b)
+            part = cells.bin_lists(dl, first=1, n=nb - 2)
+            for b in range(nb - 2):
+                assert np.array_equal(part[b].download()[2], want[b + 1][1])
+            # persistence form: the atoms of the list at t0 in the bin at t0 and t1
+            pl = cells.pair_lists(k0, k1, dl)
+            for b in range(nb):
+                off_b, idx_b = pl[b].download()
+                woff, widx = [0], []
+                for a, c in keys:
+                    at = _list_atoms(kind, a, sub, per)
+                    m = (cell[a] == b) & (cell[c] == b)
+                    sel = np.nonzero(m)[0] if at is None else at[m[at]]
+                    widx.extend(sel.tolist())
+                    woff.append(len(widx))
+                assert np.array_equal(off_b, woff), (name, kind, b)
+                assert np.array_equal(idx_b, widx), (name, kind, b)
+        print("%s cells: %d bins, lists match" % (name, nb))
+    cells = tables[0][1]
+    with pytest.raises(_abi.AmdatError):
+        cells.bin_lists(None, first=10, n=5)                       # past the table
+    with pytest.raises(_abi.AmdatError):
+        cells.pair_lists([0, 0], [1, 1])                           # a pair given twice
+
+
 def test_pair_keyed_lists(ctx, qvec):
     print("case: pair-keyed bin lists at the C ABI, 2x2x2 bins", flush=True)
     tr = shifted()
