@@ -2546,11 +2546,13 @@ double fs_rows_bytes(const amdat_batch *batch, const JobParams &prm, int PC, int
 // Autotuner.  Starts from the cost model's settings `model` and keeps, for
 // the S(k) items and the Fs items separately, the fastest of:
 //  1. vectors per thread and plan mode: S(k) R = 2, 4, 8 unpaired, R = 4, 8
-//     paired and grouped; Fs R = 2, 4, 8.  Timed on about an eighth of the
-//     trajectories (every k-th by size): per-update speed.
-//  2. granularity, timed on every trajectory (the queue's balance depends
-//     on the whole job): S(k) frames per item x1/2, x1, x2; Fs pairs per
-//     item (whole gaps, 2, 1), then atom parts per pair chunk (1, 2, 4).
+//     paired and grouped, timed over every 8th frame of every trajectory
+//     (a sample of trajectories instead leaves too few items per block:
+//     its time measures the queue's tail, not the per-update speed); Fs R =
+//     2, 4, 8.
+//  2. granularity (the queue's balance depends on the whole job): S(k)
+//     frames per item x1/2, x1, x2; Fs pairs per item (whole gaps, 2, 1),
+//     then atom parts per pair chunk (1, 2, 4).
 // Each part is timed alone (S(k)-only or Fs-only jobs).  Finally the whole
 // job (with MSD) is timed with the model's and the tuned settings and the
 // faster is kept, so the tuned launch is never slower than the model's as
@@ -2558,14 +2560,7 @@ double fs_rows_bytes(const amdat_batch *batch, const JobParams &prm, int PC, int
 int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const JobChoice &model, TuneResult *res)
 {
     const int n = batch->n;
-    std::vector<int> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
-        return list_n(batch->trajs[a], batch->lists[a]) > list_n(batch->trajs[b], batch->lists[b]);
-    });
-    std::vector<char> sample(n, 0), all(n, 1);
-    const int step = std::max(1, n / 8);
-    for (int i = 0; i < n; i += step) sample[idx[i]] = 1;
+    std::vector<char> all(n, 1);
     auto time_of = [&](const JobParams &p, const std::vector<char> &use, const JobChoice &c, float *ms,
                        int *fg_out) -> int {
         JobCache jc;
@@ -2580,6 +2575,15 @@ int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const
         JobParams p = prm;
         p.fs_sh = nullptr;
         p.msd = false;
+        // per-update speed on every trajectory over every 8th frame: the
+        // queue keeps its shape (item count, sizes, balance), at an eighth
+        // of the work
+        JobParams ps = p;
+        if (ps.frames.size() >= 16) {
+            std::vector<int> fr;
+            for (size_t i = 0; i < ps.frames.size(); i += 8) fr.push_back(ps.frames[i]);
+            ps.frames.swap(fr);
+        }
         static const int cand[7][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {4, 4}, {8, 4}};
         float t_best = 1e30f;
         for (const auto &cd : cand) {
@@ -2588,7 +2592,7 @@ int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const
             c.SM = cd[1];
             c.FG = 0;
             float ms = 0.0f;
-            int rc = time_of(p, sample, c, &ms, nullptr);
+            int rc = time_of(ps, all, c, &ms, nullptr);
             if (rc) return rc;
             if (ms < t_best) { t_best = ms; best.RS = c.RS; best.SM = c.SM; }
         }
@@ -2620,7 +2624,7 @@ int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const
             c.PC = 0;
             c.AF = 1;
             float ms = 0.0f;
-            int rc = time_of(p, sample, c, &ms, nullptr);
+            int rc = time_of(p, all, c, &ms, nullptr);
             if (rc) return rc;
             if (ms < t_best) { t_best = ms; best.RF = R; }
         }

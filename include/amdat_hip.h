@@ -542,9 +542,9 @@ int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *p
  * by the same model, whole time gaps per Fs item, no atom parts.
  * AMDAT_RUN_TUNE times, once per argument set and separately for the S(k)
  * and the Fs items: vectors per thread and plan mode (k-vectors per LDS
- * tile: 1024, 2048 or 4096) on about an eighth of the trajectories, then on
- * all of them the S(k) frames per item (x1/2, x1, x2 the model's), the Fs
- * pairs per item (whole gaps, 2, 1) and Fs atom parts (1, 2, 4); it keeps
+ * tile: 1024, 2048 or 4096; S(k) over every 8th frame), then the S(k)
+ * frames per item (x1/2, x1, x2 the model's), the Fs pairs per item (whole
+ * gaps, 2, 1) and Fs atom parts (1, 2, 4); it keeps
  * the tuned settings only when the whole job runs no slower with them than
  * with the model's (amdat_batch_job_settings reports both timings).  S(k)
  * items never split a frame's atoms (|rho|^2 is not additive over atom
