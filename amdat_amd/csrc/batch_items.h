@@ -61,8 +61,9 @@ __device__ __forceinline__ void fs_item(const BatchTraj &t, int chunk, int r, in
     double *row = t.out + (size_t)r * t.K;
     const int p_begin = pc_off[pc], p_end = pc_off[pc + 1];
     for (int pq = p_begin; pq < p_end; ++pq) {
-        const int p = order[pq];
-        const int t0 = T0[p], t1 = T1[p];
+        // the same pair in every lane: scalar registers
+        const int p = __builtin_amdgcn_readfirstlane(order[pq]);
+        const int t0 = __builtin_amdgcn_readfirstlane(T0[p]), t1 = __builtin_amdgcn_readfirstlane(T1[p]);
         double sum[R];
         fs_block<R, MM>(t.x, t.y, t.z, t.ix, t.iy, t.iz, t.box + 3 * t0, t.box + 3 * t1, t.list, (size_t)t0 * t.N,
                         (size_t)t1 * t.N, a_begin, a_end, t.inv_ell, kp.kpack + (size_t)chunk * kp.KB,

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void job_kernel(BatchJob j)
     for (;;) {
         const int q = next_item(j.counter, &s_item);
         if (q >= j.n_items) break;
-        const int4 it = j.items[q];
+        const int4 it = uniform_item(j.items, q);
         if (it.w == 0)
             sk_item<RS, MM, PS>(j.sk_trajs[it.x], it.y, it.z, j.frames, j.n_frames, j.group_size, j.sk, tab, anc,
                                 sU, sValid);

@@ -90,6 +90,9 @@ __device__ __forceinline__ int range_gs_code(int4 r, int wave) { return (int)(((
 // tile ahead by fetch_tile) and computing PER anchors d = lane_in_row * PER
 // + q.  Needs no barrier before it: it runs after the previous tile's group
 // stage has read anc and sU.
+// Lanes are dealt to rows axis-major (lane e: axis c = e / (LANES A), atom
+// i), so the axis is uniform within a wave: the coordinate array is chosen
+// by a scalar and the loads of a wave are contiguous.
 template <int MM>
 struct AnchorLayout {
     static constexpr int NR = 3 * Geom<MM>::A;                 // table rows of a tile
@@ -97,7 +100,18 @@ struct AnchorLayout {
     static constexpr int LANES = 8 / PER;                        // lanes per row
     static constexpr int N = NR * LANES;                         // lanes with work
     static_assert(N <= kPhaseBlock, "one anchor item per lane");
+    static_assert(Geom<MM>::A % (64 / LANES) == 0, "one axis per wave");
 };
+
+// Axis (wave-uniform) and atom of anchor lane e.
+template <int MM>
+__device__ __forceinline__ void anchor_row(int e, int &c, int &i)
+{
+    using AL = AnchorLayout<MM>;
+    const int rr = e / AL::LANES;
+    c = __builtin_amdgcn_readfirstlane(rr / Geom<MM>::A);
+    i = rr - c * Geom<MM>::A;
+}
 
 // Issue the loads of the tile starting at a0 (the lanes of the anchor
 // stage, one row each); the values are consumed by tile_anchors after the
@@ -106,8 +120,11 @@ template <int MM, typename UF>
 __device__ __forceinline__ void fetch_tile(const UF &uf, int a0, int a_end, typename UF::Raw &raw)
 {
     using AL = AnchorLayout<MM>;
-    const int e = threadIdx.x, row = e / AL::LANES, i = row / 3, c = row - 3 * (row / 3);
-    if (e < AL::N && a0 + i < a_end) raw = uf.load(a0 + i, c);
+    const int e = threadIdx.x;
+    if (e >= AL::N) return;  // whole waves
+    int c, i;
+    anchor_row<MM>(e, c, i);
+    if (a0 + i < a_end) raw = uf.load(a0 + i, c);
 }
 
 template <int MM, typename UF>
@@ -116,9 +133,10 @@ __device__ __forceinline__ void tile_anchors(const UF &uf, int a0, int a_end, co
 {
     using AL = AnchorLayout<MM>;
     const int e = threadIdx.x;
-    if (e >= AL::N) return;
-    const int row = e / AL::LANES, q0 = (e - row * AL::LANES) * AL::PER;
-    const int i = row / 3, c = row - 3 * (row / 3);
+    if (e >= AL::N) return;  // whole waves
+    int c, i;
+    anchor_row<MM>(e, c, i);
+    const int row = 3 * i + c, q0 = (e % AL::LANES) * AL::PER;
     const bool valid = a0 + i < a_end;
     const uint32_t u = valid ? uf.convert(raw, c) : 0u;
 #pragma unroll
@@ -335,8 +353,8 @@ struct CoordFrac {
 };
 
 // Fixed-point fraction of a displacement dr = (x1 + i1*L1) - (x0 + i0*L0),
-// rebuilt exactly in FP64 (Fs); the pair's box lengths L0/L1 are loaded at
-// construction.
+// rebuilt exactly in FP64 (Fs); the pair's box lengths are held by value
+// (an aggregate, so that the functor lives in registers).
 struct DispFrac {
     const float *X, *Y, *Z;
     const int16_t *IX, *IY, *IZ;
@@ -348,19 +366,13 @@ struct DispFrac {
         float x0, x1;
         uint32_t i01;  // i0 in the low 16 bits, i1 in the high
     };
-    __device__ DispFrac(const float *X_, const float *Y_, const float *Z_, const int16_t *IX_, const int16_t *IY_,
-                        const int16_t *IZ_, const float *L0, const float *L1, const int *list_, size_t o0_,
-                        size_t o1_, double inv_ell_)
-        : X(X_), Y(Y_), Z(Z_), IX(IX_), IY(IY_), IZ(IZ_), list(list_), o0(o0_), o1(o1_), inv_ell(inv_ell_),
-          l0x(L0[0]), l0y(L0[1]), l0z(L0[2]), l1x(L1[0]), l1y(L1[1]), l1z(L1[2])
-    {
-    }
+    // c is wave-uniform.
     __device__ __forceinline__ Raw load(int a, int c) const
     {
-        const int j = list ? list[a] : a;
-        const size_t p0 = o0 + j, p1 = o1 + j;
         const float *P = c == 0 ? X : (c == 1 ? Y : Z);
         const int16_t *I = c == 0 ? IX : (c == 1 ? IY : IZ);
+        const int j = list ? list[a] : a;
+        const size_t p0 = o0 + j, p1 = o1 + j;
         return Raw{P[p0], P[p1], (uint32_t)(uint16_t)I[p0] | ((uint32_t)(uint16_t)I[p1] << 16)};
     }
     __device__ __forceinline__ uint32_t convert(const Raw &r, int c) const
@@ -405,7 +417,7 @@ __device__ __forceinline__ void fs_block(const float *__restrict__ X, const floa
                                          double inv_ell, const uint32_t *__restrict__ kp, int4 rng, float2 *tab,
                                          float2 *anc, uint32_t *sU, int *sValid, double (&s)[R])
 {
-    const DispFrac uf(X, Y, Z, IX, IY, IZ, L0, L1, list, o0, o1, inv_ell);
+    const DispFrac uf{X, Y, Z, IX, IY, IZ, list, o0, o1, inv_ell, L0[0], L0[1], L0[2], L1[0], L1[1], L1[2]};
     double unused[R];
     phase_tiles<R, MM, true, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, s, unused);
 }
@@ -456,6 +468,15 @@ __device__ __forceinline__ bool xcd_atom_block(int n_blocks, int n_pairs, int &b
     __shared__ float2 anc[3 * Geom<MM>::A * 8];          \
     __shared__ uint32_t sU[3 * Geom<MM>::A];           \
     __shared__ int sValid[Geom<MM>::A]
+
+// A work item read by every lane of a wave (the same address), kept in
+// scalar registers.
+__device__ __forceinline__ int4 uniform_item(const int4 *__restrict__ items, int q)
+{
+    const int4 it = items[q];
+    return make_int4(__builtin_amdgcn_readfirstlane(it.x), __builtin_amdgcn_readfirstlane(it.y),
+                     __builtin_amdgcn_readfirstlane(it.z), __builtin_amdgcn_readfirstlane(it.w));
+}
 
 // Next work item of a persistent launch: thread 0 draws from the queue
 // counter and the block reads the value from LDS.

@@ -381,6 +381,12 @@ def bench_extras(args):
     # (FP32 xyz + int16 images), 16 B per atom-frame for the bins
     fr_all = len(np.unique(np.concatenate([t0, t1])))
     fr_gap = len(np.unique(np.concatenate([g0, g1])))
+    # persistence form of 2x2x2 bins: one key per distinct frame pair
+    keys = sorted(set(zip(t0.tolist(), t1.tolist())))
+    k0 = np.array([k[0] for k in keys], np.int32)
+    k1 = np.array([k[1] for k in keys], np.int32)
+    bin_cells = _abi.Cells.assign(ctx, d, (2, 2, 2), lo, hi)
+    bin_pl = bin_cells.pair_lists(k0, k1)
     calls = {
         "msd_ngp": (lambda: _abi.msd_ngp(ctx, d, None, P), 18.0 * N * fr_all),
         "displacements (100 van Hove bins)": (lambda: _abi.displacements(ctx, d, None, P, 100, 3.0),
@@ -395,6 +401,13 @@ def bench_extras(args):
                                                            point=(Lh, Lh, Lh)), 16.0 * N * T),
         "bins_distance list (1,000 atoms)": (lambda: _abi.bins_distance(ctx, d, None, _abi.DIST_LIST, 0.25, 16,
                                                                         cluster=cluster), 16.0 * N * T),
+        "cells_assign + bin_lists (2x2x2, device)": (
+            lambda: [b.close() for b in _abi.Cells.assign(ctx, d, (2, 2, 2), lo, hi).bin_lists()], 16.0 * N * T),
+        "cells_assign + pair_lists (2x2x2, persistence keys)": (
+            lambda: [b.close() for b in _abi.Cells.assign(ctx, d, (2, 2, 2), lo, hi).pair_lists(k0, k1)],
+            8.0 * N * len(k0) + 4.0 * N * T),
+        "msd_ngp on one pair-keyed bin list (one launch)": (lambda: _abi.msd_ngp(ctx, d, bin_pl[0], P),
+                                                             18.0 * N * fr_all / 8),
     }
     res = {}
     for name, (fn, nbytes) in calls.items():
