@@ -20,7 +20,7 @@ namespace {
 template <int R, int MM, bool GROUPED = false>
 __device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, const int *__restrict__ frames,
                                         int n_frames, int group_size, const KPlan &kp, float2 *tab, float2 *anc,
-                                        uint32_t *sU, int *sValid)
+                                        uint32_t *sU)
 {
     const int f_begin = g * group_size, f_end = min(n_frames, f_begin + group_size);
     const int *kd = kp.kdst + (size_t)chunk * kp.KB;
@@ -28,7 +28,7 @@ __device__ __forceinline__ void sk_item(const BatchTraj &t, int chunk, int g, co
     for (int f = f_begin; f < f_end; ++f) {
         double re[R], im[R];
         rho_block<R, MM, GROUPED>(t.x, t.y, t.z, t.list, (size_t)frames[f] * (size_t)t.N, 0, t.n_list, t.inv_ell,
-                                  kp.kpack + (size_t)chunk * kp.KB, kp.range[chunk], tab, anc, sU, sValid, re, im);
+                                  kp.kpack + (size_t)chunk * kp.KB, kp.range[chunk], tab, anc, sU, re, im);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int d = kd[r * kPhaseBlock + threadIdx.x];
@@ -52,7 +52,7 @@ template <int R, int MM>
 __device__ __forceinline__ void fs_item(const BatchTraj &t, int chunk, int r, int split, const int *__restrict__ T0,
                                         const int *__restrict__ T1, const int *__restrict__ pc_off,
                                         const int *__restrict__ order, const KPlan &kp, float2 *tab, float2 *anc,
-                                        uint32_t *sU, int *sValid)
+                                        uint32_t *sU)
 {
     const int pc = r / split, s = r - (r / split) * split;
     const int a_begin = (int)(((long long)s * t.n_list) / split);
@@ -67,7 +67,7 @@ __device__ __forceinline__ void fs_item(const BatchTraj &t, int chunk, int r, in
         double sum[R];
         fs_block<R, MM>(t.x, t.y, t.z, t.ix, t.iy, t.iz, t.box + 3 * t0, t.box + 3 * t1, t.list, (size_t)t0 * t.N,
                         (size_t)t1 * t.N, a_begin, a_end, t.inv_ell, kp.kpack + (size_t)chunk * kp.KB,
-                        kp.range[chunk], tab, anc, sU, sValid, sum);
+                        kp.range[chunk], tab, anc, sU, sum);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int d = kd[q * kPhaseBlock + threadIdx.x];

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void sk_batch_kernel(const BatchTra
         const int q = next_item(counter, &s_item);
         if (q >= n_items) break;
         const int4 it = uniform_item(items, q);
-        sk_item<R, MM, GROUPED>(trajs[it.x], it.y, it.z, frames, n_frames, group_size, kp, tab, anc, sU, sValid);
+        sk_item<R, MM, GROUPED>(trajs[it.x], it.y, it.z, frames, n_frames, group_size, kp, tab, anc, sU);
     }
 }
 
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_batch_kernel(const BatchTra
         const int q = next_item(counter, &s_item);
         if (q >= n_items) break;
         const int4 it = uniform_item(items, q);
-        fs_item<R, MM>(trajs[it.x], it.y, it.z, 1, T0, T1, group_off, order, kp, tab, anc, sU, sValid);
+        fs_item<R, MM>(trajs[it.x], it.y, it.z, 1, T0, T1, group_off, order, kp, tab, anc, sU);
     }
 }
 

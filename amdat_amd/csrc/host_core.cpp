@@ -672,6 +672,50 @@ void fill_padding(std::vector<std::vector<Vec>> &units, size_t h0, int R, int GS
     }
 }
 
+// Waves w and w + 4 of a 512-thread block run on one SIMD (measured on
+// gfx950 from HW_ID: in every block the two share a SIMD, while which SIMD
+// wave 0 lands on varies).  Waves of different group sizes differ in work
+// per atom, R ((c - 4) + 4 / GS) VALU ops per lane, and the block meets a
+// barrier every tile, so its tile time is that of its busiest SIMD.  Per
+// chunk, the active waves (a prefix: the kernels skip waves past the
+// chunk's units) are permuted so that the largest SIMD load is smallest;
+// ties keep the first order found, so the layout stays deterministic.
+void balance_simds(std::vector<std::vector<Vec>> &units, std::vector<char> &unit_g, int n_units, int R, double c)
+{
+    const int n_chunks = (int)(units.size() / 512);
+    for (int ch = 0; ch < n_chunks; ch++) {
+        const int n_act = std::min(512, n_units - ch * 512);
+        const int a = (n_act + 63) / 64;
+        if (a <= 4) continue;  // one wave per SIMD at most
+        double load[8];
+        for (int w = 0; w < a; w++) load[w] = R * ((c - 4.0) + 4.0 / unit_g[(size_t)ch * 512 + w * 64]);
+        std::array<int, 8> perm{}, best{};
+        for (int w = 0; w < 8; w++) perm[w] = best[w] = w;
+        auto worst = [&](const std::array<int, 8> &p) {
+            double m = 0.0;
+            for (int s = 0; s < 4; s++) m = std::max(m, load[p[s]] + (s + 4 < a ? load[p[s + 4]] : 0.0));
+            return m;
+        };
+        double best_cost = worst(perm);
+        while (std::next_permutation(perm.begin(), perm.begin() + a)) {
+            const double cst = worst(perm);
+            if (cst < best_cost - 1e-9) { best_cost = cst; best = perm; }
+        }
+        std::vector<std::vector<Vec>> moved((size_t)a * 64);
+        std::vector<char> moved_g((size_t)a * 64);
+        for (int w = 0; w < a; w++)
+            for (int l = 0; l < 64; l++) {
+                const size_t src = (size_t)ch * 512 + (size_t)best[w] * 64 + l;
+                moved[(size_t)w * 64 + l] = std::move(units[src]);
+                moved_g[(size_t)w * 64 + l] = unit_g[src];
+            }
+        for (size_t i = 0; i < moved.size(); i++) {
+            units[(size_t)ch * 512 + i] = std::move(moved[i]);
+            unit_g[(size_t)ch * 512 + i] = moved_g[i];
+        }
+    }
+}
+
 }  // namespace
 
 PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c, bool place)
@@ -860,6 +904,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
     } else if (!grouped) {
         for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) order_half_wave(units, h0, R, unit_g[h0], E);
     }
+    if (grouped || mixed || quad) balance_simds(units, unit_g, n_units, R, c);
     // emit
     const size_t n = (size_t)L.n_chunks * L.KB;
     L.pack.resize(n);

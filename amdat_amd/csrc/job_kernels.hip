@@ -26,10 +26,10 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void job_kernel(BatchJob j)
         const int4 it = uniform_item(j.items, q);
         if (it.w == 0)
             sk_item<RS, MM, PS>(j.sk_trajs[it.x], it.y, it.z, j.frames, j.n_frames, j.group_size, j.sk, tab, anc,
-                                sU, sValid);
+                                sU);
         else if (it.w == 1)
             fs_item<RF, MM>(j.fs_trajs[it.x], it.y, it.z, j.fs_split, j.t0, j.t1, j.fs_pc, j.order, j.fs, tab, anc,
-                            sU, sValid);
+                            sU);
         else
             msd_item(j.msd_trajs[it.x], it.x, it.y, j.ntg, j.t0, j.t1, j.group_off, j.order, j.s2, j.s4, red);
     }

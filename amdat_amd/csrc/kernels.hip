@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void rho_c32_kernel(
     const int a_end = min(n_list, a_begin + per_split);
     double re[R], im[R];
     rho_block<R, MM, GROUPED>(X, Y, Z, list, (size_t)frames[fslot] * (size_t)N, a_begin, a_end, inv_ell,
-                              kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid, re, im);
+                              kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, re, im);
     rho_write<R>(kdst + (size_t)chunk * KB, re, im, out + ((size_t)fslot * n_splits + split) * (size_t)K);
 }
 
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kPhaseBlock, 4) void fs_c32_kernel(
     const int a_end = min(n_list, a_begin + per_split);
     double sum[R];
     fs_block<R, MM, GROUPED>(X, Y, Z, IX, IY, IZ, box + 3 * t0, box + 3 * t1, list, (size_t)t0 * N, (size_t)t1 * N,
-                             a_begin, a_end, inv_ell, kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU, sValid,
+                             a_begin, a_end, inv_ell, kpack + (size_t)chunk * KB, range[chunk], tab, anc, sU,
                              sum);
     fs_write<R>(kdst + (size_t)chunk * KB, sum, out + ((size_t)pslot * n_splits + split) * (size_t)K);
 }

@@ -34,9 +34,9 @@ namespace amdat {
 namespace {
 
 constexpr int kWaves = kPhaseBlock / 64;
-// Prefetch depth (updates in flight) of the hot loops.  (Fs with 8 in
-// flight ran no faster on the 64,000-atom workload and spills in its
-// unpaired 32-atom-tile variant.)
+// Prefetch depth (updates in flight) of the hot loops.  (Fs with 6 or 8 in
+// flight ran no faster on the 64,000-atom workload: 244 and 241 ms against
+// 243 ms.)
 constexpr int kDepth = 4;
 
 // Table geometry for half-width MM: the sections are padded to whole
@@ -84,12 +84,13 @@ __device__ __forceinline__ int range_units(int4 r) { return r.w & 0xFFFF; }
 __device__ __forceinline__ int range_gs_code(int4 r, int wave) { return (int)(((unsigned)r.w >> (16 + 2 * wave)) & 3u); }
 
 // Anchor stage of a tile: the fixed-point fractions sU[3A] (row 3i+c =
-// atom i, axis c; invalid atoms carry sValid[i] == 0 and u = 0) and the
-// exact small anchors e(d), d < 8, into anc[d][3A].  Row r is handled by
-// LANES consecutive lanes, each converting the row's coordinate (loaded one
-// tile ahead by fetch_tile) and computing PER anchors d = lane_in_row * PER
-// + q.  Needs no barrier before it: it runs after the previous tile's group
-// stage has read anc and sU.
+// atom i, axis c) and the exact small anchors e(d), d < 8, into
+// anc[d][3A]; atoms past the split's end get zero anchors, so every entry
+// the group stage forms for them is zero and the hot loop adds nothing for
+// them.  Row r is handled by LANES consecutive lanes, each converting the
+// row's coordinate (loaded one tile ahead by fetch_tile) and computing PER
+// anchors d = lane_in_row * PER + q.  Needs no barrier before it: it runs
+// after the previous tile's group stage has read anc and sU.
 // Lanes are dealt to rows axis-major (lane e: axis c = e / (LANES A), atom
 // i), so the axis is uniform within a wave: the coordinate array is chosen
 // by a scalar and the loads of a wave are contiguous.
@@ -124,12 +125,19 @@ __device__ __forceinline__ void fetch_tile(const UF &uf, int a0, int a_end, type
     if (e >= AL::N) return;  // whole waves
     int c, i;
     anchor_row<MM>(e, c, i);
-    if (a0 + i < a_end) raw = uf.load(a0 + i, c);
+    if (a0 + i < a_end) {
+        // one branch per axis (c is wave-uniform), each reading its fields
+        // by name: selecting a field by c would index the functor in
+        // memory, a per-lane stack copy
+        if (c == 0) raw = uf.template load<0>(a0 + i);
+        else if (c == 1) raw = uf.template load<1>(a0 + i);
+        else raw = uf.template load<2>(a0 + i);
+    }
 }
 
 template <int MM, typename UF>
 __device__ __forceinline__ void tile_anchors(const UF &uf, int a0, int a_end, const typename UF::Raw &raw,
-                                             float2 *anc, uint32_t *sU, int *sValid)
+                                             float2 *anc, uint32_t *sU)
 {
     using AL = AnchorLayout<MM>;
     const int e = threadIdx.x;
@@ -138,13 +146,14 @@ __device__ __forceinline__ void tile_anchors(const UF &uf, int a0, int a_end, co
     anchor_row<MM>(e, c, i);
     const int row = 3 * i + c, q0 = (e % AL::LANES) * AL::PER;
     const bool valid = a0 + i < a_end;
-    const uint32_t u = valid ? uf.convert(raw, c) : 0u;
+    uint32_t u = 0u;
+    if (valid) u = c == 0 ? uf.template convert<0>(raw) : (c == 1 ? uf.template convert<1>(raw) : uf.template convert<2>(raw));
 #pragma unroll
-    for (int q = 0; q < AL::PER; ++q) anc[(q0 + q) * AL::NR + row] = sincos_turns(u * (uint32_t)(q0 + q));
-    if (q0 == 0) {
-        sU[row] = u;
-        if (c == 0) sValid[i] = valid;
+    for (int q = 0; q < AL::PER; ++q) {
+        const float2 a = sincos_turns(u * (uint32_t)(q0 + q));
+        anc[(q0 + q) * AL::NR + row] = valid ? a : make_float2(0.0f, 0.0f);
     }
+    if (q0 == 0) sU[row] = u;
 }
 
 // Group stage of a tile, over the entries the chunk reads only: x in
@@ -154,43 +163,89 @@ __device__ __forceinline__ void tile_anchors(const UF &uf, int a0, int a_end, co
 // conjugates at -m (the sections are padded to whole groups, so no store
 // needs a range test).  Neighbouring lanes work on neighbouring atoms,
 // whose rows start W entries apart, so their stores and anchor reads fall
-// in distinct banks.  Runs after a barrier that follows tile_anchors.
-template <int MM>
-__device__ __forceinline__ void build_groups(float2 *tab, const float2 *anc, const uint32_t *sU, const int *sValid,
-                                             int4 rng)
+// in distinct banks.
+// The items of a thread are the same for every tile of the block's chunk.
+// Item it = tid + 512 n covers every group of the chunk's sections within
+// two rounds (n < 2, checked by the static_assert below); its word packs
+// bits 0-13 the index (float2) of entry +8j in the tile table, i W +
+// section offset + 8j, bits 14-20 the row 3i + c, bits 21-25 j and bit 26
+// (conjugates written at -m, on x and y), or is all ones for no item.  The
+// first round's word is formed once per block and kept; the second round,
+// which only chunks with wide sections reach, is formed per tile.
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+
+// Block-uniform section counts of a chunk: groups on x from jx0, then y, z.
+struct BuildRange {
+    int jx0, njx, njy, njz;
+    __device__ __forceinline__ int n_groups() const { return njx + njy + njz; }
+};
+
+__device__ __forceinline__ BuildRange build_range(int4 rng)
 {
-    using Gm = Geom<MM>;
-    constexpr int NR = 3 * Gm::A;  // table rows of a tile
-    const int tid = threadIdx.x;
     const int xlo = rng.x, xhi = rng.y, my = range_my(rng), mz = range_mz(rng);
     // |m| needed on x: [xa, xb]
     const int xa = xlo >= 0 ? xlo : (xhi <= 0 ? -xhi : 0);
     const int xb = max(abs(xlo), abs(xhi));
-    const int jx0 = xa >> 3, njx = (xb >> 3) - jx0 + 1;
-    const int njy = (my >> 3) + 1, njz = (mz >> 3) + 1;
-    const int nj = njx + njy + njz;
-    for (int it = tid; it < nj << Gm::LOG2A; it += kPhaseBlock) {
-        const int i = it & (Gm::A - 1), k = it >> Gm::LOG2A;
-        const int c = k < njx ? 0 : (k < njx + njy ? 1 : 2);
-        const int j = c == 0 ? jx0 + k : (c == 1 ? k - njx : k - njx - njy);
-        const int off = c == 0 ? Gm::OX : (c == 1 ? Gm::OY : Gm::OZ);
-        const int row = 3 * i + c;
-        float2 a8 = sincos_turns(sU[row] * (uint32_t)(8 * j));
-        if (!sValid[i]) a8 = make_float2(0.0f, 0.0f);
-        // entries m = 8j + d (d = 0..7) and, on x and y, -m: all inside the
-        // section (8j + 7 <= H), each written by this lane only; e(0) is
-        // written last so it keeps a +0 imaginary part
-        float2 v[8];
+    BuildRange br;
+    br.jx0 = xa >> 3;
+    br.njx = (xb >> 3) - br.jx0 + 1;
+    br.njy = (my >> 3) + 1;
+    br.njz = (mz >> 3) + 1;
+    return br;
+}
+
+template <int MM>
+__device__ __forceinline__ uint32_t build_item(const BuildRange &br, int it)
+{
+    using Gm = Geom<MM>;
+    static_assert(3 * (Gm::H / 8 + 1) * Gm::A <= 2 * kPhaseBlock, "two rounds of build items");
+    static_assert(Gm::A * Gm::W < (1 << 14) && 3 * Gm::A < (1 << 7) && Gm::H / 8 < (1 << 5), "item packing");
+    if (it >= br.n_groups() << Gm::LOG2A) return kNoItem;
+    const int i = it & (Gm::A - 1), k = it >> Gm::LOG2A;
+    const bool cx = k < br.njx, cy = k < br.njx + br.njy;
+    const int j = cx ? br.jx0 + k : (cy ? k - br.njx : k - br.njx - br.njy);
+    const int off = cx ? Gm::OX : (cy ? Gm::OY : Gm::OZ);
+    const int c = cx ? 0 : (cy ? 1 : 2);
+    return (uint32_t)(i * Gm::W + off + 8 * j) | ((uint32_t)(3 * i + c) << 14) | ((uint32_t)j << 21) |
+           (cy ? (1u << 26) : 0u);
+}
+
+template <int MM>
+__device__ __forceinline__ void build_one(float2 *tab, const float2 *anc, const uint32_t *sU, uint32_t w)
+{
+    constexpr int NR = 3 * Geom<MM>::A;  // table rows of a tile
+    // an empty asm that "changes" w: the addresses below are unpacked each
+    // tile instead of being hoisted out of the tile loop, where they would
+    // hold eight more registers through the hot loop
+    asm volatile("" : "+v"(w));
+    const int up = (int)(w & 0x3FFFu), row = (int)((w >> 14) & 0x7Fu), j = (int)((w >> 21) & 0x1Fu);
+    const float2 a8 = sincos_turns(sU[row] * (uint32_t)(8 * j));
+    // entries m = 8j + d (d = 0..7) and, on x and y, -m: all inside the
+    // section (8j + 7 <= H), each written by this lane only; e(0) is
+    // written last so it keeps a +0 imaginary part
+    float2 v[8];
 #pragma unroll
-        for (int d = 0; d < 8; ++d) v[d] = cmul(a8, anc[d * NR + row]);
-        float2 *up = tab + i * Gm::W + off + 8 * j;
-        if (c != 2) {
-            float2 *dn = tab + i * Gm::W + off - 8 * j - 7;
+    for (int d = 0; d < 8; ++d) v[d] = cmul(a8, anc[d * NR + row]);
+    if (w & (1u << 26)) {
+        float2 *dn = tab + up - 16 * j - 7;
 #pragma unroll
-            for (int d = 0; d < 8; ++d) dn[7 - d] = make_float2(v[d].x, -v[d].y);
-        }
+        for (int d = 0; d < 8; ++d) dn[7 - d] = make_float2(v[d].x, -v[d].y);
+    }
+    float2 *u = tab + up;
 #pragma unroll
-        for (int d = 0; d < 8; ++d) up[d] = v[d];
+    for (int d = 0; d < 8; ++d) u[d] = v[d];
+}
+
+template <int MM>
+__device__ __forceinline__ void build_groups(float2 *tab, const float2 *anc, const uint32_t *sU,
+                                             const BuildRange &br, uint32_t item0)
+{
+    if (item0 != kNoItem) build_one<MM>(tab, anc, sU, item0);
+    // second round: whole waves skip it unless the chunk's sections need it
+    const int it1 = threadIdx.x + kPhaseBlock;
+    if (__builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u)) + kPhaseBlock < (br.n_groups() << Geom<MM>::LOG2A)) {
+        const uint32_t w1 = build_item<MM>(br, it1);
+        if (w1 != kNoItem) build_one<MM>(tab, anc, sU, w1);
     }
 }
 
@@ -252,7 +307,7 @@ __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (
 // 4 + 4/GS VALU per update (8 for GS = 1, 4.5 for GS = 8), Fs 2 + 4/GS.
 template <int R, int MM, bool FS, bool GROUPED, typename UF>
 __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
-                                            float2 *tab, float2 *anc, uint32_t *sU, int *sValid, const UF uf,
+                                            float2 *tab, float2 *anc, uint32_t *sU, const UF uf,
                                             double (&s0)[R], double (&s1)[R])
 {
     using Gm = Geom<MM>;
@@ -279,9 +334,11 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
     // Tile pipeline, two barriers per tile: the coordinates of tile t+1 are
     // loaded before tile t's hot loop and its anchors computed after it, so
     // neither the load latency nor the anchor stage waits at a barrier.
+    const BuildRange br = build_range(rng);
+    const uint32_t item0 = build_item<MM>(br, threadIdx.x);
     typename UF::Raw raw{};
     fetch_tile<MM>(uf, a_begin, a_end, raw);
-    tile_anchors<MM>(uf, a_begin, a_end, raw, anc, sU, sValid);
+    tile_anchors<MM>(uf, a_begin, a_end, raw, anc, sU);
     if (a_begin + Gm::A < a_end) fetch_tile<MM>(uf, a_begin + Gm::A, a_end, raw);
     // Groups of FT tiles; the FP32 partials of a group are added into the
     // FP64 sums once, after its last tile: every 32 atoms for rho
@@ -295,7 +352,7 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         const int g_end = min(a_end, g0 + FT * Gm::A);
         for (int a0 = g0; a0 < g_end; a0 += Gm::A) {
             __syncthreads();  // anchors of tile a0 written; the previous tile's reads of tab done
-            build_groups<MM>(tab, anc, sU, sValid, rng);
+            build_groups<MM>(tab, anc, sU, br, item0);
             __syncthreads();  // tables of tile a0 complete
             if (active) {
                 if constexpr (GROUPED) {
@@ -322,7 +379,7 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             }
             const int an = a0 + Gm::A;  // block-uniform
             if (an < a_end) {
-                tile_anchors<MM>(uf, an, a_end, raw, anc, sU, sValid);
+                tile_anchors<MM>(uf, an, a_end, raw, anc, sU);
                 if (an + Gm::A < a_end) fetch_tile<MM>(uf, an + Gm::A, a_end, raw);
             }
         }
@@ -334,8 +391,8 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
     }
 }
 
-// Fixed-point fraction of a coordinate (rho): load() fetches the FP32
-// coordinate of list position a on axis c, convert() forms the fraction.
+// Fixed-point fraction of a coordinate (rho): load<C>() fetches the FP32
+// coordinate of list position a on axis C, convert<C>() forms the fraction.
 struct CoordFrac {
     const float *X, *Y, *Z;
     const int *list;
@@ -344,12 +401,17 @@ struct CoordFrac {
     struct Raw {
         float v;
     };
-    __device__ __forceinline__ Raw load(int a, int c) const
+    template <int C>
+    __device__ __forceinline__ Raw load(int a) const
     {
-        const size_t o = foff + (list ? list[a] : a);
-        return Raw{c == 0 ? X[o] : (c == 1 ? Y[o] : Z[o])};
+        const float *P = C == 0 ? X : (C == 1 ? Y : Z);
+        return Raw{P[foff + (list ? list[a] : a)]};
     }
-    __device__ __forceinline__ uint32_t convert(const Raw &r, int) const { return turns_fixed((double)r.v * inv_ell); }
+    template <int C>
+    __device__ __forceinline__ uint32_t convert(const Raw &r) const
+    {
+        return turns_fixed((double)r.v * inv_ell);
+    }
 };
 
 // Fixed-point fraction of a displacement dr = (x1 + i1*L1) - (x0 + i0*L0),
@@ -366,18 +428,19 @@ struct DispFrac {
         float x0, x1;
         uint32_t i01;  // i0 in the low 16 bits, i1 in the high
     };
-    // c is wave-uniform.
-    __device__ __forceinline__ Raw load(int a, int c) const
+    template <int C>
+    __device__ __forceinline__ Raw load(int a) const
     {
-        const float *P = c == 0 ? X : (c == 1 ? Y : Z);
-        const int16_t *I = c == 0 ? IX : (c == 1 ? IY : IZ);
+        const float *P = C == 0 ? X : (C == 1 ? Y : Z);
+        const int16_t *I = C == 0 ? IX : (C == 1 ? IY : IZ);
         const int j = list ? list[a] : a;
         const size_t p0 = o0 + j, p1 = o1 + j;
         return Raw{P[p0], P[p1], (uint32_t)(uint16_t)I[p0] | ((uint32_t)(uint16_t)I[p1] << 16)};
     }
-    __device__ __forceinline__ uint32_t convert(const Raw &r, int c) const
+    template <int C>
+    __device__ __forceinline__ uint32_t convert(const Raw &r) const
     {
-        const float l0 = c == 0 ? l0x : (c == 1 ? l0y : l0z), l1 = c == 0 ? l1x : (c == 1 ? l1y : l1z);
+        const float l0 = C == 0 ? l0x : (C == 1 ? l0y : l0z), l1 = C == 0 ? l1x : (C == 1 ? l1y : l1z);
         const int i0 = (int16_t)(r.i01 & 0xFFFFu), i1 = (int16_t)(r.i01 >> 16);
         const double d = ((double)r.x1 - (double)r.x0) + ((double)i1 * (double)l1 - (double)i0 * (double)l0);
         return turns_fixed(d * inv_ell);
@@ -389,11 +452,11 @@ template <int R, int MM, bool GROUPED = false>
 __device__ __forceinline__ void rho_block(const float *__restrict__ X, const float *__restrict__ Y,
                                           const float *__restrict__ Z, const int *__restrict__ list, size_t foff,
                                           int a_begin, int a_end, double inv_ell, const uint32_t *__restrict__ kp,
-                                          int4 rng, float2 *tab, float2 *anc, uint32_t *sU, int *sValid,
+                                          int4 rng, float2 *tab, float2 *anc, uint32_t *sU,
                                           double (&re)[R], double (&im)[R])
 {
     const CoordFrac uf{X, Y, Z, list, foff, inv_ell};
-    phase_tiles<R, MM, false, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, re, im);
+    phase_tiles<R, MM, false, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, uf, re, im);
 }
 
 template <int R>
@@ -415,11 +478,11 @@ __device__ __forceinline__ void fs_block(const float *__restrict__ X, const floa
                                          const float *__restrict__ L0, const float *__restrict__ L1,
                                          const int *__restrict__ list, size_t o0, size_t o1, int a_begin, int a_end,
                                          double inv_ell, const uint32_t *__restrict__ kp, int4 rng, float2 *tab,
-                                         float2 *anc, uint32_t *sU, int *sValid, double (&s)[R])
+                                         float2 *anc, uint32_t *sU, double (&s)[R])
 {
     const DispFrac uf{X, Y, Z, IX, IY, IZ, list, o0, o1, inv_ell, L0[0], L0[1], L0[2], L1[0], L1[1], L1[2]};
     double unused[R];
-    phase_tiles<R, MM, true, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, sValid, uf, s, unused);
+    phase_tiles<R, MM, true, GROUPED>(kp, rng, a_begin, a_end, tab, anc, sU, uf, s, unused);
 }
 
 template <int R>
@@ -466,8 +529,7 @@ __device__ __forceinline__ bool xcd_atom_block(int n_blocks, int n_pairs, int &b
 #define AMDAT_PHASE_SHARED(MM)                         \
     extern __shared__ float2 tab[];                    \
     __shared__ float2 anc[3 * Geom<MM>::A * 8];          \
-    __shared__ uint32_t sU[3 * Geom<MM>::A];           \
-    __shared__ int sValid[Geom<MM>::A]
+    __shared__ uint32_t sU[3 * Geom<MM>::A]
 
 // A work item read by every lane of a wave (the same address), kept in
 // scalar registers.
