@@ -314,31 +314,6 @@ __device__ __forceinline__ void slot_pointers(const uint32_t *__restrict__ kp, c
     }
 }
 
-// The hot loop of one tile for the wave's group code (wave-uniform).
-template <int R, int MM, bool FS, bool GROUPED>
-__device__ __forceinline__ void hot_tile_any(int gs_code, const float2 *px, const float2 *const (&py)[R],
-                                             const float2 *const (&pz)[R], float (&p0)[R], float (&p1)[R])
-{
-    if constexpr (GROUPED) {
-        if constexpr (R >= 8) {
-            if (gs_code == 3) {
-                hot_tile<R, MM, FS, 8>(px, py, pz, p0, p1);
-                return;
-            }
-        }
-        if constexpr (R >= 4) {
-            if (gs_code == 2) {
-                hot_tile<R, MM, FS, 4>(px, py, pz, p0, p1);
-                return;
-            }
-        }
-        if (gs_code == 1) hot_tile<R, MM, FS, 2>(px, py, pz, p0, p1);
-        else hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
-    } else {
-        hot_tile<R, MM, FS, 1>(px, py, pz, p0, p1);
-    }
-}
-
 // kpack: (nx + 128) | (ny + 128) << 8 | nz << 16, bit 24 = roles of y and z
 // swapped (see below).
 // Hot loop over the atom tiles of one block, shared by the single-trajectory
@@ -350,18 +325,15 @@ __device__ __forceinline__ void hot_tile_any(int gs_code, const float2 *px, cons
 // share ny (or nz, with the roles of y and z swapped), so q = e_x e_y is
 // read and formed once per group and each slot adds q e_z: rho costs
 // 4 + 4/GS VALU per update (8 for GS = 1, 4.5 for GS = 8), Fs 2 + 4/GS.
-template <int R, int MM, bool FS, bool GROUPED, typename UF>
-__device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
-                                            float2 *tab, float2 *anc, uint32_t *sU, const UF uf,
-                                            double (&s0)[R], double (&s1)[R])
+// The tile loop of a wave whose groups hold GS slots (GS = 1 in plans
+// without groups): every wave of the block runs the same tiles and meets
+// the same barriers; only the hot loop differs with GS.
+template <int R, int MM, bool FS, int GS, typename UF>
+__device__ __forceinline__ void phase_tiles_gs(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
+                                               float2 *tab, float2 *anc, uint32_t *sU, const UF &uf, bool active,
+                                               double (&s0)[R], double (&s1)[R])
 {
     using Gm = Geom<MM>;
-    const int tid = threadIdx.x;
-    // waves whose units all lie past the chunk's last real unit skip the
-    // sums; grouped plans give each wave its group size
-    const int wave0 = __builtin_amdgcn_readfirstlane(tid >> 6) * 64;
-    const bool active = wave0 < range_units(rng);
-    const int gs_code = GROUPED ? range_gs_code(rng, wave0 >> 6) : 0;
     const float2 *px, *py[R], *pz[R];
     slot_pointers<R, MM>(kp, tab, px, py, pz);
 #pragma unroll
@@ -389,7 +361,7 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             __syncthreads();  // anchors of tile a0 written; the previous tile's reads of tab done
             build_groups<MM>(tab, anc, sU, br, item0);
             __syncthreads();  // tables of tile a0 complete
-            if (active) hot_tile_any<R, MM, FS, GROUPED>(gs_code, px, py, pz, p0, p1);
+            if (active) hot_tile<R, MM, FS, GS>(px, py, pz, p0, p1);
             const int an = a0 + Gm::A;  // block-uniform
             if (an < a_end) {
                 tile_anchors<MM>(uf, an, a_end, raw, anc, sU);
@@ -402,6 +374,40 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
             if constexpr (!FS) s1[r] += (double)p1[r];
         }
     }
+}
+
+// Hot loop over the atom tiles of one block (see phase_tiles_gs), with the
+// wave's group size chosen once, outside the tile loop: the partial sums
+// then stay in the same registers from tile to tile.
+template <int R, int MM, bool FS, bool GROUPED, typename UF>
+__device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
+                                            float2 *tab, float2 *anc, uint32_t *sU, const UF uf,
+                                            double (&s0)[R], double (&s1)[R])
+{
+    // waves whose units all lie past the chunk's last real unit skip the
+    // sums; grouped plans give each wave its group size (wave-uniform)
+    const int wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64;
+    const bool active = wave0 < range_units(rng);
+    if constexpr (GROUPED) {
+        const int gs_code = range_gs_code(rng, wave0 >> 6);
+        if constexpr (R >= 8) {
+            if (gs_code == 3) {
+                phase_tiles_gs<R, MM, FS, 8>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+                return;
+            }
+        }
+        if constexpr (R >= 4) {
+            if (gs_code == 2) {
+                phase_tiles_gs<R, MM, FS, 4>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+                return;
+            }
+        }
+        if (gs_code == 1) {
+            phase_tiles_gs<R, MM, FS, 2>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+            return;
+        }
+    }
+    phase_tiles_gs<R, MM, FS, 1>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
 }
 
 // Fixed-point fraction of a coordinate (rho): load<C>() fetches the FP32
