@@ -430,6 +430,22 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
     return AMDAT_OK;
 }
 
+// Plan cost parameters of the kernels that read a plan, by precision:
+// the table kernels (FP32 tables, FP64 tables) gain from grouped slots
+// (c VALU per unpaired update: rho 9, Fs 6); the per-update sincos
+// kernels (plain FP64, exact-phase FP32) do not, and pay as much for a
+// padding slot as for a vector, so they take ungrouped plans with few
+// padding slots (c: tens of operations per update).
+struct PlanCost {
+    double c;
+    bool grouped;
+};
+PlanCost plan_cost(int precision, bool fs)
+{
+    if (precision == AMDAT_PREC_FP64 || precision == AMDAT_PREC_FP32X) return {40.0, false};
+    return {fs ? 6.0 : 9.0, true};
+}
+
 // Atom splits so that the launch holds enough blocks to fill the chip,
 // while each split still streams at least 8 tiles of atoms.
 int choose_splits(long blocks_without_split, int n_list)
@@ -1419,7 +1435,8 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
         std::iota(fr.begin(), fr.end(), 0);
     }
     KPlanOwned *plan = nullptr;
-    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan);
+    const PlanCost pc = plan_cost(precision, false);
+    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, pc.c, pc.grouped);
     if (rc) return rc;
     const KPlan &kp = plan->kp;
     const int nl = list_n(tr, list);
@@ -1670,7 +1687,8 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     KPlanOwned *plan = nullptr;
-    rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, 6.0);
+    const PlanCost pc = plan_cost(precision, true);
+    rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, pc.c, pc.grouped);
     if (rc) return rc;
     const KPlan &kp = plan->kp;
     const int ntg = pairs->ntg, nsh = last_shell - first_shell + 1;
@@ -1924,9 +1942,10 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
         return set_error(AMDAT_ERANGE, "amdat_sk_loss_grad: shell range outside the table");
     if (!(length_scale > 0.0f)) return set_error(AMDAT_EINVAL, "amdat_sk_loss_grad: length scale must be > 0");
     HIP_TRY(hipSetDevice(ctx->device));
-    // the plan the single-trajectory S(k) path chooses (grouped slots allowed)
+    // the plan of the exact-phase kernel, which computes a phase per slot
     KPlanOwned *plan = nullptr;
-    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, 9.0, true);
+    const PlanCost pc = plan_cost(AMDAT_PREC_FP32X, false);
+    int rc = build_plan(ctx, sh, first_shell, last_shell, &plan, 0, pc.c, pc.grouped);
     if (rc) return rc;
     const KPlan &kp = plan->kp;
     const int K = kp.K, nsh = last_shell - first_shell + 1, v0 = sh->off[first_shell];
