@@ -123,7 +123,7 @@ struct amdat_ctx {
     // Grow-only device scratch (ctx_scratch), for entry points that
     // synchronise their stream before returning: one call's kernels are done
     // with it before the next call writes it.
-    DevBuf<char> ws[4];
+    DevBuf<char> ws[6];
 };
 
 namespace {
@@ -198,6 +198,7 @@ struct amdat_shells {
     std::map<std::tuple<int, int, int>, std::pair<int, int>> choice;               // (first, last, c) -> (R, mode)
     int hint_R = 0, hint_pair = 0;  // amdat_shells_plan_hint (0: automatic)
     DevBuf<int> d_nvec;             // the integer vectors on the device, uploaded on first use
+    DevBuf<int> d_off;              // the shell offsets on the device, uploaded on first use
 };
 
 struct amdat_pairs {
@@ -1985,37 +1986,33 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     HIP_TRY(launch_rho(AMDAT_PREC_FP32X, view, nullptr, n_atoms, dfr, 1, kp, inv_ell, n_splits,
                        n_splits > 1 ? part : rho, st));
     if (n_splits > 1) HIP_TRY(launch_split_reduce2(part, 1, n_splits, K, rho, st));
-    std::vector<double2> h(K);
-    HIP_TRY(hipMemcpyAsync(h.data(), rho, sizeof(double2) * K, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    // S per shell, loss, and the adjoint weights q_v = c_v rho_v (FP64 on the host: K values)
-    std::vector<float2> q(K);
-    double L = 0.0;
-    for (int s = 0; s < nsh; s++) {
-        const int a = sh->off[first_shell + s] - v0, b = sh->off[first_shell + s + 1] - v0;
-        const int H = b - a;
-        double acc = 0.0;
-        for (int v = a; v < b; v++) acc += h[v].x * h[v].x + h[v].y * h[v].y;
-        const double Ss = H ? acc / ((double)H * n_atoms) : 0.0;
-        if (S) S[s] = Ss;
-        const double d = Ss - target[s];
-        L += weights[s] * d * d;
-        const double c = H ? -4.0 * weights[s] * d / ((double)H * n_atoms) : 0.0;
-        for (int v = a; v < b; v++) q[v] = make_float2((float)(c * h[v].x), (float)(c * h[v].y));
-    }
-    *loss = L;
-    if (!d_gx) return AMDAT_OK;
-    // backward: the adjoint kernel recomputes every phase
+    // S per shell, the loss and the adjoint weights q_v = c_v rho_v on the
+    // device (FP64, K values): no host round trip between the forward and
+    // the backward kernels
+    if (!sh->d_off.p) HIP_TRY(sh->d_off.upload(sh->off.data(), sh->off.size(), st));
+    double *wt = nullptr, *res = nullptr;
+    HIP_TRY(ctx_scratch(ctx, 4, (size_t)2 * nsh, &wt));
+    HIP_TRY(ctx_scratch(ctx, 5, (size_t)1 + 2 * nsh, &res));
+    HIP_TRY(hipMemcpyAsync(wt, weights, sizeof(double) * nsh, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(wt + nsh, target, sizeof(double) * nsh, hipMemcpyHostToDevice, st));
     float2 *dq = nullptr;
     HIP_TRY(ctx_scratch(ctx, 2, (size_t)K, &dq));
-    HIP_TRY(hipMemcpyAsync(dq, q.data(), sizeof(float2) * K, hipMemcpyHostToDevice, st));
-    if (!sh->d_nvec.p) HIP_TRY(sh->d_nvec.upload(sh->nvec.data(), sh->nvec.size(), st));
-    const int g_splits = sk_adjoint_splits(n_atoms, K);
-    double *g_part = nullptr;
-    if (g_splits > 1) HIP_TRY(ctx_scratch(ctx, 3, (size_t)g_splits * 3 * n_atoms, &g_part));
-    HIP_TRY(launch_sk_adjoint(d_x, d_y, d_z, n_atoms, inv_ell, sh->d_nvec.p + 3 * (size_t)v0, dq, K, g_splits, g_part,
-                              d_gx, d_gy, d_gz, st));
+    HIP_TRY(launch_adjoint_weights(rho, sh->d_off.p + first_shell, nsh, n_atoms, wt, res, dq, st));
+    if (d_gx) {
+        // backward: the adjoint kernel recomputes every phase
+        if (!sh->d_nvec.p) HIP_TRY(sh->d_nvec.upload(sh->nvec.data(), sh->nvec.size(), st));
+        const int g_splits = sk_adjoint_splits(n_atoms, K);
+        double *g_part = nullptr;
+        if (g_splits > 1) HIP_TRY(ctx_scratch(ctx, 3, (size_t)g_splits * 3 * n_atoms, &g_part));
+        HIP_TRY(launch_sk_adjoint(d_x, d_y, d_z, n_atoms, inv_ell, sh->d_nvec.p + 3 * (size_t)v0, dq, K, g_splits,
+                                  g_part, d_gx, d_gy, d_gz, st));
+    }
+    std::vector<double> h((size_t)1 + nsh);
+    HIP_TRY(hipMemcpyAsync(h.data(), res, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    *loss = h[0];
+    if (S)
+        for (int s = 0; s < nsh; s++) S[s] = h[1 + s];
     return AMDAT_OK;
 }
 

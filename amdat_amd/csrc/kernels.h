@@ -206,6 +206,12 @@ hipError_t launch_fs_batch_reduce(const BatchTraj *trajs, int n_traj, const int 
 // the K vectors in n_splits ranges (sk_adjoint_splits); with more than one,
 // part holds n_splits * 3 * N doubles of partial sums.
 int sk_adjoint_splits(int N, int K);
+// Adjoint weights from the forward sums of one frame: res[0] = loss,
+// res[1 .. n_shells] = S per shell (res holds 1 + 2 n_shells doubles),
+// q[K] = c_s rho_v; wt[2 n_shells] = weights then targets.  shell_off:
+// the n_shells + 1 offsets of the shells in the table.
+hipError_t launch_adjoint_weights(const double2 *rho, const int *shell_off, int n_shells, int n_atoms,
+                                  const double *wt, double *res, float2 *q, hipStream_t st);
 hipError_t launch_sk_adjoint(const float *x, const float *y, const float *z, int N, double inv_ell, const int *nvec,
                              const float2 *q, int K, int n_splits, double *part, double *gx, double *gy, double *gz,
                              hipStream_t st);

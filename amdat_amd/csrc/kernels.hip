@@ -151,21 +151,76 @@ __global__ __launch_bounds__(kPhaseBlock) void fs_f64_kernel(
     }
 }
 
+// out[f][k] = sum over s of in[f][s][k].  A block takes 32 consecutive k of
+// one row and its 8 wave-halves (32 lanes each) sum the splits s = g, g+8,
+// g+16, ... (g = 0..7) in order; the eight partials are then added in
+// order g = 0..7, so the result does not depend on the launch.  Each
+// half-wave reads 32 consecutive entries (512 bytes) per split.
+constexpr int kSplitCols = 32, kSplitGroups = kBlock / kSplitCols;
 __global__ __launch_bounds__(kBlock) void split_reduce2_kernel(const double2 *__restrict__ in, int n_rows,
                                                                 int n_splits, int K, double2 *__restrict__ out)
 {
-    const size_t n = (size_t)n_rows * K;
-    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock) {
-        const size_t f = e / K, k = e - f * K;
-        double re = 0.0, im = 0.0;
-        for (int s = 0; s < n_splits; ++s) {
-            const double2 v = in[(f * n_splits + s) * K + k];
+    __shared__ double2 part[kSplitGroups][kSplitCols];
+    const int col_blocks = (K + kSplitCols - 1) / kSplitCols;
+    const int f = blockIdx.x / col_blocks, k0 = (blockIdx.x - f * col_blocks) * kSplitCols;
+    const int c = threadIdx.x % kSplitCols, g = threadIdx.x / kSplitCols;
+    const int k = k0 + c;
+    double re = 0.0, im = 0.0;
+    if (f < n_rows && k < K)
+        for (int sp = g; sp < n_splits; sp += kSplitGroups) {
+            const double2 v = in[((size_t)f * n_splits + sp) * K + k];
             re += v.x;
             im += v.y;
         }
-        out[e] = make_double2(re, im);
+    part[g][c] = make_double2(re, im);
+    __syncthreads();
+    if (g == 0 && f < n_rows && k < K) {
+        double2 acc = part[0][c];
+        for (int q = 1; q < kSplitGroups; ++q) {
+            acc.x += part[q][c].x;
+            acc.y += part[q][c].y;
+        }
+        out[(size_t)f * K + k] = acc;
     }
 }
+
+// Adjoint weights of the S(k) misfit from the forward sums rho[K] (one
+// frame): per shell s (vectors shell_off[s] - shell_off[0] ...), S_s =
+// sum |rho_v|^2 / (H_s N), c_s = -4 w_s (S_s - T_s) / (H_s N), q_v = c_s
+// rho_v (FP32); the loss sum_s w_s (S_s - T_s)^2 added in shell order by
+// thread 0.  One block; wave w takes shells w, w + 4, ...: its lanes sum
+// vectors l, l + 64, ... and a fixed shuffle tree adds the lanes, so the
+// result does not depend on the launch.  res[0] = loss, res[1 + s] = S_s.
+__global__ __launch_bounds__(kBlock) void adjoint_weights_kernel(const double2 *__restrict__ rho,
+                                                                  const int *__restrict__ shell_off, int n_shells,
+                                                                  int n_atoms, const double *__restrict__ wt,
+                                                                  double *__restrict__ res, float2 *__restrict__ q)
+{
+    // wt[s] = weight, wt[n_shells + s] = target; res[1 + n_shells + s] holds
+    // the shell's loss term until thread 0 adds them
+    const int v0 = shell_off[0], lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int sh = wave; sh < n_shells; sh += kBlock / 64) {
+        const int a = shell_off[sh] - v0, b = shell_off[sh + 1] - v0, H = b - a;
+        double acc = 0.0;
+        for (int v = a + lane; v < b; v += 64) acc += rho[v].x * rho[v].x + rho[v].y * rho[v].y;
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        const double S = H ? acc / ((double)H * n_atoms) : 0.0;
+        const double d = S - wt[n_shells + sh];
+        const double c = H ? -4.0 * wt[sh] * d / ((double)H * n_atoms) : 0.0;
+        for (int v = a + lane; v < b; v += 64) q[v] = make_float2((float)(c * rho[v].x), (float)(c * rho[v].y));
+        if (lane == 0) {
+            res[1 + sh] = S;
+            res[1 + n_shells + sh] = wt[sh] * d * d;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double L = 0.0;
+        for (int sh = 0; sh < n_shells; ++sh) L += res[1 + n_shells + sh];
+        res[0] = L;
+    }
+}
+
 
 // grid = (columns, groups).
 __global__ __launch_bounds__(kBlock) void corr_shell_kernel(
@@ -913,12 +968,22 @@ hipError_t launch_fs(int precision, const TrajDev &tr, const int *list, int n_li
 
 hipError_t launch_split_reduce2(const double2 *in, int n_rows, int n_splits, int K, double2 *out, hipStream_t st)
 {
-    const size_t n = (size_t)n_rows * K;
-    int blocks = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 65535);
-    if (blocks < 1) blocks = 1;
+    if (n_rows <= 0 || K <= 0) return hipSuccess;
+    const long blocks = (long)n_rows * ((K + kSplitCols - 1) / kSplitCols);
+    if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
     (void)hipGetLastError();
 
-    hipLaunchKernelGGL(split_reduce2_kernel, dim3(blocks), dim3(kBlock), 0, st, in, n_rows, n_splits, K, out);
+    hipLaunchKernelGGL(split_reduce2_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, in, n_rows, n_splits, K,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_weights(const double2 *rho, const int *shell_off, int n_shells, int n_atoms,
+                                  const double *wt, double *res, float2 *q, hipStream_t st)
+{
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(adjoint_weights_kernel, dim3(1), dim3(kBlock), 0, st, rho, shell_off, n_shells, n_atoms, wt,
+                       res, q);
     return hipGetLastError();
 }
 
