@@ -331,17 +331,16 @@ int shell_range_M(const amdat_shells *sh, int first, int last)
     return M;
 }
 
-// Modelled cost of a layout per atom: every slot of a wave that runs the
-// sums costs c VALU ops per lane in waves of single slots, c - 4 + 4/GS in
-// waves of groups of GS slots (q = e_x e_y formed once per group: c - 2 for
-// pairs, c - 3 for quads, c - 3.5 for groups of 8), counted in whole SIMD
-// rounds of 4 waves per chunk, and every chunk the fixed part of choose_R.
-// (Scale-form Fs, 1,500 vectors in one chunk: R = 8 paired, 5 active
-// waves, ran 11.4 s against 10.1 s for R = 4 unpaired with 7.)
+// Modelled cost of a layout per atom (PlanLayout::simd_cost): per chunk,
+// the busiest SIMD (its two waves' slots x (c - 4 + 4/GS) VALU operations
+// per lane, so uneven group sizes on one SIMD count) and the table build
+// and anchors of the chunk's sections.  64,000-atom workload: rho (28,181
+// vectors) ran 377 ms in plan mode 4 and 375 ms in mode 5, Fs (26,243
+// vectors) 238 and 230 ms.
 double layout_cost(const PlanLayout &L, double c)
 {
-    return (double)L.n_chunks * 12000.0 + (double)L.gs8_slots * (c - 3.5) + (double)L.quad_slots * (c - 3.0) +
-           (double)L.paired_slots * (c - 2.0) + (double)L.unpaired_slots * c;
+    (void)c;
+    return L.simd_cost;
 }
 
 // R_force > 0 fixes the vectors per thread and the layout mode to
@@ -379,7 +378,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
             // reads spread over the banks (64,000-atom workload: rho 542 ms
             // in mode 2 against 464 ms in mode 1, although it issues fewer
             // VALU ops).
-            const int cand[7][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {4, 4}, {8, 4}};
+            const int cand[8][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {4, 4}, {8, 4}, {8, 5}};
             for (const auto &cd : cand) {
                 if (cd[1] && !allow_pair) continue;
                 PlanLayout L =
@@ -388,7 +387,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
                 if (cost < best * 0.98) {
                     best = cost;
                     lay = std::move(L);
-                    have_layout = cd[1] != 4;  // a grouped plan is placed once chosen
+                    have_layout = cd[1] < 4;  // a grouped plan is placed once chosen
                     R = cd[0];
                     mode = cd[1];
                 }
@@ -642,7 +641,8 @@ int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair)
     clear_error();
     if (!sh) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: null table");
     if (R != 0 && R != 2 && R != 4 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: R must be 0, 2, 4 or 8");
-    if (pair < 0 || pair > 4) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pair must be 0 to 4");
+    if (pair < 0 || pair > 5) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pair must be 0 to 5");
+    if (pair == 5 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: mode 5 needs R = 8");
     if (pair && R != 4 && R != 8) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: pairs need R = 4 or 8");
     sh->hint_R = R;
     sh->hint_pair = pair;

@@ -49,12 +49,17 @@ struct PlanLayout {
     bool pair = false;           // grouped waves: a group's slots share ny, or nz (kPackSwap)
     bool quad = false;           // mode 3
     long gs8_slots = 0, quad_slots = 0, paired_slots = 0, unpaired_slots = 0;
+    double simd_cost = 0.0;      // per atom, summed over chunks: 4 SIMDs x 64 lanes x the busiest
+                                 // SIMD's VALU ops (its two waves' slots x (c - 4 + 4/GS))
                                  // slots of waves that run the sums, by group size 8, 4, 2, 1,
                                  // each chunk's rounded up to whole SIMD rounds (4 waves)
 };
 // mode 0: unpaired; 1: paired, vectors without a partner padded; 2: paired,
 // vectors without a partner in unpaired waves; 3: quads sharing ny where a
 // ny class holds four, then pairs as mode 1, in waves of one kind each;
+// 5: grouped as 4, with waves of single slots holding R/2 vectors per
+// thread and waves of pairs 3R/4 (R = 8; dead slots past them, range bit
+// 15 set), so that waves carry about equal work per atom;
 // 4: grouped, classes sharing nz or ny cut into groups of 8, 4, 2 and 1
 // slots (cheapest at c VALU ops per unpaired slot: one accumulate of c - 4
 // per slot and one product of 4 per group), waves of one group size each,

@@ -542,9 +542,11 @@ struct UnitReads {
 // does not raise the summed clash cost.  hw[h] lists the unit indices of
 // half-wave h.
 void optimise_reads(std::vector<std::vector<Vec>> &units, const std::vector<std::vector<int>> &hw, int R, int GS,
-                    const Entries &E, int W, uint64_t seed)
+                    const Entries &E, int W, uint64_t seed, int NS)
 {
-    const int H = (int)hw.size(), G = R / GS;
+    // NS: slots that hold vectors (the rest are dead and must stay past
+    // them); moves only permute the first NS / GS groups
+    const int H = (int)hw.size(), G = R / GS, GA = NS / GS;
     if (H == 0) return;
     const UnitReads ur{R, GS, E};
     std::vector<ReadCounters> rc(H);
@@ -586,7 +588,7 @@ void optimise_reads(std::vector<std::vector<Vec>> &units, const std::vector<std:
         tried++;
         if (kind == 0 && GS > 1) {
             // two slots of one group: the group's shared read is unchanged
-            const int g = (int)((rnd >> 32) % G), i = (int)((rnd >> 40) % GS), j = (int)((rnd >> 48) % GS);
+            const int g = (int)((rnd >> 32) % GA), i = (int)((rnd >> 40) % GS), j = (int)((rnd >> 48) % GS);
             if (i == j) continue;
             const int ri = g * GS + i, rj = g * GS + j;
             long d = ra.change(ri, ur.own(a[ri]), -1) + ra.change(rj, ur.own(a[rj]), -1) +
@@ -599,9 +601,9 @@ void optimise_reads(std::vector<std::vector<Vec>> &units, const std::vector<std:
                 ra.change(ri, ur.own(a[ri]), +1);
                 ra.change(rj, ur.own(a[rj]), +1);
             }
-        } else if (kind == 1 && G > 1) {
+        } else if (kind == 1 && GA > 1) {
             // two groups of one unit
-            const int g1 = (int)((rnd >> 32) % G), g2 = (int)((rnd >> 40) % G);
+            const int g1 = (int)((rnd >> 32) % GA), g2 = (int)((rnd >> 40) % GA);
             if (g1 == g2) continue;
             long d = ur.apply(ra, a, -1);
             std::vector<Vec> b = a;
@@ -691,15 +693,21 @@ void fill_padding(std::vector<std::vector<Vec>> &units, size_t h0, int R, int GS
 // chunk, the active waves (a prefix: the kernels skip waves past the
 // chunk's units) are permuted so that the largest SIMD load is smallest;
 // ties keep the first order found, so the layout stays deterministic.
-void balance_simds(std::vector<std::vector<Vec>> &units, std::vector<char> &unit_g, int n_units, int R, double c)
+template <typename SlotsOf>
+void balance_simds(std::vector<std::vector<Vec>> &units, std::vector<char> &unit_g, int n_units, int R, double c,
+                   const SlotsOf &slots_of)
 {
+    (void)R;
     const int n_chunks = (int)(units.size() / 512);
     for (int ch = 0; ch < n_chunks; ch++) {
         const int n_act = std::min(512, n_units - ch * 512);
         const int a = (n_act + 63) / 64;
         if (a <= 4) continue;  // one wave per SIMD at most
         double load[8];
-        for (int w = 0; w < a; w++) load[w] = R * ((c - 4.0) + 4.0 / unit_g[(size_t)ch * 512 + w * 64]);
+        for (int w = 0; w < a; w++) {
+            const int gs = unit_g[(size_t)ch * 512 + w * 64];
+            load[w] = slots_of(gs) * ((c - 4.0) + 4.0 / gs);
+        }
         std::array<int, 8> perm{}, best{};
         for (int w = 0; w < 8; w++) perm[w] = best[w] = w;
         auto worst = [&](const std::array<int, 8> &p) {
@@ -732,7 +740,12 @@ void balance_simds(std::vector<std::vector<Vec>> &units, std::vector<char> &unit
 PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c, bool place)
 {
     const Entries E{table_oy(MM), table_oz(MM)};
-    const bool pair = mode != 0, mixed = mode == 2, quad = mode == 3, grouped = mode == 4;
+    const bool pair = mode != 0, mixed = mode == 2, quad = mode == 3, grouped = mode == 4 || mode == 5;
+    // mode 5: waves of single slots hold R/2 vectors per thread and waves of
+    // pairs 3R/4 (R = 8), so that every wave carries about the same work per
+    // atom: slots past a unit's count are dead (padding the kernels skip)
+    const bool short_waves = mode == 5 && R == 8;
+    auto slots_of = [&](int gs) { return short_waves ? (gs == 1 ? R / 2 : (gs == 2 ? 3 * R / 4 : R)) : R; };
     const int W = std::max(5 * table_h(MM) + 3, 257);
     PlanLayout L;
     L.R = R;
@@ -813,7 +826,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
                 }
             }
             for (int k = 0; k < 4; k++) {
-                const int s = kinds[k], per = R / s;
+                const int s = kinds[k], per = slots_of(s) / s;
                 if (s > gmax) continue;
                 for (size_t i = 0; i < pieces[k].size(); i += per) {
                     std::vector<Vec> un;
@@ -821,6 +834,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
                         if (i + j < pieces[k].size()) un.insert(un.end(), pieces[k][i + j].begin(), pieces[k][i + j].end());
                         else for (int q = 0; q < s; q++) un.push_back(Vec{nx, 0, 0, -1});
                     }
+                    while ((int)un.size() < R) un.push_back(Vec{nx, 0, 0, -1});  // dead slots
                     pend[k].push_back(std::move(un));
                 }
                 emit(pend[k], sent[k], (char)s, false);
@@ -925,7 +939,8 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
         auto work = [&]() {
             for (size_t i; (i = next.fetch_add(1)) < tasks.size();)
                 optimise_reads(units, tasks[i].hw, R, tasks[i].gs, E, W,
-                               0x9E3779B97F4A7C15ull * (uint64_t)(tasks[i].c * 16 + tasks[i].gs + 1));
+                               0x9E3779B97F4A7C15ull * (uint64_t)(tasks[i].c * 16 + tasks[i].gs + 1),
+                               slots_of(tasks[i].gs));
         };
         const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
         const size_t n_threads = std::min<size_t>(std::min<size_t>(hc, 32), tasks.size());
@@ -937,7 +952,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
     } else if (!grouped) {
         for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) order_half_wave(units, h0, R, unit_g[h0], E);
     }
-    if (grouped || mixed || quad) balance_simds(units, unit_g, n_units, R, c);
+    if (grouped || mixed || quad) balance_simds(units, unit_g, n_units, R, c, slots_of);
     // emit
     const size_t n = (size_t)L.n_chunks * L.KB;
     L.pack.resize(n);
@@ -961,7 +976,8 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
         }
         L.range[4 * ch] = xlo;
         L.range[4 * ch + 1] = xhi;
-        L.range[4 * ch + 2] = my | (mz << 16);
+        // bit 15: the plan's waves of singles and pairs hold fewer slots
+        L.range[4 * ch + 2] = my | (mz << 16) | (short_waves ? (1 << 15) : 0);
         const int n_act = std::min(512, n_units - ch * 512);
         // two bits per wave: its group code (group size 1 << code)
         unsigned codes = 0;
@@ -973,16 +989,33 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
             a_by_g[cd]++;
         }
         L.range[4 * ch + 3] = (int)((unsigned)n_act | (codes << 16));
+        // The chunk's time per atom follows its busiest SIMD: waves w and
+        // w + 4 share one, and each wave's lanes issue its slots x
+        // (c - 4 + 4/GS) VALU operations per atom.
+        {
+            double load[8] = {0, 0, 0, 0, 0, 0, 0, 0}, busiest = 0.0;
+            for (int w = 0; w < 8 && w * 64 < n_act; w++) {
+                const int gs = unit_g[(size_t)ch * 512 + w * 64];
+                load[w] = slots_of(gs) * ((c - 4.0) + 4.0 / gs);
+            }
+            for (int q = 0; q < 4; q++) busiest = std::max(busiest, load[q] + load[q + 4]);
+            // the per-tile table build, spread over every SIMD: about 95
+            // VALU per (axis, group of 8 entries) item and 1,400 for the
+            // anchors, per atom
+            const int xa = xlo >= 0 ? xlo : (xhi <= 0 ? -xhi : 0), xb = std::max(std::abs(xlo), std::abs(xhi));
+            const int items = (xb >> 3) - (xa >> 3) + 1 + (my >> 3) + 1 + (mz >> 3) + 1;
+            L.simd_cost += 4.0 * 64.0 * busiest + 95.0 * items + 1400.0;
+        }
         // Work of the waves that run the sums, in slots.  A block's 8 waves
         // share 4 SIMDs two by two, so a chunk with a active waves takes as
         // long as ceil(a / 4) * 4 of them.
         const int a = (n_act + 63) / 64;
         if (a > 0) {
             const double scale = (double)(((a + 3) / 4) * 4) / a;
-            L.gs8_slots += (long)(a_by_g[3] * 64.0 * R * scale);
-            L.quad_slots += (long)(a_by_g[2] * 64.0 * R * scale);
-            L.paired_slots += (long)(a_by_g[1] * 64.0 * R * scale);
-            L.unpaired_slots += (long)(a_by_g[0] * 64.0 * R * scale);
+            L.gs8_slots += (long)(a_by_g[3] * 64.0 * slots_of(8) * scale);
+            L.quad_slots += (long)(a_by_g[2] * 64.0 * slots_of(4) * scale);
+            L.paired_slots += (long)(a_by_g[1] * 64.0 * slots_of(2) * scale);
+            L.unpaired_slots += (long)(a_by_g[0] * 64.0 * slots_of(1) * scale);
         }
         // conflict estimate of the chunk's reads
         for (int r = 0; r < R; r++)
@@ -1007,7 +1040,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
 extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst,
                                           int *range, int *n_chunks, long *conflicts)
 {
-    if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks || pair < 0 || pair > 4) return AMDAT_EINVAL;
+    if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks || pair < 0 || pair > 5) return AMDAT_EINVAL;
     if (pair && R < 4) return AMDAT_EINVAL;
     int M = 0;
     for (int i = 0; i < 3 * K; i++) M = std::max(M, std::abs(nvec[i]));

@@ -78,8 +78,11 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 // them skip the hot loop) | group codes << 16: two bits per wave w at
 // 16 + 2w, group size GS = 1 << code (slots g*GS .. g*GS + GS - 1 of a
 // thread share the lead slot's e_x e_y, or e_x e_z).
-__device__ __forceinline__ int range_my(int4 r) { return r.z & 0xFFFF; }
+__device__ __forceinline__ int range_my(int4 r) { return r.z & 0x7FFF; }
 __device__ __forceinline__ int range_mz(int4 r) { return r.z >> 16; }
+// z bit 15: waves of single slots hold R/2 vectors per thread and waves of
+// pairs 3R/4 (plan mode 5); the slots past them are dead
+__device__ __forceinline__ bool range_short(int4 r) { return (r.z >> 15) & 1; }
 __device__ __forceinline__ int range_units(int4 r) { return r.w & 0xFFFF; }
 __device__ __forceinline__ int range_gs_code(int4 r, int wave) { return (int)(((unsigned)r.w >> (16 + 2 * wave)) & 3u); }
 
@@ -256,30 +259,32 @@ __device__ __forceinline__ void build_groups(float2 *tab, const float2 *anc, con
 // GS slots in a group (1, 2 or 4; R is a multiple of GS) share the factor
 // behind py, so only the group's first slot reads it and forms
 // q = e_x e_y.
-template <int R, int MM, bool FS, int GS>
+// NS (<= R, a multiple of GS): slots that hold vectors; the rest of the R
+// are dead and neither read nor summed.
+template <int R, int MM, bool FS, int GS, int NS = R>
 __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (&py)[R], const float2 *const (&pz)[R],
                                          float (&p0)[R], float (&p1)[R])
 {
     using Gm = Geom<MM>;
-    static_assert(R % GS == 0, "slots per thread must hold whole groups");
+    static_assert(R % GS == 0 && NS % GS == 0 && NS <= R, "slots per thread must hold whole groups");
     float2 by[kDepth], bz[kDepth];
 #pragma unroll
     for (int u = 0; u < kDepth; ++u) {
-        if (u % GS == 0) by[u] = py[u % R][(u / R) * Gm::W];
+        if (u % GS == 0) by[u] = py[u % NS][(u / NS) * Gm::W];
         else by[u] = make_float2(0.0f, 0.0f);
-        bz[u] = pz[u % R][(u / R) * Gm::W];
+        bz[u] = pz[u % NS][(u / NS) * Gm::W];
     }
     float2 ex = px[0], ex_next = ex;
     float qr = 0.0f, qi = 0.0f;
 #pragma clang loop unroll(full)
-    for (int u = 0; u < Gm::A * R; ++u) {
-        const int a = u / R, r = u % R, slot = u % kDepth;
+    for (int u = 0; u < Gm::A * NS; ++u) {
+        const int a = u / NS, r = u % NS, slot = u % kDepth;
         const bool lead = r % GS == 0;  // this update forms q = e_x e_y
         const float2 ey = by[slot], ez = bz[slot];
-        if (u + kDepth < Gm::A * R) {
+        if (u + kDepth < Gm::A * NS) {
             const int v = u + kDepth;
-            if (v % GS == 0) by[slot] = py[v % R][(v / R) * Gm::W];
-            bz[slot] = pz[v % R][(v / R) * Gm::W];
+            if (v % GS == 0) by[slot] = py[v % NS][(v / NS) * Gm::W];
+            bz[slot] = pz[v % NS][(v / NS) * Gm::W];
         }
         if (r == 0 && a + 1 < Gm::A) ex_next = px[(a + 1) * Gm::W];
         __builtin_amdgcn_sched_barrier(0);
@@ -290,7 +295,7 @@ __device__ __forceinline__ void hot_tile(const float2 *px, const float2 *const (
         p0[r] = fmaf(qr, ez.x, fmaf(-qi, ez.y, p0[r]));
         if constexpr (!FS) p1[r] = fmaf(qr, ez.y, fmaf(qi, ez.x, p1[r]));
         __builtin_amdgcn_sched_barrier(0);
-        if (r == R - 1) ex = ex_next;
+        if (r == NS - 1) ex = ex_next;
     }
 }
 
@@ -328,7 +333,7 @@ __device__ __forceinline__ void slot_pointers(const uint32_t *__restrict__ kp, c
 // The tile loop of a wave whose groups hold GS slots (GS = 1 in plans
 // without groups): every wave of the block runs the same tiles and meets
 // the same barriers; only the hot loop differs with GS.
-template <int R, int MM, bool FS, int GS, typename UF>
+template <int R, int MM, bool FS, int GS, int NS, typename UF>
 __device__ __forceinline__ void phase_tiles_gs(const uint32_t *__restrict__ kp, int4 rng, int a_begin, int a_end,
                                                float2 *tab, float2 *anc, uint32_t *sU, const UF &uf, bool active,
                                                double (&s0)[R], double (&s1)[R])
@@ -361,7 +366,7 @@ __device__ __forceinline__ void phase_tiles_gs(const uint32_t *__restrict__ kp, 
             __syncthreads();  // anchors of tile a0 written; the previous tile's reads of tab done
             build_groups<MM>(tab, anc, sU, br, item0);
             __syncthreads();  // tables of tile a0 complete
-            if (active) hot_tile<R, MM, FS, GS>(px, py, pz, p0, p1);
+            if (active) hot_tile<R, MM, FS, GS, NS>(px, py, pz, p0, p1);
             const int an = a0 + Gm::A;  // block-uniform
             if (an < a_end) {
                 tile_anchors<MM>(uf, an, a_end, raw, anc, sU);
@@ -392,22 +397,32 @@ __device__ __forceinline__ void phase_tiles(const uint32_t *__restrict__ kp, int
         const int gs_code = range_gs_code(rng, wave0 >> 6);
         if constexpr (R >= 8) {
             if (gs_code == 3) {
-                phase_tiles_gs<R, MM, FS, 8>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+                phase_tiles_gs<R, MM, FS, 8, R>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
                 return;
             }
         }
         if constexpr (R >= 4) {
             if (gs_code == 2) {
-                phase_tiles_gs<R, MM, FS, 4>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+                phase_tiles_gs<R, MM, FS, 4, R>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+                return;
+            }
+        }
+        if constexpr (R == 8) {
+            // plan mode 5: singles and pairs in waves of fewer slots
+            if (range_short(rng)) {
+                if (gs_code == 1)
+                    phase_tiles_gs<R, MM, FS, 2, 3 * R / 4>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+                else
+                    phase_tiles_gs<R, MM, FS, 1, R / 2>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
                 return;
             }
         }
         if (gs_code == 1) {
-            phase_tiles_gs<R, MM, FS, 2>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+            phase_tiles_gs<R, MM, FS, 2, R>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
             return;
         }
     }
-    phase_tiles_gs<R, MM, FS, 1>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
+    phase_tiles_gs<R, MM, FS, 1, R>(kp, rng, a_begin, a_end, tab, anc, sU, uf, active, s0, s1);
 }
 
 // Fixed-point fraction of a coordinate (rho): load<C>() fetches the FP32

@@ -149,10 +149,11 @@ int amdat_shells_upload(amdat_ctx *ctx, int n_shells, const int *shell_off, cons
  * that share ny or nz (R = 4 or 8; pair = 1: vectors without a partner are
  * padded, 2: they run in unpaired waves, 3: quads of four vectors sharing
  * ny first, then pairs as 1, 4: groups of 8, 4, 2 or 1 vectors sharing nz
- * or ny, placed against LDS bank clashes) for the single-trajectory phase
- * kernels run on
- * this table; R = 0 restores the automatic choice (lowest modelled cost).
- * Results do not depend on it beyond FP32 rounding. */
+ * or ny, placed against LDS bank clashes, 5 (R = 8 only): as 4, with waves
+ * of single vectors holding 4 per thread and waves of pairs 6, so that all
+ * waves carry about equal work) for the single-trajectory phase kernels run
+ * on this table; R = 0 restores the automatic choice (lowest modelled
+ * cost).  Results do not depend on it beyond FP32 rounding. */
 int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair);
 int amdat_shells_free(amdat_ctx *ctx, amdat_shells *sh);
 

@@ -1,8 +1,9 @@
 """Every plan layout gives the same results: rho per vector and Fs on the
 same inputs with R = 2, 4, 8 unpaired, R = 4, 8 in both paired modes
 (slots sharing ny or nz; partnerless vectors padded, or in unpaired waves),
-in quads, and grouped (groups of 8, 4, 2, 1 slots sharing ny or nz),
-each against the FP64 oracle at the 1e-6 gate, on the plain form
+in quads, and grouped (groups of 8, 4, 2, 1 slots sharing ny or nz; in
+mode 5 with waves of singles and pairs holding fewer slots), each against
+the FP64 oracle at the 1e-6 gate, on the plain form
 (N = 1,000, L = 9.41, exponential 5 20 1.3) with all 99 reference shells."""
 import os
 
@@ -17,7 +18,7 @@ pytestmark = pytest.mark.gpu
 orc.set_threads(min(16, os.cpu_count() or 1))
 
 TOL = 1e-6
-PLANS = [(2, 0), (4, 0), (8, 0), (4, 1), (8, 1), (4, 2), (8, 2), (4, 3), (8, 3), (4, 4), (8, 4)]
+PLANS = [(2, 0), (4, 0), (8, 0), (4, 1), (8, 1), (4, 2), (8, 2), (4, 3), (8, 3), (4, 4), (8, 4), (8, 5)]
 
 
 @pytest.fixture(scope="module")
@@ -55,6 +56,6 @@ def test_rho_and_fs_each_plan(ctx, data, R, pair):
 def test_hint_rejects_bad_values(ctx, data):
     tr, d, off, n, *_ = data
     sh = _abi.Shells(ctx, off, n)
-    for R, pair in [(3, 0), (2, 1), (16, 0), (4, 5), (2, 3), (2, 4)]:
+    for R, pair in [(3, 0), (2, 1), (16, 0), (4, 5), (2, 3), (2, 4), (8, 6)]:
         with pytest.raises(_abi.AmdatError):
             sh.plan_hint(R, pair)

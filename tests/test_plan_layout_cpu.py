@@ -200,17 +200,22 @@ def test_quad_layout_invariants(qvec, bins, R):
     assert int((real & isq[:, None, :]).sum()) == int(np.sum(cls // 4) * 4)
 
 
+@pytest.mark.parametrize("mode", [4, 5])
 @pytest.mark.parametrize("bins,R", [((2, 100), 8), ((2, 100), 4), ((40, 41), 8), ((2, 3), 4), ((95, 100), 8)])
-def test_grouped_layout_invariants(qvec, bins, R):
-    """Grouped plans (mode 4): every vector exactly once with its flip; a
-    thread's slots share nx; in a wave of group code g the slots of each
-    group of 1 << g share the lead slot's roles and shared component (nz
-    when swapped, else ny), which the kernel's product q is formed from;
-    codes fit R; ranges cover every slot; units past the active count hold
-    no vector."""
+def test_grouped_layout_invariants(qvec, bins, R, mode):
+    """Grouped plans (modes 4 and 5): every vector exactly once with its
+    flip; a thread's slots share nx; in a wave of group code g the slots of
+    each group of 1 << g share the lead slot's roles and shared component
+    (nz when swapped, else ny), which the kernel's product q is formed
+    from; codes fit R; ranges cover every slot; units past the active count
+    hold no vector.  Mode 5 with R = 8 marks its chunks (range bit 15) and
+    keeps the slots its kernels skip empty: 4..7 in waves of singles, 6..7
+    in waves of pairs."""
     off, nvec = ref_shells(qvec, *bins)
     K = len(nvec)
-    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R, pair=4)
+    n, nx, ny, nz, dst, rng, conflicts = layout(nvec, R, pair=mode)
+    short = mode == 5 and R == 8
+    assert np.all(((rng[:, 2] >> 15) & 1) == (1 if short else 0))
     sw = layout.swap
     real = dst >= 0
     idx = dst[real] & ~(1 << 30)
@@ -229,6 +234,9 @@ def test_grouped_layout_invariants(qvec, bins, R):
                 continue
             gs = 1 << codes[c, w]
             lanes = slice(w * 64, w * 64 + 64)
+            if short and gs <= 2:
+                ns = 4 if gs == 1 else 6
+                assert not np.any(real[c, ns:, lanes])
             for g0 in range(0, R, gs):
                 lead_sw, lead_key = sw[c, g0, lanes], key[c, g0, lanes]
                 for r in range(g0, g0 + gs):
@@ -236,7 +244,7 @@ def test_grouped_layout_invariants(qvec, bins, R):
                     assert np.all(sw[c, r, lanes][m] == lead_sw[m])
                     assert np.all(key[c, r, lanes][m] == lead_key[m])
     xlo, xhi = rng[:, 0][:, None, None], rng[:, 1][:, None, None]
-    my, mz = (rng[:, 2] & 0xFFFF)[:, None, None], (rng[:, 2] >> 16)[:, None, None]
+    my, mz = (rng[:, 2] & 0x7FFF)[:, None, None], (rng[:, 2] >> 16)[:, None, None]
     assert np.all((nx >= xlo) & (nx <= xhi) & (np.abs(ny) <= my) & (nz <= mz))
     for c in range(n):
         unit_real = real[c].any(axis=0)
