@@ -244,7 +244,7 @@ struct JobParams {
 // Launch settings of a batch job (the autotuner's knobs).
 struct JobChoice {
     int RS = 0;  // S(k): vectors per thread
-    int SM = 0;  // S(k): plan mode (0 unpaired, 1 paired, 4 grouped)
+    int SM = 0;  // S(k): plan mode (0 unpaired, 1 paired, 4 grouped, 5 grouped with short waves)
     int FG = 0;  // S(k): frames per item (0: from the item target)
     int RF = 0;  // Fs: vectors per thread
     int PC = 0;  // Fs: pairs per item (0: all pairs of a time gap)
@@ -334,9 +334,7 @@ int shell_range_M(const amdat_shells *sh, int first, int last)
 // Modelled cost of a layout per atom (PlanLayout::simd_cost): per chunk,
 // the busiest SIMD (its two waves' slots x (c - 4 + 4/GS) VALU operations
 // per lane, so uneven group sizes on one SIMD count) and the table build
-// and anchors of the chunk's sections.  64,000-atom workload: rho (28,181
-// vectors) ran 377 ms in plan mode 4 and 375 ms in mode 5, Fs (26,243
-// vectors) 238 and 230 ms.
+// and anchors of the chunk's sections.
 double layout_cost(const PlanLayout &L, double c)
 {
     (void)c;
@@ -381,6 +379,12 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
             const int cand[8][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {4, 4}, {8, 4}, {8, 5}};
             for (const auto &cd : cand) {
                 if (cd[1] && !allow_pair) continue;
+                // Mode 5 evens out the waves' work at the cost of more
+                // chunks; it pays where the kernel waits on LDS as well as
+                // VALU (Fs: 238 -> 229 ms on the 64,000-atom workload), not
+                // for rho, whose VALU issue is saturated (375 against 377
+                // ms, with 2% more VALU operations and more bank conflicts)
+                if (cd[1] == 5 && c > 7.0) continue;
                 PlanLayout L =
                     plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, cd[0], cd[1], plan_table_width(M), c, false);
                 const double cost = layout_cost(L, c);
@@ -2600,7 +2604,7 @@ int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const
             for (size_t i = 0; i < ps.frames.size(); i += 8) fr.push_back(ps.frames[i]);
             ps.frames.swap(fr);
         }
-        static const int cand[7][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {4, 4}, {8, 4}};
+        static const int cand[8][2] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {4, 4}, {8, 4}, {8, 5}};
         float t_best = 1e30f;
         for (const auto &cd : cand) {
             JobChoice c = best;
@@ -2870,10 +2874,12 @@ int amdat_batch_use_settings(amdat_batch *batch, const amdat_job_settings *s)
     auto r_ok = [](int R) { return R == 0 || R == 2 || R == 4 || R == 8; };
     if (!r_ok(s->sk_R) || !r_ok(s->fs_R))
         return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: vectors per thread must be 2, 4 or 8");
-    if (s->sk_plan_mode != 0 && s->sk_plan_mode != 1 && s->sk_plan_mode != 4)
-        return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: S(k) plan mode must be 0, 1 or 4");
+    if (s->sk_plan_mode != 0 && s->sk_plan_mode != 1 && s->sk_plan_mode != 4 && s->sk_plan_mode != 5)
+        return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: S(k) plan mode must be 0, 1, 4 or 5");
     if (s->sk_plan_mode != 0 && s->sk_R != 4 && s->sk_R != 8)
         return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: paired and grouped S(k) plans need R = 4 or 8");
+    if (s->sk_plan_mode == 5 && s->sk_R != 8)
+        return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: S(k) plan mode 5 needs R = 8");
     if (s->sk_frames_per_item < 0 || s->fs_pairs_per_item < 0 || s->fs_atom_parts < 0 || s->fs_atom_parts > 64)
         return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: frames, pairs >= 0 and atom parts in 0..64");
     JobChoice f;

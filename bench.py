@@ -34,16 +34,22 @@ ERROR_METRIC = ("max |y - y64| / max(|y64|, s): s = |S| for S(k), S(k) of the sh
 
 
 def cpu_model():
-    """The host CPU's model name (as lscpu reports it) and the logical CPUs."""
-    name = "unknown"
+    """The host CPU's model name (as lscpu reports it, with its family and
+    model numbers, which name the core generation where a virtual machine
+    shortens the name) and the logical CPUs."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
-                if line.startswith("model name"):
-                    name = line.split(":", 1)[1].strip()
-                    break
+                if not line.strip():
+                    break  # the first processor's block
+                k, _, v = line.partition(":")
+                fields[k.strip()] = v.strip()
     except OSError:
         pass
+    name = fields.get("model name", "unknown")
+    if "cpu family" in fields and "model" in fields:
+        name += " (family %s, model %s)" % (fields["cpu family"], fields["model"])
     return name, os.cpu_count() or 1
 
 

@@ -569,7 +569,8 @@ int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *paired
 
 /* Every launch setting of the last job (no counterpart in the reference,
  * which has no batched launch): S(k) vectors per thread, plan mode (0
- * unpaired, 1 paired, 4 grouped in 8, 4 or 2 slots), frames per item and
+ * unpaired, 1 paired, 4 grouped in 8, 4, 2 or 1 slots, 5 grouped with waves
+ * of singles and pairs holding 4 and 6 slots), frames per item and
  * frame groups; Fs vectors per thread, pairs per item (0: whole time
  * gaps), atom parts per pair chunk and pair chunks; atoms per LDS tile and
  * table half-width; work items and resident blocks of the persistent
@@ -593,8 +594,8 @@ int amdat_batch_job_settings(const amdat_batch *batch, amdat_job_settings *out);
  * are ignored, and AMDAT_RUN_TUNE is then ignored.  s == NULL returns to
  * the cost model and tuner.  For settings read back from
  * amdat_batch_job_settings of a tuned run, or for tests of each item form.
- * EINVAL: R not 2, 4 or 8; plan mode not 0, 1 or 4 (1 and 4 need R = 4 or
- * 8); negative counts; more than 64 atom parts. */
+ * EINVAL: R not 2, 4 or 8; plan mode not 0, 1, 4 or 5 (1 and 4 need R = 4
+ * or 8, 5 needs R = 8); negative counts; more than 64 atom parts. */
 int amdat_batch_use_settings(amdat_batch *batch, const amdat_job_settings *s);
 
 #ifdef __cplusplus
