@@ -97,10 +97,16 @@ __device__ __forceinline__ int range_gs_code(int4 r, int wave) { return (int)(((
 // Lanes are dealt to rows axis-major (lane e: axis c = e / (LANES A), atom
 // i), so the axis is uniform within a wave: the coordinate array is chosen
 // by a scalar and the loads of a wave are contiguous.
+// PER is chosen so that at most half the block's lanes (4 waves; 3 for
+// every table width here) take part: a wave runs the row's conversion once
+// whatever its lanes' d, so fewer lanes with more anchors each issue fewer
+// operations, and the stage no longer puts two waves' work on one SIMD
+// (waves w and w + 4 share one).  On the 64,000-atom workload: rho
+// (A = 16, PER 1 -> 2) 379 -> 372 ms, Fs (A = 32, PER 2 -> 4) 230 -> 222 ms.
 template <int MM>
 struct AnchorLayout {
-    static constexpr int NR = 3 * Geom<MM>::A;                 // table rows of a tile
-    static constexpr int PER = (NR * 8 <= kPhaseBlock) ? 1 : 2;  // anchors per lane
+    static constexpr int NR = 3 * Geom<MM>::A;  // table rows of a tile
+    static constexpr int PER = (NR * 8 <= kPhaseBlock / 2) ? 1 : ((NR * 8 <= kPhaseBlock) ? 2 : 4);  // anchors per lane
     static constexpr int LANES = 8 / PER;                        // lanes per row
     static constexpr int N = NR * LANES;                         // lanes with work
     static_assert(N <= kPhaseBlock, "one anchor item per lane");
