@@ -104,3 +104,15 @@ def rho_rel(got, ref, n_atoms):
     error where the sum is coherent (|rho| above sqrt(N)) and the error on
     the sqrt(N) scale of a sum of N random phases otherwise."""
     return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), np.sqrt(n_atoms))))
+
+
+def random_vectors(M, n_shells=3, per_shell=60, seed=0):
+    """Integer vectors with components in [-M, M] (at least one at +-M per
+    shell), including nz < 0, in shells of per_shell vectors."""
+    rng = np.random.default_rng(seed + M)
+    n = rng.integers(-M, M + 1, size=(n_shells * per_shell, 3)).astype(np.int32)
+    n[::per_shell, 0] = M
+    n[1::per_shell, 2] = -M
+    n[np.all(n == 0, axis=1)] = (1, 0, 0)
+    off = np.arange(0, n_shells * per_shell + 1, per_shell).astype(np.int32)
+    return off, n

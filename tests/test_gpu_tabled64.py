@@ -14,7 +14,7 @@ import pytest
 
 from amdat_amd import _abi, synth
 from oracle import oracle as orc
-from tests.conftest import ref_shells, rho_rel
+from tests.conftest import random_vectors, ref_shells, rho_rel
 
 pytestmark = pytest.mark.gpu
 orc.set_threads(min(16, os.cpu_count() or 1))
@@ -54,18 +54,6 @@ def test_tabled64_each_plan(ctx, data, R, pair):
     print("R=%d pair=%d tabled FP64: rho per-vector err %.3e, Fs err %.3e" % (R, pair, err, eF))
     assert err < TOL_T64
     assert eF < TOL_T64
-
-
-def random_vectors(M, n_shells=3, per_shell=60, seed=0):
-    """Integer vectors with components in [-M, M] (at least one at +-M per
-    shell), including nz < 0, in shells of per_shell vectors."""
-    rng = np.random.default_rng(seed + M)
-    n = rng.integers(-M, M + 1, size=(n_shells * per_shell, 3)).astype(np.int32)
-    n[::per_shell, 0] = M
-    n[1::per_shell, 2] = -M
-    n[np.all(n == 0, axis=1)] = (1, 0, 0)
-    off = np.arange(0, n_shells * per_shell + 1, per_shell).astype(np.int32)
-    return off, n
 
 
 @pytest.mark.parametrize("M", [20, 45, 95, 127])
