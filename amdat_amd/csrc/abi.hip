@@ -584,7 +584,7 @@ extern "C" {
 
 const char *amdat_last_error(void) { return g_last_error.c_str(); }
 
-int amdat_ctx_create(int device, amdat_ctx **out)
+static int amdat_ctx_create_body(int device, amdat_ctx **out)
 {
     if (!out) return set_error(AMDAT_EINVAL, "amdat_ctx_create: null output");
     int n = 0;
@@ -600,7 +600,12 @@ int amdat_ctx_create(int device, amdat_ctx **out)
     return AMDAT_OK;
 }
 
-int amdat_ctx_destroy(amdat_ctx *ctx)
+int amdat_ctx_create(int device, amdat_ctx **out)
+{
+    AMDAT_GUARD("amdat_ctx_create", amdat_ctx_create_body(device, out))
+}
+
+static int amdat_ctx_destroy_body(amdat_ctx *ctx)
 {
     if (!ctx) return AMDAT_OK;
     (void)hipSetDevice(ctx->device);
@@ -609,21 +614,36 @@ int amdat_ctx_destroy(amdat_ctx *ctx)
     return AMDAT_OK;
 }
 
-int amdat_ctx_set_stream(amdat_ctx *ctx, void *stream)
+int amdat_ctx_destroy(amdat_ctx *ctx)
+{
+    AMDAT_GUARD("amdat_ctx_destroy", amdat_ctx_destroy_body(ctx))
+}
+
+static int amdat_ctx_set_stream_body(amdat_ctx *ctx, void *stream)
 {
     if (!ctx) return set_error(AMDAT_EINVAL, "null context");
     ctx->stream = (hipStream_t)stream;
     return AMDAT_OK;
 }
 
-int amdat_ctx_synchronize(amdat_ctx *ctx)
+int amdat_ctx_set_stream(amdat_ctx *ctx, void *stream)
+{
+    AMDAT_GUARD("amdat_ctx_set_stream", amdat_ctx_set_stream_body(ctx, stream))
+}
+
+static int amdat_ctx_synchronize_body(amdat_ctx *ctx)
 {
     if (!ctx) return set_error(AMDAT_EINVAL, "null context");
     HIP_TRY(hipStreamSynchronize(stream_of(ctx)));
     return AMDAT_OK;
 }
 
-int amdat_shells_upload(amdat_ctx *ctx, int n_shells, const int *shell_off, const int *nvec, amdat_shells **out)
+int amdat_ctx_synchronize(amdat_ctx *ctx)
+{
+    AMDAT_GUARD("amdat_ctx_synchronize", amdat_ctx_synchronize_body(ctx))
+}
+
+static int amdat_shells_upload_body(amdat_ctx *ctx, int n_shells, const int *shell_off, const int *nvec, amdat_shells **out)
 {
     if (!ctx || !shell_off || !out || n_shells < 1) return set_error(AMDAT_EINVAL, "amdat_shells_upload: bad arguments");
     const int K = shell_off[n_shells];
@@ -640,7 +660,12 @@ int amdat_shells_upload(amdat_ctx *ctx, int n_shells, const int *shell_off, cons
     return AMDAT_OK;
 }
 
-int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair)
+int amdat_shells_upload(amdat_ctx *ctx, int n_shells, const int *shell_off, const int *nvec, amdat_shells **out)
+{
+    AMDAT_GUARD("amdat_shells_upload", amdat_shells_upload_body(ctx, n_shells, shell_off, nvec, out))
+}
+
+static int amdat_shells_plan_hint_body(amdat_shells *sh, int R, int pair)
 {
     clear_error();
     if (!sh) return set_error(AMDAT_EINVAL, "amdat_shells_plan_hint: null table");
@@ -653,10 +678,21 @@ int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair)
     return AMDAT_OK;
 }
 
-int amdat_shells_free(amdat_ctx *, amdat_shells *sh)
+int amdat_shells_plan_hint(amdat_shells *sh, int R, int pair)
 {
+    AMDAT_GUARD("amdat_shells_plan_hint", amdat_shells_plan_hint_body(sh, R, pair))
+}
+
+static int amdat_shells_free_body(amdat_ctx *ctx_unused, amdat_shells *sh)
+{
+    (void)ctx_unused;
     delete sh;
     return AMDAT_OK;
+}
+
+int amdat_shells_free(amdat_ctx *ctx_unused, amdat_shells *sh)
+{
+    AMDAT_GUARD("amdat_shells_free", amdat_shells_free_body(ctx_unused, sh))
 }
 
 static int traj_finish(amdat_ctx *ctx, amdat_traj *t, int n_atoms, int n_frames, const float *box)
@@ -670,7 +706,7 @@ static int traj_finish(amdat_ctx *ctx, amdat_traj *t, int n_atoms, int n_frames,
     return AMDAT_OK;
 }
 
-int amdat_traj_upload(amdat_ctx *ctx, int n_atoms, int n_frames, const float *x, const float *y, const float *z,
+static int amdat_traj_upload_body(amdat_ctx *ctx, int n_atoms, int n_frames, const float *x, const float *y, const float *z,
                       const int16_t *ix, const int16_t *iy, const int16_t *iz, const float *box, amdat_traj **out)
 {
     if (!ctx || !out || n_atoms < 1 || n_frames < 1 || !x || !y || !z || !ix || !iy || !iz || !box)
@@ -693,7 +729,13 @@ int amdat_traj_upload(amdat_ctx *ctx, int n_atoms, int n_frames, const float *x,
     return AMDAT_OK;
 }
 
-int amdat_traj_wrap_device(amdat_ctx *ctx, int n_atoms, int n_frames, const float *d_x, const float *d_y,
+int amdat_traj_upload(amdat_ctx *ctx, int n_atoms, int n_frames, const float *x, const float *y, const float *z,
+                      const int16_t *ix, const int16_t *iy, const int16_t *iz, const float *box, amdat_traj **out)
+{
+    AMDAT_GUARD("amdat_traj_upload", amdat_traj_upload_body(ctx, n_atoms, n_frames, x, y, z, ix, iy, iz, box, out))
+}
+
+static int amdat_traj_wrap_device_body(amdat_ctx *ctx, int n_atoms, int n_frames, const float *d_x, const float *d_y,
                            const float *d_z, const int16_t *d_ix, const int16_t *d_iy, const int16_t *d_iz,
                            const float *box_host, amdat_traj **out)
 {
@@ -708,13 +750,26 @@ int amdat_traj_wrap_device(amdat_ctx *ctx, int n_atoms, int n_frames, const floa
     return AMDAT_OK;
 }
 
-int amdat_traj_free(amdat_ctx *, amdat_traj *tr)
+int amdat_traj_wrap_device(amdat_ctx *ctx, int n_atoms, int n_frames, const float *d_x, const float *d_y,
+                           const float *d_z, const int16_t *d_ix, const int16_t *d_iy, const int16_t *d_iz,
+                           const float *box_host, amdat_traj **out)
 {
+    AMDAT_GUARD("amdat_traj_wrap_device", amdat_traj_wrap_device_body(ctx, n_atoms, n_frames, d_x, d_y, d_z, d_ix, d_iy, d_iz, box_host, out))
+}
+
+static int amdat_traj_free_body(amdat_ctx *ctx_unused, amdat_traj *tr)
+{
+    (void)ctx_unused;
     delete tr;
     return AMDAT_OK;
 }
 
-int amdat_traj_shape(const amdat_traj *tr, int *n_atoms, int *n_frames, float *box)
+int amdat_traj_free(amdat_ctx *ctx_unused, amdat_traj *tr)
+{
+    AMDAT_GUARD("amdat_traj_free", amdat_traj_free_body(ctx_unused, tr))
+}
+
+static int amdat_traj_shape_body(const amdat_traj *tr, int *n_atoms, int *n_frames, float *box)
 {
     if (!tr) return set_error(AMDAT_EINVAL, "amdat_traj_shape: null trajectory");
     if (n_atoms) *n_atoms = tr->dev.N;
@@ -723,11 +778,21 @@ int amdat_traj_shape(const amdat_traj *tr, int *n_atoms, int *n_frames, float *b
     return AMDAT_OK;
 }
 
-int amdat_traj_bounds(const amdat_traj *tr, float *lo)
+int amdat_traj_shape(const amdat_traj *tr, int *n_atoms, int *n_frames, float *box)
+{
+    AMDAT_GUARD("amdat_traj_shape", amdat_traj_shape_body(tr, n_atoms, n_frames, box))
+}
+
+static int amdat_traj_bounds_body(const amdat_traj *tr, float *lo)
 {
     if (!tr || !lo) return set_error(AMDAT_EINVAL, "amdat_traj_bounds: null argument");
     std::copy(tr->lo.begin(), tr->lo.end(), lo);
     return AMDAT_OK;
+}
+
+int amdat_traj_bounds(const amdat_traj *tr, float *lo)
+{
+    AMDAT_GUARD("amdat_traj_bounds", amdat_traj_bounds_body(tr, lo))
 }
 
 }  // extern "C"
@@ -757,7 +822,7 @@ int cells_rect(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, i
 
 extern "C" {
 
-int amdat_bins_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+static int amdat_bins_assign_body(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
                       const float *hi, int *cell)
 {
     clear_error();
@@ -771,7 +836,13 @@ int amdat_bins_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_y
     return AMDAT_OK;
 }
 
-int amdat_cells_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+int amdat_bins_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+                      const float *hi, int *cell)
+{
+    AMDAT_GUARD("amdat_bins_assign", amdat_bins_assign_body(ctx, tr, n_xbins, n_ybins, n_zbins, lo, hi, cell))
+}
+
+static int amdat_cells_assign_body(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
                        const float *hi, amdat_cells **out)
 {
     clear_error();
@@ -784,6 +855,12 @@ int amdat_cells_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_
     c->n_cells = n_xbins * n_ybins * n_zbins;
     *out = c.release();
     return AMDAT_OK;
+}
+
+int amdat_cells_assign(amdat_ctx *ctx, const amdat_traj *tr, int n_xbins, int n_ybins, int n_zbins, const float *lo,
+                       const float *hi, amdat_cells **out)
+{
+    AMDAT_GUARD("amdat_cells_assign", amdat_cells_assign_body(ctx, tr, n_xbins, n_ybins, n_zbins, lo, hi, out))
 }
 
 }  // extern "C"
@@ -877,7 +954,7 @@ int cells_dist(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, i
 
 extern "C" {
 
-int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+static int amdat_bins_distance_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
                         const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
                         int n_bins, int *cell)
 {
@@ -893,7 +970,14 @@ int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
     return AMDAT_OK;
 }
 
-int amdat_cells_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+int amdat_bins_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+                        const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
+                        int n_bins, int *cell)
+{
+    AMDAT_GUARD("amdat_bins_distance", amdat_bins_distance_body(ctx, tr, binned, kind, point, axis, direction, cluster, thickness, n_bins, cell))
+}
+
+static int amdat_cells_distance_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
                          const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
                          int n_bins, amdat_cells **out)
 {
@@ -910,7 +994,14 @@ int amdat_cells_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list 
     return AMDAT_OK;
 }
 
-int amdat_cells_download(amdat_ctx *ctx, const amdat_cells *c, int *cell)
+int amdat_cells_distance(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *binned, int kind,
+                         const float *point, int axis, int direction, const amdat_list *cluster, float thickness,
+                         int n_bins, amdat_cells **out)
+{
+    AMDAT_GUARD("amdat_cells_distance", amdat_cells_distance_body(ctx, tr, binned, kind, point, axis, direction, cluster, thickness, n_bins, out))
+}
+
+static int amdat_cells_download_body(amdat_ctx *ctx, const amdat_cells *c, int *cell)
 {
     clear_error();
     if (!ctx || !c || !cell) return set_error(AMDAT_EINVAL, "amdat_cells_download: null argument");
@@ -921,11 +1012,21 @@ int amdat_cells_download(amdat_ctx *ctx, const amdat_cells *c, int *cell)
     return AMDAT_OK;
 }
 
-int amdat_cells_free(amdat_ctx *ctx, amdat_cells *c)
+int amdat_cells_download(amdat_ctx *ctx, const amdat_cells *c, int *cell)
+{
+    AMDAT_GUARD("amdat_cells_download", amdat_cells_download_body(ctx, c, cell))
+}
+
+static int amdat_cells_free_body(amdat_ctx *ctx, amdat_cells *c)
 {
     if (ctx && c) (void)hipSetDevice(ctx->device);
     delete c;
     return AMDAT_OK;
+}
+
+int amdat_cells_free(amdat_ctx *ctx, amdat_cells *c)
+{
+    AMDAT_GUARD("amdat_cells_free", amdat_cells_free_body(ctx, c))
 }
 
 }  // extern "C"
@@ -1018,7 +1119,7 @@ int cells_args(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int fi
 
 extern "C" {
 
-int amdat_cells_bin_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int first_bin, int n_bins,
+static int amdat_cells_bin_lists_body(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int first_bin, int n_bins,
                           amdat_list **lists)
 {
     clear_error();
@@ -1039,7 +1140,13 @@ int amdat_cells_bin_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_list
     return AMDAT_OK;
 }
 
-int amdat_cells_pair_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int n_keys, const int *t0,
+int amdat_cells_bin_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int first_bin, int n_bins,
+                          amdat_list **lists)
+{
+    AMDAT_GUARD("amdat_cells_bin_lists", amdat_cells_bin_lists_body(ctx, c, l, first_bin, n_bins, lists))
+}
+
+static int amdat_cells_pair_lists_body(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int n_keys, const int *t0,
                            const int *t1, int first_bin, int n_bins, amdat_list **lists)
 {
     clear_error();
@@ -1068,7 +1175,13 @@ int amdat_cells_pair_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_lis
     return AMDAT_OK;
 }
 
-int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_atoms, int n_frames,
+int amdat_cells_pair_lists(amdat_ctx *ctx, const amdat_cells *c, const amdat_list *l, int n_keys, const int *t0,
+                           const int *t1, int first_bin, int n_bins, amdat_list **lists)
+{
+    AMDAT_GUARD("amdat_cells_pair_lists", amdat_cells_pair_lists_body(ctx, c, l, n_keys, t0, t1, first_bin, n_bins, lists))
+}
+
+static int amdat_traj_read_body(amdat_ctx *ctx, const char *format, const char *path, int n_atoms, int n_frames,
                     const float *box3, int threads, int *types, amdat_traj **out)
 {
     clear_error();
@@ -1164,7 +1277,13 @@ int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_
     return AMDAT_OK;
 }
 
-int amdat_list_create(amdat_ctx *ctx, const amdat_traj *tr, int n, const int *indices, amdat_list **out)
+int amdat_traj_read(amdat_ctx *ctx, const char *format, const char *path, int n_atoms, int n_frames,
+                    const float *box3, int threads, int *types, amdat_traj **out)
+{
+    AMDAT_GUARD("amdat_traj_read", amdat_traj_read_body(ctx, format, path, n_atoms, n_frames, box3, threads, types, out))
+}
+
+static int amdat_list_create_body(amdat_ctx *ctx, const amdat_traj *tr, int n, const int *indices, amdat_list **out)
 {
     if (!ctx || !tr || !out || n < 0) return set_error(AMDAT_EINVAL, "amdat_list_create: bad arguments");
     auto l = std::make_unique<amdat_list>();
@@ -1182,13 +1301,24 @@ int amdat_list_create(amdat_ctx *ctx, const amdat_traj *tr, int n, const int *in
     return AMDAT_OK;
 }
 
-int amdat_list_free(amdat_ctx *, amdat_list *l)
+int amdat_list_create(amdat_ctx *ctx, const amdat_traj *tr, int n, const int *indices, amdat_list **out)
 {
+    AMDAT_GUARD("amdat_list_create", amdat_list_create_body(ctx, tr, n, indices, out))
+}
+
+static int amdat_list_free_body(amdat_ctx *ctx_unused, amdat_list *l)
+{
+    (void)ctx_unused;
     delete l;
     return AMDAT_OK;
 }
 
-int amdat_list_create_timed(amdat_ctx *ctx, const amdat_traj *tr, int n_times, const int *time_conv,
+int amdat_list_free(amdat_ctx *ctx_unused, amdat_list *l)
+{
+    AMDAT_GUARD("amdat_list_free", amdat_list_free_body(ctx_unused, l))
+}
+
+static int amdat_list_create_timed_body(amdat_ctx *ctx, const amdat_traj *tr, int n_times, const int *time_conv,
                             const int *off, const int *indices, amdat_list **out)
 {
     clear_error();
@@ -1214,7 +1344,13 @@ int amdat_list_create_timed(amdat_ctx *ctx, const amdat_traj *tr, int n_times, c
     return AMDAT_OK;
 }
 
-int amdat_list_create_paired(amdat_ctx *ctx, const amdat_traj *tr, int n_keys, const int *t0, const int *t1,
+int amdat_list_create_timed(amdat_ctx *ctx, const amdat_traj *tr, int n_times, const int *time_conv,
+                            const int *off, const int *indices, amdat_list **out)
+{
+    AMDAT_GUARD("amdat_list_create_timed", amdat_list_create_timed_body(ctx, tr, n_times, time_conv, off, indices, out))
+}
+
+static int amdat_list_create_paired_body(amdat_ctx *ctx, const amdat_traj *tr, int n_keys, const int *t0, const int *t1,
                              const int *off, const int *indices, amdat_list **out)
 {
     clear_error();
@@ -1245,7 +1381,13 @@ int amdat_list_create_paired(amdat_ctx *ctx, const amdat_traj *tr, int n_keys, c
     return AMDAT_OK;
 }
 
-int amdat_list_shape(const amdat_list *l, int *n_times, int *n_entries)
+int amdat_list_create_paired(amdat_ctx *ctx, const amdat_traj *tr, int n_keys, const int *t0, const int *t1,
+                             const int *off, const int *indices, amdat_list **out)
+{
+    AMDAT_GUARD("amdat_list_create_paired", amdat_list_create_paired_body(ctx, tr, n_keys, t0, t1, off, indices, out))
+}
+
+static int amdat_list_shape_body(const amdat_list *l, int *n_times, int *n_entries)
 {
     if (!l) return set_error(AMDAT_EINVAL, "amdat_list_shape: null list");
     if (n_times) *n_times = l->n_times;
@@ -1253,7 +1395,12 @@ int amdat_list_shape(const amdat_list *l, int *n_times, int *n_entries)
     return AMDAT_OK;
 }
 
-int amdat_list_download(amdat_ctx *ctx, const amdat_list *l, int *time_conv, int *off, int *indices)
+int amdat_list_shape(const amdat_list *l, int *n_times, int *n_entries)
+{
+    AMDAT_GUARD("amdat_list_shape", amdat_list_shape_body(l, n_times, n_entries))
+}
+
+static int amdat_list_download_body(amdat_ctx *ctx, const amdat_list *l, int *time_conv, int *off, int *indices)
 {
     clear_error();
     if (!ctx || !l) return set_error(AMDAT_EINVAL, "amdat_list_download: null argument");
@@ -1274,7 +1421,12 @@ int amdat_list_download(amdat_ctx *ctx, const amdat_list *l, int *time_conv, int
     return AMDAT_OK;
 }
 
-int amdat_list_select_displacement(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *in, int n_exponentials,
+int amdat_list_download(amdat_ctx *ctx, const amdat_list *l, int *time_conv, int *off, int *indices)
+{
+    AMDAT_GUARD("amdat_list_download", amdat_list_download_body(ctx, l, time_conv, off, indices))
+}
+
+static int amdat_list_select_displacement_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *in, int n_exponentials,
                                    int n_exponential_steps, int frt, int limit, int time_index, int mode, float lo,
                                    float hi, amdat_list **out)
 {
@@ -1368,7 +1520,14 @@ int amdat_list_select_displacement(amdat_ctx *ctx, const amdat_traj *tr, const a
     return AMDAT_OK;
 }
 
-int amdat_pairs_upload(amdat_ctx *ctx, int n_pairs, const int *tg, const int *t0, const int *t1, int n_timegaps,
+int amdat_list_select_displacement(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *in, int n_exponentials,
+                                   int n_exponential_steps, int frt, int limit, int time_index, int mode, float lo,
+                                   float hi, amdat_list **out)
+{
+    AMDAT_GUARD("amdat_list_select_displacement", amdat_list_select_displacement_body(ctx, tr, in, n_exponentials, n_exponential_steps, frt, limit, time_index, mode, lo, hi, out))
+}
+
+static int amdat_pairs_upload_body(amdat_ctx *ctx, int n_pairs, const int *tg, const int *t0, const int *t1, int n_timegaps,
                        amdat_pairs **out)
 {
     if (!ctx || !out || n_pairs < 0 || n_timegaps < 1 || (n_pairs > 0 && (!tg || !t0 || !t1)))
@@ -1398,10 +1557,22 @@ int amdat_pairs_upload(amdat_ctx *ctx, int n_pairs, const int *tg, const int *t0
     return AMDAT_OK;
 }
 
-int amdat_pairs_free(amdat_ctx *, amdat_pairs *p)
+int amdat_pairs_upload(amdat_ctx *ctx, int n_pairs, const int *tg, const int *t0, const int *t1, int n_timegaps,
+                       amdat_pairs **out)
 {
+    AMDAT_GUARD("amdat_pairs_upload", amdat_pairs_upload_body(ctx, n_pairs, tg, t0, t1, n_timegaps, out))
+}
+
+static int amdat_pairs_free_body(amdat_ctx *ctx_unused, amdat_pairs *p)
+{
+    (void)ctx_unused;
     delete p;
     return AMDAT_OK;
+}
+
+int amdat_pairs_free(amdat_ctx *ctx_unused, amdat_pairs *p)
+{
+    AMDAT_GUARD("amdat_pairs_free", amdat_pairs_free_body(ctx_unused, p))
 }
 
 static int check_pairs_frames(const amdat_pairs *p, int T)
@@ -1411,7 +1582,7 @@ static int check_pairs_frames(const amdat_pairs *p, int T)
     return AMDAT_OK;
 }
 
-int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_shells *sh_c,
+static int amdat_rho_compute_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_shells *sh_c,
                       int first_shell, int last_shell, float length_scale, const int *frames, int n_frames,
                       int precision, amdat_rho **out)
 {
@@ -1509,7 +1680,14 @@ int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *li
     return AMDAT_OK;
 }
 
-int amdat_rho_download(amdat_ctx *ctx, const amdat_rho *rho, double *out, size_t n_doubles)
+int amdat_rho_compute(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_shells *sh_c,
+                      int first_shell, int last_shell, float length_scale, const int *frames, int n_frames,
+                      int precision, amdat_rho **out)
+{
+    AMDAT_GUARD("amdat_rho_compute", amdat_rho_compute_body(ctx, tr, list, sh_c, first_shell, last_shell, length_scale, frames, n_frames, precision, out))
+}
+
+static int amdat_rho_download_body(amdat_ctx *ctx, const amdat_rho *rho, double *out, size_t n_doubles)
 {
     if (!ctx || !rho || !out) return set_error(AMDAT_EINVAL, "amdat_rho_download: null argument");
     const size_t need = (size_t)rho->n_frames * rho->K * 2;
@@ -1519,7 +1697,12 @@ int amdat_rho_download(amdat_ctx *ctx, const amdat_rho *rho, double *out, size_t
     return AMDAT_OK;
 }
 
-int amdat_rho_shape(const amdat_rho *rho, int *n_frames, int *n_vectors)
+int amdat_rho_download(amdat_ctx *ctx, const amdat_rho *rho, double *out, size_t n_doubles)
+{
+    AMDAT_GUARD("amdat_rho_download", amdat_rho_download_body(ctx, rho, out, n_doubles))
+}
+
+static int amdat_rho_shape_body(const amdat_rho *rho, int *n_frames, int *n_vectors)
 {
     if (!rho) return set_error(AMDAT_EINVAL, "amdat_rho_shape: null argument");
     if (n_frames) *n_frames = rho->n_frames;
@@ -1527,13 +1710,24 @@ int amdat_rho_shape(const amdat_rho *rho, int *n_frames, int *n_vectors)
     return AMDAT_OK;
 }
 
-int amdat_rho_free(amdat_ctx *, amdat_rho *rho)
+int amdat_rho_shape(const amdat_rho *rho, int *n_frames, int *n_vectors)
 {
+    AMDAT_GUARD("amdat_rho_shape", amdat_rho_shape_body(rho, n_frames, n_vectors))
+}
+
+static int amdat_rho_free_body(amdat_ctx *ctx_unused, amdat_rho *rho)
+{
+    (void)ctx_unused;
     delete rho;
     return AMDAT_OK;
 }
 
-int amdat_sk(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, double *S)
+int amdat_rho_free(amdat_ctx *ctx_unused, amdat_rho *rho)
+{
+    AMDAT_GUARD("amdat_rho_free", amdat_rho_free_body(ctx_unused, rho))
+}
+
+static int amdat_sk_body(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, double *S)
 {
     clear_error();
     if (!ctx || !r1 || !r2 || !S) return set_error(AMDAT_EINVAL, "amdat_sk: null argument");
@@ -1562,6 +1756,11 @@ int amdat_sk(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, double *S
     HIP_TRY(hipMemcpyAsync(S, d_out.p, sizeof(double) * r1->n_shells, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
+}
+
+int amdat_sk(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, double *S)
+{
+    AMDAT_GUARD("amdat_sk", amdat_sk_body(ctx, r1, r2, S))
 }
 
 }  // extern "C"
@@ -1613,7 +1812,7 @@ int fkt_columns(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const 
 
 extern "C" {
 
-int amdat_fkt(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs, int wn1, int wn2,
+static int amdat_fkt_body(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs, int wn1, int wn2,
               double *F, double *count)
 {
     clear_error();
@@ -1624,7 +1823,13 @@ int amdat_fkt(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const am
     return fkt_columns(ctx, r1, r2, pairs, wn2 - wn1 + 1, nullptr, nullptr, wn1, F, count, "amdat_fkt");
 }
 
-int amdat_fkt_vectors(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs,
+int amdat_fkt(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs, int wn1, int wn2,
+              double *F, double *count)
+{
+    AMDAT_GUARD("amdat_fkt", amdat_fkt_body(ctx, r1, r2, pairs, wn1, wn2, F, count))
+}
+
+static int amdat_fkt_vectors_body(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs,
                       int n_cols, const int *v_first, const int *v_count, double *F, double *count)
 {
     clear_error();
@@ -1645,7 +1850,13 @@ int amdat_fkt_vectors(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, 
     return fkt_columns(ctx, r1, r2, pairs, n_cols, d_b.p, d_e.p, 0, F, count, "amdat_fkt_vectors");
 }
 
-int amdat_rho_sum(amdat_ctx *ctx, const amdat_rho *a, const amdat_rho *b, amdat_rho **out)
+int amdat_fkt_vectors(amdat_ctx *ctx, const amdat_rho *r1, const amdat_rho *r2, const amdat_pairs *pairs,
+                      int n_cols, const int *v_first, const int *v_count, double *F, double *count)
+{
+    AMDAT_GUARD("amdat_fkt_vectors", amdat_fkt_vectors_body(ctx, r1, r2, pairs, n_cols, v_first, v_count, F, count))
+}
+
+static int amdat_rho_sum_body(amdat_ctx *ctx, const amdat_rho *a, const amdat_rho *b, amdat_rho **out)
 {
     clear_error();
     if (!ctx || !a || !b || !out) return set_error(AMDAT_EINVAL, "amdat_rho_sum: null argument");
@@ -1675,7 +1886,12 @@ int amdat_rho_sum(amdat_ctx *ctx, const amdat_rho *a, const amdat_rho *b, amdat_
     return AMDAT_OK;
 }
 
-int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_shells *sh_c, int first_shell,
+int amdat_rho_sum(amdat_ctx *ctx, const amdat_rho *a, const amdat_rho *b, amdat_rho **out)
+{
+    AMDAT_GUARD("amdat_rho_sum", amdat_rho_sum_body(ctx, a, b, out))
+}
+
+static int amdat_fs_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_shells *sh_c, int first_shell,
              int last_shell, float length_scale, const amdat_pairs *pairs, int precision, double *Fs, double *count)
 {
     clear_error();
@@ -1735,7 +1951,13 @@ int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const
     return AMDAT_OK;
 }
 
-int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs, double *s2,
+int amdat_fs(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_shells *sh_c, int first_shell,
+             int last_shell, float length_scale, const amdat_pairs *pairs, int precision, double *Fs, double *count)
+{
+    AMDAT_GUARD("amdat_fs", amdat_fs_body(ctx, tr, list, sh_c, first_shell, last_shell, length_scale, pairs, precision, Fs, count))
+}
+
+static int amdat_msd_ngp_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs, double *s2,
                   double *s4, double *count)
 {
     clear_error();
@@ -1778,7 +2000,13 @@ int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, 
     return AMDAT_OK;
 }
 
-int amdat_msd_ngp_noalloc(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+int amdat_msd_ngp(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs, double *s2,
+                  double *s4, double *count)
+{
+    AMDAT_GUARD("amdat_msd_ngp", amdat_msd_ngp_body(ctx, tr, list, pairs, s2, s4, count))
+}
+
+static int amdat_msd_ngp_noalloc_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
                           double *s2, double *s4, double *count)
 {
     clear_error();
@@ -1825,6 +2053,12 @@ int amdat_msd_ngp_noalloc(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list
     HIP_TRY(hipMemcpyAsync(s4, d4, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
+}
+
+int amdat_msd_ngp_noalloc(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                          double *s2, double *s4, double *count)
+{
+    AMDAT_GUARD("amdat_msd_ngp_noalloc", amdat_msd_ngp_noalloc_body(ctx, tr, list, pairs, s2, s4, count))
 }
 
 }  // extern "C"
@@ -1889,7 +2123,7 @@ int displacements_impl(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *l
 
 extern "C" {
 
-int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+static int amdat_displacements_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
                         int n_bins, float max_range, double *sum_d, double *sum_d2, unsigned long long *hist,
                         double *count)
 {
@@ -1903,7 +2137,14 @@ int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *
     return displacements_impl(ctx, tr, list, pairs, n_bins, 0, bin_size, sum_d, sum_d2, hist, count);
 }
 
-int amdat_displacement_dist(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+int amdat_displacements(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                        int n_bins, float max_range, double *sum_d, double *sum_d2, unsigned long long *hist,
+                        double *count)
+{
+    AMDAT_GUARD("amdat_displacements", amdat_displacements_body(ctx, tr, list, pairs, n_bins, max_range, sum_d, sum_d2, hist, count))
+}
+
+static int amdat_displacement_dist_body(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
                             int n_bins, float max_value, unsigned long long *counts, double *sum_d2,
                             double *sum_d4, double *count)
 {
@@ -1932,7 +2173,14 @@ int amdat_displacement_dist(amdat_ctx *ctx, const amdat_traj *tr, const amdat_li
     return AMDAT_OK;
 }
 
-int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const float *d_y, const float *d_z,
+int amdat_displacement_dist(amdat_ctx *ctx, const amdat_traj *tr, const amdat_list *list, const amdat_pairs *pairs,
+                            int n_bins, float max_value, unsigned long long *counts, double *sum_d2,
+                            double *sum_d4, double *count)
+{
+    AMDAT_GUARD("amdat_displacement_dist", amdat_displacement_dist_body(ctx, tr, list, pairs, n_bins, max_value, counts, sum_d2, sum_d4, count))
+}
+
+static int amdat_sk_loss_grad_body(amdat_ctx *ctx, int n_atoms, const float *d_x, const float *d_y, const float *d_z,
                        const amdat_shells *sh_c, int first_shell, int last_shell, float length_scale,
                        const double *weights, const double *target, double *loss, double *S, double *d_gx,
                        double *d_gy, double *d_gz)
@@ -2020,7 +2268,15 @@ int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const floa
     return AMDAT_OK;
 }
 
-int amdat_batch_create(amdat_ctx *ctx, int n_traj, const amdat_traj *const *trajs, const amdat_list *const *lists,
+int amdat_sk_loss_grad(amdat_ctx *ctx, int n_atoms, const float *d_x, const float *d_y, const float *d_z,
+                       const amdat_shells *sh_c, int first_shell, int last_shell, float length_scale,
+                       const double *weights, const double *target, double *loss, double *S, double *d_gx,
+                       double *d_gy, double *d_gz)
+{
+    AMDAT_GUARD("amdat_sk_loss_grad", amdat_sk_loss_grad_body(ctx, n_atoms, d_x, d_y, d_z, sh_c, first_shell, last_shell, length_scale, weights, target, loss, S, d_gx, d_gy, d_gz))
+}
+
+static int amdat_batch_create_body(amdat_ctx *ctx, int n_traj, const amdat_traj *const *trajs, const amdat_list *const *lists,
                        amdat_batch **out)
 {
     clear_error();
@@ -2047,10 +2303,22 @@ int amdat_batch_create(amdat_ctx *ctx, int n_traj, const amdat_traj *const *traj
     return AMDAT_OK;
 }
 
-int amdat_batch_free(amdat_ctx *, amdat_batch *b)
+int amdat_batch_create(amdat_ctx *ctx, int n_traj, const amdat_traj *const *trajs, const amdat_list *const *lists,
+                       amdat_batch **out)
 {
+    AMDAT_GUARD("amdat_batch_create", amdat_batch_create_body(ctx, n_traj, trajs, lists, out))
+}
+
+static int amdat_batch_free_body(amdat_ctx *ctx_unused, amdat_batch *b)
+{
+    (void)ctx_unused;
     delete b;
     return AMDAT_OK;
+}
+
+int amdat_batch_free(amdat_ctx *ctx_unused, amdat_batch *b)
+{
+    AMDAT_GUARD("amdat_batch_free", amdat_batch_free_body(ctx_unused, b))
 }
 
 static BatchTraj batch_entry(const amdat_batch *b, int i, double inv_ell, int K, double *out)
@@ -2080,7 +2348,7 @@ static void order_items(std::vector<int4> &items, const std::vector<double> &cos
     items.swap(sorted);
 }
 
-int amdat_batch_sk(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c, int first_shell, int last_shell,
+static int amdat_batch_sk_body(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c, int first_shell, int last_shell,
                    const float *length_scales, const int *frames, int n_frames, double *S, double *count)
 {
     clear_error();
@@ -2153,7 +2421,13 @@ int amdat_batch_sk(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
     return AMDAT_OK;
 }
 
-int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c, const int *first_shell,
+int amdat_batch_sk(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c, int first_shell, int last_shell,
+                   const float *length_scales, const int *frames, int n_frames, double *S, double *count)
+{
+    AMDAT_GUARD("amdat_batch_sk", amdat_batch_sk_body(ctx, batch, sh_c, first_shell, last_shell, length_scales, frames, n_frames, S, count))
+}
+
+static int amdat_batch_fs_body(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c, const int *first_shell,
                    const int *last_shell, const float *length_scales, const amdat_pairs *pairs, double *Fs,
                    double *count)
 {
@@ -2261,7 +2535,14 @@ int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c,
     return AMDAT_OK;
 }
 
-int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *pairs, double *s2, double *s4,
+int amdat_batch_fs(amdat_ctx *ctx, amdat_batch *batch, const amdat_shells *sh_c, const int *first_shell,
+                   const int *last_shell, const float *length_scales, const amdat_pairs *pairs, double *Fs,
+                   double *count)
+{
+    AMDAT_GUARD("amdat_batch_fs", amdat_batch_fs_body(ctx, batch, sh_c, first_shell, last_shell, length_scales, pairs, Fs, count))
+}
+
+static int amdat_batch_msd_ngp_body(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *pairs, double *s2, double *s4,
                         double *count)
 {
     clear_error();
@@ -2300,6 +2581,12 @@ int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *p
     HIP_TRY(hipMemcpyAsync(s4, d4.p, sizeof(double) * n * ntg, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return AMDAT_OK;
+}
+
+int amdat_batch_msd_ngp(amdat_ctx *ctx, amdat_batch *batch, const amdat_pairs *pairs, double *s2, double *s4,
+                        double *count)
+{
+    AMDAT_GUARD("amdat_batch_msd_ngp", amdat_batch_msd_ngp_body(ctx, batch, pairs, s2, s4, count))
 }
 
 }  // extern "C"
@@ -2695,7 +2982,7 @@ int autotune_job(amdat_ctx *ctx, amdat_batch *batch, const JobParams &prm, const
 
 extern "C" {
 
-int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scales, const amdat_shells *sk_shells,
+static int amdat_batch_run_body(amdat_ctx *ctx, amdat_batch *batch, const float *length_scales, const amdat_shells *sk_shells,
                     int sk_first, int sk_last, const int *frames, int n_frames, const amdat_shells *fs_shells,
                     const int *fs_first, const int *fs_last, const amdat_pairs *pairs, int want_msd, int flags,
                     double *S, double *S_count, double *Fs, double *Fs_count, double *s2, double *s4,
@@ -2850,7 +3137,16 @@ int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scal
     return AMDAT_OK;
 }
 
-int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *paired, int *n_items, int *graph)
+int amdat_batch_run(amdat_ctx *ctx, amdat_batch *batch, const float *length_scales, const amdat_shells *sk_shells,
+                    int sk_first, int sk_last, const int *frames, int n_frames, const amdat_shells *fs_shells,
+                    const int *fs_first, const int *fs_last, const amdat_pairs *pairs, int want_msd, int flags,
+                    double *S, double *S_count, double *Fs, double *Fs_count, double *s2, double *s4,
+                    double *msd_count)
+{
+    AMDAT_GUARD("amdat_batch_run", amdat_batch_run_body(ctx, batch, length_scales, sk_shells, sk_first, sk_last, frames, n_frames, fs_shells, fs_first, fs_last, pairs, want_msd, flags, S, S_count, Fs, Fs_count, s2, s4, msd_count))
+}
+
+static int amdat_batch_job_info_body(const amdat_batch *batch, int *RS, int *RF, int *paired, int *n_items, int *graph)
 {
     if (!batch || !batch->job) return set_error(AMDAT_EINVAL, "amdat_batch_job_info: no job has run");
     if (RS) *RS = batch->job->ch.RS;
@@ -2861,7 +3157,12 @@ int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *paired
     return AMDAT_OK;
 }
 
-int amdat_batch_use_settings(amdat_batch *batch, const amdat_job_settings *s)
+int amdat_batch_job_info(const amdat_batch *batch, int *RS, int *RF, int *paired, int *n_items, int *graph)
+{
+    AMDAT_GUARD("amdat_batch_job_info", amdat_batch_job_info_body(batch, RS, RF, paired, n_items, graph))
+}
+
+static int amdat_batch_use_settings_body(amdat_batch *batch, const amdat_job_settings *s)
 {
     clear_error();
     if (!batch) return set_error(AMDAT_EINVAL, "amdat_batch_use_settings: null batch");
@@ -2895,7 +3196,12 @@ int amdat_batch_use_settings(amdat_batch *batch, const amdat_job_settings *s)
     return AMDAT_OK;
 }
 
-int amdat_batch_job_settings(const amdat_batch *batch, amdat_job_settings *out)
+int amdat_batch_use_settings(amdat_batch *batch, const amdat_job_settings *s)
+{
+    AMDAT_GUARD("amdat_batch_use_settings", amdat_batch_use_settings_body(batch, s))
+}
+
+static int amdat_batch_job_settings_body(const amdat_batch *batch, amdat_job_settings *out)
 {
     if (!batch || !batch->job) return set_error(AMDAT_EINVAL, "amdat_batch_job_settings: no job has run");
     if (!out) return set_error(AMDAT_EINVAL, "amdat_batch_job_settings: null argument");
@@ -2917,6 +3223,11 @@ int amdat_batch_job_settings(const amdat_batch *batch, amdat_job_settings *out)
     out->model_ms = jc.model_ms;
     out->tuned_ms = jc.tuned_ms;
     return AMDAT_OK;
+}
+
+int amdat_batch_job_settings(const amdat_batch *batch, amdat_job_settings *out)
+{
+    AMDAT_GUARD("amdat_batch_job_settings", amdat_batch_job_settings_body(batch, out))
 }
 
 }  // extern "C"

@@ -3,6 +3,8 @@
 
 #include <stdint.h>
 
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -11,6 +13,20 @@ namespace amdat {
 // Record a message for amdat_last_error() on this thread and return code.
 int set_error(int code, const std::string &msg);
 void clear_error();
+
+// Body of an entry point of the C ABI: an exception thrown below it (a
+// failed host allocation, a thread that cannot start) becomes a status
+// instead of unwinding through the C caller's frames.
+#define AMDAT_GUARD(NAME, CALL)                                                          \
+    try {                                                                                \
+        return CALL;                                                                     \
+    } catch (const std::bad_alloc &) {                                                   \
+        return ::amdat::set_error(AMDAT_ENOMEM, NAME ": host allocation failed");        \
+    } catch (const std::exception &amdat_guard_e) {                                      \
+        return ::amdat::set_error(AMDAT_EINVAL, std::string(NAME ": ") + amdat_guard_e.what()); \
+    } catch (...) {                                                                      \
+        return ::amdat::set_error(AMDAT_EINVAL, NAME ": unexpected error");              \
+    }
 
 // Block layout of a k-vector plan (host side of the phase kernels).
 // Vectors are canonicalised (a vector with nz < 0 is flipped and flagged

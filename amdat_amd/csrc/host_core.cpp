@@ -22,21 +22,31 @@ extern "C" {
 
 int amdat_version(void) { return 1 * 10000 + 0 * 100 + 0; }
 
-int amdat_ts_n_timesteps(int nexp, int nes)
+static int amdat_ts_n_timesteps_body(int nexp, int nes)
 {
     if (nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "time scheme: n_exponentials must be >= 1 and n_exponential_steps >= 0");
     return nes * nexp + 1;
 }
 
-int amdat_ts_n_timegaps(int nexp, int nes, int frt)
+int amdat_ts_n_timesteps(int nexp, int nes)
+{
+    AMDAT_GUARD("amdat_ts_n_timesteps", amdat_ts_n_timesteps_body(nexp, nes))
+}
+
+static int amdat_ts_n_timegaps_body(int nexp, int nes, int frt)
 {
     if (nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "time scheme: n_exponentials must be >= 1 and n_exponential_steps >= 0");
     return nes + nexp - 1 + (frt ? 0 : 1);
 }
 
+int amdat_ts_n_timegaps(int nexp, int nes, int frt)
+{
+    AMDAT_GUARD("amdat_ts_n_timegaps", amdat_ts_n_timegaps_body(nexp, nes, frt))
+}
+
 // System::create_timelist, src/system.cpp:3542-3568.  exp_base is a float
 // member; pow(float, int) is evaluated in double.
-int amdat_ts_timelist(int nexp, int nes, float base, int first_exp, float dt, float *out, int cap)
+static int amdat_ts_timelist_body(int nexp, int nes, float base, int first_exp, float dt, float *out, int cap)
 {
     if (!out || nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "amdat_ts_timelist: bad arguments");
     int T = nes * nexp + 1;
@@ -56,8 +66,13 @@ int amdat_ts_timelist(int nexp, int nes, float base, int first_exp, float dt, fl
     return AMDAT_OK;
 }
 
+int amdat_ts_timelist(int nexp, int nes, float base, int first_exp, float dt, float *out, int cap)
+{
+    AMDAT_GUARD("amdat_ts_timelist", amdat_ts_timelist_body(nexp, nes, base, first_exp, dt, out, cap))
+}
+
 // System::displacement_times, src/system.cpp:3763-3784.
-int amdat_ts_displacement_times(int nexp, int nes, int frt, const float *tl, float *out, int cap)
+static int amdat_ts_displacement_times_body(int nexp, int nes, int frt, const float *tl, float *out, int cap)
 {
     if (!tl || !out || nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "amdat_ts_displacement_times: bad arguments");
     int ntg = nes + nexp - 1 + (frt ? 0 : 1);
@@ -70,10 +85,15 @@ int amdat_ts_displacement_times(int nexp, int nes, int frt, const float *tl, flo
     return AMDAT_OK;
 }
 
+int amdat_ts_displacement_times(int nexp, int nes, int frt, const float *tl, float *out, int cap)
+{
+    AMDAT_GUARD("amdat_ts_displacement_times", amdat_ts_displacement_times_body(nexp, nes, frt, tl, out, cap))
+}
+
 // System::displacement_list, src/system.cpp:4090-4179.  The pair order is
 // the reference's: in-block gaps by gap then block, cross-block gaps by gap,
 // block offset, block; then the (0, last) pair when frt == 0.
-int amdat_ts_pairs(int nexp, int nes, int frt, int fullblock, int limit, int *tg, int *t0, int *t1,
+static int amdat_ts_pairs_body(int nexp, int nes, int frt, int fullblock, int limit, int *tg, int *t0, int *t1,
                    int cap, int *n_pairs)
 {
     if (nexp < 1 || nes < 0 || limit < 0) return set_error(AMDAT_EINVAL, "amdat_ts_pairs: bad time scheme");
@@ -113,13 +133,19 @@ int amdat_ts_pairs(int nexp, int nes, int frt, int fullblock, int limit, int *tg
     return AMDAT_OK;
 }
 
+int amdat_ts_pairs(int nexp, int nes, int frt, int fullblock, int limit, int *tg, int *t0, int *t1,
+                   int cap, int *n_pairs)
+{
+    AMDAT_GUARD("amdat_ts_pairs", amdat_ts_pairs_body(nexp, nes, frt, fullblock, limit, tg, t0, t1, cap, n_pairs))
+}
+
 // System::displacement_list(analysis, timegap, false), src/system.cpp:
 // 4349-4398: the pairs of one time gap with the first time of each block
 // only (in-block gaps: every block; cross-block gaps: blocks that reach;
 // the last gap: frames 0 and n_timesteps - 1, which with frt = 1 is time
 // gap n_timegaps, one past the full enumeration).  limit > 0 keeps the
 // first limit pairs.
-int amdat_ts_gap_pairs(int nexp, int nes, int frt, int limit, int time_index, int *tg, int *t0, int *t1, int cap,
+static int amdat_ts_gap_pairs_body(int nexp, int nes, int frt, int limit, int time_index, int *tg, int *t0, int *t1, int cap,
                        int *n_pairs)
 {
     if (nexp < 1 || nes < 1 || limit < 0) return set_error(AMDAT_EINVAL, "amdat_ts_gap_pairs: bad time scheme");
@@ -154,9 +180,15 @@ int amdat_ts_gap_pairs(int nexp, int nes, int frt, int limit, int time_index, in
     return AMDAT_OK;
 }
 
+int amdat_ts_gap_pairs(int nexp, int nes, int frt, int limit, int time_index, int *tg, int *t0, int *t1, int cap,
+                       int *n_pairs)
+{
+    AMDAT_GUARD("amdat_ts_gap_pairs", amdat_ts_gap_pairs_body(nexp, nes, frt, limit, time_index, tg, t0, t1, cap, n_pairs))
+}
+
 // System::timegap_weighting, src/system.cpp:4599-4628 (cross-block entries
 // carry the reference's +1).
-int amdat_ts_weighting(int nexp, int nes, int frt, int fullblock, int limit, int *w, int cap)
+static int amdat_ts_weighting_body(int nexp, int nes, int frt, int fullblock, int limit, int *w, int cap)
 {
     if (!w || nexp < 1 || nes < 0) return set_error(AMDAT_EINVAL, "amdat_ts_weighting: bad arguments");
     int f = frt ? 1 : 0;
@@ -173,6 +205,11 @@ int amdat_ts_weighting(int nexp, int nes, int frt, int fullblock, int limit, int
     }
     if (!frt) w[ntg - 1] = 1;
     return AMDAT_OK;
+}
+
+int amdat_ts_weighting(int nexp, int nes, int frt, int fullblock, int limit, int *w, int cap)
+{
+    AMDAT_GUARD("amdat_ts_weighting", amdat_ts_weighting_body(nexp, nes, frt, fullblock, limit, w, cap))
 }
 
 // Wave_Vectors::read_vectors, src/wave_vectors.cpp:430-484: one file per
@@ -240,16 +277,28 @@ static int read_shell_files(const char *stem, const char *plane, int first_bin, 
     return AMDAT_OK;
 }
 
-int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_off, int *nvec,
+static int amdat_shells_read_body(const char *stem, int first_bin, int n_bins, int *shell_off, int *nvec,
                       int cap_vectors, int *n_vectors)
 {
     return read_shell_files(stem, "xyz", first_bin, n_bins, shell_off, nvec, cap_vectors, n_vectors, nullptr);
 }
 
-int amdat_shells_read_plane(const char *stem, const char *plane, int first_bin, int n_bins, int *shell_off,
+int amdat_shells_read(const char *stem, int first_bin, int n_bins, int *shell_off, int *nvec,
+                      int cap_vectors, int *n_vectors)
+{
+    AMDAT_GUARD("amdat_shells_read", amdat_shells_read_body(stem, first_bin, n_bins, shell_off, nvec, cap_vectors, n_vectors))
+}
+
+static int amdat_shells_read_plane_body(const char *stem, const char *plane, int first_bin, int n_bins, int *shell_off,
                             int *nvec, int cap_vectors, int *n_vectors, int *denominator)
 {
     return read_shell_files(stem, plane, first_bin, n_bins, shell_off, nvec, cap_vectors, n_vectors, denominator);
+}
+
+int amdat_shells_read_plane(const char *stem, const char *plane, int first_bin, int n_bins, int *shell_off,
+                            int *nvec, int cap_vectors, int *n_vectors, int *denominator)
+{
+    AMDAT_GUARD("amdat_shells_read_plane", amdat_shells_read_plane_body(stem, plane, first_bin, n_bins, shell_off, nvec, cap_vectors, n_vectors, denominator))
 }
 
 namespace {
@@ -267,7 +316,7 @@ inline uint64_t splitmix64(uint64_t &s)
 // at most max_per_shell per shell by a seeded shuffle.  The reference
 // ships one fixed random choice; this generator makes a reproducible one
 // of the same sizes.
-int amdat_shells_generate(int first_bin, int n_bins, int max_per_shell, uint64_t seed,
+static int amdat_shells_generate_body(int first_bin, int n_bins, int max_per_shell, uint64_t seed,
                           int *shell_off, int *nvec, int cap_vectors, int *n_vectors)
 {
     if (!shell_off || n_bins < 1 || first_bin < 1 || first_bin + n_bins - 1 > 300 || max_per_shell < 1)
@@ -313,6 +362,12 @@ int amdat_shells_generate(int first_bin, int n_bins, int max_per_shell, uint64_t
     if (n_vectors) *n_vectors = n;
     if (n > cap_vectors) return set_error(AMDAT_EINVAL, "amdat_shells_generate: vector capacity too small");
     return AMDAT_OK;
+}
+
+int amdat_shells_generate(int first_bin, int n_bins, int max_per_shell, uint64_t seed,
+                          int *shell_off, int *nvec, int cap_vectors, int *n_vectors)
+{
+    AMDAT_GUARD("amdat_shells_generate", amdat_shells_generate_body(first_bin, n_bins, max_per_shell, seed, shell_off, nvec, cap_vectors, n_vectors))
 }
 
 }  // extern "C"
@@ -945,7 +1000,15 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
         const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
         const size_t n_threads = std::min<size_t>(std::min<size_t>(hc, 32), tasks.size());
         std::vector<std::thread> pool;
-        for (size_t t = 1; t < n_threads; t++) pool.emplace_back(work);
+        for (size_t t = 1; t < n_threads; t++) {
+            // a thread that cannot be started leaves its share to the others
+            // (the tasks are drawn from one counter)
+            try {
+                pool.emplace_back(work);
+            } catch (...) {
+                break;
+            }
+        }
         work();
         for (auto &th : pool) th.join();
         for (size_t h0 = 0; h0 < units.size(); h0 += kLanesPerPass) fill_padding(units, h0, R, unit_g[h0], E);
@@ -1037,7 +1100,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
 
 }  // namespace amdat
 
-extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst,
+static int amdat_internal_plan_layout_body(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst,
                                           int *range, int *n_chunks, long *conflicts)
 {
     if (!nvec || K < 1 || (R != 2 && R != 4 && R != 8) || !n_chunks || pair < 0 || pair > 5) return AMDAT_EINVAL;
@@ -1054,4 +1117,10 @@ extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, int pai
         std::copy(L.range.begin(), L.range.end(), range);
     }
     return AMDAT_OK;
+}
+
+extern "C" int amdat_internal_plan_layout(const int *nvec, int K, int R, int pair, uint32_t *pack, int *dst,
+                                          int *range, int *n_chunks, long *conflicts)
+{
+    AMDAT_GUARD("amdat_internal_plan_layout", amdat_internal_plan_layout_body(nvec, K, R, pair, pack, dst, range, n_chunks, conflicts))
 }

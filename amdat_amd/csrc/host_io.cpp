@@ -68,11 +68,20 @@ void parallel_for(int n, int threads, F f)
         return;
     }
     std::atomic<int> next{0};
+    auto worker = [&] {
+        for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
+    };
+    // the calling thread works too; a thread that cannot be started leaves
+    // its share to the others (the indices are drawn from one counter)
     std::vector<std::thread> pool;
-    for (int t = 0; t < threads; t++)
-        pool.emplace_back([&] {
-            for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
-        });
+    for (int t = 1; t < threads; t++) {
+        try {
+            pool.emplace_back(worker);
+        } catch (...) {
+            break;
+        }
+    }
+    worker();
     for (auto &th : pool) th.join();
 }
 
@@ -383,7 +392,7 @@ using namespace amdat;
 
 extern "C" {
 
-int amdat_read_custom(const char *path, int n_atoms, int n_frames, int threads, float *x, float *y, float *z,
+static int amdat_read_custom_body(const char *path, int n_atoms, int n_frames, int threads, float *x, float *y, float *z,
                       int16_t *ix, int16_t *iy, int16_t *iz, float *box, float *box_lo, int *types)
 {
     if (!path || n_atoms < 1 || n_frames < 1 || !x || !y || !z || !ix || !iy || !iz || !box)
@@ -397,7 +406,13 @@ int amdat_read_custom(const char *path, int n_atoms, int n_frames, int threads, 
     return parse_custom(f, off, 0, n_frames, n_atoms, threads, x, y, z, ix, iy, iz, box, box_lo, types);
 }
 
-int amdat_read_xyz(const char *path, int n_atoms, int n_frames, const float *box3, int threads, float *x, float *y,
+int amdat_read_custom(const char *path, int n_atoms, int n_frames, int threads, float *x, float *y, float *z,
+                      int16_t *ix, int16_t *iy, int16_t *iz, float *box, float *box_lo, int *types)
+{
+    AMDAT_GUARD("amdat_read_custom", amdat_read_custom_body(path, n_atoms, n_frames, threads, x, y, z, ix, iy, iz, box, box_lo, types))
+}
+
+static int amdat_read_xyz_body(const char *path, int n_atoms, int n_frames, const float *box3, int threads, float *x, float *y,
                    float *z, int16_t *ix, int16_t *iy, int16_t *iz, int *types)
 {
     if (!path || n_atoms < 1 || n_frames < 1 || !box3 || !x || !y || !z || !ix || !iy || !iz)
@@ -411,6 +426,12 @@ int amdat_read_xyz(const char *path, int n_atoms, int n_frames, const float *box
     rc = parse_xyz(f, off, 0, n_frames, n_atoms, threads, x, y, z, types);
     if (rc) return rc;
     return unwrap_images(n_atoms, n_frames, box3, x, y, z, ix, iy, iz, nullptr, nullptr, threads);
+}
+
+int amdat_read_xyz(const char *path, int n_atoms, int n_frames, const float *box3, int threads, float *x, float *y,
+                   float *z, int16_t *ix, int16_t *iy, int16_t *iz, int *types)
+{
+    AMDAT_GUARD("amdat_read_xyz", amdat_read_xyz_body(path, n_atoms, n_frames, box3, threads, x, y, z, ix, iy, iz, types))
 }
 
 }  // extern "C"
