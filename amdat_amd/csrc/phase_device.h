@@ -106,9 +106,10 @@ __device__ __forceinline__ int range_gs_code(int4 r, int wave) { return (int)(((
 template <int MM>
 struct AnchorLayout {
     static constexpr int NR = 3 * Geom<MM>::A;  // table rows of a tile
-    static constexpr int PER = (NR * 8 <= kPhaseBlock / 2) ? 1 : ((NR * 8 <= kPhaseBlock) ? 2 : 4);  // anchors per lane
-    static constexpr int LANES = 8 / PER;                        // lanes per row
-    static constexpr int N = NR * LANES;                         // lanes with work
+    // anchors per lane
+    static constexpr int PER = (NR * 8 <= kPhaseBlock / 2) ? 1 : ((NR * 8 <= kPhaseBlock) ? 2 : 4);
+    static constexpr int LANES = 8 / PER;  // lanes per row
+    static constexpr int N = NR * LANES;   // lanes with work
     static_assert(N <= kPhaseBlock, "one anchor item per lane");
     static_assert(Geom<MM>::A % (64 / LANES) == 0, "one axis per wave");
 };
