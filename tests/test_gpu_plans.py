@@ -83,12 +83,13 @@ def grouped_vectors(M, seed=3):
 _width_cases = {}
 
 
-@pytest.mark.parametrize("M", [20, 45, 127])
+@pytest.mark.parametrize("M", [20, 45, 95, 127])
 @pytest.mark.parametrize("R,pair", [(2, 0), (8, 1), (8, 4), (8, 5)])
 def test_plans_at_table_widths(ctx, M, R, pair):
-    """Each layout at table half-widths 24, 50 and 127 (32, 32 and 8 atoms
-    per tile; 4, 4 and 1 anchors per lane): rho per vector and Fs against
-    the FP64 oracle at the 1e-6 gate, 300 atoms, 4 frames."""
+    """Each layout at table half-widths 24, 50, 100 and 127 (32, 32, 16 and
+    8 atoms per tile; 4, 4, 2 and 1 anchors per lane; 300 atoms, so every
+    width ends on a partial tile): rho per vector and Fs against the FP64
+    oracle at the 1e-6 gate, 4 frames."""
     print("case: plan R=%d mode %d, components up to %d" % (R, pair, M), flush=True)
     if M not in _width_cases:
         tr = synth.random_walk(300, 6.3, 4, 0.05, seed=7 + M)
