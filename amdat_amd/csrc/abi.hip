@@ -371,6 +371,7 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
             mode = ch->second.second;
         } else {
             double best = 1e300;
+            int best_waves = 0;
             // Mode 2 is left to amdat_shells_plan_hint: its unpaired waves
             // gather the partnerless vectors of many nx groups, whose table
             // reads spread over the banks (64,000-atom workload: rho 542 ms
@@ -388,8 +389,16 @@ int build_plan(amdat_ctx *ctx, amdat_shells *sh, int first, int last, KPlanOwned
                 PlanLayout L =
                     plan_layout(sh->nvec.data() + 3 * (size_t)v0, K, cd[0], cd[1], plan_table_width(M), c, false);
                 const double cost = layout_cost(L, c);
-                if (cost < best * 0.98) {
+                // an exact tie goes to the layout that keeps at least as
+                // many waves running the sums (a block with few has little
+                // to hide its latencies with), then to the later candidate:
+                // the scale form's 5 Fs shells (1,500 vectors) tie at R = 4
+                // and 8 unpaired and R = 8 mode 5, which ran 9.5, 11.2 and
+                // 9.2 s with 7, 4 and 7 such waves
+                const bool tie = std::fabs(cost - best) <= 1e-9 * best && L.active_waves >= best_waves;
+                if (cost < best * 0.98 || tie) {
                     best = cost;
+                    best_waves = L.active_waves;
                     lay = std::move(L);
                     have_layout = cd[1] < 4;  // a grouped plan is placed once chosen
                     R = cd[0];

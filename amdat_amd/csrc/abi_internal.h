@@ -65,6 +65,7 @@ struct PlanLayout {
     bool pair = false;           // grouped waves: a group's slots share ny, or nz (kPackSwap)
     bool quad = false;           // mode 3
     long gs8_slots = 0, quad_slots = 0, paired_slots = 0, unpaired_slots = 0;
+    int active_waves = 0;        // waves that run the sums, summed over chunks
     double simd_cost = 0.0;      // per atom, summed over chunks: 4 SIMDs x 64 lanes x the busiest
                                  // SIMD's VALU ops (its two waves' slots x (c - 4 + 4/GS))
                                  // slots of waves that run the sums, by group size 8, 4, 2, 1,

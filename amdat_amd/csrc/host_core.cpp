@@ -1052,6 +1052,7 @@ PlanLayout plan_layout(const int *nvec, int K, int R, int mode, int MM, double c
             a_by_g[cd]++;
         }
         L.range[4 * ch + 3] = (int)((unsigned)n_act | (codes << 16));
+        L.active_waves += (n_act + 63) / 64;
         // The chunk's time per atom follows its busiest SIMD: waves w and
         // w + 4 share one, and each wave's lanes issue its slots x
         // (c - 4 + 4/GS) VALU operations per atom.
